@@ -1,0 +1,193 @@
+/*
+ * streams_mi355x.h -- C ABI of the MI355X (gfx950) per-RK-stage hot path of the
+ * STREAmS-2 single-ideal-gas solver.
+ *
+ * The library owns every device array of one flow solve behind an opaque
+ * handle.  Each entry point below replaces one type-bound procedure of the
+ * reference backend object `equation_singleideal_cpu_object`
+ * (code/src/singleideal/singleideal_cpu.F90:29-138) or one device-side
+ * helper of the AMD backend (code/src/singleideal/kernels_amd.F90 bind(C)
+ * interfaces <-> kernels_amd_cpp.cpp extern "C" launchers).  A Fortran
+ * backend binds them with `bind(c, name="...")` interfaces; plain C types
+ * only: int32 scalars by value, double scalars by value, host arrays as
+ * pointers, results through pointers.
+ *
+ * Array conventions (identical to the reference, Fortran column-major):
+ *   w      (1-ng:nx+ng, 1-ng:ny+ng, 1-ng:nz+ng, 1:5)   conserved variables
+ *   w_aux  (1-ng:nx+ng, 1-ng:ny+ng, 1-ng:nz+ng, 1:10)  rho,u,v,w,H,T,mu,sensor,|omega|,div/3
+ *   fl     (1:nx, 1:ny, 1:nz, 1:5)                     right-hand side
+ *   x,y,z  (1-ng:n+ng) node coordinates, yn (1:ny+1) cell faces in y
+ *   metric arrays dcsidx, dcsidxs, dcsidx2 (1:nx), detady.. (1:ny), dzitdz.. (1:nz)
+ *
+ * Every function returns 0 on success and a non-zero ST_ERR_* code on
+ * failure; st_last_error() returns a message for the calling thread.  The
+ * Fortran side turns a non-zero status into fail_input_any/mpi_abort, as the
+ * reference does for its own input errors (parameters.F90:166-171).
+ */
+#ifndef STREAMS_MI355X_H
+#define STREAMS_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ST_OK               0
+#define ST_ERR_ARG          1   /* invalid argument / unsupported option     */
+#define ST_ERR_HIP          2   /* HIP runtime error                          */
+#define ST_ERR_NOMEM        3   /* device allocation failed                   */
+#define ST_ERR_STATE        4   /* call order violated (e.g. no grid set)     */
+#define ST_ERR_NAN          5   /* NaN residual (reference 'boom!!!' path)    */
+
+#define ST_MAX_CP_COEFF     16
+
+/* Precision of the flux/stress evaluation.  FP64 everywhere is the plain
+ * form; MIXED evaluates the convective reconstructions and viscous stresses
+ * in FP32 while the state, RK accumulation, T/p recovery, WENO smoothness
+ * indicators and all reductions stay FP64. */
+#define ST_PREC_FP64        0
+#define ST_PREC_MIXED       1
+
+/* Scalar parameters of the path: the fields of equation_singleideal_object
+ * that the hot loop reads (singleideal.F90:58-125) plus the grid sizes. */
+typedef struct st_params {
+    int32_t nx, ny, nz, ng;          /* interior cells per direction, ghost width (3)     */
+    int32_t ep_order;                /* central convective order (2,4,6,8)                */
+    int32_t visc_order;              /* viscous order (== ep_order)                       */
+    int32_t nkeep;                   /* KEEP(n): 0 KEEP, >0 KEEP(n), <0 non-KEEP split    */
+    int32_t weno_scheme;             /* 1..4, WENO order 2*weno_scheme-1 (3 = WENO5)      */
+    int32_t weno_version;            /* 0 Jiang-Shu, 1 WENO-Z                             */
+    int32_t flux_splitting;          /* 0 Roe characteristic (1 Rusanov: not supported)   */
+    int32_t conservative_viscous;    /* 0 Laplacian form, 1 conservative form             */
+    int32_t visc_model;              /* 0 inviscid, 1 power law, 2 Sutherland             */
+    int32_t calorically_perfect;     /* 1 constant cp, 0 cp(T) polynomial                 */
+    int32_t indx_cp_l, indx_cp_r;    /* exponent range of the cp polynomial               */
+    int32_t bctags[6];               /* xmin,xmax,ymin,ymax,zmin,zmax: 0 periodic, 6 wall */
+    int32_t bctags_nr[6];            /* characteristic BC flags (0 = none)                */
+    int32_t is_y_staggered;          /* 1 for the channel (cell-centred y)                */
+    int32_t channel_case;            /* 1 enables force_rhs / force_var                   */
+    int32_t correct_bound_ord;       /* 1 lowers the stencil order at non-periodic faces  */
+    int32_t iter_dt_recompute;       /* CFL dt recomputed every this many steps           */
+    int32_t precision;               /* ST_PREC_FP64 or ST_PREC_MIXED                     */
+    int32_t pad0;
+    double rgas0, prandtl, t0, rho0, u0, l0, mu0;
+    double sutherland_s, t_ref_dim, powerlaw_vtexp;
+    double sensor_threshold;
+    double t_wall;                   /* isothermal wall temperature                        */
+    double theta_wall;               /* >= -1 activates bulk-temperature control          */
+    double t_bulk_target;
+    double volchan;                  /* channel volume (yn(ny+1)-yn(1))*nxmax*nzmax        */
+    double cfl;                      /* > 0 CFL number, < 0 fixed dt = -cfl               */
+    double tol_iter_nr;              /* Newton tolerance for T(e) (1e-12 in FP64)          */
+    double cp_coeff[ST_MAX_CP_COEFF];/* cp_coeff(indx_cp_l:indx_cp_r+1), index l-indx_cp_l */
+    double cv_coeff[ST_MAX_CP_COEFF];
+} st_params;
+
+/* Device-resident scalars of the run, copied to the host on request. */
+typedef struct st_scalars {
+    double dt;          /* current time step                                   */
+    double time;        /* simulated time                                      */
+    double dpdx;        /* accumulated forcing of the current step             */
+    double rhobulk, ubulk, tbulk;
+    double residual_rhou;
+    double rhomin, rhomax, tmin, tmax, pmin, pmax;
+    int64_t icyc;       /* completed steps                                     */
+} st_scalars;
+
+typedef struct st_solver st_solver;
+
+/* ---- lifecycle: device allocation owned by the library -------------------
+ * Replaces the hipfort hipMalloc of the backend arrays
+ * (base_amd.F90:1211-1413, singleideal_amd.F90:1268-1327) and the setup of
+ * bc_preproc / correct_bc_order (singleideal.F90:4703-4750, 946-993). */
+int st_solver_create(const st_params *p, st_solver **out);
+int st_solver_destroy(st_solver *s);
+/* Bytes of HBM one solve of these sizes occupies (admission control). */
+int64_t st_solver_footprint(const st_params *p);
+
+/* Grid and 1-D metrics (field.F90:456-516 set_local_grid output). */
+int st_set_grid(st_solver *s,
+                const double *x, const double *y, const double *z, const double *yn,
+                const double *dcsidx, const double *dcsidxs, const double *dcsidx2,
+                const double *detady, const double *detadys, const double *detady2,
+                const double *dzitdz, const double *dzitdzs, const double *dzitdz2);
+
+/* copy_from_field / copy_to_field (base_cpu.F90:1914-1929). */
+int st_copy_from_field(st_solver *s, const double *w_host);
+int st_copy_to_field(st_solver *s, double *w_host);
+/* Read / write one w_aux variable (1..10) over the ghosted box. */
+int st_get_aux(st_solver *s, int32_t var, double *host);
+int st_set_aux(st_solver *s, int32_t var, const double *host);
+/* Read the current right-hand side fl (interior, 5 variables). */
+int st_get_fl(st_solver *s, double *host);
+int st_get_scalars(st_solver *s, st_scalars *out);
+int st_set_time(st_solver *s, double time, double dt, int64_t icyc);
+
+/* ---- procedures of the backend object ------------------------------------ */
+/* update_ghost (singleideal_cpu.F90:1064-1138): physical BC ghosts, then the
+ * periodic ghost copy of bcswap when do_swap != 0. */
+int st_update_ghost(st_solver *s, int32_t do_swap);
+/* bcswap (base_cpu.F90:1326-1435), single-GPU periodic copy. */
+int st_bcswap(st_solver *s);
+/* compute_aux (singleideal_cpu.F90:281-330). */
+int st_compute_aux(st_solver *s, int32_t central, int32_t ghost);
+/* visflx(mode) (singleideal_cpu.F90:186-279): 0 full+sensor, 1 div, 3 sensor,
+ * 4 no-sensor; accumulates into fl. */
+int st_visflx(st_solver *s, int32_t mode);
+/* euler_x/y/z (singleideal_cpu.F90:674-843): flux + divergence into fl. */
+int st_euler_x(st_solver *s);
+int st_euler_y(st_solver *s);
+int st_euler_z(st_solver *s);
+/* zero_flux_subroutine (kernels_cpu.F90:9-25). */
+int st_zero_flux(st_solver *s);
+/* Start-of-run sequence of run() (singleideal_cpu.F90:1547-1594):
+ * update_ghost, compute_aux, visflx(0), sensor ghost swap, zero_flux. */
+int st_init_run(st_solver *s);
+/* compute_dt (singleideal_cpu.F90:1476-1517); dt_out may be NULL, in which
+ * case dt stays on the device and no host synchronisation happens. */
+int st_compute_dt(st_solver *s, double *dt_out);
+/* rk_sync (singleideal_cpu.F90:332-409): one full Wray RK3 time step. */
+int st_rk_sync(st_solver *s);
+/* The integration loop body of run() (singleideal_cpu.F90:1599-1645) for
+ * nsteps steps: dt every iter_dt_recompute steps, rk_sync, time += dt.  No
+ * host synchronisation inside. */
+int st_advance(st_solver *s, int32_t nsteps);
+/* compute_residual (singleideal_cpu.F90:1339-1364): residual of rho*u and
+ * rho/T/p extrema into the scalars; returns ST_ERR_NAN on a NaN residual. */
+int st_compute_residual(st_solver *s, st_scalars *out);
+/* count_weno (singleideal_cpu.F90:1366-1395): fraction of WENO faces. */
+int st_count_weno(st_solver *s, double frac_xyz[3]);
+
+/* Measurement helper, no counterpart in the reference (its only timer is the
+ * 'averaged timing' of run(), singleideal_cpu.F90:1661-1666): runs nsteps
+ * steps of st_advance with HIP events recorded on the solver's stream around
+ * every kernel and returns, per kernel family ST_K_*, the mean duration of
+ * one launch in ms (ms[]) and the number of launches (count[]). */
+#define ST_K_AUX        0
+#define ST_K_EULER_X    1
+#define ST_K_VISC       2
+#define ST_K_EULER_Y    3
+#define ST_K_EULER_Z    4
+#define ST_K_SENSOR     5
+#define ST_K_VISC_DIV   6
+#define ST_K_FINISH     7
+#define ST_K_UPDATE     8
+#define ST_K_FORCE_VAR  9
+#define ST_K_DT         10
+#define ST_K_STEP_END   11
+#define ST_NKFAM        12
+int st_kernel_times(st_solver *s, int32_t nsteps, double ms[ST_NKFAM], int32_t count[ST_NKFAM]);
+
+/* Stream used by every launch of this solver (NULL = the library's own). */
+int st_set_stream(st_solver *s, void *hip_stream);
+int st_synchronize(st_solver *s);
+
+const char *st_last_error(void);
+const char *st_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STREAMS_MI355X_H */

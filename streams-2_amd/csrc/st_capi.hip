@@ -1,0 +1,643 @@
+// st_capi.hip -- extern "C" boundary of libstreams_mi355x (declared in
+// include/streams_mi355x.h) and the host-side ordering of the hot loop.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "st_solver.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail(ST_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define NEED(s)                                                                \
+    do {                                                                       \
+        if (!(s)) return fail(ST_ERR_ARG, "null solver handle");               \
+        if (!(s)->grid_set) return fail(ST_ERR_STATE, "st_set_grid not called"); \
+    } while (0)
+
+void deriv_coeffs(int order1, int order2, double d1[4], double d2[5]) {
+    // central-difference weights, get_deriv_coeffs (grid.F90:143-190)
+    for (int l = 0; l < 4; l++) d1[l] = 0.0;
+    for (int l = 0; l < 5; l++) d2[l] = 0.0;
+    switch (order1 / 2) {
+    case 1: d1[0] = 0.5; break;
+    case 2: d1[0] = 2.0 / 3.0; d1[1] = -1.0 / 12.0; break;
+    case 3: d1[0] = 0.75; d1[1] = -0.15; d1[2] = 1.0 / 60.0; break;
+    case 4: d1[0] = 4.0 / 5.0; d1[1] = -1.0 / 5.0; d1[2] = 4.0 / 105.0; d1[3] = -1.0 / 280.0; break;
+    }
+    switch (order2 / 2) {
+    case 1: d2[0] = -2.0; d2[1] = 1.0; break;
+    case 2: d2[0] = -2.5; d2[1] = 4.0 / 3.0; d2[2] = -1.0 / 12.0; break;
+    case 3: d2[0] = -245.0 / 90.0; d2[1] = 1.5; d2[2] = -0.15; d2[3] = 1.0 / 90.0; break;
+    case 4: d2[0] = -205.0 / 72.0; d2[1] = 8.0 / 5.0; d2[2] = -1.0 / 5.0; d2[3] = 8.0 / 315.0; d2[4] = -1.0 / 560.0; break;
+    }
+}
+
+int validate(const st_params *p) {
+    if (!p) return fail(ST_ERR_ARG, "null params");
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(ST_ERR_ARG, "grid sizes must be positive");
+    if (p->ng != 3) return fail(ST_ERR_ARG, "ng must be 3 (stencil half width of the gfx950 kernels)");
+    if (p->nx < p->ng || p->ny < p->ng || p->nz < p->ng)
+        return fail(ST_ERR_ARG, "each grid size must be >= ng (singleideal.F90:578)");
+    if (p->ep_order < 2 || p->ep_order > 6 || (p->ep_order & 1))
+        return fail(ST_ERR_ARG, "ep_order must be 2, 4 or 6 with ng = 3");
+    if (p->visc_order != p->ep_order) return fail(ST_ERR_ARG, "viscous order must be equal to ep_order");
+    if (p->weno_scheme < 1 || p->weno_scheme > 3) return fail(ST_ERR_ARG, "weno_scheme must be 1..3 with ng = 3");
+    if (p->weno_version != 0 && p->weno_version != 1) return fail(ST_ERR_ARG, "weno_version must be 0 or 1");
+    if (p->flux_splitting != 0) return fail(ST_ERR_ARG, "flux_splitting=1 (Rusanov) is not supported");
+    if (p->conservative_viscous != 0) return fail(ST_ERR_ARG, "conservative_viscous=1 is not supported");
+    if (p->calorically_perfect != 0 && p->calorically_perfect != 1)
+        return fail(ST_ERR_ARG, "calorically_perfect must be 0 or 1");
+    if (p->indx_cp_r - p->indx_cp_l + 2 > ST_MAX_CP_COEFF) return fail(ST_ERR_ARG, "too many cp coefficients");
+    if (p->precision != ST_PREC_FP64) return fail(ST_ERR_ARG, "only the FP64 precision is available");
+    for (int l = 0; l < 6; l++) {
+        int t = p->bctags[l];
+        if (t != 0 && !(t == 6 && p->is_y_staggered && (l == 2 || l == 3)))
+            return fail(ST_ERR_ARG, "boundary tag " + std::to_string(t) + " on face " + std::to_string(l + 1) +
+                                        " is not supported (periodic 0 and staggered wall 6 on y)");
+        if (p->bctags_nr[l] != 0 && !(t == 6 && p->is_y_staggered))
+            return fail(ST_ERR_ARG, "characteristic boundary treatment is not supported");
+    }
+    for (int d = 0; d < 3; d++)
+        if ((p->bctags[2 * d] == 0) != (p->bctags[2 * d + 1] == 0))
+            return fail(ST_ERR_ARG, "a periodic direction needs tag 0 on both faces");
+    double vol = ((double)p->nx + 6) * ((double)p->ny + 6) * ((double)p->nz + 6);
+    if (vol * 10.0 > 2147483647.0) return fail(ST_ERR_ARG, "grid too large for 32-bit indexing of w_aux");
+    return ST_OK;
+}
+
+// bc_preproc (singleideal.F90:4703-4750) and correct_bc_order
+// (:946-993) for a single block per direction.
+void build_tables(st_solver *s, int *eoc_host[3]) {
+    const st_params &p = s->p;
+    st::KPar &k = s->kp;
+    int nr[6];
+    for (int l = 0; l < 6; l++) {
+        nr[l] = p.bctags_nr[l];
+        k.fzf[l] = 0;
+        if (p.bctags[l] == 6 && p.is_y_staggered) {
+            k.fzf[l] = 1;
+            nr[l] = 0;
+        }
+    }
+    const int n[3] = {p.nx, p.ny, p.nz};
+    for (int d = 0; d < 3; d++) {
+        k.eul_lo[d] = 1 + (nr[2 * d] != 0 ? 1 : 0);
+        k.eul_hi[d] = n[d] - (nr[2 * d + 1] != 0 ? 1 : 0);
+        k.periodic[d] = p.bctags[2 * d] == 0 ? 1 : 0;
+    }
+    k.wall_stag[0] = p.bctags[2] == 6 && p.is_y_staggered;
+    k.wall_stag[1] = p.bctags[3] == 6 && p.is_y_staggered;
+    const int ss = p.ep_order / 2 > p.weno_scheme ? p.ep_order / 2 : p.weno_scheme;
+    for (int d = 0; d < 3; d++) {
+        int *e = eoc_host[d];
+        for (int f = 0; f <= n[d]; f++) e[f] = 0;
+        if (p.correct_bound_ord > 0 && (p.bctags[2 * d] != 0 || p.bctags[2 * d + 1] != 0)) {
+            e[0] = -ss + 1;
+            for (int f = 1; f <= ss - 1; f++) e[f] = -ss + f;
+            e[n[d]] = -ss + 1;
+            for (int f = 1; f <= ss - 1; f++) e[n[d] - f] = -ss + f;
+        }
+    }
+}
+
+void fill_kpar(st_solver *s) {
+    const st_params &p = s->p;
+    st::KPar &k = s->kp;
+    k.lmax_base = p.ep_order / 2;
+    k.visc_lmax = p.visc_order / 2;
+    k.weno_scheme = p.weno_scheme;
+    k.weno_version = p.weno_version;
+    k.nkeep = p.nkeep;
+    k.calorically_perfect = p.calorically_perfect;
+    k.indx_cp_l = p.indx_cp_l;
+    k.indx_cp_r = p.indx_cp_r;
+    k.visc_model = p.visc_model;
+    k.ndim = p.nz > 1 ? 3 : 2;
+    k.rgas0 = p.rgas0; k.prandtl = p.prandtl; k.t0 = p.t0; k.rho0 = p.rho0; k.u0 = p.u0; k.l0 = p.l0;
+    k.mu0 = p.mu0; k.sutherland_s = p.sutherland_s; k.t_ref_dim = p.t_ref_dim; k.vtexp = p.powerlaw_vtexp;
+    k.sensor_threshold = p.sensor_threshold; k.t_wall = p.t_wall; k.t_bulk_target = p.t_bulk_target;
+    k.volchan = p.volchan; k.cfl = p.cfl; k.tol_iter_nr = p.tol_iter_nr;
+    for (int l = 0; l < ST_MAX_CP_COEFF; l++) { k.cp[l] = p.cp_coeff[l]; k.cv[l] = p.cv_coeff[l]; }
+    double d1[4], d2[5];
+    deriv_coeffs(p.ep_order, p.ep_order, d1, d2);
+    for (int l = 0; l < 4; l++) k.cd1[l] = d1[l];
+    for (int lm = 1; lm <= 4; lm++) {
+        double a[4], b[5];
+        deriv_coeffs(2 * lm, 2 * lm, a, b);
+        for (int l = 0; l < 4; l++) k.cd1tab[lm - 1][l] = a[l];
+    }
+    deriv_coeffs(p.visc_order, p.visc_order, d1, d2);
+    for (int l = 0; l < 4; l++) k.vd1[l] = d1[l];
+    for (int l = 0; l < 5; l++) k.vd2[l] = d2[l];
+}
+
+template <class T>
+hipError_t dalloc(T **p, size_t n) {
+    return hipMalloc((void **)p, n * sizeof(T) > 0 ? n * sizeof(T) : 8);
+}
+
+void free_all(st_solver *s) {
+    void *ptrs[] = {s->w, s->w_aux, s->F[0], s->F[1], s->wallprop, s->eoc[0], s->eoc[1], s->eoc[2], s->x, s->y,
+                    s->z, s->yn, s->dcsidx, s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz,
+                    s->dzitdzs, s->dzitdz2, s->ds, s->partials};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (s->hs) (void)hipHostFree(s->hs);
+    if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+int pull_scalars(st_solver *s) {
+    HIPCHK(hipMemcpyAsync(s->hs, s->ds, sizeof(st::DevScalars), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
+// event pair around one launch when st_kernel_times is active
+struct TimeScope {
+    st_solver *s;
+    int slot;
+    TimeScope(st_solver *s_, int fam) : s(s_), slot(-1) {
+        KTimer *t = s->timer;
+        if (t && t->n < t->made) {
+            slot = t->n++;
+            t->fam[slot] = fam;
+            (void)hipEventRecord(t->ev[slot][0], s->stream);
+        }
+    }
+    ~TimeScope() {
+        if (slot >= 0) (void)hipEventRecord(s->timer->ev[slot][1], s->stream);
+    }
+};
+#define TIMED(fam, expr)           \
+    do {                           \
+        TimeScope ts_(s, fam);     \
+        HIPCHK(expr);              \
+    } while (0)
+
+// one RK stage of rk_sync (singleideal_cpu.F90:352-406), fused
+int stage(st_solver *s, int istep) {
+    using namespace st;
+    double *fprev = s->F[s->cur];   // RHS of the previous stage (reference: fl before init_flux)
+    double *fcur = s->F[s->cur ^ 1];
+    const Geo &g = s->g;
+    // bcswap + compute_aux(ghost=1): aux over the box, periodic ghosts from images
+    TIMED(ST_K_AUX, launch_aux(s, 1 - g.ng, g.nx + g.ng, 1 - g.ng, g.ny + g.ng, 1 - g.ng, g.nz + g.ng, true));
+    TIMED(ST_K_EULER_X, launch_euler_x(s, fcur, true));
+    TIMED(ST_K_VISC, launch_visc(s, fcur, 4, true));      // + bcextr_var / bcswap_var of w_aux(10)
+    TIMED(ST_K_EULER_Y, launch_euler_y(s, fcur));
+    if (s->kp.ndim == 3) TIMED(ST_K_EULER_Z, launch_euler_z(s, fcur));
+    if (istep == 2) TIMED(ST_K_SENSOR, launch_sensor(s, true));  // visflx(3) + bcswap_var(w_aux 8)
+    TIMED(ST_K_VISC_DIV, launch_visc_div(s, fcur, fprev, istep, s->p.channel_case != 0));
+    if (s->p.channel_case) TIMED(ST_K_FINISH, launch_force_finish(s, istep));
+    TIMED(ST_K_UPDATE, launch_update(s, fprev, fcur, istep));
+    if (s->p.channel_case && s->p.theta_wall >= -1.0) TIMED(ST_K_FORCE_VAR, launch_force_var(s));
+    s->cur ^= 1;
+    s->last_stage = istep;
+    return ST_OK;
+}
+
+int do_rk_sync(st_solver *s) {
+    for (int istep = 0; istep < 3; istep++) {
+        int rc = stage(s, istep);
+        if (rc) return rc;
+    }
+    return ST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *st_last_error(void) { return g_err.c_str(); }
+const char *st_version(void) { return "streams_mi355x 0.1 (gfx950)"; }
+
+int64_t st_solver_footprint(const st_params *p) {
+    if (!p) return -1;
+    const int64_t sx = p->nx + 2 * p->ng, sy = p->ny + 2 * p->ng, sz = p->nz + 2 * p->ng;
+    const int64_t nvol = sx * sy * sz, nint = (int64_t)p->nx * p->ny * p->nz;
+    const int64_t blocks = (int64_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * p->nz;
+    return 8 * (15 * nvol + 10 * nint + 3 * sx * sz + 7 * blocks) + 4 * (p->nx + p->ny + p->nz + 3) +
+           8 * (2 * sx + 2 * sy + 2 * sz + 3 * (p->nx + p->ny + p->nz) + p->ny + 1) + 4096;
+}
+
+int st_solver_create(const st_params *p, st_solver **out) {
+    if (!out) return fail(ST_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int rc = validate(p);
+    if (rc) return rc;
+    st_solver *s = new (std::nothrow) st_solver();
+    if (!s) return fail(ST_ERR_NOMEM, "host allocation failed");
+    std::memset((void *)s, 0, sizeof(st_solver));
+    s->p = *p;
+    st::Geo &g = s->g;
+    g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.ng = p->ng;
+    g.sx = p->nx + 2 * p->ng; g.sy = p->ny + 2 * p->ng; g.sz = p->nz + 2 * p->ng;
+    g.sxy = g.sx * g.sy;
+    g.nvol = g.sxy * g.sz;
+    g.nint = p->nx * p->ny * p->nz;
+    s->rhork[0] = 0.0; s->rhork[1] = -17.0 / 60.0; s->rhork[2] = -5.0 / 12.0;   // Wray RK3,
+    s->gamrk[0] = 8.0 / 15.0; s->gamrk[1] = 5.0 / 12.0; s->gamrk[2] = 3.0 / 4.0; // singleideal.F90:4780-4782
+    s->last_stage = -1;
+    fill_kpar(s);
+    hipError_t e = hipGetDevice(&s->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(ST_ERR_HIP, std::string("stream creation: ") + hipGetErrorString(e));
+    }
+    s->own_stream = true;
+    const size_t nvol = g.nvol, nint = g.nint;
+    const int nblk = ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz;
+    s->npart = 7 * nblk;
+    hipError_t es[32];
+    int ne = 0;
+    es[ne++] = dalloc(&s->w, 5 * nvol);
+    es[ne++] = dalloc(&s->w_aux, 10 * nvol);
+    es[ne++] = dalloc(&s->F[0], 5 * nint);
+    es[ne++] = dalloc(&s->F[1], 5 * nint);
+    es[ne++] = dalloc(&s->wallprop, (size_t)3 * g.sx * g.sz);
+    es[ne++] = dalloc(&s->eoc[0], (size_t)g.nx + 1);
+    es[ne++] = dalloc(&s->eoc[1], (size_t)g.ny + 1);
+    es[ne++] = dalloc(&s->eoc[2], (size_t)g.nz + 1);
+    es[ne++] = dalloc(&s->x, (size_t)g.sx);
+    es[ne++] = dalloc(&s->y, (size_t)g.sy);
+    es[ne++] = dalloc(&s->z, (size_t)g.sz);
+    es[ne++] = dalloc(&s->yn, (size_t)g.ny + 1);
+    es[ne++] = dalloc(&s->dcsidx, (size_t)g.nx);
+    es[ne++] = dalloc(&s->dcsidxs, (size_t)g.nx);
+    es[ne++] = dalloc(&s->dcsidx2, (size_t)g.nx);
+    es[ne++] = dalloc(&s->detady, (size_t)g.ny);
+    es[ne++] = dalloc(&s->detadys, (size_t)g.ny);
+    es[ne++] = dalloc(&s->detady2, (size_t)g.ny);
+    es[ne++] = dalloc(&s->dzitdz, (size_t)g.nz);
+    es[ne++] = dalloc(&s->dzitdzs, (size_t)g.nz);
+    es[ne++] = dalloc(&s->dzitdz2, (size_t)g.nz);
+    es[ne++] = dalloc(&s->ds, 1);
+    es[ne++] = dalloc(&s->partials, (size_t)s->npart);
+    es[ne++] = hipHostMalloc((void **)&s->hs, sizeof(st::DevScalars), hipHostMallocDefault);
+    for (int q = 0; q < ne; q++) {
+        if (es[q] != hipSuccess) {
+            free_all(s);
+            delete s;
+            return fail(ST_ERR_NOMEM, std::string("device allocation failed: ") + hipGetErrorString(es[q]));
+        }
+    }
+    // tables
+    int *eh[3];
+    int *buf = (int *)std::calloc((size_t)(g.nx + g.ny + g.nz + 3), sizeof(int));
+    eh[0] = buf; eh[1] = buf + g.nx + 1; eh[2] = buf + g.nx + g.ny + 2;
+    build_tables(s, eh);
+    hipError_t ec = hipSuccess;
+    if (ec == hipSuccess) ec = hipMemcpy(s->eoc[0], eh[0], sizeof(int) * (g.nx + 1), hipMemcpyHostToDevice);
+    if (ec == hipSuccess) ec = hipMemcpy(s->eoc[1], eh[1], sizeof(int) * (g.ny + 1), hipMemcpyHostToDevice);
+    if (ec == hipSuccess) ec = hipMemcpy(s->eoc[2], eh[2], sizeof(int) * (g.nz + 1), hipMemcpyHostToDevice);
+    std::free(buf);
+    // zero state (w_aux = 0 as in the reference alloc, singleideal.F90:1596)
+    if (ec == hipSuccess) ec = hipMemset(s->w, 0, sizeof(double) * 5 * nvol);
+    if (ec == hipSuccess) ec = hipMemset(s->w_aux, 0, sizeof(double) * 10 * nvol);
+    if (ec == hipSuccess) ec = hipMemset(s->F[0], 0, sizeof(double) * 5 * nint);
+    if (ec == hipSuccess) ec = hipMemset(s->F[1], 0, sizeof(double) * 5 * nint);
+    if (ec == hipSuccess) ec = hipMemset(s->wallprop, 0, sizeof(double) * 3 * g.sx * g.sz);
+    std::memset(s->hs, 0, sizeof(st::DevScalars));
+    if (p->cfl < 0) s->hs->dt = -p->cfl;
+    if (ec == hipSuccess) ec = hipMemcpy(s->ds, s->hs, sizeof(st::DevScalars), hipMemcpyHostToDevice);
+    if (ec != hipSuccess) {
+        free_all(s);
+        delete s;
+        return fail(ST_ERR_HIP, std::string("initialisation: ") + hipGetErrorString(ec));
+    }
+    *out = s;
+    return ST_OK;
+}
+
+int st_solver_destroy(st_solver *s) {
+    if (!s) return ST_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return ST_OK;
+}
+
+int st_set_grid(st_solver *s, const double *x, const double *y, const double *z, const double *yn,
+                const double *dcsidx, const double *dcsidxs, const double *dcsidx2, const double *detady,
+                const double *detadys, const double *detady2, const double *dzitdz, const double *dzitdzs,
+                const double *dzitdz2) {
+    if (!s) return fail(ST_ERR_ARG, "null solver handle");
+    const double *src[13] = {x, y, z, yn, dcsidx, dcsidxs, dcsidx2, detady, detadys, detady2, dzitdz, dzitdzs, dzitdz2};
+    double *dst[13] = {s->x, s->y, s->z, s->yn, s->dcsidx, s->dcsidxs, s->dcsidx2,
+                       s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2};
+    const st::Geo &g = s->g;
+    const size_t n[13] = {(size_t)g.sx, (size_t)g.sy, (size_t)g.sz, (size_t)g.ny + 1, (size_t)g.nx, (size_t)g.nx,
+                          (size_t)g.nx, (size_t)g.ny, (size_t)g.ny, (size_t)g.ny, (size_t)g.nz, (size_t)g.nz,
+                          (size_t)g.nz};
+    for (int q = 0; q < 13; q++) {
+        if (!src[q]) return fail(ST_ERR_ARG, "null grid array " + std::to_string(q));
+        HIPCHK(hipMemcpyAsync(dst[q], src[q], n[q] * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->grid_set = true;
+    return ST_OK;
+}
+
+int st_copy_from_field(st_solver *s, const double *w_host) {
+    NEED(s);
+    if (!w_host) return fail(ST_ERR_ARG, "null host array");
+    HIPCHK(hipMemcpyAsync(s->w, w_host, sizeof(double) * 5 * (size_t)s->g.nvol, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
+int st_copy_to_field(st_solver *s, double *w_host) {
+    NEED(s);
+    if (!w_host) return fail(ST_ERR_ARG, "null host array");
+    HIPCHK(hipMemcpyAsync(w_host, s->w, sizeof(double) * 5 * (size_t)s->g.nvol, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
+int st_get_aux(st_solver *s, int32_t var, double *host) {
+    NEED(s);
+    if (var < 1 || var > 10 || !host) return fail(ST_ERR_ARG, "var must be 1..10");
+    const size_t nv = s->g.nvol;
+    HIPCHK(hipMemcpyAsync(host, s->w_aux + (size_t)(var - 1) * nv, sizeof(double) * nv, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
+int st_set_aux(st_solver *s, int32_t var, const double *host) {
+    NEED(s);
+    if (var < 1 || var > 10 || !host) return fail(ST_ERR_ARG, "var must be 1..10");
+    const size_t nv = s->g.nvol;
+    HIPCHK(hipMemcpyAsync(s->w_aux + (size_t)(var - 1) * nv, host, sizeof(double) * nv, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
+int st_get_fl(st_solver *s, double *host) {
+    NEED(s);
+    if (!host) return fail(ST_ERR_ARG, "null host array");
+    HIPCHK(hipMemcpyAsync(host, s->F[s->cur], sizeof(double) * 5 * (size_t)s->g.nint, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
+int st_get_scalars(st_solver *s, st_scalars *out) {
+    if (!s || !out) return fail(ST_ERR_ARG, "null argument");
+    int rc = pull_scalars(s);
+    if (rc) return rc;
+    const st::DevScalars &d = *s->hs;
+    out->dt = d.dt; out->time = d.time; out->dpdx = d.dpdx;
+    out->rhobulk = d.rhobulk; out->ubulk = d.ubulk; out->tbulk = d.tbulk;
+    out->residual_rhou = d.residual;
+    out->rhomin = d.rhomin; out->rhomax = d.rhomax; out->tmin = d.tmin; out->tmax = d.tmax;
+    out->pmin = d.pmin; out->pmax = d.pmax;
+    out->icyc = d.icyc;
+    return ST_OK;
+}
+
+int st_set_time(st_solver *s, double time, double dt, int64_t icyc) {
+    if (!s) return fail(ST_ERR_ARG, "null solver handle");
+    int rc = pull_scalars(s);
+    if (rc) return rc;
+    s->hs->time = time;
+    s->hs->dt = dt;
+    s->hs->icyc = icyc;
+    HIPCHK(hipMemcpyAsync(s->ds, s->hs, sizeof(st::DevScalars), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->icyc = icyc;
+    s->icyc0 = icyc;
+    return ST_OK;
+}
+
+int st_update_ghost(st_solver *s, int32_t do_swap) {
+    NEED(s);
+    HIPCHK(st::launch_wall_ghosts(s));
+    if (do_swap) HIPCHK(st::launch_fill_periodic(s, s->w, 5));
+    return ST_OK;
+}
+
+int st_bcswap(st_solver *s) {
+    NEED(s);
+    HIPCHK(st::launch_fill_periodic(s, s->w, 5));
+    return ST_OK;
+}
+
+int st_compute_aux(st_solver *s, int32_t central, int32_t ghost) {
+    NEED(s);
+    const st::Geo &g = s->g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz, ng = g.ng;
+    if (central == 1 && ghost == 1) {
+        HIPCHK(st::launch_aux(s, 1 - ng, nx + ng, 1 - ng, ny + ng, 1 - ng, nz + ng, false));
+    } else if (central == 1 && ghost == 0) {
+        HIPCHK(st::launch_aux(s, 1, nx, 1, ny, 1, nz, false));
+    } else if (central == 0 && ghost == 1) {
+        HIPCHK(st::launch_aux(s, 1 - ng, 0, 1 - ng, ny + ng, 1 - ng, nz + ng, false));
+        HIPCHK(st::launch_aux(s, nx + 1, nx + ng, 1 - ng, ny + ng, 1 - ng, nz + ng, false));
+        HIPCHK(st::launch_aux(s, 1 - ng, nx + ng, 1 - ng, 0, 1 - ng, nz + ng, false));
+        HIPCHK(st::launch_aux(s, 1 - ng, nx + ng, ny + 1, ny + ng, 1 - ng, nz + ng, false));
+        HIPCHK(st::launch_aux(s, 1 - ng, nx + ng, 1 - ng, ny + ng, 1 - ng, 0, false));
+        HIPCHK(st::launch_aux(s, 1 - ng, nx + ng, 1 - ng, ny + ng, nz + 1, nz + ng, false));
+    }
+    return ST_OK;
+}
+
+int st_visflx(st_solver *s, int32_t mode) {
+    NEED(s);
+    double *fl = s->F[s->cur];
+    switch (mode) {
+    case 0: HIPCHK(st::launch_visc(s, fl, 0, false)); break;
+    case 1: HIPCHK(st::launch_visc_div(s, fl, s->F[s->cur ^ 1], -1, false)); break;
+    case 3: HIPCHK(st::launch_sensor(s, false)); break;
+    case 4: HIPCHK(st::launch_visc(s, fl, 4, false)); break;
+    default: return fail(ST_ERR_ARG, "visflx mode " + std::to_string(mode) + " is not supported");
+    }
+    return ST_OK;
+}
+
+int st_euler_x(st_solver *s) {
+    NEED(s);
+    HIPCHK(st::launch_euler_x(s, s->F[s->cur], false));
+    return ST_OK;
+}
+int st_euler_y(st_solver *s) {
+    NEED(s);
+    HIPCHK(st::launch_euler_y(s, s->F[s->cur]));
+    return ST_OK;
+}
+int st_euler_z(st_solver *s) {
+    NEED(s);
+    HIPCHK(st::launch_euler_z(s, s->F[s->cur]));
+    return ST_OK;
+}
+
+int st_zero_flux(st_solver *s) {
+    NEED(s);
+    HIPCHK(st::launch_zero(s, s->F[0], (size_t)5 * s->g.nint));
+    HIPCHK(st::launch_zero(s, s->F[1], (size_t)5 * s->g.nint));
+    return ST_OK;
+}
+
+// Periodic ghost swap / extrapolation of one aux variable (bcswap_var,
+// bcextr_var); used by st_init_run and exposed through the tests.
+static int swap_var(st_solver *s, int var, bool extrapolate) {
+    double *a = s->w_aux + (size_t)(var - 1) * s->g.nvol;
+    if (extrapolate) HIPCHK(st::launch_extrapolate_var(s, a));
+    HIPCHK(st::launch_fill_periodic(s, a, 1));
+    return ST_OK;
+}
+
+int st_init_run(st_solver *s) {
+    NEED(s);
+    int rc;
+    if ((rc = st_update_ghost(s, 1))) return rc;
+    if ((rc = st_compute_aux(s, 1, 1))) return rc;
+    if ((rc = st_zero_flux(s))) return rc;
+    HIPCHK(st::launch_visc(s, s->F[s->cur], 0, false));
+    if ((rc = swap_var(s, 8, false))) return rc;
+    if ((rc = st_zero_flux(s))) return rc;
+    s->last_stage = -1;
+    return st_compute_dt(s, nullptr);
+}
+
+int st_compute_dt(st_solver *s, double *dt_out) {
+    NEED(s);
+    if (s->p.cfl >= 0) {
+        HIPCHK(st::launch_dt(s));
+    }
+    if (dt_out) {
+        int rc = pull_scalars(s);
+        if (rc) return rc;
+        *dt_out = s->hs->dt;
+    }
+    return ST_OK;
+}
+
+int st_rk_sync(st_solver *s) {
+    NEED(s);
+    return do_rk_sync(s);
+}
+
+int st_advance(st_solver *s, int32_t nsteps) {
+    NEED(s);
+    const int iter_dt = s->p.iter_dt_recompute > 0 ? s->p.iter_dt_recompute : 1;
+    for (int n = 0; n < nsteps; n++) {
+        s->icyc += 1;
+        if (s->p.cfl > 0 && ((s->icyc - s->icyc0) % iter_dt) == 0) TIMED(ST_K_DT, st::launch_dt(s));
+        int rc = do_rk_sync(s);
+        if (rc) return rc;
+        TIMED(ST_K_STEP_END, st::launch_step_end(s));
+    }
+    return ST_OK;
+}
+
+int st_kernel_times(st_solver *s, int32_t nsteps, double ms[ST_NKFAM], int32_t count[ST_NKFAM]) {
+    NEED(s);
+    if (!ms || !count || nsteps < 1) return fail(ST_ERR_ARG, "bad arguments");
+    KTimer *t = new (std::nothrow) KTimer();
+    if (!t) return fail(ST_ERR_NOMEM, "timer allocation failed");
+    t->n = 0;
+    const int need = nsteps * 40 < KTimer::kMax ? nsteps * 40 : KTimer::kMax;
+    int made = 0;
+    hipError_t e = hipSuccess;
+    for (; made < need && e == hipSuccess; made++) {
+        e = hipEventCreate(&t->ev[made][0]);
+        if (e == hipSuccess) e = hipEventCreate(&t->ev[made][1]);
+    }
+    int rc = ST_OK;
+    t->made = e == hipSuccess ? made : 0;
+    if (e != hipSuccess) {
+        rc = fail(ST_ERR_HIP, std::string("event creation: ") + hipGetErrorString(e));
+    } else {
+        s->timer = t;
+        rc = st_advance(s, nsteps);
+        s->timer = nullptr;
+        if (rc == ST_OK) {
+            e = hipStreamSynchronize(s->stream);
+            if (e != hipSuccess) rc = fail(ST_ERR_HIP, hipGetErrorString(e));
+        }
+        if (rc == ST_OK) {
+            for (int f = 0; f < ST_NKFAM; f++) { ms[f] = 0.0; count[f] = 0; }
+            for (int q = 0; q < t->n && q < made; q++) {
+                float el = 0.f;
+                if (hipEventElapsedTime(&el, t->ev[q][0], t->ev[q][1]) == hipSuccess) {
+                    ms[t->fam[q]] += el;
+                    count[t->fam[q]] += 1;
+                }
+            }
+            for (int f = 0; f < ST_NKFAM; f++)
+                if (count[f]) ms[f] /= count[f];
+        }
+    }
+    for (int q = 0; q < made; q++) {
+        (void)hipEventDestroy(t->ev[q][0]);
+        (void)hipEventDestroy(t->ev[q][1]);
+    }
+    s->timer = nullptr;
+    delete t;
+    return rc;
+}
+
+int st_compute_residual(st_solver *s, st_scalars *out) {
+    NEED(s);
+    HIPCHK(st::launch_residual(s));
+    st_scalars tmp;
+    int rc = st_get_scalars(s, out ? out : &tmp);
+    if (rc) return rc;
+    const double r = s->hs->residual;
+    if (r != r) return fail(ST_ERR_NAN, "NaN residual (boom!!!)");
+    return ST_OK;
+}
+
+int st_count_weno(st_solver *s, double frac_xyz[3]) {
+    NEED(s);
+    if (!frac_xyz) return fail(ST_ERR_ARG, "null output");
+    HIPCHK(st::launch_zero(s, s->ds->cnt, 3));
+    HIPCHK(st::launch_count_weno(s));
+    int rc = pull_scalars(s);
+    if (rc) return rc;
+    const st::KPar &k = s->kp;
+    const double nx = s->g.nx, ny = s->g.ny, nz = s->g.nz;
+    frac_xyz[0] = s->hs->cnt[0] / ((double)(k.eul_hi[0] - k.eul_lo[0] + 1) * ny * nz);
+    frac_xyz[1] = s->hs->cnt[1] / (nx * (double)(k.eul_hi[1] - k.eul_lo[1] + 1) * nz);
+    frac_xyz[2] = s->hs->cnt[2] / (nx * ny * (double)(k.eul_hi[2] - k.eul_lo[2] + 1));
+    return ST_OK;
+}
+
+int st_set_stream(st_solver *s, void *hip_stream) {
+    if (!s) return fail(ST_ERR_ARG, "null solver handle");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (hip_stream) {
+        if (s->own_stream) HIPCHK(hipStreamDestroy(s->stream));
+        s->stream = (hipStream_t)hip_stream;
+        s->own_stream = false;
+    } else if (!s->own_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->own_stream = true;
+    }
+    return ST_OK;
+}
+
+int st_synchronize(st_solver *s) {
+    if (!s) return fail(ST_ERR_ARG, "null solver handle");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
+}  // extern "C"

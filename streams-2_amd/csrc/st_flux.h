@@ -1,0 +1,393 @@
+// st_flux.h -- convective face flux of the hybrid scheme (device code).
+//
+// One face between stencil points s=H-1 and s=H of a 2H-point window
+// (H = 3: offsets -2..+3 around the face).  The window holds the primitive
+// variables rho,u,v,w,H,T of w_aux(1:6) plus the pressure p = T*rho*rgas0
+// and the internal energy e = H - p/rho - |u|^2/2 of each point, computed
+// once per point instead of once per point pair as in the reference (the
+// value is the same).  DIR selects the normal direction; the arithmetic of
+// each branch follows the reference kernel of that direction:
+//   central KEEP / non-KEEP split : kernels_cpu.F90:466-618 (x), 1854-2004 (y), 1109-1257 (z)
+//   Roe + WENO characteristic flux: kernels_cpu.F90:619-682 (x), 2005-2064 (y), 1258-1318 (z)
+//   compute_roe_average           : kernels_cpu.F90:3013-3089
+//   eigenvectors_x/y/z            : kernels_cpu.F90:2728-2782, 2842-2896, 2956-3010
+//   wenorec_1d                    : kernels_cpu.F90:2526-2724 (orders 1-3, WENO-JS and WENO-Z)
+#pragma once
+
+#include "st_internal.h"
+
+namespace st {
+
+struct Pt {
+    double r, u, v, w, h, t;  // w_aux(1:6)
+    double p, e;              // pressure and internal energy
+};
+
+__device__ __forceinline__ Pt make_pt(double r, double u, double v, double w, double h, double t, double rgas0) {
+    Pt q;
+    q.r = r; q.u = u; q.v = v; q.w = w; q.h = h; q.t = t;
+    q.p = t * r * rgas0;
+    q.e = h - q.p / r - 0.5 * (u * u + v * v + w * w);
+    return q;
+}
+
+template <int DIR>
+__device__ __forceinline__ double un_of(const Pt &q) {
+    return DIR == 0 ? q.u : (DIR == 1 ? q.v : q.w);
+}
+
+__device__ __forceinline__ double sqr(double a) { return a * a; }
+
+// ------------------------------------------------------------ central flux
+template <int DIR, int H>
+__device__ __forceinline__ void central_flux(const KPar &P, const Pt (&q)[2 * H], int lmax, bool zero_flux,
+                                             double fh[5]) {
+    double ft1 = 0.0, ft2 = 0.0, ft3 = 0.0, ft4 = 0.0, ft5 = 0.0, ft6 = 0.0;
+    const double *cd = P.cd1tab[lmax - 1];
+    const int nkeep = P.nkeep;
+#pragma unroll
+    for (int l = 1; l <= H; l++) {
+        if (l <= lmax) {
+            double uvs1 = 0.0, uvs2 = 0.0, uvs3 = 0.0, uvs4 = 0.0, uvs6 = 0.0;
+            double uvs5_i = 0.0, uvs5_k = 0.0, uvs5_p = 0.0, uvs5 = 0.0;
+#pragma unroll
+            for (int m = 0; m < l; m++) {
+                const Pt &a = q[H - 1 - m];
+                const Pt &b = q[H - 1 - m + l];
+                const double una = un_of<DIR>(a), unb = un_of<DIR>(b);
+                const double rhom = a.r + b.r;
+                if (nkeep >= 0) {
+                    const double eem = a.e + b.e;
+                    double uv_part = (una + unb) * rhom;
+                    double deef = 1.0;
+                    if (nkeep > 0) {
+                        // KEEP(n) correction factors (kernels_cpu.F90:505-531)
+                        double drho = 2.0 * (b.r - a.r) / rhom;
+                        double dee = 2.0 * (b.e - a.e) / eem;
+                        double t_sdr = (0.5 * drho) * (0.5 * drho);
+                        double t_sde = (0.5 * dee) * (0.5 * dee);
+                        const double t2_sdr = t_sdr, t2_sde = t_sde;
+                        double sdr = 1.0 + t_sdr / 3.0;
+                        double sde = 1.0 + t_sde;
+                        double sne = 1.0 + t_sde / 3.0;
+                        for (int n = 2; n <= nkeep; n++) {
+                            int n2 = 2 * n;
+                            t_sdr = t2_sdr * t_sdr;
+                            t_sde = t2_sde * t_sde;
+                            sdr = sdr + t_sdr / (1.0 + n2);
+                            sde = sde + t_sde;
+                            sne = sne + t_sde / (1.0 + n2);
+                        }
+                        uv_part = uv_part * (1.0 / sdr);
+                        deef = sne / sde;
+                    }
+                    uvs1 = uvs1 + uv_part * 2.0;
+                    uvs2 = uvs2 + uv_part * (a.u + b.u);
+                    uvs3 = uvs3 + uv_part * (a.v + b.v);
+                    uvs4 = uvs4 + uv_part * (a.w + b.w);
+                    uvs5_i = uvs5_i + uv_part * eem * deef;
+                    uvs5_k = uvs5_k + uv_part * (a.u * b.u + a.v * b.v + a.w * b.w);
+                    uvs5_p = uvs5_p + 4.0 * (una * b.p + unb * a.p);
+                    uvs6 = uvs6 + 2.0 * (a.p + b.p);
+                } else {
+                    const double uv_part = (una + unb) * rhom;
+                    uvs1 = uvs1 + uv_part * 2.0;
+                    uvs2 = uvs2 + uv_part * (a.u + b.u);
+                    uvs3 = uvs3 + uv_part * (a.v + b.v);
+                    uvs4 = uvs4 + uv_part * (a.w + b.w);
+                    uvs5 = uvs5 + uv_part * (a.h + b.h);
+                    uvs6 = uvs6 + 2.0 * (a.p + b.p);
+                }
+            }
+            const double c = cd[l - 1];
+            ft1 = ft1 + c * uvs1;
+            ft2 = ft2 + c * uvs2;
+            ft3 = ft3 + c * uvs3;
+            ft4 = ft4 + c * uvs4;
+            ft5 = ft5 + c * (nkeep >= 0 ? (uvs5_i + uvs5_k + uvs5_p) : uvs5);
+            ft6 = ft6 + c * uvs6;
+        }
+    }
+    fh[0] = 0.25 * ft1;
+    fh[1] = 0.25 * ft2;
+    fh[2] = 0.25 * ft3;
+    fh[3] = 0.25 * ft4;
+    fh[4] = 0.25 * ft5;
+    if (zero_flux) {
+        fh[0] = 0.0; fh[1] = 0.0; fh[2] = 0.0; fh[3] = 0.0; fh[4] = 0.0;
+    }
+    fh[1 + DIR] = fh[1 + DIR] + 0.5 * ft6;
+}
+
+// ------------------------------------------------------------ Roe average
+struct Roe {
+    double b1, b2, b3, c, ci, h, uu, vv, ww;
+};
+
+// compute_roe_average between points A (left) and B (right)
+__device__ __forceinline__ Roe roe_average(const KPar &P, const Pt &A, const Pt &B) {
+    Roe s;
+    double r = B.r / A.r;
+    r = sqrt(r);
+    const double rho = r * A.r;
+    const double rp1 = 1.0 / (r + 1.0);
+    const double uu = (r * B.u + A.u) * rp1;
+    const double vv = (r * B.v + A.v) * rp1;
+    const double ww = (r * B.w + A.w) * rp1;
+    const double h = (r * B.h + A.h) * rp1;
+    const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+    const double hbar = h - qq - cpc(P, P.indx_cp_r + 1) * P.t0;
+    double tt, gamloc;
+    if (P.calorically_perfect == 1) {
+        tt = P.t0 + hbar / cpc(P, 0);
+        gamloc = cpc(P, 0) / (cpc(P, 0) - P.rgas0);
+    } else {
+        const double t0 = P.t0;
+        tt = (r * B.t + A.t) * rp1;
+        double told = tt;
+        for (int iter = 1; iter <= 50; iter++) {
+            double num = 0.0, den = 0.0;
+            for (int ll = P.indx_cp_l; ll <= P.indx_cp_r; ll++) {
+                double tpow = ipow(told / t0, ll);
+                if (ll == -1) {
+                    den = den + cpc(P, ll) * tpow;
+                    num = num + cpc(P, ll) * log(told / t0);
+                } else {
+                    double tpowp = (told / t0) * tpow;
+                    den = den + cpc(P, ll) * tpow;
+                    num = num + cpc(P, ll) * (tpowp - 1.0) / (ll + 1.0);
+                }
+            }
+            num = num * t0;
+            tt = told + (hbar - num) / den;
+            if (fabs(tt - told) < P.tol_iter_nr) break;
+            told = tt;
+        }
+        double cploc = 0.0;
+        for (int ll = P.indx_cp_l; ll <= P.indx_cp_r; ll++) cploc = cploc + cpc(P, ll) * ipow(tt / t0, ll);
+        gamloc = cploc / (cploc - P.rgas0);
+    }
+    const double cc = gamloc * tt * P.rgas0;
+    const double gm1loc = gamloc - 1.0;
+    s.c = sqrt(cc);
+    s.ci = 1.0 / s.c;
+    const double p_rho = tt * P.rgas0;
+    const double p_e = rho * gm1loc;
+    const double etot = h - tt * P.rgas0;
+    s.b3 = etot - rho * p_rho / p_e;
+    s.b2 = p_e / (rho * cc);
+    s.b1 = p_rho / cc - s.b2 * (etot - 2.0 * qq);
+    s.h = h;
+    s.uu = uu;
+    s.vv = vv;
+    s.ww = ww;
+    return s;
+}
+
+// Left eigenvector m (component v = 0..4) and right eigenvector m of the
+// eigenvectors_x/y/z routines.  The y and z sets are the x set with the
+// velocity components permuted (normal, tangential 1, tangential 2) =
+// (v,u,w) for y and (w,u,v) for z; the component order stays (rho, rho*u,
+// rho*v, rho*w, rho*E) so the sums below run in the reference order.
+template <int DIR>
+__device__ __forceinline__ void eig_left(const Roe &s, int m, double el[5]) {
+    const double un = DIR == 0 ? s.uu : (DIR == 1 ? s.vv : s.ww);
+    const double t1 = DIR == 0 ? s.vv : s.uu;
+    const double t2 = DIR == 2 ? s.vv : s.ww;
+    const int in = 1 + DIR;                    // component of the normal momentum
+    const int it1 = DIR == 0 ? 2 : 1;          // component of tangential momentum 1
+    const int it2 = DIR == 2 ? 2 : 3;          // component of tangential momentum 2
+    double col[5];
+    if (m == 0 || m == 4) {
+        const double sg = m == 0 ? 1.0 : -1.0;
+        col[0] = 0.5 * (s.b1 + sg * un * s.ci);
+        col[in] = -0.5 * (s.b2 * un + sg * s.ci);
+        col[it1] = -0.5 * (s.b2 * t1);
+        col[it2] = -0.5 * (s.b2 * t2);
+        col[4] = 0.5 * s.b2;
+    } else if (m == 1) {
+        col[0] = 1.0 - s.b1;
+        col[1] = s.b2 * s.uu;
+        col[2] = s.b2 * s.vv;
+        col[3] = s.b2 * s.ww;
+        col[4] = -s.b2;
+    } else if (m == 2) {
+        col[0] = -t1; col[in] = 0.0; col[it1] = 1.0; col[it2] = 0.0; col[4] = 0.0;
+    } else {
+        col[0] = -t2; col[in] = 0.0; col[it1] = 0.0; col[it2] = 1.0; col[4] = 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) el[v] = col[v];
+}
+
+template <int DIR>
+__device__ __forceinline__ void eig_right(const Roe &s, int m, double er[5]) {
+    const double un = DIR == 0 ? s.uu : (DIR == 1 ? s.vv : s.ww);
+    const double t1 = DIR == 0 ? s.vv : s.uu;
+    const double t2 = DIR == 2 ? s.vv : s.ww;
+    const int in = 1 + DIR;
+    const int it1 = DIR == 0 ? 2 : 1;
+    const int it2 = DIR == 2 ? 2 : 3;
+    double row[5];
+    if (m == 0 || m == 4) {
+        const double sg = m == 0 ? -1.0 : 1.0;
+        row[0] = 1.0;
+        row[in] = m == 0 ? un - s.c : un + s.c;
+        row[it1] = t1;
+        row[it2] = t2;
+        row[4] = m == 0 ? s.h - un * s.c : s.h + un * s.c;
+        (void)sg;
+    } else if (m == 1) {
+        row[0] = 1.0; row[1] = s.uu; row[2] = s.vv; row[3] = s.ww; row[4] = s.b3;
+    } else if (m == 2) {
+        row[0] = 0.0; row[in] = 0.0; row[it1] = 1.0; row[it2] = 0.0; row[4] = t1;
+    } else {
+        row[0] = 0.0; row[in] = 0.0; row[it1] = 0.0; row[it2] = 1.0; row[4] = t2;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) er[v] = row[v];
+}
+
+// wenorec_1d for one characteristic field, iweno = 3 (window of 6 values:
+// vp/vm index 0..5 <-> l = 1..6 of the reference, i = 3 <-> index 2).
+__device__ __forceinline__ double wenorec3(const double vp[6], const double vm[6], int ord, int version,
+                                           double betascale) {
+#define VP(o) vp[2 + (o)]
+#define VM(o) vm[2 + (o)]
+    if (ord <= 1) return VP(0) + VM(1);
+    if (ord == 2) {
+        const double d0 = 1.0 / 3.0, d1 = 2.0 / 3.0;
+        double bp0 = sqr(VP(0) - VP(-1)), bp1 = sqr(VP(1) - VP(0));
+        bp1 = betascale * bp1;
+        bp0 = betascale * bp0;
+        double bm0 = sqr(VM(2) - VM(1)), bm1 = sqr(VM(1) - VM(0));
+        bm1 = betascale * bm1;
+        bm0 = betascale * bm0;
+        bp0 = d0 / sqr(0.000001 + bp0);
+        bm0 = d0 / sqr(0.000001 + bm0);
+        double sump = bp0, summ = bm0;
+        bp1 = d1 / sqr(0.000001 + bp1);
+        bm1 = d1 / sqr(0.000001 + bm1);
+        sump = sump + bp1;
+        summ = summ + bm1;
+        bp0 = bp0 / sump; bp1 = bp1 / sump;
+        bm0 = bm0 / summ; bm1 = bm1 / summ;
+        const double vminus = bp0 * (-VP(-1) + 3.0 * VP(0)) + bp1 * (VP(0) + VP(1));
+        const double vplus = bm0 * (-VM(2) + 3.0 * VM(1)) + bm1 * (VM(0) + VM(1));
+        return 0.5 * (vminus + vplus);
+    }
+    const double dw0 = 1.0 / 10.0, dw1 = 6.0 / 10.0, dw2 = 3.0 / 10.0;
+    const double d0 = 13.0 / 12.0, d1 = 1.0 / 4.0;
+    const double c0 = 1.0 / 3.0, c1 = 5.0 / 6.0, c2 = -1.0 / 6.0, c3 = -7.0 / 6.0, c4 = 11.0 / 6.0;
+    double bp2 = d0 * sqr(VP(0) - 2.0 * VP(1) + VP(2)) + d1 * sqr(3.0 * VP(0) - 4.0 * VP(1) + VP(2));
+    double bp1 = d0 * sqr(VP(-1) - 2.0 * VP(0) + VP(1)) + d1 * sqr(VP(-1) - VP(1));
+    double bp0 = d0 * sqr(VP(0) - 2.0 * VP(-1) + VP(-2)) + d1 * sqr(3.0 * VP(0) - 4.0 * VP(-1) + VP(-2));
+    bp2 = betascale * bp2;
+    bp1 = betascale * bp1;
+    bp0 = betascale * bp0;
+    double bm2 = d0 * sqr(VM(1) - 2.0 * VM(0) + VM(-1)) + d1 * sqr(3.0 * VM(1) - 4.0 * VM(0) + VM(-1));
+    double bm1 = d0 * sqr(VM(2) - 2.0 * VM(1) + VM(0)) + d1 * sqr(VM(2) - VM(0));
+    double bm0 = d0 * sqr(VM(1) - 2.0 * VM(2) + VM(3)) + d1 * sqr(3.0 * VM(1) - 4.0 * VM(2) + VM(3));
+    bm2 = betascale * bm2;
+    bm1 = betascale * bm1;
+    bm0 = betascale * bm0;
+    if (version == 0) {
+        bp0 = dw0 / sqr(0.000001 + bp0);
+        bm0 = dw0 / sqr(0.000001 + bm0);
+        bp1 = dw1 / sqr(0.000001 + bp1);
+        bm1 = dw1 / sqr(0.000001 + bm1);
+        bp2 = dw2 / sqr(0.000001 + bp2);
+        bm2 = dw2 / sqr(0.000001 + bm2);
+    } else {
+        // WENO-Z; eps40 is the single-precision literal 1.e-35 of the reference
+        const double eps40 = (double)1.e-35f;
+        const double tau5p = fabs(bp0 - bp2) + eps40;
+        const double tau5m = fabs(bm0 - bm2) + eps40;
+        bp0 = (bp0 + eps40) / (bp0 + tau5p);
+        bm0 = (bm0 + eps40) / (bm0 + tau5m);
+        bp1 = (bp1 + eps40) / (bp1 + tau5p);
+        bm1 = (bm1 + eps40) / (bm1 + tau5m);
+        bp2 = (bp2 + eps40) / (bp2 + tau5p);
+        bm2 = (bm2 + eps40) / (bm2 + tau5m);
+        bp0 = dw0 / bp0; bm0 = dw0 / bm0;
+        bp1 = dw1 / bp1; bm1 = dw1 / bm1;
+        bp2 = dw2 / bp2; bm2 = dw2 / bm2;
+    }
+    const double sump = bp0 + bp1 + bp2;
+    const double summ = bm0 + bm1 + bm2;
+    bp0 = bp0 / sump; bp1 = bp1 / sump; bp2 = bp2 / sump;
+    bm0 = bm0 / summ; bm1 = bm1 / summ; bm2 = bm2 / summ;
+    const double vminus = bp2 * (c0 * VP(0) + c1 * VP(1) + c2 * VP(2)) +
+                          bp1 * (c2 * VP(-1) + c1 * VP(0) + c0 * VP(1)) +
+                          bp0 * (c0 * VP(-2) + c3 * VP(-1) + c4 * VP(0));
+    const double vplus = bm2 * (c0 * VM(1) + c1 * VM(0) + c2 * VM(-1)) +
+                         bm1 * (c2 * VM(2) + c1 * VM(1) + c0 * VM(0)) +
+                         bm0 * (c0 * VM(3) + c3 * VM(2) + c4 * VM(1));
+    return vminus + vplus;
+#undef VP
+#undef VM
+}
+
+// Characteristic WENO flux at the face of a 6-point window (H = 3).
+template <int DIR>
+__device__ __forceinline__ void weno_flux(const KPar &P, const Pt (&q)[6], int eoc, double fh[5]) {
+    const Roe s = roe_average(P, q[2], q[3]);
+    double evmax[3] = {-1.0, -1.0, -1.0};  // |un-c|, |un|, |un+c|
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+        const double un = un_of<DIR>(q[l]);
+        const double tt = q[l].t;
+        const double gamloc = gamloc_of(P, tt);
+        const double c = sqrt(gamloc * P.rgas0 * tt);
+        evmax[0] = fmax(fabs(un - c), evmax[0]);
+        evmax[1] = fmax(fabs(un), evmax[1]);
+        evmax[2] = fmax(fabs(un + c), evmax[2]);
+    }
+    int ord = P.weno_scheme + eoc;
+    if (ord < 1) ord = 1;
+    const double u0_2 = P.u0 * P.u0;
+    const double r2u2 = P.rho0 * P.rho0 * u0_2;
+    const double r2u4 = r2u2 * u0_2;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        double el[5];
+        eig_left<DIR>(s, m, el);
+        const double ev = m == 0 ? evmax[0] : (m == 4 ? evmax[2] : evmax[1]);
+        double gp[6], gm[6];
+#pragma unroll
+        for (int l = 0; l < 6; l++) {
+            const Pt &a = q[l];
+            const double rhoun = a.r * un_of<DIR>(a);
+            const double pp = a.r * a.t * P.rgas0;
+            double f1 = a.u * rhoun, f2 = a.v * rhoun, f3 = a.w * rhoun;
+            if (DIR == 0) f1 = f1 + pp;
+            if (DIR == 1) f2 = f2 + pp;
+            if (DIR == 2) f3 = f3 + pp;
+            double wc = 0.0, gc = 0.0;
+            wc = wc + el[0] * a.r;
+            gc = gc + el[0] * rhoun;
+            wc = wc + el[1] * a.r * a.u;
+            gc = gc + el[1] * f1;
+            wc = wc + el[2] * a.r * a.v;
+            gc = gc + el[2] * f2;
+            wc = wc + el[3] * a.r * a.w;
+            gc = gc + el[3] * f3;
+            wc = wc + el[4] * (a.r * a.h - pp);
+            gc = gc + el[4] * (a.h * rhoun);
+            const double c = 0.5 * (gc + ev * wc);
+            gp[l] = c;
+            gm[l] = gc - c;
+        }
+        const double bs = (m == 2 || m == 3) ? 1.0 / r2u4 : 1.0 / r2u2;
+        const double fi = wenorec3(gp, gm, ord, P.weno_version, bs);
+        double er[5];
+        eig_right<DIR>(s, m, er);
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[v] = acc[v] + er[v] * fi;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) fh[v] = acc[v];
+}
+
+}  // namespace st

@@ -1,0 +1,156 @@
+// st_internal.h -- internal types and device helpers of libstreams_mi355x.
+//
+// Device layout: every ghosted array keeps the reference layout, one
+// contiguous Fortran-order block per variable (x fastest, lower bound 1-ng),
+// so host<->device copies of w are single memcpys and a wavefront's 64
+// consecutive lanes read 64 consecutive doubles along x.  The right-hand
+// side fl is double-buffered: F[cur] holds the RHS of the stage being built,
+// F[cur^1] the RHS of the previous stage, which replaces the reference's fln
+// register (init_flux/update_flux, kernels_cpu.F90:29-47,3400-3418).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/streams_mi355x.h"
+
+namespace st {
+
+// ---------------------------------------------------------------- geometry
+struct Geo {
+    int nx, ny, nz, ng;
+    int sx, sy, sz;      // ghosted extents
+    int sxy;             // sx*sy
+    int nvol;            // points of one ghosted variable
+    int nint;            // points of one interior variable
+};
+
+__host__ __device__ __forceinline__ int gidx(const Geo &g, int i, int j, int k) {
+    return (i + g.ng - 1) + g.sx * ((j + g.ng - 1) + g.sy * (k + g.ng - 1));
+}
+__host__ __device__ __forceinline__ int iidx(const Geo &g, int i, int j, int k) {
+    return (i - 1) + g.nx * ((j - 1) + g.ny * (k - 1));
+}
+
+// ------------------------------------------------------- kernel parameters
+// The scalars a kernel needs, passed by value.
+struct KPar {
+    int lmax_base;         // ep_order/2
+    int visc_lmax;         // visc_order/2
+    int weno_scheme, weno_version, nkeep;
+    int calorically_perfect, indx_cp_l, indx_cp_r;
+    int visc_model;
+    int fzf[6];            // force_zero_flux
+    int periodic[3];
+    int wall_stag[2];      // ymin / ymax staggered isothermal wall
+    int eul_lo[3], eul_hi[3];
+    int ndim;
+    double rgas0, prandtl, t0, rho0, u0, l0, mu0;
+    double sutherland_s, t_ref_dim, vtexp;
+    double sensor_threshold, t_wall, t_bulk_target, volchan, cfl, tol_iter_nr;
+    double cp[ST_MAX_CP_COEFF], cv[ST_MAX_CP_COEFF];
+    double cd1[4];         // first-derivative weights of ep_order (l = 1..4)
+    double vd1[4];         // first-derivative weights of visc_order
+    double vd2[5];         // second-derivative weights of visc_order (l = 0..4)
+    double cd1tab[4][4];   // coeff_deriv1(l, lmax) for lmax = 1..4 (boundary orders)
+};
+
+// Device-resident scalars of a run.
+struct DevScalars {
+    double dt, time, dpdx, rhobulk, ubulk, tbulk;
+    double b1, b2;                   // bulk5g(1:2)/volchan of the last force_rhs
+    double residual, rhomin, rhomax, tmin, tmax, pmin, pmax;
+    double bulkt;
+    unsigned long long dtinv_bits;   // running max of the CFL spectral radii
+    long long icyc;
+    double cnt[3];                   // WENO face counts
+};
+
+// ------------------------------------------------------------- EOS helpers
+// x**n for integer n by binary exponentiation (the reference's integer power)
+__device__ __forceinline__ double ipow(double x, int n) {
+    bool inv = n < 0;
+    unsigned un = inv ? (unsigned)(-n) : (unsigned)n;
+    double r = 1.0, b = x;
+    while (un) {
+        if (un & 1u) r *= b;
+        b *= b;
+        un >>= 1;
+    }
+    return inv ? 1.0 / r : r;
+}
+
+__device__ __forceinline__ double cpc(const KPar &P, int l) { return P.cp[l - P.indx_cp_l]; }
+__device__ __forceinline__ double cvc(const KPar &P, int l) { return P.cv[l - P.indx_cp_l]; }
+
+// get_gamloc_dev, kernels_cpu.F90:7096-7113
+__device__ __forceinline__ double gamloc_of(const KPar &P, double tt) {
+    double cploc;
+    if (P.calorically_perfect == 1) {
+        cploc = cpc(P, 0);
+    } else {
+        cploc = 0.0;
+        for (int l = P.indx_cp_l; l <= P.indx_cp_r; l++) cploc = cploc + cpc(P, l) * ipow(tt / P.t0, l);
+    }
+    return cploc / (cploc - P.rgas0);
+}
+
+__device__ __forceinline__ double cploc_of(const KPar &P, double tt) {
+    if (P.calorically_perfect == 1) return cpc(P, 0);
+    double cploc = 0.0;
+    for (int l = P.indx_cp_l; l <= P.indx_cp_r; l++) cploc = cploc + cpc(P, l) * ipow(tt / P.t0, l);
+    return cploc;
+}
+
+// get_temperature_from_e_dev, kernels_cpu.F90:7116-7152
+__device__ __forceinline__ double temperature_from_e(const KPar &P, double ee, double t_start) {
+    const double t0 = P.t0;
+    double ebar = ee - cvc(P, P.indx_cp_r + 1) * t0;
+    if (P.calorically_perfect == 1) return t0 + ebar / cvc(P, 0);
+    double t_old = t_start, tt = t_start;
+    for (int iter = 1; iter <= 50; iter++) {
+        double den = 0.0, num = 0.0;
+        for (int l = P.indx_cp_l; l <= P.indx_cp_r; l++) {
+            double t_pow = ipow(t_old / t0, l);
+            if (l == -1) {
+                den = den + cvc(P, l) * t_pow;
+                num = num + cvc(P, l) * log(t_old / t0);
+            } else {
+                double t_powp = (t_old / t0) * t_pow;
+                den = den + cvc(P, l) * t_pow;
+                num = num + cvc(P, l) * (t_powp - 1.0) / (l + 1.0);
+            }
+        }
+        num = num * t0;
+        tt = t_old + (ebar - num) / den;
+        if (fabs(tt - t_old) < P.tol_iter_nr) break;
+        t_old = tt;
+    }
+    return tt;
+}
+
+// get_e_from_temperature_dev, kernels_cpu.F90:7155-7176
+__device__ __forceinline__ double e_from_temperature(const KPar &P, double tt) {
+    const double t0 = P.t0;
+    double ee = cvc(P, P.indx_cp_r + 1);
+    if (P.calorically_perfect == 1) {
+        ee = ee + cvc(P, 0) * (tt / t0 - 1.0);
+    } else {
+        for (int l = P.indx_cp_l; l <= P.indx_cp_r; l++) {
+            if (l == -1) ee = ee + cvc(P, l) * log(tt / t0);
+            else ee = ee + cvc(P, l) / (l + 1.0) * (ipow(tt / t0, l + 1) - 1.0);
+        }
+    }
+    return ee * t0;
+}
+
+// viscosity law of eval_aux_subroutine, kernels_cpu.F90:6998-7005
+__device__ __forceinline__ double viscosity_of(const KPar &P, double tt) {
+    if (P.visc_model == 1) return P.mu0 * pow(tt / P.t0, P.vtexp);
+    if (P.visc_model == 2)
+        return P.mu0 * pow(tt / P.t0, 1.5) * (1.0 + P.sutherland_s / P.t_ref_dim) /
+               (tt / P.t0 + P.sutherland_s / P.t_ref_dim);
+    return 0.0;
+}
+
+}  // namespace st

@@ -1,0 +1,1055 @@
+// st_kernels.hip -- gfx950 kernels of one RK stage and their launchers.
+//
+// Stage pipeline of the fused path (st_rk_sync), one kernel per line:
+//   aux        w -> w_aux(1:7) over the ghosted box, periodic ghosts read
+//              from their images (replaces bcswap + compute_aux)
+//   euler_x    x faces + divergence, writes F[cur] (replaces init_flux's
+//              fl=0 and the first accumulation)
+//   visc       Laplacian viscous terms, w_aux(9,10) and the ghosts of
+//              w_aux(10) (bcextr_var + bcswap_var)
+//   euler_y    y faces, marching along y, accumulates into F[cur]
+//   euler_z    z faces, marching along z
+//   sensor     stage 3 only: Ducros sensor and its periodic ghosts
+//   visc_div   grad(div/3) terms + block partial sums of force_rhs
+//   finish     bulk sums -> dpdx, rho_b, u_b, T_b (one workgroup)
+//   update     fln = -rho_k dt F[prev] - gamma_k dt F[cur] - forcing,
+//              w += fln, isothermal-wall ghosts
+// Reductions use per-block partials summed in a fixed order, so a run is
+// bitwise reproducible; the CFL maximum uses an atomic max on the bit
+// pattern of non-negative doubles, which is order independent.
+#include "st_flux.h"
+#include "st_solver.h"
+
+namespace st {
+
+constexpr int H = 3;           // stencil half width of the flux windows
+constexpr int XSEG = 63;       // cells per 64-lane x segment
+constexpr int RED_THREADS = 256;
+
+// -------------------------------------------------------------- utilities
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
+    return v;
+}
+
+// Block sum of NV values in a fixed order; result valid in thread 0.
+__device__ __forceinline__ int flat_block() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+__device__ __forceinline__ int flat_thread() { return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z); }
+
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[NV]) {
+    const int ft = flat_thread();
+    const int lane = ft & 63;
+    const int wid = ft >> 6;
+    const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = wave_sum(v[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) lds[wid][q] = v[q];
+    }
+    __syncthreads();
+    if (ft == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double a = lds[0][q];
+            for (int w = 1; w < nw; w++) a += lds[w][q];
+            v[q] = a;
+        }
+    }
+}
+
+// ------------------------------------------------------------ ghost fills
+// Periodic copy of bcswap (base_cpu.F90:410-450, 669-745 with the local
+// neighbour shortcut :1382-1417) for nv ghosted variables: faces of the
+// periodic directions only, edges and corners untouched.
+__global__ void k_fill_periodic(Geo g, KPar P, double *__restrict__ a, int nv) {
+    const int ng = g.ng;
+    const int nxf = P.periodic[0] ? 2 * ng * g.ny * g.nz : 0;
+    const int nyf = P.periodic[1] ? 2 * ng * g.nx * g.nz : 0;
+    const int nzf = (P.periodic[2] && P.ndim == 3) ? 2 * ng * g.nx * g.ny : 0;
+    const int tot = nxf + nyf + nzf;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot * nv; t += gridDim.x * blockDim.x) {
+        const int m = t / tot;
+        int r = t - m * tot;
+        int dst, src;
+        if (r < nxf) {
+            const int l = r % (2 * ng);
+            const int jk = r / (2 * ng);
+            const int j = jk % g.ny + 1, k = jk / g.ny + 1;
+            if (l < ng) { dst = gidx(g, 1 - ng + l, j, k); src = gidx(g, g.nx - ng + 1 + l, j, k); }
+            else { dst = gidx(g, g.nx + 1 + (l - ng), j, k); src = gidx(g, 1 + (l - ng), j, k); }
+        } else if ((r -= nxf) < nyf) {
+            const int i = r % g.nx + 1;
+            const int lk = r / g.nx;
+            const int l = lk % (2 * ng), k = lk / (2 * ng) + 1;
+            if (l < ng) { dst = gidx(g, i, 1 - ng + l, k); src = gidx(g, i, g.ny - ng + 1 + l, k); }
+            else { dst = gidx(g, i, g.ny + 1 + (l - ng), k); src = gidx(g, i, 1 + (l - ng), k); }
+        } else {
+            r -= nyf;
+            const int i = r % g.nx + 1;
+            const int jl = r / g.nx;
+            const int j = jl % g.ny + 1, l = jl / g.ny;
+            if (l < ng) { dst = gidx(g, i, j, 1 - ng + l); src = gidx(g, i, j, g.nz - ng + 1 + l); }
+            else { dst = gidx(g, i, j, g.nz + 1 + (l - ng)); src = gidx(g, i, j, 1 + (l - ng)); }
+        }
+        double *b = a + (size_t)m * g.nvol;
+        b[dst] = b[src];
+    }
+}
+
+// bcextr_var_subroutine (kernels_cpu.F90:6199-6254): zero-gradient ghosts of
+// one variable on all six faces.
+__global__ void k_extrapolate_var(Geo g, double *__restrict__ a) {
+    const int ng = g.ng;
+    const int nxf = 2 * ng * g.ny * g.nz, nyf = 2 * ng * g.nx * g.nz, nzf = 2 * ng * g.nx * g.ny;
+    const int tot = nxf + nyf + nzf;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
+        int r = t, dst, src;
+        if (r < nxf) {
+            const int l = r % (2 * ng), jk = r / (2 * ng);
+            const int j = jk % g.ny + 1, k = jk / g.ny + 1;
+            if (l < ng) { dst = gidx(g, -l, j, k); src = gidx(g, 1, j, k); }
+            else { dst = gidx(g, g.nx + 1 + (l - ng), j, k); src = gidx(g, g.nx, j, k); }
+        } else if ((r -= nxf) < nyf) {
+            const int i = r % g.nx + 1, lk = r / g.nx;
+            const int l = lk % (2 * ng), k = lk / (2 * ng) + 1;
+            if (l < ng) { dst = gidx(g, i, -l, k); src = gidx(g, i, 1, k); }
+            else { dst = gidx(g, i, g.ny + 1 + (l - ng), k); src = gidx(g, i, g.ny, k); }
+        } else {
+            r -= nyf;
+            const int i = r % g.nx + 1, jl = r / g.nx;
+            const int j = jl % g.ny + 1, l = jl / g.ny;
+            if (l < ng) { dst = gidx(g, i, j, -l); src = gidx(g, i, j, 1); }
+            else { dst = gidx(g, i, j, g.nz + 1 + (l - ng)); src = gidx(g, i, j, g.nz); }
+        }
+        a[dst] = a[src];
+    }
+}
+
+// bcwall_staggered_subroutine (kernels_cpu.F90:6612-6678) for one ghost
+// point: the mirror of interior point js, written at jg.
+__device__ __forceinline__ void wall_ghost(const Geo &g, const KPar &P, double *__restrict__ w,
+                                           const double *__restrict__ wa, int i, int js, int jg, int k) {
+    const int s = gidx(g, i, js, k), d = gidx(g, i, jg, k);
+    const size_t nv = g.nvol;
+    double rho = w[s];
+    const double uu = w[nv + s] / rho;
+    const double vv = w[2 * nv + s] / rho;
+    const double ww = w[3 * nv + s] / rho;
+    const double rhoe = w[4 * nv + s];
+    const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+    double ee = rhoe / rho - qq;
+    double tt = temperature_from_e(P, ee, wa[5 * nv + s]);
+    const double pp = rho * tt * P.rgas0;
+    tt = 2.0 * P.t_wall - tt;
+    ee = e_from_temperature(P, tt);
+    rho = pp / tt / P.rgas0;
+    w[d] = rho;
+    w[nv + d] = -rho * uu;
+    w[2 * nv + d] = -rho * vv;
+    w[3 * nv + d] = -rho * ww;
+    w[4 * nv + d] = rho * (ee + qq);
+}
+
+__global__ void k_wall_ghosts(Geo g, KPar P, double *__restrict__ w, const double *__restrict__ wa) {
+    const int tot = 2 * g.ng * g.nx * g.nz;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
+        const int i = t % g.nx + 1;
+        const int r = t / g.nx;
+        const int l = r % (2 * g.ng);
+        const int k = r / (2 * g.ng) + 1;
+        if (l < g.ng) {
+            if (P.wall_stag[0]) wall_ghost(g, P, w, wa, i, l + 1, -l, k);
+        } else {
+            const int ll = l - g.ng + 1;
+            if (P.wall_stag[1]) wall_ghost(g, P, w, wa, i, g.ny + 1 - ll, g.ny + ll, k);
+        }
+    }
+}
+
+// -------------------------------------------------------------------- aux
+// eval_aux_subroutine (kernels_cpu.F90:6960-7011) on the box
+// [i0,i1]x[j0,j1]x[k0,k1]; with image=true a ghost point of a periodic
+// direction reads w at its periodic image, which is the value bcswap
+// would have copied there.
+__global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__restrict__ w, double *__restrict__ wa,
+                                             int i0, int ni, int j0, int nj, int k0, int nk, int image) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * nj * nk) return;
+    const int i = i0 + t % ni;
+    const int jk = t / ni;
+    const int j = j0 + jk % nj;
+    const int k = k0 + jk / nj;
+    int is = i, js = j, ks = k;
+    if (image) {
+        if (P.periodic[0]) is = i < 1 ? i + g.nx : (i > g.nx ? i - g.nx : i);
+        if (P.periodic[1]) js = j < 1 ? j + g.ny : (j > g.ny ? j - g.ny : j);
+        if (P.periodic[2]) ks = k < 1 ? k + g.nz : (k > g.nz ? k - g.nz : k);
+    }
+    const int s = gidx(g, is, js, ks), d = gidx(g, i, j, k);
+    const size_t nv = g.nvol;
+    const double rho = w[s], rhou = w[nv + s], rhov = w[2 * nv + s], rhow = w[3 * nv + s], rhoe = w[4 * nv + s];
+    const double ri = 1.0 / rho;
+    const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
+    const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+    const double ee = rhoe / rho - qq;
+    const double tt = temperature_from_e(P, ee, wa[5 * nv + d]);
+    const double pp = rho * tt * P.rgas0;
+    wa[d] = rho;
+    wa[nv + d] = uu;
+    wa[2 * nv + d] = vv;
+    wa[3 * nv + d] = ww;
+    wa[4 * nv + d] = (rhoe + pp) / rho;
+    wa[5 * nv + d] = tt;
+    wa[6 * nv + d] = viscosity_of(P, tt);
+}
+
+// --------------------------------------------------------------- Euler x
+// One 64-lane workgroup per 63-cell segment of an x row.  Lane t computes
+// face i0-1+t into LDS; lanes 1..63 then form the divergence of their cell
+// (euler_x_fluxes_hybrid_kernel + euler_x_update_subroutine,
+// kernels_cpu.F90:425-689, 944-963).  init = true writes the divergence
+// (the stage's first contribution after init_flux's fl = 0).
+template <bool INIT>
+__global__ __launch_bounds__(64) void k_euler_x(Geo g, KPar P, const double *__restrict__ wa,
+                                                const int *__restrict__ eoc, const double *__restrict__ dcsidx,
+                                                double *__restrict__ fl) {
+    __shared__ double sfh[5][64];
+    const int nseg = (g.nx + XSEG - 1) / XSEG;
+    const int seg = blockIdx.x % nseg;
+    const int row = blockIdx.x / nseg;
+    const int j = row % g.ny + 1, k = row / g.ny + 1;
+    const int lane = threadIdx.x;
+    const int i0 = 1 + XSEG * seg;
+    const int ilast = min(i0 + XSEG - 1, g.nx);
+    const int f = i0 - 1 + lane;
+    const int lo = P.eul_lo[0], hi = P.eul_hi[0];
+    if (f <= ilast && f >= lo - 1 && f <= hi) {
+        const size_t nv = g.nvol;
+        const int base = gidx(g, f, j, k);
+        Pt q[2 * H];
+        bool shk = false;
+#pragma unroll
+        for (int s = 0; s < 2 * H; s++) {
+            const int o = base + s - (H - 1);
+            q[s] = make_pt(wa[o], wa[nv + o], wa[2 * nv + o], wa[3 * nv + o], wa[4 * nv + o], wa[5 * nv + o], P.rgas0);
+            if (s >= H - P.weno_scheme && s < H + P.weno_scheme) shk |= wa[7 * nv + o] > P.sensor_threshold;
+        }
+        const int e = eoc[f];
+        double fh[5];
+        if (!shk) {
+            int lmax = P.lmax_base + e;
+            lmax = lmax < 1 ? 1 : lmax;
+            const bool zf = (f == 0 && P.fzf[0] == 1) || (f == g.nx && P.fzf[1] == 1);
+            central_flux<0, H>(P, q, lmax, zf, fh);
+        } else {
+            weno_flux<0>(P, q, e, fh);
+        }
+#pragma unroll
+        for (int m = 0; m < 5; m++) sfh[m][lane] = fh[m];
+    }
+    __syncthreads();
+    if (lane >= 1 && f <= ilast) {
+        const int i = f;
+        const int o = iidx(g, i, j, k);
+        const bool in = i >= lo && i <= hi;
+        const double d = dcsidx[i - 1];
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            if (INIT) {
+                fl[(size_t)m * g.nint + o] = in ? (sfh[m][lane] - sfh[m][lane - 1]) * d : 0.0;
+            } else if (in) {
+                fl[(size_t)m * g.nint + o] += (sfh[m][lane] - sfh[m][lane - 1]) * d;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ Euler y / z
+// One thread per (i, row) marching over a chunk of cells along DIR with a
+// sliding 6-point window in registers; each w_aux value is loaded once per
+// chunk and each face flux computed once (euler_y/z_hybrid_kernel +
+// euler_y/z_update_subroutine, kernels_cpu.F90:1808-2071, 991-1010 and
+// 1063-1325, 1039-1058).
+template <int DIR>
+__global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double *__restrict__ wa,
+                                                     const int *__restrict__ eoc, const double *__restrict__ metric,
+                                                     double *__restrict__ fl, int chunk, int nchunk) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int no = DIR == 1 ? g.nz : g.ny;  // the other transverse extent
+    if (t >= g.nx * no * nchunk) return;
+    const int i = t % g.nx + 1;
+    const int r = t / g.nx;
+    const int o = r % no + 1;
+    const int c = r / no;
+    const int lo = P.eul_lo[DIR], hi = P.eul_hi[DIR];
+    const int c0 = lo + c * chunk;
+    if (c0 > hi) return;
+    const int c1 = min(c0 + chunk - 1, hi);
+    const int n = DIR == 1 ? g.ny : g.nz;
+    const int stride = DIR == 1 ? g.sx : g.sxy;
+    const size_t nv = g.nvol;
+    // point index along DIR -> ghosted offset
+    const int p0 = DIR == 1 ? gidx(g, i, 0, o) : gidx(g, i, o, 0);  // offset of along-index 0
+    Pt q[6];
+    double sen[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        const int a = p0 + (c0 - 3 + s) * stride;
+        q[s] = make_pt(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0);
+        sen[s] = wa[7 * nv + a];
+    }
+    double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int ws = P.weno_scheme;
+    for (int f = c0 - 1; f <= c1; f++) {
+        bool shk = false;
+#pragma unroll
+        for (int s = 0; s < 6; s++)
+            if (s >= H - ws && s < H + ws) shk |= sen[s] > P.sensor_threshold;
+        const int e = eoc[f];
+        double fh[5];
+        if (!shk) {
+            int lmax = P.lmax_base + e;
+            lmax = lmax < 1 ? 1 : lmax;
+            const bool zf = (f == 0 && P.fzf[2 * DIR] == 1) || (f == n && P.fzf[2 * DIR + 1] == 1);
+            central_flux<DIR, H>(P, q, lmax, zf, fh);
+        } else {
+            weno_flux<DIR>(P, q, e, fh);
+        }
+        if (f >= c0) {
+            const int oi = DIR == 1 ? iidx(g, i, f, o) : iidx(g, i, o, f);
+            const double d = metric[f - 1];
+#pragma unroll
+            for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] += (fh[m] - fprev[m]) * d;
+        }
+#pragma unroll
+        for (int m = 0; m < 5; m++) fprev[m] = fh[m];
+        if (f < c1) {
+#pragma unroll
+            for (int s = 0; s < 5; s++) {
+                q[s] = q[s + 1];
+                sen[s] = sen[s + 1];
+            }
+            const int a = p0 + (f + 4) * stride;
+            q[5] = make_pt(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0);
+            sen[5] = wa[7 * nv + a];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- viscous
+// Ghost images of an interior value: periodic copy for periodic directions
+// (bcswap_var), zero-gradient copy otherwise (bcextr_var).
+__device__ __forceinline__ void write_images(const Geo &g, const KPar &P, double *__restrict__ a, int i, int j, int k,
+                                             double v, bool extrapolate) {
+    const int ng = g.ng;
+    if (P.periodic[0]) {
+        if (i <= ng) a[gidx(g, i + g.nx, j, k)] = v;
+        if (i > g.nx - ng) a[gidx(g, i - g.nx, j, k)] = v;
+    } else if (extrapolate) {
+        if (i == 1) for (int l = 1; l <= ng; l++) a[gidx(g, 1 - l, j, k)] = v;
+        if (i == g.nx) for (int l = 1; l <= ng; l++) a[gidx(g, g.nx + l, j, k)] = v;
+    }
+    if (P.periodic[1]) {
+        if (j <= ng) a[gidx(g, i, j + g.ny, k)] = v;
+        if (j > g.ny - ng) a[gidx(g, i, j - g.ny, k)] = v;
+    } else if (extrapolate) {
+        if (j == 1) for (int l = 1; l <= ng; l++) a[gidx(g, i, 1 - l, k)] = v;
+        if (j == g.ny) for (int l = 1; l <= ng; l++) a[gidx(g, i, g.ny + l, k)] = v;
+    }
+    if (P.periodic[2] && P.ndim == 3) {
+        if (k <= ng) a[gidx(g, i, j, k + g.nz)] = v;
+        if (k > g.nz - ng) a[gidx(g, i, j, k - g.nz)] = v;
+    } else if (extrapolate) {
+        if (k == 1) for (int l = 1; l <= ng; l++) a[gidx(g, i, j, 1 - l)] = v;
+        if (k == g.nz) for (int l = 1; l <= ng; l++) a[gidx(g, i, j, g.nz + l)] = v;
+    }
+}
+
+// visflx_subroutine (mode 0, kernels_cpu.F90:3598-3777) and
+// visflx_nosensor_subroutine (mode 4, :3996-4180).
+template <bool SENSOR>
+__global__ __launch_bounds__(256) void k_visc(Geo g, KPar P, double *__restrict__ wa, double *__restrict__ fl,
+                                              const double *__restrict__ dx1, const double *__restrict__ dxs,
+                                              const double *__restrict__ dx2, const double *__restrict__ dy1,
+                                              const double *__restrict__ dys, const double *__restrict__ dy2,
+                                              const double *__restrict__ dz1, const double *__restrict__ dzs,
+                                              const double *__restrict__ dz2, double *__restrict__ wallprop,
+                                              int images) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const size_t nv = g.nvol;
+    const int c = gidx(g, i, j, k);
+    const double *U = wa + nv, *V = wa + 2 * nv, *W = wa + 3 * nv, *T = wa + 5 * nv, *M = wa + 6 * nv;
+    const double uu = U[c], vv = V[c], ww = W[c], tt = T[c], mu = M[c];
+    const int lmax = P.visc_lmax;
+    double ux = 0, vx = 0, wx = 0, tx = 0, mux = 0;
+    double uy = 0, vy = 0, wy = 0, ty = 0, muy = 0;
+    double uz = 0, vz = 0, wz = 0, tz = 0, muz = 0;
+    const double c20 = P.vd2[0];
+    double ulapx = c20 * uu, ulapy = ulapx, ulapz = ulapx;
+    double vlapx = c20 * vv, vlapy = vlapx, vlapz = vlapx;
+    double wlapx = c20 * ww, wlapy = wlapx, wlapz = wlapx;
+    double tlapx = c20 * tt, tlapy = tlapx, tlapz = tlapx;
+    const int sy = g.sx, sz = g.sxy;
+#pragma unroll
+    for (int l = 1; l <= 4; l++) {
+        if (l <= lmax) {
+            const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
+            const int p = c + l, m = c - l;
+            ux = ux + ccl * (U[p] - U[m]);
+            ulapx = ulapx + clapl * (U[p] + U[m]);
+            vx = vx + ccl * (V[p] - V[m]);
+            vlapx = vlapx + clapl * (V[p] + V[m]);
+            wx = wx + ccl * (W[p] - W[m]);
+            wlapx = wlapx + clapl * (W[p] + W[m]);
+            tx = tx + ccl * (T[p] - T[m]);
+            tlapx = tlapx + clapl * (T[p] + T[m]);
+            mux = mux + ccl * (M[p] - M[m]);
+        }
+    }
+#pragma unroll
+    for (int l = 1; l <= 4; l++) {
+        if (l <= lmax) {
+            const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
+            const int p = c + l * sy, m = c - l * sy;
+            uy = uy + ccl * (U[p] - U[m]);
+            ulapy = ulapy + clapl * (U[p] + U[m]);
+            vy = vy + ccl * (V[p] - V[m]);
+            vlapy = vlapy + clapl * (V[p] + V[m]);
+            wy = wy + ccl * (W[p] - W[m]);
+            wlapy = wlapy + clapl * (W[p] + W[m]);
+            ty = ty + ccl * (T[p] - T[m]);
+            tlapy = tlapy + clapl * (T[p] + T[m]);
+            muy = muy + ccl * (M[p] - M[m]);
+        }
+    }
+#pragma unroll
+    for (int l = 1; l <= 4; l++) {
+        if (l <= lmax) {
+            const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
+            const int p = c + l * sz, m = c - l * sz;
+            uz = uz + ccl * (U[p] - U[m]);
+            ulapz = ulapz + clapl * (U[p] + U[m]);
+            vz = vz + ccl * (V[p] - V[m]);
+            vlapz = vlapz + clapl * (V[p] + V[m]);
+            wz = wz + ccl * (W[p] - W[m]);
+            wlapz = wlapz + clapl * (W[p] + W[m]);
+            tz = tz + ccl * (T[p] - T[m]);
+            tlapz = tlapz + clapl * (T[p] + T[m]);
+            muz = muz + ccl * (M[p] - M[m]);
+        }
+    }
+    const double ax = dx1[i - 1], ay = dy1[j - 1], az = dz1[k - 1];
+    ux = ux * ax; vx = vx * ax; wx = wx * ax; tx = tx * ax; mux = mux * ax;
+    uy = uy * ay; vy = vy * ay; wy = wy * ay; ty = ty * ay; muy = muy * ay;
+    uz = uz * az; vz = vz * az; wz = wz * az; tz = tz * az; muz = muz * az;
+    const double sx_ = dxs[i - 1], sy_ = dys[j - 1], sz_ = dzs[k - 1];
+    const double qx = dx2[i - 1], qy = dy2[j - 1], qz = dz2[k - 1];
+    ulapx = ulapx * sx_ + ux * qx; vlapx = vlapx * sx_ + vx * qx;
+    wlapx = wlapx * sx_ + wx * qx; tlapx = tlapx * sx_ + tx * qx;
+    ulapy = ulapy * sy_ + uy * qy; vlapy = vlapy * sy_ + vy * qy;
+    wlapy = wlapy * sy_ + wy * qy; tlapy = tlapy * sy_ + ty * qy;
+    ulapz = ulapz * sz_ + uz * qz; vlapz = vlapz * sz_ + vz * qz;
+    wlapz = wlapz * sz_ + wz * qz; tlapz = tlapz * sz_ + tz * qz;
+    const double ulap = ulapx + ulapy + ulapz;
+    const double vlap = vlapx + vlapy + vlapz;
+    const double wlap = wlapx + wlapy + wlapz;
+    const double tlap = tlapx + tlapy + tlapz;
+    const double div = ux + vy + wz;
+    const double div3l = div / 3.0;
+    wa[9 * nv + c] = div3l;
+    const double omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
+    const double omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
+    wa[8 * nv + c] = sqrt(omod2);
+    if (SENSOR) {
+        const double u0l0 = P.u0 / P.l0;
+        double sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
+        sv = fmax(sv, 0.0);
+        wa[7 * nv + c] = sv * sv;
+        if (images) write_images(g, P, wa + 7 * nv, i, j, k, sv * sv, false);
+    }
+    if (images && !SENSOR) write_images(g, P, wa + 9 * nv, i, j, k, div3l, true);
+    const double cploc = cploc_of(P, tt);
+    const double sig11 = 2.0 * (ux - div3l), sig12 = uy + vx, sig13 = uz + wx;
+    const double sig22 = 2.0 * (vy - div3l), sig23 = vz + wy, sig33 = 2.0 * (wz - div3l);
+    const double sigx = mux * sig11 + muy * sig12 + muz * sig13 + mu * ulap;
+    const double sigy = mux * sig12 + muy * sig22 + muz * sig23 + mu * vlap;
+    const double sigz = mux * sig13 + muy * sig23 + muz * sig33 + mu * wlap;
+    const double sigqt = (mux * tx + muy * ty + muz * tz + mu * tlap) * cploc / P.prandtl;
+    const double sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig12 * vx + sig22 * vy + sig23 * vz +
+                          sig13 * wx + sig23 * wy + sig33 * wz) * mu;
+    const double sigq = sigx * uu + sigy * vv + sigz * ww + sigah + sigqt;
+    const int oi = iidx(g, i, j, k);
+    const size_t ni = g.nint;
+    fl[ni + oi] -= sigx;
+    fl[2 * ni + oi] -= sigy;
+    fl[3 * ni + oi] -= sigz;
+    fl[4 * ni + oi] -= sigq;
+    if (j == 1) {
+        const size_t wsz = (size_t)g.sx * g.sz;
+        const size_t wp = (size_t)(i + g.ng - 1) + (size_t)g.sx * (k + g.ng - 1);
+        wallprop[wp] = mu * uy;
+        wallprop[wsz + wp] = mu * wy;
+        wallprop[2 * wsz + wp] = mu * ty * cploc / P.prandtl;
+    }
+}
+
+// sensor_subroutine (kernels_cpu.F90:4726-4744) + bcswap_var of w_aux(8)
+__global__ __launch_bounds__(256) void k_sensor(Geo g, KPar P, double *__restrict__ wa, int images) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const size_t nv = g.nvol;
+    const int c = gidx(g, i, j, k);
+    const double om = wa[8 * nv + c];
+    const double omod2 = om * om;
+    const double div = 3.0 * wa[9 * nv + c];
+    const double u0l0 = P.u0 / P.l0;
+    double sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
+    sv = fmax(sv, 0.0);
+    wa[7 * nv + c] = sv * sv;
+    if (images) write_images(g, P, wa + 7 * nv, i, j, k, sv * sv, false);
+}
+
+// visflx_div_subroutine (kernels_cpu.F90:3484-3532).  With REDUCE the
+// kernel also forms fln = -rho_k dt F[prev] - gamma_k dt F[cur] of
+// update_flux (:3400-3418) for rho and rho*u and writes the block partial
+// sums of force_rhs_1_subroutine (:3157-3193).
+template <bool REDUCE>
+__global__ __launch_bounds__(256) void k_visc_div(Geo g, KPar P, const double *__restrict__ wa, double *__restrict__ fl,
+                                                  const double *__restrict__ flp, const double *__restrict__ w,
+                                                  const double *__restrict__ dx1, const double *__restrict__ dy1,
+                                                  const double *__restrict__ dz1, const double *__restrict__ yn,
+                                                  const DevScalars *__restrict__ ds, double rkr, double rkg,
+                                                  double *__restrict__ partials) {
+    __shared__ double lds[4][5];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i <= g.nx && j <= g.ny) {
+        const size_t nv = g.nvol;
+        const int c = gidx(g, i, j, k);
+        const double *D = wa + 9 * nv;
+        const double uu = wa[nv + c], vv = wa[2 * nv + c], ww = wa[3 * nv + c], mu = wa[6 * nv + c];
+        double dxv = 0.0, dyv = 0.0, dzv = 0.0;
+        const int sy = g.sx, sz = g.sxy;
+#pragma unroll
+        for (int l = 1; l <= 4; l++) {
+            if (l <= P.visc_lmax) {
+                const double ccl = P.vd1[l - 1];
+                dxv = dxv + ccl * (D[c + l] - D[c - l]);
+                dyv = dyv + ccl * (D[c + l * sy] - D[c - l * sy]);
+                dzv = dzv + ccl * (D[c + l * sz] - D[c - l * sz]);
+            }
+        }
+        dxv = dxv * dx1[i - 1];
+        dyv = dyv * dy1[j - 1];
+        dzv = dzv * dz1[k - 1];
+        const double sigx = mu * dxv, sigy = mu * dyv, sigz = mu * dzv;
+        const double sigq = sigx * uu + sigy * vv + sigz * ww;
+        const int oi = iidx(g, i, j, k);
+        const size_t ni = g.nint;
+        const double f2 = fl[ni + oi] - sigx;
+        fl[ni + oi] = f2;
+        fl[2 * ni + oi] -= sigy;
+        fl[3 * ni + oi] -= sigz;
+        fl[4 * ni + oi] -= sigq;
+        if (REDUCE) {
+            const double dt = ds->dt;
+            const double rhodt = rkr * dt, gamdt = rkg * dt;
+            const double dy = yn[j] - yn[j - 1];
+            const double fln1 = -rhodt * flp[oi] - gamdt * fl[oi];
+            const double fln2 = -rhodt * flp[ni + oi] - gamdt * f2;
+            const double w1 = w[c], w2 = w[nv + c];
+            acc[0] = fln1 * dy;
+            acc[1] = fln2 * dy;
+            acc[2] = w1 * dy;
+            acc[3] = w2 * dy;
+            acc[4] = w2 * dy * wa[5 * nv + c];
+        }
+    }
+    if (REDUCE) {
+        block_sum<5>(acc, lds);
+        if (flat_thread() == 0) {
+            double *pp = partials + (size_t)flat_block() * 5;
+#pragma unroll
+            for (int q = 0; q < 5; q++) pp[q] = acc[q];
+        }
+    }
+}
+
+// force_rhs (singleideal_cpu.F90:845-889): sums of the partials, bulk values
+__global__ __launch_bounds__(RED_THREADS) void k_force_finish(KPar P, const double *__restrict__ partials, int nblk,
+                                                              DevScalars *ds, int first_stage) {
+    __shared__ double lds[RED_THREADS / 64][5];
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) acc[q] += partials[(size_t)b * 5 + q];
+    }
+    block_sum<5>(acc, lds);
+    if (threadIdx.x == 0) {
+        const double v = P.volchan;
+        const double b1 = acc[0] / v, b2 = acc[1] / v, b3 = acc[2] / v, b4 = acc[3] / v, b5 = acc[4] / v;
+        ds->dpdx = (first_stage ? 0.0 : ds->dpdx) + b2;
+        ds->rhobulk = b3;
+        ds->ubulk = b4 / ds->rhobulk;
+        ds->tbulk = b5 / ds->rhobulk / ds->ubulk;
+        ds->b1 = b1;
+        ds->b2 = b2;
+    }
+}
+
+// update_flux + force_rhs_2 + update_field (kernels_cpu.F90:3400-3442,
+// 3092-3116) fused, then the isothermal-wall ghosts of update_ghost
+// (bcwall_staggered) from the new state when force_var is off.  With
+// force_var on, also recovers T from the new E (force_var_1_subroutine,
+// :3241-3278) and writes the block partial sums of rho*u*T*dy.
+template <bool FORCE_VAR>
+__global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restrict__ w, double *__restrict__ wa,
+                                                const double *__restrict__ flp, const double *__restrict__ fl,
+                                                const DevScalars *__restrict__ ds, double rkr, double rkg, int channel,
+                                                const double *__restrict__ yn, double *__restrict__ partials) {
+    __shared__ double lds[4][1];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double acc[1] = {0.0};
+    if (i <= g.nx && j <= g.ny) {
+        const size_t nv = g.nvol, ni = g.nint;
+        const int c = gidx(g, i, j, k);
+        const int oi = iidx(g, i, j, k);
+        const double dt = ds->dt;
+        const double rhodt = rkr * dt, gamdt = rkg * dt;
+        double fln[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) fln[m] = -rhodt * flp[m * ni + oi] - gamdt * fl[m * ni + oi];
+        if (channel) {
+            const double b1 = ds->b1, b2 = ds->b2;
+            const double uu = wa[nv + c];
+            fln[0] = fln[0] - b1;
+            fln[1] = fln[1] - b2;
+            fln[4] = fln[4] - uu * b2;
+        }
+        double wn[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            wn[m] = w[m * nv + c] + fln[m];
+            w[m * nv + c] = wn[m];
+        }
+        if (FORCE_VAR) {
+            const double dy = yn[j] - yn[j - 1];
+            const double ri = 1.0 / wn[0];
+            const double uu = wn[1] * ri, vv = wn[2] * ri, ww = wn[3] * ri;
+            const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+            const double ee = wn[4] / wn[0] - qq;
+            const double tt = temperature_from_e(P, ee, wa[5 * nv + c]);
+            wa[5 * nv + c] = tt;
+            acc[0] = wn[1] * tt * dy;
+        } else {
+            if (P.wall_stag[0] && j <= g.ng) wall_ghost(g, P, w, wa, i, j, 1 - j, k);
+            if (P.wall_stag[1] && j > g.ny - g.ng) wall_ghost(g, P, w, wa, i, j, 2 * g.ny + 1 - j, k);
+        }
+    }
+    if (FORCE_VAR) {
+        block_sum<1>(acc, lds);
+        if (flat_thread() == 0) partials[flat_block()] = acc[0];
+    }
+}
+
+// force_var (singleideal_cpu.F90:890-933, grid_dim 1) bulk temperature
+__global__ __launch_bounds__(RED_THREADS) void k_force_var_finish(KPar P, const double *__restrict__ partials, int nblk,
+                                                                  DevScalars *ds) {
+    __shared__ double lds[RED_THREADS / 64][1];
+    double acc[1] = {0.0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc[0] += partials[b];
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) {
+        double bulktg = acc[0] / P.volchan;
+        bulktg = bulktg / ds->rhobulk / ds->ubulk;
+        ds->bulkt = P.t_bulk_target - bulktg;
+    }
+}
+
+// force_var_2_subroutine (kernels_cpu.F90:3282-3314) + wall ghosts
+__global__ __launch_bounds__(256) void k_force_var2(Geo g, KPar P, double *__restrict__ w, const double *__restrict__ wa,
+                                                    const DevScalars *__restrict__ ds) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const size_t nv = g.nvol;
+    const int c = gidx(g, i, j, k);
+    const double rho = w[c], rhou = w[nv + c], rhov = w[2 * nv + c], rhow = w[3 * nv + c];
+    const double ttnew = wa[5 * nv + c] + ds->bulkt;
+    const double ee = e_from_temperature(P, ttnew);
+    w[4 * nv + c] = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
+}
+
+// ---------------------------------------------------------------- CFL dt
+// compute_aux(central=1, ghost=0) + compute_dt_subroutine (kernels_cpu.F90:
+// 6960-7011, 6814-6880): primitives recomputed from w in registers (w_aux
+// written only for T when cp(T) needs it as the Newton start), block max of
+// the nine spectral radii, atomic max of the block maxima.
+__global__ __launch_bounds__(256) void k_dt(Geo g, KPar P, const double *__restrict__ w, double *__restrict__ wa,
+                                            const double *__restrict__ dx1, const double *__restrict__ dy1,
+                                            const double *__restrict__ dz1, const double *__restrict__ dxs,
+                                            const double *__restrict__ dys, const double *__restrict__ dzs,
+                                            DevScalars *ds) {
+    __shared__ double lds[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double mx = 0.0;
+    if (i <= g.nx && j <= g.ny) {
+        const size_t nv = g.nvol;
+        const int c = gidx(g, i, j, k);
+        const double rho = w[c];
+        const double ri = 1.0 / rho;
+        const double uu = w[nv + c] * ri, vv = w[2 * nv + c] * ri, ww = w[3 * nv + c] * ri;
+        const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+        const double ee = w[4 * nv + c] / rho - qq;
+        const double tt = temperature_from_e(P, ee, wa[5 * nv + c]);
+        if (P.calorically_perfect != 1) wa[5 * nv + c] = tt;
+        const double mu = viscosity_of(P, tt);
+        const double nu = ri * mu;
+        const double kor = nu / P.prandtl;
+        const double gamloc = gamloc_of(P, tt);
+        const double cs = sqrt(gamloc * P.rgas0 * tt);
+        const double ax = dx1[i - 1], ay = dy1[j - 1], az = dz1[k - 1];
+        const double sx = dxs[i - 1], sy = dys[j - 1], sz = dzs[k - 1];
+        mx = fmax(mx, (fabs(uu) + cs) * ax);
+        mx = fmax(mx, (fabs(vv) + cs) * ay);
+        mx = fmax(mx, (fabs(ww) + cs) * az);
+        mx = fmax(mx, nu * sx);
+        mx = fmax(mx, nu * sy);
+        mx = fmax(mx, nu * sz);
+        mx = fmax(mx, kor * sx);
+        mx = fmax(mx, kor * sy);
+        mx = fmax(mx, kor * sz);
+    }
+    mx = wave_max(mx);
+    const int lane = flat_thread() & 63, wid = flat_thread() >> 6;
+    if (lane == 0) lds[wid] = mx;
+    __syncthreads();
+    if (flat_thread() == 0) {
+        const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
+        double m = lds[0];
+        for (int q = 1; q < nw; q++) m = fmax(m, lds[q]);
+        atomicMax(&ds->dtinv_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// dt = cfl / max (singleideal_cpu.F90:1507-1514); resets the running max
+__global__ void k_dt_finish(KPar P, DevScalars *ds) {
+    const double dtinv_max = __longlong_as_double((long long)ds->dtinv_bits);
+    const double dt_min = 1.0 / dtinv_max;
+    ds->dt = P.cfl * dt_min;
+    ds->dtinv_bits = 0ull;
+}
+
+// end of a step of run(): icyc += 1, time += dt (singleideal_cpu.F90:1604,1642)
+__global__ void k_step_end(DevScalars *ds) {
+    ds->icyc = ds->icyc + 1;
+    ds->time = ds->time + ds->dt;
+}
+
+// ------------------------------------------------------------- residual
+// compute_residual_subroutine + compute_rho_t_p_minmax_subroutine
+// (kernels_cpu.F90:6682-6701, 6765-6810): fln(2) of the last stage is
+// rebuilt from the two stored right-hand sides and the forcing.
+__global__ __launch_bounds__(256) void k_residual(Geo g, KPar P, const double *__restrict__ wa,
+                                                  const double *__restrict__ flp, const double *__restrict__ fl,
+                                                  const DevScalars *__restrict__ ds, double rkr, double rkg, int channel,
+                                                  double *__restrict__ partials) {
+    __shared__ double lds[4][7];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double r = 0.0;
+    double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+    double mxv[3] = {-100.0, -100.0, -100.0};
+    if (i <= g.nx && j <= g.ny) {
+        const size_t nv = g.nvol, ni = g.nint;
+        const int c = gidx(g, i, j, k), oi = iidx(g, i, j, k);
+        const double dt = ds->dt;
+        double fln2 = -(rkr * dt) * flp[ni + oi] - (rkg * dt) * fl[ni + oi];
+        if (channel) fln2 = fln2 - ds->b2;
+        const double q = fln2 / dt;
+        r = q * q;
+        const double rho = wa[c], tt = wa[5 * nv + c];
+        const double pp = rho * tt * P.rgas0;
+        mn[0] = rho; mn[1] = tt; mn[2] = pp;
+        mxv[0] = rho; mxv[1] = tt; mxv[2] = pp;
+    }
+    r = wave_sum(r);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        mn[q] = wave_min(mn[q]);
+        mxv[q] = wave_max(mxv[q]);
+    }
+    const int lane = flat_thread() & 63, wid = flat_thread() >> 6;
+    if (lane == 0) {
+        lds[wid][0] = r;
+        for (int q = 0; q < 3; q++) { lds[wid][1 + q] = mn[q]; lds[wid][4 + q] = mxv[q]; }
+    }
+    __syncthreads();
+    if (flat_thread() == 0) {
+        const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
+        double o[7];
+        for (int q = 0; q < 7; q++) o[q] = lds[0][q];
+        for (int wv = 1; wv < nw; wv++) {
+            o[0] += lds[wv][0];
+            for (int q = 1; q < 4; q++) o[q] = fmin(o[q], lds[wv][q]);
+            for (int q = 4; q < 7; q++) o[q] = fmax(o[q], lds[wv][q]);
+        }
+        double *pp = partials + (size_t)flat_block() * 7;
+        for (int q = 0; q < 7; q++) pp[q] = o[q];
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_residual_finish(Geo g, const double *__restrict__ partials, int nblk,
+                                                                 DevScalars *ds) {
+    __shared__ double lds[RED_THREADS][7];
+    double o[7] = {0.0, 1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308, -100.0, -100.0, -100.0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const double *pp = partials + (size_t)b * 7;
+        o[0] += pp[0];
+        for (int q = 1; q < 4; q++) o[q] = fmin(o[q], pp[q]);
+        for (int q = 4; q < 7; q++) o[q] = fmax(o[q], pp[q]);
+    }
+    for (int q = 0; q < 7; q++) lds[threadIdx.x][q] = o[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int t = 1; t < (int)blockDim.x; t++) {
+            o[0] += lds[t][0];
+            for (int q = 1; q < 4; q++) o[q] = fmin(o[q], lds[t][q]);
+            for (int q = 4; q < 7; q++) o[q] = fmax(o[q], lds[t][q]);
+        }
+        const double restemp = (double)g.nx * (double)g.ny * (double)g.nz;
+        ds->residual = sqrt(o[0] / restemp);
+        ds->rhomin = o[1]; ds->tmin = o[2]; ds->pmin = o[3];
+        ds->rhomax = o[4]; ds->tmax = o[5]; ds->pmax = o[6];
+    }
+}
+
+// count_weno_subroutine (kernels_cpu.F90:52-113)
+__global__ __launch_bounds__(256) void k_count_weno(Geo g, KPar P, const double *__restrict__ wa, DevScalars *ds) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double cnt[3] = {0.0, 0.0, 0.0};
+    if (i <= g.nx && j <= g.ny) {
+        const double *S = wa + 7 * (size_t)g.nvol;
+        const int c = gidx(g, i, j, k);
+        const int ws = P.weno_scheme;
+        const int st[3] = {1, g.sx, g.sxy};
+        const int idx[3] = {i, j, k};
+        for (int d = 0; d < 3; d++) {
+            if (idx[d] < P.eul_lo[d] || idx[d] > P.eul_hi[d]) continue;
+            bool shk = false;
+            for (int s = -ws + 1; s <= ws; s++) shk |= S[c + s * st[d]] > P.sensor_threshold;
+            cnt[d] = shk ? 1.0 : 0.0;
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        double v = wave_sum(cnt[d]);
+        if ((flat_thread() & 63) == 0 && v > 0.0) atomicAdd(&ds->cnt[d], v);
+    }
+}
+
+__global__ void k_zero(double *__restrict__ a, size_t n) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) a[t] = 0.0;
+}
+
+// ================================================================ launchers
+static inline dim3 cell_grid(const Geo &g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz); }
+static inline int cell_blocks(const Geo &g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz; }
+static const dim3 kCellBlock(64, 4, 1);
+
+static inline int grid1d(size_t n, int bs) {
+    size_t b = (n + bs - 1) / bs;
+    return (int)(b > 65535 * 16 ? 65535 * 16 : (b < 1 ? 1 : b));
+}
+
+hipError_t launch_fill_periodic(st_solver *s, double *arr, int nv) {
+    const Geo &g = s->g;
+    size_t tot = (size_t)2 * g.ng * ((size_t)g.ny * g.nz + (size_t)g.nx * g.nz + (size_t)g.nx * g.ny) * nv;
+    hipLaunchKernelGGL(k_fill_periodic, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g, s->kp, arr, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_extrapolate_var(st_solver *s, double *arr) {
+    const Geo &g = s->g;
+    size_t tot = (size_t)2 * g.ng * ((size_t)g.ny * g.nz + (size_t)g.nx * g.nz + (size_t)g.nx * g.ny);
+    hipLaunchKernelGGL(k_extrapolate_var, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g, arr);
+    return hipGetLastError();
+}
+
+hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int k1, bool image) {
+    const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+    if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
+    const size_t tot = (size_t)ni * nj * nk;
+    hipLaunchKernelGGL(k_aux, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, s->g, s->kp, s->w,
+                       s->w_aux, i0, ni, j0, nj, k0, nk, image ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_wall_ghosts(st_solver *s) {
+    const Geo &g = s->g;
+    if (!s->kp.wall_stag[0] && !s->kp.wall_stag[1]) return hipSuccess;
+    size_t tot = (size_t)2 * g.ng * g.nx * g.nz;
+    hipLaunchKernelGGL(k_wall_ghosts, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g, s->kp, s->w, s->w_aux);
+    return hipGetLastError();
+}
+
+hipError_t launch_euler_x(st_solver *s, double *fl, bool init) {
+    const Geo &g = s->g;
+    if (s->kp.eul_hi[0] < s->kp.eul_lo[0]) {
+        if (init) return launch_zero(s, fl, (size_t)5 * g.nint);
+        return hipSuccess;
+    }
+    const int nseg = (g.nx + XSEG - 1) / XSEG;
+    dim3 grid((unsigned)(nseg * g.ny * g.nz));
+    if (init)
+        hipLaunchKernelGGL(k_euler_x<true>, grid, dim3(64), 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl);
+    else
+        hipLaunchKernelGGL(k_euler_x<false>, grid, dim3(64), 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl);
+    return hipGetLastError();
+}
+
+static hipError_t launch_march(st_solver *s, double *fl, int dir) {
+    const Geo &g = s->g;
+    const int lo = s->kp.eul_lo[dir], hi = s->kp.eul_hi[dir];
+    if (hi < lo) return hipSuccess;
+    const int ncell = hi - lo + 1;
+    const int no = dir == 1 ? g.nz : g.ny;
+    // enough threads to fill the chip: aim at >= 4 waves per SIMD over 256 CUs
+    const long target = 256L * 4 * 4 * 64;
+    int nchunk = (int)((target + (long)g.nx * no - 1) / ((long)g.nx * no));
+    nchunk = nchunk < 1 ? 1 : nchunk;
+    int chunk = (ncell + nchunk - 1) / nchunk;
+    if (chunk < 8) chunk = 8 < ncell ? 8 : ncell;
+    nchunk = (ncell + chunk - 1) / chunk;
+    const size_t tot = (size_t)g.nx * no * nchunk;
+    dim3 grid((unsigned)((tot + 255) / 256));
+    if (dir == 1)
+        hipLaunchKernelGGL(k_euler_march<1>, grid, dim3(256), 0, s->stream, g, s->kp, s->w_aux, s->eoc[1], s->detady, fl,
+                           chunk, nchunk);
+    else
+        hipLaunchKernelGGL(k_euler_march<2>, grid, dim3(256), 0, s->stream, g, s->kp, s->w_aux, s->eoc[2], s->dzitdz, fl,
+                           chunk, nchunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_euler_y(st_solver *s, double *fl) { return launch_march(s, fl, 1); }
+hipError_t launch_euler_z(st_solver *s, double *fl) { return launch_march(s, fl, 2); }
+
+hipError_t launch_visc(st_solver *s, double *fl, int mode, bool ghost_images) {
+    const Geo &g = s->g;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_visc<true>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, s->dcsidx,
+                           s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2,
+                           s->wallprop, ghost_images ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_visc<false>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, s->dcsidx,
+                           s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2,
+                           s->wallprop, ghost_images ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_sensor(st_solver *s, bool ghost_images) {
+    hipLaunchKernelGGL(k_sensor, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, s->w_aux, ghost_images ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_visc_div(st_solver *s, double *fl, const double *fl_prev, int stage, bool reduce) {
+    const Geo &g = s->g;
+    const double rkr = stage >= 0 ? s->rhork[stage] : 0.0, rkg = stage >= 0 ? s->gamrk[stage] : 0.0;
+    if (reduce)
+        hipLaunchKernelGGL(k_visc_div<true>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, fl_prev,
+                           s->w, s->dcsidx, s->detady, s->dzitdz, s->yn, s->ds, rkr, rkg, s->partials);
+    else
+        hipLaunchKernelGGL(k_visc_div<false>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, fl_prev,
+                           s->w, s->dcsidx, s->detady, s->dzitdz, s->yn, s->ds, rkr, rkg, s->partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_finish(st_solver *s, int stage) {
+    hipLaunchKernelGGL(k_force_finish, dim3(1), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(s->g),
+                       s->ds, stage == 0 ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_update(st_solver *s, const double *fl_prev, const double *fl, int stage) {
+    const Geo &g = s->g;
+    const bool fv = s->p.channel_case && s->p.theta_wall >= -1.0;
+    if (fv)
+        hipLaunchKernelGGL(k_update<true>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, s->w_aux, fl_prev, fl,
+                           s->ds, s->rhork[stage], s->gamrk[stage], s->p.channel_case, s->yn, s->partials);
+    else
+        hipLaunchKernelGGL(k_update<false>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, s->w_aux, fl_prev,
+                           fl, s->ds, s->rhork[stage], s->gamrk[stage], s->p.channel_case, s->yn, s->partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_var(st_solver *s) {
+    const Geo &g = s->g;
+    hipLaunchKernelGGL(k_force_var_finish, dim3(1), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(g),
+                       s->ds);
+    hipLaunchKernelGGL(k_force_var2, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, s->w_aux, s->ds);
+    return launch_wall_ghosts(s);
+}
+
+hipError_t launch_dt(st_solver *s) {
+    const Geo &g = s->g;
+    hipLaunchKernelGGL(k_dt, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, s->w_aux, s->dcsidx, s->detady,
+                       s->dzitdz, s->dcsidxs, s->detadys, s->dzitdzs, s->ds);
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(1), 0, s->stream, s->kp, s->ds);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_end(st_solver *s) {
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, s->stream, s->ds);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(st_solver *s) {
+    const Geo &g = s->g;
+    const int st = s->last_stage < 0 ? 0 : s->last_stage;
+    const double rkr = s->last_stage < 0 ? 0.0 : s->rhork[st];
+    const double rkg = s->last_stage < 0 ? 0.0 : s->gamrk[st];
+    hipLaunchKernelGGL(k_residual, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, s->F[s->cur ^ 1],
+                       s->F[s->cur], s->ds, rkr, rkg, s->p.channel_case, s->partials);
+    hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(RED_THREADS), 0, s->stream, g, s->partials, cell_blocks(g), s->ds);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_weno(st_solver *s) {
+    hipLaunchKernelGGL(k_count_weno, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, s->w_aux, s->ds);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(st_solver *s, double *a, size_t n) {
+    hipLaunchKernelGGL(k_zero, dim3(grid1d(n, 256)), dim3(256), 0, s->stream, a, n);
+    return hipGetLastError();
+}
+
+}  // namespace st

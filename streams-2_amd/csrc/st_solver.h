@@ -1,0 +1,67 @@
+// st_solver.h -- host-side solver state shared by the C ABI and the launchers.
+#pragma once
+
+#include "st_internal.h"
+
+struct st_solver {
+    st_params p;
+    st::Geo g;
+    st::KPar kp;
+    hipStream_t stream;
+    bool own_stream;
+    int device;
+    // state
+    double *w;          // 5 ghosted variables
+    double *w_aux;      // 10 ghosted variables
+    double *F[2];       // right-hand sides of the current and the previous stage
+    double *wallprop;   // 3 x (sx*sz), wallprop(:,:,2:4)
+    int *eoc[3];        // ep_ord_change per direction, faces 0..n
+    // grid
+    double *x, *y, *z, *yn;
+    double *dcsidx, *dcsidxs, *dcsidx2, *detady, *detadys, *detady2, *dzitdz, *dzitdzs, *dzitdz2;
+    // scalars and reduction scratch
+    st::DevScalars *ds;
+    st::DevScalars *hs;  // pinned host mirror
+    double *partials;
+    int npart;           // capacity in doubles
+    int cur;             // F[cur] is the right-hand side being built
+    double rhork[3], gamrk[3];
+    int last_stage;      // RK stage (0..2) of the last completed stage, -1 none
+    bool grid_set;
+    int64_t icyc, icyc0; // host mirror of the step counter
+    // optional per-kernel timing (st_kernel_times)
+    struct KTimer *timer;
+};
+
+// HIP events around each kernel family while st_kernel_times runs
+struct KTimer {
+    static const int kMax = 1 << 15;
+    hipEvent_t ev[kMax][2];
+    int fam[kMax];
+    int n;      // slots used
+    int made;   // slots whose events exist
+};
+
+namespace st {
+
+// launchers (st_kernels.hip); all enqueue on s->stream and return a hipError_t
+hipError_t launch_fill_periodic(st_solver *s, double *arr, int nv);
+hipError_t launch_extrapolate_var(st_solver *s, double *arr);
+hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int k1, bool image);
+hipError_t launch_wall_ghosts(st_solver *s);
+hipError_t launch_euler_x(st_solver *s, double *fl, bool init);
+hipError_t launch_euler_y(st_solver *s, double *fl);
+hipError_t launch_euler_z(st_solver *s, double *fl);
+hipError_t launch_visc(st_solver *s, double *fl, int mode, bool ghost_images);
+hipError_t launch_sensor(st_solver *s, bool ghost_images);
+hipError_t launch_visc_div(st_solver *s, double *fl, const double *fl_prev, int stage, bool reduce);
+hipError_t launch_force_finish(st_solver *s, int stage);
+hipError_t launch_update(st_solver *s, const double *fl_prev, const double *fl, int stage);
+hipError_t launch_force_var(st_solver *s);
+hipError_t launch_dt(st_solver *s);
+hipError_t launch_step_end(st_solver *s);
+hipError_t launch_residual(st_solver *s);
+hipError_t launch_count_weno(st_solver *s);
+hipError_t launch_zero(st_solver *s, double *a, size_t n);
+
+}  // namespace st

@@ -1,0 +1,147 @@
+"""Host-side mirror of the reference backend object for one flow solve.
+
+``ChannelSolver`` exposes the type-bound procedures of
+``equation_singleideal_cpu_object`` (code/src/singleideal/singleideal_cpu.F90)
+that the hot loop uses -- ``update_ghost``, ``compute_aux``, ``visflx``,
+``euler_x/y/z``, ``compute_dt``, ``rk_sync``, ``compute_residual``,
+``count_weno`` -- each a thin call through the C ABI of libstreams_mi355x.
+Fields are numpy arrays in the reference's Fortran layout, indexed
+[m, k, j, i] (ghosted extents nz+2ng, ny+2ng, nx+2ng).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .grid import Grid
+from .params import StParams, StScalars
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class ChannelSolver:
+    def __init__(self, params: StParams, grid: Grid):
+        self.L = capi.lib()
+        self.p = params
+        h = C.c_void_p()
+        capi.check(self.L.st_solver_create(C.byref(params), C.byref(h)))
+        self.h = h
+        ng = params.ng
+        self.shape_g = (params.nz + 2 * ng, params.ny + 2 * ng, params.nx + 2 * ng)
+        self.shape_i = (params.nz, params.ny, params.nx)
+        self._grid = [np.ascontiguousarray(a, dtype=np.float64) for a in grid.arrays()]
+        capi.check(self.L.st_set_grid(self.h, *[_dp(a) for a in self._grid]))
+
+    # -------------------------------------------------------------- lifecycle
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.st_solver_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------ data
+    def copy_from_field(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.shape == (5,) + self.shape_g, w.shape
+        capi.check(self.L.st_copy_from_field(self.h, _dp(w)))
+
+    def copy_to_field(self) -> np.ndarray:
+        w = np.empty((5,) + self.shape_g)
+        capi.check(self.L.st_copy_to_field(self.h, _dp(w)))
+        return w
+
+    def get_aux(self, var: int) -> np.ndarray:
+        a = np.empty(self.shape_g)
+        capi.check(self.L.st_get_aux(self.h, var, _dp(a)))
+        return a
+
+    def set_aux(self, var: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        capi.check(self.L.st_set_aux(self.h, var, _dp(a)))
+
+    def get_fl(self) -> np.ndarray:
+        a = np.empty((5,) + self.shape_i)
+        capi.check(self.L.st_get_fl(self.h, _dp(a)))
+        return a
+
+    def scalars(self) -> dict:
+        s = StScalars()
+        capi.check(self.L.st_get_scalars(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def set_time(self, time: float, dt: float, icyc: int):
+        capi.check(self.L.st_set_time(self.h, time, dt, icyc))
+
+    # ------------------------------------------------------------ procedures
+    def update_ghost(self, do_swap: int = 1):
+        capi.check(self.L.st_update_ghost(self.h, do_swap))
+
+    def bcswap(self):
+        capi.check(self.L.st_bcswap(self.h))
+
+    def compute_aux(self, central: int = 1, ghost: int = 1):
+        capi.check(self.L.st_compute_aux(self.h, central, ghost))
+
+    def visflx(self, mode: int):
+        capi.check(self.L.st_visflx(self.h, mode))
+
+    def euler_x(self):
+        capi.check(self.L.st_euler_x(self.h))
+
+    def euler_y(self):
+        capi.check(self.L.st_euler_y(self.h))
+
+    def euler_z(self):
+        capi.check(self.L.st_euler_z(self.h))
+
+    def zero_flux(self):
+        capi.check(self.L.st_zero_flux(self.h))
+
+    def init_run(self):
+        capi.check(self.L.st_init_run(self.h))
+
+    def compute_dt(self) -> float:
+        dt = C.c_double()
+        capi.check(self.L.st_compute_dt(self.h, C.byref(dt)))
+        return dt.value
+
+    def rk_sync(self):
+        capi.check(self.L.st_rk_sync(self.h))
+
+    def advance(self, nsteps: int):
+        capi.check(self.L.st_advance(self.h, nsteps))
+
+    def compute_residual(self) -> dict:
+        s = StScalars()
+        capi.check(self.L.st_compute_residual(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def count_weno(self) -> np.ndarray:
+        out = np.zeros(3)
+        capi.check(self.L.st_count_weno(self.h, _dp(out)))
+        return out
+
+    def kernel_times(self, nsteps: int) -> dict:
+        """Mean ms per launch of each kernel family over nsteps timed steps."""
+        ms = (C.c_double * len(capi.KERNEL_FAMILIES))()
+        cnt = (C.c_int32 * len(capi.KERNEL_FAMILIES))()
+        capi.check(self.L.st_kernel_times(self.h, nsteps, ms, cnt))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(capi.KERNEL_FAMILIES) if cnt[i]}
+
+    def synchronize(self):
+        capi.check(self.L.st_synchronize(self.h))

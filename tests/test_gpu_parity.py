@@ -1,0 +1,208 @@
+"""Parity of the HIP path with the CPU oracle on identical seeded inputs.
+
+Unit level: each procedure of the backend object on the same input state,
+compared with the oracle's restatement of the reference routine (relative L2
+per variable <= 1e-12: the two differ only by rounding, e.g. fused
+multiply-adds on the GPU).  Trajectory level: init + N Wray RK3 steps,
+relative L2 per conserved variable <= 1e-10 (the FP64 bar of the task).
+"""
+import numpy as np
+import pytest
+
+from helpers import fill_ghosts_wrap, interior, rel_l2
+from oracle import Oracle
+from streams_mi355x import make_channel
+from streams_mi355x.params import params_copy
+from streams_mi355x.solver import ChannelSolver
+
+pytestmark = pytest.mark.gpu
+
+UNIT_TOL = 1e-12
+TRAJ_TOL = 1e-10
+
+
+def _pair(nx=32, ny=24, nz=16, threshold=0.9, seed=0, **kw):
+    c = make_channel(nx=nx, ny=ny, nz=nz, sensor_threshold=threshold, seed=seed, **kw)
+    o = Oracle(c.params, c.grid)
+    g = ChannelSolver(c.params, c.grid)
+    return c, o, g
+
+
+def _load_state(c, o, g, sensor_seed=None):
+    """Same w (ghosts filled everywhere) and, optionally, a random sensor."""
+    w = fill_ghosts_wrap(c.w0)
+    o.w[:] = w
+    g.copy_from_field(w)
+    o.compute_aux(1, 1)
+    g.compute_aux(1, 1)
+    if sensor_seed is not None:
+        rng = np.random.default_rng(sensor_seed)
+        s = rng.uniform(0.0, 1.0, size=o.w_aux[7].shape)
+        o.w_aux[7][:] = s
+        g.set_aux(8, s)
+    return w
+
+
+def test_aux_parity():
+    c, o, g = _pair()
+    _load_state(c, o, g)
+    for v in range(1, 8):
+        assert rel_l2(g.get_aux(v), o.w_aux[v - 1]) <= UNIT_TOL, v
+
+
+@pytest.mark.parametrize("sensor_seed", [None, 1])
+@pytest.mark.parametrize("direction", ["x", "y", "z"])
+def test_euler_parity(direction, sensor_seed):
+    # sensor_seed None: sensor 0 everywhere -> all central faces;
+    # sensor_seed 1: uniform random sensor with threshold 0.9 -> mixed faces
+    c, o, g = _pair()
+    _load_state(c, o, g, sensor_seed)
+    o.zero_flux()
+    g.zero_flux()
+    getattr(o, "euler_" + direction)()
+    getattr(g, "euler_" + direction)()
+    fo, fg = o.fl, g.get_fl()
+    for m in range(5):
+        assert rel_l2(fg[m], fo[m]) <= UNIT_TOL, (direction, m, rel_l2(fg[m], fo[m]))
+
+
+def test_euler_full_weno_parity():
+    c, o, g = _pair(threshold=-1.0)
+    _load_state(c, o, g)
+    for d in ("x", "y", "z"):
+        o.zero_flux()
+        g.zero_flux()
+        getattr(o, "euler_" + d)()
+        getattr(g, "euler_" + d)()
+        fo, fg = o.fl, g.get_fl()
+        for m in range(5):
+            assert rel_l2(fg[m], fo[m]) <= UNIT_TOL, (d, m, rel_l2(fg[m], fo[m]))
+
+
+@pytest.mark.parametrize("mode", [4, 0])
+def test_visflx_parity(mode):
+    c, o, g = _pair()
+    _load_state(c, o, g)
+    o.zero_flux()
+    g.zero_flux()
+    o.visflx(mode)
+    g.visflx(mode)
+    fo, fg = o.fl, g.get_fl()
+    for m in range(1, 5):
+        assert rel_l2(fg[m], fo[m]) <= UNIT_TOL, (mode, m)
+    ng = 3
+    for v in ([8, 9, 10] if mode == 0 else [9, 10]):
+        a = interior(g.get_aux(v)[None])[0]
+        b = interior(o.w_aux[v - 1][None])[0]
+        assert rel_l2(a, b) <= UNIT_TOL, (mode, v)
+
+
+def test_visflx_div_and_sensor_parity():
+    c, o, g = _pair()
+    _load_state(c, o, g)
+    o.zero_flux()
+    o.visflx(4)
+    o.bcextr_var(10)
+    o.bcswap_var(10)
+    g.set_aux(9, o.w_aux[8])
+    g.set_aux(10, o.w_aux[9])
+    o.zero_flux()
+    g.zero_flux()
+    o.visflx(1)
+    g.visflx(1)
+    fo, fg = o.fl, g.get_fl()
+    for m in range(1, 5):
+        assert rel_l2(fg[m], fo[m]) <= UNIT_TOL, m
+    o.visflx(3)
+    g.visflx(3)
+    a = interior(g.get_aux(8)[None])[0]
+    b = interior(o.w_aux[7][None])[0]
+    assert np.max(np.abs(a - b)) <= 1e-13
+
+
+def test_ghost_fill_parity():
+    c, o, g = _pair()
+    w = c.w0.copy()
+    o.w[:] = w
+    g.copy_from_field(w)
+    o.compute_aux(1, 1)
+    g.compute_aux(1, 1)
+    o.update_ghost(1)
+    g.update_ghost(1)
+    wg = g.copy_to_field()
+    ng = 3
+    # every ghost point the reference writes: x/z periodic faces, y wall faces
+    for m in range(5):
+        a, b = wg[m], o.w[m]
+        assert rel_l2(a[ng:-ng, ng:-ng, :], b[ng:-ng, ng:-ng, :]) <= UNIT_TOL
+        assert rel_l2(a[ng:-ng, :, ng:-ng], b[ng:-ng, :, ng:-ng]) <= UNIT_TOL
+        assert rel_l2(a[:, ng:-ng, ng:-ng], b[:, ng:-ng, ng:-ng]) <= UNIT_TOL
+
+
+def test_dt_parity():
+    c, o, g = _pair()
+    o.w[:] = c.w0
+    g.copy_from_field(c.w0)
+    o.init_run()
+    g.init_run()
+    dto = o.scalars()["dt"]
+    dtg = g.compute_dt()
+    assert abs(dtg - dto) <= 1e-14 * dto
+
+
+def _run_pair(nsteps, **kw):
+    c, o, g = _pair(**kw)
+    o.w[:] = c.w0
+    g.copy_from_field(c.w0)
+    o.init_run()
+    g.init_run()
+    o.advance(nsteps)
+    g.advance(nsteps)
+    return c, o, g
+
+
+@pytest.mark.parametrize("threshold", [0.9, 0.0, -1.0])
+def test_trajectory_small(threshold):
+    c, o, g = _run_pair(20, threshold=threshold)
+    wg = g.copy_to_field()
+    for m in range(5):
+        err = rel_l2(interior(wg)[m], interior(o.w)[m])
+        assert err <= TRAJ_TOL, (threshold, m, err)
+    so, sg = o.scalars(), g.scalars()
+    for key in ("dt", "time", "rhobulk", "ubulk", "tbulk"):
+        assert abs(sg[key] - so[key]) <= 1e-12 * abs(so[key]), key
+    assert abs(sg["dpdx"] - so["dpdx"]) <= 1e-9 * abs(so["dpdx"]) + 1e-15
+    assert sg["icyc"] == so["icyc"] == 20
+
+
+def test_residual_and_weno_count():
+    c, o, g = _run_pair(5, threshold=0.0)
+    o.compute_residual()
+    sg = g.compute_residual()
+    so = o.scalars()
+    assert abs(sg["residual_rhou"] - so["residual_rhou"]) <= 1e-10 * so["residual_rhou"]
+    for key in ("rhomin", "rhomax", "tmin", "tmax", "pmin", "pmax"):
+        assert abs(sg[key] - so[key]) <= 1e-13 * abs(so[key]), key
+    assert np.allclose(g.count_weno(), o.count_weno(), rtol=0, atol=0)
+
+
+def test_trajectory_c1_100_steps():
+    """Config C1 (128x96x64 channel, M_b 1.5, Re_tau 220) over 100 steps."""
+    c, o, g = _run_pair(100, nx=128, ny=96, nz=64)
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    print("C1 100-step rel L2 per variable:", errs)
+    assert max(errs) <= TRAJ_TOL, errs
+
+
+def test_bitwise_reproducible():
+    c = make_channel()
+    outs = []
+    for _ in range(2):
+        g = ChannelSolver(c.params, c.grid)
+        g.copy_from_field(c.w0)
+        g.init_run()
+        g.advance(10)
+        outs.append(g.copy_to_field())
+        g.close()
+    assert np.array_equal(interior(outs[0]), interior(outs[1]))
