@@ -54,7 +54,24 @@ def parse():
     ap.add_argument("--retau", type=float, default=550.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain device copy of 2 GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_probe():
+    """A plain 2 GiB device-to-device copy, launched a few times: the HBM
+    counters of this dispatch against its known 2 GiB read + 2 GiB write give
+    the scale factor applied to the solver kernels' FETCH_SIZE/WRITE_SIZE."""
+    import torch
+
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n}))
 
 
 def cpu_baseline(args):
@@ -88,6 +105,9 @@ def cpu_baseline(args):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the hot path runs on a single MI355X; multi-GPU sharding is design-only)", file=sys.stderr)

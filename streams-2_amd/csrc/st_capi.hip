@@ -156,7 +156,7 @@ hipError_t dalloc(T **p, size_t n) {
 void free_all(st_solver *s) {
     void *ptrs[] = {s->w, s->w_aux, s->F[0], s->F[1], s->wallprop, s->eoc[0], s->eoc[1], s->eoc[2], s->x, s->y,
                     s->z, s->yn, s->dcsidx, s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz,
-                    s->dzitdzs, s->dzitdz2, s->ds, s->partials};
+                    s->dzitdzs, s->dzitdz2, s->ds, s->partials, s->dirty, s->counters};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->hs) (void)hipHostFree(s->hs);
@@ -233,7 +233,7 @@ int64_t st_solver_footprint(const st_params *p) {
     const int64_t sx = p->nx + 2 * p->ng, sy = p->ny + 2 * p->ng, sz = p->nz + 2 * p->ng;
     const int64_t nvol = sx * sy * sz, nint = (int64_t)p->nx * p->ny * p->nz;
     const int64_t blocks = (int64_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * p->nz;
-    return 8 * (15 * nvol + 10 * nint + 3 * sx * sz + 7 * blocks) + 4 * (p->nx + p->ny + p->nz + 3) +
+    return 8 * (15 * nvol + 10 * nint + 3 * sx * sz + 7 * blocks + 1024) + 4 * (nint + p->nx + p->ny + p->nz + 3 + 4) +
            8 * (2 * sx + 2 * sy + 2 * sz + 3 * (p->nx + p->ny + p->nz) + p->ny + 1) + 4096;
 }
 
@@ -265,7 +265,7 @@ int st_solver_create(const st_params *p, st_solver **out) {
     s->own_stream = true;
     const size_t nvol = g.nvol, nint = g.nint;
     const int nblk = ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz;
-    s->npart = 7 * nblk;
+    s->npart = 7 * nblk + 128 * 8;   // per-block partials + second-level sums
     hipError_t es[32];
     int ne = 0;
     es[ne++] = dalloc(&s->w, 5 * nvol);
@@ -291,6 +291,8 @@ int st_solver_create(const st_params *p, st_solver **out) {
     es[ne++] = dalloc(&s->dzitdz2, (size_t)g.nz);
     es[ne++] = dalloc(&s->ds, 1);
     es[ne++] = dalloc(&s->partials, (size_t)s->npart);
+    es[ne++] = dalloc(&s->dirty, nint);
+    es[ne++] = dalloc(&s->counters, 4);
     es[ne++] = hipHostMalloc((void **)&s->hs, sizeof(st::DevScalars), hipHostMallocDefault);
     for (int q = 0; q < ne; q++) {
         if (es[q] != hipSuccess) {
@@ -315,6 +317,7 @@ int st_solver_create(const st_params *p, st_solver **out) {
     if (ec == hipSuccess) ec = hipMemset(s->F[0], 0, sizeof(double) * 5 * nint);
     if (ec == hipSuccess) ec = hipMemset(s->F[1], 0, sizeof(double) * 5 * nint);
     if (ec == hipSuccess) ec = hipMemset(s->wallprop, 0, sizeof(double) * 3 * g.sx * g.sz);
+    if (ec == hipSuccess) ec = hipMemset(s->counters, 0, sizeof(int) * 4);
     std::memset(s->hs, 0, sizeof(st::DevScalars));
     if (p->cfl < 0) s->hs->dt = -p->cfl;
     if (ec == hipSuccess) ec = hipMemcpy(s->ds, s->hs, sizeof(st::DevScalars), hipMemcpyHostToDevice);

@@ -23,11 +23,15 @@ struct Pt {
     double p, e;              // pressure and internal energy
 };
 
-__device__ __forceinline__ Pt make_pt(double r, double u, double v, double w, double h, double t, double rgas0) {
+// e carries the internal energy for the KEEP split (nkeep >= 0) and the
+// total enthalpy H for the non-KEEP split (nkeep < 0), the only energy each
+// split uses.
+__device__ __forceinline__ Pt make_pt(double r, double u, double v, double w, double h, double t, double rgas0,
+                                      bool keep) {
     Pt q;
     q.r = r; q.u = u; q.v = v; q.w = w; q.h = h; q.t = t;
     q.p = t * r * rgas0;
-    q.e = h - q.p / r - 0.5 * (u * u + v * v + w * w);
+    q.e = keep ? h - q.p / r - 0.5 * (u * u + v * v + w * w) : h;
     return q;
 }
 
@@ -36,15 +40,44 @@ __device__ __forceinline__ double un_of(const Pt &q) {
     return DIR == 0 ? q.u : (DIR == 1 ? q.v : q.w);
 }
 
+// The six values the central split needs at a point: rho, u, v, w, p, e
+// (H and T enter only through p and e).
+struct Pc {
+    double r, u, v, w, p, e;
+};
+
+__device__ __forceinline__ Pc make_pc(double r, double u, double v, double w, double h, double t, double rgas0,
+                                      bool keep) {
+    Pc q;
+    q.r = r; q.u = u; q.v = v; q.w = w;
+    q.p = t * r * rgas0;
+    q.e = keep ? h - q.p / r - 0.5 * (u * u + v * v + w * w) : h;
+    return q;
+}
+
+__device__ __forceinline__ Pc pc_of(const Pt &a) {
+    Pc q;
+    q.r = a.r; q.u = a.u; q.v = a.v; q.w = a.w; q.p = a.p; q.e = a.e;
+    return q;
+}
+
+template <int DIR>
+__device__ __forceinline__ double un_of(const Pc &q) {
+    return DIR == 0 ? q.u : (DIR == 1 ? q.v : q.w);
+}
+
 __device__ __forceinline__ double sqr(double a) { return a * a; }
 
 // ------------------------------------------------------------ central flux
-template <int DIR, int H>
-__device__ __forceinline__ void central_flux(const KPar &P, const Pt (&q)[2 * H], int lmax, bool zero_flux,
+// KM selects the split at compile time: 0 KEEP (nkeep = 0, the default),
+// 1 KEEP(n) with nkeep > 0, -1 the non-KEEP split (nkeep < 0); the runtime
+// nkeep of the parameters must agree with it.
+template <int DIR, int H, class PT, int KM>
+__device__ __forceinline__ void central_flux(const KPar &P, const PT (&q)[2 * H], int lmax, bool zero_flux,
                                              double fh[5]) {
     double ft1 = 0.0, ft2 = 0.0, ft3 = 0.0, ft4 = 0.0, ft5 = 0.0, ft6 = 0.0;
     const double *cd = P.cd1tab[lmax - 1];
-    const int nkeep = P.nkeep;
+    const int nkeep = KM == 0 ? 0 : (KM < 0 ? -1 : P.nkeep);
 #pragma unroll
     for (int l = 1; l <= H; l++) {
         if (l <= lmax) {
@@ -52,15 +85,15 @@ __device__ __forceinline__ void central_flux(const KPar &P, const Pt (&q)[2 * H]
             double uvs5_i = 0.0, uvs5_k = 0.0, uvs5_p = 0.0, uvs5 = 0.0;
 #pragma unroll
             for (int m = 0; m < l; m++) {
-                const Pt &a = q[H - 1 - m];
-                const Pt &b = q[H - 1 - m + l];
+                const PT &a = q[H - 1 - m];
+                const PT &b = q[H - 1 - m + l];
                 const double una = un_of<DIR>(a), unb = un_of<DIR>(b);
                 const double rhom = a.r + b.r;
                 if (nkeep >= 0) {
                     const double eem = a.e + b.e;
                     double uv_part = (una + unb) * rhom;
                     double deef = 1.0;
-                    if (nkeep > 0) {
+                    if (KM > 0) {
                         // KEEP(n) correction factors (kernels_cpu.F90:505-531)
                         double drho = 2.0 * (b.r - a.r) / rhom;
                         double dee = 2.0 * (b.e - a.e) / eem;
@@ -95,7 +128,7 @@ __device__ __forceinline__ void central_flux(const KPar &P, const Pt (&q)[2 * H]
                     uvs2 = uvs2 + uv_part * (a.u + b.u);
                     uvs3 = uvs3 + uv_part * (a.v + b.v);
                     uvs4 = uvs4 + uv_part * (a.w + b.w);
-                    uvs5 = uvs5 + uv_part * (a.h + b.h);
+                    uvs5 = uvs5 + uv_part * (a.e + b.e);
                     uvs6 = uvs6 + 2.0 * (a.p + b.p);
                 }
             }

@@ -216,17 +216,39 @@ __global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__rest
     wa[6 * nv + d] = viscosity_of(P, tt);
 }
 
+// ------------------------------------------------------- dirty-cell list
+// Faces whose sensor window exceeds the threshold take the characteristic
+// WENO flux.  The main sweep kernels evaluate only central faces, so their
+// register budget stays that of the KEEP split; a cell with a WENO face on
+// either side is appended to a list (wave-aggregated atomic) and handled by
+// k_weno_fix, which recomputes both faces of the cell with the full hybrid
+// logic and applies the same divergence as the reference.
+__device__ __forceinline__ void list_append(int *__restrict__ list, int *__restrict__ count, bool pred, int value) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+__global__ void k_reset_count(int *__restrict__ count) { *count = 0; }
+
 // --------------------------------------------------------------- Euler x
 // One 64-lane workgroup per 63-cell segment of an x row.  Lane t computes
-// face i0-1+t into LDS; lanes 1..63 then form the divergence of their cell
-// (euler_x_fluxes_hybrid_kernel + euler_x_update_subroutine,
-// kernels_cpu.F90:425-689, 944-963).  init = true writes the divergence
-// (the stage's first contribution after init_flux's fl = 0).
-template <bool INIT>
+// central face i0-1+t into LDS; lanes 1..63 then form the divergence of their
+// cell (euler_x_fluxes_hybrid_kernel + euler_x_update_subroutine,
+// kernels_cpu.F90:425-689, 944-963).  init = true writes the divergence (the
+// stage's first contribution after init_flux's fl = 0).
+template <bool INIT, int KM>
 __global__ __launch_bounds__(64) void k_euler_x(Geo g, KPar P, const double *__restrict__ wa,
                                                 const int *__restrict__ eoc, const double *__restrict__ dcsidx,
-                                                double *__restrict__ fl) {
+                                                double *__restrict__ fl, int *__restrict__ list,
+                                                int *__restrict__ count) {
     __shared__ double sfh[5][64];
+    __shared__ int sweno[64];
     const int nseg = (g.nx + XSEG - 1) / XSEG;
     const int seg = blockIdx.x % nseg;
     const int row = blockIdx.x / nseg;
@@ -236,57 +258,68 @@ __global__ __launch_bounds__(64) void k_euler_x(Geo g, KPar P, const double *__r
     const int ilast = min(i0 + XSEG - 1, g.nx);
     const int f = i0 - 1 + lane;
     const int lo = P.eul_lo[0], hi = P.eul_hi[0];
+    const bool keep = P.nkeep >= 0;
+    int weno = 0;
     if (f <= ilast && f >= lo - 1 && f <= hi) {
         const size_t nv = g.nvol;
         const int base = gidx(g, f, j, k);
-        Pt q[2 * H];
         bool shk = false;
 #pragma unroll
         for (int s = 0; s < 2 * H; s++) {
-            const int o = base + s - (H - 1);
-            q[s] = make_pt(wa[o], wa[nv + o], wa[2 * nv + o], wa[3 * nv + o], wa[4 * nv + o], wa[5 * nv + o], P.rgas0);
-            if (s >= H - P.weno_scheme && s < H + P.weno_scheme) shk |= wa[7 * nv + o] > P.sensor_threshold;
+            if (s >= H - P.weno_scheme && s < H + P.weno_scheme)
+                shk |= wa[7 * nv + base + s - (H - 1)] > P.sensor_threshold;
         }
-        const int e = eoc[f];
-        double fh[5];
         if (!shk) {
-            int lmax = P.lmax_base + e;
+            Pc q[2 * H];
+#pragma unroll
+            for (int s = 0; s < 2 * H; s++) {
+                const int o = base + s - (H - 1);
+                q[s] = make_pc(wa[o], wa[nv + o], wa[2 * nv + o], wa[3 * nv + o], wa[4 * nv + o], wa[5 * nv + o],
+                               P.rgas0, keep);
+            }
+            int lmax = P.lmax_base + eoc[f];
             lmax = lmax < 1 ? 1 : lmax;
             const bool zf = (f == 0 && P.fzf[0] == 1) || (f == g.nx && P.fzf[1] == 1);
-            central_flux<0, H>(P, q, lmax, zf, fh);
-        } else {
-            weno_flux<0>(P, q, e, fh);
-        }
+            double fh[5];
+            central_flux<0, H, Pc, KM>(P, q, lmax, zf, fh);
 #pragma unroll
-        for (int m = 0; m < 5; m++) sfh[m][lane] = fh[m];
+            for (int m = 0; m < 5; m++) sfh[m][lane] = fh[m];
+        } else {
+            weno = 1;
+        }
     }
+    sweno[lane] = weno;
     __syncthreads();
-    if (lane >= 1 && f <= ilast) {
-        const int i = f;
+    const int i = f;
+    const bool cell = lane >= 1 && i <= ilast;
+    const bool in = cell && i >= lo && i <= hi;
+    const bool dirty = in && (sweno[lane] | sweno[lane - (lane >= 1 ? 1 : 0)]);
+    if (cell) {
         const int o = iidx(g, i, j, k);
-        const bool in = i >= lo && i <= hi;
         const double d = dcsidx[i - 1];
 #pragma unroll
         for (int m = 0; m < 5; m++) {
             if (INIT) {
-                fl[(size_t)m * g.nint + o] = in ? (sfh[m][lane] - sfh[m][lane - 1]) * d : 0.0;
-            } else if (in) {
+                fl[(size_t)m * g.nint + o] = (in && !dirty) ? (sfh[m][lane] - sfh[m][lane - 1]) * d : 0.0;
+            } else if (in && !dirty) {
                 fl[(size_t)m * g.nint + o] += (sfh[m][lane] - sfh[m][lane - 1]) * d;
             }
         }
     }
+    list_append(list, count, dirty, cell ? iidx(g, i, j, k) : 0);
 }
 
 // ------------------------------------------------------------ Euler y / z
 // One thread per (i, row) marching over a chunk of cells along DIR with a
-// sliding 6-point window in registers; each w_aux value is loaded once per
-// chunk and each face flux computed once (euler_y/z_hybrid_kernel +
-// euler_y/z_update_subroutine, kernels_cpu.F90:1808-2071, 991-1010 and
-// 1063-1325, 1039-1058).
-template <int DIR>
+// sliding 6-point window in registers and a one-point prefetch; each w_aux
+// value is loaded once per chunk and each central face computed once
+// (euler_y/z_hybrid_kernel + euler_y/z_update_subroutine,
+// kernels_cpu.F90:1808-2071, 991-1010 and 1063-1325, 1039-1058).
+template <int DIR, int KM>
 __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double *__restrict__ wa,
                                                      const int *__restrict__ eoc, const double *__restrict__ metric,
-                                                     double *__restrict__ fl, int chunk, int nchunk) {
+                                                     double *__restrict__ fl, int chunk, int nchunk,
+                                                     int *__restrict__ list, int *__restrict__ count) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int no = DIR == 1 ? g.nz : g.ny;  // the other transverse extent
     if (t >= g.nx * no * nchunk) return;
@@ -301,51 +334,138 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double
     const int n = DIR == 1 ? g.ny : g.nz;
     const int stride = DIR == 1 ? g.sx : g.sxy;
     const size_t nv = g.nvol;
-    // point index along DIR -> ghosted offset
+    const bool keep = P.nkeep >= 0;
+    const double thr = P.sensor_threshold;
+    const int ws = P.weno_scheme;
+    // sensor window bits that decide a face: s in [H-ws, H+ws)
+    const unsigned smask = ((1u << (2 * ws)) - 1u) << (H - ws);
     const int p0 = DIR == 1 ? gidx(g, i, 0, o) : gidx(g, i, o, 0);  // offset of along-index 0
-    Pt q[6];
-    double sen[6];
+    Pc q[6];
+    unsigned sen = 0;
 #pragma unroll
     for (int s = 0; s < 6; s++) {
         const int a = p0 + (c0 - 3 + s) * stride;
-        q[s] = make_pt(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0);
-        sen[s] = wa[7 * nv + a];
+        q[s] = make_pc(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0,
+                       keep);
+        sen |= (wa[7 * nv + a] > thr ? 1u : 0u) << s;
     }
     double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    const int ws = P.weno_scheme;
+    bool wprev = false;
     for (int f = c0 - 1; f <= c1; f++) {
-        bool shk = false;
-#pragma unroll
-        for (int s = 0; s < 6; s++)
-            if (s >= H - ws && s < H + ws) shk |= sen[s] > P.sensor_threshold;
-        const int e = eoc[f];
+        // prefetch the point entering the window next
+        double nr = 0, nu = 0, nvv = 0, nw = 0, nh = 0, nt = 0, ns = 0;
+        const bool more = f < c1;
+        if (more) {
+            const int a = p0 + (f + 4) * stride;
+            nr = wa[a]; nu = wa[nv + a]; nvv = wa[2 * nv + a]; nw = wa[3 * nv + a];
+            nh = wa[4 * nv + a]; nt = wa[5 * nv + a]; ns = wa[7 * nv + a];
+        }
+        const bool wcur = (sen & smask) != 0u;
         double fh[5];
-        if (!shk) {
-            int lmax = P.lmax_base + e;
+        if (!wcur) {
+            int lmax = P.lmax_base + eoc[f];
             lmax = lmax < 1 ? 1 : lmax;
             const bool zf = (f == 0 && P.fzf[2 * DIR] == 1) || (f == n && P.fzf[2 * DIR + 1] == 1);
-            central_flux<DIR, H>(P, q, lmax, zf, fh);
+            central_flux<DIR, H, Pc, KM>(P, q, lmax, zf, fh);
         } else {
-            weno_flux<DIR>(P, q, e, fh);
-        }
-        if (f >= c0) {
-            const int oi = DIR == 1 ? iidx(g, i, f, o) : iidx(g, i, o, f);
-            const double d = metric[f - 1];
 #pragma unroll
-            for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] += (fh[m] - fprev[m]) * d;
+            for (int m = 0; m < 5; m++) fh[m] = 0.0;
         }
+        bool dirty = false;
+        if (f >= c0) {
+            dirty = wcur || wprev;
+            if (!dirty) {
+                const int oi = DIR == 1 ? iidx(g, i, f, o) : iidx(g, i, o, f);
+                const double d = metric[f - 1];
+#pragma unroll
+                for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] += (fh[m] - fprev[m]) * d;
+            }
+        }
+        list_append(list, count, dirty, DIR == 1 ? iidx(g, i, f < 1 ? 1 : f, o) : iidx(g, i, o, f < 1 ? 1 : f));
 #pragma unroll
         for (int m = 0; m < 5; m++) fprev[m] = fh[m];
-        if (f < c1) {
+        wprev = wcur;
+        if (more) {
 #pragma unroll
-            for (int s = 0; s < 5; s++) {
-                q[s] = q[s + 1];
-                sen[s] = sen[s + 1];
-            }
-            const int a = p0 + (f + 4) * stride;
-            q[5] = make_pt(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0);
-            sen[5] = wa[7 * nv + a];
+            for (int s = 0; s < 5; s++) q[s] = q[s + 1];
+            q[5] = make_pc(nr, nu, nvv, nw, nh, nt, P.rgas0, keep);
+            sen = (sen >> 1) | ((ns > thr ? 1u : 0u) << 5);
         }
+    }
+}
+
+// ------------------------------------------------------------ WENO fix-up
+// Hybrid flux of the face between window points 2 and 3 of q (offsets
+// -2..+3): central when no sensor value of the window exceeds the threshold,
+// characteristic WENO otherwise (the branch of the reference kernels).
+template <int DIR>
+__device__ __forceinline__ void hybrid_face(const KPar &P, const Pt (&q)[6], const double (&sen)[6], int f, int e,
+                                            int n, double fh[5]) {
+    bool shk = false;
+#pragma unroll
+    for (int s = 0; s < 6; s++)
+        if (s >= H - P.weno_scheme && s < H + P.weno_scheme) shk |= sen[s] > P.sensor_threshold;
+    if (!shk) {
+        Pc c[6];
+#pragma unroll
+        for (int s = 0; s < 6; s++) c[s] = pc_of(q[s]);
+        int lmax = P.lmax_base + e;
+        lmax = lmax < 1 ? 1 : lmax;
+        const bool zf = (f == 0 && P.fzf[2 * DIR] == 1) || (f == n && P.fzf[2 * DIR + 1] == 1);
+        if (P.nkeep == 0) central_flux<DIR, H, Pc, 0>(P, c, lmax, zf, fh);
+        else if (P.nkeep > 0) central_flux<DIR, H, Pc, 1>(P, c, lmax, zf, fh);
+        else central_flux<DIR, H, Pc, -1>(P, c, lmax, zf, fh);
+    } else {
+        weno_flux<DIR>(P, q, e, fh);
+    }
+}
+
+// Cells of the list: both faces recomputed with the hybrid logic, then
+// fl += (fhat(c) - fhat(c-1)) * metric(c).
+template <int DIR>
+__global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const double *__restrict__ wa,
+                                                 const int *__restrict__ eoc, const double *__restrict__ metric,
+                                                 double *__restrict__ fl, const int *__restrict__ list,
+                                                 const int *__restrict__ count) {
+    const int ncell = *count;
+    const size_t nv = g.nvol;
+    const bool keep = P.nkeep >= 0;
+    const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
+    const int stride = DIR == 0 ? 1 : (DIR == 1 ? g.sx : g.sxy);
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ncell; e += gridDim.x * blockDim.x) {
+        const int oi = list[e];
+        const int i = oi % g.nx + 1;
+        const int j = (oi / g.nx) % g.ny + 1;
+        const int k = oi / (g.nx * g.ny) + 1;
+        const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
+        const int base = gidx(g, i, j, k);
+        Pt q[7];
+        double sen[7];
+#pragma unroll
+        for (int s = 0; s < 7; s++) {
+            const int a = base + (s - 3) * stride;
+            q[s] = make_pt(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a],
+                           P.rgas0, keep);
+            sen[s] = wa[7 * nv + a];
+        }
+        double flo[5], fhi[5];
+        {
+            Pt ql[6];
+            double sl[6];
+#pragma unroll
+            for (int s = 0; s < 6; s++) { ql[s] = q[s]; sl[s] = sen[s]; }
+            hybrid_face<DIR>(P, ql, sl, c - 1, eoc[c - 1], n, flo);
+        }
+        {
+            Pt qh[6];
+            double sh[6];
+#pragma unroll
+            for (int s = 0; s < 6; s++) { qh[s] = q[s + 1]; sh[s] = sen[s + 1]; }
+            hybrid_face<DIR>(P, qh, sh, c, eoc[c], n, fhi);
+        }
+        const double d = metric[c - 1];
+#pragma unroll
+        for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] += (fhi[m] - flo[m]) * d;
     }
 }
 
@@ -595,17 +715,53 @@ __global__ __launch_bounds__(256) void k_visc_div(Geo g, KPar P, const double *_
     }
 }
 
-// force_rhs (singleideal_cpu.F90:845-889): sums of the partials, bulk values
+// Two-level sum of per-block partials: RB workgroups each add a fixed range
+// of partials, the last workgroup to finish (atomic ticket) adds the RB
+// results in index order, so the sum does not depend on scheduling.
+constexpr int RB = 128;
+
+template <int NV>
+__device__ __forceinline__ bool two_level_sum(const double *__restrict__ partials, int nblk, double *__restrict__ stage2,
+                                              int *__restrict__ ticket, double (&out)[NV], double (*lds)[NV]) {
+    __shared__ bool last;
+    const int per = (nblk + gridDim.x - 1) / gridDim.x;
+    const int b0 = blockIdx.x * per, b1 = min(b0 + per, nblk);
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) acc[q] = 0.0;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) acc[q] += partials[(size_t)b * NV + q];
+    }
+    block_sum<NV>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) stage2[blockIdx.x * NV + q] = acc[q];
+        __threadfence();
+        last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return false;
+    if (threadIdx.x == 0) {
+        __threadfence();
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double a = 0.0;
+            for (int b = 0; b < (int)gridDim.x; b++) a += ((volatile double *)stage2)[b * NV + q];
+            out[q] = a;
+        }
+        *ticket = 0;
+    }
+    return threadIdx.x == 0;
+}
+
+// force_rhs (singleideal_cpu.F90:845-889): bulk sums -> dpdx, rho_b, u_b, T_b
 __global__ __launch_bounds__(RED_THREADS) void k_force_finish(KPar P, const double *__restrict__ partials, int nblk,
+                                                              double *__restrict__ stage2, int *__restrict__ ticket,
                                                               DevScalars *ds, int first_stage) {
     __shared__ double lds[RED_THREADS / 64][5];
-    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-#pragma unroll
-        for (int q = 0; q < 5; q++) acc[q] += partials[(size_t)b * 5 + q];
-    }
-    block_sum<5>(acc, lds);
-    if (threadIdx.x == 0) {
+    double acc[5];
+    if (two_level_sum<5>(partials, nblk, stage2, ticket, acc, lds)) {
         const double v = P.volchan;
         const double b1 = acc[0] / v, b2 = acc[1] / v, b3 = acc[2] / v, b4 = acc[3] / v, b5 = acc[4] / v;
         ds->dpdx = (first_stage ? 0.0 : ds->dpdx) + b2;
@@ -676,12 +832,11 @@ __global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restric
 
 // force_var (singleideal_cpu.F90:890-933, grid_dim 1) bulk temperature
 __global__ __launch_bounds__(RED_THREADS) void k_force_var_finish(KPar P, const double *__restrict__ partials, int nblk,
+                                                                  double *__restrict__ stage2, int *__restrict__ ticket,
                                                                   DevScalars *ds) {
     __shared__ double lds[RED_THREADS / 64][1];
-    double acc[1] = {0.0};
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc[0] += partials[b];
-    block_sum<1>(acc, lds);
-    if (threadIdx.x == 0) {
+    double acc[1];
+    if (two_level_sum<1>(partials, nblk, stage2, ticket, acc, lds)) {
         double bulktg = acc[0] / P.volchan;
         bulktg = bulktg / ds->rhobulk / ds->ubulk;
         ds->bulkt = P.t_bulk_target - bulktg;
@@ -712,7 +867,7 @@ __global__ __launch_bounds__(256) void k_dt(Geo g, KPar P, const double *__restr
                                             const double *__restrict__ dx1, const double *__restrict__ dy1,
                                             const double *__restrict__ dz1, const double *__restrict__ dxs,
                                             const double *__restrict__ dys, const double *__restrict__ dzs,
-                                            DevScalars *ds) {
+                                            double *__restrict__ partials) {
     __shared__ double lds[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -753,16 +908,25 @@ __global__ __launch_bounds__(256) void k_dt(Geo g, KPar P, const double *__restr
         const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
         double m = lds[0];
         for (int q = 1; q < nw; q++) m = fmax(m, lds[q]);
-        atomicMax(&ds->dtinv_bits, (unsigned long long)__double_as_longlong(m));
+        partials[flat_block()] = m;
     }
 }
 
-// dt = cfl / max (singleideal_cpu.F90:1507-1514); resets the running max
-__global__ void k_dt_finish(KPar P, DevScalars *ds) {
-    const double dtinv_max = __longlong_as_double((long long)ds->dtinv_bits);
-    const double dt_min = 1.0 / dtinv_max;
-    ds->dt = P.cfl * dt_min;
-    ds->dtinv_bits = 0ull;
+// dt = cfl / max (singleideal_cpu.F90:1507-1514) over the block maxima
+__global__ __launch_bounds__(RED_THREADS) void k_dt_finish(KPar P, const double *__restrict__ partials, int nblk,
+                                                           DevScalars *ds) {
+    __shared__ double lds[RED_THREADS / 64];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) m = fmax(m, partials[b]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double dtinv_max = lds[0];
+        for (int q = 1; q < RED_THREADS / 64; q++) dtinv_max = fmax(dtinv_max, lds[q]);
+        const double dt_min = 1.0 / dtinv_max;
+        ds->dt = P.cfl * dt_min;
+    }
 }
 
 // end of a step of run(): icyc += 1, time += dt (singleideal_cpu.F90:1604,1642)
@@ -920,6 +1084,21 @@ hipError_t launch_wall_ghosts(st_solver *s) {
     return hipGetLastError();
 }
 
+static hipError_t launch_fix(st_solver *s, double *fl, int dir) {
+    const Geo &g = s->g;
+    const dim3 grid(1024), block(64);
+    if (dir == 0)
+        hipLaunchKernelGGL(k_weno_fix<0>, grid, block, 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl,
+                           s->dirty, s->counters);
+    else if (dir == 1)
+        hipLaunchKernelGGL(k_weno_fix<1>, grid, block, 0, s->stream, g, s->kp, s->w_aux, s->eoc[1], s->detady, fl,
+                           s->dirty, s->counters);
+    else
+        hipLaunchKernelGGL(k_weno_fix<2>, grid, block, 0, s->stream, g, s->kp, s->w_aux, s->eoc[2], s->dzitdz, fl,
+                           s->dirty, s->counters);
+    return hipGetLastError();
+}
+
 hipError_t launch_euler_x(st_solver *s, double *fl, bool init) {
     const Geo &g = s->g;
     if (s->kp.eul_hi[0] < s->kp.eul_lo[0]) {
@@ -928,11 +1107,20 @@ hipError_t launch_euler_x(st_solver *s, double *fl, bool init) {
     }
     const int nseg = (g.nx + XSEG - 1) / XSEG;
     dim3 grid((unsigned)(nseg * g.ny * g.nz));
-    if (init)
-        hipLaunchKernelGGL(k_euler_x<true>, grid, dim3(64), 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl);
-    else
-        hipLaunchKernelGGL(k_euler_x<false>, grid, dim3(64), 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl);
-    return hipGetLastError();
+    hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, s->stream, s->counters);
+    const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
+#define ST_LAUNCH_X(I, K)                                                                                          \
+    hipLaunchKernelGGL((k_euler_x<I, K>), grid, dim3(64), 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl, \
+                       s->dirty, s->counters)
+    if (init) {
+        if (km == 0) ST_LAUNCH_X(true, 0); else if (km > 0) ST_LAUNCH_X(true, 1); else ST_LAUNCH_X(true, -1);
+    } else {
+        if (km == 0) ST_LAUNCH_X(false, 0); else if (km > 0) ST_LAUNCH_X(false, 1); else ST_LAUNCH_X(false, -1);
+    }
+#undef ST_LAUNCH_X
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_fix(s, fl, 0);
 }
 
 static hipError_t launch_march(st_solver *s, double *fl, int dir) {
@@ -950,13 +1138,24 @@ static hipError_t launch_march(st_solver *s, double *fl, int dir) {
     nchunk = (ncell + chunk - 1) / chunk;
     const size_t tot = (size_t)g.nx * no * nchunk;
     dim3 grid((unsigned)((tot + 255) / 256));
-    if (dir == 1)
-        hipLaunchKernelGGL(k_euler_march<1>, grid, dim3(256), 0, s->stream, g, s->kp, s->w_aux, s->eoc[1], s->detady, fl,
-                           chunk, nchunk);
-    else
-        hipLaunchKernelGGL(k_euler_march<2>, grid, dim3(256), 0, s->stream, g, s->kp, s->w_aux, s->eoc[2], s->dzitdz, fl,
-                           chunk, nchunk);
-    return hipGetLastError();
+    hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, s->stream, s->counters);
+    const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
+#define ST_LAUNCH_M(D, K, E, M)                                                                                     \
+    hipLaunchKernelGGL((k_euler_march<D, K>), grid, dim3(256), 0, s->stream, g, s->kp, s->w_aux, E, M, fl, chunk, \
+                       nchunk, s->dirty, s->counters)
+    if (dir == 1) {
+        if (km == 0) ST_LAUNCH_M(1, 0, s->eoc[1], s->detady);
+        else if (km > 0) ST_LAUNCH_M(1, 1, s->eoc[1], s->detady);
+        else ST_LAUNCH_M(1, -1, s->eoc[1], s->detady);
+    } else {
+        if (km == 0) ST_LAUNCH_M(2, 0, s->eoc[2], s->dzitdz);
+        else if (km > 0) ST_LAUNCH_M(2, 1, s->eoc[2], s->dzitdz);
+        else ST_LAUNCH_M(2, -1, s->eoc[2], s->dzitdz);
+    }
+#undef ST_LAUNCH_M
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_fix(s, fl, dir);
 }
 
 hipError_t launch_euler_y(st_solver *s, double *fl) { return launch_march(s, fl, 1); }
@@ -993,8 +1192,8 @@ hipError_t launch_visc_div(st_solver *s, double *fl, const double *fl_prev, int 
 }
 
 hipError_t launch_force_finish(st_solver *s, int stage) {
-    hipLaunchKernelGGL(k_force_finish, dim3(1), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(s->g),
-                       s->ds, stage == 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_force_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials,
+                       cell_blocks(s->g), s->partials + s->npart - RB * 8, s->counters + 1, s->ds, stage == 0 ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -1012,8 +1211,8 @@ hipError_t launch_update(st_solver *s, const double *fl_prev, const double *fl, 
 
 hipError_t launch_force_var(st_solver *s) {
     const Geo &g = s->g;
-    hipLaunchKernelGGL(k_force_var_finish, dim3(1), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(g),
-                       s->ds);
+    hipLaunchKernelGGL(k_force_var_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials,
+                       cell_blocks(g), s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
     hipLaunchKernelGGL(k_force_var2, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, s->w_aux, s->ds);
     return launch_wall_ghosts(s);
 }
@@ -1021,8 +1220,8 @@ hipError_t launch_force_var(st_solver *s) {
 hipError_t launch_dt(st_solver *s) {
     const Geo &g = s->g;
     hipLaunchKernelGGL(k_dt, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, s->w_aux, s->dcsidx, s->detady,
-                       s->dzitdz, s->dcsidxs, s->detadys, s->dzitdzs, s->ds);
-    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(1), 0, s->stream, s->kp, s->ds);
+                       s->dzitdz, s->dcsidxs, s->detadys, s->dzitdzs, s->partials);
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(g), s->ds);
     return hipGetLastError();
 }
 
