@@ -40,6 +40,7 @@ def lib():
         L.or_eoc.argtypes = [vp]
         L.or_get_scalars.argtypes = [vp, C.POINTER(StScalars)]
         L.or_set_time.argtypes = [vp, C.c_double, C.c_double, C.c_int64]
+        L.or_rk_coeffs.argtypes = [vp, dp, dp]
         L.or_get_deriv_coeffs.argtypes = [dp, dp, C.c_int, C.c_int]
         L.or_get_gamloc.restype = C.c_double
         L.or_get_gamloc.argtypes = [vp, C.c_double]
@@ -126,6 +127,11 @@ class Oracle:
             raise AttributeError(name)
         fn = getattr(lib(), "or_" + name)
         return lambda *a: fn(self.h, *a)
+
+    def rk_coeffs(self):
+        rho, gam = np.zeros(3), np.zeros(3)
+        self.L.or_rk_coeffs(self.h, _dp(rho), _dp(gam))
+        return rho, gam
 
     def count_weno(self):
         out = np.zeros(3)

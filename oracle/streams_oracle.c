@@ -240,6 +240,11 @@ void or_get_scalars(const or_state *o, st_scalars *s)
     s->icyc = o->icyc;
 }
 
+void or_rk_coeffs(const or_state *o, double rho[3], double gam[3])
+{
+    for (int q = 0; q < 3; q++) { rho[q] = o->rhork[q]; gam[q] = o->gamrk[q]; }
+}
+
 void or_set_time(or_state *o, double time, double dt, int64_t icyc)
 {
     o->time = time; o->dt = dt; o->icyc = icyc; o->icyc0 = icyc;

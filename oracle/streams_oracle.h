@@ -63,6 +63,7 @@ double *or_fhat(or_state *o);
 int *or_eoc(or_state *o);
 void or_get_scalars(const or_state *o, st_scalars *out);
 void or_set_time(or_state *o, double time, double dt, int64_t icyc);
+void or_rk_coeffs(const or_state *o, double rho[3], double gam[3]);
 
 /* building blocks, each following one reference routine */
 void or_get_deriv_coeffs(double *d1, double *d2, int order1, int order2);

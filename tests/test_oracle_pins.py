@@ -1,0 +1,500 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+Each test checks one oracle routine against a closed form, an invariant, a
+convergence rate or a brute-force evaluation written independently of the
+oracle's formula, chosen so that a plausible restatement mistake (a dropped
+term, a wrong sign, index or coefficient, a transposed eigenvector) fails.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+from numpy.polynomial import polynomial as npoly
+from scipy.signal import convolve
+
+from helpers import box_case, conserved_from_primitive
+from oracle import Oracle, lib
+from streams_mi355x import make_channel
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# --------------------------------------------------------------- coefficients
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+def test_derivative_weights_exact_on_polynomials(order):
+    """get_deriv_coeffs: the order-2m central stencils differentiate x^q exactly."""
+    d1, d2 = np.zeros(4), np.zeros(5)
+    lib().or_get_deriv_coeffs(_dp(d1), _dp(d2), order, order)
+    m = order // 2
+    for q in range(0, order + 1):
+        val = sum(d1[l - 1] * (l ** q - (-l) ** q) for l in range(1, m + 1))
+        assert abs(val - (1.0 if q == 1 else 0.0)) < 1e-13, (order, q)
+    for q in range(0, order + 2):
+        val = (d2[0] if q == 0 else 0.0) + sum(d2[l] * (l ** q + (-l) ** q) for l in range(1, m + 1))
+        assert abs(val - (2.0 if q == 2 else 0.0)) < 1e-12, (order, q)
+    # the stencil does not extend beyond m points
+    assert np.all(d1[m:] == 0.0) and np.all(d2[m + 1:] == 0.0)
+
+
+def test_wray_rk3_is_third_order():
+    """Two-register Wray scheme: published coefficients and amplification
+    1 + z + z^2/2 + z^3/6 for y' = lambda y (w_k = w_{k-1} + g_k z w_{k-1} + r_k z w_{k-2})."""
+    p, g, _, _ = box_case()
+    o = Oracle(p, g)
+    rho, gam = o.rk_coeffs()
+    assert np.allclose(gam, [8 / 15, 5 / 12, 3 / 4], rtol=0, atol=1e-16)
+    assert np.allclose(rho, [0.0, -17 / 60, -5 / 12], rtol=0, atol=1e-16)
+    assert abs(np.sum(rho + gam) - 1.0) < 1e-15
+    prev2 = np.array([0.0])
+    prev = np.array([1.0])
+    for k in range(3):
+        cur = npoly.polyadd(prev, npoly.polyadd(gam[k] * npoly.polymulx(prev), rho[k] * npoly.polymulx(prev2)))
+        prev2, prev = prev, cur
+    assert np.allclose(prev[:4], [1.0, 1.0, 0.5, 1.0 / 6.0], atol=1e-15)
+
+
+# ----------------------------------------------------------------------- EOS
+def test_eos_closed_forms():
+    p, g, _, _ = box_case()
+    o = Oracle(p, g)
+    gam = 1.4
+    cv = 1.0 / (gam - 1.0)
+    for T in (0.3, 1.0, 2.7):
+        e = lib().or_get_e_from_temperature(o.h, T)
+        assert abs(e - cv * T) < 1e-14 * cv * T
+        assert abs(lib().or_get_temperature_from_e(o.h, e, 1.0) - T) < 1e-14
+        assert abs(lib().or_get_gamloc(o.h, T) - gam) < 1e-15
+
+
+def test_eval_aux_closed_form():
+    p, g, _, co = box_case(nx=8, ny=6, nz=5)
+    o = Oracle(p, g)
+    rng = np.random.default_rng(3)
+    sh = o.w.shape[1:]
+    rho = rng.uniform(0.5, 2.0, sh)
+    u, v, w = (rng.uniform(-1, 1, sh) for _ in range(3))
+    T = rng.uniform(0.5, 2.0, sh)
+    o.w[:] = conserved_from_primitive(rho, u, v, w, T)
+    o.compute_aux(1, 1)
+    wa = o.w_aux
+    p_ = rho * T
+    assert np.allclose(wa[0], rho, rtol=1e-15)
+    for a, b in ((wa[1], u), (wa[2], v), (wa[3], w)):
+        assert np.allclose(a, b, rtol=1e-13, atol=1e-15)
+    assert np.allclose(wa[5], T, rtol=1e-13)
+    E = 2.5 * T + 0.5 * (u * u + v * v + w * w)
+    assert np.allclose(wa[4], E + p_ / rho, rtol=1e-13)
+    assert np.allclose(wa[6], p.mu0 * T ** 0.76, rtol=1e-13)
+
+
+# ------------------------------------------------------------ central fluxes
+def _box_field(o, co, rho, u, v, w, T):
+    X, Y, Z = np.meshgrid(co[0], co[1], co[2], indexing="ij")
+    X, Y, Z = X.transpose(2, 1, 0), Y.transpose(2, 1, 0), Z.transpose(2, 1, 0)  # [k, j, i]
+    fields = [f(X, Y, Z) if callable(f) else np.full(X.shape, float(f)) for f in (rho, u, v, w, T)]
+    o.w[:] = conserved_from_primitive(*fields)
+    o.compute_aux(1, 1)
+    return fields
+
+
+def _sweep(o, d):
+    o.zero_flux()
+    getattr(o, "euler_" + "xyz"[d])()
+    return o.fl.copy()
+
+
+def test_free_stream_preserved_exactly():
+    p, g, _, co = box_case()
+    o = Oracle(p, g)
+    _box_field(o, co, 1.3, 0.7, -0.4, 0.25, 1.1)
+    for d in range(3):
+        assert np.array_equal(_sweep(o, d), np.zeros_like(o.fl)), d
+
+
+def _smooth(o, co):
+    return _box_field(o, co,
+                      lambda x, y, z: 1.0 + 0.2 * np.sin(x) * np.cos(y) + 0.1 * np.cos(z),
+                      lambda x, y, z: 0.5 + 0.3 * np.sin(y + z),
+                      lambda x, y, z: 0.2 * np.cos(x - z),
+                      lambda x, y, z: 0.1 + 0.2 * np.sin(x + y),
+                      lambda x, y, z: 1.0 + 0.1 * np.cos(x) * np.sin(z))
+
+
+def test_flux_divergence_telescopes_on_periodic_box():
+    p, g, _, co = box_case()
+    o = Oracle(p, g)
+    _smooth(o, co)
+    for d in range(3):
+        fl = _sweep(o, d)
+        for m in range(5):
+            tot = fl[m].sum(axis=2 - d)
+            assert np.max(np.abs(tot)) <= 1e-12 * np.abs(fl[m]).sum(axis=2 - d).max(), (d, m)
+
+
+def test_keep_preserves_kinetic_energy_at_uniform_pressure():
+    """Sum of u.F_mom - |u|^2/2 F_mass vanishes for the KEEP split (random
+    density and velocity, T = p/rho so the pressure is uniform)."""
+    p, g, _, co = box_case(nx=12, ny=10, nz=8)
+    o = Oracle(p, g)
+    rng = np.random.default_rng(7)
+    inner = (slice(3, -3),) * 3
+    sh = tuple(n for n in (p.nz, p.ny, p.nx))
+    rho_i = rng.uniform(0.5, 1.5, sh)
+    u_i, v_i, w_i = (rng.uniform(-1, 1, sh) for _ in range(3))
+    wrap = lambda a: np.pad(a, 3, mode="wrap")
+    rho, u, v, w = (wrap(a) for a in (rho_i, u_i, v_i, w_i))
+    T = 1.0 / rho
+    o.w[:] = conserved_from_primitive(rho, u, v, w, T)
+    o.compute_aux(1, 1)
+    for d in range(3):
+        fl = _sweep(o, d)
+        ke = u_i * fl[1] + v_i * fl[2] + w_i * fl[3] - 0.5 * (u_i ** 2 + v_i ** 2 + w_i ** 2) * fl[0]
+        scale = np.abs(u_i * fl[1]).sum()
+        assert abs(ke.sum()) <= 1e-13 * scale, (d, ke.sum(), scale)
+    del inner
+
+
+def _spectral_derivative(f, L, axis):
+    n = f.shape[axis]
+    k = np.fft.fftfreq(n, d=L / n) * 2 * np.pi
+    shape = [1, 1, 1]
+    shape[axis] = n
+    return np.real(np.fft.ifft(1j * k.reshape(shape) * np.fft.fft(f, axis=axis), axis=axis))
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_central_flux_converges_at_sixth_order(d):
+    """fl of one sweep against the spectral derivative of the Euler flux
+    (rho u_n, rho u u_n + p delta, rho H u_n); error ratio on halving dx."""
+    errs = []
+    for n in (16, 32):
+        p, g, _, co = box_case(nx=n, ny=n, nz=n)
+        o = Oracle(p, g)
+        rho, u, v, w, T = (a[3:-3, 3:-3, 3:-3] for a in _smooth(o, co))
+        fl = _sweep(o, d)
+        un = (u, v, w)[d]
+        pres = rho * T
+        H = 2.5 * T + 0.5 * (u * u + v * v + w * w) + T
+        F = [rho * un, rho * u * un, rho * v * un, rho * w * un, rho * H * un]
+        F[1 + d] = F[1 + d] + pres
+        axis = 2 - d
+        err = max(np.max(np.abs(fl[m] - _spectral_derivative(F[m], 2 * np.pi, axis))) for m in range(5))
+        errs.append(err)
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 5.5, (errs, rate)
+
+
+# ----------------------------------------------------------------------- WENO
+def _wenorec(gp, gm, ord_, version=0):
+    vp = np.zeros((5, 9))
+    vm = np.zeros((5, 9))
+    vp[:, 1:1 + gp.shape[1]] = gp
+    vm[:, 1:1 + gm.shape[1]] = gm
+    out = np.zeros(5)
+    lib().or_wenorec_1d(_dp(vp), _dp(vm), _dp(out), 3 if ord_ <= 3 else 4, ord_, version, 1.0, 1.0)
+    return out
+
+
+@pytest.mark.parametrize("version", [0, 1])
+@pytest.mark.parametrize("ord_", [1, 2, 3])
+def test_weno_reproduces_constants(ord_, version):
+    gp = np.tile(np.arange(1.0, 6.0)[:, None], (1, 6))
+    gm = -0.5 * gp
+    out = _wenorec(gp, gm, ord_, version)
+    assert np.allclose(out, gp[:, 0] + gm[:, 0], rtol=1e-14)
+
+
+def test_weno5_reproduces_linear_data_at_the_face():
+    l = np.arange(1, 7, dtype=float)  # l = 1..6, face between l = 3 and 4
+    gp = np.array([0.3 + 0.7 * l + s for s in range(5)])
+    gm = np.array([-0.2 + 1.1 * l - s for s in range(5)])
+    out = _wenorec(gp, gm, 3)
+    expect = (0.3 + 0.7 * 3.5 + np.arange(5)) + (-0.2 + 1.1 * 3.5 - np.arange(5))
+    assert np.allclose(out, expect, rtol=1e-13)
+
+
+def test_weno5_smooth_data_recovers_ideal_upwind_stencil():
+    """Small-amplitude smooth data: weights -> (0.1, 0.6, 0.3), so the
+    reconstruction is the fifth-order upwind stencil (2,-13,47,27,-3)/60."""
+    x = np.arange(1, 7) * 0.1
+    gp = np.array([1e-5 * np.sin(x + s) for s in range(5)])
+    gm = np.array([1e-5 * np.cos(x - s) for s in range(5)])
+    out = _wenorec(gp, gm, 3)
+    ideal_p = (2 * gp[:, 0] - 13 * gp[:, 1] + 47 * gp[:, 2] + 27 * gp[:, 3] - 3 * gp[:, 4]) / 60
+    ideal_m = (2 * gm[:, 5] - 13 * gm[:, 4] + 47 * gm[:, 3] + 27 * gm[:, 2] - 3 * gm[:, 1]) / 60
+    assert np.allclose(out, ideal_p + ideal_m, rtol=1e-6, atol=0)
+
+
+def test_weno5_step_selects_smooth_stencil():
+    gp = np.array([[1.0, 1.0, 1.0, 1.0, 0.0, 0.0]] * 5)  # jump between l=4 and l=5
+    gm = np.zeros((5, 6))
+    out = _wenorec(gp, gm, 3)
+    assert np.all(np.abs(out - 1.0) < 1e-6)
+
+
+# ------------------------------------------------------------- Roe / eigen
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_roe_eigenvectors_orthonormal_and_roe_property(d):
+    p, g, _, co = box_case(nx=8, ny=8, nz=8)
+    o = Oracle(p, g)
+    rng = np.random.default_rng(11 + d)
+    sh = o.w.shape[1:]
+    rho = rng.uniform(0.5, 2.0, sh)
+    u, v, w = (rng.uniform(-1, 1, sh) for _ in range(3))
+    T = rng.uniform(0.5, 2.0, sh)
+    o.w[:] = conserved_from_primitive(rho, u, v, w, T)
+    o.compute_aux(1, 1)
+    i, j, k = 4, 4, 4
+    roe = np.zeros(9)
+    lib().or_compute_roe_average(o.h, d, i, j, k, _dp(roe))
+    el = np.zeros((5, 5))
+    er = np.zeros((5, 5))
+    lib().or_eigenvectors(d, _dp(roe), _dp(el), _dp(er))
+    Lm = el.T  # rows = left eigenvectors
+    Rm = er.T  # columns = right eigenvectors
+    assert np.allclose(Lm @ Rm, np.eye(5), atol=1e-13)
+    ia = (k + 2, j + 2, i + 2)
+    ib = tuple(n + (1 if 2 - ax == d else 0) for ax, n in enumerate(ia))
+
+    def state(ix):
+        r, uu, vv, ww, tt = rho[ix], u[ix], v[ix], w[ix], T[ix]
+        U = conserved_from_primitive(np.array(r), np.array(uu), np.array(vv), np.array(ww), np.array(tt))
+        un = (uu, vv, ww)[d]
+        pres = r * tt
+        F = np.array([r * un, r * uu * un, r * vv * un, r * ww * un, (U[4] + pres) * un])
+        F[1 + d] += pres
+        return np.asarray(U, dtype=float), F
+
+    UA, FA = state(ia)
+    UB, FB = state(ib)
+    un = roe[6 + d]
+    c = roe[3]
+    lam = np.diag([un - c, un, un, un, un + c])
+    assert np.allclose(Rm @ lam @ Lm @ (UB - UA), FB - FA, rtol=1e-11, atol=1e-12)
+
+
+# ------------------------------------------------------------------- viscous
+class Poly3:
+    """Polynomial in (x,y,z) with coefficient array c[a,b,c] of x^a y^b z^c."""
+
+    def __init__(self, c):
+        self.c = np.asarray(c, dtype=float)
+
+    def __call__(self, X, Y, Z):
+        return npoly.polyval3d(X, Y, Z, self.c)
+
+    def d(self, axis):
+        return Poly3(npoly.polyder(self.c, axis=axis))
+
+    def __mul__(self, o):
+        if isinstance(o, Poly3):
+            return Poly3(convolve(self.c, o.c))
+        return Poly3(self.c * o)
+
+    __rmul__ = __mul__
+
+    def __add__(self, o):
+        s = [max(a, b) for a, b in zip(self.c.shape, o.c.shape)]
+        a = np.zeros(s)
+        a[tuple(slice(0, n) for n in self.c.shape)] += self.c
+        a[tuple(slice(0, n) for n in o.c.shape)] += o.c
+        return Poly3(a)
+
+    def __sub__(self, o):
+        return self + o * -1.0
+
+
+def test_viscous_terms_on_manufactured_polynomials():
+    """visflx(4) + visflx(1) equal minus the full divergence of the viscous
+    stress and of u.tau + k grad T for polynomial fields (exact for the
+    sixth-order stencils)."""
+    p, g, _, co = box_case(nx=10, ny=9, nz=8, L=(1.0, 0.9, 0.8), mu0=1.0, visc_model=1)
+    o = Oracle(p, g)
+    rng = np.random.default_rng(5)
+
+    def rnd(const):
+        c = rng.uniform(-0.3, 0.3, (3, 3, 3))
+        c[0, 0, 0] = const
+        return Poly3(c)
+
+    U, V, W, T, M = rnd(0.2), rnd(-0.1), rnd(0.3), rnd(1.5), rnd(1.2)
+    X, Y, Z = np.meshgrid(co[0], co[1], co[2], indexing="ij")
+    X, Y, Z = (a.transpose(2, 1, 0) for a in (X, Y, Z))
+    sh = X.shape
+    o.w_aux[:] = 0.0
+    o.w_aux[0] = 1.0
+    o.w_aux[1], o.w_aux[2], o.w_aux[3] = U(X, Y, Z), V(X, Y, Z), W(X, Y, Z)
+    o.w_aux[5], o.w_aux[6] = T(X, Y, Z), M(X, Y, Z)
+    vel = [U, V, W]
+    grad = [[f.d(a) for a in range(3)] for f in vel]  # grad[i][j] = d u_i / d x_j
+    div = grad[0][0] + grad[1][1] + grad[2][2]
+    o.w_aux[9] = (div * (1.0 / 3.0))(X, Y, Z)  # ghosts of div/3 from the exact field
+    o.zero_flux()
+    o.visflx(4)
+    o.w_aux[9] = (div * (1.0 / 3.0))(X, Y, Z)
+    o.visflx(1)
+    tau = [[M * (grad[i][j] + grad[j][i]) for j in range(3)] for i in range(3)]
+    for i in range(3):
+        tau[i][i] = tau[i][i] - M * div * (2.0 / 3.0)
+    cp = p.cp_coeff[0]
+    kcond = M * (cp / p.prandtl)
+    inner = (slice(3, -3),) * 3
+    Xi, Yi, Zi = X[inner], Y[inner], Z[inner]
+    for i in range(3):
+        rhs = tau[i][0].d(0) + tau[i][1].d(1) + tau[i][2].d(2)
+        ref = -rhs(Xi, Yi, Zi)
+        assert np.allclose(o.fl[1 + i], ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max()), i
+    energy = Poly3(np.zeros((1, 1, 1)))
+    for j in range(3):
+        flux_j = vel[0] * tau[0][j] + vel[1] * tau[1][j] + vel[2] * tau[2][j] + kcond * T.d(j)
+        energy = energy + flux_j.d(j)
+    ref5 = -energy(Xi, Yi, Zi)
+    assert np.allclose(o.fl[4], ref5, rtol=1e-9, atol=1e-9 * np.abs(ref5).max())
+    # Ducros sensor of visflx(0) on the same fields
+    o.visflx(0)
+    dv = div(Xi, Yi, Zi)
+    om = [grad[2][1] - grad[1][2], grad[0][2] - grad[2][0], grad[1][0] - grad[0][1]]
+    om2 = sum(f(Xi, Yi, Zi) ** 2 for f in om)
+    sens = np.maximum(-dv / np.sqrt(om2 + dv ** 2 + (p.u0 / p.l0) ** 2), 0.0) ** 2
+    assert np.allclose(o.w_aux[7][inner], sens, rtol=1e-9, atol=1e-14)
+    del sh
+
+
+# ------------------------------------------------------- reductions / ghosts
+def test_compute_dt_brute_force():
+    c = make_channel(nx=16, ny=12, nz=8)
+    o = Oracle(c.params, c.grid)
+    o.w[:] = c.w0
+    o.init_run()
+    wa = o.w_aux[:, 3:-3, 3:-3, 3:-3]
+    cs = np.sqrt(1.4 * wa[5])
+    nu = wa[6] / wa[0]
+    g = c.grid
+    dx, dy, dz = g.dcsidx[None, None, :], g.detady[None, :, None], g.dzitdz[:, None, None]
+    cand = [(np.abs(wa[1]) + cs) * dx, (np.abs(wa[2]) + cs) * dy, (np.abs(wa[3]) + cs) * dz,
+            nu * dx ** 2, nu * dy ** 2, nu * dz ** 2]
+    cand += [a / 0.72 for a in cand[3:]]
+    dt = c.params.cfl / max(a.max() for a in cand)
+    assert abs(o.scalars()["dt"] - dt) <= 1e-13 * dt
+
+
+def test_force_rhs_brute_force():
+    c = make_channel(nx=16, ny=12, nz=8)
+    o = Oracle(c.params, c.grid)
+    o.w[:] = c.w0
+    o.init_run()
+    rng = np.random.default_rng(2)
+    o.fln[:] = rng.normal(size=o.fln.shape)
+    fln0 = o.fln.copy()
+    w = o.w[:, 3:-3, 3:-3, 3:-3]
+    T = o.w_aux[5, 3:-3, 3:-3, 3:-3]
+    u = o.w_aux[1, 3:-3, 3:-3, 3:-3]
+    dy = np.diff(c.grid.yn)[None, :, None]
+    V = c.params.volchan
+    b = [(fln0[0] * dy).sum() / V, (fln0[1] * dy).sum() / V, (w[0] * dy).sum() / V, (w[1] * dy).sum() / V,
+         (w[1] * dy * T).sum() / V]
+    o.force_rhs()
+    s = o.scalars()
+    assert abs(s["rhobulk"] - b[2]) < 1e-13 * b[2]
+    assert abs(s["ubulk"] - b[3] / b[2]) < 1e-13 * abs(b[3] / b[2])
+    assert abs(s["tbulk"] - b[4] / b[2] / (b[3] / b[2])) < 1e-13
+    assert np.allclose(o.fln[0], fln0[0] - b[0], rtol=0, atol=1e-12)
+    assert np.allclose(o.fln[1], fln0[1] - b[1], rtol=0, atol=1e-12)
+    assert np.allclose(o.fln[4], fln0[4] - u * b[1], rtol=0, atol=1e-12)
+    assert np.array_equal(o.fln[2], fln0[2])
+
+
+def test_ghost_fills_periodic_images_and_isothermal_wall():
+    c = make_channel(nx=12, ny=10, nz=8)
+    o = Oracle(c.params, c.grid)
+    o.w[:] = c.w0
+    o.compute_aux(1, 1)
+    o.update_ghost(1)
+    w = o.w
+    ng, nx, ny, nz = 3, 12, 10, 8
+    J, K = slice(ng, ng + ny), slice(ng, ng + nz)
+    # x periodic: ghost i = 1-l equals interior nx+1-l; ghost nx+l equals l
+    for l in range(1, ng + 1):
+        assert np.array_equal(w[:, K, J, ng - l], w[:, K, J, ng + nx - l])
+        assert np.array_equal(w[:, K, J, ng + nx - 1 + l], w[:, K, J, ng + l - 1])
+        assert np.array_equal(w[:, ng - l, J, ng:ng + nx], w[:, ng + nz - l, J, ng:ng + nx])
+    # staggered isothermal wall at ymin: T_g = 2 T_w - T, p_g = p, velocity reversed
+    I = slice(ng, ng + nx)
+    for l in range(1, ng + 1):
+        gi, si = ng - l, ng + l - 1
+        rho_s, rho_g = w[0, K, si, I], w[0, K, gi, I]
+        u_s = w[1:4, K, si, I] / rho_s
+        u_g = w[1:4, K, gi, I] / rho_g
+        T_s = (w[4, K, si, I] / rho_s - 0.5 * (u_s ** 2).sum(0)) / 2.5
+        T_g = (w[4, K, gi, I] / rho_g - 0.5 * (u_g ** 2).sum(0)) / 2.5
+        assert np.allclose(u_g, -u_s, rtol=1e-13, atol=1e-15)
+        assert np.allclose(T_g, 2 * c.params.t_wall - T_s, rtol=1e-12)
+        assert np.allclose(rho_g * T_g, rho_s * T_s, rtol=1e-12)
+
+
+def test_channel_at_rest_stays_at_rest():
+    """Uniform fluid at rest at the wall temperature: every term of the RK
+    stage (fluxes, viscous terms, forcing, wall ghosts) vanishes; only the
+    rounding of the boundary-order pressure weights (e.g. 0.75*4 - 0.15*8 +
+    12/60 = 2 in binary) leaves 1e-16 residues."""
+    c = make_channel(nx=12, ny=10, nz=8, noise=0.0)
+    o = Oracle(c.params, c.grid)
+    w = c.w0.copy()
+    w[1:4] = 0.0
+    w[4] = np.where(w[0] > 0, w[0] * 2.5 * c.params.t_wall, 0.0)
+    o.w[:] = w
+    o.init_run()
+    o.advance(3)
+    assert np.max(np.abs(o.w[:, 3:-3, 3:-3, 3:-3] - w[:, 3:-3, 3:-3, 3:-3])) < 1e-14
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_keep_split_matches_published_node_form(d):
+    """The flux-form sweep equals the node (difference) form of the KEEP
+    scheme of Kuya, Totani & Kawai (J. Comput. Phys. 375, 2018):
+    D_i = 2/dx sum_l a_l [T(i,i+l) - T(i-l,i)] with T = rho_bar u_bar phi_bar
+    for mass and momentum, p_bar for the normal momentum, and
+    rho_bar u_bar e_bar + rho_bar u_bar (u_a.u_b)/2 + (u_a p_b + u_b p_a)/2
+    for the total energy (bars: two-point arithmetic means)."""
+    p, g, dx, co = box_case(nx=12, ny=11, nz=10)
+    o = Oracle(p, g)
+    rng = np.random.default_rng(17 + d)
+    sh = (p.nz, p.ny, p.nx)
+    wrap = lambda a: np.pad(a, 3, mode="wrap")
+    rho, u, v, w, T = (wrap(a) for a in (rng.uniform(0.6, 1.4, sh), rng.uniform(-1, 1, sh),
+                                          rng.uniform(-1, 1, sh), rng.uniform(-1, 1, sh),
+                                          rng.uniform(0.7, 1.3, sh)))
+    o.w[:] = conserved_from_primitive(rho, u, v, w, T)
+    o.compute_aux(1, 1)
+    fl = _sweep(o, d)
+    pres = rho * T
+    e = 2.5 * T
+    a = [0.75, -0.15, 1.0 / 60.0]
+    axis = 2 - d
+    un = (u, v, w)[d]
+
+    def sh_(f, s):
+        return np.roll(f, -s, axis=axis)[3:-3, 3:-3, 3:-3]
+
+    def pair(s0, s1):
+        rb = 0.5 * (sh_(rho, s0) + sh_(rho, s1))
+        ub = 0.5 * (sh_(un, s0) + sh_(un, s1))
+        m = rb * ub
+        out = [m, m * 0.5 * (sh_(u, s0) + sh_(u, s1)), m * 0.5 * (sh_(v, s0) + sh_(v, s1)),
+               m * 0.5 * (sh_(w, s0) + sh_(w, s1))]
+        out[1 + d] = out[1 + d] + 0.5 * (sh_(pres, s0) + sh_(pres, s1))
+        uu = sh_(u, s0) * sh_(u, s1) + sh_(v, s0) * sh_(v, s1) + sh_(w, s0) * sh_(w, s1)
+        out.append(m * 0.5 * (sh_(e, s0) + sh_(e, s1)) + m * 0.5 * uu
+                   + 0.5 * (sh_(un, s0) * sh_(pres, s1) + sh_(un, s1) * sh_(pres, s0)))
+        return out
+
+    D = [0.0] * 5
+    for l in range(1, 4):
+        fwd, bwd = pair(0, l), pair(-l, 0)
+        for m in range(5):
+            D[m] = D[m] + 2.0 * a[l - 1] * (fwd[m] - bwd[m])
+    for m in range(5):
+        ref = D[m] / dx[d]
+        assert np.allclose(fl[m], ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max()), (d, m)
