@@ -23,7 +23,6 @@
 namespace st {
 
 constexpr int H = 3;           // stencil half width of the flux windows
-constexpr int XSEG = 63;       // cells per 64-lane x segment
 constexpr int RED_THREADS = 256;
 
 // -------------------------------------------------------------- utilities
@@ -236,85 +235,80 @@ __device__ __forceinline__ void list_append(int *__restrict__ list, int *__restr
 
 __global__ void k_reset_count(int *__restrict__ count) { *count = 0; }
 
-// --------------------------------------------------------------- Euler x
-// One 64-lane workgroup per 63-cell segment of an x row.  Lane t computes
-// central face i0-1+t into LDS; lanes 1..63 then form the divergence of their
-// cell (euler_x_fluxes_hybrid_kernel + euler_x_update_subroutine,
-// kernels_cpu.F90:425-689, 944-963).  init = true writes the divergence (the
-// stage's first contribution after init_flux's fl = 0).
-template <bool INIT, int KM>
-__global__ __launch_bounds__(64) void k_euler_x(Geo g, KPar P, const double *__restrict__ wa,
-                                                const int *__restrict__ eoc, const double *__restrict__ dcsidx,
-                                                double *__restrict__ fl, int *__restrict__ list,
-                                                int *__restrict__ count) {
-    __shared__ double sfh[5][64];
-    __shared__ int sweno[64];
-    const int nseg = (g.nx + XSEG - 1) / XSEG;
-    const int seg = blockIdx.x % nseg;
-    const int row = blockIdx.x / nseg;
-    const int j = row % g.ny + 1, k = row / g.ny + 1;
-    const int lane = threadIdx.x;
-    const int i0 = 1 + XSEG * seg;
-    const int ilast = min(i0 + XSEG - 1, g.nx);
-    const int f = i0 - 1 + lane;
-    const int lo = P.eul_lo[0], hi = P.eul_hi[0];
-    const bool keep = P.nkeep >= 0;
-    int weno = 0;
-    if (f <= ilast && f >= lo - 1 && f <= hi) {
-        const size_t nv = g.nvol;
-        const int base = gidx(g, f, j, k);
-        bool shk = false;
-#pragma unroll
-        for (int s = 0; s < 2 * H; s++) {
-            if (s >= H - P.weno_scheme && s < H + P.weno_scheme)
-                shk |= wa[7 * nv + base + s - (H - 1)] > P.sensor_threshold;
-        }
-        if (!shk) {
-            Pc q[2 * H];
-#pragma unroll
-            for (int s = 0; s < 2 * H; s++) {
-                const int o = base + s - (H - 1);
-                q[s] = make_pc(wa[o], wa[nv + o], wa[2 * nv + o], wa[3 * nv + o], wa[4 * nv + o], wa[5 * nv + o],
-                               P.rgas0, keep);
+// ------------------------------------------------------------ Euler y / z
+// Streaming form of the central split along DIR.  One thread per (i, row)
+// marches over a chunk of points; when point b arrives it forms its three
+// pairs (b-l, b), l = 1..3, once, and adds c_l(face) times each pair to the
+// three faces b-3..b-1 it straddles (the pair sums of the reference's
+// double loop over l and m, kernels_cpu.F90:1861-1936, regrouped by pair:
+// each pair is evaluated once instead of once per face).  Face b-3 is then
+// complete; its divergence goes to cell b-3 (euler_y/z_update_subroutine,
+// :991-1010, 1039-1058).  Faces whose sensor window exceeds the threshold
+// are left to k_weno_fix through the dirty-cell list.
+struct Acc {
+    double m, mu, mv, mw, e, pp;
+};
+
+template <int DIR, int KM>
+__device__ __forceinline__ Acc pair_terms(const KPar &P, const Pc &a, const Pc &b) {
+    const double una = un_of<DIR>(a), unb = un_of<DIR>(b);
+    const double rhom = a.r + b.r;
+    double uv = (una + unb) * rhom;
+    Acc r;
+    if (KM >= 0) {
+        const double eem = a.e + b.e;
+        double deef = 1.0;
+        if (KM > 0) {
+            const double drho = 2.0 * (b.r - a.r) / rhom;
+            const double dee = 2.0 * (b.e - a.e) / eem;
+            double t_sdr = (0.5 * drho) * (0.5 * drho);
+            double t_sde = (0.5 * dee) * (0.5 * dee);
+            const double t2_sdr = t_sdr, t2_sde = t_sde;
+            double sdr = 1.0 + t_sdr / 3.0, sde = 1.0 + t_sde, sne = 1.0 + t_sde / 3.0;
+            for (int n = 2; n <= P.nkeep; n++) {
+                const int n2 = 2 * n;
+                t_sdr = t2_sdr * t_sdr;
+                t_sde = t2_sde * t_sde;
+                sdr = sdr + t_sdr / (1.0 + n2);
+                sde = sde + t_sde;
+                sne = sne + t_sde / (1.0 + n2);
             }
-            int lmax = P.lmax_base + eoc[f];
-            lmax = lmax < 1 ? 1 : lmax;
-            const bool zf = (f == 0 && P.fzf[0] == 1) || (f == g.nx && P.fzf[1] == 1);
-            double fh[5];
-            central_flux<0, H, Pc, KM>(P, q, lmax, zf, fh);
-#pragma unroll
-            for (int m = 0; m < 5; m++) sfh[m][lane] = fh[m];
-        } else {
-            weno = 1;
+            uv = uv * (1.0 / sdr);
+            deef = sne / sde;
         }
+        r.e = uv * eem * deef + uv * (a.u * b.u + a.v * b.v + a.w * b.w) + 4.0 * (una * b.p + unb * a.p);
+    } else {
+        r.e = uv * (a.e + b.e);
     }
-    sweno[lane] = weno;
-    __syncthreads();
-    const int i = f;
-    const bool cell = lane >= 1 && i <= ilast;
-    const bool in = cell && i >= lo && i <= hi;
-    const bool dirty = in && (sweno[lane] | sweno[lane - (lane >= 1 ? 1 : 0)]);
-    if (cell) {
-        const int o = iidx(g, i, j, k);
-        const double d = dcsidx[i - 1];
-#pragma unroll
-        for (int m = 0; m < 5; m++) {
-            if (INIT) {
-                fl[(size_t)m * g.nint + o] = (in && !dirty) ? (sfh[m][lane] - sfh[m][lane - 1]) * d : 0.0;
-            } else if (in && !dirty) {
-                fl[(size_t)m * g.nint + o] += (sfh[m][lane] - sfh[m][lane - 1]) * d;
-            }
-        }
-    }
-    list_append(list, count, dirty, cell ? iidx(g, i, j, k) : 0);
+    r.m = uv * 2.0;
+    r.mu = uv * (a.u + b.u);
+    r.mv = uv * (a.v + b.v);
+    r.mw = uv * (a.w + b.w);
+    r.pp = 2.0 * (a.p + b.p);
+    return r;
 }
 
-// ------------------------------------------------------------ Euler y / z
-// One thread per (i, row) marching over a chunk of cells along DIR with a
-// sliding 6-point window in registers and a one-point prefetch; each w_aux
-// value is loaded once per chunk and each central face computed once
-// (euler_y/z_hybrid_kernel + euler_y/z_update_subroutine,
-// kernels_cpu.F90:1808-2071, 991-1010 and 1063-1325, 1039-1058).
+__device__ __forceinline__ void acc_add(Acc &A, double c, const Acc &p) {
+    A.m += c * p.m;
+    A.mu += c * p.mu;
+    A.mv += c * p.mv;
+    A.mw += c * p.mw;
+    A.e += c * p.e;
+    A.pp += c * p.pp;
+}
+
+__device__ __forceinline__ Acc acc_zero() {
+    Acc a;
+    a.m = a.mu = a.mv = a.mw = a.e = a.pp = 0.0;
+    return a;
+}
+
+// coeff_deriv1(l, lmax) of a face with effective half width lm (1..3)
+__device__ __forceinline__ double face_coeff(const KPar &P, int lm, int l) {
+    if (l > lm) return 0.0;
+    return lm == 3 ? P.cd1tab[2][l - 1] : (lm == 2 ? P.cd1tab[1][l - 1] : P.cd1tab[0][l - 1]);
+}
+
 template <int DIR, int KM>
 __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double *__restrict__ wa,
                                                      const int *__restrict__ eoc, const double *__restrict__ metric,
@@ -334,63 +328,239 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double
     const int n = DIR == 1 ? g.ny : g.nz;
     const int stride = DIR == 1 ? g.sx : g.sxy;
     const size_t nv = g.nvol;
-    const bool keep = P.nkeep >= 0;
+    const bool keep = KM >= 0;
     const double thr = P.sensor_threshold;
     const int ws = P.weno_scheme;
-    // sensor window bits that decide a face: s in [H-ws, H+ws)
-    const unsigned smask = ((1u << (2 * ws)) - 1u) << (H - ws);
+    // bit s of hist = sensor flag of point b-s; a face f = b-3 uses points
+    // f-ws+1 .. f+ws, i.e. bits 3-ws .. 2+ws
+    const unsigned smask = ((1u << (2 * ws)) - 1u) << (3 - ws);
     const int p0 = DIR == 1 ? gidx(g, i, 0, o) : gidx(g, i, o, 0);  // offset of along-index 0
-    Pc q[6];
-    unsigned sen = 0;
-#pragma unroll
-    for (int s = 0; s < 6; s++) {
-        const int a = p0 + (c0 - 3 + s) * stride;
-        q[s] = make_pc(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0,
-                       keep);
-        sen |= (wa[7 * nv + a] > thr ? 1u : 0u) << s;
+    const int lb = P.lmax_base;
+    auto lm_of = [&](int f) {
+        const int ff = f < 0 ? 0 : (f > n ? n : f);
+        const int v = lb + eoc[ff];
+        return v < 1 ? 1 : v;
+    };
+    Pc q0, q1, q2;
+    unsigned hist = 0;
+    {
+        int a = p0 + (c0 - 3) * stride;
+        q0 = make_pc(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0, keep);
+        hist |= (wa[7 * nv + a] > thr ? 1u : 0u) << 2;
+        a += stride;
+        q1 = make_pc(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0, keep);
+        hist |= (wa[7 * nv + a] > thr ? 1u : 0u) << 1;
+        a += stride;
+        q2 = make_pc(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0, keep);
+        hist |= (wa[7 * nv + a] > thr ? 1u : 0u);
     }
+    Acc A0 = acc_zero(), A1 = acc_zero(), A2 = acc_zero();  // faces b-3, b-2, b-1
+    int lm0 = lm_of(c0 - 3), lm1 = lm_of(c0 - 2), lm2 = lm_of(c0 - 1);
     double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     bool wprev = false;
-    for (int f = c0 - 1; f <= c1; f++) {
-        // prefetch the point entering the window next
-        double nr = 0, nu = 0, nvv = 0, nw = 0, nh = 0, nt = 0, ns = 0;
-        const bool more = f < c1;
-        if (more) {
-            const int a = p0 + (f + 4) * stride;
+    // prefetch of the next point
+    int a = p0 + c0 * stride;
+    double nr = wa[a], nu = wa[nv + a], nvv = wa[2 * nv + a], nw = wa[3 * nv + a], nh = wa[4 * nv + a],
+           nt = wa[5 * nv + a], ns = wa[7 * nv + a];
+    for (int b = c0; b <= c1 + 3; b++) {
+        // issue the read of fl at the cell completed in this iteration first,
+        // so that its latency overlaps the pair arithmetic below
+        double flv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        const int fc = b - 3;
+        const int oic = DIR == 1 ? iidx(g, i, fc < 1 ? 1 : fc, o) : iidx(g, i, o, fc < 1 ? 1 : fc);
+        if (fc >= c0) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) flv[m] = fl[(size_t)m * g.nint + oic];
+        }
+        const Pc qb = make_pc(nr, nu, nvv, nw, nh, nt, P.rgas0, keep);
+        hist = (hist << 1) | (ns > thr ? 1u : 0u);
+        if (b < c1 + 3) {
+            a += stride;
             nr = wa[a]; nu = wa[nv + a]; nvv = wa[2 * nv + a]; nw = wa[3 * nv + a];
             nh = wa[4 * nv + a]; nt = wa[5 * nv + a]; ns = wa[7 * nv + a];
         }
-        const bool wcur = (sen & smask) != 0u;
+        {
+            const Acc p3 = pair_terms<DIR, KM>(P, q0, qb);
+            acc_add(A0, face_coeff(P, lm0, 3), p3);
+            acc_add(A1, face_coeff(P, lm1, 3), p3);
+            acc_add(A2, face_coeff(P, lm2, 3), p3);
+        }
+        {
+            const Acc p2 = pair_terms<DIR, KM>(P, q1, qb);
+            acc_add(A1, face_coeff(P, lm1, 2), p2);
+            acc_add(A2, face_coeff(P, lm2, 2), p2);
+        }
+        {
+            const Acc p1 = pair_terms<DIR, KM>(P, q2, qb);
+            acc_add(A2, face_coeff(P, lm2, 1), p1);
+        }
+        const int f = b - 3;  // face complete
+        if (f >= c0 - 1) {
+            // (the fl values of cell f were loaded at the top of the iteration)
+            const bool wcur = (hist & smask) != 0u;
+            double fh[5];
+            fh[0] = 0.25 * A0.m;
+            fh[1] = 0.25 * A0.mu;
+            fh[2] = 0.25 * A0.mv;
+            fh[3] = 0.25 * A0.mw;
+            fh[4] = 0.25 * A0.e;
+            if ((f == 0 && P.fzf[2 * DIR] == 1) || (f == n && P.fzf[2 * DIR + 1] == 1)) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) fh[m] = 0.0;
+            }
+            fh[1 + DIR] = fh[1 + DIR] + 0.5 * A0.pp;
+            bool dirty = false;
+            if (f >= c0) {
+                dirty = wcur || wprev;
+                if (!dirty) {
+                    const double d = metric[f - 1];
+#pragma unroll
+                    for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oic] = flv[m] + (fh[m] - fprev[m]) * d;
+                }
+            }
+            list_append(list, count, dirty, DIR == 1 ? iidx(g, i, f < 1 ? 1 : f, o) : iidx(g, i, o, f < 1 ? 1 : f));
+#pragma unroll
+            for (int m = 0; m < 5; m++) fprev[m] = fh[m];
+            wprev = wcur;
+        }
+        A0 = A1;
+        A1 = A2;
+        A2 = acc_zero();
+        lm0 = lm1;
+        lm1 = lm2;
+        lm2 = lm_of(b);
+        q0 = q1;
+        q1 = q2;
+        q2 = qb;
+    }
+}
+
+// --------------------------------------------------------------- Euler x
+// Lane-parallel streaming form along x.  A wave covers 64 consecutive
+// points i0-3 .. i0+60 of one (j,k) row: each lane loads one point, forms
+// the three pairs (b-l, b) ending at its point and their coefficient-weighted
+// sums for the faces b-1, b-2, b-3; face f is then assembled from three
+// lanes with cross-lane shuffles, and the divergence of cell f takes face
+// f-1 from the neighbouring lane (euler_x_fluxes_hybrid_kernel +
+// euler_x_update_subroutine, kernels_cpu.F90:425-689, 944-963, with the
+// pair sums regrouped by pair).  Faces 59 per wave, cells 58.
+constexpr int XW = 58;
+
+__device__ __forceinline__ Pc shfl_up_pc(const Pc &q, int d) {
+    Pc r;
+    r.r = __shfl_up(q.r, d, 64);
+    r.u = __shfl_up(q.u, d, 64);
+    r.v = __shfl_up(q.v, d, 64);
+    r.w = __shfl_up(q.w, d, 64);
+    r.p = __shfl_up(q.p, d, 64);
+    r.e = __shfl_up(q.e, d, 64);
+    return r;
+}
+
+__device__ __forceinline__ Acc shfl_down_acc(const Acc &q, int d) {
+    Acc r;
+    r.m = __shfl_down(q.m, d, 64);
+    r.mu = __shfl_down(q.mu, d, 64);
+    r.mv = __shfl_down(q.mv, d, 64);
+    r.mw = __shfl_down(q.mw, d, 64);
+    r.e = __shfl_down(q.e, d, 64);
+    r.pp = __shfl_down(q.pp, d, 64);
+    return r;
+}
+
+template <bool INIT, int KM>
+__global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const double *__restrict__ wa,
+                                                 const int *__restrict__ eoc, const double *__restrict__ dcsidx,
+                                                 double *__restrict__ fl, int *__restrict__ list,
+                                                 int *__restrict__ count) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int nseg = (g.nx + XW - 1) / XW;
+    const int nwork = nseg * g.ny * g.nz;
+    const int lo = P.eul_lo[0], hi = P.eul_hi[0];
+    const bool keep = KM >= 0;
+    const size_t nv = g.nvol;
+    const int ws = P.weno_scheme;
+    const unsigned long long wmask = (1ull << (2 * ws)) - 1ull;
+    const int lb = P.lmax_base;
+    auto lm_of = [&](int f) {
+        const int ff = f < 0 ? 0 : (f > g.nx ? g.nx : f);
+        const int v = lb + eoc[ff];
+        return v < 1 ? 1 : v;
+    };
+    for (int wk = wave; wk < nwork; wk += nwaves) {
+        const int seg = wk % nseg;
+        const int row = wk / nseg;
+        const int j = row % g.ny + 1, k = row / g.ny + 1;
+        const int i0 = 1 + XW * seg;
+        const int pt = i0 - 3 + lane;                     // point held by this lane
+        const int pl = pt > g.nx + 3 ? g.nx + 3 : pt;     // clamp the load into the ghosted row
+        const int a = gidx(g, pl, j, k);
+        const Pc q = make_pc(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a],
+                             P.rgas0, keep);
+        const bool flag = wa[7 * nv + a] > P.sensor_threshold;
+        const Pc q1 = shfl_up_pc(q, 1), q2 = shfl_up_pc(q, 2), q3 = shfl_up_pc(q, 3);
+        const int lmA = lm_of(pt - 1), lmB = lm_of(pt - 2), lmC = lm_of(pt - 3);
+        Acc Q1 = acc_zero(), Q2 = acc_zero(), Q3 = acc_zero();
+        {
+            const Acc p3 = pair_terms<0, KM>(P, q3, q);
+            acc_add(Q1, face_coeff(P, lmA, 3), p3);
+            acc_add(Q2, face_coeff(P, lmB, 3), p3);
+            acc_add(Q3, face_coeff(P, lmC, 3), p3);
+        }
+        {
+            const Acc p2 = pair_terms<0, KM>(P, q2, q);
+            acc_add(Q1, face_coeff(P, lmA, 2), p2);
+            acc_add(Q2, face_coeff(P, lmB, 2), p2);
+        }
+        {
+            const Acc p1 = pair_terms<0, KM>(P, q1, q);
+            acc_add(Q1, face_coeff(P, lmA, 1), p1);
+        }
+        // face f = pt: pairs from the lanes of points f+1, f+2, f+3
+        Acc F = shfl_down_acc(Q1, 1);
+        {
+            const Acc t2 = shfl_down_acc(Q2, 2);
+            F.m += t2.m; F.mu += t2.mu; F.mv += t2.mv; F.mw += t2.mw; F.e += t2.e; F.pp += t2.pp;
+            const Acc t3 = shfl_down_acc(Q3, 3);
+            F.m += t3.m; F.mu += t3.mu; F.mv += t3.mv; F.mw += t3.mw; F.e += t3.e; F.pp += t3.pp;
+        }
+        const int f = pt;
         double fh[5];
-        if (!wcur) {
-            int lmax = P.lmax_base + eoc[f];
-            lmax = lmax < 1 ? 1 : lmax;
-            const bool zf = (f == 0 && P.fzf[2 * DIR] == 1) || (f == n && P.fzf[2 * DIR + 1] == 1);
-            central_flux<DIR, H, Pc, KM>(P, q, lmax, zf, fh);
-        } else {
+        fh[0] = 0.25 * F.m;
+        fh[1] = 0.25 * F.mu;
+        fh[2] = 0.25 * F.mv;
+        fh[3] = 0.25 * F.mw;
+        fh[4] = 0.25 * F.e;
+        if ((f == 0 && P.fzf[0] == 1) || (f == g.nx && P.fzf[1] == 1)) {
 #pragma unroll
             for (int m = 0; m < 5; m++) fh[m] = 0.0;
         }
-        bool dirty = false;
-        if (f >= c0) {
-            dirty = wcur || wprev;
-            if (!dirty) {
-                const int oi = DIR == 1 ? iidx(g, i, f, o) : iidx(g, i, o, f);
-                const double d = metric[f - 1];
+        fh[1] = fh[1] + 0.5 * F.pp;
+        // WENO face: a sensor flag among points f-ws+1 .. f+ws
+        const unsigned long long fm = __ballot(flag);
+        const int sh = lane - ws + 1;
+        const bool wf = lane >= 2 && lane <= 60 && ((fm >> (sh < 0 ? 0 : sh)) & wmask) != 0ull;
+        // cell c = pt, needs face c-1 from the lane below
+        double fl_[5];
 #pragma unroll
-                for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] += (fh[m] - fprev[m]) * d;
+        for (int m = 0; m < 5; m++) fl_[m] = __shfl_up(fh[m], 1, 64);
+        const bool wl = __shfl_up(wf ? 1 : 0, 1, 64) != 0;
+        const int c = pt;
+        const bool cell = lane >= 3 && lane <= 60 && c <= g.nx;
+        const bool in = cell && c >= lo && c <= hi;
+        const bool dirty = in && (wf || wl);
+        if (cell) {
+            const int o = iidx(g, c, j, k);
+            const double d = dcsidx[c - 1];
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                if (INIT) fl[(size_t)m * g.nint + o] = (in && !dirty) ? (fh[m] - fl_[m]) * d : 0.0;
+                else if (in && !dirty) fl[(size_t)m * g.nint + o] += (fh[m] - fl_[m]) * d;
             }
         }
-        list_append(list, count, dirty, DIR == 1 ? iidx(g, i, f < 1 ? 1 : f, o) : iidx(g, i, o, f < 1 ? 1 : f));
-#pragma unroll
-        for (int m = 0; m < 5; m++) fprev[m] = fh[m];
-        wprev = wcur;
-        if (more) {
-#pragma unroll
-            for (int s = 0; s < 5; s++) q[s] = q[s + 1];
-            q[5] = make_pc(nr, nu, nvv, nw, nh, nt, P.rgas0, keep);
-            sen = (sen >> 1) | ((ns > thr ? 1u : 0u) << 5);
-        }
+        list_append(list, count, dirty, cell ? iidx(g, c, j, k) : 0);
     }
 }
 
@@ -1105,12 +1275,14 @@ hipError_t launch_euler_x(st_solver *s, double *fl, bool init) {
         if (init) return launch_zero(s, fl, (size_t)5 * g.nint);
         return hipSuccess;
     }
-    const int nseg = (g.nx + XSEG - 1) / XSEG;
-    dim3 grid((unsigned)(nseg * g.ny * g.nz));
+    const long nwork = (long)((g.nx + XW - 1) / XW) * g.ny * g.nz;
+    long nblk = (nwork + 3) / 4;
+    if (nblk > 256L * 4) nblk = 256L * 4;  // resident workgroups at 4 waves per SIMD
+    dim3 grid((unsigned)nblk);
     hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, s->stream, s->counters);
     const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
 #define ST_LAUNCH_X(I, K)                                                                                          \
-    hipLaunchKernelGGL((k_euler_x<I, K>), grid, dim3(64), 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl, \
+    hipLaunchKernelGGL((k_euler_x<I, K>), grid, dim3(256), 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl, \
                        s->dirty, s->counters)
     if (init) {
         if (km == 0) ST_LAUNCH_X(true, 0); else if (km > 0) ST_LAUNCH_X(true, 1); else ST_LAUNCH_X(true, -1);
@@ -1129,13 +1301,12 @@ static hipError_t launch_march(st_solver *s, double *fl, int dir) {
     if (hi < lo) return hipSuccess;
     const int ncell = hi - lo + 1;
     const int no = dir == 1 ? g.nz : g.ny;
-    // enough threads to fill the chip: aim at >= 4 waves per SIMD over 256 CUs
-    const long target = 256L * 4 * 4 * 64;
-    int nchunk = (int)((target + (long)g.nx * no - 1) / ((long)g.nx * no));
-    nchunk = nchunk < 1 ? 1 : nchunk;
-    int chunk = (ncell + nchunk - 1) / nchunk;
-    if (chunk < 8) chunk = 8 < ncell ? 8 : ncell;
-    nchunk = (ncell + chunk - 1) / chunk;
+    // chunks of ~32 cells (3 warm-up points each) so that the grid holds
+    // several times the resident waves of the chip
+    int chunk = ncell < 32 ? ncell : 32;
+    const long minthreads = 256L * 16 * 64 * 4;
+    while (chunk > 8 && (long)g.nx * no * ((ncell + chunk - 1) / chunk) < minthreads) chunk /= 2;
+    const int nchunk = (ncell + chunk - 1) / chunk;
     const size_t tot = (size_t)g.nx * no * nchunk;
     dim3 grid((unsigned)((tot + 255) / 256));
     hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, s->stream, s->counters);

@@ -67,9 +67,13 @@ class StreamsError(RuntimeError):
         self.code = code
 
 
-def lib(path: str = LIB_PATH):
-    """Load the library once; raise if it has not been built."""
+def lib(path: str = None):
+    """Load the library once; raise if it has not been built.  The
+    STREAMS_MI355X_LIB environment variable selects another build of the
+    same library (A/B timing of kernel variants)."""
     global _LIB
+    if path is None:
+        path = os.environ.get("STREAMS_MI355X_LIB", LIB_PATH)
     if _LIB is None:
         if not os.path.exists(path):
             raise RuntimeError(f"{LIB_NAME} not built at {path}: run make -C streams-2_amd")
