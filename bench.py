@@ -130,6 +130,7 @@ def main():
     value = cells * 3 * args.steps / el / 1e9
     res = g.compute_residual()
     finite = res["residual_rhou"] == res["residual_rhou"]
+    weno_frac = [float(x) for x in g.count_weno()]
 
     # per-kernel timing on the solver's stream (separate, short pass)
     kt = g.kernel_times(min(20, max(3, args.steps // 10)))
@@ -160,7 +161,8 @@ def main():
         "data": "synthetic (seeded laminar Poiseuille + 10% uniform velocity noise, seed 0)",
         "config": {"workload": f"channel {nx}x{ny}x{nz}, M_b 1.5, Re_tau {args.retau:g}, FP64 plain, "
                                "KEEP6 + WENO5 (Ducros > 0.9), Wray RK3, CFL 0.8",
-                   "grid": [nx, ny, nz], "precision": "fp64", "finite": bool(finite)},
+                   "grid": [nx, ny, nz], "precision": "fp64", "finite": bool(finite),
+                   "weno_face_fraction_xyz": weno_frac},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": per * units, "avg_launch_ms": ms},

@@ -233,7 +233,7 @@ int64_t st_solver_footprint(const st_params *p) {
     const int64_t sx = p->nx + 2 * p->ng, sy = p->ny + 2 * p->ng, sz = p->nz + 2 * p->ng;
     const int64_t nvol = sx * sy * sz, nint = (int64_t)p->nx * p->ny * p->nz;
     const int64_t blocks = (int64_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * p->nz;
-    return 8 * (15 * nvol + 10 * nint + 3 * sx * sz + 7 * blocks + 1024) + 4 * (nint + p->nx + p->ny + p->nz + 3 + 4) +
+    return 8 * (15 * nvol + 10 * nint + 3 * sx * sz + 7 * blocks + 1024) + (nint + 4 * (p->nx + p->ny + p->nz + 3 + 4)) +
            8 * (2 * sx + 2 * sy + 2 * sz + 3 * (p->nx + p->ny + p->nz) + p->ny + 1) + 4096;
 }
 
@@ -291,7 +291,8 @@ int st_solver_create(const st_params *p, st_solver **out) {
     es[ne++] = dalloc(&s->dzitdz2, (size_t)g.nz);
     es[ne++] = dalloc(&s->ds, 1);
     es[ne++] = dalloc(&s->partials, (size_t)s->npart);
-    es[ne++] = dalloc(&s->dirty, nint);
+    es[ne++] = dalloc(&s->dirty, nint + 64);  // per-cell flags + the any-dirty word
+    if (es[ne - 1] == hipSuccess) es[ne - 1] = hipMemset(s->dirty, 0, nint + 64);
     es[ne++] = dalloc(&s->counters, 4);
     es[ne++] = hipHostMalloc((void **)&s->hs, sizeof(st::DevScalars), hipHostMallocDefault);
     for (int q = 0; q < ne; q++) {

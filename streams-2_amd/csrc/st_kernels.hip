@@ -215,25 +215,22 @@ __global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__rest
     wa[6 * nv + d] = viscosity_of(P, tt);
 }
 
-// ------------------------------------------------------- dirty-cell list
+// ------------------------------------------------------- dirty-cell flags
 // Faces whose sensor window exceeds the threshold take the characteristic
 // WENO flux.  The main sweep kernels evaluate only central faces, so their
 // register budget stays that of the KEEP split; a cell with a WENO face on
-// either side is appended to a list (wave-aggregated atomic) and handled by
-// k_weno_fix, which recomputes both faces of the cell with the full hybrid
-// logic and applies the same divergence as the reference.
-__device__ __forceinline__ void list_append(int *__restrict__ list, int *__restrict__ count, bool pred, int value) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0ull) return;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+// either side gets its flag byte set and is handled by k_weno_fix, which
+// recomputes both faces of the cell with the full hybrid logic, applies the
+// same divergence as the reference and clears the flag.
+// flags[nint] doubles as the any-dirty word of the sweep: one plain store per
+// wave that has a dirty lane (no returned value, so the wave never waits).
+__device__ __forceinline__ void mark_dirty(unsigned char *__restrict__ flags, int nint, bool dirty, int cell) {
+    const unsigned long long mask = __ballot(dirty);
+    if (dirty) flags[cell] = 1;
+    if (mask != 0ull && __lane_id() == __ffsll((long long)mask) - 1) flags[nint] = 1;
 }
 
-__global__ void k_reset_count(int *__restrict__ count) { *count = 0; }
+__global__ void k_clear_any(unsigned char *__restrict__ flags, int nint) { flags[nint] = 0; }
 
 // ------------------------------------------------------------ Euler y / z
 // Streaming form of the central split along DIR.  One thread per (i, row)
@@ -313,7 +310,7 @@ template <int DIR, int KM>
 __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double *__restrict__ wa,
                                                      const int *__restrict__ eoc, const double *__restrict__ metric,
                                                      double *__restrict__ fl, int chunk, int nchunk,
-                                                     int *__restrict__ list, int *__restrict__ count) {
+                                                     unsigned char *__restrict__ flags) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int no = DIR == 1 ? g.nz : g.ny;  // the other transverse extent
     if (t >= g.nx * no * nchunk) return;
@@ -418,7 +415,7 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double
                     for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oic] = flv[m] + (fh[m] - fprev[m]) * d;
                 }
             }
-            list_append(list, count, dirty, DIR == 1 ? iidx(g, i, f < 1 ? 1 : f, o) : iidx(g, i, o, f < 1 ? 1 : f));
+            mark_dirty(flags, g.nint, dirty, oic);
 #pragma unroll
             for (int m = 0; m < 5; m++) fprev[m] = fh[m];
             wprev = wcur;
@@ -471,8 +468,7 @@ __device__ __forceinline__ Acc shfl_down_acc(const Acc &q, int d) {
 template <bool INIT, int KM>
 __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const double *__restrict__ wa,
                                                  const int *__restrict__ eoc, const double *__restrict__ dcsidx,
-                                                 double *__restrict__ fl, int *__restrict__ list,
-                                                 int *__restrict__ count) {
+                                                 double *__restrict__ fl, unsigned char *__restrict__ flags) {
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -560,7 +556,7 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const double *__
                 else if (in && !dirty) fl[(size_t)m * g.nint + o] += (fh[m] - fl_[m]) * d;
             }
         }
-        list_append(list, count, dirty, cell ? iidx(g, c, j, k) : 0);
+        mark_dirty(flags, g.nint, dirty, cell ? iidx(g, c, j, k) : 0);
     }
 }
 
@@ -590,53 +586,103 @@ __device__ __forceinline__ void hybrid_face(const KPar &P, const Pt (&q)[6], con
     }
 }
 
-// Cells of the list: both faces recomputed with the hybrid logic, then
-// fl += (fhat(c) - fhat(c-1)) * metric(c).
+// Flagged cells: both faces recomputed with the hybrid logic, then
+// fl += (fhat(c) - fhat(c-1)) * metric(c), and the flag cleared.  One
+// 64-lane workgroup per grid-stride lane set; flagged cells are compacted
+// into an LDS queue with a ballot so the WENO work runs on full waves.
 template <int DIR>
-__global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const double *__restrict__ wa,
-                                                 const int *__restrict__ eoc, const double *__restrict__ metric,
-                                                 double *__restrict__ fl, const int *__restrict__ list,
-                                                 const int *__restrict__ count) {
-    const int ncell = *count;
+__device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const double *__restrict__ wa,
+                                         const int *__restrict__ eoc, const double *__restrict__ metric,
+                                         double *__restrict__ fl, int oi) {
     const size_t nv = g.nvol;
     const bool keep = P.nkeep >= 0;
     const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
     const int stride = DIR == 0 ? 1 : (DIR == 1 ? g.sx : g.sxy);
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ncell; e += gridDim.x * blockDim.x) {
-        const int oi = list[e];
-        const int i = oi % g.nx + 1;
-        const int j = (oi / g.nx) % g.ny + 1;
-        const int k = oi / (g.nx * g.ny) + 1;
-        const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
-        const int base = gidx(g, i, j, k);
-        Pt q[7];
-        double sen[7];
+    const int i = oi % g.nx + 1;
+    const int j = (oi / g.nx) % g.ny + 1;
+    const int k = oi / (g.nx * g.ny) + 1;
+    const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
+    const int base = gidx(g, i, j, k);
+    Pt q[7];
+    double sen[7];
 #pragma unroll
-        for (int s = 0; s < 7; s++) {
-            const int a = base + (s - 3) * stride;
-            q[s] = make_pt(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a],
-                           P.rgas0, keep);
-            sen[s] = wa[7 * nv + a];
-        }
-        double flo[5], fhi[5];
-        {
-            Pt ql[6];
-            double sl[6];
-#pragma unroll
-            for (int s = 0; s < 6; s++) { ql[s] = q[s]; sl[s] = sen[s]; }
-            hybrid_face<DIR>(P, ql, sl, c - 1, eoc[c - 1], n, flo);
-        }
-        {
-            Pt qh[6];
-            double sh[6];
-#pragma unroll
-            for (int s = 0; s < 6; s++) { qh[s] = q[s + 1]; sh[s] = sen[s + 1]; }
-            hybrid_face<DIR>(P, qh, sh, c, eoc[c], n, fhi);
-        }
-        const double d = metric[c - 1];
-#pragma unroll
-        for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] += (fhi[m] - flo[m]) * d;
+    for (int s = 0; s < 7; s++) {
+        const int a = base + (s - 3) * stride;
+        q[s] = make_pt(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0,
+                       keep);
+        sen[s] = wa[7 * nv + a];
     }
+    double flo[5], fhi[5];
+    {
+        Pt ql[6];
+        double sl[6];
+#pragma unroll
+        for (int s = 0; s < 6; s++) { ql[s] = q[s]; sl[s] = sen[s]; }
+        hybrid_face<DIR>(P, ql, sl, c - 1, eoc[c - 1], n, flo);
+    }
+    {
+        Pt qh[6];
+        double sh[6];
+#pragma unroll
+        for (int s = 0; s < 6; s++) { qh[s] = q[s + 1]; sh[s] = sen[s + 1]; }
+        hybrid_face<DIR>(P, qh, sh, c, eoc[c], n, fhi);
+    }
+    const double d = metric[c - 1];
+#pragma unroll
+    for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] += (fhi[m] - flo[m]) * d;
+}
+
+template <int DIR>
+__global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const double *__restrict__ wa,
+                                                 const int *__restrict__ eoc, const double *__restrict__ metric,
+                                                 double *__restrict__ fl, unsigned char *__restrict__ flags,
+                                                 int region) {
+    __shared__ int queue[64 * 5];
+    const int lane = threadIdx.x;
+    int qn = 0;  // uniform across the (single-wave) workgroup
+    const int ncell = g.nint;
+    if (flags[ncell] == 0) return;  // no dirty cell in this sweep
+    const int r0 = blockIdx.x * region;
+    const int r1 = min(r0 + region, ncell);
+    // each lane reads 8 flags at once (one 64-bit word), 4 words in flight
+    const unsigned long long *words = (const unsigned long long *)flags;
+    for (int base = r0; base < r1; base += 4 * 512) {
+        unsigned long long wd[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int idx0 = base + u * 512 + lane * 8;
+            wd[u] = idx0 < r1 ? words[idx0 >> 3] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int idx0 = base + u * 512 + lane * 8;
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const bool f = ((wd[u] >> (8 * b)) & 0xffull) != 0ull && idx0 + b < r1;
+                const unsigned long long mask = __ballot(f);
+                if (f) {
+                    queue[qn + __popcll(mask & ((1ull << lane) - 1ull))] = idx0 + b;
+                    flags[idx0 + b] = 0;
+                }
+                qn += __popcll(mask);
+                if (qn >= 64 * 4) {  // keep room for the next byte position
+                    __syncthreads();
+                    while (qn >= 64) {
+                        fix_cell<DIR>(g, P, wa, eoc, metric, fl, queue[qn - 64 + lane]);
+                        qn -= 64;
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        __syncthreads();
+        while (qn >= 64) {
+            fix_cell<DIR>(g, P, wa, eoc, metric, fl, queue[qn - 64 + lane]);
+            qn -= 64;
+        }
+        __syncthreads();
+    }
+    if (lane < qn) fix_cell<DIR>(g, P, wa, eoc, metric, fl, queue[lane]);
 }
 
 // ---------------------------------------------------------------- viscous
@@ -1256,16 +1302,22 @@ hipError_t launch_wall_ghosts(st_solver *s) {
 
 static hipError_t launch_fix(st_solver *s, double *fl, int dir) {
     const Geo &g = s->g;
-    const dim3 grid(1024), block(64);
+    // one 64-lane workgroup per contiguous region of flags; regions large
+    // enough that the LDS queue fills whole waves of WENO work
+    int nb = (g.nint + 16383) / 16384;
+    nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
+    int region = (g.nint + nb - 1) / nb;
+    region = (region + 2047) / 2048 * 2048;  // whole 64-bit flag words per lane
+    const dim3 grid((unsigned)nb), block(64);
     if (dir == 0)
         hipLaunchKernelGGL(k_weno_fix<0>, grid, block, 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl,
-                           s->dirty, s->counters);
+                           s->dirty, region);
     else if (dir == 1)
         hipLaunchKernelGGL(k_weno_fix<1>, grid, block, 0, s->stream, g, s->kp, s->w_aux, s->eoc[1], s->detady, fl,
-                           s->dirty, s->counters);
+                           s->dirty, region);
     else
         hipLaunchKernelGGL(k_weno_fix<2>, grid, block, 0, s->stream, g, s->kp, s->w_aux, s->eoc[2], s->dzitdz, fl,
-                           s->dirty, s->counters);
+                           s->dirty, region);
     return hipGetLastError();
 }
 
@@ -1275,15 +1327,15 @@ hipError_t launch_euler_x(st_solver *s, double *fl, bool init) {
         if (init) return launch_zero(s, fl, (size_t)5 * g.nint);
         return hipSuccess;
     }
+    hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
     const long nwork = (long)((g.nx + XW - 1) / XW) * g.ny * g.nz;
     long nblk = (nwork + 3) / 4;
     if (nblk > 256L * 4) nblk = 256L * 4;  // resident workgroups at 4 waves per SIMD
     dim3 grid((unsigned)nblk);
-    hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, s->stream, s->counters);
     const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
 #define ST_LAUNCH_X(I, K)                                                                                          \
     hipLaunchKernelGGL((k_euler_x<I, K>), grid, dim3(256), 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl, \
-                       s->dirty, s->counters)
+                       s->dirty)
     if (init) {
         if (km == 0) ST_LAUNCH_X(true, 0); else if (km > 0) ST_LAUNCH_X(true, 1); else ST_LAUNCH_X(true, -1);
     } else {
@@ -1308,12 +1360,12 @@ static hipError_t launch_march(st_solver *s, double *fl, int dir) {
     while (chunk > 8 && (long)g.nx * no * ((ncell + chunk - 1) / chunk) < minthreads) chunk /= 2;
     const int nchunk = (ncell + chunk - 1) / chunk;
     const size_t tot = (size_t)g.nx * no * nchunk;
+    hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
     dim3 grid((unsigned)((tot + 255) / 256));
-    hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(1), 0, s->stream, s->counters);
     const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
 #define ST_LAUNCH_M(D, K, E, M)                                                                                     \
     hipLaunchKernelGGL((k_euler_march<D, K>), grid, dim3(256), 0, s->stream, g, s->kp, s->w_aux, E, M, fl, chunk, \
-                       nchunk, s->dirty, s->counters)
+                       nchunk, s->dirty)
     if (dir == 1) {
         if (km == 0) ST_LAUNCH_M(1, 0, s->eoc[1], s->detady);
         else if (km > 0) ST_LAUNCH_M(1, 1, s->eoc[1], s->detady);

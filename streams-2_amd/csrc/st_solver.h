@@ -24,8 +24,8 @@ struct st_solver {
     st::DevScalars *hs;  // pinned host mirror
     double *partials;
     int npart;           // capacity in doubles
-    int *dirty;          // cells with a WENO face (one sweep direction at a time)
-    int *counters;       // [0] length of dirty, [1] reduction ticket
+    unsigned char *dirty;  // per-cell flag: a WENO face on either side (one sweep at a time)
+    int *counters;         // [1] reduction ticket
     int cur;             // F[cur] is the right-hand side being built
     double rhork[3], gamrk[3];
     int last_stage;      // RK stage (0..2) of the last completed stage, -1 none
