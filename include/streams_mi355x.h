@@ -70,7 +70,7 @@ typedef struct st_params {
     int32_t correct_bound_ord;       /* 1 lowers the stencil order at non-periodic faces  */
     int32_t iter_dt_recompute;       /* CFL dt recomputed every this many steps           */
     int32_t precision;               /* ST_PREC_FP64 or ST_PREC_MIXED                     */
-    int32_t pad0;
+    int32_t pad0, pad1;              /* explicit: the record has no implicit padding      */
     double rgas0, prandtl, t0, rho0, u0, l0, mu0;
     double sutherland_s, t_ref_dim, powerlaw_vtexp;
     double sensor_threshold;
@@ -132,8 +132,10 @@ int st_update_ghost(st_solver *s, int32_t do_swap);
 int st_bcswap(st_solver *s);
 /* compute_aux (singleideal_cpu.F90:281-330). */
 int st_compute_aux(st_solver *s, int32_t central, int32_t ghost);
-/* visflx(mode) (singleideal_cpu.F90:186-279): 0 full+sensor, 1 div, 3 sensor,
- * 4 no-sensor; accumulates into fl. */
+/* visflx(mode) (singleideal_cpu.F90:186-279), accumulating into fl:
+ *   0 Laplacian form + sensor, 1 div/3 terms, 3 sensor, 4 Laplacian form;
+ *   conservative form (conservative_viscous=1): 2 cell terms + sensor,
+ *   5 face terms, 6 cell terms, 7 second-order div/3 terms. */
 int st_visflx(st_solver *s, int32_t mode);
 /* euler_x/y/z (singleideal_cpu.F90:674-843): flux + divergence into fl. */
 int st_euler_x(st_solver *s);
@@ -142,7 +144,8 @@ int st_euler_z(st_solver *s);
 /* zero_flux_subroutine (kernels_cpu.F90:9-25). */
 int st_zero_flux(st_solver *s);
 /* Start-of-run sequence of run() (singleideal_cpu.F90:1547-1594):
- * update_ghost, compute_aux, visflx(0), sensor ghost swap, zero_flux. */
+ * update_ghost, compute_aux, visflx(0) (visflx(2) for the conservative
+ * form), sensor ghost swap, zero_flux, compute_dt. */
 int st_init_run(st_solver *s);
 /* compute_dt (singleideal_cpu.F90:1476-1517); dt_out may be NULL, in which
  * case dt stays on the device and no host synchronisation happens. */

@@ -33,7 +33,7 @@ def lib():
         L.or_create.argtypes = [C.POINTER(StParams)]
         L.or_destroy.argtypes = [vp]
         L.or_set_grid.argtypes = [vp] + [dp] * 13
-        for name in ("or_w", "or_w_aux", "or_fl", "or_fln", "or_fhat"):
+        for name in ("or_w", "or_w_aux", "or_fl", "or_fln", "or_fhat", "or_wallprop"):
             getattr(L, name).restype = dp
             getattr(L, name).argtypes = [vp]
         L.or_eoc.restype = ip

@@ -22,8 +22,12 @@
  *   or_visflx                 manufactured polynomial fields
  *   or_compute_dt, reductions brute force
  *   ghost fills               exact equality with index formulas
- * The conservative viscous form (conservative_viscous=1), the Rusanov split
- * and the BL/SBLI boundary conditions are not restated here.
+ *   conservative viscous      manufactured fields on which the second-order
+ *   (modes 2, 5, 6, 7)        face/cell differences are exact
+ *   whole channel path        the reference's golden progress row
+ *                             (tests/tests.ini, [subsonic_channel])
+ * The Rusanov split and the BL/SBLI boundary conditions are not restated
+ * here.
  */
 #include "streams_oracle.h"
 
@@ -228,6 +232,7 @@ double *or_w_aux(or_state *o) { return o->w_aux; }
 double *or_fl(or_state *o) { return o->fl; }
 double *or_fln(or_state *o) { return o->fln; }
 double *or_fhat(or_state *o) { return o->fhat; }
+double *or_wallprop(or_state *o) { return o->wallprop; }
 int *or_eoc(or_state *o) { return o->eoc; }
 
 void or_get_scalars(const or_state *o, st_scalars *s)
@@ -1125,13 +1130,209 @@ static void sensor(or_state *o)
             }
 }
 
+/* ------------------------------------------- conservative viscous form */
+#define XC(i) o->x[(i) + o->ng - 1]
+#define YC(j) o->y[(j) + o->ng - 1]
+#define ZC(k) o->z[(k) + o->ng - 1]
+
+static double cp_at(const or_state *o, double tt)
+{
+    const st_params *p = &o->p;
+    if (p->calorically_perfect == 1) return CP(0);
+    double cploc = 0.0;
+    for (int ll = p->indx_cp_l; ll <= p->indx_cp_r; ll++) cploc = cploc + CP(ll) * ipow(tt / p->t0, ll);
+    return cploc;
+}
+
+/* visflx_reduced_ord2_subroutine, kernels_cpu.F90:4623-4721 (visflx modes 2
+ * and 6): second-order cell terms grad(mu).sigma and the dissipation, div/3
+ * and |omega| into w_aux(10), w_aux(9), wall shear into wallprop and, with
+ * update_sensor, the Ducros sensor. */
+static void visflx_reduced_ord2(or_state *o, int update_sensor)
+{
+    const st_params *p = &o->p;
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= o->nz; k++)
+        for (int j = 1; j <= o->ny; j++)
+            for (int i = 1; i <= o->nx; i++) {
+                double uu = WA(i, j, k, 2), vv = WA(i, j, k, 3), ww = WA(i, j, k, 4);
+                double tt = WA(i, j, k, 6), mu = WA(i, j, k, 7);
+                double cploc = cp_at(o, tt);
+                double dxl = 1.0 / (XC(i + 1) - XC(i - 1));
+                double dyl = 1.0 / (YC(j + 1) - YC(j - 1));
+                double dzl = 1.0 / (ZC(k + 1) - ZC(k - 1));
+                double ux = dxl * (WA(i + 1, j, k, 2) - WA(i - 1, j, k, 2));
+                double vx = dxl * (WA(i + 1, j, k, 3) - WA(i - 1, j, k, 3));
+                double wx = dxl * (WA(i + 1, j, k, 4) - WA(i - 1, j, k, 4));
+                double mux = dxl * (WA(i + 1, j, k, 7) - WA(i - 1, j, k, 7));
+                double uy = dyl * (WA(i, j + 1, k, 2) - WA(i, j - 1, k, 2));
+                double vy = dyl * (WA(i, j + 1, k, 3) - WA(i, j - 1, k, 3));
+                double wy = dyl * (WA(i, j + 1, k, 4) - WA(i, j - 1, k, 4));
+                double ty = dyl * (WA(i, j + 1, k, 6) - WA(i, j - 1, k, 6));
+                double muy = dyl * (WA(i, j + 1, k, 7) - WA(i, j - 1, k, 7));
+                double uz = dzl * (WA(i, j, k + 1, 2) - WA(i, j, k - 1, 2));
+                double vz = dzl * (WA(i, j, k + 1, 3) - WA(i, j, k - 1, 3));
+                double wz = dzl * (WA(i, j, k + 1, 4) - WA(i, j, k - 1, 4));
+                double muz = dzl * (WA(i, j, k + 1, 7) - WA(i, j, k - 1, 7));
+                if (j == 1) {
+                    size_t wp = (size_t)(i + o->ng - 1) + (size_t)o->sx * (size_t)(k + o->ng - 1);
+                    size_t ws = (size_t)o->sx * o->sz;
+                    o->wallprop[0 * ws + wp] = mu * uy;
+                    o->wallprop[1 * ws + wp] = mu * wy;
+                    o->wallprop[2 * ws + wp] = mu * ty * cploc / p->prandtl;
+                }
+                double div = ux + vy + wz;
+                double div3l = div / 3.0;
+                WA(i, j, k, 10) = div3l;
+                double omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
+                double omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
+                WA(i, j, k, 9) = sqrt(omod2);
+                if (update_sensor == 1) {
+                    double s = -div / sqrt(omod2 + div * div + sq(p->u0 / p->l0));
+                    s = dmax(s, 0.0);
+                    WA(i, j, k, 8) = s * s;
+                }
+                double sig11 = ux - 2.0 * div3l, sig12 = vx, sig13 = wx;
+                double sig21 = uy, sig22 = vy - 2.0 * div3l, sig23 = wy;
+                double sig31 = uz, sig32 = vz, sig33 = wz - 2.0 * div3l;
+                double sigx = mux * sig11 + muy * sig12 + muz * sig13;
+                double sigy = mux * sig21 + muy * sig22 + muz * sig23;
+                double sigz = mux * sig31 + muy * sig32 + muz * sig33;
+                double sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig21 * vx + sig22 * vy + sig23 * vz +
+                                sig31 * wx + sig32 * wy + sig33 * wz) * mu;
+                double sigq = sigx * uu + sigy * vv + sigz * ww + sigah;
+                FL(i, j, k, 2) = FL(i, j, k, 2) - sigx;
+                FL(i, j, k, 3) = FL(i, j, k, 3) - sigy;
+                FL(i, j, k, 4) = FL(i, j, k, 4) - sigz;
+                FL(i, j, k, 5) = FL(i, j, k, 5) - sigq;
+            }
+}
+
+/* Face flux of the conservative Laplacian terms between points a (left) and
+ * b (right) whose node distance is h: mu_face * d(u,v,w)/dn and
+ * mu_face * (cp/Pr dT/dn + d(|u|^2/2)/dn); kernels_cpu.F90:4805-4836. */
+static void cons_face(const or_state *o, size_t a, size_t b, double h, double out[4])
+{
+    const st_params *p = &o->p;
+    const double *wa = o->w_aux;
+    size_t nv = o->nvol;
+    double uu = wa[1 * nv + a], uup = wa[1 * nv + b];
+    double vv = wa[2 * nv + a], vvp = wa[2 * nv + b];
+    double ww = wa[3 * nv + a], wwp = wa[3 * nv + b];
+    double tt = wa[5 * nv + a], ttp = wa[5 * nv + b];
+    double mu = wa[6 * nv + a], mup = wa[6 * nv + b];
+    double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+    double qqp = 0.5 * (uup * uup + vvp * vvp + wwp * wwp);
+    double cploc = p->calorically_perfect == 1 ? CP(0) : cp_at(o, 0.5 * (tt + ttp));
+    double sigx = uup - uu, sigy = vvp - vv, sigz = wwp - ww;
+    double sigq_tt = ttp - tt, sigq_qq = qqp - qq;
+    double muf = mu + mup;
+    muf = 0.5 * muf / h;
+    out[0] = sigx * muf;
+    out[1] = sigy * muf;
+    out[2] = sigz * muf;
+    out[3] = (sigq_tt * cploc / p->prandtl + sigq_qq) * muf;
+}
+
+/* visflx_x_subroutine, kernels_cpu.F90:4779-4852 (visflx mode 5, x part):
+ * face values into fhat(i = 0..nx), then fl += dxhl*(fhat(i)-fhat(i-1)). */
+static void visflx_cons_x(or_state *o)
+{
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= o->nz; k++)
+        for (int j = 1; j <= o->ny; j++)
+            for (int i = 0; i <= o->nx; i++) {
+                double s[4];
+                cons_face(o, G(o, i, j, k), G(o, i + 1, j, k), XC(i + 1) - XC(i), s);
+                FH(i, j, k, 2) = -s[0];
+                FH(i, j, k, 3) = -s[1];
+                FH(i, j, k, 4) = -s[2];
+                FH(i, j, k, 5) = -s[3];
+            }
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= o->nz; k++)
+        for (int j = 1; j <= o->ny; j++)
+            for (int i = 1; i <= o->nx; i++) {
+                double dxhl = 2.0 / (XC(i + 1) - XC(i - 1));
+                for (int iv = 2; iv <= 5; iv++)
+                    FL(i, j, k, iv) = FL(i, j, k, iv) + dxhl * (FH(i, j, k, iv) - FH(i - 1, j, k, iv));
+            }
+}
+
+/* visflx_y_subroutine (kernels_cpu.F90:4856-4926) for dir 1 and
+ * visflx_z_subroutine (:4929-4999) for dir 2: a march over the faces
+ * l = 0..n carrying the previous face flux times the cell's dyhl, so that
+ * fl(l) += (flux(l-1/2)*dyhl(l)) - flux(l+1/2)*dyhl(l). */
+static void visflx_cons_march(or_state *o, int dir)
+{
+    int n = dir == 1 ? o->ny : o->nz;
+    int na = dir == 1 ? o->nx : o->nx;
+    int nb = dir == 1 ? o->nz : o->ny;
+    const double *c = dir == 1 ? o->y : o->z;
+#define CC(l) c[(l) + o->ng - 1]
+#pragma omp parallel for schedule(static)
+    for (int b = 1; b <= nb; b++)
+        for (int i = 1; i <= na; i++) {
+            double fo[4] = {0, 0, 0, 0}, dhl = 0.0;
+            for (int l = 0; l <= n; l++) {
+                size_t pa, pb;
+                if (dir == 1) { pa = G(o, i, l, b); pb = G(o, i, l + 1, b); }
+                else          { pa = G(o, i, b, l); pb = G(o, i, b, l + 1); }
+                double s[4];
+                cons_face(o, pa, pb, CC(l + 1) - CC(l), s);
+                if (l > 0) {
+                    for (int q = 0; q < 4; q++) {
+                        double *f = dir == 1 ? &FL(i, l, b, q + 2) : &FL(i, b, l, q + 2);
+                        *f = *f + fo[q] - s[q] * dhl;
+                    }
+                }
+                if (l < n) {
+                    dhl = 2.0 / (CC(l + 2) - CC(l));
+                    for (int q = 0; q < 4; q++) fo[q] = s[q] * dhl;
+                }
+            }
+        }
+#undef CC
+}
+
+/* visflx_div_ord2_subroutine, kernels_cpu.F90:3446-3480 (visflx mode 7) */
+static void visflx_div_ord2(or_state *o)
+{
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= o->nz; k++)
+        for (int j = 1; j <= o->ny; j++)
+            for (int i = 1; i <= o->nx; i++) {
+                double uu = WA(i, j, k, 2), vv = WA(i, j, k, 3), ww = WA(i, j, k, 4);
+                double mu = WA(i, j, k, 7);
+                double dxl = mu / (XC(i + 1) - XC(i - 1));
+                double dyl = mu / (YC(j + 1) - YC(j - 1));
+                double dzl = mu / (ZC(k + 1) - ZC(k - 1));
+                double sigx = dxl * (WA(i + 1, j, k, 10) - WA(i - 1, j, k, 10));
+                double sigy = dyl * (WA(i, j + 1, k, 10) - WA(i, j - 1, k, 10));
+                double sigz = dzl * (WA(i, j, k + 1, 10) - WA(i, j, k - 1, 10));
+                double sigq = sigx * uu + sigy * vv + sigz * ww;
+                FL(i, j, k, 2) = FL(i, j, k, 2) - sigx;
+                FL(i, j, k, 3) = FL(i, j, k, 3) - sigy;
+                FL(i, j, k, 4) = FL(i, j, k, 4) - sigz;
+                FL(i, j, k, 5) = FL(i, j, k, 5) - sigq;
+            }
+}
+
 /* visflx(mode), singleideal_cpu.F90:186-279 (grid_dim == 1 branches) */
 void or_visflx(or_state *o, int mode)
 {
     if (mode == 0) visflx_lapl(o, 1);
     else if (mode == 1) visflx_div(o);
+    else if (mode == 2) visflx_reduced_ord2(o, 1);
     else if (mode == 3) sensor(o);
     else if (mode == 4) visflx_lapl(o, 0);
+    else if (mode == 5) {
+        visflx_cons_x(o);
+        visflx_cons_march(o, 1);
+        visflx_cons_march(o, 2);
+    }
+    else if (mode == 6) visflx_reduced_ord2(o, 0);
+    else if (mode == 7) visflx_div_ord2(o);
 }
 
 /* -------------------------------------------------------------- forcing */
@@ -1241,8 +1442,8 @@ void or_compute_dt(or_state *o)
     o->dt = p->cfl * dt_min;
 }
 
-/* rk_sync, singleideal_cpu.F90:332-409 (Wray RK3, conservative_viscous=0,
- * grid_dim 1, no limiter/sponge/tripping) */
+/* rk_sync, singleideal_cpu.F90:332-409 (Wray RK3, grid_dim 1, no
+ * limiter/sponge/tripping) */
 void or_rk_sync(or_state *o)
 {
     if (o->p.channel_case) o->dpdx = 0.0;
@@ -1253,7 +1454,12 @@ void or_rk_sync(or_state *o)
         or_bcswap(o);
         or_compute_aux(o, 1, 1);
         or_euler_x(o);
-        or_visflx(o, 4);
+        if (o->p.conservative_viscous == 1) {
+            or_visflx(o, 6);
+            or_visflx(o, 5);
+        } else {
+            or_visflx(o, 4);
+        }
         or_bcextr_var(o, 10);
         or_bcswap_var(o, 10);
         or_euler_y(o);
@@ -1262,7 +1468,7 @@ void or_rk_sync(or_state *o)
             or_visflx(o, 3);
             or_bcswap_var(o, 8);
         }
-        or_visflx(o, 1);
+        or_visflx(o, o->p.conservative_viscous == 1 ? 7 : 1);
         or_update_flux(o, gamdt);
         if (o->p.channel_case) or_force_rhs(o);
         or_update_field(o);
@@ -1276,7 +1482,7 @@ void or_init_run(or_state *o)
 {
     or_update_ghost(o, 1);
     or_compute_aux(o, 1, 1);
-    or_visflx(o, 0);
+    or_visflx(o, o->p.conservative_viscous == 1 ? 2 : 0);
     or_bcswap_var(o, 8);
     or_zero_flux(o);
     or_compute_dt(o);

@@ -60,6 +60,7 @@ double *or_w_aux(or_state *o);
 double *or_fl(or_state *o);
 double *or_fln(or_state *o);
 double *or_fhat(or_state *o);
+double *or_wallprop(or_state *o);
 int *or_eoc(or_state *o);
 void or_get_scalars(const or_state *o, st_scalars *out);
 void or_set_time(or_state *o, double time, double dt, int64_t icyc);

@@ -61,7 +61,8 @@ int validate(const st_params *p) {
     if (p->weno_scheme < 1 || p->weno_scheme > 3) return fail(ST_ERR_ARG, "weno_scheme must be 1..3 with ng = 3");
     if (p->weno_version != 0 && p->weno_version != 1) return fail(ST_ERR_ARG, "weno_version must be 0 or 1");
     if (p->flux_splitting != 0) return fail(ST_ERR_ARG, "flux_splitting=1 (Rusanov) is not supported");
-    if (p->conservative_viscous != 0) return fail(ST_ERR_ARG, "conservative_viscous=1 is not supported");
+    if (p->conservative_viscous != 0 && p->conservative_viscous != 1)
+        return fail(ST_ERR_ARG, "conservative_viscous must be 0 or 1");
     if (p->calorically_perfect != 0 && p->calorically_perfect != 1)
         return fail(ST_ERR_ARG, "calorically_perfect must be 0 or 1");
     if (p->indx_cp_r - p->indx_cp_l + 2 > ST_MAX_CP_COEFF) return fail(ST_ERR_ARG, "too many cp coefficients");
@@ -200,11 +201,14 @@ int stage(st_solver *s, int istep) {
     // bcswap + compute_aux(ghost=1): aux over the box, periodic ghosts from images
     TIMED(ST_K_AUX, launch_aux(s, 1 - g.ng, g.nx + g.ng, 1 - g.ng, g.ny + g.ng, 1 - g.ng, g.nz + g.ng, true));
     TIMED(ST_K_EULER_X, launch_euler_x(s, fcur, true));
-    TIMED(ST_K_VISC, launch_visc(s, fcur, 4, true));      // + bcextr_var / bcswap_var of w_aux(10)
+    const bool cons = s->p.conservative_viscous == 1;
+    // visflx(4), or visflx(6) + visflx(5); + bcextr_var / bcswap_var of w_aux(10)
+    if (cons) TIMED(ST_K_VISC, launch_visc_cons(s, fcur, 56, true));
+    else TIMED(ST_K_VISC, launch_visc(s, fcur, 4, true));
     TIMED(ST_K_EULER_Y, launch_euler_y(s, fcur));
     if (s->kp.ndim == 3) TIMED(ST_K_EULER_Z, launch_euler_z(s, fcur));
     if (istep == 2) TIMED(ST_K_SENSOR, launch_sensor(s, true));  // visflx(3) + bcswap_var(w_aux 8)
-    TIMED(ST_K_VISC_DIV, launch_visc_div(s, fcur, fprev, istep, s->p.channel_case != 0));
+    TIMED(ST_K_VISC_DIV, launch_visc_div(s, fcur, fprev, istep, s->p.channel_case != 0, cons));  // visflx(1) or (7)
     if (s->p.channel_case) TIMED(ST_K_FINISH, launch_force_finish(s, istep));
     TIMED(ST_K_UPDATE, launch_update(s, fprev, fcur, istep));
     if (s->p.channel_case && s->p.theta_wall >= -1.0) TIMED(ST_K_FORCE_VAR, launch_force_var(s));
@@ -467,9 +471,13 @@ int st_visflx(st_solver *s, int32_t mode) {
     double *fl = s->F[s->cur];
     switch (mode) {
     case 0: HIPCHK(st::launch_visc(s, fl, 0, false)); break;
-    case 1: HIPCHK(st::launch_visc_div(s, fl, s->F[s->cur ^ 1], -1, false)); break;
+    case 1: HIPCHK(st::launch_visc_div(s, fl, s->F[s->cur ^ 1], -1, false, false)); break;
+    case 2: HIPCHK(st::launch_visc_cons(s, fl, 2, false)); break;
     case 3: HIPCHK(st::launch_sensor(s, false)); break;
     case 4: HIPCHK(st::launch_visc(s, fl, 4, false)); break;
+    case 5: HIPCHK(st::launch_visc_cons(s, fl, 5, false)); break;
+    case 6: HIPCHK(st::launch_visc_cons(s, fl, 6, false)); break;
+    case 7: HIPCHK(st::launch_visc_div(s, fl, s->F[s->cur ^ 1], -1, false, true)); break;
     default: return fail(ST_ERR_ARG, "visflx mode " + std::to_string(mode) + " is not supported");
     }
     return ST_OK;
@@ -513,7 +521,9 @@ int st_init_run(st_solver *s) {
     if ((rc = st_update_ghost(s, 1))) return rc;
     if ((rc = st_compute_aux(s, 1, 1))) return rc;
     if ((rc = st_zero_flux(s))) return rc;
-    HIPCHK(st::launch_visc(s, s->F[s->cur], 0, false));
+    // visflx(2) for the conservative viscous form, else visflx(0)
+    if (s->p.conservative_viscous == 1) HIPCHK(st::launch_visc_cons(s, s->F[s->cur], 2, false));
+    else HIPCHK(st::launch_visc(s, s->F[s->cur], 0, false));
     if ((rc = swap_var(s, 8, false))) return rc;
     if ((rc = st_zero_flux(s))) return rc;
     s->last_stage = -1;

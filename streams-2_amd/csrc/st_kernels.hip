@@ -845,6 +845,147 @@ __global__ __launch_bounds__(256) void k_visc(Geo g, KPar P, double *__restrict_
     }
 }
 
+// ------------------------------------------------ conservative viscous form
+// Point values the second-order conservative terms need: u, v, w, T, mu.
+struct VP {
+    double u, v, w, t, m;
+};
+__device__ __forceinline__ VP load_vp(const double *__restrict__ wa, size_t nv, int c) {
+    VP p;
+    p.u = wa[nv + c];
+    p.v = wa[2 * nv + c];
+    p.w = wa[3 * nv + c];
+    p.t = wa[5 * nv + c];
+    p.m = wa[6 * nv + c];
+    return p;
+}
+
+// Face flux between points a (left) and b (right) at node distance h
+// (kernels_cpu.F90:4805-4836): mu_f (du, dv, dw) and
+// mu_f (cp/Pr dT + d(|u|^2/2)), mu_f = (mu_a + mu_b)/2/h.
+__device__ __forceinline__ void cons_face(const KPar &P, const VP &a, const VP &b, double h, double s[4]) {
+    const double qq = 0.5 * (a.u * a.u + a.v * a.v + a.w * a.w);
+    const double qqp = 0.5 * (b.u * b.u + b.v * b.v + b.w * b.w);
+    const double cploc = P.calorically_perfect == 1 ? P.cp[0 - P.indx_cp_l] : cploc_of(P, 0.5 * (a.t + b.t));
+    double muf = a.m + b.m;
+    muf = 0.5 * muf / h;
+    s[0] = (b.u - a.u) * muf;
+    s[1] = (b.v - a.v) * muf;
+    s[2] = (b.w - a.w) * muf;
+    s[3] = ((b.t - a.t) * cploc / P.prandtl + (qqp - qq)) * muf;
+}
+
+// Conservative viscous terms of one cell.
+//   CELL:  visflx_reduced_ord2_subroutine (kernels_cpu.F90:4623-4721,
+//          visflx modes 2/6): grad(mu).sigma + dissipation, div/3 and |omega|
+//          into w_aux(10)/(9), wall shear into wallprop, Ducros sensor when
+//          SENSOR (mode 2);
+//   FACES: visflx_x/y/z_subroutine (:4779-4999, mode 5), the two x faces,
+//          then the y faces, then the z faces of the cell, with the
+//          reference's per-cell summation order.
+// ``images`` also writes the ghost images of w_aux(10) (bcextr_var mode 1
+// on walls, bcswap_var on periodic sides).
+template <bool CELL, bool FACES, bool SENSOR>
+__global__ __launch_bounds__(256) void k_visc_cons(Geo g, KPar P, double *__restrict__ wa, double *__restrict__ fl,
+                                                   const double *__restrict__ xc, const double *__restrict__ yc,
+                                                   const double *__restrict__ zc, double *__restrict__ wallprop,
+                                                   int images) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const size_t nv = g.nvol;
+    const int c = gidx(g, i, j, k);
+    const int sy = g.sx, sz = g.sxy;
+    const VP p0 = load_vp(wa, nv, c);
+    const VP xm = load_vp(wa, nv, c - 1), xp = load_vp(wa, nv, c + 1);
+    const VP ym = load_vp(wa, nv, c - sy), yp = load_vp(wa, nv, c + sy);
+    const VP zm = load_vp(wa, nv, c - sz), zp = load_vp(wa, nv, c + sz);
+    const int ng = g.ng;
+    const double xi_m = xc[i - 1 + ng - 1], xi = xc[i + ng - 1], xi_p = xc[i + 1 + ng - 1];
+    const double yj_m = yc[j - 1 + ng - 1], yj = yc[j + ng - 1], yj_p = yc[j + 1 + ng - 1];
+    const double zk_m = zc[k - 1 + ng - 1], zk = zc[k + ng - 1], zk_p = zc[k + 1 + ng - 1];
+    const int oi = iidx(g, i, j, k);
+    const size_t ni = g.nint;
+    double f2 = fl[ni + oi], f3 = fl[2 * ni + oi], f4 = fl[3 * ni + oi], f5 = fl[4 * ni + oi];
+    if (CELL) {
+        const double cploc = cploc_of(P, p0.t);
+        const double dxl = 1.0 / (xi_p - xi_m);
+        const double dyl = 1.0 / (yj_p - yj_m);
+        const double dzl = 1.0 / (zk_p - zk_m);
+        const double ux = dxl * (xp.u - xm.u), vx = dxl * (xp.v - xm.v), wx = dxl * (xp.w - xm.w);
+        const double mux = dxl * (xp.m - xm.m);
+        const double uy = dyl * (yp.u - ym.u), vy = dyl * (yp.v - ym.v), wy = dyl * (yp.w - ym.w);
+        const double ty = dyl * (yp.t - ym.t), muy = dyl * (yp.m - ym.m);
+        const double uz = dzl * (zp.u - zm.u), vz = dzl * (zp.v - zm.v), wz = dzl * (zp.w - zm.w);
+        const double muz = dzl * (zp.m - zm.m);
+        const double mu = p0.m;
+        if (j == 1) {
+            const size_t wsz = (size_t)g.sx * g.sz;
+            const size_t wp = (size_t)(i + ng - 1) + (size_t)g.sx * (k + ng - 1);
+            wallprop[wp] = mu * uy;
+            wallprop[wsz + wp] = mu * wy;
+            wallprop[2 * wsz + wp] = mu * ty * cploc / P.prandtl;
+        }
+        const double div = ux + vy + wz;
+        const double div3l = div / 3.0;
+        wa[9 * nv + c] = div3l;
+        if (images) write_images(g, P, wa + 9 * nv, i, j, k, div3l, true);
+        const double omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
+        const double omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
+        wa[8 * nv + c] = sqrt(omod2);
+        if (SENSOR) {
+            const double u0l0 = P.u0 / P.l0;
+            double sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
+            sv = fmax(sv, 0.0);
+            wa[7 * nv + c] = sv * sv;
+        }
+        const double sig11 = ux - 2.0 * div3l, sig12 = vx, sig13 = wx;
+        const double sig21 = uy, sig22 = vy - 2.0 * div3l, sig23 = wy;
+        const double sig31 = uz, sig32 = vz, sig33 = wz - 2.0 * div3l;
+        const double sigx = mux * sig11 + muy * sig12 + muz * sig13;
+        const double sigy = mux * sig21 + muy * sig22 + muz * sig23;
+        const double sigz = mux * sig31 + muy * sig32 + muz * sig33;
+        const double sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig21 * vx + sig22 * vy + sig23 * vz +
+                              sig31 * wx + sig32 * wy + sig33 * wz) * mu;
+        const double sigq = sigx * p0.u + sigy * p0.v + sigz * p0.w + sigah;
+        f2 = f2 - sigx;
+        f3 = f3 - sigy;
+        f4 = f4 - sigz;
+        f5 = f5 - sigq;
+    }
+    if (FACES) {
+        double sl[4], sr[4];
+        // x: fhat = -flux, fl += dxhl*(fhat(i) - fhat(i-1))
+        cons_face(P, xm, p0, xi - xi_m, sl);
+        cons_face(P, p0, xp, xi_p - xi, sr);
+        const double dxhl = 2.0 / (xi_p - xi_m);
+        f2 = f2 + dxhl * (-sr[0] - -sl[0]);
+        f3 = f3 + dxhl * (-sr[1] - -sl[1]);
+        f4 = f4 + dxhl * (-sr[2] - -sl[2]);
+        f5 = f5 + dxhl * (-sr[3] - -sl[3]);
+        // y: fl = fl + flux(j-1/2)*dyhl - flux(j+1/2)*dyhl
+        cons_face(P, ym, p0, yj - yj_m, sl);
+        cons_face(P, p0, yp, yj_p - yj, sr);
+        const double dyhl = 2.0 / (yj_p - yj_m);
+        f2 = f2 + sl[0] * dyhl - sr[0] * dyhl;
+        f3 = f3 + sl[1] * dyhl - sr[1] * dyhl;
+        f4 = f4 + sl[2] * dyhl - sr[2] * dyhl;
+        f5 = f5 + sl[3] * dyhl - sr[3] * dyhl;
+        cons_face(P, zm, p0, zk - zk_m, sl);
+        cons_face(P, p0, zp, zk_p - zk, sr);
+        const double dzhl = 2.0 / (zk_p - zk_m);
+        f2 = f2 + sl[0] * dzhl - sr[0] * dzhl;
+        f3 = f3 + sl[1] * dzhl - sr[1] * dzhl;
+        f4 = f4 + sl[2] * dzhl - sr[2] * dzhl;
+        f5 = f5 + sl[3] * dzhl - sr[3] * dzhl;
+    }
+    fl[ni + oi] = f2;
+    fl[2 * ni + oi] = f3;
+    fl[3 * ni + oi] = f4;
+    fl[4 * ni + oi] = f5;
+}
+
 // sensor_subroutine (kernels_cpu.F90:4726-4744) + bcswap_var of w_aux(8)
 __global__ __launch_bounds__(256) void k_sensor(Geo g, KPar P, double *__restrict__ wa, int images) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -899,6 +1040,64 @@ __global__ __launch_bounds__(256) void k_visc_div(Geo g, KPar P, const double *_
         dyv = dyv * dy1[j - 1];
         dzv = dzv * dz1[k - 1];
         const double sigx = mu * dxv, sigy = mu * dyv, sigz = mu * dzv;
+        const double sigq = sigx * uu + sigy * vv + sigz * ww;
+        const int oi = iidx(g, i, j, k);
+        const size_t ni = g.nint;
+        const double f2 = fl[ni + oi] - sigx;
+        fl[ni + oi] = f2;
+        fl[2 * ni + oi] -= sigy;
+        fl[3 * ni + oi] -= sigz;
+        fl[4 * ni + oi] -= sigq;
+        if (REDUCE) {
+            const double dt = ds->dt;
+            const double rhodt = rkr * dt, gamdt = rkg * dt;
+            const double dy = yn[j] - yn[j - 1];
+            const double fln1 = -rhodt * flp[oi] - gamdt * fl[oi];
+            const double fln2 = -rhodt * flp[ni + oi] - gamdt * f2;
+            const double w1 = w[c], w2 = w[nv + c];
+            acc[0] = fln1 * dy;
+            acc[1] = fln2 * dy;
+            acc[2] = w1 * dy;
+            acc[3] = w2 * dy;
+            acc[4] = w2 * dy * wa[5 * nv + c];
+        }
+    }
+    if (REDUCE) {
+        block_sum<5>(acc, lds);
+        if (flat_thread() == 0) {
+            double *pp = partials + (size_t)flat_block() * 5;
+#pragma unroll
+            for (int q = 0; q < 5; q++) pp[q] = acc[q];
+        }
+    }
+}
+
+// visflx_div_ord2_subroutine (kernels_cpu.F90:3446-3480, visflx mode 7);
+// REDUCE as in k_visc_div.
+template <bool REDUCE>
+__global__ __launch_bounds__(256) void k_visc_div2(Geo g, KPar P, const double *__restrict__ wa, double *__restrict__ fl,
+                                                   const double *__restrict__ flp, const double *__restrict__ w,
+                                                   const double *__restrict__ xc, const double *__restrict__ yc,
+                                                   const double *__restrict__ zc, const double *__restrict__ yn,
+                                                   const DevScalars *__restrict__ ds, double rkr, double rkg,
+                                                   double *__restrict__ partials) {
+    __shared__ double lds[4][5];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i <= g.nx && j <= g.ny) {
+        const size_t nv = g.nvol;
+        const int c = gidx(g, i, j, k);
+        const double *D = wa + 9 * nv;
+        const double uu = wa[nv + c], vv = wa[2 * nv + c], ww = wa[3 * nv + c], mu = wa[6 * nv + c];
+        const int sy = g.sx, sz = g.sxy, ng = g.ng;
+        const double dxl = mu / (xc[i + 1 + ng - 1] - xc[i - 1 + ng - 1]);
+        const double dyl = mu / (yc[j + 1 + ng - 1] - yc[j - 1 + ng - 1]);
+        const double dzl = mu / (zc[k + 1 + ng - 1] - zc[k - 1 + ng - 1]);
+        const double sigx = dxl * (D[c + 1] - D[c - 1]);
+        const double sigy = dyl * (D[c + sy] - D[c - sy]);
+        const double sigz = dzl * (D[c + sz] - D[c - sz]);
         const double sigq = sigx * uu + sigy * vv + sigz * ww;
         const int oi = iidx(g, i, j, k);
         const size_t ni = g.nint;
@@ -1397,14 +1596,41 @@ hipError_t launch_visc(st_solver *s, double *fl, int mode, bool ghost_images) {
     return hipGetLastError();
 }
 
+// Conservative viscous form: mode 2 (cell terms + sensor), 6 (cell terms),
+// 5 (face terms) and 56 (6 then 5 in one pass, the rk_sync order).
+hipError_t launch_visc_cons(st_solver *s, double *fl, int mode, bool ghost_images) {
+    const Geo &g = s->g;
+#define ST_LAUNCH_VC(C_, F_, S_)                                                                                     \
+    hipLaunchKernelGGL((k_visc_cons<C_, F_, S_>), cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, s->x, \
+                       s->y, s->z, s->wallprop, ghost_images ? 1 : 0)
+    switch (mode) {
+    case 2: ST_LAUNCH_VC(true, false, true); break;
+    case 5: ST_LAUNCH_VC(false, true, false); break;
+    case 6: ST_LAUNCH_VC(true, false, false); break;
+    case 56: ST_LAUNCH_VC(true, true, false); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef ST_LAUNCH_VC
+    return hipGetLastError();
+}
+
 hipError_t launch_sensor(st_solver *s, bool ghost_images) {
     hipLaunchKernelGGL(k_sensor, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, s->w_aux, ghost_images ? 1 : 0);
     return hipGetLastError();
 }
 
-hipError_t launch_visc_div(st_solver *s, double *fl, const double *fl_prev, int stage, bool reduce) {
+hipError_t launch_visc_div(st_solver *s, double *fl, const double *fl_prev, int stage, bool reduce, bool ord2) {
     const Geo &g = s->g;
     const double rkr = stage >= 0 ? s->rhork[stage] : 0.0, rkg = stage >= 0 ? s->gamrk[stage] : 0.0;
+    if (ord2) {
+        if (reduce)
+            hipLaunchKernelGGL(k_visc_div2<true>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl,
+                               fl_prev, s->w, s->x, s->y, s->z, s->yn, s->ds, rkr, rkg, s->partials);
+        else
+            hipLaunchKernelGGL(k_visc_div2<false>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl,
+                               fl_prev, s->w, s->x, s->y, s->z, s->yn, s->ds, rkr, rkg, s->partials);
+        return hipGetLastError();
+    }
     if (reduce)
         hipLaunchKernelGGL(k_visc_div<true>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, fl_prev,
                            s->w, s->dcsidx, s->detady, s->dzitdz, s->yn, s->ds, rkr, rkg, s->partials);

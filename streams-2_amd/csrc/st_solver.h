@@ -55,8 +55,10 @@ hipError_t launch_euler_x(st_solver *s, double *fl, bool init);
 hipError_t launch_euler_y(st_solver *s, double *fl);
 hipError_t launch_euler_z(st_solver *s, double *fl);
 hipError_t launch_visc(st_solver *s, double *fl, int mode, bool ghost_images);
+hipError_t launch_visc_cons(st_solver *s, double *fl, int mode, bool ghost_images);
 hipError_t launch_sensor(st_solver *s, bool ghost_images);
-hipError_t launch_visc_div(st_solver *s, double *fl, const double *fl_prev, int stage, bool reduce);
+// visflx mode 1, or mode 7 (second order, conservative viscous form) with ord2
+hipError_t launch_visc_div(st_solver *s, double *fl, const double *fl_prev, int stage, bool reduce, bool ord2);
 hipError_t launch_force_finish(st_solver *s, int stage);
 hipError_t launch_update(st_solver *s, const double *fl_prev, const double *fl, int stage);
 hipError_t launch_force_var(st_solver *s);

@@ -35,7 +35,7 @@ class StParams(C.Structure):
         ("bctags", C.c_int32 * 6), ("bctags_nr", C.c_int32 * 6),
         ("is_y_staggered", C.c_int32), ("channel_case", C.c_int32),
         ("correct_bound_ord", C.c_int32), ("iter_dt_recompute", C.c_int32),
-        ("precision", C.c_int32), ("pad0", C.c_int32),
+        ("precision", C.c_int32), ("pad0", C.c_int32), ("pad1", C.c_int32),
         ("rgas0", C.c_double), ("prandtl", C.c_double), ("t0", C.c_double), ("rho0", C.c_double),
         ("u0", C.c_double), ("l0", C.c_double), ("mu0", C.c_double),
         ("sutherland_s", C.c_double), ("t_ref_dim", C.c_double), ("powerlaw_vtexp", C.c_double),
@@ -76,6 +76,7 @@ class FlowCase:
     gam: float = 1.4
     prandtl: float = 0.72
     visc_model: int = 1              # 1 power law, 2 Sutherland
+    conservative_viscous: int = 0    # 1: second-order conservative viscous form
     vt_exp: float = 0.76
     s_suth: float = 110.4
     t_ref: float = 160.0
@@ -242,7 +243,7 @@ def channel_params(case: FlowCase, yn: np.ndarray, nxmax=None, nzmax=None) -> St
     p.weno_scheme = case.weno_scheme
     p.weno_version = case.weno_version
     p.flux_splitting = 0
-    p.conservative_viscous = 0
+    p.conservative_viscous = case.conservative_viscous
     p.visc_model = case.visc_model
     p.calorically_perfect = 1
     p.indx_cp_l, p.indx_cp_r = icl, icr

@@ -145,3 +145,22 @@ class ChannelSolver:
 
     def synchronize(self):
         capi.check(self.L.st_synchronize(self.h))
+
+
+PROGRESS_COLUMNS = ("icyc", "dt", "time", "residual_rhou", "-dpdx/dt", "rhobulk", "ubulk", "tbulk",
+                    "rhomin", "rhomax", "tmin", "tmax", "pmin", "pmax")
+
+
+def progress_row(s: dict, p: StParams) -> list:
+    """The channel row of progress.out (print_progress,
+    singleideal_cpu.F90:1695-1732, grid_dim 1): residual and dpdx scaled by
+    l0 / (rho0 u0^2), dpdx divided by dt."""
+    res = s["residual_rhou"] / (p.u0 ** 2) / p.rho0 * p.l0
+    dpdx = s["dpdx"] / (p.u0 ** 2) / p.rho0 * p.l0
+    return [s["icyc"], s["dt"], s["time"], res, -dpdx / s["dt"], s["rhobulk"], s["ubulk"], s["tbulk"],
+            s["rhomin"], s["rhomax"], s["tmin"], s["tmax"], s["pmin"], s["pmax"]]
+
+
+def progress_text(row: list) -> list:
+    """Columns as the reference prints them: '(1i10,20es20.10)'."""
+    return [str(int(row[0]))] + ["%.10E" % v for v in row[1:]]

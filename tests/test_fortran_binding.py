@@ -1,0 +1,57 @@
+"""The Fortran iso_c_binding path: record layout (CPU) and a full run of the
+Fortran driver through the C ABI compared with the oracle (GPU)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import interior, rel_l2
+from streams_mi355x.fortran_case import DRIVER, read_result, run_driver, write_case
+from streams_mi355x.params import StParams, StScalars
+
+
+def test_fortran_records_match_the_c_header():
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (make -C streams-2_amd/fortran)")
+    out = subprocess.run([DRIVER, "--sizes"], capture_output=True, text=True, timeout=60).stdout.split()
+    assert int(out[0]) == C.sizeof(StParams)
+    assert int(out[1]) == C.sizeof(StScalars)
+
+
+def test_fortran_reads_the_case_file(tmp_path):
+    """flang transfers a bind(c) record component by component, so any
+    implicit padding in st_params would shift every later read."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (make -C streams-2_amd/fortran)")
+    from streams_mi355x import make_channel
+
+    c = make_channel(nx=32, ny=24, nz=16)
+    case = tmp_path / "case.bin"
+    write_case(str(case), c.params, c.grid, c.w0, 7)
+    out = subprocess.run([DRIVER, "--read", str(case)], capture_output=True, text=True, timeout=60).stdout.split()
+    assert [int(v) for v in out[:5]] == [32, 24, 16, 3, 7]
+    assert float(out[5]) == c.params.cfl
+    assert abs(float(out[6]) - c.grid.yn.sum()) <= 1e-12
+    assert abs(float(out[7]) - c.w0.sum()) <= 1e-12 * abs(c.w0.sum())
+
+
+@pytest.mark.gpu
+def test_fortran_driver_matches_oracle(tmp_path):
+    from oracle import Oracle
+    from streams_mi355x import make_channel
+
+    c = make_channel(nx=32, ny=24, nz=16, sensor_threshold=0.0)
+    case = tmp_path / "case.bin"
+    outp = tmp_path / "out.bin"
+    write_case(str(case), c.params, c.grid, c.w0, 10)
+    log = run_driver(str(case), str(outp))
+    w, s = read_result(str(outp), c.w0.shape)
+    o = Oracle(c.params, c.grid)
+    o.w[:] = c.w0
+    o.init_run()
+    o.advance(10)
+    for m in range(5):
+        assert rel_l2(interior(w)[m], interior(o.w)[m]) <= 1e-10, (m, log)
+    assert s["icyc"] == 10

@@ -28,9 +28,15 @@ def _pair(nx=32, ny=24, nz=16, threshold=0.9, seed=0, **kw):
     return c, o, g
 
 
-def _load_state(c, o, g, sensor_seed=None):
-    """Same w (ghosts filled everywhere) and, optionally, a random sensor."""
-    w = fill_ghosts_wrap(c.w0)
+def _load_state(c, o, g, sensor_seed=None, energy_noise=0.0):
+    """Same w (ghosts filled everywhere) and, optionally, a random sensor.
+    ``energy_noise`` perturbs the total energy (hence T and mu) by a seeded
+    relative amount, so that viscosity gradients are not zero."""
+    w0 = c.w0.copy()
+    if energy_noise:
+        rng = np.random.default_rng(7)
+        w0[4] *= 1.0 + energy_noise * rng.uniform(-1.0, 1.0, size=w0[4].shape)
+    w = fill_ghosts_wrap(w0)
     o.w[:] = w
     g.copy_from_field(w)
     o.compute_aux(1, 1)
@@ -206,3 +212,71 @@ def test_bitwise_reproducible():
         outs.append(g.copy_to_field())
         g.close()
     assert np.array_equal(interior(outs[0]), interior(outs[1]))
+
+
+# ------------------------------------------------ conservative viscous form
+@pytest.mark.parametrize("mode", [6, 5, 2])
+def test_visflx_conservative_parity(mode):
+    c, o, g = _pair(conservative_viscous=1, visc_model=2)
+    _load_state(c, o, g, energy_noise=0.05)
+    o.zero_flux()
+    g.zero_flux()
+    o.visflx(mode)
+    g.visflx(mode)
+    fo, fg = o.fl, g.get_fl()
+    for m in range(1, 5):
+        assert rel_l2(fg[m], fo[m]) <= UNIT_TOL, (mode, m, rel_l2(fg[m], fo[m]))
+    if mode != 5:
+        for v in ([8, 9, 10] if mode == 2 else [9, 10]):
+            a = interior(g.get_aux(v)[None])[0]
+            b = interior(o.w_aux[v - 1][None])[0]
+            assert rel_l2(a, b) <= UNIT_TOL, (mode, v)
+
+
+def test_visflx_div_ord2_parity():
+    c, o, g = _pair(conservative_viscous=1, visc_model=2)
+    _load_state(c, o, g, energy_noise=0.05)
+    o.zero_flux()
+    o.visflx(6)
+    o.bcextr_var(10)
+    o.bcswap_var(10)
+    g.set_aux(10, o.w_aux[9])
+    o.zero_flux()
+    g.zero_flux()
+    o.visflx(7)
+    g.visflx(7)
+    fo, fg = o.fl, g.get_fl()
+    for m in range(1, 5):
+        assert rel_l2(fg[m], fo[m]) <= UNIT_TOL, m
+
+
+@pytest.mark.parametrize("threshold", [0.9, -1.0])
+def test_trajectory_conservative_viscous(threshold):
+    c, o, g = _run_pair(20, threshold=threshold, conservative_viscous=1, visc_model=2)
+    wg = g.copy_to_field()
+    for m in range(5):
+        err = rel_l2(interior(wg)[m], interior(o.w)[m])
+        assert err <= TRAJ_TOL, (threshold, m, err)
+    so, sg = o.scalars(), g.scalars()
+    for key in ("dt", "time", "rhobulk", "ubulk", "tbulk"):
+        assert abs(sg[key] - so[key]) <= 1e-12 * abs(so[key]), key
+
+
+def test_trajectory_subsonic_channel_100_steps():
+    """examples/subsonic_channel at 80x60x80 (the golden-row deck) over 100
+    steps: conservative viscous form, Sutherland viscosity, rank-seeded init."""
+    from streams_mi355x import make_subsonic_channel
+
+    c = make_subsonic_channel()
+    o = Oracle(c.params, c.grid)
+    g = ChannelSolver(c.params, c.grid)
+    o.w[:] = c.w0
+    g.copy_from_field(c.w0)
+    o.init_run()
+    g.init_run()
+    o.advance(100)
+    g.advance(100)
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    print("subsonic channel 100-step rel L2 per variable:", errs)
+    assert max(errs) <= TRAJ_TOL, errs

@@ -362,6 +362,132 @@ def test_viscous_terms_on_manufactured_polynomials():
     del sh
 
 
+def _cons_fields(o, co, U, V, W, T, M):
+    X, Y, Z = np.meshgrid(co[0], co[1], co[2], indexing="ij")
+    X, Y, Z = (a.transpose(2, 1, 0) for a in (X, Y, Z))
+    o.w_aux[:] = 0.0
+    o.w_aux[0] = 1.0
+    o.w_aux[1], o.w_aux[2], o.w_aux[3] = U(X, Y, Z), V(X, Y, Z), W(X, Y, Z)
+    o.w_aux[5], o.w_aux[6] = T(X, Y, Z), M(X, Y, Z)
+    return X, Y, Z
+
+
+def _run_conservative(o, div3, X, Y, Z):
+    """visflx(6) + visflx(5), exact ghosts of div/3, visflx(7) (rk_sync order)."""
+    o.zero_flux()
+    o.visflx(6)
+    o.visflx(5)
+    inner = (slice(3, -3),) * 3
+    d_in = o.w_aux[9][inner].copy()
+    o.w_aux[9] = div3(X, Y, Z)
+    o.visflx(7)
+    return d_in
+
+
+def _rand_poly(rng, shape, const):
+    c = rng.uniform(-0.3, 0.3, shape)
+    c[(0,) * 3] = const
+    return Poly3(c)
+
+
+def test_conservative_viscous_momentum_on_manufactured_fields():
+    """visflx(6)+(5)+(7) (conservative_viscous=1) equal minus the divergence
+    of the viscous stress for velocities quadratic and viscosity linear in
+    each coordinate, where every second-order face and node difference of
+    the form is exact; div/3 of visflx(6) equals the exact divergence/3."""
+    p, g, _, co = box_case(nx=9, ny=8, nz=7, L=(1.0, 0.9, 0.8), mu0=1.0)
+    p.conservative_viscous = 1
+    o = Oracle(p, g)
+    rng = np.random.default_rng(11)
+    U, V, W = (_rand_poly(rng, (3, 3, 3), c) for c in (0.2, -0.1, 0.3))
+    T = _rand_poly(rng, (2, 2, 2), 1.4)
+    M = _rand_poly(rng, (2, 2, 2), 1.2)
+    X, Y, Z = _cons_fields(o, co, U, V, W, T, M)
+    vel = [U, V, W]
+    grad = [[f.d(a) for a in range(3)] for f in vel]
+    div = grad[0][0] + grad[1][1] + grad[2][2]
+    div3 = div * (1.0 / 3.0)
+    d_in = _run_conservative(o, div3, X, Y, Z)
+    inner = (slice(3, -3),) * 3
+    Xi, Yi, Zi = X[inner], Y[inner], Z[inner]
+    assert np.allclose(d_in, div3(Xi, Yi, Zi), rtol=1e-10, atol=1e-12)
+    om = [grad[2][1] - grad[1][2], grad[0][2] - grad[2][0], grad[1][0] - grad[0][1]]
+    om_mod = np.sqrt(sum(f(Xi, Yi, Zi) ** 2 for f in om))
+    assert np.allclose(o.w_aux[8][inner], om_mod, rtol=1e-10, atol=1e-12)
+    tau = [[M * (grad[i][j] + grad[j][i]) for j in range(3)] for i in range(3)]
+    for i in range(3):
+        tau[i][i] = tau[i][i] - M * div * (2.0 / 3.0)
+    for i in range(3):
+        ref = -(tau[i][0].d(0) + tau[i][1].d(1) + tau[i][2].d(2))(Xi, Yi, Zi)
+        assert np.allclose(o.fl[1 + i], ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max()), i
+    assert np.all(o.fl[0] == 0.0)
+
+
+def test_conservative_viscous_energy_on_manufactured_fields():
+    """Energy row of the conservative form: minus the divergence of
+    u.tau + k grad T, exact for velocities and viscosity linear and the
+    temperature quadratic in each coordinate (|u|^2/2 then is quadratic)."""
+    p, g, _, co = box_case(nx=9, ny=8, nz=7, L=(1.0, 0.9, 0.8), mu0=1.0)
+    p.conservative_viscous = 1
+    o = Oracle(p, g)
+    rng = np.random.default_rng(12)
+    U, V, W = (_rand_poly(rng, (2, 2, 2), c) for c in (0.2, -0.1, 0.3))
+    T = _rand_poly(rng, (3, 3, 3), 1.4)
+    M = _rand_poly(rng, (2, 2, 2), 1.2)
+    X, Y, Z = _cons_fields(o, co, U, V, W, T, M)
+    vel = [U, V, W]
+    grad = [[f.d(a) for a in range(3)] for f in vel]
+    div = grad[0][0] + grad[1][1] + grad[2][2]
+    _run_conservative(o, div * (1.0 / 3.0), X, Y, Z)
+    tau = [[M * (grad[i][j] + grad[j][i]) for j in range(3)] for i in range(3)]
+    for i in range(3):
+        tau[i][i] = tau[i][i] - M * div * (2.0 / 3.0)
+    kcond = M * (p.cp_coeff[0] / p.prandtl)
+    energy = Poly3(np.zeros((1, 1, 1)))
+    for j in range(3):
+        energy = energy + (vel[0] * tau[0][j] + vel[1] * tau[1][j] + vel[2] * tau[2][j] + kcond * T.d(j)).d(j)
+    inner = (slice(3, -3),) * 3
+    ref5 = -energy(X[inner], Y[inner], Z[inner])
+    assert np.allclose(o.fl[4], ref5, rtol=1e-9, atol=1e-9 * np.abs(ref5).max())
+
+
+def test_conservative_viscous_sensor_and_wall_shear():
+    """visflx(2) stores the Ducros sensor of the exact divergence and
+    vorticity, adds the same cell terms as visflx(6), and records
+    mu du/dy, mu dw/dy, k dT/dy of the first cell row in wallprop."""
+    p, g, _, co = box_case(nx=9, ny=8, nz=7, L=(1.0, 0.9, 0.8), mu0=1.0)
+    p.conservative_viscous = 1
+    o = Oracle(p, g)
+    rng = np.random.default_rng(13)
+    U, V, W = (_rand_poly(rng, (3, 3, 3), c) for c in (0.2, -0.1, 0.3))
+    T = _rand_poly(rng, (3, 3, 3), 1.4)
+    M = _rand_poly(rng, (2, 2, 2), 1.2)
+    X, Y, Z = _cons_fields(o, co, U, V, W, T, M)
+    o.zero_flux()
+    o.visflx(6)
+    fl6 = o.fl.copy()
+    o.zero_flux()
+    o.visflx(2)
+    assert np.array_equal(o.fl, fl6)
+    grad = [[f.d(a) for a in range(3)] for f in (U, V, W)]
+    inner = (slice(3, -3),) * 3
+    Xi, Yi, Zi = X[inner], Y[inner], Z[inner]
+    dv = (grad[0][0] + grad[1][1] + grad[2][2])(Xi, Yi, Zi)
+    om = [grad[2][1] - grad[1][2], grad[0][2] - grad[2][0], grad[1][0] - grad[0][1]]
+    om2 = sum(f(Xi, Yi, Zi) ** 2 for f in om)
+    sens = np.maximum(-dv / np.sqrt(om2 + dv ** 2 + (p.u0 / p.l0) ** 2), 0.0) ** 2
+    assert np.allclose(o.w_aux[7][inner], sens, rtol=1e-9, atol=1e-14)
+    wp = np.ctypeslib.as_array(o.L.or_wallprop(o.h), shape=(3 * (p.nz + 6) * (p.nx + 6),))
+    wp = wp.reshape(3, p.nz + 6, p.nx + 6)[:, 3:-3, 3:-3]
+    row = (slice(3, -3), 3, slice(3, -3))
+    mu = M(X[row], Y[row], Z[row])
+    assert np.allclose(wp[0], mu * grad[0][1](X[row], Y[row], Z[row]), rtol=1e-10, atol=1e-13)
+    assert np.allclose(wp[1], mu * grad[2][1](X[row], Y[row], Z[row]), rtol=1e-10, atol=1e-13)
+    ty = T.d(1)(X[row], Y[row], Z[row])
+    # T is quadratic in y: the node difference carries the exact derivative
+    assert np.allclose(wp[2], mu * ty * p.cp_coeff[0] / p.prandtl, rtol=1e-10, atol=1e-13)
+
+
 # ------------------------------------------------------- reductions / ghosts
 def test_compute_dt_brute_force():
     c = make_channel(nx=16, ny=12, nz=8)
