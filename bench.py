@@ -43,6 +43,21 @@ ALGO_BYTES = {
 }
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of the same workload (scaled by the copy
+    probe's factors), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != workload:
+        return None
+    return d.get("bytes_per_launch", {}).get(kernel)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,7 +179,8 @@ def main():
                    "grid": [nx, ny, nz], "precision": "fp64", "finite": bool(finite),
                    "weno_face_fraction_xyz": weno_frac},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"channel_{nx}x{ny}x{nz}_fp64", dom),
                      "algorithmic_bytes_per_launch": per * units, "avg_launch_ms": ms},
         "kernels": per_kernel,
     }

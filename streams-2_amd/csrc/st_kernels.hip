@@ -714,139 +714,7 @@ __device__ __forceinline__ void write_images(const Geo &g, const KPar &P, double
     }
 }
 
-// visflx_subroutine (mode 0, kernels_cpu.F90:3598-3777) and
-// visflx_nosensor_subroutine (mode 4, :3996-4180).
-template <bool SENSOR>
-__global__ __launch_bounds__(256) void k_visc(Geo g, KPar P, double *__restrict__ wa, double *__restrict__ fl,
-                                              const double *__restrict__ dx1, const double *__restrict__ dxs,
-                                              const double *__restrict__ dx2, const double *__restrict__ dy1,
-                                              const double *__restrict__ dys, const double *__restrict__ dy2,
-                                              const double *__restrict__ dz1, const double *__restrict__ dzs,
-                                              const double *__restrict__ dz2, double *__restrict__ wallprop,
-                                              int images) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.nx || j > g.ny) return;
-    const size_t nv = g.nvol;
-    const int c = gidx(g, i, j, k);
-    const double *U = wa + nv, *V = wa + 2 * nv, *W = wa + 3 * nv, *T = wa + 5 * nv, *M = wa + 6 * nv;
-    const double uu = U[c], vv = V[c], ww = W[c], tt = T[c], mu = M[c];
-    const int lmax = P.visc_lmax;
-    double ux = 0, vx = 0, wx = 0, tx = 0, mux = 0;
-    double uy = 0, vy = 0, wy = 0, ty = 0, muy = 0;
-    double uz = 0, vz = 0, wz = 0, tz = 0, muz = 0;
-    const double c20 = P.vd2[0];
-    double ulapx = c20 * uu, ulapy = ulapx, ulapz = ulapx;
-    double vlapx = c20 * vv, vlapy = vlapx, vlapz = vlapx;
-    double wlapx = c20 * ww, wlapy = wlapx, wlapz = wlapx;
-    double tlapx = c20 * tt, tlapy = tlapx, tlapz = tlapx;
-    const int sy = g.sx, sz = g.sxy;
-#pragma unroll
-    for (int l = 1; l <= 4; l++) {
-        if (l <= lmax) {
-            const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
-            const int p = c + l, m = c - l;
-            ux = ux + ccl * (U[p] - U[m]);
-            ulapx = ulapx + clapl * (U[p] + U[m]);
-            vx = vx + ccl * (V[p] - V[m]);
-            vlapx = vlapx + clapl * (V[p] + V[m]);
-            wx = wx + ccl * (W[p] - W[m]);
-            wlapx = wlapx + clapl * (W[p] + W[m]);
-            tx = tx + ccl * (T[p] - T[m]);
-            tlapx = tlapx + clapl * (T[p] + T[m]);
-            mux = mux + ccl * (M[p] - M[m]);
-        }
-    }
-#pragma unroll
-    for (int l = 1; l <= 4; l++) {
-        if (l <= lmax) {
-            const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
-            const int p = c + l * sy, m = c - l * sy;
-            uy = uy + ccl * (U[p] - U[m]);
-            ulapy = ulapy + clapl * (U[p] + U[m]);
-            vy = vy + ccl * (V[p] - V[m]);
-            vlapy = vlapy + clapl * (V[p] + V[m]);
-            wy = wy + ccl * (W[p] - W[m]);
-            wlapy = wlapy + clapl * (W[p] + W[m]);
-            ty = ty + ccl * (T[p] - T[m]);
-            tlapy = tlapy + clapl * (T[p] + T[m]);
-            muy = muy + ccl * (M[p] - M[m]);
-        }
-    }
-#pragma unroll
-    for (int l = 1; l <= 4; l++) {
-        if (l <= lmax) {
-            const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
-            const int p = c + l * sz, m = c - l * sz;
-            uz = uz + ccl * (U[p] - U[m]);
-            ulapz = ulapz + clapl * (U[p] + U[m]);
-            vz = vz + ccl * (V[p] - V[m]);
-            vlapz = vlapz + clapl * (V[p] + V[m]);
-            wz = wz + ccl * (W[p] - W[m]);
-            wlapz = wlapz + clapl * (W[p] + W[m]);
-            tz = tz + ccl * (T[p] - T[m]);
-            tlapz = tlapz + clapl * (T[p] + T[m]);
-            muz = muz + ccl * (M[p] - M[m]);
-        }
-    }
-    const double ax = dx1[i - 1], ay = dy1[j - 1], az = dz1[k - 1];
-    ux = ux * ax; vx = vx * ax; wx = wx * ax; tx = tx * ax; mux = mux * ax;
-    uy = uy * ay; vy = vy * ay; wy = wy * ay; ty = ty * ay; muy = muy * ay;
-    uz = uz * az; vz = vz * az; wz = wz * az; tz = tz * az; muz = muz * az;
-    const double sx_ = dxs[i - 1], sy_ = dys[j - 1], sz_ = dzs[k - 1];
-    const double qx = dx2[i - 1], qy = dy2[j - 1], qz = dz2[k - 1];
-    ulapx = ulapx * sx_ + ux * qx; vlapx = vlapx * sx_ + vx * qx;
-    wlapx = wlapx * sx_ + wx * qx; tlapx = tlapx * sx_ + tx * qx;
-    ulapy = ulapy * sy_ + uy * qy; vlapy = vlapy * sy_ + vy * qy;
-    wlapy = wlapy * sy_ + wy * qy; tlapy = tlapy * sy_ + ty * qy;
-    ulapz = ulapz * sz_ + uz * qz; vlapz = vlapz * sz_ + vz * qz;
-    wlapz = wlapz * sz_ + wz * qz; tlapz = tlapz * sz_ + tz * qz;
-    const double ulap = ulapx + ulapy + ulapz;
-    const double vlap = vlapx + vlapy + vlapz;
-    const double wlap = wlapx + wlapy + wlapz;
-    const double tlap = tlapx + tlapy + tlapz;
-    const double div = ux + vy + wz;
-    const double div3l = div / 3.0;
-    wa[9 * nv + c] = div3l;
-    const double omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
-    const double omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
-    wa[8 * nv + c] = sqrt(omod2);
-    if (SENSOR) {
-        const double u0l0 = P.u0 / P.l0;
-        double sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
-        sv = fmax(sv, 0.0);
-        wa[7 * nv + c] = sv * sv;
-        if (images) write_images(g, P, wa + 7 * nv, i, j, k, sv * sv, false);
-    }
-    if (images && !SENSOR) write_images(g, P, wa + 9 * nv, i, j, k, div3l, true);
-    const double cploc = cploc_of(P, tt);
-    const double sig11 = 2.0 * (ux - div3l), sig12 = uy + vx, sig13 = uz + wx;
-    const double sig22 = 2.0 * (vy - div3l), sig23 = vz + wy, sig33 = 2.0 * (wz - div3l);
-    const double sigx = mux * sig11 + muy * sig12 + muz * sig13 + mu * ulap;
-    const double sigy = mux * sig12 + muy * sig22 + muz * sig23 + mu * vlap;
-    const double sigz = mux * sig13 + muy * sig23 + muz * sig33 + mu * wlap;
-    const double sigqt = (mux * tx + muy * ty + muz * tz + mu * tlap) * cploc / P.prandtl;
-    const double sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig12 * vx + sig22 * vy + sig23 * vz +
-                          sig13 * wx + sig23 * wy + sig33 * wz) * mu;
-    const double sigq = sigx * uu + sigy * vv + sigz * ww + sigah + sigqt;
-    const int oi = iidx(g, i, j, k);
-    const size_t ni = g.nint;
-    fl[ni + oi] -= sigx;
-    fl[2 * ni + oi] -= sigy;
-    fl[3 * ni + oi] -= sigz;
-    fl[4 * ni + oi] -= sigq;
-    if (j == 1) {
-        const size_t wsz = (size_t)g.sx * g.sz;
-        const size_t wp = (size_t)(i + g.ng - 1) + (size_t)g.sx * (k + g.ng - 1);
-        wallprop[wp] = mu * uy;
-        wallprop[wsz + wp] = mu * wy;
-        wallprop[2 * wsz + wp] = mu * ty * cploc / P.prandtl;
-    }
-}
-
-// ------------------------------------------------ conservative viscous form
-// Point values the second-order conservative terms need: u, v, w, T, mu.
+// Point values of the viscous stencils: u, v, w, T, mu (w_aux 2, 3, 4, 6, 7).
 struct VP {
     double u, v, w, t, m;
 };
@@ -860,6 +728,201 @@ __device__ __forceinline__ VP load_vp(const double *__restrict__ wa, size_t nv, 
     return p;
 }
 
+// visflx_subroutine (mode 0, kernels_cpu.F90:3598-3777, SENSOR) and
+// visflx_nosensor_subroutine (mode 4, :3996-4180), z-marching: a VBX x VBY
+// column tile walks a chunk of planes.  The z stencil of u, v, w, T, mu
+// lives in a 7-plane register window (each plane is read from HBM once per
+// chunk); the x and y stencils come from an LDS copy of the current plane
+// with a 3-point halo.  Loads of plane k+1 (centre, halo, and the fl of
+// plane k) are issued before the work on plane k, and the LDS tile is
+// double-buffered, so one barrier per plane suffices.  ``images`` also
+// writes the ghost images of w_aux(10) (or of the sensor with SENSOR).
+constexpr int VBX = 32, VBY = 8;
+// each lane fills at most one x-halo and one y-halo point
+static_assert(VBX >= 2 * H && VBY >= 2 * H, "tile narrower than two halos");
+
+template <bool SENSOR>
+__global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__restrict__ wa, double *__restrict__ fl,
+                                                      const double *__restrict__ dx1, const double *__restrict__ dxs,
+                                                      const double *__restrict__ dx2, const double *__restrict__ dy1,
+                                                      const double *__restrict__ dys, const double *__restrict__ dy2,
+                                                      const double *__restrict__ dz1, const double *__restrict__ dzs,
+                                                      const double *__restrict__ dz2, double *__restrict__ wallprop,
+                                                      int images, int chunk) {
+    // two plane buffers: plane k is read while plane k+1 is written
+    __shared__ double tiles[2][5][VBY + 2 * H][VBX + 2 * H];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = blockIdx.x * VBX + tx + 1;
+    const int j = blockIdx.y * VBY + ty + 1;
+    const int k0 = blockIdx.z * chunk + 1;
+    const int k1 = min(k0 + chunk - 1, g.nz);
+    const bool act = i <= g.nx && j <= g.ny;
+    // loads stay inside the ghosted box; only inactive lanes see clamped values
+    const int ii = min(i, g.nx + H), jj = min(j, g.ny + H);
+    const size_t nv = g.nvol;
+    const int sz = g.sxy;
+    const int lmax = P.visc_lmax;
+    const int col = gidx(g, ii, jj, 0);  // + k*sz
+    // halo point of this lane (x halo for the first/last H lanes of a row,
+    // y halo for the first/last H rows), as a column offset + LDS slot
+    const bool hx = tx < H || tx >= VBX - H;
+    const bool hy = ty < H || ty >= VBY - H;
+    const int colx = hx ? gidx(g, tx < H ? i - H : min(i + H, g.nx + H), jj, 0) : col;
+    const int coly = hy ? gidx(g, ii, ty < H ? j - H : min(j + H, g.ny + H), 0) : col;
+    const int lx = tx < H ? tx : tx + 2 * H, ly = ty < H ? ty : ty + 2 * H;
+    VP win[2 * H + 1];
+#pragma unroll
+    for (int q = 0; q < 2 * H; q++) win[q + 1] = load_vp(wa, nv, col + (k0 - H + q) * sz);
+    VP nxt = load_vp(wa, nv, col + (k0 + H) * sz);
+    VP hxv = load_vp(wa, nv, colx + k0 * sz), hyv = load_vp(wa, nv, coly + k0 * sz);
+    const double ax = dx1[min(i, g.nx) - 1], ay = dy1[min(j, g.ny) - 1];
+    const double sx_ = dxs[min(i, g.nx) - 1], sy_ = dys[min(j, g.ny) - 1];
+    const double qx = dx2[min(i, g.nx) - 1], qy = dy2[min(j, g.ny) - 1];
+    int buf = 0;
+    for (int k = k0; k <= k1; k++, buf ^= 1) {
+#pragma unroll
+        for (int q = 0; q < 2 * H; q++) win[q] = win[q + 1];
+        win[2 * H] = nxt;
+        const VP &c0 = win[H];
+        double(*tile)[VBY + 2 * H][VBX + 2 * H] = tiles[buf];
+        tile[0][ty + H][tx + H] = c0.u;
+        tile[1][ty + H][tx + H] = c0.v;
+        tile[2][ty + H][tx + H] = c0.w;
+        tile[3][ty + H][tx + H] = c0.t;
+        tile[4][ty + H][tx + H] = c0.m;
+        if (hx) {
+            tile[0][ty + H][lx] = hxv.u; tile[1][ty + H][lx] = hxv.v; tile[2][ty + H][lx] = hxv.w;
+            tile[3][ty + H][lx] = hxv.t; tile[4][ty + H][lx] = hxv.m;
+        }
+        if (hy) {
+            tile[0][ly][tx + H] = hyv.u; tile[1][ly][tx + H] = hyv.v; tile[2][ly][tx + H] = hyv.w;
+            tile[3][ly][tx + H] = hyv.t; tile[4][ly][tx + H] = hyv.m;
+        }
+        // right-hand side of this plane, read ahead of the stencil work
+        const int oi = iidx(g, min(i, g.nx), min(j, g.ny), k);
+        const size_t ni = g.nint;
+        const double fo2 = fl[ni + oi], fo3 = fl[2 * ni + oi], fo4 = fl[3 * ni + oi], fo5 = fl[4 * ni + oi];
+        // loads of plane k+1 overlap the work on plane k
+        if (k < k1) {
+            nxt = load_vp(wa, nv, col + (k + 1 + H) * sz);
+            if (hx) hxv = load_vp(wa, nv, colx + (k + 1) * sz);
+            if (hy) hyv = load_vp(wa, nv, coly + (k + 1) * sz);
+        }
+        __syncthreads();
+        if (act) {
+            const double uu = c0.u, vv = c0.v, ww = c0.w, tt = c0.t, mu = c0.m;
+            double ux = 0, vx = 0, wx = 0, tx_ = 0, mux = 0;
+            double uy = 0, vy = 0, wy = 0, ty_ = 0, muy = 0;
+            double uz = 0, vz = 0, wz = 0, tz = 0, muz = 0;
+            const double c20 = P.vd2[0];
+            double ulapx = c20 * uu, ulapy = ulapx, ulapz = ulapx;
+            double vlapx = c20 * vv, vlapy = vlapx, vlapz = vlapx;
+            double wlapx = c20 * ww, wlapy = wlapx, wlapz = wlapx;
+            double tlapx = c20 * tt, tlapy = tlapx, tlapz = tlapx;
+#pragma unroll
+            for (int l = 1; l <= H; l++) {
+                if (l <= lmax) {
+                    const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
+                    const int r = ty + H, cp = tx + H + l, cm = tx + H - l;
+                    ux = ux + ccl * (tile[0][r][cp] - tile[0][r][cm]);
+                    ulapx = ulapx + clapl * (tile[0][r][cp] + tile[0][r][cm]);
+                    vx = vx + ccl * (tile[1][r][cp] - tile[1][r][cm]);
+                    vlapx = vlapx + clapl * (tile[1][r][cp] + tile[1][r][cm]);
+                    wx = wx + ccl * (tile[2][r][cp] - tile[2][r][cm]);
+                    wlapx = wlapx + clapl * (tile[2][r][cp] + tile[2][r][cm]);
+                    tx_ = tx_ + ccl * (tile[3][r][cp] - tile[3][r][cm]);
+                    tlapx = tlapx + clapl * (tile[3][r][cp] + tile[3][r][cm]);
+                    mux = mux + ccl * (tile[4][r][cp] - tile[4][r][cm]);
+                }
+            }
+#pragma unroll
+            for (int l = 1; l <= H; l++) {
+                if (l <= lmax) {
+                    const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
+                    const int c = tx + H, rp = ty + H + l, rm = ty + H - l;
+                    uy = uy + ccl * (tile[0][rp][c] - tile[0][rm][c]);
+                    ulapy = ulapy + clapl * (tile[0][rp][c] + tile[0][rm][c]);
+                    vy = vy + ccl * (tile[1][rp][c] - tile[1][rm][c]);
+                    vlapy = vlapy + clapl * (tile[1][rp][c] + tile[1][rm][c]);
+                    wy = wy + ccl * (tile[2][rp][c] - tile[2][rm][c]);
+                    wlapy = wlapy + clapl * (tile[2][rp][c] + tile[2][rm][c]);
+                    ty_ = ty_ + ccl * (tile[3][rp][c] - tile[3][rm][c]);
+                    tlapy = tlapy + clapl * (tile[3][rp][c] + tile[3][rm][c]);
+                    muy = muy + ccl * (tile[4][rp][c] - tile[4][rm][c]);
+                }
+            }
+#pragma unroll
+            for (int l = 1; l <= H; l++) {
+                if (l <= lmax) {
+                    const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
+                    const VP &pp = win[H + l], &pm = win[H - l];
+                    uz = uz + ccl * (pp.u - pm.u);
+                    ulapz = ulapz + clapl * (pp.u + pm.u);
+                    vz = vz + ccl * (pp.v - pm.v);
+                    vlapz = vlapz + clapl * (pp.v + pm.v);
+                    wz = wz + ccl * (pp.w - pm.w);
+                    wlapz = wlapz + clapl * (pp.w + pm.w);
+                    tz = tz + ccl * (pp.t - pm.t);
+                    tlapz = tlapz + clapl * (pp.t + pm.t);
+                    muz = muz + ccl * (pp.m - pm.m);
+                }
+            }
+            const double az = dz1[k - 1];
+            ux = ux * ax; vx = vx * ax; wx = wx * ax; tx_ = tx_ * ax; mux = mux * ax;
+            uy = uy * ay; vy = vy * ay; wy = wy * ay; ty_ = ty_ * ay; muy = muy * ay;
+            uz = uz * az; vz = vz * az; wz = wz * az; tz = tz * az; muz = muz * az;
+            const double sz_ = dzs[k - 1], qz = dz2[k - 1];
+            ulapx = ulapx * sx_ + ux * qx; vlapx = vlapx * sx_ + vx * qx;
+            wlapx = wlapx * sx_ + wx * qx; tlapx = tlapx * sx_ + tx_ * qx;
+            ulapy = ulapy * sy_ + uy * qy; vlapy = vlapy * sy_ + vy * qy;
+            wlapy = wlapy * sy_ + wy * qy; tlapy = tlapy * sy_ + ty_ * qy;
+            ulapz = ulapz * sz_ + uz * qz; vlapz = vlapz * sz_ + vz * qz;
+            wlapz = wlapz * sz_ + wz * qz; tlapz = tlapz * sz_ + tz * qz;
+            const double ulap = ulapx + ulapy + ulapz;
+            const double vlap = vlapx + vlapy + vlapz;
+            const double wlap = wlapx + wlapy + wlapz;
+            const double tlap = tlapx + tlapy + tlapz;
+            const double div = ux + vy + wz;
+            const double div3l = div / 3.0;
+            const int c = gidx(g, i, j, k);
+            wa[9 * nv + c] = div3l;
+            const double omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
+            const double omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
+            wa[8 * nv + c] = sqrt(omod2);
+            if (SENSOR) {
+                const double u0l0 = P.u0 / P.l0;
+                double sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
+                sv = fmax(sv, 0.0);
+                wa[7 * nv + c] = sv * sv;
+                if (images) write_images(g, P, wa + 7 * nv, i, j, k, sv * sv, false);
+            }
+            if (images && !SENSOR) write_images(g, P, wa + 9 * nv, i, j, k, div3l, true);
+            const double cploc = cploc_of(P, tt);
+            const double sig11 = 2.0 * (ux - div3l), sig12 = uy + vx, sig13 = uz + wx;
+            const double sig22 = 2.0 * (vy - div3l), sig23 = vz + wy, sig33 = 2.0 * (wz - div3l);
+            const double sigx = mux * sig11 + muy * sig12 + muz * sig13 + mu * ulap;
+            const double sigy = mux * sig12 + muy * sig22 + muz * sig23 + mu * vlap;
+            const double sigz = mux * sig13 + muy * sig23 + muz * sig33 + mu * wlap;
+            const double sigqt = (mux * tx_ + muy * ty_ + muz * tz + mu * tlap) * cploc / P.prandtl;
+            const double sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig12 * vx + sig22 * vy + sig23 * vz +
+                                  sig13 * wx + sig23 * wy + sig33 * wz) * mu;
+            const double sigq = sigx * uu + sigy * vv + sigz * ww + sigah + sigqt;
+            fl[ni + oi] = fo2 - sigx;
+            fl[2 * ni + oi] = fo3 - sigy;
+            fl[3 * ni + oi] = fo4 - sigz;
+            fl[4 * ni + oi] = fo5 - sigq;
+            if (j == 1) {
+                const size_t wsz = (size_t)g.sx * g.sz;
+                const size_t wp = (size_t)(i + g.ng - 1) + (size_t)g.sx * (k + g.ng - 1);
+                wallprop[wp] = mu * uy;
+                wallprop[wsz + wp] = mu * wy;
+                wallprop[2 * wsz + wp] = mu * ty_ * cploc / P.prandtl;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------ conservative viscous form
 // Face flux between points a (left) and b (right) at node distance h
 // (kernels_cpu.F90:4805-4836): mu_f (du, dv, dw) and
 // mu_f (cp/Pr dT + d(|u|^2/2)), mu_f = (mu_a + mu_b)/2/h.
@@ -1585,14 +1648,21 @@ hipError_t launch_euler_z(st_solver *s, double *fl) { return launch_march(s, fl,
 
 hipError_t launch_visc(st_solver *s, double *fl, int mode, bool ghost_images) {
     const Geo &g = s->g;
+    const int bxy = ((g.nx + VBX - 1) / VBX) * ((g.ny + VBY - 1) / VBY);
+    // chunks of up to 32 planes (6 warm-up planes each), enough of them to
+    // give every CU several tiles
+    int chunk = 32;
+    while (chunk > 8 && (long)bxy * ((g.nz + chunk - 1) / chunk) < 2048) chunk /= 2;
+    const dim3 grid((g.nx + VBX - 1) / VBX, (g.ny + VBY - 1) / VBY, (g.nz + chunk - 1) / chunk);
+    const dim3 block(VBX, VBY);
     if (mode == 0)
-        hipLaunchKernelGGL(k_visc<true>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, s->dcsidx,
-                           s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2,
-                           s->wallprop, ghost_images ? 1 : 0);
+        hipLaunchKernelGGL(k_visc_zm<true>, grid, block, 0, s->stream, g, s->kp, s->w_aux, fl, s->dcsidx, s->dcsidxs,
+                           s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2,
+                           s->wallprop, ghost_images ? 1 : 0, chunk);
     else
-        hipLaunchKernelGGL(k_visc<false>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, s->dcsidx,
-                           s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2,
-                           s->wallprop, ghost_images ? 1 : 0);
+        hipLaunchKernelGGL(k_visc_zm<false>, grid, block, 0, s->stream, g, s->kp, s->w_aux, fl, s->dcsidx, s->dcsidxs,
+                           s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2,
+                           s->wallprop, ghost_images ? 1 : 0, chunk);
     return hipGetLastError();
 }
 
