@@ -58,6 +58,73 @@ def measured_traffic(workload, kernel):
     return d.get("bytes_per_launch", {}).get(kernel)
 
 
+def timed_run(g, c, warmup, steps):
+    """init + warmup steps untimed, then ``steps`` RK3 steps timed with the
+    solver's stream drained on both sides; returns seconds."""
+    g.copy_from_field(c.w0)
+    g.init_run()
+    g.advance(warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.advance(steps)
+    g.synchronize()
+    return time.perf_counter() - t0
+
+
+def mixed_form(args, c, value_fp64):
+    """The mixed form (FP32 w_aux, right-hand sides, central fluxes and
+    viscous stresses; FP64 state, T/p recovery, WENO, RK and reductions) on
+    the same workload: its rate, and its relative L2 error per conserved
+    variable against the FP64 form after 100 steps from the same field."""
+    import numpy as np
+    from streams_mi355x import PREC_MIXED
+    from streams_mi355x.params import params_copy
+    from streams_mi355x.solver import ChannelSolver
+
+    pm = params_copy(c.params)
+    pm.precision = PREC_MIXED
+    gm = ChannelSolver(pm, c.grid)
+    el = timed_run(gm, c, args.warmup, args.steps)
+    value = args.nx * args.ny * args.nz * 3 * args.steps / el / 1e9
+    kt = gm.kernel_times(min(20, max(3, args.steps // 10)))
+    nerr = 100
+    fields = []
+    for g in (gm, None):
+        if g is None:
+            g = ChannelSolver(c.params, c.grid)
+        g.copy_from_field(c.w0)
+        g.init_run()
+        g.advance(nerr)
+        fields.append(g.copy_to_field()[:, 3:-3, 3:-3, 3:-3])
+        g.close()
+    errs = [float(np.linalg.norm(fields[0][m] - fields[1][m]) / np.linalg.norm(fields[1][m])) for m in range(5)]
+    return {"value": value, "unit": "Gcell-updates/s per RK stage", "ms_per_step": el / args.steps * 1e3,
+            "speedup_vs_fp64": value / value_fp64, "rel_l2_vs_fp64_after_100_steps": errs,
+            "kernels_ms": {k: round(v[0], 4) for k, v in kt.items()}}
+
+
+def serving(args):
+    """Serving workload: a seeded stream of independent channel solves
+    (grids uniform in [64,256]x[64,128]x[64,128], bulk Mach 0.3-3.0, Re_tau
+    100-500, Poisson arrivals), each advancing ``--serve-steps`` RK3 steps,
+    FP64.  Continuous batching through the library's server against the same
+    kind of requests run one after another through the plain path."""
+    from streams_mi355x.serve import cell_updates, run_sequential, run_serving, serving_workload
+
+    reqs = serving_workload(args.serve_requests, args.serve_steps, rate=args.serve_rate, seed=0)
+    seq_n = min(len(reqs), args.serve_seq_sample)
+    seq = run_sequential(reqs[:seq_n])
+    srv = run_serving(reqs, hbm_bytes=int(args.serve_hbm_gib * (1 << 30)), max_slots=args.serve_slots,
+                      steps_per_graph=10, autotune=True)
+    return {"requests": len(reqs), "steps_per_request": args.serve_steps, "arrival_rate_per_s": args.serve_rate,
+            "value": srv["gcups"], "unit": "Gcell-updates/s per RK stage", "p50_s": srv["p50_s"],
+            "p99_s": srv["p99_s"], "mean_queue_s": srv["mean_queue_s"], "makespan_s": srv["makespan_s"],
+            "sequential_value": seq["gcups"], "sequential_sample": f"first {seq_n} requests, plain path, one by one",
+            "speedup_vs_sequential": srv["gcups"] / seq["gcups"], "slots": args.serve_slots,
+            "hbm_capacity_gib": args.serve_hbm_gib, "tuned_shape_classes": srv["stats"]["tuned_classes"],
+            "total_cell_updates": cell_updates(reqs)}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +136,14 @@ def parse():
     ap.add_argument("--retau", type=float, default=550.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-steps", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-hbm-gib", type=float, default=64.0)
+    ap.add_argument("--serve-seq-sample", type=int, default=16)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of 2 GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -133,14 +208,7 @@ def main():
     nx, ny, nz = args.nx, args.ny, args.nz
     c = make_channel(nx=nx, ny=ny, nz=nz, retau=args.retau)
     g = ChannelSolver(c.params, c.grid)
-    g.copy_from_field(c.w0)
-    g.init_run()
-    g.advance(args.warmup)
-    g.synchronize()
-    t0 = time.perf_counter()
-    g.advance(args.steps)
-    g.synchronize()
-    el = time.perf_counter() - t0
+    el = timed_run(g, c, args.warmup, args.steps)
     cells = nx * ny * nz
     value = cells * 3 * args.steps / el / 1e9
     res = g.compute_residual()
@@ -162,7 +230,7 @@ def main():
                   for k, v in kt.items()}
 
     out = {
-        "metric": "Gcell-updates/s per RK stage (FP64)",
+        "metric": "Gcell-updates/s per RK stage, FP64 vs mixed; p50/p99 latency of batched solves",
         "value": value,
         "unit": "Gcell-updates/s per RK stage",
         "n_gpus": 1,
@@ -185,6 +253,10 @@ def main():
         "kernels": per_kernel,
     }
     g.close()
+    if not args.no_mixed:
+        out["mixed"] = mixed_form(args, c, value)
+    if not args.no_serve:
+        out["serving"] = serving(args)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args)

@@ -183,11 +183,46 @@ int st_count_weno(st_solver *s, double frac_xyz[3]);
 int st_kernel_times(st_solver *s, int32_t nsteps, double ms[ST_NKFAM], int32_t count[ST_NKFAM]);
 
 /* Stream used by every launch of this solver (NULL = the library's own). */
+/* Launch shapes (no reference counterpart; tuning only): planes per tile
+ * of the z-marching viscous kernel and points per thread of the y/z flux
+ * marches, 0 = automatic.  Results do not depend on them. */
+int st_set_tuning(st_solver *s, int32_t visc_chunk, int32_t march_chunk);
 int st_set_stream(st_solver *s, void *hip_stream);
 int st_synchronize(st_solver *s);
 
 const char *st_last_error(void);
 const char *st_version(void);
+
+/* ---- serving loop (no reference counterpart) ----------------------------
+ * Continuous batching of many independent solves on one GPU: requests are
+ * admitted in arrival order as slots and HBM (st_solver_footprint) free up,
+ * each runs on its own stream from a HIP graph of steps_per_graph steps
+ * (0: no graph), and their work overlaps on the device.  autotune != 0
+ * picks launch shapes per grid-shape class (st_set_tuning).  A result is
+ * identical to the same solve run alone through st_advance. */
+typedef struct st_server st_server;
+int st_server_create(int64_t hbm_capacity_bytes, int32_t max_slots, int32_t steps_per_graph, int32_t autotune,
+                     st_server **out);
+int st_server_destroy(st_server *sv);
+/* Queue one solve: parameters, the 13 grid arrays of st_set_grid, the
+ * initial field w (5 ghosted variables) and the number of RK3 steps after
+ * st_init_run.  Inputs are copied; *ticket identifies the request. */
+int st_server_submit(st_server *sv, const st_params *p,
+                     const double *x, const double *y, const double *z, const double *yn,
+                     const double *dcsidx, const double *dcsidxs, const double *dcsidx2,
+                     const double *detady, const double *detadys, const double *detady2,
+                     const double *dzitdz, const double *dzitdzs, const double *dzitdz2,
+                     const double *w0, int32_t nsteps, int64_t *ticket);
+/* Wait (timeout_s < 0: no limit) for a request, copy its final field (may
+ * be NULL), scalars after st_compute_residual (may be NULL) and its submit /
+ * admit / done times in seconds since server creation (may be NULL), then
+ * forget it.  ST_ERR_STATE on time-out. */
+int st_server_wait(st_server *sv, int64_t ticket, double timeout_s, double *w_out, st_scalars *scalars,
+                   double times[3]);
+/* [0] queued, [1] running, [2] completed, [3] failed, [4] HBM bytes reserved,
+ * [5] tuned shape classes. */
+int st_server_stats(st_server *sv, int64_t out[6]);
+const char *st_server_last_error(void);
 
 #ifdef __cplusplus
 }

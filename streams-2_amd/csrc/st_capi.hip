@@ -7,6 +7,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "st_solver.h"
 
@@ -66,7 +67,8 @@ int validate(const st_params *p) {
     if (p->calorically_perfect != 0 && p->calorically_perfect != 1)
         return fail(ST_ERR_ARG, "calorically_perfect must be 0 or 1");
     if (p->indx_cp_r - p->indx_cp_l + 2 > ST_MAX_CP_COEFF) return fail(ST_ERR_ARG, "too many cp coefficients");
-    if (p->precision != ST_PREC_FP64) return fail(ST_ERR_ARG, "only the FP64 precision is available");
+    if (p->precision != ST_PREC_FP64 && p->precision != ST_PREC_MIXED)
+        return fail(ST_ERR_ARG, "precision must be ST_PREC_FP64 or ST_PREC_MIXED");
     for (int l = 0; l < 6; l++) {
         int t = p->bctags[l];
         if (t != 0 && !(t == 6 && p->is_y_staggered && (l == 2 || l == 3)))
@@ -132,6 +134,7 @@ void fill_kpar(st_solver *s) {
     k.visc_model = p.visc_model;
     k.ndim = p.nz > 1 ? 3 : 2;
     k.rgas0 = p.rgas0; k.prandtl = p.prandtl; k.t0 = p.t0; k.rho0 = p.rho0; k.u0 = p.u0; k.l0 = p.l0;
+    k.p_ref = p.rho0 * p.rgas0 * p.t0;
     k.mu0 = p.mu0; k.sutherland_s = p.sutherland_s; k.t_ref_dim = p.t_ref_dim; k.vtexp = p.powerlaw_vtexp;
     k.sensor_threshold = p.sensor_threshold; k.t_wall = p.t_wall; k.t_bulk_target = p.t_bulk_target;
     k.volchan = p.volchan; k.cfl = p.cfl; k.tol_iter_nr = p.tol_iter_nr;
@@ -148,6 +151,11 @@ void fill_kpar(st_solver *s) {
     for (int l = 0; l < 4; l++) k.vd1[l] = d1[l];
     for (int l = 0; l < 5; l++) k.vd2[l] = d2[l];
 }
+
+// bytes of one w_aux / right-hand-side value
+size_t rsize(const st_solver *s) { return s->mixed ? sizeof(float) : sizeof(double); }
+// w_aux variables: the 10 of the reference, plus p - p_ref in the mixed form
+size_t naux(const st_solver *s) { return s->mixed ? 11 : 10; }
 
 template <class T>
 hipError_t dalloc(T **p, size_t n) {
@@ -195,8 +203,8 @@ struct TimeScope {
 // one RK stage of rk_sync (singleideal_cpu.F90:352-406), fused
 int stage(st_solver *s, int istep) {
     using namespace st;
-    double *fprev = s->F[s->cur];   // RHS of the previous stage (reference: fl before init_flux)
-    double *fcur = s->F[s->cur ^ 1];
+    void *fprev = s->F[s->cur];   // RHS of the previous stage (reference: fl before init_flux)
+    void *fcur = s->F[s->cur ^ 1];
     const Geo &g = s->g;
     // bcswap + compute_aux(ghost=1): aux over the box, periodic ghosts from images
     TIMED(ST_K_AUX, launch_aux(s, 1 - g.ng, g.nx + g.ng, 1 - g.ng, g.ny + g.ng, 1 - g.ng, g.nz + g.ng, true));
@@ -237,7 +245,10 @@ int64_t st_solver_footprint(const st_params *p) {
     const int64_t sx = p->nx + 2 * p->ng, sy = p->ny + 2 * p->ng, sz = p->nz + 2 * p->ng;
     const int64_t nvol = sx * sy * sz, nint = (int64_t)p->nx * p->ny * p->nz;
     const int64_t blocks = (int64_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * p->nz;
-    return 8 * (15 * nvol + 10 * nint + 3 * sx * sz + 7 * blocks + 1024) + (nint + 4 * (p->nx + p->ny + p->nz + 3 + 4)) +
+    const int64_t rs = p->precision == ST_PREC_MIXED ? 4 : 8;  // bytes of a w_aux / right-hand-side value
+    const int64_t na = p->precision == ST_PREC_MIXED ? 11 : 10;
+    return 8 * (5 * nvol + 3 * sx * sz + 7 * blocks + 1024) + rs * (na * nvol + 10 * nint) +
+           (nint + 4 * (p->nx + p->ny + p->nz + 3 + 4)) +
            8 * (2 * sx + 2 * sy + 2 * sz + 3 * (p->nx + p->ny + p->nz) + p->ny + 1) + 4096;
 }
 
@@ -259,6 +270,7 @@ int st_solver_create(const st_params *p, st_solver **out) {
     s->rhork[0] = 0.0; s->rhork[1] = -17.0 / 60.0; s->rhork[2] = -5.0 / 12.0;   // Wray RK3,
     s->gamrk[0] = 8.0 / 15.0; s->gamrk[1] = 5.0 / 12.0; s->gamrk[2] = 3.0 / 4.0; // singleideal.F90:4780-4782
     s->last_stage = -1;
+    s->mixed = p->precision == ST_PREC_MIXED;
     fill_kpar(s);
     hipError_t e = hipGetDevice(&s->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -273,9 +285,9 @@ int st_solver_create(const st_params *p, st_solver **out) {
     hipError_t es[32];
     int ne = 0;
     es[ne++] = dalloc(&s->w, 5 * nvol);
-    es[ne++] = dalloc(&s->w_aux, 10 * nvol);
-    es[ne++] = dalloc(&s->F[0], 5 * nint);
-    es[ne++] = dalloc(&s->F[1], 5 * nint);
+    es[ne++] = hipMalloc(&s->w_aux, rsize(s) * naux(s) * nvol);
+    es[ne++] = hipMalloc(&s->F[0], rsize(s) * 5 * nint);
+    es[ne++] = hipMalloc(&s->F[1], rsize(s) * 5 * nint);
     es[ne++] = dalloc(&s->wallprop, (size_t)3 * g.sx * g.sz);
     es[ne++] = dalloc(&s->eoc[0], (size_t)g.nx + 1);
     es[ne++] = dalloc(&s->eoc[1], (size_t)g.ny + 1);
@@ -318,9 +330,9 @@ int st_solver_create(const st_params *p, st_solver **out) {
     std::free(buf);
     // zero state (w_aux = 0 as in the reference alloc, singleideal.F90:1596)
     if (ec == hipSuccess) ec = hipMemset(s->w, 0, sizeof(double) * 5 * nvol);
-    if (ec == hipSuccess) ec = hipMemset(s->w_aux, 0, sizeof(double) * 10 * nvol);
-    if (ec == hipSuccess) ec = hipMemset(s->F[0], 0, sizeof(double) * 5 * nint);
-    if (ec == hipSuccess) ec = hipMemset(s->F[1], 0, sizeof(double) * 5 * nint);
+    if (ec == hipSuccess) ec = hipMemset(s->w_aux, 0, rsize(s) * naux(s) * nvol);
+    if (ec == hipSuccess) ec = hipMemset(s->F[0], 0, rsize(s) * 5 * nint);
+    if (ec == hipSuccess) ec = hipMemset(s->F[1], 0, rsize(s) * 5 * nint);
     if (ec == hipSuccess) ec = hipMemset(s->wallprop, 0, sizeof(double) * 3 * g.sx * g.sz);
     if (ec == hipSuccess) ec = hipMemset(s->counters, 0, sizeof(int) * 4);
     std::memset(s->hs, 0, sizeof(st::DevScalars));
@@ -380,30 +392,51 @@ int st_copy_to_field(st_solver *s, double *w_host) {
     return ST_OK;
 }
 
+// Device array of n values (double, or float in the mixed form) to/from a
+// host double array.
+static int to_host(st_solver *s, double *host, const void *dev, size_t n) {
+    if (!s->mixed) {
+        HIPCHK(hipMemcpyAsync(host, dev, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return ST_OK;
+    }
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), dev, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t t = 0; t < n; t++) host[t] = tmp[t];
+    return ST_OK;
+}
+
+static int from_host(st_solver *s, void *dev, const double *host, size_t n) {
+    if (!s->mixed) {
+        HIPCHK(hipMemcpyAsync(dev, host, sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return ST_OK;
+    }
+    std::vector<float> tmp(host, host + n);
+    HIPCHK(hipMemcpyAsync(dev, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
 int st_get_aux(st_solver *s, int32_t var, double *host) {
     NEED(s);
     if (var < 1 || var > 10 || !host) return fail(ST_ERR_ARG, "var must be 1..10");
     const size_t nv = s->g.nvol;
-    HIPCHK(hipMemcpyAsync(host, s->w_aux + (size_t)(var - 1) * nv, sizeof(double) * nv, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return ST_OK;
+    return to_host(s, host, (const char *)s->w_aux + rsize(s) * (size_t)(var - 1) * nv, nv);
 }
 
 int st_set_aux(st_solver *s, int32_t var, const double *host) {
     NEED(s);
     if (var < 1 || var > 10 || !host) return fail(ST_ERR_ARG, "var must be 1..10");
     const size_t nv = s->g.nvol;
-    HIPCHK(hipMemcpyAsync(s->w_aux + (size_t)(var - 1) * nv, host, sizeof(double) * nv, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return ST_OK;
+    return from_host(s, (char *)s->w_aux + rsize(s) * (size_t)(var - 1) * nv, host, nv);
 }
 
 int st_get_fl(st_solver *s, double *host) {
     NEED(s);
     if (!host) return fail(ST_ERR_ARG, "null host array");
-    HIPCHK(hipMemcpyAsync(host, s->F[s->cur], sizeof(double) * 5 * (size_t)s->g.nint, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return ST_OK;
+    return to_host(s, host, s->F[s->cur], 5 * (size_t)s->g.nint);
 }
 
 int st_get_scalars(st_solver *s, st_scalars *out) {
@@ -468,7 +501,7 @@ int st_compute_aux(st_solver *s, int32_t central, int32_t ghost) {
 
 int st_visflx(st_solver *s, int32_t mode) {
     NEED(s);
-    double *fl = s->F[s->cur];
+    void *fl = s->F[s->cur];
     switch (mode) {
     case 0: HIPCHK(st::launch_visc(s, fl, 0, false)); break;
     case 1: HIPCHK(st::launch_visc_div(s, fl, s->F[s->cur ^ 1], -1, false, false)); break;
@@ -501,17 +534,16 @@ int st_euler_z(st_solver *s) {
 
 int st_zero_flux(st_solver *s) {
     NEED(s);
-    HIPCHK(st::launch_zero(s, s->F[0], (size_t)5 * s->g.nint));
-    HIPCHK(st::launch_zero(s, s->F[1], (size_t)5 * s->g.nint));
+    HIPCHK(st::launch_zero_rhs(s, s->F[0]));
+    HIPCHK(st::launch_zero_rhs(s, s->F[1]));
     return ST_OK;
 }
 
 // Periodic ghost swap / extrapolation of one aux variable (bcswap_var,
 // bcextr_var); used by st_init_run and exposed through the tests.
 static int swap_var(st_solver *s, int var, bool extrapolate) {
-    double *a = s->w_aux + (size_t)(var - 1) * s->g.nvol;
-    if (extrapolate) HIPCHK(st::launch_extrapolate_var(s, a));
-    HIPCHK(st::launch_fill_periodic(s, a, 1));
+    if (extrapolate) HIPCHK(st::launch_extrapolate_aux(s, var));
+    HIPCHK(st::launch_fill_periodic_aux(s, var));
     return ST_OK;
 }
 
@@ -631,6 +663,15 @@ int st_count_weno(st_solver *s, double frac_xyz[3]) {
     frac_xyz[0] = s->hs->cnt[0] / ((double)(k.eul_hi[0] - k.eul_lo[0] + 1) * ny * nz);
     frac_xyz[1] = s->hs->cnt[1] / (nx * (double)(k.eul_hi[1] - k.eul_lo[1] + 1) * nz);
     frac_xyz[2] = s->hs->cnt[2] / (nx * ny * (double)(k.eul_hi[2] - k.eul_lo[2] + 1));
+    return ST_OK;
+}
+
+int st_set_tuning(st_solver *s, int32_t visc_chunk, int32_t march_chunk) {
+    NEED(s);
+    if (visc_chunk < 0 || visc_chunk > 1024 || march_chunk < 0 || march_chunk > 1024)
+        return fail(ST_ERR_ARG, "chunks must be 0 (automatic) or 1..1024");
+    s->visc_chunk = visc_chunk;
+    s->march_chunk = march_chunk;
     return ST_OK;
 }
 

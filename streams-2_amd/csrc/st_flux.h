@@ -41,18 +41,40 @@ __device__ __forceinline__ double un_of(const Pt &q) {
 }
 
 // The six values the central split needs at a point: rho, u, v, w, p, e
-// (H and T enter only through p and e).
-struct Pc {
-    double r, u, v, w, p, e;
+// (H and T enter only through p and e).  R is the arithmetic type of the
+// central sweeps: double in the FP64 form, float in the mixed form.
+template <typename R>
+struct PcR {
+    R r, u, v, w, p, e;
 };
+using Pc = PcR<double>;
+
+// tp is T in the FP64 form (p = T*rho*rgas0) and the stored deviation
+// p - p_ref in the mixed form (FP32), where q.p then holds that deviation:
+// a constant part of p cancels from the momentum-flux difference, so only
+// the energy flux adds p_ref back (pair_terms).
+template <typename R>
+__device__ __forceinline__ PcR<R> make_pcr(R r, R u, R v, R w, R h, R tp, R rgas0, R pref, bool keep) {
+    PcR<R> q;
+    q.r = r; q.u = u; q.v = v; q.w = w;
+    if constexpr (sizeof(R) == sizeof(double)) {
+        q.p = tp * r * rgas0;
+        q.e = keep ? h - q.p / r - R(0.5) * (u * u + v * v + w * w) : h;
+    } else {
+        q.p = tp;
+        q.e = keep ? h - (tp + pref) / r - R(0.5) * (u * u + v * v + w * w) : h;
+    }
+    return q;
+}
+
+// w_aux slot read as tp by make_pcr: T (6th variable) in FP64, p - p_ref
+// (11th) in the mixed form
+template <typename R>
+constexpr int tp_slot() { return sizeof(R) == sizeof(double) ? 5 : 10; }
 
 __device__ __forceinline__ Pc make_pc(double r, double u, double v, double w, double h, double t, double rgas0,
                                       bool keep) {
-    Pc q;
-    q.r = r; q.u = u; q.v = v; q.w = w;
-    q.p = t * r * rgas0;
-    q.e = keep ? h - q.p / r - 0.5 * (u * u + v * v + w * w) : h;
-    return q;
+    return make_pcr<double>(r, u, v, w, h, t, rgas0, 0.0, keep);
 }
 
 __device__ __forceinline__ Pc pc_of(const Pt &a) {
@@ -61,8 +83,8 @@ __device__ __forceinline__ Pc pc_of(const Pt &a) {
     return q;
 }
 
-template <int DIR>
-__device__ __forceinline__ double un_of(const Pc &q) {
+template <int DIR, typename R>
+__device__ __forceinline__ R un_of(const PcR<R> &q) {
     return DIR == 0 ? q.u : (DIR == 1 ? q.v : q.w);
 }
 

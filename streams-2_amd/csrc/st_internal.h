@@ -46,6 +46,7 @@ struct KPar {
     int eul_lo[3], eul_hi[3];
     int ndim;
     double rgas0, prandtl, t0, rho0, u0, l0, mu0;
+    double p_ref;          // rho0*rgas0*t0: the mixed form stores p - p_ref
     double sutherland_s, t_ref_dim, vtexp;
     double sensor_threshold, t_wall, t_bulk_target, volchan, cfl, tol_iter_nr;
     double cp[ST_MAX_CP_COEFF], cv[ST_MAX_CP_COEFF];

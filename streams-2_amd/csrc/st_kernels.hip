@@ -73,7 +73,8 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[NV]) {
 // Periodic copy of bcswap (base_cpu.F90:410-450, 669-745 with the local
 // neighbour shortcut :1382-1417) for nv ghosted variables: faces of the
 // periodic directions only, edges and corners untouched.
-__global__ void k_fill_periodic(Geo g, KPar P, double *__restrict__ a, int nv) {
+template <typename T>
+__global__ void k_fill_periodic(Geo g, KPar P, T *__restrict__ a, int nv) {
     const int ng = g.ng;
     const int nxf = P.periodic[0] ? 2 * ng * g.ny * g.nz : 0;
     const int nyf = P.periodic[1] ? 2 * ng * g.nx * g.nz : 0;
@@ -103,14 +104,15 @@ __global__ void k_fill_periodic(Geo g, KPar P, double *__restrict__ a, int nv) {
             if (l < ng) { dst = gidx(g, i, j, 1 - ng + l); src = gidx(g, i, j, g.nz - ng + 1 + l); }
             else { dst = gidx(g, i, j, g.nz + 1 + (l - ng)); src = gidx(g, i, j, 1 + (l - ng)); }
         }
-        double *b = a + (size_t)m * g.nvol;
+        T *b = a + (size_t)m * g.nvol;
         b[dst] = b[src];
     }
 }
 
 // bcextr_var_subroutine (kernels_cpu.F90:6199-6254): zero-gradient ghosts of
 // one variable on all six faces.
-__global__ void k_extrapolate_var(Geo g, double *__restrict__ a) {
+template <typename T>
+__global__ void k_extrapolate_var(Geo g, T *__restrict__ a) {
     const int ng = g.ng;
     const int nxf = 2 * ng * g.ny * g.nz, nyf = 2 * ng * g.nx * g.nz, nzf = 2 * ng * g.nx * g.ny;
     const int tot = nxf + nyf + nzf;
@@ -139,8 +141,9 @@ __global__ void k_extrapolate_var(Geo g, double *__restrict__ a) {
 
 // bcwall_staggered_subroutine (kernels_cpu.F90:6612-6678) for one ghost
 // point: the mirror of interior point js, written at jg.
+template <typename R>
 __device__ __forceinline__ void wall_ghost(const Geo &g, const KPar &P, double *__restrict__ w,
-                                           const double *__restrict__ wa, int i, int js, int jg, int k) {
+                                           const R *__restrict__ wa, int i, int js, int jg, int k) {
     const int s = gidx(g, i, js, k), d = gidx(g, i, jg, k);
     const size_t nv = g.nvol;
     double rho = w[s];
@@ -150,7 +153,7 @@ __device__ __forceinline__ void wall_ghost(const Geo &g, const KPar &P, double *
     const double rhoe = w[4 * nv + s];
     const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
     double ee = rhoe / rho - qq;
-    double tt = temperature_from_e(P, ee, wa[5 * nv + s]);
+    double tt = temperature_from_e(P, ee, double(wa[5 * nv + s]));
     const double pp = rho * tt * P.rgas0;
     tt = 2.0 * P.t_wall - tt;
     ee = e_from_temperature(P, tt);
@@ -162,7 +165,8 @@ __device__ __forceinline__ void wall_ghost(const Geo &g, const KPar &P, double *
     w[4 * nv + d] = rho * (ee + qq);
 }
 
-__global__ void k_wall_ghosts(Geo g, KPar P, double *__restrict__ w, const double *__restrict__ wa) {
+template <typename R>
+__global__ void k_wall_ghosts(Geo g, KPar P, double *__restrict__ w, const R *__restrict__ wa) {
     const int tot = 2 * g.ng * g.nx * g.nz;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
         const int i = t % g.nx + 1;
@@ -183,7 +187,8 @@ __global__ void k_wall_ghosts(Geo g, KPar P, double *__restrict__ w, const doubl
 // [i0,i1]x[j0,j1]x[k0,k1]; with image=true a ghost point of a periodic
 // direction reads w at its periodic image, which is the value bcswap
 // would have copied there.
-__global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__restrict__ w, double *__restrict__ wa,
+template <typename R>
+__global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__restrict__ w, R *__restrict__ wa,
                                              int i0, int ni, int j0, int nj, int k0, int nk, int image) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni * nj * nk) return;
@@ -204,15 +209,16 @@ __global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__rest
     const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
     const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
     const double ee = rhoe / rho - qq;
-    const double tt = temperature_from_e(P, ee, wa[5 * nv + d]);
+    const double tt = temperature_from_e(P, ee, double(wa[5 * nv + d]));
     const double pp = rho * tt * P.rgas0;
-    wa[d] = rho;
-    wa[nv + d] = uu;
-    wa[2 * nv + d] = vv;
-    wa[3 * nv + d] = ww;
-    wa[4 * nv + d] = (rhoe + pp) / rho;
-    wa[5 * nv + d] = tt;
-    wa[6 * nv + d] = viscosity_of(P, tt);
+    wa[d] = R(rho);
+    wa[nv + d] = R(uu);
+    wa[2 * nv + d] = R(vv);
+    wa[3 * nv + d] = R(ww);
+    wa[4 * nv + d] = R((rhoe + pp) / rho);
+    wa[5 * nv + d] = R(tt);
+    wa[6 * nv + d] = R(viscosity_of(P, tt));
+    if constexpr (sizeof(R) != sizeof(double)) wa[10 * nv + d] = R(pp - P.p_ref);
 }
 
 // ------------------------------------------------------- dirty-cell flags
@@ -242,50 +248,56 @@ __global__ void k_clear_any(unsigned char *__restrict__ flags, int nint) { flags
 // complete; its divergence goes to cell b-3 (euler_y/z_update_subroutine,
 // :991-1010, 1039-1058).  Faces whose sensor window exceeds the threshold
 // are left to k_weno_fix through the dirty-cell list.
+template <typename R>
 struct Acc {
-    double m, mu, mv, mw, e, pp;
+    R m, mu, mv, mw, e, pp;
 };
 
-template <int DIR, int KM>
-__device__ __forceinline__ Acc pair_terms(const KPar &P, const Pc &a, const Pc &b) {
-    const double una = un_of<DIR>(a), unb = un_of<DIR>(b);
-    const double rhom = a.r + b.r;
-    double uv = (una + unb) * rhom;
-    Acc r;
+template <typename R, int DIR, int KM>
+__device__ __forceinline__ Acc<R> pair_terms(const KPar &P, const PcR<R> &a, const PcR<R> &b, R pref) {
+    const R una = un_of<DIR>(a), unb = un_of<DIR>(b);
+    const R rhom = a.r + b.r;
+    R uv = (una + unb) * rhom;
+    Acc<R> r;
     if (KM >= 0) {
-        const double eem = a.e + b.e;
-        double deef = 1.0;
+        const R eem = a.e + b.e;
+        R deef = R(1.0);
         if (KM > 0) {
-            const double drho = 2.0 * (b.r - a.r) / rhom;
-            const double dee = 2.0 * (b.e - a.e) / eem;
-            double t_sdr = (0.5 * drho) * (0.5 * drho);
-            double t_sde = (0.5 * dee) * (0.5 * dee);
-            const double t2_sdr = t_sdr, t2_sde = t_sde;
-            double sdr = 1.0 + t_sdr / 3.0, sde = 1.0 + t_sde, sne = 1.0 + t_sde / 3.0;
+            const R drho = R(2.0) * (b.r - a.r) / rhom;
+            const R dee = R(2.0) * (b.e - a.e) / eem;
+            R t_sdr = (R(0.5) * drho) * (R(0.5) * drho);
+            R t_sde = (R(0.5) * dee) * (R(0.5) * dee);
+            const R t2_sdr = t_sdr, t2_sde = t_sde;
+            R sdr = R(1.0) + t_sdr / R(3.0), sde = R(1.0) + t_sde, sne = R(1.0) + t_sde / R(3.0);
             for (int n = 2; n <= P.nkeep; n++) {
                 const int n2 = 2 * n;
                 t_sdr = t2_sdr * t_sdr;
                 t_sde = t2_sde * t_sde;
-                sdr = sdr + t_sdr / (1.0 + n2);
+                sdr = sdr + t_sdr / (R(1.0) + R(n2));
                 sde = sde + t_sde;
-                sne = sne + t_sde / (1.0 + n2);
+                sne = sne + t_sde / (R(1.0) + R(n2));
             }
-            uv = uv * (1.0 / sdr);
+            uv = uv * (R(1.0) / sdr);
             deef = sne / sde;
         }
-        r.e = uv * eem * deef + uv * (a.u * b.u + a.v * b.v + a.w * b.w) + 4.0 * (una * b.p + unb * a.p);
+        if constexpr (sizeof(R) == sizeof(double))
+            r.e = uv * eem * deef + uv * (a.u * b.u + a.v * b.v + a.w * b.w) + R(4.0) * (una * b.p + unb * a.p);
+        else
+            r.e = uv * eem * deef + uv * (a.u * b.u + a.v * b.v + a.w * b.w) +
+                  R(4.0) * (una * (b.p + pref) + unb * (a.p + pref));
     } else {
         r.e = uv * (a.e + b.e);
     }
-    r.m = uv * 2.0;
+    r.m = uv * R(2.0);
     r.mu = uv * (a.u + b.u);
     r.mv = uv * (a.v + b.v);
     r.mw = uv * (a.w + b.w);
-    r.pp = 2.0 * (a.p + b.p);
+    r.pp = R(2.0) * (a.p + b.p);
     return r;
 }
 
-__device__ __forceinline__ void acc_add(Acc &A, double c, const Acc &p) {
+template <typename R>
+__device__ __forceinline__ void acc_add(Acc<R> &A, R c, const Acc<R> &p) {
     A.m += c * p.m;
     A.mu += c * p.mu;
     A.mv += c * p.mv;
@@ -294,22 +306,24 @@ __device__ __forceinline__ void acc_add(Acc &A, double c, const Acc &p) {
     A.pp += c * p.pp;
 }
 
-__device__ __forceinline__ Acc acc_zero() {
-    Acc a;
-    a.m = a.mu = a.mv = a.mw = a.e = a.pp = 0.0;
+template <typename R>
+__device__ __forceinline__ Acc<R> acc_zero() {
+    Acc<R> a;
+    a.m = a.mu = a.mv = a.mw = a.e = a.pp = R(0.0);
     return a;
 }
 
 // coeff_deriv1(l, lmax) of a face with effective half width lm (1..3)
-__device__ __forceinline__ double face_coeff(const KPar &P, int lm, int l) {
-    if (l > lm) return 0.0;
-    return lm == 3 ? P.cd1tab[2][l - 1] : (lm == 2 ? P.cd1tab[1][l - 1] : P.cd1tab[0][l - 1]);
+template <typename R>
+__device__ __forceinline__ R face_coeff(const KPar &P, int lm, int l) {
+    if (l > lm) return R(0.0);
+    return R(lm == 3 ? P.cd1tab[2][l - 1] : (lm == 2 ? P.cd1tab[1][l - 1] : P.cd1tab[0][l - 1]));
 }
 
-template <int DIR, int KM>
-__global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double *__restrict__ wa,
+template <typename R, int DIR, int KM>
+__global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__restrict__ wa,
                                                      const int *__restrict__ eoc, const double *__restrict__ metric,
-                                                     double *__restrict__ fl, int chunk, int nchunk,
+                                                     R *__restrict__ fl, int chunk, int nchunk,
                                                      unsigned char *__restrict__ flags) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int no = DIR == 1 ? g.nz : g.ny;  // the other transverse extent
@@ -326,7 +340,9 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double
     const int stride = DIR == 1 ? g.sx : g.sxy;
     const size_t nv = g.nvol;
     const bool keep = KM >= 0;
-    const double thr = P.sensor_threshold;
+    const R thr = R(P.sensor_threshold);
+    const R rgas0 = R(P.rgas0), pref = R(P.p_ref);
+    constexpr int TS = tp_slot<R>();
     const int ws = P.weno_scheme;
     // bit s of hist = sensor flag of point b-s; a face f = b-3 uses points
     // f-ws+1 .. f+ws, i.e. bits 3-ws .. 2+ws
@@ -338,79 +354,79 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double
         const int v = lb + eoc[ff];
         return v < 1 ? 1 : v;
     };
-    Pc q0, q1, q2;
+    PcR<R> q0, q1, q2;
     unsigned hist = 0;
     {
         int a = p0 + (c0 - 3) * stride;
-        q0 = make_pc(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0, keep);
+        q0 = make_pcr<R>(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[TS * nv + a], rgas0, pref, keep);
         hist |= (wa[7 * nv + a] > thr ? 1u : 0u) << 2;
         a += stride;
-        q1 = make_pc(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0, keep);
+        q1 = make_pcr<R>(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[TS * nv + a], rgas0, pref, keep);
         hist |= (wa[7 * nv + a] > thr ? 1u : 0u) << 1;
         a += stride;
-        q2 = make_pc(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0, keep);
+        q2 = make_pcr<R>(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[TS * nv + a], rgas0, pref, keep);
         hist |= (wa[7 * nv + a] > thr ? 1u : 0u);
     }
-    Acc A0 = acc_zero(), A1 = acc_zero(), A2 = acc_zero();  // faces b-3, b-2, b-1
+    Acc<R> A0 = acc_zero<R>(), A1 = acc_zero<R>(), A2 = acc_zero<R>();  // faces b-3, b-2, b-1
     int lm0 = lm_of(c0 - 3), lm1 = lm_of(c0 - 2), lm2 = lm_of(c0 - 1);
-    double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    R fprev[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
     bool wprev = false;
     // prefetch of the next point
     int a = p0 + c0 * stride;
-    double nr = wa[a], nu = wa[nv + a], nvv = wa[2 * nv + a], nw = wa[3 * nv + a], nh = wa[4 * nv + a],
-           nt = wa[5 * nv + a], ns = wa[7 * nv + a];
+    R nr = wa[a], nu = wa[nv + a], nvv = wa[2 * nv + a], nw = wa[3 * nv + a], nh = wa[4 * nv + a],
+      nt = wa[TS * nv + a], ns = wa[7 * nv + a];
     for (int b = c0; b <= c1 + 3; b++) {
         // issue the read of fl at the cell completed in this iteration first,
         // so that its latency overlaps the pair arithmetic below
-        double flv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        R flv[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
         const int fc = b - 3;
         const int oic = DIR == 1 ? iidx(g, i, fc < 1 ? 1 : fc, o) : iidx(g, i, o, fc < 1 ? 1 : fc);
         if (fc >= c0) {
 #pragma unroll
             for (int m = 0; m < 5; m++) flv[m] = fl[(size_t)m * g.nint + oic];
         }
-        const Pc qb = make_pc(nr, nu, nvv, nw, nh, nt, P.rgas0, keep);
+        const PcR<R> qb = make_pcr<R>(nr, nu, nvv, nw, nh, nt, rgas0, pref, keep);
         hist = (hist << 1) | (ns > thr ? 1u : 0u);
         if (b < c1 + 3) {
             a += stride;
             nr = wa[a]; nu = wa[nv + a]; nvv = wa[2 * nv + a]; nw = wa[3 * nv + a];
-            nh = wa[4 * nv + a]; nt = wa[5 * nv + a]; ns = wa[7 * nv + a];
+            nh = wa[4 * nv + a]; nt = wa[TS * nv + a]; ns = wa[7 * nv + a];
         }
         {
-            const Acc p3 = pair_terms<DIR, KM>(P, q0, qb);
-            acc_add(A0, face_coeff(P, lm0, 3), p3);
-            acc_add(A1, face_coeff(P, lm1, 3), p3);
-            acc_add(A2, face_coeff(P, lm2, 3), p3);
+            const Acc<R> p3 = pair_terms<R, DIR, KM>(P, q0, qb, pref);
+            acc_add(A0, face_coeff<R>(P, lm0, 3), p3);
+            acc_add(A1, face_coeff<R>(P, lm1, 3), p3);
+            acc_add(A2, face_coeff<R>(P, lm2, 3), p3);
         }
         {
-            const Acc p2 = pair_terms<DIR, KM>(P, q1, qb);
-            acc_add(A1, face_coeff(P, lm1, 2), p2);
-            acc_add(A2, face_coeff(P, lm2, 2), p2);
+            const Acc<R> p2 = pair_terms<R, DIR, KM>(P, q1, qb, pref);
+            acc_add(A1, face_coeff<R>(P, lm1, 2), p2);
+            acc_add(A2, face_coeff<R>(P, lm2, 2), p2);
         }
         {
-            const Acc p1 = pair_terms<DIR, KM>(P, q2, qb);
-            acc_add(A2, face_coeff(P, lm2, 1), p1);
+            const Acc<R> p1 = pair_terms<R, DIR, KM>(P, q2, qb, pref);
+            acc_add(A2, face_coeff<R>(P, lm2, 1), p1);
         }
         const int f = b - 3;  // face complete
         if (f >= c0 - 1) {
             // (the fl values of cell f were loaded at the top of the iteration)
             const bool wcur = (hist & smask) != 0u;
-            double fh[5];
-            fh[0] = 0.25 * A0.m;
-            fh[1] = 0.25 * A0.mu;
-            fh[2] = 0.25 * A0.mv;
-            fh[3] = 0.25 * A0.mw;
-            fh[4] = 0.25 * A0.e;
+            R fh[5];
+            fh[0] = R(0.25) * A0.m;
+            fh[1] = R(0.25) * A0.mu;
+            fh[2] = R(0.25) * A0.mv;
+            fh[3] = R(0.25) * A0.mw;
+            fh[4] = R(0.25) * A0.e;
             if ((f == 0 && P.fzf[2 * DIR] == 1) || (f == n && P.fzf[2 * DIR + 1] == 1)) {
 #pragma unroll
-                for (int m = 0; m < 5; m++) fh[m] = 0.0;
+                for (int m = 0; m < 5; m++) fh[m] = R(0.0);
             }
-            fh[1 + DIR] = fh[1 + DIR] + 0.5 * A0.pp;
+            fh[1 + DIR] = fh[1 + DIR] + R(0.5) * A0.pp;
             bool dirty = false;
             if (f >= c0) {
                 dirty = wcur || wprev;
                 if (!dirty) {
-                    const double d = metric[f - 1];
+                    const R d = R(metric[f - 1]);
 #pragma unroll
                     for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oic] = flv[m] + (fh[m] - fprev[m]) * d;
                 }
@@ -422,7 +438,7 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double
         }
         A0 = A1;
         A1 = A2;
-        A2 = acc_zero();
+        A2 = acc_zero<R>();
         lm0 = lm1;
         lm1 = lm2;
         lm2 = lm_of(b);
@@ -443,8 +459,9 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const double
 // pair sums regrouped by pair).  Faces 59 per wave, cells 58.
 constexpr int XW = 58;
 
-__device__ __forceinline__ Pc shfl_up_pc(const Pc &q, int d) {
-    Pc r;
+template <typename R>
+__device__ __forceinline__ PcR<R> shfl_up_pc(const PcR<R> &q, int d) {
+    PcR<R> r;
     r.r = __shfl_up(q.r, d, 64);
     r.u = __shfl_up(q.u, d, 64);
     r.v = __shfl_up(q.v, d, 64);
@@ -454,8 +471,9 @@ __device__ __forceinline__ Pc shfl_up_pc(const Pc &q, int d) {
     return r;
 }
 
-__device__ __forceinline__ Acc shfl_down_acc(const Acc &q, int d) {
-    Acc r;
+template <typename R>
+__device__ __forceinline__ Acc<R> shfl_down_acc(const Acc<R> &q, int d) {
+    Acc<R> r;
     r.m = __shfl_down(q.m, d, 64);
     r.mu = __shfl_down(q.mu, d, 64);
     r.mv = __shfl_down(q.mv, d, 64);
@@ -465,10 +483,10 @@ __device__ __forceinline__ Acc shfl_down_acc(const Acc &q, int d) {
     return r;
 }
 
-template <bool INIT, int KM>
-__global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const double *__restrict__ wa,
+template <typename R, bool INIT, int KM>
+__global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const R *__restrict__ wa,
                                                  const int *__restrict__ eoc, const double *__restrict__ dcsidx,
-                                                 double *__restrict__ fl, unsigned char *__restrict__ flags) {
+                                                 R *__restrict__ fl, unsigned char *__restrict__ flags) {
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -479,6 +497,7 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const double *__
     const size_t nv = g.nvol;
     const int ws = P.weno_scheme;
     const unsigned long long wmask = (1ull << (2 * ws)) - 1ull;
+    const R pref = R(P.p_ref);
     const int lb = P.lmax_base;
     auto lm_of = [&](int f) {
         const int ff = f < 0 ? 0 : (f > g.nx ? g.nx : f);
@@ -493,53 +512,53 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const double *__
         const int pt = i0 - 3 + lane;                     // point held by this lane
         const int pl = pt > g.nx + 3 ? g.nx + 3 : pt;     // clamp the load into the ghosted row
         const int a = gidx(g, pl, j, k);
-        const Pc q = make_pc(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a],
-                             P.rgas0, keep);
-        const bool flag = wa[7 * nv + a] > P.sensor_threshold;
-        const Pc q1 = shfl_up_pc(q, 1), q2 = shfl_up_pc(q, 2), q3 = shfl_up_pc(q, 3);
+        const PcR<R> q = make_pcr<R>(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a],
+                                     wa[tp_slot<R>() * nv + a], R(P.rgas0), pref, keep);
+        const bool flag = wa[7 * nv + a] > R(P.sensor_threshold);
+        const PcR<R> q1 = shfl_up_pc(q, 1), q2 = shfl_up_pc(q, 2), q3 = shfl_up_pc(q, 3);
         const int lmA = lm_of(pt - 1), lmB = lm_of(pt - 2), lmC = lm_of(pt - 3);
-        Acc Q1 = acc_zero(), Q2 = acc_zero(), Q3 = acc_zero();
+        Acc<R> Q1 = acc_zero<R>(), Q2 = acc_zero<R>(), Q3 = acc_zero<R>();
         {
-            const Acc p3 = pair_terms<0, KM>(P, q3, q);
-            acc_add(Q1, face_coeff(P, lmA, 3), p3);
-            acc_add(Q2, face_coeff(P, lmB, 3), p3);
-            acc_add(Q3, face_coeff(P, lmC, 3), p3);
+            const Acc<R> p3 = pair_terms<R, 0, KM>(P, q3, q, pref);
+            acc_add(Q1, face_coeff<R>(P, lmA, 3), p3);
+            acc_add(Q2, face_coeff<R>(P, lmB, 3), p3);
+            acc_add(Q3, face_coeff<R>(P, lmC, 3), p3);
         }
         {
-            const Acc p2 = pair_terms<0, KM>(P, q2, q);
-            acc_add(Q1, face_coeff(P, lmA, 2), p2);
-            acc_add(Q2, face_coeff(P, lmB, 2), p2);
+            const Acc<R> p2 = pair_terms<R, 0, KM>(P, q2, q, pref);
+            acc_add(Q1, face_coeff<R>(P, lmA, 2), p2);
+            acc_add(Q2, face_coeff<R>(P, lmB, 2), p2);
         }
         {
-            const Acc p1 = pair_terms<0, KM>(P, q1, q);
-            acc_add(Q1, face_coeff(P, lmA, 1), p1);
+            const Acc<R> p1 = pair_terms<R, 0, KM>(P, q1, q, pref);
+            acc_add(Q1, face_coeff<R>(P, lmA, 1), p1);
         }
         // face f = pt: pairs from the lanes of points f+1, f+2, f+3
-        Acc F = shfl_down_acc(Q1, 1);
+        Acc<R> F = shfl_down_acc(Q1, 1);
         {
-            const Acc t2 = shfl_down_acc(Q2, 2);
+            const Acc<R> t2 = shfl_down_acc(Q2, 2);
             F.m += t2.m; F.mu += t2.mu; F.mv += t2.mv; F.mw += t2.mw; F.e += t2.e; F.pp += t2.pp;
-            const Acc t3 = shfl_down_acc(Q3, 3);
+            const Acc<R> t3 = shfl_down_acc(Q3, 3);
             F.m += t3.m; F.mu += t3.mu; F.mv += t3.mv; F.mw += t3.mw; F.e += t3.e; F.pp += t3.pp;
         }
         const int f = pt;
-        double fh[5];
-        fh[0] = 0.25 * F.m;
-        fh[1] = 0.25 * F.mu;
-        fh[2] = 0.25 * F.mv;
-        fh[3] = 0.25 * F.mw;
-        fh[4] = 0.25 * F.e;
+        R fh[5];
+        fh[0] = R(0.25) * F.m;
+        fh[1] = R(0.25) * F.mu;
+        fh[2] = R(0.25) * F.mv;
+        fh[3] = R(0.25) * F.mw;
+        fh[4] = R(0.25) * F.e;
         if ((f == 0 && P.fzf[0] == 1) || (f == g.nx && P.fzf[1] == 1)) {
 #pragma unroll
-            for (int m = 0; m < 5; m++) fh[m] = 0.0;
+            for (int m = 0; m < 5; m++) fh[m] = R(0.0);
         }
-        fh[1] = fh[1] + 0.5 * F.pp;
+        fh[1] = fh[1] + R(0.5) * F.pp;
         // WENO face: a sensor flag among points f-ws+1 .. f+ws
         const unsigned long long fm = __ballot(flag);
         const int sh = lane - ws + 1;
         const bool wf = lane >= 2 && lane <= 60 && ((fm >> (sh < 0 ? 0 : sh)) & wmask) != 0ull;
         // cell c = pt, needs face c-1 from the lane below
-        double fl_[5];
+        R fl_[5];
 #pragma unroll
         for (int m = 0; m < 5; m++) fl_[m] = __shfl_up(fh[m], 1, 64);
         const bool wl = __shfl_up(wf ? 1 : 0, 1, 64) != 0;
@@ -549,10 +568,10 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const double *__
         const bool dirty = in && (wf || wl);
         if (cell) {
             const int o = iidx(g, c, j, k);
-            const double d = dcsidx[c - 1];
+            const R d = R(dcsidx[c - 1]);
 #pragma unroll
             for (int m = 0; m < 5; m++) {
-                if (INIT) fl[(size_t)m * g.nint + o] = (in && !dirty) ? (fh[m] - fl_[m]) * d : 0.0;
+                if (INIT) fl[(size_t)m * g.nint + o] = (in && !dirty) ? (fh[m] - fl_[m]) * d : R(0.0);
                 else if (in && !dirty) fl[(size_t)m * g.nint + o] += (fh[m] - fl_[m]) * d;
             }
         }
@@ -590,10 +609,10 @@ __device__ __forceinline__ void hybrid_face(const KPar &P, const Pt (&q)[6], con
 // fl += (fhat(c) - fhat(c-1)) * metric(c), and the flag cleared.  One
 // 64-lane workgroup per grid-stride lane set; flagged cells are compacted
 // into an LDS queue with a ballot so the WENO work runs on full waves.
-template <int DIR>
-__device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const double *__restrict__ wa,
+template <typename R, int DIR>
+__device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const R *__restrict__ wa,
                                          const int *__restrict__ eoc, const double *__restrict__ metric,
-                                         double *__restrict__ fl, int oi) {
+                                         R *__restrict__ fl, int oi) {
     const size_t nv = g.nvol;
     const bool keep = P.nkeep >= 0;
     const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
@@ -608,9 +627,9 @@ __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const doub
 #pragma unroll
     for (int s = 0; s < 7; s++) {
         const int a = base + (s - 3) * stride;
-        q[s] = make_pt(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[5 * nv + a], P.rgas0,
-                       keep);
-        sen[s] = wa[7 * nv + a];
+        q[s] = make_pt(double(wa[a]), double(wa[nv + a]), double(wa[2 * nv + a]), double(wa[3 * nv + a]),
+                       double(wa[4 * nv + a]), double(wa[5 * nv + a]), P.rgas0, keep);
+        sen[s] = double(wa[7 * nv + a]);
     }
     double flo[5], fhi[5];
     {
@@ -629,13 +648,13 @@ __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const doub
     }
     const double d = metric[c - 1];
 #pragma unroll
-    for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] += (fhi[m] - flo[m]) * d;
+    for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] = R(double(fl[(size_t)m * g.nint + oi]) + (fhi[m] - flo[m]) * d);
 }
 
-template <int DIR>
-__global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const double *__restrict__ wa,
+template <typename R, int DIR>
+__global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const R *__restrict__ wa,
                                                  const int *__restrict__ eoc, const double *__restrict__ metric,
-                                                 double *__restrict__ fl, unsigned char *__restrict__ flags,
+                                                 R *__restrict__ fl, unsigned char *__restrict__ flags,
                                                  int region) {
     __shared__ int queue[64 * 5];
     const int lane = threadIdx.x;
@@ -668,7 +687,7 @@ __global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const double *__
                 if (qn >= 64 * 4) {  // keep room for the next byte position
                     __syncthreads();
                     while (qn >= 64) {
-                        fix_cell<DIR>(g, P, wa, eoc, metric, fl, queue[qn - 64 + lane]);
+                        fix_cell<R, DIR>(g, P, wa, eoc, metric, fl, queue[qn - 64 + lane]);
                         qn -= 64;
                     }
                     __syncthreads();
@@ -677,19 +696,20 @@ __global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const double *__
         }
         __syncthreads();
         while (qn >= 64) {
-            fix_cell<DIR>(g, P, wa, eoc, metric, fl, queue[qn - 64 + lane]);
+            fix_cell<R, DIR>(g, P, wa, eoc, metric, fl, queue[qn - 64 + lane]);
             qn -= 64;
         }
         __syncthreads();
     }
-    if (lane < qn) fix_cell<DIR>(g, P, wa, eoc, metric, fl, queue[lane]);
+    if (lane < qn) fix_cell<R, DIR>(g, P, wa, eoc, metric, fl, queue[lane]);
 }
 
 // ---------------------------------------------------------------- viscous
 // Ghost images of an interior value: periodic copy for periodic directions
 // (bcswap_var), zero-gradient copy otherwise (bcextr_var).
-__device__ __forceinline__ void write_images(const Geo &g, const KPar &P, double *__restrict__ a, int i, int j, int k,
-                                             double v, bool extrapolate) {
+template <typename R>
+__device__ __forceinline__ void write_images(const Geo &g, const KPar &P, R *__restrict__ a, int i, int j, int k,
+                                             R v, bool extrapolate) {
     const int ng = g.ng;
     if (P.periodic[0]) {
         if (i <= ng) a[gidx(g, i + g.nx, j, k)] = v;
@@ -715,11 +735,13 @@ __device__ __forceinline__ void write_images(const Geo &g, const KPar &P, double
 }
 
 // Point values of the viscous stencils: u, v, w, T, mu (w_aux 2, 3, 4, 6, 7).
+template <typename R>
 struct VP {
-    double u, v, w, t, m;
+    R u, v, w, t, m;
 };
-__device__ __forceinline__ VP load_vp(const double *__restrict__ wa, size_t nv, int c) {
-    VP p;
+template <typename R>
+__device__ __forceinline__ VP<R> load_vp(const R *__restrict__ wa, size_t nv, int c) {
+    VP<R> p;
     p.u = wa[nv + c];
     p.v = wa[2 * nv + c];
     p.w = wa[3 * nv + c];
@@ -741,8 +763,8 @@ constexpr int VBX = 32, VBY = 8;
 // each lane fills at most one x-halo and one y-halo point
 static_assert(VBX >= 2 * H && VBY >= 2 * H, "tile narrower than two halos");
 
-template <bool SENSOR>
-__global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__restrict__ wa, double *__restrict__ fl,
+template <typename R, bool SENSOR>
+__global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, R *__restrict__ wa, R *__restrict__ fl,
                                                       const double *__restrict__ dx1, const double *__restrict__ dxs,
                                                       const double *__restrict__ dx2, const double *__restrict__ dy1,
                                                       const double *__restrict__ dys, const double *__restrict__ dy2,
@@ -750,7 +772,7 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__r
                                                       const double *__restrict__ dz2, double *__restrict__ wallprop,
                                                       int images, int chunk) {
     // two plane buffers: plane k is read while plane k+1 is written
-    __shared__ double tiles[2][5][VBY + 2 * H][VBX + 2 * H];
+    __shared__ R tiles[2][5][VBY + 2 * H][VBX + 2 * H];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i = blockIdx.x * VBX + tx + 1;
     const int j = blockIdx.y * VBY + ty + 1;
@@ -770,21 +792,21 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__r
     const int colx = hx ? gidx(g, tx < H ? i - H : min(i + H, g.nx + H), jj, 0) : col;
     const int coly = hy ? gidx(g, ii, ty < H ? j - H : min(j + H, g.ny + H), 0) : col;
     const int lx = tx < H ? tx : tx + 2 * H, ly = ty < H ? ty : ty + 2 * H;
-    VP win[2 * H + 1];
+    VP<R> win[2 * H + 1];
 #pragma unroll
     for (int q = 0; q < 2 * H; q++) win[q + 1] = load_vp(wa, nv, col + (k0 - H + q) * sz);
-    VP nxt = load_vp(wa, nv, col + (k0 + H) * sz);
-    VP hxv = load_vp(wa, nv, colx + k0 * sz), hyv = load_vp(wa, nv, coly + k0 * sz);
-    const double ax = dx1[min(i, g.nx) - 1], ay = dy1[min(j, g.ny) - 1];
-    const double sx_ = dxs[min(i, g.nx) - 1], sy_ = dys[min(j, g.ny) - 1];
-    const double qx = dx2[min(i, g.nx) - 1], qy = dy2[min(j, g.ny) - 1];
+    VP<R> nxt = load_vp(wa, nv, col + (k0 + H) * sz);
+    VP<R> hxv = load_vp(wa, nv, colx + k0 * sz), hyv = load_vp(wa, nv, coly + k0 * sz);
+    const R ax = R(dx1[min(i, g.nx) - 1]), ay = R(dy1[min(j, g.ny) - 1]);
+    const R sx_ = R(dxs[min(i, g.nx) - 1]), sy_ = R(dys[min(j, g.ny) - 1]);
+    const R qx = R(dx2[min(i, g.nx) - 1]), qy = R(dy2[min(j, g.ny) - 1]);
     int buf = 0;
     for (int k = k0; k <= k1; k++, buf ^= 1) {
 #pragma unroll
         for (int q = 0; q < 2 * H; q++) win[q] = win[q + 1];
         win[2 * H] = nxt;
-        const VP &c0 = win[H];
-        double(*tile)[VBY + 2 * H][VBX + 2 * H] = tiles[buf];
+        const VP<R> &c0 = win[H];
+        R(*tile)[VBY + 2 * H][VBX + 2 * H] = tiles[buf];
         tile[0][ty + H][tx + H] = c0.u;
         tile[1][ty + H][tx + H] = c0.v;
         tile[2][ty + H][tx + H] = c0.w;
@@ -801,7 +823,7 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__r
         // right-hand side of this plane, read ahead of the stencil work
         const int oi = iidx(g, min(i, g.nx), min(j, g.ny), k);
         const size_t ni = g.nint;
-        const double fo2 = fl[ni + oi], fo3 = fl[2 * ni + oi], fo4 = fl[3 * ni + oi], fo5 = fl[4 * ni + oi];
+        const R fo2 = fl[ni + oi], fo3 = fl[2 * ni + oi], fo4 = fl[3 * ni + oi], fo5 = fl[4 * ni + oi];
         // loads of plane k+1 overlap the work on plane k
         if (k < k1) {
             nxt = load_vp(wa, nv, col + (k + 1 + H) * sz);
@@ -810,19 +832,19 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__r
         }
         __syncthreads();
         if (act) {
-            const double uu = c0.u, vv = c0.v, ww = c0.w, tt = c0.t, mu = c0.m;
-            double ux = 0, vx = 0, wx = 0, tx_ = 0, mux = 0;
-            double uy = 0, vy = 0, wy = 0, ty_ = 0, muy = 0;
-            double uz = 0, vz = 0, wz = 0, tz = 0, muz = 0;
-            const double c20 = P.vd2[0];
-            double ulapx = c20 * uu, ulapy = ulapx, ulapz = ulapx;
-            double vlapx = c20 * vv, vlapy = vlapx, vlapz = vlapx;
-            double wlapx = c20 * ww, wlapy = wlapx, wlapz = wlapx;
-            double tlapx = c20 * tt, tlapy = tlapx, tlapz = tlapx;
+            const R uu = c0.u, vv = c0.v, ww = c0.w, tt = c0.t, mu = c0.m;
+            R ux = R(0), vx = R(0), wx = R(0), tx_ = R(0), mux = R(0);
+            R uy = R(0), vy = R(0), wy = R(0), ty_ = R(0), muy = R(0);
+            R uz = R(0), vz = R(0), wz = R(0), tz = R(0), muz = R(0);
+            const R c20 = R(P.vd2[0]);
+            R ulapx = c20 * uu, ulapy = ulapx, ulapz = ulapx;
+            R vlapx = c20 * vv, vlapy = vlapx, vlapz = vlapx;
+            R wlapx = c20 * ww, wlapy = wlapx, wlapz = wlapx;
+            R tlapx = c20 * tt, tlapy = tlapx, tlapz = tlapx;
 #pragma unroll
             for (int l = 1; l <= H; l++) {
                 if (l <= lmax) {
-                    const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
+                    const R ccl = R(P.vd1[l - 1]), clapl = R(P.vd2[l]);
                     const int r = ty + H, cp = tx + H + l, cm = tx + H - l;
                     ux = ux + ccl * (tile[0][r][cp] - tile[0][r][cm]);
                     ulapx = ulapx + clapl * (tile[0][r][cp] + tile[0][r][cm]);
@@ -838,7 +860,7 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__r
 #pragma unroll
             for (int l = 1; l <= H; l++) {
                 if (l <= lmax) {
-                    const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
+                    const R ccl = R(P.vd1[l - 1]), clapl = R(P.vd2[l]);
                     const int c = tx + H, rp = ty + H + l, rm = ty + H - l;
                     uy = uy + ccl * (tile[0][rp][c] - tile[0][rm][c]);
                     ulapy = ulapy + clapl * (tile[0][rp][c] + tile[0][rm][c]);
@@ -854,8 +876,8 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__r
 #pragma unroll
             for (int l = 1; l <= H; l++) {
                 if (l <= lmax) {
-                    const double ccl = P.vd1[l - 1], clapl = P.vd2[l];
-                    const VP &pp = win[H + l], &pm = win[H - l];
+                    const R ccl = R(P.vd1[l - 1]), clapl = R(P.vd2[l]);
+                    const VP<R> &pp = win[H + l], &pm = win[H - l];
                     uz = uz + ccl * (pp.u - pm.u);
                     ulapz = ulapz + clapl * (pp.u + pm.u);
                     vz = vz + ccl * (pp.v - pm.v);
@@ -867,46 +889,46 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__r
                     muz = muz + ccl * (pp.m - pm.m);
                 }
             }
-            const double az = dz1[k - 1];
+            const R az = R(dz1[k - 1]);
             ux = ux * ax; vx = vx * ax; wx = wx * ax; tx_ = tx_ * ax; mux = mux * ax;
             uy = uy * ay; vy = vy * ay; wy = wy * ay; ty_ = ty_ * ay; muy = muy * ay;
             uz = uz * az; vz = vz * az; wz = wz * az; tz = tz * az; muz = muz * az;
-            const double sz_ = dzs[k - 1], qz = dz2[k - 1];
+            const R sz_ = R(dzs[k - 1]), qz = R(dz2[k - 1]);
             ulapx = ulapx * sx_ + ux * qx; vlapx = vlapx * sx_ + vx * qx;
             wlapx = wlapx * sx_ + wx * qx; tlapx = tlapx * sx_ + tx_ * qx;
             ulapy = ulapy * sy_ + uy * qy; vlapy = vlapy * sy_ + vy * qy;
             wlapy = wlapy * sy_ + wy * qy; tlapy = tlapy * sy_ + ty_ * qy;
             ulapz = ulapz * sz_ + uz * qz; vlapz = vlapz * sz_ + vz * qz;
             wlapz = wlapz * sz_ + wz * qz; tlapz = tlapz * sz_ + tz * qz;
-            const double ulap = ulapx + ulapy + ulapz;
-            const double vlap = vlapx + vlapy + vlapz;
-            const double wlap = wlapx + wlapy + wlapz;
-            const double tlap = tlapx + tlapy + tlapz;
-            const double div = ux + vy + wz;
-            const double div3l = div / 3.0;
+            const R ulap = ulapx + ulapy + ulapz;
+            const R vlap = vlapx + vlapy + vlapz;
+            const R wlap = wlapx + wlapy + wlapz;
+            const R tlap = tlapx + tlapy + tlapz;
+            const R div = ux + vy + wz;
+            const R div3l = div / R(3.0);
             const int c = gidx(g, i, j, k);
             wa[9 * nv + c] = div3l;
-            const double omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
-            const double omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
+            const R omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
+            const R omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
             wa[8 * nv + c] = sqrt(omod2);
             if (SENSOR) {
-                const double u0l0 = P.u0 / P.l0;
-                double sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
-                sv = fmax(sv, 0.0);
+                const R u0l0 = R(P.u0 / P.l0);
+                R sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
+                sv = fmax(sv, R(0.0));
                 wa[7 * nv + c] = sv * sv;
                 if (images) write_images(g, P, wa + 7 * nv, i, j, k, sv * sv, false);
             }
             if (images && !SENSOR) write_images(g, P, wa + 9 * nv, i, j, k, div3l, true);
-            const double cploc = cploc_of(P, tt);
-            const double sig11 = 2.0 * (ux - div3l), sig12 = uy + vx, sig13 = uz + wx;
-            const double sig22 = 2.0 * (vy - div3l), sig23 = vz + wy, sig33 = 2.0 * (wz - div3l);
-            const double sigx = mux * sig11 + muy * sig12 + muz * sig13 + mu * ulap;
-            const double sigy = mux * sig12 + muy * sig22 + muz * sig23 + mu * vlap;
-            const double sigz = mux * sig13 + muy * sig23 + muz * sig33 + mu * wlap;
-            const double sigqt = (mux * tx_ + muy * ty_ + muz * tz + mu * tlap) * cploc / P.prandtl;
-            const double sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig12 * vx + sig22 * vy + sig23 * vz +
+            const R cploc = R(cploc_of(P, double(tt)));
+            const R sig11 = R(2.0) * (ux - div3l), sig12 = uy + vx, sig13 = uz + wx;
+            const R sig22 = R(2.0) * (vy - div3l), sig23 = vz + wy, sig33 = R(2.0) * (wz - div3l);
+            const R sigx = mux * sig11 + muy * sig12 + muz * sig13 + mu * ulap;
+            const R sigy = mux * sig12 + muy * sig22 + muz * sig23 + mu * vlap;
+            const R sigz = mux * sig13 + muy * sig23 + muz * sig33 + mu * wlap;
+            const R sigqt = (mux * tx_ + muy * ty_ + muz * tz + mu * tlap) * cploc / R(P.prandtl);
+            const R sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig12 * vx + sig22 * vy + sig23 * vz +
                                   sig13 * wx + sig23 * wy + sig33 * wz) * mu;
-            const double sigq = sigx * uu + sigy * vv + sigz * ww + sigah + sigqt;
+            const R sigq = sigx * uu + sigy * vv + sigz * ww + sigah + sigqt;
             fl[ni + oi] = fo2 - sigx;
             fl[2 * ni + oi] = fo3 - sigy;
             fl[3 * ni + oi] = fo4 - sigz;
@@ -914,9 +936,9 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__r
             if (j == 1) {
                 const size_t wsz = (size_t)g.sx * g.sz;
                 const size_t wp = (size_t)(i + g.ng - 1) + (size_t)g.sx * (k + g.ng - 1);
-                wallprop[wp] = mu * uy;
-                wallprop[wsz + wp] = mu * wy;
-                wallprop[2 * wsz + wp] = mu * ty_ * cploc / P.prandtl;
+                wallprop[wp] = double(mu * uy);
+                wallprop[wsz + wp] = double(mu * wy);
+                wallprop[2 * wsz + wp] = double(mu * ty_ * cploc / R(P.prandtl));
             }
         }
     }
@@ -926,16 +948,17 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, double *__r
 // Face flux between points a (left) and b (right) at node distance h
 // (kernels_cpu.F90:4805-4836): mu_f (du, dv, dw) and
 // mu_f (cp/Pr dT + d(|u|^2/2)), mu_f = (mu_a + mu_b)/2/h.
-__device__ __forceinline__ void cons_face(const KPar &P, const VP &a, const VP &b, double h, double s[4]) {
-    const double qq = 0.5 * (a.u * a.u + a.v * a.v + a.w * a.w);
-    const double qqp = 0.5 * (b.u * b.u + b.v * b.v + b.w * b.w);
-    const double cploc = P.calorically_perfect == 1 ? P.cp[0 - P.indx_cp_l] : cploc_of(P, 0.5 * (a.t + b.t));
-    double muf = a.m + b.m;
-    muf = 0.5 * muf / h;
+template <typename R>
+__device__ __forceinline__ void cons_face(const KPar &P, const VP<R> &a, const VP<R> &b, R h, R s[4]) {
+    const R qq = R(0.5) * (a.u * a.u + a.v * a.v + a.w * a.w);
+    const R qqp = R(0.5) * (b.u * b.u + b.v * b.v + b.w * b.w);
+    const R cploc = R(P.calorically_perfect == 1 ? P.cp[0 - P.indx_cp_l] : cploc_of(P, 0.5 * double(a.t + b.t)));
+    R muf = a.m + b.m;
+    muf = R(0.5) * muf / h;
     s[0] = (b.u - a.u) * muf;
     s[1] = (b.v - a.v) * muf;
     s[2] = (b.w - a.w) * muf;
-    s[3] = ((b.t - a.t) * cploc / P.prandtl + (qqp - qq)) * muf;
+    s[3] = ((b.t - a.t) * cploc / R(P.prandtl) + (qqp - qq)) * muf;
 }
 
 // Conservative viscous terms of one cell.
@@ -948,8 +971,8 @@ __device__ __forceinline__ void cons_face(const KPar &P, const VP &a, const VP &
 //          reference's per-cell summation order.
 // ``images`` also writes the ghost images of w_aux(10) (bcextr_var mode 1
 // on walls, bcswap_var on periodic sides).
-template <bool CELL, bool FACES, bool SENSOR>
-__global__ __launch_bounds__(256) void k_visc_cons(Geo g, KPar P, double *__restrict__ wa, double *__restrict__ fl,
+template <typename R, bool CELL, bool FACES, bool SENSOR>
+__global__ __launch_bounds__(256) void k_visc_cons(Geo g, KPar P, R *__restrict__ wa, R *__restrict__ fl,
                                                    const double *__restrict__ xc, const double *__restrict__ yc,
                                                    const double *__restrict__ zc, double *__restrict__ wallprop,
                                                    int images) {
@@ -960,84 +983,84 @@ __global__ __launch_bounds__(256) void k_visc_cons(Geo g, KPar P, double *__rest
     const size_t nv = g.nvol;
     const int c = gidx(g, i, j, k);
     const int sy = g.sx, sz = g.sxy;
-    const VP p0 = load_vp(wa, nv, c);
-    const VP xm = load_vp(wa, nv, c - 1), xp = load_vp(wa, nv, c + 1);
-    const VP ym = load_vp(wa, nv, c - sy), yp = load_vp(wa, nv, c + sy);
-    const VP zm = load_vp(wa, nv, c - sz), zp = load_vp(wa, nv, c + sz);
+    const VP<R> p0 = load_vp(wa, nv, c);
+    const VP<R> xm = load_vp(wa, nv, c - 1), xp = load_vp(wa, nv, c + 1);
+    const VP<R> ym = load_vp(wa, nv, c - sy), yp = load_vp(wa, nv, c + sy);
+    const VP<R> zm = load_vp(wa, nv, c - sz), zp = load_vp(wa, nv, c + sz);
     const int ng = g.ng;
     const double xi_m = xc[i - 1 + ng - 1], xi = xc[i + ng - 1], xi_p = xc[i + 1 + ng - 1];
     const double yj_m = yc[j - 1 + ng - 1], yj = yc[j + ng - 1], yj_p = yc[j + 1 + ng - 1];
     const double zk_m = zc[k - 1 + ng - 1], zk = zc[k + ng - 1], zk_p = zc[k + 1 + ng - 1];
     const int oi = iidx(g, i, j, k);
     const size_t ni = g.nint;
-    double f2 = fl[ni + oi], f3 = fl[2 * ni + oi], f4 = fl[3 * ni + oi], f5 = fl[4 * ni + oi];
+    R f2 = fl[ni + oi], f3 = fl[2 * ni + oi], f4 = fl[3 * ni + oi], f5 = fl[4 * ni + oi];
     if (CELL) {
-        const double cploc = cploc_of(P, p0.t);
-        const double dxl = 1.0 / (xi_p - xi_m);
-        const double dyl = 1.0 / (yj_p - yj_m);
-        const double dzl = 1.0 / (zk_p - zk_m);
-        const double ux = dxl * (xp.u - xm.u), vx = dxl * (xp.v - xm.v), wx = dxl * (xp.w - xm.w);
-        const double mux = dxl * (xp.m - xm.m);
-        const double uy = dyl * (yp.u - ym.u), vy = dyl * (yp.v - ym.v), wy = dyl * (yp.w - ym.w);
-        const double ty = dyl * (yp.t - ym.t), muy = dyl * (yp.m - ym.m);
-        const double uz = dzl * (zp.u - zm.u), vz = dzl * (zp.v - zm.v), wz = dzl * (zp.w - zm.w);
-        const double muz = dzl * (zp.m - zm.m);
-        const double mu = p0.m;
+        const R cploc = R(cploc_of(P, double(p0.t)));
+        const R dxl = R(1.0) / R(xi_p - xi_m);
+        const R dyl = R(1.0) / R(yj_p - yj_m);
+        const R dzl = R(1.0) / R(zk_p - zk_m);
+        const R ux = dxl * (xp.u - xm.u), vx = dxl * (xp.v - xm.v), wx = dxl * (xp.w - xm.w);
+        const R mux = dxl * (xp.m - xm.m);
+        const R uy = dyl * (yp.u - ym.u), vy = dyl * (yp.v - ym.v), wy = dyl * (yp.w - ym.w);
+        const R ty = dyl * (yp.t - ym.t), muy = dyl * (yp.m - ym.m);
+        const R uz = dzl * (zp.u - zm.u), vz = dzl * (zp.v - zm.v), wz = dzl * (zp.w - zm.w);
+        const R muz = dzl * (zp.m - zm.m);
+        const R mu = p0.m;
         if (j == 1) {
             const size_t wsz = (size_t)g.sx * g.sz;
             const size_t wp = (size_t)(i + ng - 1) + (size_t)g.sx * (k + ng - 1);
-            wallprop[wp] = mu * uy;
-            wallprop[wsz + wp] = mu * wy;
-            wallprop[2 * wsz + wp] = mu * ty * cploc / P.prandtl;
+            wallprop[wp] = double(mu * uy);
+            wallprop[wsz + wp] = double(mu * wy);
+            wallprop[2 * wsz + wp] = double(mu * ty * cploc / R(P.prandtl));
         }
-        const double div = ux + vy + wz;
-        const double div3l = div / 3.0;
+        const R div = ux + vy + wz;
+        const R div3l = div / R(3.0);
         wa[9 * nv + c] = div3l;
         if (images) write_images(g, P, wa + 9 * nv, i, j, k, div3l, true);
-        const double omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
-        const double omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
+        const R omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
+        const R omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
         wa[8 * nv + c] = sqrt(omod2);
         if (SENSOR) {
-            const double u0l0 = P.u0 / P.l0;
-            double sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
-            sv = fmax(sv, 0.0);
+            const R u0l0 = R(P.u0 / P.l0);
+            R sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
+            sv = fmax(sv, R(0.0));
             wa[7 * nv + c] = sv * sv;
         }
-        const double sig11 = ux - 2.0 * div3l, sig12 = vx, sig13 = wx;
-        const double sig21 = uy, sig22 = vy - 2.0 * div3l, sig23 = wy;
-        const double sig31 = uz, sig32 = vz, sig33 = wz - 2.0 * div3l;
-        const double sigx = mux * sig11 + muy * sig12 + muz * sig13;
-        const double sigy = mux * sig21 + muy * sig22 + muz * sig23;
-        const double sigz = mux * sig31 + muy * sig32 + muz * sig33;
-        const double sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig21 * vx + sig22 * vy + sig23 * vz +
+        const R sig11 = ux - R(2.0) * div3l, sig12 = vx, sig13 = wx;
+        const R sig21 = uy, sig22 = vy - R(2.0) * div3l, sig23 = wy;
+        const R sig31 = uz, sig32 = vz, sig33 = wz - R(2.0) * div3l;
+        const R sigx = mux * sig11 + muy * sig12 + muz * sig13;
+        const R sigy = mux * sig21 + muy * sig22 + muz * sig23;
+        const R sigz = mux * sig31 + muy * sig32 + muz * sig33;
+        const R sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig21 * vx + sig22 * vy + sig23 * vz +
                               sig31 * wx + sig32 * wy + sig33 * wz) * mu;
-        const double sigq = sigx * p0.u + sigy * p0.v + sigz * p0.w + sigah;
+        const R sigq = sigx * p0.u + sigy * p0.v + sigz * p0.w + sigah;
         f2 = f2 - sigx;
         f3 = f3 - sigy;
         f4 = f4 - sigz;
         f5 = f5 - sigq;
     }
     if (FACES) {
-        double sl[4], sr[4];
+        R sl[4], sr[4];
         // x: fhat = -flux, fl += dxhl*(fhat(i) - fhat(i-1))
-        cons_face(P, xm, p0, xi - xi_m, sl);
-        cons_face(P, p0, xp, xi_p - xi, sr);
-        const double dxhl = 2.0 / (xi_p - xi_m);
+        cons_face(P, xm, p0, R(xi - xi_m), sl);
+        cons_face(P, p0, xp, R(xi_p - xi), sr);
+        const R dxhl = R(2.0) / R(xi_p - xi_m);
         f2 = f2 + dxhl * (-sr[0] - -sl[0]);
         f3 = f3 + dxhl * (-sr[1] - -sl[1]);
         f4 = f4 + dxhl * (-sr[2] - -sl[2]);
         f5 = f5 + dxhl * (-sr[3] - -sl[3]);
         // y: fl = fl + flux(j-1/2)*dyhl - flux(j+1/2)*dyhl
-        cons_face(P, ym, p0, yj - yj_m, sl);
-        cons_face(P, p0, yp, yj_p - yj, sr);
-        const double dyhl = 2.0 / (yj_p - yj_m);
+        cons_face(P, ym, p0, R(yj - yj_m), sl);
+        cons_face(P, p0, yp, R(yj_p - yj), sr);
+        const R dyhl = R(2.0) / R(yj_p - yj_m);
         f2 = f2 + sl[0] * dyhl - sr[0] * dyhl;
         f3 = f3 + sl[1] * dyhl - sr[1] * dyhl;
         f4 = f4 + sl[2] * dyhl - sr[2] * dyhl;
         f5 = f5 + sl[3] * dyhl - sr[3] * dyhl;
-        cons_face(P, zm, p0, zk - zk_m, sl);
-        cons_face(P, p0, zp, zk_p - zk, sr);
-        const double dzhl = 2.0 / (zk_p - zk_m);
+        cons_face(P, zm, p0, R(zk - zk_m), sl);
+        cons_face(P, p0, zp, R(zk_p - zk), sr);
+        const R dzhl = R(2.0) / R(zk_p - zk_m);
         f2 = f2 + sl[0] * dzhl - sr[0] * dzhl;
         f3 = f3 + sl[1] * dzhl - sr[1] * dzhl;
         f4 = f4 + sl[2] * dzhl - sr[2] * dzhl;
@@ -1050,19 +1073,20 @@ __global__ __launch_bounds__(256) void k_visc_cons(Geo g, KPar P, double *__rest
 }
 
 // sensor_subroutine (kernels_cpu.F90:4726-4744) + bcswap_var of w_aux(8)
-__global__ __launch_bounds__(256) void k_sensor(Geo g, KPar P, double *__restrict__ wa, int images) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_sensor(Geo g, KPar P, R *__restrict__ wa, int images) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.nx || j > g.ny) return;
     const size_t nv = g.nvol;
     const int c = gidx(g, i, j, k);
-    const double om = wa[8 * nv + c];
-    const double omod2 = om * om;
-    const double div = 3.0 * wa[9 * nv + c];
-    const double u0l0 = P.u0 / P.l0;
-    double sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
-    sv = fmax(sv, 0.0);
+    const R om = wa[8 * nv + c];
+    const R omod2 = om * om;
+    const R div = R(3.0) * wa[9 * nv + c];
+    const R u0l0 = R(P.u0 / P.l0);
+    R sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
+    sv = fmax(sv, R(0.0));
     wa[7 * nv + c] = sv * sv;
     if (images) write_images(g, P, wa + 7 * nv, i, j, k, sv * sv, false);
 }
@@ -1071,9 +1095,9 @@ __global__ __launch_bounds__(256) void k_sensor(Geo g, KPar P, double *__restric
 // kernel also forms fln = -rho_k dt F[prev] - gamma_k dt F[cur] of
 // update_flux (:3400-3418) for rho and rho*u and writes the block partial
 // sums of force_rhs_1_subroutine (:3157-3193).
-template <bool REDUCE>
-__global__ __launch_bounds__(256) void k_visc_div(Geo g, KPar P, const double *__restrict__ wa, double *__restrict__ fl,
-                                                  const double *__restrict__ flp, const double *__restrict__ w,
+template <typename R, bool REDUCE>
+__global__ __launch_bounds__(256) void k_visc_div(Geo g, KPar P, const R *__restrict__ wa, R *__restrict__ fl,
+                                                  const R *__restrict__ flp, const double *__restrict__ w,
                                                   const double *__restrict__ dx1, const double *__restrict__ dy1,
                                                   const double *__restrict__ dz1, const double *__restrict__ yn,
                                                   const DevScalars *__restrict__ ds, double rkr, double rkg,
@@ -1086,27 +1110,27 @@ __global__ __launch_bounds__(256) void k_visc_div(Geo g, KPar P, const double *_
     if (i <= g.nx && j <= g.ny) {
         const size_t nv = g.nvol;
         const int c = gidx(g, i, j, k);
-        const double *D = wa + 9 * nv;
-        const double uu = wa[nv + c], vv = wa[2 * nv + c], ww = wa[3 * nv + c], mu = wa[6 * nv + c];
-        double dxv = 0.0, dyv = 0.0, dzv = 0.0;
+        const R *D = wa + 9 * nv;
+        const R uu = wa[nv + c], vv = wa[2 * nv + c], ww = wa[3 * nv + c], mu = wa[6 * nv + c];
+        R dxv = R(0.0), dyv = R(0.0), dzv = R(0.0);
         const int sy = g.sx, sz = g.sxy;
 #pragma unroll
         for (int l = 1; l <= 4; l++) {
             if (l <= P.visc_lmax) {
-                const double ccl = P.vd1[l - 1];
+                const R ccl = R(P.vd1[l - 1]);
                 dxv = dxv + ccl * (D[c + l] - D[c - l]);
                 dyv = dyv + ccl * (D[c + l * sy] - D[c - l * sy]);
                 dzv = dzv + ccl * (D[c + l * sz] - D[c - l * sz]);
             }
         }
-        dxv = dxv * dx1[i - 1];
-        dyv = dyv * dy1[j - 1];
-        dzv = dzv * dz1[k - 1];
-        const double sigx = mu * dxv, sigy = mu * dyv, sigz = mu * dzv;
-        const double sigq = sigx * uu + sigy * vv + sigz * ww;
+        dxv = dxv * R(dx1[i - 1]);
+        dyv = dyv * R(dy1[j - 1]);
+        dzv = dzv * R(dz1[k - 1]);
+        const R sigx = mu * dxv, sigy = mu * dyv, sigz = mu * dzv;
+        const R sigq = sigx * uu + sigy * vv + sigz * ww;
         const int oi = iidx(g, i, j, k);
         const size_t ni = g.nint;
-        const double f2 = fl[ni + oi] - sigx;
+        const R f2 = fl[ni + oi] - sigx;
         fl[ni + oi] = f2;
         fl[2 * ni + oi] -= sigy;
         fl[3 * ni + oi] -= sigz;
@@ -1115,14 +1139,14 @@ __global__ __launch_bounds__(256) void k_visc_div(Geo g, KPar P, const double *_
             const double dt = ds->dt;
             const double rhodt = rkr * dt, gamdt = rkg * dt;
             const double dy = yn[j] - yn[j - 1];
-            const double fln1 = -rhodt * flp[oi] - gamdt * fl[oi];
-            const double fln2 = -rhodt * flp[ni + oi] - gamdt * f2;
+            const double fln1 = -rhodt * double(flp[oi]) - gamdt * double(fl[oi]);
+            const double fln2 = -rhodt * double(flp[ni + oi]) - gamdt * double(f2);
             const double w1 = w[c], w2 = w[nv + c];
             acc[0] = fln1 * dy;
             acc[1] = fln2 * dy;
             acc[2] = w1 * dy;
             acc[3] = w2 * dy;
-            acc[4] = w2 * dy * wa[5 * nv + c];
+            acc[4] = w2 * dy * double(wa[5 * nv + c]);
         }
     }
     if (REDUCE) {
@@ -1137,9 +1161,9 @@ __global__ __launch_bounds__(256) void k_visc_div(Geo g, KPar P, const double *_
 
 // visflx_div_ord2_subroutine (kernels_cpu.F90:3446-3480, visflx mode 7);
 // REDUCE as in k_visc_div.
-template <bool REDUCE>
-__global__ __launch_bounds__(256) void k_visc_div2(Geo g, KPar P, const double *__restrict__ wa, double *__restrict__ fl,
-                                                   const double *__restrict__ flp, const double *__restrict__ w,
+template <typename R, bool REDUCE>
+__global__ __launch_bounds__(256) void k_visc_div2(Geo g, KPar P, const R *__restrict__ wa, R *__restrict__ fl,
+                                                   const R *__restrict__ flp, const double *__restrict__ w,
                                                    const double *__restrict__ xc, const double *__restrict__ yc,
                                                    const double *__restrict__ zc, const double *__restrict__ yn,
                                                    const DevScalars *__restrict__ ds, double rkr, double rkg,
@@ -1152,19 +1176,19 @@ __global__ __launch_bounds__(256) void k_visc_div2(Geo g, KPar P, const double *
     if (i <= g.nx && j <= g.ny) {
         const size_t nv = g.nvol;
         const int c = gidx(g, i, j, k);
-        const double *D = wa + 9 * nv;
-        const double uu = wa[nv + c], vv = wa[2 * nv + c], ww = wa[3 * nv + c], mu = wa[6 * nv + c];
+        const R *D = wa + 9 * nv;
+        const R uu = wa[nv + c], vv = wa[2 * nv + c], ww = wa[3 * nv + c], mu = wa[6 * nv + c];
         const int sy = g.sx, sz = g.sxy, ng = g.ng;
-        const double dxl = mu / (xc[i + 1 + ng - 1] - xc[i - 1 + ng - 1]);
-        const double dyl = mu / (yc[j + 1 + ng - 1] - yc[j - 1 + ng - 1]);
-        const double dzl = mu / (zc[k + 1 + ng - 1] - zc[k - 1 + ng - 1]);
-        const double sigx = dxl * (D[c + 1] - D[c - 1]);
-        const double sigy = dyl * (D[c + sy] - D[c - sy]);
-        const double sigz = dzl * (D[c + sz] - D[c - sz]);
-        const double sigq = sigx * uu + sigy * vv + sigz * ww;
+        const R dxl = mu / R(xc[i + 1 + ng - 1] - xc[i - 1 + ng - 1]);
+        const R dyl = mu / R(yc[j + 1 + ng - 1] - yc[j - 1 + ng - 1]);
+        const R dzl = mu / R(zc[k + 1 + ng - 1] - zc[k - 1 + ng - 1]);
+        const R sigx = dxl * (D[c + 1] - D[c - 1]);
+        const R sigy = dyl * (D[c + sy] - D[c - sy]);
+        const R sigz = dzl * (D[c + sz] - D[c - sz]);
+        const R sigq = sigx * uu + sigy * vv + sigz * ww;
         const int oi = iidx(g, i, j, k);
         const size_t ni = g.nint;
-        const double f2 = fl[ni + oi] - sigx;
+        const R f2 = fl[ni + oi] - sigx;
         fl[ni + oi] = f2;
         fl[2 * ni + oi] -= sigy;
         fl[3 * ni + oi] -= sigz;
@@ -1173,14 +1197,14 @@ __global__ __launch_bounds__(256) void k_visc_div2(Geo g, KPar P, const double *
             const double dt = ds->dt;
             const double rhodt = rkr * dt, gamdt = rkg * dt;
             const double dy = yn[j] - yn[j - 1];
-            const double fln1 = -rhodt * flp[oi] - gamdt * fl[oi];
-            const double fln2 = -rhodt * flp[ni + oi] - gamdt * f2;
+            const double fln1 = -rhodt * double(flp[oi]) - gamdt * double(fl[oi]);
+            const double fln2 = -rhodt * double(flp[ni + oi]) - gamdt * double(f2);
             const double w1 = w[c], w2 = w[nv + c];
             acc[0] = fln1 * dy;
             acc[1] = fln2 * dy;
             acc[2] = w1 * dy;
             acc[3] = w2 * dy;
-            acc[4] = w2 * dy * wa[5 * nv + c];
+            acc[4] = w2 * dy * double(wa[5 * nv + c]);
         }
     }
     if (REDUCE) {
@@ -1256,9 +1280,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_force_finish(KPar P, const doub
 // (bcwall_staggered) from the new state when force_var is off.  With
 // force_var on, also recovers T from the new E (force_var_1_subroutine,
 // :3241-3278) and writes the block partial sums of rho*u*T*dy.
-template <bool FORCE_VAR>
-__global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restrict__ w, double *__restrict__ wa,
-                                                const double *__restrict__ flp, const double *__restrict__ fl,
+template <typename R, bool FORCE_VAR>
+__global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restrict__ w, R *__restrict__ wa,
+                                                const R *__restrict__ flp, const R *__restrict__ fl,
                                                 const DevScalars *__restrict__ ds, double rkr, double rkg, int channel,
                                                 const double *__restrict__ yn, double *__restrict__ partials) {
     __shared__ double lds[4][1];
@@ -1274,10 +1298,10 @@ __global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restric
         const double rhodt = rkr * dt, gamdt = rkg * dt;
         double fln[5];
 #pragma unroll
-        for (int m = 0; m < 5; m++) fln[m] = -rhodt * flp[m * ni + oi] - gamdt * fl[m * ni + oi];
+        for (int m = 0; m < 5; m++) fln[m] = -rhodt * double(flp[m * ni + oi]) - gamdt * double(fl[m * ni + oi]);
         if (channel) {
             const double b1 = ds->b1, b2 = ds->b2;
-            const double uu = wa[nv + c];
+            const double uu = double(wa[nv + c]);
             fln[0] = fln[0] - b1;
             fln[1] = fln[1] - b2;
             fln[4] = fln[4] - uu * b2;
@@ -1294,8 +1318,8 @@ __global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restric
             const double uu = wn[1] * ri, vv = wn[2] * ri, ww = wn[3] * ri;
             const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
             const double ee = wn[4] / wn[0] - qq;
-            const double tt = temperature_from_e(P, ee, wa[5 * nv + c]);
-            wa[5 * nv + c] = tt;
+            const double tt = temperature_from_e(P, ee, double(wa[5 * nv + c]));
+            wa[5 * nv + c] = R(tt);
             acc[0] = wn[1] * tt * dy;
         } else {
             if (P.wall_stag[0] && j <= g.ng) wall_ghost(g, P, w, wa, i, j, 1 - j, k);
@@ -1322,7 +1346,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_force_var_finish(KPar P, const 
 }
 
 // force_var_2_subroutine (kernels_cpu.F90:3282-3314) + wall ghosts
-__global__ __launch_bounds__(256) void k_force_var2(Geo g, KPar P, double *__restrict__ w, const double *__restrict__ wa,
+template <typename R>
+__global__ __launch_bounds__(256) void k_force_var2(Geo g, KPar P, double *__restrict__ w, const R *__restrict__ wa,
                                                     const DevScalars *__restrict__ ds) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -1331,7 +1356,7 @@ __global__ __launch_bounds__(256) void k_force_var2(Geo g, KPar P, double *__res
     const size_t nv = g.nvol;
     const int c = gidx(g, i, j, k);
     const double rho = w[c], rhou = w[nv + c], rhov = w[2 * nv + c], rhow = w[3 * nv + c];
-    const double ttnew = wa[5 * nv + c] + ds->bulkt;
+    const double ttnew = double(wa[5 * nv + c]) + ds->bulkt;
     const double ee = e_from_temperature(P, ttnew);
     w[4 * nv + c] = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
 }
@@ -1341,7 +1366,8 @@ __global__ __launch_bounds__(256) void k_force_var2(Geo g, KPar P, double *__res
 // 6960-7011, 6814-6880): primitives recomputed from w in registers (w_aux
 // written only for T when cp(T) needs it as the Newton start), block max of
 // the nine spectral radii, atomic max of the block maxima.
-__global__ __launch_bounds__(256) void k_dt(Geo g, KPar P, const double *__restrict__ w, double *__restrict__ wa,
+template <typename R>
+__global__ __launch_bounds__(256) void k_dt(Geo g, KPar P, const double *__restrict__ w, R *__restrict__ wa,
                                             const double *__restrict__ dx1, const double *__restrict__ dy1,
                                             const double *__restrict__ dz1, const double *__restrict__ dxs,
                                             const double *__restrict__ dys, const double *__restrict__ dzs,
@@ -1359,8 +1385,8 @@ __global__ __launch_bounds__(256) void k_dt(Geo g, KPar P, const double *__restr
         const double uu = w[nv + c] * ri, vv = w[2 * nv + c] * ri, ww = w[3 * nv + c] * ri;
         const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
         const double ee = w[4 * nv + c] / rho - qq;
-        const double tt = temperature_from_e(P, ee, wa[5 * nv + c]);
-        if (P.calorically_perfect != 1) wa[5 * nv + c] = tt;
+        const double tt = temperature_from_e(P, ee, double(wa[5 * nv + c]));
+        if (P.calorically_perfect != 1) wa[5 * nv + c] = R(tt);
         const double mu = viscosity_of(P, tt);
         const double nu = ri * mu;
         const double kor = nu / P.prandtl;
@@ -1417,8 +1443,9 @@ __global__ void k_step_end(DevScalars *ds) {
 // compute_residual_subroutine + compute_rho_t_p_minmax_subroutine
 // (kernels_cpu.F90:6682-6701, 6765-6810): fln(2) of the last stage is
 // rebuilt from the two stored right-hand sides and the forcing.
-__global__ __launch_bounds__(256) void k_residual(Geo g, KPar P, const double *__restrict__ wa,
-                                                  const double *__restrict__ flp, const double *__restrict__ fl,
+template <typename R>
+__global__ __launch_bounds__(256) void k_residual(Geo g, KPar P, const R *__restrict__ wa,
+                                                  const R *__restrict__ flp, const R *__restrict__ fl,
                                                   const DevScalars *__restrict__ ds, double rkr, double rkg, int channel,
                                                   double *__restrict__ partials) {
     __shared__ double lds[4][7];
@@ -1432,11 +1459,11 @@ __global__ __launch_bounds__(256) void k_residual(Geo g, KPar P, const double *_
         const size_t nv = g.nvol, ni = g.nint;
         const int c = gidx(g, i, j, k), oi = iidx(g, i, j, k);
         const double dt = ds->dt;
-        double fln2 = -(rkr * dt) * flp[ni + oi] - (rkg * dt) * fl[ni + oi];
+        double fln2 = -(rkr * dt) * double(flp[ni + oi]) - (rkg * dt) * double(fl[ni + oi]);
         if (channel) fln2 = fln2 - ds->b2;
         const double q = fln2 / dt;
         r = q * q;
-        const double rho = wa[c], tt = wa[5 * nv + c];
+        const double rho = double(wa[c]), tt = double(wa[5 * nv + c]);
         const double pp = rho * tt * P.rgas0;
         mn[0] = rho; mn[1] = tt; mn[2] = pp;
         mxv[0] = rho; mxv[1] = tt; mxv[2] = pp;
@@ -1493,13 +1520,14 @@ __global__ __launch_bounds__(RED_THREADS) void k_residual_finish(Geo g, const do
 }
 
 // count_weno_subroutine (kernels_cpu.F90:52-113)
-__global__ __launch_bounds__(256) void k_count_weno(Geo g, KPar P, const double *__restrict__ wa, DevScalars *ds) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_count_weno(Geo g, KPar P, const R *__restrict__ wa, DevScalars *ds) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     double cnt[3] = {0.0, 0.0, 0.0};
     if (i <= g.nx && j <= g.ny) {
-        const double *S = wa + 7 * (size_t)g.nvol;
+        const R *S = wa + 7 * (size_t)g.nvol;
         const int c = gidx(g, i, j, k);
         const int ws = P.weno_scheme;
         const int st[3] = {1, g.sx, g.sxy};
@@ -1507,7 +1535,7 @@ __global__ __launch_bounds__(256) void k_count_weno(Geo g, KPar P, const double 
         for (int d = 0; d < 3; d++) {
             if (idx[d] < P.eul_lo[d] || idx[d] > P.eul_hi[d]) continue;
             bool shk = false;
-            for (int s = -ws + 1; s <= ws; s++) shk |= S[c + s * st[d]] > P.sensor_threshold;
+            for (int s = -ws + 1; s <= ws; s++) shk |= S[c + s * st[d]] > R(P.sensor_threshold);
             cnt[d] = shk ? 1.0 : 0.0;
         }
     }
@@ -1517,8 +1545,9 @@ __global__ __launch_bounds__(256) void k_count_weno(Geo g, KPar P, const double 
     }
 }
 
-__global__ void k_zero(double *__restrict__ a, size_t n) {
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) a[t] = 0.0;
+template <typename T>
+__global__ void k_zero(T *__restrict__ a, size_t n) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) a[t] = T(0);
 }
 
 // ================================================================ launchers
@@ -1531,17 +1560,39 @@ static inline int grid1d(size_t n, int bs) {
     return (int)(b > 65535 * 16 ? 65535 * 16 : (b < 1 ? 1 : b));
 }
 
-hipError_t launch_fill_periodic(st_solver *s, double *arr, int nv) {
+// The element type of w_aux and of the right-hand sides follows the
+// precision of the solve: double (FP64 form) or float (mixed form).  Each
+// public launcher dispatches to its typed instance.
+#define ST_DISPATCH(call)                         \
+    do {                                          \
+        if (s->mixed) {                           \
+            using R = float;                      \
+            call;                                 \
+        } else {                                  \
+            using R = double;                     \
+            call;                                 \
+        }                                         \
+    } while (0)
+
+template <typename T>
+static hipError_t fill_periodic_t(st_solver *s, T *arr, int nv) {
     const Geo &g = s->g;
     size_t tot = (size_t)2 * g.ng * ((size_t)g.ny * g.nz + (size_t)g.nx * g.nz + (size_t)g.nx * g.ny) * nv;
-    hipLaunchKernelGGL(k_fill_periodic, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g, s->kp, arr, nv);
+    hipLaunchKernelGGL(k_fill_periodic<T>, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g, s->kp, arr, nv);
     return hipGetLastError();
 }
 
-hipError_t launch_extrapolate_var(st_solver *s, double *arr) {
+hipError_t launch_fill_periodic(st_solver *s, double *arr, int nv) { return fill_periodic_t(s, arr, nv); }
+
+hipError_t launch_fill_periodic_aux(st_solver *s, int var) {
+    ST_DISPATCH(return fill_periodic_t(s, (R *)s->w_aux + (size_t)(var - 1) * s->g.nvol, 1));
+}
+
+hipError_t launch_extrapolate_aux(st_solver *s, int var) {
     const Geo &g = s->g;
     size_t tot = (size_t)2 * g.ng * ((size_t)g.ny * g.nz + (size_t)g.nx * g.nz + (size_t)g.nx * g.ny);
-    hipLaunchKernelGGL(k_extrapolate_var, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g, arr);
+    ST_DISPATCH(hipLaunchKernelGGL(k_extrapolate_var<R>, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g,
+                                   (R *)s->w_aux + (size_t)(var - 1) * g.nvol));
     return hipGetLastError();
 }
 
@@ -1549,8 +1600,8 @@ hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int 
     const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
     if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
     const size_t tot = (size_t)ni * nj * nk;
-    hipLaunchKernelGGL(k_aux, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, s->g, s->kp, s->w,
-                       s->w_aux, i0, ni, j0, nj, k0, nk, image ? 1 : 0);
+    ST_DISPATCH(hipLaunchKernelGGL(k_aux<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, s->g, s->kp,
+                                   s->w, (R *)s->w_aux, i0, ni, j0, nj, k0, nk, image ? 1 : 0));
     return hipGetLastError();
 }
 
@@ -1558,11 +1609,13 @@ hipError_t launch_wall_ghosts(st_solver *s) {
     const Geo &g = s->g;
     if (!s->kp.wall_stag[0] && !s->kp.wall_stag[1]) return hipSuccess;
     size_t tot = (size_t)2 * g.ng * g.nx * g.nz;
-    hipLaunchKernelGGL(k_wall_ghosts, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g, s->kp, s->w, s->w_aux);
+    ST_DISPATCH(hipLaunchKernelGGL(k_wall_ghosts<R>, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g, s->kp, s->w,
+                                   (const R *)s->w_aux));
     return hipGetLastError();
 }
 
-static hipError_t launch_fix(st_solver *s, double *fl, int dir) {
+template <typename R>
+static hipError_t launch_fix(st_solver *s, R *fl, int dir) {
     const Geo &g = s->g;
     // one 64-lane workgroup per contiguous region of flags; regions large
     // enough that the LDS queue fills whole waves of WENO work
@@ -1571,23 +1624,26 @@ static hipError_t launch_fix(st_solver *s, double *fl, int dir) {
     int region = (g.nint + nb - 1) / nb;
     region = (region + 2047) / 2048 * 2048;  // whole 64-bit flag words per lane
     const dim3 grid((unsigned)nb), block(64);
+    const R *wa = (const R *)s->w_aux;
     if (dir == 0)
-        hipLaunchKernelGGL(k_weno_fix<0>, grid, block, 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl,
+        hipLaunchKernelGGL((k_weno_fix<R, 0>), grid, block, 0, s->stream, g, s->kp, wa, s->eoc[0], s->dcsidx, fl,
                            s->dirty, region);
     else if (dir == 1)
-        hipLaunchKernelGGL(k_weno_fix<1>, grid, block, 0, s->stream, g, s->kp, s->w_aux, s->eoc[1], s->detady, fl,
+        hipLaunchKernelGGL((k_weno_fix<R, 1>), grid, block, 0, s->stream, g, s->kp, wa, s->eoc[1], s->detady, fl,
                            s->dirty, region);
     else
-        hipLaunchKernelGGL(k_weno_fix<2>, grid, block, 0, s->stream, g, s->kp, s->w_aux, s->eoc[2], s->dzitdz, fl,
+        hipLaunchKernelGGL((k_weno_fix<R, 2>), grid, block, 0, s->stream, g, s->kp, wa, s->eoc[2], s->dzitdz, fl,
                            s->dirty, region);
     return hipGetLastError();
 }
 
-hipError_t launch_euler_x(st_solver *s, double *fl, bool init) {
+template <typename R>
+static hipError_t euler_x_t(st_solver *s, R *fl, bool init) {
     const Geo &g = s->g;
     if (s->kp.eul_hi[0] < s->kp.eul_lo[0]) {
-        if (init) return launch_zero(s, fl, (size_t)5 * g.nint);
-        return hipSuccess;
+        if (init) hipLaunchKernelGGL(k_zero<R>, dim3(grid1d((size_t)5 * g.nint, 256)), dim3(256), 0, s->stream, fl,
+                                     (size_t)5 * g.nint);
+        return hipGetLastError();
     }
     hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
     const long nwork = (long)((g.nx + XW - 1) / XW) * g.ny * g.nz;
@@ -1596,8 +1652,8 @@ hipError_t launch_euler_x(st_solver *s, double *fl, bool init) {
     dim3 grid((unsigned)nblk);
     const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
 #define ST_LAUNCH_X(I, K)                                                                                          \
-    hipLaunchKernelGGL((k_euler_x<I, K>), grid, dim3(256), 0, s->stream, g, s->kp, s->w_aux, s->eoc[0], s->dcsidx, fl, \
-                       s->dirty)
+    hipLaunchKernelGGL((k_euler_x<R, I, K>), grid, dim3(256), 0, s->stream, g, s->kp, (const R *)s->w_aux, s->eoc[0], \
+                       s->dcsidx, fl, s->dirty)
     if (init) {
         if (km == 0) ST_LAUNCH_X(true, 0); else if (km > 0) ST_LAUNCH_X(true, 1); else ST_LAUNCH_X(true, -1);
     } else {
@@ -1609,7 +1665,10 @@ hipError_t launch_euler_x(st_solver *s, double *fl, bool init) {
     return launch_fix(s, fl, 0);
 }
 
-static hipError_t launch_march(st_solver *s, double *fl, int dir) {
+hipError_t launch_euler_x(st_solver *s, void *fl, bool init) { ST_DISPATCH(return euler_x_t(s, (R *)fl, init)); }
+
+template <typename R>
+static hipError_t launch_march(st_solver *s, R *fl, int dir) {
     const Geo &g = s->g;
     const int lo = s->kp.eul_lo[dir], hi = s->kp.eul_hi[dir];
     if (hi < lo) return hipSuccess;
@@ -1620,14 +1679,15 @@ static hipError_t launch_march(st_solver *s, double *fl, int dir) {
     int chunk = ncell < 32 ? ncell : 32;
     const long minthreads = 256L * 16 * 64 * 4;
     while (chunk > 8 && (long)g.nx * no * ((ncell + chunk - 1) / chunk) < minthreads) chunk /= 2;
+    if (s->march_chunk > 0) chunk = s->march_chunk < ncell ? s->march_chunk : ncell;
     const int nchunk = (ncell + chunk - 1) / chunk;
     const size_t tot = (size_t)g.nx * no * nchunk;
     hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
     dim3 grid((unsigned)((tot + 255) / 256));
     const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
 #define ST_LAUNCH_M(D, K, E, M)                                                                                     \
-    hipLaunchKernelGGL((k_euler_march<D, K>), grid, dim3(256), 0, s->stream, g, s->kp, s->w_aux, E, M, fl, chunk, \
-                       nchunk, s->dirty)
+    hipLaunchKernelGGL((k_euler_march<R, D, K>), grid, dim3(256), 0, s->stream, g, s->kp, (const R *)s->w_aux, E, M, \
+                       fl, chunk, nchunk, s->dirty)
     if (dir == 1) {
         if (km == 0) ST_LAUNCH_M(1, 0, s->eoc[1], s->detady);
         else if (km > 0) ST_LAUNCH_M(1, 1, s->eoc[1], s->detady);
@@ -1643,36 +1703,36 @@ static hipError_t launch_march(st_solver *s, double *fl, int dir) {
     return launch_fix(s, fl, dir);
 }
 
-hipError_t launch_euler_y(st_solver *s, double *fl) { return launch_march(s, fl, 1); }
-hipError_t launch_euler_z(st_solver *s, double *fl) { return launch_march(s, fl, 2); }
+hipError_t launch_euler_y(st_solver *s, void *fl) { ST_DISPATCH(return launch_march(s, (R *)fl, 1)); }
+hipError_t launch_euler_z(st_solver *s, void *fl) { ST_DISPATCH(return launch_march(s, (R *)fl, 2)); }
 
-hipError_t launch_visc(st_solver *s, double *fl, int mode, bool ghost_images) {
+hipError_t launch_visc(st_solver *s, void *fl, int mode, bool ghost_images) {
     const Geo &g = s->g;
     const int bxy = ((g.nx + VBX - 1) / VBX) * ((g.ny + VBY - 1) / VBY);
     // chunks of up to 32 planes (6 warm-up planes each), enough of them to
     // give every CU several tiles
     int chunk = 32;
     while (chunk > 8 && (long)bxy * ((g.nz + chunk - 1) / chunk) < 2048) chunk /= 2;
+    if (s->visc_chunk > 0) chunk = s->visc_chunk;
     const dim3 grid((g.nx + VBX - 1) / VBX, (g.ny + VBY - 1) / VBY, (g.nz + chunk - 1) / chunk);
     const dim3 block(VBX, VBY);
-    if (mode == 0)
-        hipLaunchKernelGGL(k_visc_zm<true>, grid, block, 0, s->stream, g, s->kp, s->w_aux, fl, s->dcsidx, s->dcsidxs,
-                           s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2,
-                           s->wallprop, ghost_images ? 1 : 0, chunk);
-    else
-        hipLaunchKernelGGL(k_visc_zm<false>, grid, block, 0, s->stream, g, s->kp, s->w_aux, fl, s->dcsidx, s->dcsidxs,
-                           s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2,
-                           s->wallprop, ghost_images ? 1 : 0, chunk);
+#define ST_LAUNCH_V(S_)                                                                                            \
+    hipLaunchKernelGGL((k_visc_zm<R, S_>), grid, block, 0, s->stream, g, s->kp, (R *)s->w_aux, (R *)fl, s->dcsidx,   \
+                       s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2, \
+                       s->wallprop, ghost_images ? 1 : 0, chunk)
+    if (mode == 0) ST_DISPATCH(ST_LAUNCH_V(true));
+    else ST_DISPATCH(ST_LAUNCH_V(false));
+#undef ST_LAUNCH_V
     return hipGetLastError();
 }
 
 // Conservative viscous form: mode 2 (cell terms + sensor), 6 (cell terms),
 // 5 (face terms) and 56 (6 then 5 in one pass, the rk_sync order).
-hipError_t launch_visc_cons(st_solver *s, double *fl, int mode, bool ghost_images) {
+hipError_t launch_visc_cons(st_solver *s, void *fl, int mode, bool ghost_images) {
     const Geo &g = s->g;
-#define ST_LAUNCH_VC(C_, F_, S_)                                                                                     \
-    hipLaunchKernelGGL((k_visc_cons<C_, F_, S_>), cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, s->x, \
-                       s->y, s->z, s->wallprop, ghost_images ? 1 : 0)
+#define ST_LAUNCH_VC(C_, F_, S_)                                                                                      \
+    ST_DISPATCH(hipLaunchKernelGGL((k_visc_cons<R, C_, F_, S_>), cell_grid(g), kCellBlock, 0, s->stream, g, s->kp,   \
+                                   (R *)s->w_aux, (R *)fl, s->x, s->y, s->z, s->wallprop, ghost_images ? 1 : 0))
     switch (mode) {
     case 2: ST_LAUNCH_VC(true, false, true); break;
     case 5: ST_LAUNCH_VC(false, true, false); break;
@@ -1685,28 +1745,26 @@ hipError_t launch_visc_cons(st_solver *s, double *fl, int mode, bool ghost_image
 }
 
 hipError_t launch_sensor(st_solver *s, bool ghost_images) {
-    hipLaunchKernelGGL(k_sensor, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, s->w_aux, ghost_images ? 1 : 0);
+    ST_DISPATCH(hipLaunchKernelGGL(k_sensor<R>, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, (R *)s->w_aux,
+                                   ghost_images ? 1 : 0));
     return hipGetLastError();
 }
 
-hipError_t launch_visc_div(st_solver *s, double *fl, const double *fl_prev, int stage, bool reduce, bool ord2) {
+hipError_t launch_visc_div(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce, bool ord2) {
     const Geo &g = s->g;
     const double rkr = stage >= 0 ? s->rhork[stage] : 0.0, rkg = stage >= 0 ? s->gamrk[stage] : 0.0;
+#define ST_LAUNCH_D(K_, RED_, A_, B_, C_)                                                                           \
+    ST_DISPATCH(hipLaunchKernelGGL((K_<R, RED_>), cell_grid(g), kCellBlock, 0, s->stream, g, s->kp,                  \
+                                   (const R *)s->w_aux, (R *)fl, (const R *)fl_prev, s->w, A_, B_, C_, s->yn, s->ds,  \
+                                   rkr, rkg, s->partials))
     if (ord2) {
-        if (reduce)
-            hipLaunchKernelGGL(k_visc_div2<true>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl,
-                               fl_prev, s->w, s->x, s->y, s->z, s->yn, s->ds, rkr, rkg, s->partials);
-        else
-            hipLaunchKernelGGL(k_visc_div2<false>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl,
-                               fl_prev, s->w, s->x, s->y, s->z, s->yn, s->ds, rkr, rkg, s->partials);
-        return hipGetLastError();
+        if (reduce) ST_LAUNCH_D(k_visc_div2, true, s->x, s->y, s->z);
+        else ST_LAUNCH_D(k_visc_div2, false, s->x, s->y, s->z);
+    } else {
+        if (reduce) ST_LAUNCH_D(k_visc_div, true, s->dcsidx, s->detady, s->dzitdz);
+        else ST_LAUNCH_D(k_visc_div, false, s->dcsidx, s->detady, s->dzitdz);
     }
-    if (reduce)
-        hipLaunchKernelGGL(k_visc_div<true>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, fl_prev,
-                           s->w, s->dcsidx, s->detady, s->dzitdz, s->yn, s->ds, rkr, rkg, s->partials);
-    else
-        hipLaunchKernelGGL(k_visc_div<false>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, fl, fl_prev,
-                           s->w, s->dcsidx, s->detady, s->dzitdz, s->yn, s->ds, rkr, rkg, s->partials);
+#undef ST_LAUNCH_D
     return hipGetLastError();
 }
 
@@ -1716,15 +1774,16 @@ hipError_t launch_force_finish(st_solver *s, int stage) {
     return hipGetLastError();
 }
 
-hipError_t launch_update(st_solver *s, const double *fl_prev, const double *fl, int stage) {
+hipError_t launch_update(st_solver *s, const void *fl_prev, const void *fl, int stage) {
     const Geo &g = s->g;
     const bool fv = s->p.channel_case && s->p.theta_wall >= -1.0;
-    if (fv)
-        hipLaunchKernelGGL(k_update<true>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, s->w_aux, fl_prev, fl,
-                           s->ds, s->rhork[stage], s->gamrk[stage], s->p.channel_case, s->yn, s->partials);
-    else
-        hipLaunchKernelGGL(k_update<false>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, s->w_aux, fl_prev,
-                           fl, s->ds, s->rhork[stage], s->gamrk[stage], s->p.channel_case, s->yn, s->partials);
+#define ST_LAUNCH_U(FV_)                                                                                             \
+    ST_DISPATCH(hipLaunchKernelGGL((k_update<R, FV_>), cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w,        \
+                                   (R *)s->w_aux, (const R *)fl_prev, (const R *)fl, s->ds, s->rhork[stage],            \
+                                   s->gamrk[stage], s->p.channel_case, s->yn, s->partials))
+    if (fv) ST_LAUNCH_U(true);
+    else ST_LAUNCH_U(false);
+#undef ST_LAUNCH_U
     return hipGetLastError();
 }
 
@@ -1732,14 +1791,15 @@ hipError_t launch_force_var(st_solver *s) {
     const Geo &g = s->g;
     hipLaunchKernelGGL(k_force_var_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials,
                        cell_blocks(g), s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
-    hipLaunchKernelGGL(k_force_var2, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, s->w_aux, s->ds);
+    ST_DISPATCH(hipLaunchKernelGGL(k_force_var2<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w,
+                                   (const R *)s->w_aux, s->ds));
     return launch_wall_ghosts(s);
 }
 
 hipError_t launch_dt(st_solver *s) {
     const Geo &g = s->g;
-    hipLaunchKernelGGL(k_dt, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, s->w_aux, s->dcsidx, s->detady,
-                       s->dzitdz, s->dcsidxs, s->detadys, s->dzitdzs, s->partials);
+    ST_DISPATCH(hipLaunchKernelGGL(k_dt<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, (R *)s->w_aux,
+                                   s->dcsidx, s->detady, s->dzitdz, s->dcsidxs, s->detadys, s->dzitdzs, s->partials));
     hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(g), s->ds);
     return hipGetLastError();
 }
@@ -1754,19 +1814,27 @@ hipError_t launch_residual(st_solver *s) {
     const int st = s->last_stage < 0 ? 0 : s->last_stage;
     const double rkr = s->last_stage < 0 ? 0.0 : s->rhork[st];
     const double rkg = s->last_stage < 0 ? 0.0 : s->gamrk[st];
-    hipLaunchKernelGGL(k_residual, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w_aux, s->F[s->cur ^ 1],
-                       s->F[s->cur], s->ds, rkr, rkg, s->p.channel_case, s->partials);
+    ST_DISPATCH(hipLaunchKernelGGL(k_residual<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp,
+                                   (const R *)s->w_aux, (const R *)s->F[s->cur ^ 1], (const R *)s->F[s->cur], s->ds,
+                                   rkr, rkg, s->p.channel_case, s->partials));
     hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(RED_THREADS), 0, s->stream, g, s->partials, cell_blocks(g), s->ds);
     return hipGetLastError();
 }
 
 hipError_t launch_count_weno(st_solver *s) {
-    hipLaunchKernelGGL(k_count_weno, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, s->w_aux, s->ds);
+    ST_DISPATCH(hipLaunchKernelGGL(k_count_weno<R>, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp,
+                                   (const R *)s->w_aux, s->ds));
     return hipGetLastError();
 }
 
 hipError_t launch_zero(st_solver *s, double *a, size_t n) {
-    hipLaunchKernelGGL(k_zero, dim3(grid1d(n, 256)), dim3(256), 0, s->stream, a, n);
+    hipLaunchKernelGGL(k_zero<double>, dim3(grid1d(n, 256)), dim3(256), 0, s->stream, a, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_rhs(st_solver *s, void *a) {
+    const size_t n = (size_t)5 * s->g.nint;
+    ST_DISPATCH(hipLaunchKernelGGL(k_zero<R>, dim3(grid1d(n, 256)), dim3(256), 0, s->stream, (R *)a, n));
     return hipGetLastError();
 }
 

@@ -12,8 +12,11 @@ struct st_solver {
     int device;
     // state
     double *w;          // 5 ghosted variables
-    double *w_aux;      // 10 ghosted variables
-    double *F[2];       // right-hand sides of the current and the previous stage
+    void *w_aux;        // 10 ghosted variables (double, or float in the mixed form)
+    void *F[2];         // right-hand sides of the current and the previous stage (same type)
+    bool mixed;         // ST_PREC_MIXED: w_aux and F in float
+    int visc_chunk;     // planes per tile of the z-marching viscous kernel (0: automatic)
+    int march_chunk;    // points per thread of the y/z flux marches (0: automatic)
     double *wallprop;   // 3 x (sx*sz), wallprop(:,:,2:4)
     int *eoc[3];        // ep_ord_change per direction, faces 0..n
     // grid
@@ -48,24 +51,26 @@ namespace st {
 
 // launchers (st_kernels.hip); all enqueue on s->stream and return a hipError_t
 hipError_t launch_fill_periodic(st_solver *s, double *arr, int nv);
-hipError_t launch_extrapolate_var(st_solver *s, double *arr);
+hipError_t launch_fill_periodic_aux(st_solver *s, int var);
+hipError_t launch_extrapolate_aux(st_solver *s, int var);
 hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int k1, bool image);
 hipError_t launch_wall_ghosts(st_solver *s);
-hipError_t launch_euler_x(st_solver *s, double *fl, bool init);
-hipError_t launch_euler_y(st_solver *s, double *fl);
-hipError_t launch_euler_z(st_solver *s, double *fl);
-hipError_t launch_visc(st_solver *s, double *fl, int mode, bool ghost_images);
-hipError_t launch_visc_cons(st_solver *s, double *fl, int mode, bool ghost_images);
+hipError_t launch_euler_x(st_solver *s, void *fl, bool init);
+hipError_t launch_euler_y(st_solver *s, void *fl);
+hipError_t launch_euler_z(st_solver *s, void *fl);
+hipError_t launch_visc(st_solver *s, void *fl, int mode, bool ghost_images);
+hipError_t launch_visc_cons(st_solver *s, void *fl, int mode, bool ghost_images);
 hipError_t launch_sensor(st_solver *s, bool ghost_images);
 // visflx mode 1, or mode 7 (second order, conservative viscous form) with ord2
-hipError_t launch_visc_div(st_solver *s, double *fl, const double *fl_prev, int stage, bool reduce, bool ord2);
+hipError_t launch_visc_div(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce, bool ord2);
 hipError_t launch_force_finish(st_solver *s, int stage);
-hipError_t launch_update(st_solver *s, const double *fl_prev, const double *fl, int stage);
+hipError_t launch_update(st_solver *s, const void *fl_prev, const void *fl, int stage);
 hipError_t launch_force_var(st_solver *s);
 hipError_t launch_dt(st_solver *s);
 hipError_t launch_step_end(st_solver *s);
 hipError_t launch_residual(st_solver *s);
 hipError_t launch_count_weno(st_solver *s);
 hipError_t launch_zero(st_solver *s, double *a, size_t n);
+hipError_t launch_zero_rhs(st_solver *s, void *a);
 
 }  // namespace st

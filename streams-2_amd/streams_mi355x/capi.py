@@ -54,10 +54,19 @@ _SIGNATURES = {
     "st_compute_residual": (C.c_int, [C.c_void_p, C.POINTER(StScalars)]),
     "st_count_weno": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "st_kernel_times": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "st_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "st_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "st_synchronize": (C.c_int, [C.c_void_p]),
     "st_last_error": (C.c_char_p, []),
     "st_version": (C.c_char_p, []),
+    "st_server_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "st_server_destroy": (C.c_int, [C.c_void_p]),
+    "st_server_submit": (C.c_int, [C.c_void_p, C.POINTER(StParams)] + [C.POINTER(C.c_double)] * 14
+                         + [C.c_int32, C.POINTER(C.c_int64)]),
+    "st_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(C.c_double), C.POINTER(StScalars),
+                                 C.POINTER(C.c_double)]),
+    "st_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "st_server_last_error": (C.c_char_p, []),
 }
 
 
@@ -93,4 +102,10 @@ def exported_symbols():
 def check(rc: int):
     if rc != ST_OK:
         raise StreamsError(rc, lib().st_last_error().decode())
+    return rc
+
+
+def check_server(rc: int):
+    if rc != ST_OK:
+        raise StreamsError(rc, lib().st_server_last_error().decode())
     return rc

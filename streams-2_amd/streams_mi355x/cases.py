@@ -20,7 +20,7 @@ class ChannelCase:
 
 
 def make_channel(nx=32, ny=24, nz=16, mach=1.5, retau=220.0, noise=0.10, seed=0,
-                 sensor_threshold=0.9, cfl=0.8, uniform_y=False, **kw) -> ChannelCase:
+                 sensor_threshold=0.9, cfl=0.8, uniform_y=False, noise_source="counter", **kw) -> ChannelCase:
     """Isothermal-wall channel: periodic x/z, staggered walls in y.
 
     Domain 4pi x 2 x 4pi/3, tanh-type y stretching (generate_grid_cha),
@@ -34,7 +34,7 @@ def make_channel(nx=32, ny=24, nz=16, mach=1.5, retau=220.0, noise=0.10, seed=0,
     params = channel_params(case, grid.yn)
     e_ref = params.cv_coeff[1] * params.t0  # calorically perfect e(T=t0)
     w0 = channel_initial_field(grid, params.u0, noise=noise, seed=seed, rho0=params.rho0, t0=params.t0,
-                               rgas0=params.rgas0, e_ref=e_ref)
+                               rgas0=params.rgas0, e_ref=e_ref, noise_source=noise_source)
     return ChannelCase(case, grid, params, w0)
 
 

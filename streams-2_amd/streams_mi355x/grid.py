@@ -175,8 +175,8 @@ def counter_uniform(seed: int, index: np.ndarray, stream: int) -> np.ndarray:
 
     The value of a cell depends only on its global index, never on the
     decomposition or the traversal order."""
-    key = (np.uint64(seed) * np.uint64(0x100000001B3)) ^ (np.uint64(stream) << np.uint64(56))
     with np.errstate(over="ignore"):
+        key = (np.uint64(seed) * np.uint64(0x100000001B3)) ^ (np.uint64(stream) << np.uint64(56))
         h = _splitmix64(index.astype(np.uint64) ^ key)
         h = _splitmix64(h + np.uint64(stream))
     return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
@@ -184,7 +184,7 @@ def counter_uniform(seed: int, index: np.ndarray, stream: int) -> np.ndarray:
 
 def channel_initial_field(grid: Grid, u0: float, noise: float = 0.10, seed: int = 0,
                           rho0: float = 1.0, t0: float = 1.0, rgas0: float = 1.0, cv0: float = 2.5,
-                          e_ref: float = None) -> np.ndarray:
+                          e_ref: float = None, noise_source: str = "counter") -> np.ndarray:
     """Laminar Poiseuille profile u = 1.5*u_b*(1-y^2) plus seeded noise.
 
     ``noise`` is the peak-to-peak velocity perturbation as a fraction of u0:
@@ -192,6 +192,8 @@ def channel_initial_field(grid: Grid, u0: float, noise: float = 0.10, seed: int 
     counter_uniform keyed by (seed, global cell index, component).  Density
     and temperature are uniform (rho0, t0); total energy follows the
     calorically perfect e(T) of get_e_from_temperature (kernels_cpu.F90:7155).
+    ``noise_source="pcg"`` draws the noise from numpy's PCG64 stream seeded
+    with ``seed`` instead (faster; used for the serving workload).
     Returns w(1-ng:nx+ng, 1-ng:ny+ng, 1-ng:nz+ng, 5) in Fortran order with
     zero ghosts.
     """
@@ -204,9 +206,13 @@ def channel_initial_field(grid: Grid, u0: float, noise: float = 0.10, seed: int 
     k = np.arange(nz, dtype=np.int64)
     gidx = (k[:, None, None] * ny + j[None, :, None]) * nx + i[None, None, :]
     ub = 1.5 * u0 * (1.0 - yc ** 2)
-    uu = np.broadcast_to(ub[None, :, None], (nz, ny, nx)) + noise * u0 * (counter_uniform(seed, gidx, 0) - 0.5)
-    vv = noise * u0 * (counter_uniform(seed, gidx, 1) - 0.5)
-    ww = noise * u0 * (counter_uniform(seed, gidx, 2) - 0.5)
+    if noise_source == "pcg":
+        r = np.random.default_rng(seed).random((3, nz, ny, nx))
+    else:
+        r = [counter_uniform(seed, gidx, m) for m in range(3)]
+    uu = np.broadcast_to(ub[None, :, None], (nz, ny, nx)) + noise * u0 * (r[0] - 0.5)
+    vv = noise * u0 * (r[1] - 0.5)
+    ww = noise * u0 * (r[2] - 0.5)
     rho = rho0
     if e_ref is None:
         # calorically perfect: e = t0*(cv(1) + cv(0)*(T/t0 - 1)), cv(1) = rgas0*gm

@@ -1,0 +1,155 @@
+"""Serving loop of many independent flow solves on one GPU (host side).
+
+``Server`` wraps the continuous-batching server of the C library
+(st_server_*): requests are admitted as slots and HBM free up, run
+concurrently on their own streams from captured graphs, and are returned
+with their submit / admit / done times.  ``serving_workload`` draws the
+seeded stream of uneven channel solves, ``run_serving`` replays it with
+Poisson arrivals and reports throughput and latency percentiles.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi
+from .cases import ChannelCase, make_channel
+from .params import StParams, StScalars
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Server:
+    def __init__(self, hbm_bytes: int, max_slots: int = 8, steps_per_graph: int = 10, autotune: bool = True):
+        self.L = capi.lib()
+        h = C.c_void_p()
+        capi.check_server(self.L.st_server_create(int(hbm_bytes), max_slots, steps_per_graph, 1 if autotune else 0,
+                                                  C.byref(h)))
+        self.h = h
+        self._shapes = {}
+
+    def submit(self, params: StParams, grid, w0: np.ndarray, nsteps: int) -> int:
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in grid.arrays()]
+        w0 = np.ascontiguousarray(w0, dtype=np.float64)
+        t = C.c_int64()
+        capi.check_server(self.L.st_server_submit(self.h, C.byref(params), *[_dp(a) for a in arrs], _dp(w0), nsteps,
+                                                  C.byref(t)))
+        self._shapes[t.value] = w0.shape
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = -1.0, want_field: bool = True):
+        shape = self._shapes.pop(ticket)
+        w = np.empty(shape) if want_field else None
+        sc = StScalars()
+        tm = (C.c_double * 3)()
+        capi.check_server(self.L.st_server_wait(self.h, ticket, timeout, _dp(w) if want_field else None,
+                                                C.byref(sc), tm))
+        return w, sc.as_dict(), (tm[0], tm[1], tm[2])
+
+    def stats(self):
+        out = (C.c_int64 * 6)()
+        capi.check_server(self.L.st_server_stats(self.h, out))
+        return dict(zip(("queued", "running", "completed", "failed", "hbm_bytes", "tuned_classes"), list(out)))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.st_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Request:
+    case: ChannelCase
+    nsteps: int
+    arrival: float   # seconds after the start of the stream
+
+
+def serving_workload(n: int = 64, nsteps: int = 200, rate: float = 8.0, seed: int = 0, precision: int = 0,
+                     nx_range=(64, 256), nyz_range=(64, 128), mach_range=(0.3, 3.0), retau_range=(100.0, 500.0),
+                     multiple: int = 8):
+    """Seeded stream of independent channel solves: grid extents uniform in
+    the given ranges (rounded to ``multiple``), bulk Mach and Re_tau uniform,
+    Poisson arrivals at ``rate`` requests/s."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    for _ in range(n):
+        nx = int(round(rng.uniform(*nx_range) / multiple)) * multiple
+        ny = int(round(rng.uniform(*nyz_range) / multiple)) * multiple
+        nz = int(round(rng.uniform(*nyz_range) / multiple)) * multiple
+        mach = float(rng.uniform(*mach_range))
+        retau = float(rng.uniform(*retau_range))
+        seed_i = int(rng.integers(0, 2 ** 31 - 1))
+        c = make_channel(nx=nx, ny=ny, nz=nz, mach=mach, retau=retau, seed=seed_i, precision=precision,
+                         noise_source="pcg")
+        t += float(rng.exponential(1.0 / rate))
+        reqs.append(Request(c, nsteps, t))
+    return reqs
+
+
+def cell_updates(reqs) -> float:
+    return float(sum(r.case.params.nx * r.case.params.ny * r.case.params.nz * 3 * r.nsteps for r in reqs))
+
+
+def run_serving(reqs, hbm_bytes: int, max_slots: int = 8, steps_per_graph: int = 10, autotune: bool = True,
+                want_fields: bool = False):
+    """Submit each request at its arrival time, wait for all; returns a dict
+    with throughput (Gcell-updates/s per RK stage over the makespan from the
+    first arrival), p50/p99 time-to-solution and the per-request results."""
+    out = []
+    with Server(hbm_bytes, max_slots, steps_per_graph, autotune) as sv:
+        t0 = time.perf_counter()
+        tickets = []
+        for r in reqs:
+            delay = r.arrival - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(sv.submit(r.case.params, r.case.grid, r.case.w0, r.nsteps))
+        for r, tk in zip(reqs, tickets):
+            w, sc, tm = sv.wait(tk, -1.0, want_fields)
+            out.append({"w": w, "scalars": sc, "submit": tm[0], "admit": tm[1], "done": tm[2]})
+        stats = sv.stats()
+    first = min(o["submit"] for o in out)
+    last = max(o["done"] for o in out)
+    lat = np.array([o["done"] - o["submit"] for o in out])
+    return {"gcups": cell_updates(reqs) / (last - first) / 1e9, "makespan_s": last - first,
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_queue_s": float(np.mean([o["admit"] - o["submit"] for o in out])),
+            "results": out, "stats": stats}
+
+
+def run_sequential(reqs, want_fields: bool = False):
+    """The same requests one after another through the plain path
+    (ChannelSolver: one stream, no graphs, no overlap)."""
+    from .solver import ChannelSolver
+
+    out = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        with ChannelSolver(r.case.params, r.case.grid) as g:
+            g.copy_from_field(r.case.w0)
+            g.init_run()
+            g.advance(r.nsteps)
+            sc = g.compute_residual()
+            w = g.copy_to_field() if want_fields else None
+        out.append({"w": w, "scalars": sc})
+    el = time.perf_counter() - t0
+    return {"gcups": cell_updates(reqs) / el / 1e9, "seconds": el, "results": out}
