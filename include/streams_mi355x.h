@@ -63,8 +63,11 @@ typedef struct st_params {
     int32_t visc_model;              /* 0 inviscid, 1 power law, 2 Sutherland             */
     int32_t calorically_perfect;     /* 1 constant cp, 0 cp(T) polynomial                 */
     int32_t indx_cp_l, indx_cp_r;    /* exponent range of the cp polynomial               */
-    int32_t bctags[6];               /* xmin,xmax,ymin,ymax,zmin,zmax: 0 periodic, 6 wall */
-    int32_t bctags_nr[6];            /* characteristic BC flags (0 = none)                */
+    int32_t bctags[6];               /* xmin,xmax,ymin,ymax,zmin,zmax: 0 periodic, 1 free
+                                        stream, 2 extrapolation, 5 symmetry (ymin), 6 wall
+                                        (ymin; ymax too when staggered), 7 oblique shock (ymax) */
+    int32_t bctags_nr[6];            /* characteristic BC: 0 none, 1 non-reflecting, 2 relaxation
+                                        to the neighbour, 3 relaxation to winf, 6 reflective wall */
     int32_t is_y_staggered;          /* 1 for the channel (cell-centred y)                */
     int32_t channel_case;            /* 1 enables force_rhs / force_var                   */
     int32_t correct_bound_ord;       /* 1 lowers the stencil order at non-periodic faces  */
@@ -82,6 +85,12 @@ typedef struct st_params {
     double tol_iter_nr;              /* Newton tolerance for T(e) (1e-12 in FP64)          */
     double cp_coeff[ST_MAX_CP_COEFF];/* cp_coeff(indx_cp_l:indx_cp_r+1), index l-indx_cp_l */
     double cv_coeff[ST_MAX_CP_COEFF];
+    /* free-stream and oblique-shock boundary data (bctags 1, 7, bctags_nr 3):
+     * winf and winf_past_shock are conserved states, xshock_imp the impinging
+     * point, shock_angle in radians, tanhfacs the smoothing length
+     * (singleideal.F90:2798-2826 set_flow_params, :2955-3017 set_oblique_shock) */
+    double winf[5], winf_past_shock[5];
+    double xshock_imp, shock_angle, tanhfacs;
 } st_params;
 
 /* Device-resident scalars of the run, copied to the host on request. */
@@ -141,6 +150,9 @@ int st_visflx(st_solver *s, int32_t mode);
 int st_euler_x(st_solver *s);
 int st_euler_y(st_solver *s);
 int st_euler_z(st_solver *s);
+/* bc_nr (singleideal_cpu.F90:1001-1062): characteristic boundary treatment
+ * of the faces with bctags_nr != 0, added to fl. */
+int st_bc_nr(st_solver *s);
 /* zero_flux_subroutine (kernels_cpu.F90:9-25). */
 int st_zero_flux(st_solver *s);
 /* Start-of-run sequence of run() (singleideal_cpu.F90:1547-1594):
@@ -179,7 +191,8 @@ int st_count_weno(st_solver *s, double frac_xyz[3]);
 #define ST_K_FORCE_VAR  9
 #define ST_K_DT         10
 #define ST_K_STEP_END   11
-#define ST_NKFAM        12
+#define ST_K_BC         12   /* physical ghost faces and characteristic boundaries */
+#define ST_NKFAM        13
 int st_kernel_times(st_solver *s, int32_t nsteps, double ms[ST_NKFAM], int32_t count[ST_NKFAM]);
 
 /* Stream used by every launch of this solver (NULL = the library's own). */

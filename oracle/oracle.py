@@ -55,7 +55,7 @@ def lib():
         L.or_compute_aux.argtypes = [vp, C.c_int, C.c_int]
         for name in ("or_bcswap", "or_zero_flux", "or_euler_x", "or_euler_y", "or_euler_z",
                      "or_force_rhs", "or_update_field", "or_force_var", "or_compute_dt",
-                     "or_rk_sync", "or_init_run", "or_compute_residual"):
+                     "or_rk_sync", "or_init_run", "or_compute_residual", "or_bc_nr"):
             getattr(L, name).argtypes = [vp]
         L.or_bcswap_var.argtypes = [vp, C.c_int]
         L.or_bcextr_var.argtypes = [vp, C.c_int]

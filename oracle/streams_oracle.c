@@ -53,6 +53,9 @@ static inline size_t I(const or_state *o, int i, int j, int k)
 #define EOC(i, j, k, d)  o->eoc[(size_t)(i) + (size_t)(o->nx + 1) * ((size_t)(j) + (size_t)(o->ny + 1) * (size_t)(k)) + \
                             (size_t)((d)-1) * (size_t)(o->nx + 1) * (o->ny + 1) * (o->nz + 1)]
 #define CP(l)            o->p.cp_coeff[(l) - o->p.indx_cp_l]
+#define XC(i)            o->x[(i) + o->ng - 1]
+#define YC(j)            o->y[(j) + o->ng - 1]
+#define ZC(k)            o->z[(k) + o->ng - 1]
 #define CV(l)            o->p.cv_coeff[(l) - o->p.indx_cp_l]
 
 static double dmax(double a, double b) { return a > b ? a : b; }
@@ -105,6 +108,7 @@ static void bc_preproc(or_state *o)
             o->force_zero_flux[ilat - 1] = 1;
             bctags_nr[ilat - 1] = 0;
         }
+        o->nr_eff[ilat - 1] = bctags_nr[ilat - 1];
         int off = bctags_nr[ilat - 1] != 0 ? 1 : 0;
         switch (ilat) {
         case 1: o->eul_imin += off; break;
@@ -455,14 +459,247 @@ static void bcwall_staggered(or_state *o, int ilat)
             }
 }
 
-/* update_ghost, singleideal_cpu.F90:1064-1138 (tags 0 and 6 of the
- * channel; other tags belong to the boundary-layer cases) */
+/* Ghost point (a, b, l) of face ilat: the two tangential indices and the
+ * depth l = 1..ng, and its mirror / boundary source. */
+static size_t ghost_at(const or_state *o, int ilat, int a, int b, int l)
+{
+    switch (ilat) {
+    case 1: return G(o, 1 - l, a, b);
+    case 2: return G(o, o->nx + l, a, b);
+    case 3: return G(o, a, 1 - l, b);
+    case 4: return G(o, a, o->ny + l, b);
+    case 5: return G(o, a, b, 1 - l);
+    default: return G(o, a, b, o->nz + l);
+    }
+}
+static size_t boundary_at(const or_state *o, int ilat, int a, int b)
+{
+    switch (ilat) {
+    case 1: return G(o, 1, a, b);
+    case 2: return G(o, o->nx, a, b);
+    case 3: return G(o, a, 1, b);
+    case 4: return G(o, a, o->ny, b);
+    case 5: return G(o, a, b, 1);
+    default: return G(o, a, b, o->nz);
+    }
+}
+/* tangential extents (first, second index) of face ilat */
+static void face_extent(const or_state *o, int ilat, int *na, int *nb)
+{
+    if (ilat <= 2) { *na = o->ny; *nb = o->nz; }
+    else if (ilat <= 4) { *na = o->nx; *nb = o->nz; }
+    else { *na = o->nx; *nb = o->ny; }
+}
+
+/* bcfree_subroutine (kernels_cpu.F90:5989-6056): ghosts = winf */
+static void bcfree(or_state *o, int ilat)
+{
+    int na, nb;
+    face_extent(o, ilat, &na, &nb);
+    for (int b = 1; b <= nb; b++) for (int a = 1; a <= na; a++) for (int l = 1; l <= o->ng; l++)
+        for (int m = 0; m < 5; m++) o->w[(size_t)m * o->nvol + ghost_at(o, ilat, a, b, l)] = o->p.winf[m];
+}
+
+/* bcextr_subroutine (kernels_cpu.F90:6323-6390): ghosts = boundary cell */
+static void bcextr(or_state *o, int ilat)
+{
+    int na, nb;
+    face_extent(o, ilat, &na, &nb);
+    for (int b = 1; b <= nb; b++) for (int a = 1; a <= na; a++) for (int l = 1; l <= o->ng; l++)
+        for (int m = 0; m < 5; m++)
+            o->w[(size_t)m * o->nvol + ghost_at(o, ilat, a, b, l)] = o->w[(size_t)m * o->nvol + boundary_at(o, ilat, a, b)];
+}
+
+/* bcsym_subroutine (kernels_cpu.F90:6394-6418): ymin only, w(i,1-l) =
+ * w(i,1+l) for all five variables */
+static void bcsym(or_state *o, int ilat)
+{
+    if (ilat != 3) return;
+    for (int k = 1; k <= o->nz; k++) for (int i = 1; i <= o->nx; i++) for (int l = 1; l <= o->ng; l++)
+        for (int m = 1; m <= 5; m++) W(i, 1 - l, k, m) = W(i, 1 + l, k, m);
+}
+
+/* bcwall_subroutine (kernels_cpu.F90:6478-6525), node-centred isothermal
+ * wall at ymin: the wall row j=1 is set to rest at T_wall, the ghosts mirror
+ * rows 1+l with T_g = 2 T_wall - T, p_g = p, u_g = -u. */
+static void bcwall_node(or_state *o, int ilat)
+{
+    const st_params *p = &o->p;
+    if (ilat != 3) return;
+    for (int k = 1; k <= o->nz; k++)
+        for (int i = 1; i <= o->nx; i++) {
+            W(i, 1, k, 2) = 0.0;
+            W(i, 1, k, 3) = 0.0;
+            W(i, 1, k, 4) = 0.0;
+            double ee = or_get_e_from_temperature(o, p->t_wall);
+            W(i, 1, k, 5) = W(i, 1, k, 1) * ee;
+            for (int l = 1; l <= o->ng; l++) {
+                double rho = W(i, 1 + l, k, 1);
+                double uu = W(i, 1 + l, k, 2) / rho;
+                double vv = W(i, 1 + l, k, 3) / rho;
+                double ww = W(i, 1 + l, k, 4) / rho;
+                double rhoe = W(i, 1 + l, k, 5);
+                double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+                ee = rhoe / rho - qq;
+                double tt = or_get_temperature_from_e(o, ee, WA(i, 1 + l, k, 6));
+                double pp = rho * tt * p->rgas0;
+                tt = 2.0 * p->t_wall - tt;
+                ee = or_get_e_from_temperature(o, tt);
+                rho = pp / tt / p->rgas0;
+                W(i, 1 - l, k, 1) = rho;
+                W(i, 1 - l, k, 2) = -rho * uu;
+                W(i, 1 - l, k, 3) = -rho * vv;
+                W(i, 1 - l, k, 4) = -rho * ww;
+                W(i, 1 - l, k, 5) = rho * (ee + qq);
+            }
+        }
+}
+
+/* bcshock_subroutine (kernels_cpu.F90:6159-6196), ymax: rows ny..ny+ng
+ * (the last interior row included) take the tanh blend of winf and the
+ * post-shock state near the shock trace, the value of row ny elsewhere. */
+static void bcshock(or_state *o, int ilat)
+{
+    const st_params *p = &o->p;
+    if (ilat != 4) return;
+    int ny = o->ny;
+    double tanhlen = 8.0 * p->tanhfacs;
+    for (int k = 1; k <= o->nz; k++)
+        for (int i = 1; i <= o->nx; i++)
+            for (int l = 0; l <= o->ng; l++) {
+                double xsh = p->xshock_imp - YC(ny + l) / tan(p->shock_angle);
+                double xx = XC(i) - xsh;
+                if (fabs(xx) > tanhlen) {
+                    for (int m = 1; m <= 5; m++) W(i, ny + l, k, m) = W(i, ny, k, m);
+                } else {
+                    double tanhf = 0.5 * (1.0 + tanh(xx / p->tanhfacs));
+                    for (int m = 1; m <= 5; m++) {
+                        double dwinf = p->winf_past_shock[m - 1] - p->winf[m - 1];
+                        W(i, ny + l, k, m) = p->winf[m - 1] + dwinf * tanhf;
+                    }
+                }
+            }
+}
+
+/* update_ghost, singleideal_cpu.F90:1064-1138: the physical boundary of
+ * each face in the order xmin..zmax, then the periodic swap. */
 void or_update_ghost(or_state *o, int do_swap)
 {
     for (int ilat = 1; ilat <= 6; ilat++) {
-        if (o->p.bctags[ilat - 1] == 6 && o->p.is_y_staggered) bcwall_staggered(o, ilat);
+        switch (o->p.bctags[ilat - 1]) {
+        case 1: bcfree(o, ilat); break;
+        case 2: bcextr(o, ilat); break;
+        case 5: bcsym(o, ilat); break;
+        case 6:
+            if (o->p.is_y_staggered) bcwall_staggered(o, ilat);
+            else bcwall_node(o, ilat);
+            break;
+        case 7: bcshock(o, ilat); break;
+        default: break;
+        }
     }
     if (do_swap) or_bcswap(o);
+}
+
+/* bc_nr_lat_x/y/z_kernel (kernels_cpu.F90:5169-5283, 5420-5540,
+ * 5688-5807) dispatched by bc_nr (singleideal_cpu.F90:1001-1062):
+ * characteristic treatment of the boundary cell of face ilat.  The
+ * one-sided second-order derivative of w along the normal is projected on
+ * the left eigenvectors at the boundary state; the wave amplitudes are
+ * multiplied by the eigenvalues (type 1 keeps only waves leaving the
+ * domain; types 2 and 3 replace entering waves by the difference to the
+ * neighbour or to winf; type 6 reflects the acoustic wave and drops the
+ * others) and the result is added to fl through the right eigenvectors and
+ * the 1-D metric. */
+static void bc_nr_face(or_state *o, int ilat, int nr_type)
+{
+    const st_params *p = &o->p;
+    const int dir = (ilat - 1) / 2, start = (ilat - 1) % 2 == 0;
+    const int sgn = start ? 1 : -1;
+    const double c_one[3] = {-1.5, 2.0, -0.5};
+    int na, nb;
+    face_extent(o, ilat, &na, &nb);
+    const int n = dir == 0 ? o->nx : (dir == 1 ? o->ny : o->nz);
+    const int ib = start ? 1 : n;
+    const size_t st = dir == 0 ? 1 : (dir == 1 ? (size_t)o->sx : (size_t)o->sx * o->sy);
+    const double *metric = dir == 0 ? o->dcsidx : (dir == 1 ? o->detady : o->dzitdz);
+    for (int b = 1; b <= nb; b++)
+        for (int a = 1; a <= na; a++) {
+            int i, j, k;
+            if (dir == 0) { i = ib; j = a; k = b; }
+            else if (dir == 1) { i = a; j = ib; k = b; }
+            else { i = a; j = b; k = ib; }
+            const size_t c = G(o, i, j, k);
+            double dw_dn[5], dw_dn_outer[5], dwc_dn[5], dwc_dn_outer[5], ev[5];
+            for (int m = 0; m < 5; m++) {
+                const double *wm = o->w + (size_t)m * o->nvol;
+                dw_dn[m] = 0.0;
+                for (int l = 1; l <= 3; l++)
+                    dw_dn[m] = dw_dn[m] + sgn * c_one[l - 1] * wm[(long)c + (long)sgn * (l - 1) * (long)st];
+                double w_target = 0.0;
+                if (nr_type == 2) w_target = wm[(long)c - (long)sgn * (long)st];
+                else if (nr_type == 3) w_target = p->winf[m];
+                dw_dn_outer[m] = sgn * (wm[c] - w_target);
+            }
+            const double rho = o->w_aux[c], uu = o->w_aux[o->nvol + c], vv = o->w_aux[2 * o->nvol + c];
+            const double ww = o->w_aux[3 * o->nvol + c], h = o->w_aux[4 * o->nvol + c];
+            const double tt = o->w_aux[5 * o->nvol + c];
+            const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+            const double gamloc = or_get_gamloc(o, tt);
+            const double cc = gamloc * tt * p->rgas0;
+            const double cs = sqrt(cc);
+            const double ci = 1.0 / cs;
+            const double p_rho = tt * p->rgas0;
+            const double p_e = rho * (gamloc - 1.0);
+            const double etot = h - tt * p->rgas0;
+            const double b3 = etot - rho * p_rho / p_e;
+            const double b2 = p_e / (rho * cc);
+            const double b1 = p_rho / cc - b2 * (etot - 2.0 * qq);
+            const double roe[9] = {b1, b2, b3, cs, ci, h, uu, vv, ww};
+            double el[5][5], er[5][5];
+            or_eigenvectors(dir, roe, el, er);
+            for (int m = 0; m < 5; m++) {
+                dwc_dn[m] = 0.0;
+                dwc_dn_outer[m] = 0.0;
+                for (int mm = 0; mm < 5; mm++) {
+                    dwc_dn[m] = dwc_dn[m] + el[mm][m] * dw_dn[mm];
+                    dwc_dn_outer[m] = dwc_dn_outer[m] + el[mm][m] * dw_dn_outer[mm];
+                }
+            }
+            const double un = dir == 0 ? uu : (dir == 1 ? vv : ww);
+            ev[0] = un - cs;
+            ev[1] = un;
+            ev[2] = ev[1];
+            ev[3] = ev[1];
+            ev[4] = un + cs;
+            if (nr_type == 1)
+                for (int l = 0; l < 5; l++) ev[l] = sgn * dmin(sgn * ev[l], 0.0);
+            if (nr_type == 2 || nr_type == 3)
+                for (int m = 0; m < 5; m++)
+                    if (sgn * ev[m] > 0.0) dwc_dn[m] = dwc_dn_outer[m];
+            for (int m = 0; m < 5; m++) dwc_dn[m] = ev[m] * dwc_dn[m];
+            if (nr_type == 6) {
+                dwc_dn[1] = 0.0;
+                dwc_dn[2] = 0.0;
+                dwc_dn[3] = 0.0;
+                if (start) dwc_dn[4] = dwc_dn[0];
+                else dwc_dn[0] = dwc_dn[4];
+            }
+            const int ic = dir == 0 ? i : (dir == 1 ? j : k);
+            for (int m = 0; m < 5; m++) {
+                double df = 0.0;
+                for (int mm = 0; mm < 5; mm++) df = df + er[mm][m] * dwc_dn[mm];
+                FL(i, j, k, m + 1) = FL(i, j, k, m + 1) + df * metric[ic - 1];
+            }
+        }
+}
+
+void or_bc_nr(or_state *o)
+{
+    for (int ilat = 1; ilat <= 6; ilat++) {
+        if (ilat >= 5 && o->nz == 1) break;  /* 2*ndim faces */
+        if (o->nr_eff[ilat - 1] > 0) bc_nr_face(o, ilat, o->nr_eff[ilat - 1]);
+    }
 }
 
 /* ---------------------------------------------------------- RK arithmetic */
@@ -1131,9 +1368,6 @@ static void sensor(or_state *o)
 }
 
 /* ------------------------------------------- conservative viscous form */
-#define XC(i) o->x[(i) + o->ng - 1]
-#define YC(j) o->y[(j) + o->ng - 1]
-#define ZC(k) o->z[(k) + o->ng - 1]
 
 static double cp_at(const or_state *o, double tt)
 {
@@ -1469,6 +1703,7 @@ void or_rk_sync(or_state *o)
             or_bcswap_var(o, 8);
         }
         or_visflx(o, o->p.conservative_viscous == 1 ? 7 : 1);
+        or_bc_nr(o);
         or_update_flux(o, gamdt);
         if (o->p.channel_case) or_force_rhs(o);
         or_update_field(o);

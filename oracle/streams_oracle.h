@@ -40,6 +40,7 @@ typedef struct or_state {
     double cd2[4][5];        /* coeff_deriv2(l,lmax) = cd2[lmax-1][l]     */
     int eul_imin, eul_imax, eul_jmin, eul_jmax, eul_kmin, eul_kmax;
     int force_zero_flux[6];
+    int nr_eff[6];           /* bctags_nr after bc_preproc               */
     double rhork[3], gamrk[3];
     double dt, time, dpdx, rhobulk, ubulk, tbulk;
     double residual_rhou, rhomin, rhomax, tmin, tmax, pmin, pmax;
@@ -82,6 +83,7 @@ void or_bcswap(or_state *o);
 void or_bcswap_var(or_state *o, int var);
 void or_bcextr_var(or_state *o, int var);
 void or_update_ghost(or_state *o, int do_swap);
+void or_bc_nr(or_state *o);
 void or_zero_flux(or_state *o);
 void or_init_flux(or_state *o, double rhodt);
 void or_euler_fluxes(or_state *o, int dir, int fstart, int fend);

@@ -69,14 +69,25 @@ int validate(const st_params *p) {
     if (p->indx_cp_r - p->indx_cp_l + 2 > ST_MAX_CP_COEFF) return fail(ST_ERR_ARG, "too many cp coefficients");
     if (p->precision != ST_PREC_FP64 && p->precision != ST_PREC_MIXED)
         return fail(ST_ERR_ARG, "precision must be ST_PREC_FP64 or ST_PREC_MIXED");
+    bool any_nr = false;
     for (int l = 0; l < 6; l++) {
-        int t = p->bctags[l];
-        if (t != 0 && !(t == 6 && p->is_y_staggered && (l == 2 || l == 3)))
+        const int t = p->bctags[l];
+        const bool ok = t == 0 || t == 1 || t == 2 || (t == 5 && l == 2) ||
+                        (t == 6 && (l == 2 || (l == 3 && p->is_y_staggered))) || (t == 7 && l == 3);
+        if (!ok)
             return fail(ST_ERR_ARG, "boundary tag " + std::to_string(t) + " on face " + std::to_string(l + 1) +
-                                        " is not supported (periodic 0 and staggered wall 6 on y)");
-        if (p->bctags_nr[l] != 0 && !(t == 6 && p->is_y_staggered))
-            return fail(ST_ERR_ARG, "characteristic boundary treatment is not supported");
+                                        " is not supported (0 periodic, 1 free stream, 2 extrapolation, 5 symmetry"
+                                        " on ymin, 6 wall on ymin (and ymax when staggered), 7 shock on ymax)");
+        const int nr = p->bctags_nr[l];
+        if (nr != 0 && nr != 1 && nr != 2 && nr != 3 && nr != 6)
+            return fail(ST_ERR_ARG, "characteristic tag " + std::to_string(nr) + " is not supported (0, 1, 2, 3, 6)");
+        if (nr != 0 && !(t == 6 && p->is_y_staggered)) {
+            if (t == 0) return fail(ST_ERR_ARG, "a characteristic boundary on a periodic face");
+            any_nr = true;
+        }
     }
+    if (any_nr && p->channel_case)
+        return fail(ST_ERR_ARG, "characteristic boundaries with the channel forcing are not supported");
     for (int d = 0; d < 3; d++)
         if ((p->bctags[2 * d] == 0) != (p->bctags[2 * d + 1] == 0))
             return fail(ST_ERR_ARG, "a periodic direction needs tag 0 on both faces");
@@ -105,8 +116,18 @@ void build_tables(st_solver *s, int *eoc_host[3]) {
         k.eul_hi[d] = n[d] - (nr[2 * d + 1] != 0 ? 1 : 0);
         k.periodic[d] = p.bctags[2 * d] == 0 ? 1 : 0;
     }
-    k.wall_stag[0] = p.bctags[2] == 6 && p.is_y_staggered;
-    k.wall_stag[1] = p.bctags[3] == 6 && p.is_y_staggered;
+    // the channel (periodic x/z, staggered walls in y) keeps its wall ghosts
+    // fused into k_update; any other face set runs the per-face kernels
+    s->general_bc = false;
+    s->has_nr = false;
+    for (int l = 0; l < 6; l++) {
+        k.bctag[l] = p.bctags[l];
+        k.nrtag[l] = nr[l];
+        s->has_nr |= nr[l] != 0;
+        if (p.bctags[l] != 0 && !(p.bctags[l] == 6 && p.is_y_staggered)) s->general_bc = true;
+    }
+    k.wall_stag[0] = !s->general_bc && p.bctags[2] == 6 && p.is_y_staggered;
+    k.wall_stag[1] = !s->general_bc && p.bctags[3] == 6 && p.is_y_staggered;
     const int ss = p.ep_order / 2 > p.weno_scheme ? p.ep_order / 2 : p.weno_scheme;
     for (int d = 0; d < 3; d++) {
         int *e = eoc_host[d];
@@ -135,6 +156,13 @@ void fill_kpar(st_solver *s) {
     k.ndim = p.nz > 1 ? 3 : 2;
     k.rgas0 = p.rgas0; k.prandtl = p.prandtl; k.t0 = p.t0; k.rho0 = p.rho0; k.u0 = p.u0; k.l0 = p.l0;
     k.p_ref = p.rho0 * p.rgas0 * p.t0;
+    for (int m = 0; m < 5; m++) {
+        k.winf[m] = p.winf[m];
+        k.winf_ps[m] = p.winf_past_shock[m];
+    }
+    k.xshock_imp = p.xshock_imp;
+    k.shock_angle = p.shock_angle;
+    k.tanhfacs = p.tanhfacs;
     k.mu0 = p.mu0; k.sutherland_s = p.sutherland_s; k.t_ref_dim = p.t_ref_dim; k.vtexp = p.powerlaw_vtexp;
     k.sensor_threshold = p.sensor_threshold; k.t_wall = p.t_wall; k.t_bulk_target = p.t_bulk_target;
     k.volchan = p.volchan; k.cfl = p.cfl; k.tol_iter_nr = p.tol_iter_nr;
@@ -218,8 +246,10 @@ int stage(st_solver *s, int istep) {
     if (istep == 2) TIMED(ST_K_SENSOR, launch_sensor(s, true));  // visflx(3) + bcswap_var(w_aux 8)
     TIMED(ST_K_VISC_DIV, launch_visc_div(s, fcur, fprev, istep, s->p.channel_case != 0, cons));  // visflx(1) or (7)
     if (s->p.channel_case) TIMED(ST_K_FINISH, launch_force_finish(s, istep));
+    if (s->has_nr) TIMED(ST_K_BC, launch_bc_nr(s, fcur));  // bc_nr
     TIMED(ST_K_UPDATE, launch_update(s, fprev, fcur, istep));
     if (s->p.channel_case && s->p.theta_wall >= -1.0) TIMED(ST_K_FORCE_VAR, launch_force_var(s));
+    if (s->general_bc) TIMED(ST_K_BC, launch_physical_ghosts(s));  // update_ghost(do_swap=0)
     s->cur ^= 1;
     s->last_stage = istep;
     return ST_OK;
@@ -469,7 +499,8 @@ int st_set_time(st_solver *s, double time, double dt, int64_t icyc) {
 
 int st_update_ghost(st_solver *s, int32_t do_swap) {
     NEED(s);
-    HIPCHK(st::launch_wall_ghosts(s));
+    if (s->general_bc) HIPCHK(st::launch_physical_ghosts(s));
+    else HIPCHK(st::launch_wall_ghosts(s));
     if (do_swap) HIPCHK(st::launch_fill_periodic(s, s->w, 5));
     return ST_OK;
 }
@@ -529,6 +560,12 @@ int st_euler_y(st_solver *s) {
 int st_euler_z(st_solver *s) {
     NEED(s);
     HIPCHK(st::launch_euler_z(s, s->F[s->cur]));
+    return ST_OK;
+}
+
+int st_bc_nr(st_solver *s) {
+    NEED(s);
+    if (s->has_nr) HIPCHK(st::launch_bc_nr(s, s->F[s->cur]));
     return ST_OK;
 }
 

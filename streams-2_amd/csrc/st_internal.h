@@ -42,11 +42,13 @@ struct KPar {
     int visc_model;
     int fzf[6];            // force_zero_flux
     int periodic[3];
-    int wall_stag[2];      // ymin / ymax staggered isothermal wall
+    int wall_stag[2];      // ymin / ymax staggered isothermal wall written by k_update
+    int bctag[6], nrtag[6];  // physical tag and effective characteristic tag of each face
     int eul_lo[3], eul_hi[3];
     int ndim;
     double rgas0, prandtl, t0, rho0, u0, l0, mu0;
     double p_ref;          // rho0*rgas0*t0: the mixed form stores p - p_ref
+    double winf[5], winf_ps[5], xshock_imp, shock_angle, tanhfacs;  // free-stream / shock boundary data
     double sutherland_s, t_ref_dim, vtexp;
     double sensor_threshold, t_wall, t_bulk_target, volchan, cfl, tol_iter_nr;
     double cp[ST_MAX_CP_COEFF], cv[ST_MAX_CP_COEFF];

@@ -182,6 +182,201 @@ __global__ void k_wall_ghosts(Geo g, KPar P, double *__restrict__ w, const R *__
     }
 }
 
+// ------------------------------------------------- physical boundary faces
+// Ghost point l (1..ng) and boundary point of face ilat (1..6 = xmin..zmax)
+// at tangential indices (a, b).
+__device__ __forceinline__ int face_ghost(const Geo &g, int ilat, int a, int b, int l) {
+    switch (ilat) {
+    case 1: return gidx(g, 1 - l, a, b);
+    case 2: return gidx(g, g.nx + l, a, b);
+    case 3: return gidx(g, a, 1 - l, b);
+    case 4: return gidx(g, a, g.ny + l, b);
+    case 5: return gidx(g, a, b, 1 - l);
+    default: return gidx(g, a, b, g.nz + l);
+    }
+}
+__device__ __forceinline__ int face_boundary(const Geo &g, int ilat, int a, int b) {
+    switch (ilat) {
+    case 1: return gidx(g, 1, a, b);
+    case 2: return gidx(g, g.nx, a, b);
+    case 3: return gidx(g, a, 1, b);
+    case 4: return gidx(g, a, g.ny, b);
+    case 5: return gidx(g, a, b, 1);
+    default: return gidx(g, a, b, g.nz);
+    }
+}
+
+// One face of update_ghost (singleideal_cpu.F90:1064-1138), one thread per
+// tangential point: free stream (bcfree_subroutine, kernels_cpu.F90:
+// 5989-6056), extrapolation (bcextr_subroutine, :6323-6390), symmetry on
+// ymin (bcsym_subroutine, :6394-6418), staggered or node isothermal wall
+// on ymin (bcwall_staggered_subroutine :6612-6678, bcwall_subroutine
+// :6478-6525), oblique shock on ymax (bcshock_subroutine, :6159-6196).
+template <typename R>
+__global__ void k_bc_face(Geo g, KPar P, double *__restrict__ w, const R *__restrict__ wa,
+                          const double *__restrict__ xc, const double *__restrict__ yc, int ilat, int tag,
+                          int staggered) {
+    const int na = ilat <= 2 ? g.ny : g.nx;
+    const int nb = ilat <= 4 ? g.nz : g.ny;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na * nb) return;
+    const int a = t % na + 1, b = t / na + 1;
+    const size_t nv = g.nvol;
+    const int ng = g.ng;
+    if (tag == 1) {
+        for (int l = 1; l <= ng; l++) {
+            const int d = face_ghost(g, ilat, a, b, l);
+            for (int m = 0; m < 5; m++) w[m * nv + d] = P.winf[m];
+        }
+    } else if (tag == 2) {
+        const int s0 = face_boundary(g, ilat, a, b);
+        for (int l = 1; l <= ng; l++) {
+            const int d = face_ghost(g, ilat, a, b, l);
+            for (int m = 0; m < 5; m++) w[m * nv + d] = w[m * nv + s0];
+        }
+    } else if (tag == 5 && ilat == 3) {
+        for (int l = 1; l <= ng; l++) {
+            const int d = gidx(g, a, 1 - l, b), s0 = gidx(g, a, 1 + l, b);
+            for (int m = 0; m < 5; m++) w[m * nv + d] = w[m * nv + s0];
+        }
+    } else if (tag == 6 && staggered) {
+        for (int l = 1; l <= ng; l++) {
+            if (ilat == 3) wall_ghost(g, P, w, wa, a, l, 1 - l, b);
+            else if (ilat == 4) wall_ghost(g, P, w, wa, a, g.ny + 1 - l, g.ny + l, b);
+        }
+    } else if (tag == 6 && ilat == 3) {
+        const int c1 = gidx(g, a, 1, b);
+        w[nv + c1] = 0.0;
+        w[2 * nv + c1] = 0.0;
+        w[3 * nv + c1] = 0.0;
+        double ee = e_from_temperature(P, P.t_wall);
+        w[4 * nv + c1] = w[c1] * ee;
+        for (int l = 1; l <= ng; l++) {
+            const int s0 = gidx(g, a, 1 + l, b), d = gidx(g, a, 1 - l, b);
+            double rho = w[s0];
+            const double uu = w[nv + s0] / rho, vv = w[2 * nv + s0] / rho, ww = w[3 * nv + s0] / rho;
+            const double rhoe = w[4 * nv + s0];
+            const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+            ee = rhoe / rho - qq;
+            double tt = temperature_from_e(P, ee, double(wa[5 * nv + s0]));
+            const double pp = rho * tt * P.rgas0;
+            tt = 2.0 * P.t_wall - tt;
+            ee = e_from_temperature(P, tt);
+            rho = pp / tt / P.rgas0;
+            w[d] = rho;
+            w[nv + d] = -rho * uu;
+            w[2 * nv + d] = -rho * vv;
+            w[3 * nv + d] = -rho * ww;
+            w[4 * nv + d] = rho * (ee + qq);
+        }
+    } else if (tag == 7 && ilat == 4) {
+        const double tanhlen = 8.0 * P.tanhfacs;
+        const int i = a, k = b;
+        const int cn = gidx(g, i, g.ny, k);
+        for (int l = 0; l <= ng; l++) {
+            const double xsh = P.xshock_imp - yc[g.ny + l + ng - 1] / tan(P.shock_angle);
+            const double xx = xc[i + ng - 1] - xsh;
+            const int d = gidx(g, i, g.ny + l, k);
+            if (fabs(xx) > tanhlen) {
+                for (int m = 0; m < 5; m++) w[m * nv + d] = w[m * nv + cn];
+            } else {
+                const double tanhf = 0.5 * (1.0 + tanh(xx / P.tanhfacs));
+                for (int m = 0; m < 5; m++) {
+                    const double dwinf = P.winf_ps[m] - P.winf[m];
+                    w[m * nv + d] = P.winf[m] + dwinf * tanhf;
+                }
+            }
+        }
+    }
+}
+
+// bc_nr_lat_x/y/z_kernel (kernels_cpu.F90:5169-5283, 5420-5540, 5688-5807),
+// one thread per point of the boundary plane, FP64 throughout: the
+// one-sided derivative of w along the normal in characteristic variables
+// at the boundary state, scaled by the (clipped) eigenvalues, back through
+// the right eigenvectors into fl.
+template <typename R, int DIR>
+__global__ void k_bc_nr(Geo g, KPar P, const double *__restrict__ w, const R *__restrict__ wa, R *__restrict__ fl,
+                        const double *__restrict__ metric, int start, int nr_type) {
+    const int na = DIR == 0 ? g.ny : g.nx;
+    const int nb = DIR == 2 ? g.ny : g.nz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na * nb) return;
+    const int a = t % na + 1, b = t / na + 1;
+    const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
+    const int ib = start ? 1 : n;
+    const int i = DIR == 0 ? ib : a, j = DIR == 1 ? ib : (DIR == 0 ? a : b), k = DIR == 2 ? ib : b;
+    const int sgn = start ? 1 : -1;
+    const int st = DIR == 0 ? 1 : (DIR == 1 ? g.sx : g.sxy);
+    const size_t nv = g.nvol;
+    const int c = gidx(g, i, j, k);
+    const double c_one[3] = {-1.5, 2.0, -0.5};
+    double dw_dn[5], dw_dn_outer[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const double *wm = w + m * nv;
+        double d = 0.0;
+        for (int l = 1; l <= 3; l++) d = d + sgn * c_one[l - 1] * wm[c + sgn * (l - 1) * st];
+        dw_dn[m] = d;
+        double w_target = 0.0;
+        if (nr_type == 2) w_target = wm[c - sgn * st];
+        else if (nr_type == 3) w_target = P.winf[m];
+        dw_dn_outer[m] = sgn * (wm[c] - w_target);
+    }
+    const double rho = wa[c], uu = wa[nv + c], vv = wa[2 * nv + c], ww = wa[3 * nv + c];
+    const double h = wa[4 * nv + c], tt = wa[5 * nv + c];
+    const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+    const double gamloc = gamloc_of(P, tt);
+    const double cc = gamloc * tt * P.rgas0;
+    Roe s;
+    s.c = sqrt(cc);
+    s.ci = 1.0 / s.c;
+    const double p_rho = tt * P.rgas0;
+    const double p_e = rho * (gamloc - 1.0);
+    const double etot = h - tt * P.rgas0;
+    s.b3 = etot - rho * p_rho / p_e;
+    s.b2 = p_e / (rho * cc);
+    s.b1 = p_rho / cc - s.b2 * (etot - 2.0 * qq);
+    s.h = h; s.uu = uu; s.vv = vv; s.ww = ww;
+    double dwc[5], dwc_outer[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        double el[5];
+        eig_left<DIR>(s, m, el);
+        double x = 0.0, y = 0.0;
+        for (int v = 0; v < 5; v++) {
+            x = x + el[v] * dw_dn[v];
+            y = y + el[v] * dw_dn_outer[v];
+        }
+        dwc[m] = x;
+        dwc_outer[m] = y;
+    }
+    const double un = DIR == 0 ? uu : (DIR == 1 ? vv : ww);
+    double ev[5] = {un - s.c, un, un, un, un + s.c};
+    if (nr_type == 1)
+        for (int l = 0; l < 5; l++) ev[l] = sgn * fmin(sgn * ev[l], 0.0);
+    if (nr_type == 2 || nr_type == 3)
+        for (int m = 0; m < 5; m++)
+            if (sgn * ev[m] > 0.0) dwc[m] = dwc_outer[m];
+    for (int m = 0; m < 5; m++) dwc[m] = ev[m] * dwc[m];
+    if (nr_type == 6) {
+        dwc[1] = 0.0;
+        dwc[2] = 0.0;
+        dwc[3] = 0.0;
+        if (start) dwc[4] = dwc[0];
+        else dwc[0] = dwc[4];
+    }
+    double df[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int mm = 0; mm < 5; mm++) {
+        double er[5];
+        eig_right<DIR>(s, mm, er);
+        for (int m = 0; m < 5; m++) df[m] = df[m] + er[m] * dwc[mm];
+    }
+    const double d = metric[ib - 1];
+    const int oi = iidx(g, i, j, k);
+    for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] = R(double(fl[(size_t)m * g.nint + oi]) + df[m] * d);
+}
+
 // -------------------------------------------------------------------- aux
 // eval_aux_subroutine (kernels_cpu.F90:6960-7011) on the box
 // [i0,i1]x[j0,j1]x[k0,k1]; with image=true a ghost point of a periodic
@@ -1785,6 +1980,43 @@ hipError_t launch_update(st_solver *s, const void *fl_prev, const void *fl, int 
     else ST_LAUNCH_U(false);
 #undef ST_LAUNCH_U
     return hipGetLastError();
+}
+
+hipError_t launch_physical_ghosts(st_solver *s) {
+    const Geo &g = s->g;
+    for (int ilat = 1; ilat <= 2 * s->kp.ndim; ilat++) {
+        const int tag = s->kp.bctag[ilat - 1];
+        if (tag == 0) continue;
+        const int n = (ilat <= 2 ? g.ny : g.nx) * (ilat <= 4 ? g.nz : g.ny);
+        ST_DISPATCH(hipLaunchKernelGGL(k_bc_face<R>, dim3((n + 127) / 128), dim3(128), 0, s->stream, g, s->kp, s->w,
+                                       (const R *)s->w_aux, s->x, s->y, ilat, tag, s->p.is_y_staggered));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_bc_nr(st_solver *s, void *fl) {
+    const Geo &g = s->g;
+    for (int ilat = 1; ilat <= 2 * s->kp.ndim; ilat++) {
+        const int nr = s->kp.nrtag[ilat - 1];
+        if (nr == 0) continue;
+        const int dir = (ilat - 1) / 2, start = (ilat - 1) % 2 == 0 ? 1 : 0;
+        const int n = (dir == 0 ? g.ny : g.nx) * (dir == 2 ? g.ny : g.nz);
+        const dim3 grid((n + 127) / 128), block(128);
+        if (dir == 0)
+            ST_DISPATCH(hipLaunchKernelGGL((k_bc_nr<R, 0>), grid, block, 0, s->stream, g, s->kp, s->w,
+                                           (const R *)s->w_aux, (R *)fl, s->dcsidx, start, nr));
+        else if (dir == 1)
+            ST_DISPATCH(hipLaunchKernelGGL((k_bc_nr<R, 1>), grid, block, 0, s->stream, g, s->kp, s->w,
+                                           (const R *)s->w_aux, (R *)fl, s->detady, start, nr));
+        else
+            ST_DISPATCH(hipLaunchKernelGGL((k_bc_nr<R, 2>), grid, block, 0, s->stream, g, s->kp, s->w,
+                                           (const R *)s->w_aux, (R *)fl, s->dzitdz, start, nr));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_force_var(st_solver *s) {

@@ -15,6 +15,8 @@ struct st_solver {
     void *w_aux;        // 10 ghosted variables (double, or float in the mixed form)
     void *F[2];         // right-hand sides of the current and the previous stage (same type)
     bool mixed;         // ST_PREC_MIXED: w_aux and F in float
+    bool general_bc;    // boundary faces other than periodic / staggered wall: per-face ghost kernels
+    bool has_nr;        // a characteristic boundary treatment is active
     int visc_chunk;     // planes per tile of the z-marching viscous kernel (0: automatic)
     int march_chunk;    // points per thread of the y/z flux marches (0: automatic)
     double *wallprop;   // 3 x (sx*sz), wallprop(:,:,2:4)
@@ -55,6 +57,11 @@ hipError_t launch_fill_periodic_aux(st_solver *s, int var);
 hipError_t launch_extrapolate_aux(st_solver *s, int var);
 hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int k1, bool image);
 hipError_t launch_wall_ghosts(st_solver *s);
+// physical ghosts of every non-periodic face in the order xmin..zmax
+// (update_ghost without the periodic swap)
+hipError_t launch_physical_ghosts(st_solver *s);
+// characteristic boundary treatment added to fl (bc_nr)
+hipError_t launch_bc_nr(st_solver *s, void *fl);
 hipError_t launch_euler_x(st_solver *s, void *fl, bool init);
 hipError_t launch_euler_y(st_solver *s, void *fl);
 hipError_t launch_euler_z(st_solver *s, void *fl);

@@ -29,6 +29,8 @@ module streams_mi355x
     real(c_double) :: sutherland_s, t_ref_dim, powerlaw_vtexp
     real(c_double) :: sensor_threshold, t_wall, theta_wall, t_bulk_target, volchan, cfl, tol_iter_nr
     real(c_double) :: cp_coeff(st_max_cp_coeff), cv_coeff(st_max_cp_coeff)
+    real(c_double) :: winf(5), winf_past_shock(5)
+    real(c_double) :: xshock_imp, shock_angle, tanhfacs
   end type st_params_t
 
   type, bind(c), public :: st_scalars_t
@@ -102,6 +104,10 @@ module streams_mi355x
       import :: c_int, c_ptr
       type(c_ptr), value :: s
     end function
+    integer(c_int) function st_bc_nr(s) bind(c, name="st_bc_nr")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: s
+    end function
     integer(c_int) function st_init_run(s) bind(c, name="st_init_run")
       import :: c_int, c_ptr
       type(c_ptr), value :: s
@@ -150,6 +156,7 @@ module streams_mi355x
     procedure :: euler_x => mi_euler_x
     procedure :: euler_y => mi_euler_y
     procedure :: euler_z => mi_euler_z
+    procedure :: bc_nr => mi_bc_nr
     procedure :: init_run => mi_init_run
     procedure :: compute_dt => mi_compute_dt
     procedure :: rk_sync => mi_rk_sync
@@ -251,6 +258,11 @@ contains
     class(equation_singleideal_mi355x_object), intent(inout) :: self
     call check(st_euler_z(self%handle), 'euler_z')
   end subroutine mi_euler_z
+
+  subroutine mi_bc_nr(self)
+    class(equation_singleideal_mi355x_object), intent(inout) :: self
+    call check(st_bc_nr(self%handle), 'bc_nr')
+  end subroutine mi_bc_nr
 
   subroutine mi_init_run(self)
     class(equation_singleideal_mi355x_object), intent(inout) :: self

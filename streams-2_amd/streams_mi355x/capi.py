@@ -19,7 +19,7 @@ ST_ERR_STATE = 4
 ST_ERR_NAN = 5
 
 KERNEL_FAMILIES = ("aux", "euler_x", "visc", "euler_y", "euler_z", "sensor", "visc_div", "finish",
-                   "update", "force_var", "dt", "step_end")
+                   "update", "force_var", "dt", "step_end", "bc")
 
 LIB_NAME = "libstreams_mi355x.so"
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -47,6 +47,7 @@ _SIGNATURES = {
     "st_euler_y": (C.c_int, [C.c_void_p]),
     "st_euler_z": (C.c_int, [C.c_void_p]),
     "st_zero_flux": (C.c_int, [C.c_void_p]),
+    "st_bc_nr": (C.c_int, [C.c_void_p]),
     "st_init_run": (C.c_int, [C.c_void_p]),
     "st_compute_dt": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "st_rk_sync": (C.c_int, [C.c_void_p]),

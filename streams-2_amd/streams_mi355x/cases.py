@@ -8,7 +8,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from .grid import Grid, channel_grid, channel_initial_field
-from .params import FlowCase, StParams, channel_params
+from .params import FlowCase, StParams, channel_params, set_fluid_prop
 
 
 @dataclass
@@ -140,3 +140,145 @@ def make_subsonic_channel(nx=80, ny=60, nz=80) -> ChannelCase:
     params = channel_params(case, grid.yn)
     w0 = init_channel_field(grid, params, case.lz, splits=(2, 1, 2), rand_type=2)
     return ChannelCase(case, grid, params, w0)
+
+
+def bl_params(nx, ny, nz, mach=2.0, reynolds=200.0, t_wall=1.6, bctags=(1, 2, 6, 2, 0, 0),
+              bctags_nr=(0, 1, 0, 1, 0, 0), sensor_threshold=0.2, cfl=0.5, conservative_viscous=1,
+              shock_angle_deg=None, xshock_imp=None, tanhfacs=None, precision=0) -> StParams:
+    """Parameters of a boundary-layer box (flow_init = 1 family): node-centred
+    grid, free-stream reference state rho0 = t0 = rgas0 = l0 = 1, u0 = M c0,
+    mu0 = u0 rho0 l0 / Re (set_flow_params, singleideal.F90:2798-2826), an
+    isothermal wall at T_wall, no forcing.  winf is the free-stream
+    conserved state; with shock_angle_deg the post-shock state of the
+    oblique-shock relations (set_oblique_shock, singleideal.F90:2955-3017)
+    fills winf_past_shock."""
+    case = FlowCase(nx=nx, ny=ny, nz=nz, mach=mach, reynolds=reynolds, sensor_threshold=sensor_threshold, cfl=cfl,
+                    conservative_viscous=conservative_viscous, precision=precision)
+    gam, gm1, rfac, icl, icr, cp, cv = set_fluid_prop(case)
+    rgas0 = t0 = rho0 = l0 = 1.0
+    c0 = math.sqrt(gam * rgas0 * t0)
+    u0 = mach * c0
+    p = StParams()
+    p.nx, p.ny, p.nz, p.ng = nx, ny, nz, 3
+    p.ep_order = p.visc_order = 6
+    p.nkeep = 0
+    p.weno_scheme = 3
+    p.weno_version = 0
+    p.flux_splitting = 0
+    p.conservative_viscous = conservative_viscous
+    p.visc_model = 2
+    p.calorically_perfect = 1
+    p.indx_cp_l, p.indx_cp_r = icl, icr
+    p.bctags[:] = list(bctags)
+    p.bctags_nr[:] = list(bctags_nr)
+    p.is_y_staggered = 0
+    p.channel_case = 0
+    p.correct_bound_ord = 1
+    p.iter_dt_recompute = 1
+    p.precision = precision
+    p.rgas0, p.prandtl, p.t0, p.rho0, p.u0, p.l0 = rgas0, case.prandtl, t0, rho0, u0, l0
+    p.mu0 = u0 * rho0 * l0 / reynolds
+    p.sutherland_s, p.t_ref_dim, p.powerlaw_vtexp = 110.4, 220.0, 0.76
+    p.sensor_threshold = sensor_threshold
+    p.t_wall = t_wall
+    p.theta_wall = 1.0
+    p.volchan = 1.0
+    p.cfl = cfl
+    p.tol_iter_nr = 1e-12
+    for i, v in enumerate(cp):
+        p.cp_coeff[i] = v
+    for i, v in enumerate(cv):
+        p.cv_coeff[i] = v
+    e0 = cv[1] * t0  # calorically perfect e(t0)
+    winf = [rho0, rho0 * u0, 0.0, 0.0, rho0 * (e0 + 0.5 * u0 ** 2)]
+    for m in range(5):
+        p.winf[m] = winf[m]
+        p.winf_past_shock[m] = winf[m]
+    if shock_angle_deg is not None:
+        beta = shock_angle_deg * math.pi / 180.0
+        sins, coss = math.sin(beta), math.cos(beta)
+        mn2 = (mach * sins) ** 2
+        mn_ps = math.sqrt((1.0 + 0.5 * gm1 * mn2) / (gam * mn2 - 0.5 * gm1))
+        vel_ratio = (2.0 + gm1 * mn2) / ((gam + 1.0) * mn2)
+        rho_ratio = 1.0 / vel_ratio
+        p_ratio = 1.0 + 2.0 * gam / (gam + 1.0) * (mn2 - 1.0)
+        t_ratio = p_ratio / rho_ratio
+        tdef = 2.0 * coss / sins * (mach ** 2 * sins ** 2 - 1.0) / (2.0 + mach ** 2 * (gam + math.cos(2.0 * beta)))
+        delta = math.atan(tdef)
+        m_ps = mn_ps / math.sin(beta - delta)
+        t_ps = t0 * t_ratio
+        c_ps = math.sqrt(gam * rgas0 * t_ps)
+        u_ps, v_ps = m_ps * c_ps * math.cos(delta), m_ps * c_ps * math.sin(delta)
+        rho_ps = rho0 * rho_ratio
+        e_ps = cv[1] * t0 + cv[0] * (t_ps - t0)
+        wps = [rho_ps, rho_ps * u_ps, -rho_ps * v_ps, 0.0, rho_ps * (e_ps + 0.5 * (u_ps ** 2 + v_ps ** 2))]
+        for m in range(5):
+            p.winf_past_shock[m] = wps[m]
+        p.shock_angle = beta
+        p.xshock_imp = xshock_imp
+        p.tanhfacs = tanhfacs
+    return p
+
+
+def bl_grid(nx, ny, nz, lx=8.0, ly=3.0, lz=2.0, beta_y=1.8) -> Grid:
+    """Node-centred grid of a boundary-layer box: uniform x (non-periodic:
+    nx-1 intervals) and periodic z, y from the wall y=0 with tanh
+    clustering (y_j = ly tanh(beta s)/tanh(beta), s = (j-1)/(ny-1)), ghosts
+    mirrored about the wall and extended linearly at the top; 6th-order
+    metrics (grid.F90:71-141)."""
+    from .grid import local_metrics, metrics, uniform_nodes
+
+    ng = 3
+    xg = uniform_nodes(nx, ng, lx, False)
+    zg = uniform_nodes(nz, ng, lz, True)
+    j = np.arange(1 - ng, ny + ng + 1, dtype=np.float64)
+    s = (j - 1.0) / (ny - 1.0)
+    yg = ly * np.tanh(beta_y * s) / math.tanh(beta_y)
+    # mirror below the wall, linear continuation above the top
+    for l in range(1, ng + 1):
+        yg[ng - l] = -yg[ng + l]
+        yg[ng + ny - 1 + l] = 2.0 * yg[ng + ny - 1] - yg[ng + ny - 1 - l]
+    yg = yg[: ny + 2 * ng]
+    yn = np.concatenate([[0.0], 0.5 * (yg[ng:ng + ny - 1] + yg[ng + 1:ng + ny]), [yg[ng + ny - 1]]])
+    dx1, dx2 = metrics(xg, nx, ng, 6)
+    dy1, dy2 = metrics(yg, ny, ng, 6)
+    dz1, dz2 = metrics(zg, nz, ng, 6)
+    return Grid(nx, ny, nz, ng, xg[: nx + 2 * ng].copy(), yg.copy(), zg[: nz + 2 * ng].copy(), yn,
+                *local_metrics(dx1, dx2), *local_metrics(dy1, dy2), *local_metrics(dz1, dz2))
+
+
+def bl_initial_field(grid: Grid, params: StParams, delta=0.6, noise=0.05, seed=0) -> np.ndarray:
+    """Seeded compressible boundary-layer-like field: u = u0 tanh(y/delta),
+    T from the Crocco-Busemann relation between T_wall and T_inf = t0 with
+    recovery factor Pr^(1/3), uniform pressure, plus ``noise`` (peak-to-peak
+    fraction of u0) of uniform velocity fluctuations damped toward the wall.
+    Ghosts are zero (filled by update_ghost)."""
+    nx, ny, nz, ng = grid.nx, grid.ny, grid.nz, grid.ng
+    u0, t0, rgas0 = params.u0, params.t0, params.rgas0
+    gam = params.cp_coeff[0] / params.cv_coeff[0]
+    rfac = params.prandtl ** (1.0 / 3.0)
+    tr = t0 * (1.0 + 0.5 * (gam - 1.0) * rfac * (u0 ** 2 / (gam * rgas0 * t0)))
+    y = grid.y[ng:ng + ny]
+    uo = np.tanh(np.maximum(y, 0.0) / delta)
+    tt = params.t_wall + (tr - params.t_wall) * uo + (t0 - tr) * uo ** 2
+    r = np.random.default_rng(seed).random((3, nz, ny, nx)) - 0.5
+    damp = (uo * (1.0 - uo) * 4.0)[None, :, None]
+    uu = u0 * uo[None, :, None] + noise * u0 * r[0] * damp
+    vv = noise * u0 * r[1] * damp
+    ww = noise * u0 * r[2] * damp
+    rho = (params.rho0 * t0 / tt)[None, :, None] * np.ones((nz, 1, nx))
+    ee = params.cv_coeff[1] * t0 + params.cv_coeff[0] * (tt - t0)
+    w = np.zeros((5, nz + 2 * ng, ny + 2 * ng, nx + 2 * ng))
+    sl = (slice(ng, ng + nz), slice(ng, ng + ny), slice(ng, ng + nx))
+    w[0][sl] = rho
+    w[1][sl] = rho * uu
+    w[2][sl] = rho * vv
+    w[3][sl] = rho * ww
+    w[4][sl] = rho * ee[None, :, None] + 0.5 * rho * (uu ** 2 + vv ** 2 + ww ** 2)
+    return w
+
+
+def make_bl_box(nx=40, ny=24, nz=16, seed=0, **kw) -> ChannelCase:
+    g = bl_grid(nx, ny, nz)
+    p = bl_params(nx, ny, nz, **kw)
+    return ChannelCase(FlowCase(nx=nx, ny=ny, nz=nz), g, p, bl_initial_field(g, p, seed=seed))

@@ -43,6 +43,8 @@ class StParams(C.Structure):
         ("t_bulk_target", C.c_double), ("volchan", C.c_double), ("cfl", C.c_double),
         ("tol_iter_nr", C.c_double),
         ("cp_coeff", C.c_double * MAX_CP), ("cv_coeff", C.c_double * MAX_CP),
+        ("winf", C.c_double * 5), ("winf_past_shock", C.c_double * 5),
+        ("xshock_imp", C.c_double), ("shock_angle", C.c_double), ("tanhfacs", C.c_double),
     ]
 
 

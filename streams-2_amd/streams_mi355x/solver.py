@@ -112,6 +112,9 @@ class ChannelSolver:
     def zero_flux(self):
         capi.check(self.L.st_zero_flux(self.h))
 
+    def bc_nr(self):
+        capi.check(self.L.st_bc_nr(self.h))
+
     def init_run(self):
         capi.check(self.L.st_init_run(self.h))
 

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_sizes_match_header():
     # st_params: 22 int32 + 6+6 int32 arrays = 34 int32 (136 B), 17 doubles, 2 x 16 doubles
-    assert C.sizeof(StParams) == 34 * 4 + 17 * 8 + 32 * 8
+    assert C.sizeof(StParams) == 34 * 4 + 17 * 8 + 32 * 8 + 13 * 8
     assert C.sizeof(StScalars) == 13 * 8 + 8
 
 

@@ -1,0 +1,86 @@
+"""Boundary conditions of the boundary-layer family on the GPU against the
+oracle: the physical ghost faces of update_ghost, the characteristic
+treatment bc_nr, and short trajectories of a boundary-layer box."""
+import numpy as np
+import pytest
+
+from helpers import interior, rel_l2
+from oracle import Oracle
+from streams_mi355x.cases import make_bl_box
+from streams_mi355x.solver import ChannelSolver
+
+pytestmark = pytest.mark.gpu
+
+NG = 3
+TRAJ_TOL = 1e-10
+
+BC_SETS = {
+    "inflow-outflow-wall-nr": dict(bctags=(1, 2, 6, 2, 0, 0), bctags_nr=(0, 1, 0, 1, 0, 0)),
+    "shock-top": dict(bctags=(1, 2, 6, 7, 0, 0), bctags_nr=(0, 1, 0, 0, 0, 0), shock_angle_deg=35.0,
+                      xshock_imp=5.0, tanhfacs=8.0 / 39),
+    "symmetry-relaxation": dict(bctags=(1, 2, 5, 1, 0, 0), bctags_nr=(3, 2, 0, 3, 0, 0)),
+    "reflective-wall": dict(bctags=(2, 2, 6, 2, 0, 0), bctags_nr=(1, 1, 6, 1, 0, 0)),
+}
+
+
+def _pair(name, **kw):
+    c = make_bl_box(**BC_SETS[name], **kw)
+    return c, Oracle(c.params, c.grid), ChannelSolver(c.params, c.grid)
+
+
+@pytest.mark.parametrize("name", list(BC_SETS))
+def test_update_ghost_parity(name):
+    c, o, g = _pair(name)
+    o.w[:] = c.w0
+    g.copy_from_field(c.w0)
+    o.compute_aux(1, 1)
+    g.compute_aux(1, 1)
+    o.update_ghost(1)
+    g.update_ghost(1)
+    wg = g.copy_to_field()
+    # every point the boundary routines write: faces (no edges or corners)
+    nz, ny, nx = c.params.nz, c.params.ny, c.params.nx
+    for m in range(5):
+        a, b = wg[m], o.w[m]
+        assert rel_l2(a[NG:-NG, NG:-NG, :], b[NG:-NG, NG:-NG, :]) <= 1e-14, (name, m)
+        assert rel_l2(a[NG:-NG, :, NG:-NG], b[NG:-NG, :, NG:-NG]) <= 1e-14, (name, m)
+        assert rel_l2(a[:, NG:-NG, NG:-NG], b[:, NG:-NG, NG:-NG]) <= 1e-14, (name, m)
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["inflow-outflow-wall-nr", "symmetry-relaxation", "reflective-wall"])
+def test_bc_nr_parity(name):
+    c, o, g = _pair(name)
+    o.w[:] = c.w0
+    g.copy_from_field(c.w0)
+    o.update_ghost(1)
+    g.update_ghost(1)
+    o.compute_aux(1, 1)
+    g.compute_aux(1, 1)
+    o.zero_flux()
+    g.zero_flux()
+    o.bc_nr()
+    g.bc_nr()
+    fo, fg = o.fl, g.get_fl()
+    assert np.abs(fo).max() > 0.0
+    for m in range(5):
+        assert rel_l2(fg[m], fo[m]) <= 1e-12, (name, m)
+    g.close()
+
+
+@pytest.mark.parametrize("name", list(BC_SETS))
+def test_bl_trajectory_20_steps(name):
+    c, o, g = _pair(name)
+    o.w[:] = c.w0
+    g.copy_from_field(c.w0)
+    o.init_run()
+    g.init_run()
+    o.advance(20)
+    g.advance(20)
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    print(name, "20-step rel L2 per variable:", errs)
+    assert max(errs) <= TRAJ_TOL, (name, errs)
+    so, sg = o.scalars(), g.scalars()
+    assert abs(sg["dt"] - so["dt"]) <= 1e-12 * so["dt"]
+    g.close()

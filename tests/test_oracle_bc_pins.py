@@ -1,0 +1,270 @@
+"""Pins of the oracle's boundary conditions of the boundary-layer family.
+
+Ghost faces are checked against index formulas and closed forms; the
+characteristic (non-reflecting) treatment against the flux Jacobian built
+by finite differences of the Euler flux, with left eigenvectors from
+numpy's eigen-decomposition of that Jacobian, so none of the oracle's
+eigenvector formulas is retyped here.
+"""
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from streams_mi355x.cases import bl_params, make_bl_box
+from streams_mi355x.params import params_copy
+
+NG = 3
+
+
+def _oracle(**kw):
+    c = make_bl_box(**kw)
+    o = Oracle(c.params, c.grid)
+    rng = np.random.default_rng(3)
+    w = c.w0.copy()
+    w[:, :, :, :] += 0.0
+    inner = (slice(NG, -NG),) * 3
+    for m in range(5):
+        w[m][inner] *= 1.0 + 0.05 * rng.uniform(-1, 1, w[m][inner].shape)
+    o.w[:] = w
+    o.compute_aux(1, 0)
+    return c, o
+
+
+def test_free_stream_extrapolation_and_symmetry_faces():
+    c, o = _oracle(bctags=(1, 2, 5, 2, 0, 0), bctags_nr=(0, 0, 0, 0, 0, 0))
+    w0 = o.w.copy()
+    o.update_ghost(0)
+    w = o.w
+    winf = np.array(c.params.winf[:])
+    ny, nx = c.params.ny, c.params.nx
+    jj, kk = slice(NG, NG + ny), slice(NG, NG + c.params.nz)
+    for l in range(1, NG + 1):
+        # xmin free stream
+        assert np.all(w[:, kk, jj, NG - l] == winf[:, None, None])
+        # xmax extrapolation of the last cell
+        assert np.array_equal(w[:, kk, jj, NG + nx - 1 + l], w0[:, kk, jj, NG + nx - 1])
+        # ymin symmetry: w(1-l) = w(1+l), all five variables, no sign change
+        assert np.array_equal(w[:, kk, NG - l, NG:NG + nx], w0[:, kk, NG + l, NG:NG + nx])
+    assert np.array_equal(w[:, kk, jj, NG:NG + nx], w0[:, kk, jj, NG:NG + nx])  # interior untouched
+
+
+def test_node_wall_face():
+    c, o = _oracle(bctags=(2, 2, 6, 2, 0, 0), bctags_nr=(0, 0, 0, 0, 0, 0))
+    p = c.params
+    w0 = o.w.copy()
+    o.update_ghost(0)
+    w = o.w
+    cv0, cv1, tw = p.cv_coeff[0], p.cv_coeff[1], p.t_wall
+    e_of = lambda t: cv1 * p.t0 + cv0 * (t - p.t0)  # calorically perfect e(T)
+    kk, ii = slice(NG, NG + p.nz), slice(NG, NG + p.nx)
+    row1 = w[:, kk, NG, ii]
+    assert np.all(row1[1:4] == 0.0)
+    assert np.allclose(row1[4], w0[0, kk, NG, ii] * e_of(tw), rtol=1e-15)
+    for l in range(1, NG + 1):
+        src = w0[:, kk, NG + l, ii]
+        rho, u = src[0], src[1:4] / src[0]
+        q = 0.5 * np.sum(u * u, axis=0)
+        t = (src[4] / rho - q - cv1 * p.t0) / cv0 + p.t0
+        pres = rho * t * p.rgas0
+        tg = 2.0 * tw - t
+        rg = pres / tg / p.rgas0
+        g = w[:, kk, NG - l, ii]
+        assert np.allclose(g[0], rg, rtol=1e-13)
+        assert np.allclose(g[1:4], -rg * u, rtol=1e-13, atol=1e-14)
+        assert np.allclose(g[4], rg * (e_of(tg) + q), rtol=1e-13)
+
+
+def test_oblique_shock_face():
+    dx = 8.0 / 39
+    c, o = _oracle(bctags=(1, 2, 6, 7, 0, 0), bctags_nr=(0, 0, 0, 0, 0, 0), shock_angle_deg=35.0,
+                   xshock_imp=5.0, tanhfacs=dx)
+    p, gr = c.params, c.grid
+    w0 = o.w.copy()
+    o.update_ghost(0)
+    w = o.w
+    ny, nx = p.ny, p.nx
+    winf, wps = np.array(p.winf[:]), np.array(p.winf_past_shock[:])
+    x = gr.x[NG:NG + nx]
+    kk = slice(NG, NG + p.nz)
+    near_any = False
+    for l in range(0, NG + 1):
+        xsh = p.xshock_imp - gr.y[NG + ny - 1 + l] / np.tan(p.shock_angle)
+        xx = x - xsh
+        near = np.abs(xx) <= 8.0 * p.tanhfacs
+        near_any |= near.any()
+        blend = winf[:, None] + (wps - winf)[:, None] * (0.5 * (1.0 + np.tanh(xx / p.tanhfacs)))[None, :]
+        got = w[:, kk, NG + ny - 1 + l, NG:NG + nx]
+        # far from the trace: row ny (possibly already reset at l = 0)
+        far_ref = w[:, kk, NG + ny - 1, NG:NG + nx]
+        assert np.allclose(got[:, :, near], blend[:, None, near], rtol=1e-14, atol=1e-14)
+        assert np.array_equal(got[:, :, ~near], far_ref[:, :, ~near])
+    assert near_any
+    # a shock state that actually differs from the free stream
+    assert np.abs(wps - winf).max() > 0.1
+
+
+# ----------------------------------------------------- characteristic faces
+def _euler_flux(wv, d, gam, rgas=1.0):
+    rho, mom, E = wv[0], wv[1:4], wv[4]
+    u = mom / rho
+    pres = (gam - 1.0) * (E - 0.5 * rho * np.dot(u, u))
+    f = np.empty(5)
+    f[0] = mom[d]
+    f[1:4] = mom * u[d]
+    f[1 + d] += pres
+    f[4] = (E + pres) * u[d]
+    return f
+
+
+def _jacobian(wv, d, gam):
+    A = np.empty((5, 5))
+    for q in range(5):
+        h = 1e-6 * max(1.0, abs(wv[q]))
+        a, b = wv.copy(), wv.copy()
+        a[q] += h
+        b[q] -= h
+        A[:, q] = (_euler_flux(a, d, gam) - _euler_flux(b, d, gam)) / (2 * h)
+    return A
+
+
+def _nr_case(face, nr_type, mach, **kw):
+    """Oracle on a box whose w is linear along the normal of ``face`` near
+    it (so the one-sided second-order derivative is exact); returns the
+    boundary states, their normal derivatives and the fl increment."""
+    nr = [0] * 6
+    nr[face] = nr_type
+    c, o = _oracle(bctags=(1, 2, 6, 2, 0, 0), bctags_nr=tuple(nr), mach=mach, **kw)
+    p, gr = c.params, c.grid
+    d = face // 2
+    rng = np.random.default_rng(7)
+    # free-stream state with its velocity turned along the face normal
+    winf = np.array(p.winf[:])
+    base = winf.copy()
+    base[1:4] = 0.0
+    base[1 + d] = winf[1]
+    coords = [gr.x, gr.y, gr.z][d]
+    span = coords.max() - coords.min()
+    slope = 0.02 * rng.uniform(-1, 1, 5) * (np.abs(base) + 0.1 * np.abs(base).max()) / span
+    nax = 3 - d  # array axis of direction d in [m, k, j, i]
+    shape = [1, 1, 1, 1]
+    shape[nax] = -1
+    wlin = base[:, None, None, None] + slope[:, None, None, None] * coords.reshape(shape)
+    o.w[:] = np.broadcast_to(wlin, o.w.shape)
+    # small transverse variation so that every boundary point differs
+    o.w[:] *= 1.0 + 0.01 * rng.uniform(-1, 1, o.w.shape)
+    o.w[:] = o.w  # keep the view
+    o.compute_aux(1, 1)
+    o.zero_flux()
+    o.bc_nr()
+    return c, o, slope, d
+
+
+@pytest.mark.parametrize("face", [1, 3])
+def test_nr_supersonic_outflow_is_the_flux_derivative(face):
+    """Type 1 at an outflow where every wave leaves: the correction equals
+    A(w_b) dw/dn of the one-sided derivative (all waves kept)."""
+    c, o, slope, d = _nr_case(face, 1, mach=3.0)
+    p = c.params
+    gam = p.cp_coeff[0] / p.cv_coeff[0]
+    n = [p.nx, p.ny, p.nz][d]
+    metric = [c.grid.dcsidx, c.grid.detady, c.grid.dzitdz][d]
+    checked = 0
+    for (k, j, i) in [(2, 5, 7), (5, 9, 11), (10, 3, 20)]:
+        idx = [i, j, k]
+        idx[d] = n  # boundary cell (1-based), end face
+        ii, jj, kk = idx
+        wb = o.w[:, NG + kk - 1, NG + jj - 1, NG + ii - 1]
+        ub = wb[1 + d] / wb[0]
+        cs = np.sqrt(gam * (gam - 1.0) * (wb[4] / wb[0] - 0.5 * np.sum((wb[1:4] / wb[0]) ** 2)))
+        if ub - cs <= 0:
+            continue
+        # one-sided derivative in index space, then by the metric
+        line = [o.w[:, NG + kk - 1, NG + jj - 1, NG + ii - 1 - q] if d == 0 else
+                (o.w[:, NG + kk - 1, NG + jj - 1 - q, NG + ii - 1] if d == 1 else
+                 o.w[:, NG + kk - 1 - q, NG + jj - 1, NG + ii - 1]) for q in range(3)]
+        dwdn = 1.5 * line[0] - 2.0 * line[1] + 0.5 * line[2]
+        ref = _jacobian(wb.copy(), d, gam) @ dwdn * metric[n - 1]
+        got = o.fl[:, kk - 1, jj - 1, ii - 1]
+        assert np.allclose(got, ref, rtol=1e-6, atol=1e-9 * np.abs(ref).max()), (got, ref)
+        checked += 1
+    assert checked > 0
+
+
+def _char_components(c, o, d, face, ii, jj, kk):
+    p = c.params
+    gam = p.cp_coeff[0] / p.cv_coeff[0]
+    wb = o.w[:, NG + kk - 1, NG + jj - 1, NG + ii - 1]
+    A = _jacobian(wb.copy(), d, gam)
+    lam, V = np.linalg.eig(A)
+    lam, V = lam.real, V.real
+    L = np.linalg.inv(V)
+    start = face % 2 == 0
+    sgn = 1 if start else -1
+    st = [(0, 0, sgn), (0, sgn, 0), (sgn, 0, 0)][d]
+    pts = [o.w[:, NG + kk - 1 + q * st[0], NG + jj - 1 + q * st[1], NG + ii - 1 + q * st[2]] for q in range(3)]
+    dwdn = sgn * (-1.5 * pts[0] + 2.0 * pts[1] - 0.5 * pts[2])
+    metric = [c.grid.dcsidx, c.grid.detady, c.grid.dzitdz][d]
+    n = [p.nx, p.ny, p.nz][d]
+    df = o.fl[:, kk - 1, jj - 1, ii - 1] / metric[(1 if start else n) - 1]
+    return lam, L, V, dwdn, df, sgn
+
+
+@pytest.mark.parametrize("face", [0, 1, 3])
+def test_nr_subsonic_keeps_only_outgoing_waves(face):
+    """Type 1 at a subsonic boundary: each outgoing characteristic component
+    of the correction is lambda times that of dw/dn, each entering one is
+    zero."""
+    c, o, slope, d = _nr_case(face, 1, mach=0.5)
+    p = c.params
+    n = [p.nx, p.ny, p.nz][d]
+    for (k, j, i) in [(3, 6, 9), (8, 10, 15)]:
+        idx = [i, j, k]
+        idx[d] = 1 if face % 2 == 0 else n
+        lam, L, V, dwdn, df, sgn = _char_components(c, o, d, face, *idx)
+        a_df, a_dw = L @ df, L @ dwdn
+        scale = np.abs(lam).max() * np.abs(a_dw).max()
+        for q in range(5):
+            if sgn * lam[q] < 0:   # leaves the domain through this face
+                assert abs(a_df[q] - lam[q] * a_dw[q]) <= 1e-6 * scale
+            else:
+                assert abs(a_df[q]) <= 1e-6 * scale
+
+
+def test_nr_relaxation_uses_the_outer_difference_for_entering_waves():
+    """Type 3 (relaxation to winf) at a subsonic outflow: the entering wave
+    takes lambda times the characteristic component of sgn*(w - winf)."""
+    face = 1
+    c, o, slope, d = _nr_case(face, 3, mach=0.5)
+    p = c.params
+    winf = np.array(p.winf[:])
+    lam, L, V, dwdn, df, sgn = _char_components(c, o, d, face, p.nx, 7, 4)
+    wb = o.w[:, NG + 4 - 1, NG + 7 - 1, NG + p.nx - 1]
+    outer = sgn * (wb - winf)
+    a_df, a_dw, a_out = L @ df, L @ dwdn, L @ outer
+    scale = np.abs(lam).max() * max(np.abs(a_dw).max(), np.abs(a_out).max())
+    for q in range(5):
+        want = lam[q] * (a_out[q] if sgn * lam[q] > 0 else a_dw[q])
+        assert abs(a_df[q] - want) <= 1e-6 * scale
+
+
+def test_nr_reflective_wall_keeps_only_acoustic_waves():
+    """Type 6 on ymin: the correction lies in the span of the two acoustic
+    right eigenvectors."""
+    face = 2
+    c, o, slope, d = _nr_case(face, 6, mach=0.5)
+    lam, L, V, dwdn, df, sgn = _char_components(c, o, d, face, 9, 1, 5)
+    un = lam[np.argsort(lam)][2]
+    a_df = L @ df
+    for q in range(5):
+        if abs(lam[q] - un) < 1e-8 * np.abs(lam).max():
+            assert abs(a_df[q]) <= 1e-8 * np.abs(a_df).max()
+    # the reflected acoustic wave has the amplitude of the incident one, so
+    # the correction is parallel to r(v-c) + r(v+c) = 2 (1, u, v, w, H)
+    wb = o.w[:, NG + 5 - 1, NG + 1 - 1, NG + 9 - 1]
+    p = c.params
+    gam = p.cp_coeff[0] / p.cv_coeff[0]
+    u = wb[1:4] / wb[0]
+    pres = (gam - 1.0) * (wb[4] - 0.5 * wb[0] * np.dot(u, u))
+    direction = np.array([1.0, u[0], u[1], u[2], (wb[4] + pres) / wb[0]])
+    assert np.abs(np.cross(df[[0, 2, 4]], direction[[0, 2, 4]])).max() <= 1e-8 * np.abs(df).max() * np.abs(direction).max()
+    assert np.allclose(df / df[0], direction, rtol=1e-8, atol=1e-10)
