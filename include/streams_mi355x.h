@@ -65,7 +65,8 @@ typedef struct st_params {
     int32_t indx_cp_l, indx_cp_r;    /* exponent range of the cp polynomial               */
     int32_t bctags[6];               /* xmin,xmax,ymin,ymax,zmin,zmax: 0 periodic, 1 free
                                         stream, 2 extrapolation, 5 symmetry (ymin), 6 wall
-                                        (ymin; ymax too when staggered), 7 oblique shock (ymax) */
+                                        (ymin; ymax too when staggered), 7 oblique shock (ymax),
+                                        10 recycling-rescaling inflow (xmin) */
     int32_t bctags_nr[6];            /* characteristic BC: 0 none, 1 non-reflecting, 2 relaxation
                                         to the neighbour, 3 relaxation to winf, 6 reflective wall */
     int32_t is_y_staggered;          /* 1 for the channel (cell-centred y)                */
@@ -150,6 +151,18 @@ int st_visflx(st_solver *s, int32_t mode);
 int st_euler_x(st_solver *s);
 int st_euler_y(st_solver *s);
 int st_euler_z(st_solver *s);
+/* Recycling-rescaling inflow of face xmin (bctags[0] == 10): the data of
+ * recyc_prepare (singleideal.F90:1512-1588).  i_recyc is the station (cell
+ * index >= ng), betarecyc / glund1 the rescaling factors, wmean_inflow the
+ * inflow mean profile wmean(1-ng:0, 1:ny, 1:4) in Fortran order, weta the
+ * blending weight (1:ny), map_j_inn / map_j_out the locateval maps (1:ny;
+ * 1..ny-1 where used), the four coordinate arrays y+ and eta of the inflow
+ * and of the station over 1-ng:ny+ng.  Without the random inflow plane
+ * (rand_type = 0).  Required before the first step of such a solve. */
+int st_set_recycling(st_solver *s, int32_t i_recyc, double betarecyc, double glund1, const double *wmean_inflow,
+                     const double *weta, const int32_t *map_j_inn, const int32_t *map_j_out,
+                     const double *yplus_inflow, const double *eta_inflow, const double *yplus_recyc,
+                     const double *eta_recyc);
 /* bc_nr (singleideal_cpu.F90:1001-1062): characteristic boundary treatment
  * of the faces with bctags_nr != 0, added to fl. */
 int st_bc_nr(st_solver *s);

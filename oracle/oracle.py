@@ -67,6 +67,8 @@ def lib():
         L.or_visflx.argtypes = [vp, C.c_int]
         L.or_advance.argtypes = [vp, C.c_int]
         L.or_count_weno.argtypes = [vp, dp]
+        L.or_set_recycling.restype = C.c_int
+        L.or_set_recycling.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, dp, ip, ip, dp, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -132,6 +134,18 @@ class Oracle:
         rho, gam = np.zeros(3), np.zeros(3)
         self.L.or_rk_coeffs(self.h, _dp(rho), _dp(gam))
         return rho, gam
+
+    def set_recycling(self, r):
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                (r.wmean, r.weta, r.yplus_inflow, r.eta_inflow, r.yplus_recyc, r.eta_recyc)]
+        maps = [np.ascontiguousarray(a, dtype=np.int32) for a in (r.map_inn, r.map_out)]
+        self._rc = (keep, maps)
+        ip = C.POINTER(C.c_int)
+        rc = self.L.or_set_recycling(self.h, r.i_recyc, r.betarecyc, r.glund1, _dp(keep[0]), _dp(keep[1]),
+                                     maps[0].ctypes.data_as(ip), maps[1].ctypes.data_as(ip),
+                                     *[_dp(a) for a in keep[2:]])
+        if rc:
+            raise ValueError("bad recycling data")
 
     def count_weno(self):
         out = np.zeros(3)

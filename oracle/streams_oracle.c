@@ -203,6 +203,7 @@ void or_destroy(or_state *o)
 {
     if (!o) return;
     free(o->w); free(o->w_aux); free(o->fl); free(o->fln); free(o->fhat); free(o->wallprop);
+    free(o->rc_wmean); free(o->rc_weta); free(o->rc_inn); free(o->rc_out); free(o->rc_y); free(o->rc_wrecyc);
     free(o->eoc); free(o->x); free(o->y); free(o->z); free(o->yn);
     free(o->dcsidx); free(o->dcsidxs); free(o->dcsidx2);
     free(o->detady); free(o->detadys); free(o->detady2);
@@ -581,11 +582,151 @@ static void bcshock(or_state *o, int ilat)
             }
 }
 
-/* update_ghost, singleideal_cpu.F90:1064-1138: the physical boundary of
- * each face in the order xmin..zmax, then the periodic swap. */
+/* Recycling setup of recyc_prepare (singleideal.F90:1512-1588), given by the
+ * caller: station i_recyc, rescaling factors, the inflow mean profile
+ * wmean(1-ng:0, 1:ny, 1:4), blending weights, the y maps between inflow
+ * and recycling station and the y+ / eta coordinates of both. */
+int or_set_recycling(or_state *o, int i_recyc, double betarecyc, double glund1, const double *wmean,
+                     const double *weta, const int *map_inn, const int *map_out, const double *yplus_inflow,
+                     const double *eta_inflow, const double *yplus_recyc, const double *eta_recyc)
+{
+    int ng = o->ng, ny = o->ny, nz = o->nz, ny2 = ny + 2 * ng;
+    if (i_recyc < ng || i_recyc > o->nx) return 1;
+    o->i_recyc = i_recyc;
+    o->betarecyc = betarecyc;
+    o->glund1 = glund1;
+    if (!o->rc_wmean) {
+        o->rc_wmean = (double *)calloc((size_t)ng * ny * 4, sizeof(double));
+        o->rc_weta = (double *)calloc(ny, sizeof(double));
+        o->rc_inn = (int *)calloc(ny, sizeof(int));
+        o->rc_out = (int *)calloc(ny, sizeof(int));
+        o->rc_y = (double *)calloc(4 * (size_t)ny2, sizeof(double));
+        o->rc_wrecyc = (double *)calloc((size_t)ng * ny * nz * 5, sizeof(double));
+    }
+    memcpy(o->rc_wmean, wmean, sizeof(double) * ng * ny * 4);
+    memcpy(o->rc_weta, weta, sizeof(double) * ny);
+    memcpy(o->rc_inn, map_inn, sizeof(int) * ny);
+    memcpy(o->rc_out, map_out, sizeof(int) * ny);
+    memcpy(o->rc_y, yplus_inflow, sizeof(double) * ny2);
+    memcpy(o->rc_y + ny2, eta_inflow, sizeof(double) * ny2);
+    memcpy(o->rc_y + 2 * ny2, yplus_recyc, sizeof(double) * ny2);
+    memcpy(o->rc_y + 3 * ny2, eta_recyc, sizeof(double) * ny2);
+    o->recyc_set = 1;
+    return 0;
+}
+
+#define WRC(i, j, k, m) o->rc_wrecyc[(size_t)((i)-1) + (size_t)o->ng * ((size_t)((j)-1) + (size_t)o->ny * ((size_t)((k)-1) + (size_t)o->nz * ((m)-1)))]
+
+/* recyc_exchange (singleideal_cpu.F90:935-999) for one block in x and z:
+ * wrecyc(i,j,k) = w(i_recyc+1-i, j, k shifted by nzmax/2)
+ * (recyc_exchange_subroutine_1/2/3, kernels_cpu.F90:5003-5060) */
+static void recyc_exchange(or_state *o)
+{
+    int nz = o->nz, h = (nz / 2) % nz;
+    for (int m = 1; m <= 5; m++)
+        for (int k = 1; k <= nz; k++) {
+            int ks = k > h ? k - h : k + nz - h;
+            for (int j = 1; j <= o->ny; j++)
+                for (int i = 1; i <= o->ng; i++) WRC(i, j, k, m) = W(o->i_recyc + 1 - i, j, ks, m);
+        }
+}
+
+/* bcrecyc (singleideal_cpu.F90:1140-1176) with bcrecyc_subroutine_1/2/3
+ * (kernels_cpu.F90:5810-5947) for rand_type = 0 (no random inflow plane):
+ * spanwise Favre averages of the recycled planes, their fluctuations
+ * interpolated to the inflow y (inner y+ and outer eta scaling blended by
+ * weta) and added, rescaled, to the inflow mean profile. */
+static void bcrecyc(or_state *o)
+{
+    const st_params *p = &o->p;
+    int ng = o->ng, ny = o->ny, nz = o->nz, ny2 = ny + 2 * ng;
+    double *av = (double *)calloc((size_t)ng * ny * 5, sizeof(double));
+#define AV(i, j, m) av[(size_t)((i)-1) + (size_t)ng * ((size_t)((j)-1) + (size_t)ny * ((m)-1))]
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= ng; i++)
+            for (int m = 1; m <= 5; m++) {
+                double a = 0.0;
+                for (int k = 1; k <= nz; k++) a = a + WRC(i, j, k, m);
+                AV(i, j, m) = a;
+            }
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= ng; i++) {
+            double ufav = AV(i, j, 2) / AV(i, j, 1);
+            double vfav = AV(i, j, 3) / AV(i, j, 1);
+            double wfav = AV(i, j, 4) / AV(i, j, 1);
+            double rhom = AV(i, j, 1) / nz;
+            for (int k = 1; k <= nz; k++) {
+                WRC(i, j, k, 2) = WRC(i, j, k, 2) / WRC(i, j, k, 1) - ufav;
+                WRC(i, j, k, 3) = WRC(i, j, k, 3) / WRC(i, j, k, 1) - vfav;
+                WRC(i, j, k, 4) = WRC(i, j, k, 4) / WRC(i, j, k, 1) - wfav;
+                WRC(i, j, k, 1) = WRC(i, j, k, 1) - rhom;
+            }
+        }
+#undef AV
+    free(av);
+    const double *ypi = o->rc_y, *eti = o->rc_y + ny2, *ypr = o->rc_y + 2 * ny2, *etr = o->rc_y + 3 * ny2;
+#define YI(a, j) a[(j) + ng - 1]
+    const double p0 = p->rho0 * p->rgas0 * p->t0;
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++) {
+            double eta = YI(eti, j);
+            double weta = o->rc_weta[j - 1];
+            double weta1 = 1.0 - weta;
+            double bdamp = 0.5 * (1.0 - tanh(4.0 * (YI(eti, j) - 2.0)));
+            int j_inn = o->rc_inn[j - 1], j_out = o->rc_out[j - 1];
+            double disty_inn = 0.0, disty_out = 0.0;
+            if (!(j == 1 || j_inn >= ny || j_out >= ny)) {
+                disty_inn = (YI(ypi, j) - YI(ypr, j_inn)) / (YI(ypr, j_inn + 1) - YI(ypr, j_inn));
+                disty_out = (eta - YI(etr, j_out)) / (YI(etr, j_out + 1) - YI(etr, j_out));
+            }
+            for (int i = 1; i <= ng; i++) {
+                double rhomean = o->rc_wmean[(size_t)(i - 1) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 0)];
+                double uumean = o->rc_wmean[(size_t)(i - 1) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 1)] / rhomean;
+                double vvmean = o->rc_wmean[(size_t)(i - 1) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 2)] / rhomean;
+                double wwmean = o->rc_wmean[(size_t)(i - 1) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 3)] / rhomean;
+                double rhofluc, ufluc, vfluc, wfluc;
+                if (j == 1 || j_inn >= ny || j_out >= ny) {
+                    rhofluc = 0.0; ufluc = 0.0; vfluc = 0.0; wfluc = 0.0;
+                } else {
+                    double f_in[5], f_out[5];
+                    for (int m = 1; m <= 4; m++) {
+                        f_in[m] = WRC(i, j_inn, k, m) * (1.0 - disty_inn) + WRC(i, j_inn + 1, k, m) * disty_inn;
+                        f_out[m] = WRC(i, j_out, k, m) * (1.0 - disty_out) + WRC(i, j_out + 1, k, m) * disty_out;
+                    }
+                    rhofluc = f_in[1] * weta1 + f_out[1] * weta;
+                    ufluc = f_in[2] * weta1 + f_out[2] * weta;
+                    vfluc = f_in[3] * weta1 + f_out[3] * weta;
+                    wfluc = f_in[4] * weta1 + f_out[4] * weta;
+                    rhofluc = rhofluc * bdamp;
+                    ufluc = ufluc * bdamp * o->betarecyc;
+                    vfluc = vfluc * bdamp * o->betarecyc;
+                    wfluc = wfluc * bdamp * o->betarecyc;
+                }
+                rhofluc = dmax(-0.75 * rhomean, rhofluc);
+                rhofluc = dmin(4.0 * rhomean, rhofluc);
+                double rho = rhomean + rhofluc;
+                double tt = p0 / rho / p->rgas0;
+                double uu = uumean + ufluc, vv = vvmean + vfluc, ww = wwmean + wfluc;
+                double rhou = rho * uu, rhov = rho * vv, rhow = rho * ww;
+                double ee = or_get_e_from_temperature(o, tt);
+                W(1 - i, j, k, 1) = rho;
+                W(1 - i, j, k, 2) = rhou;
+                W(1 - i, j, k, 3) = rhov;
+                W(1 - i, j, k, 4) = rhow;
+                W(1 - i, j, k, 5) = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
+            }
+        }
+#undef YI
+}
+
+/* update_ghost, singleideal_cpu.F90:1064-1138: the recycled planes are
+ * exchanged first, then the physical boundary of each face in the order
+ * xmin..zmax, then the periodic swap. */
 void or_update_ghost(or_state *o, int do_swap)
 {
+    if (o->p.bctags[0] == 10 && o->recyc_set) recyc_exchange(o);
     for (int ilat = 1; ilat <= 6; ilat++) {
+        if (o->p.bctags[ilat - 1] == 10 && ilat == 1 && o->recyc_set) bcrecyc(o);
         switch (o->p.bctags[ilat - 1]) {
         case 1: bcfree(o, ilat); break;
         case 2: bcextr(o, ilat); break;

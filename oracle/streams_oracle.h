@@ -41,6 +41,14 @@ typedef struct or_state {
     int eul_imin, eul_imax, eul_jmin, eul_jmax, eul_kmin, eul_kmax;
     int force_zero_flux[6];
     int nr_eff[6];           /* bctags_nr after bc_preproc               */
+    /* recycling-rescaling inflow (bctags(1) == 10) */
+    int recyc_set, i_recyc;
+    double betarecyc, glund1;
+    double *rc_wmean;        /* wmean(1-i, j, m): [(i-1) + ng*((j-1) + ny*(m-1))] */
+    double *rc_weta;         /* weta_inflow(1:ny)                         */
+    int *rc_inn, *rc_out;    /* map_j_inn / map_j_out (1:ny)              */
+    double *rc_y;            /* yplus_inflow, eta_inflow, yplus_recyc, eta_recyc (1-ng:ny+ng) */
+    double *rc_wrecyc;       /* wrecyc(1:ng, 1:ny, 1:nz, 1:5)             */
     double rhork[3], gamrk[3];
     double dt, time, dpdx, rhobulk, ubulk, tbulk;
     double residual_rhou, rhomin, rhomax, tmin, tmax, pmin, pmax;
@@ -84,6 +92,9 @@ void or_bcswap_var(or_state *o, int var);
 void or_bcextr_var(or_state *o, int var);
 void or_update_ghost(or_state *o, int do_swap);
 void or_bc_nr(or_state *o);
+int or_set_recycling(or_state *o, int i_recyc, double betarecyc, double glund1, const double *wmean,
+                     const double *weta, const int *map_inn, const int *map_out, const double *yplus_inflow,
+                     const double *eta_inflow, const double *yplus_recyc, const double *eta_recyc);
 void or_zero_flux(or_state *o);
 void or_init_flux(or_state *o, double rhodt);
 void or_euler_fluxes(or_state *o, int dir, int fstart, int fend);

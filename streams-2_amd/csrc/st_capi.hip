@@ -73,7 +73,8 @@ int validate(const st_params *p) {
     for (int l = 0; l < 6; l++) {
         const int t = p->bctags[l];
         const bool ok = t == 0 || t == 1 || t == 2 || (t == 5 && l == 2) ||
-                        (t == 6 && (l == 2 || (l == 3 && p->is_y_staggered))) || (t == 7 && l == 3);
+                        (t == 6 && (l == 2 || (l == 3 && p->is_y_staggered))) || (t == 7 && l == 3) ||
+                        (t == 10 && l == 0);
         if (!ok)
             return fail(ST_ERR_ARG, "boundary tag " + std::to_string(t) + " on face " + std::to_string(l + 1) +
                                         " is not supported (0 periodic, 1 free stream, 2 extrapolation, 5 symmetry"
@@ -120,6 +121,7 @@ void build_tables(st_solver *s, int *eoc_host[3]) {
     // fused into k_update; any other face set runs the per-face kernels
     s->general_bc = false;
     s->has_nr = false;
+    s->recyc = p.bctags[0] == 10;
     for (int l = 0; l < 6; l++) {
         k.bctag[l] = p.bctags[l];
         k.nrtag[l] = nr[l];
@@ -193,7 +195,7 @@ hipError_t dalloc(T **p, size_t n) {
 void free_all(st_solver *s) {
     void *ptrs[] = {s->w, s->w_aux, s->F[0], s->F[1], s->wallprop, s->eoc[0], s->eoc[1], s->eoc[2], s->x, s->y,
                     s->z, s->yn, s->dcsidx, s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz,
-                    s->dzitdzs, s->dzitdz2, s->ds, s->partials, s->dirty, s->counters};
+                    s->dzitdzs, s->dzitdz2, s->ds, s->partials, s->dirty, s->counters, s->rc_real, s->rc_maps};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->hs) (void)hipHostFree(s->hs);
@@ -497,8 +499,51 @@ int st_set_time(st_solver *s, double time, double dt, int64_t icyc) {
     return ST_OK;
 }
 
+int st_set_recycling(st_solver *s, int32_t i_recyc, double betarecyc, double glund1, const double *wmean_inflow,
+                     const double *weta, const int32_t *map_j_inn, const int32_t *map_j_out,
+                     const double *yplus_inflow, const double *eta_inflow, const double *yplus_recyc,
+                     const double *eta_recyc) {
+    NEED(s);
+    if (!s->recyc) return fail(ST_ERR_STATE, "bctags[0] is not 10 (recycling)");
+    const int ng = s->g.ng, ny = s->g.ny, ny2 = ny + 2 * ng;
+    if (i_recyc < ng || i_recyc > s->g.nx) return fail(ST_ERR_ARG, "i_recyc must be in [ng, nx]");
+    if (!wmean_inflow || !weta || !map_j_inn || !map_j_out || !yplus_inflow || !eta_inflow || !yplus_recyc ||
+        !eta_recyc)
+        return fail(ST_ERR_ARG, "null recycling array");
+    for (int j = 2; j <= ny; j++) {
+        const int a = map_j_inn[j - 1], b = map_j_out[j - 1];
+        if (a < 0 || b < 0 || a > ny || b > ny || ((a < ny && b < ny) && (a < 1 || b < 1)))
+            return fail(ST_ERR_ARG, "recycling y map out of range at j = " + std::to_string(j));
+    }
+    const size_t nreal = (size_t)ng * ny * 4 + ny + 4 * (size_t)ny2 + (size_t)ng * ny * 5;
+    if (!s->rc_real) {
+        HIPCHK(hipMalloc(&s->rc_real, nreal * sizeof(double)));
+        HIPCHK(hipMalloc(&s->rc_maps, 2 * (size_t)ny * sizeof(int)));
+    }
+    std::vector<double> h(nreal, 0.0);
+    size_t o = 0;
+    std::memcpy(&h[o], wmean_inflow, sizeof(double) * ng * ny * 4); o += (size_t)ng * ny * 4;
+    std::memcpy(&h[o], weta, sizeof(double) * ny); o += ny;
+    std::memcpy(&h[o], yplus_inflow, sizeof(double) * ny2); o += ny2;
+    std::memcpy(&h[o], eta_inflow, sizeof(double) * ny2); o += ny2;
+    std::memcpy(&h[o], yplus_recyc, sizeof(double) * ny2); o += ny2;
+    std::memcpy(&h[o], eta_recyc, sizeof(double) * ny2);
+    std::vector<int> hm(2 * (size_t)ny);
+    std::memcpy(hm.data(), map_j_inn, sizeof(int) * ny);
+    std::memcpy(hm.data() + ny, map_j_out, sizeof(int) * ny);
+    HIPCHK(hipMemcpyAsync(s->rc_real, h.data(), nreal * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->rc_maps, hm.data(), 2 * (size_t)ny * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->i_recyc = i_recyc;
+    s->betarecyc = betarecyc;
+    s->glund1 = glund1;
+    s->recyc_set = true;
+    return ST_OK;
+}
+
 int st_update_ghost(st_solver *s, int32_t do_swap) {
     NEED(s);
+    if (s->recyc && !s->recyc_set) return fail(ST_ERR_STATE, "st_set_recycling not called");
     if (s->general_bc) HIPCHK(st::launch_physical_ghosts(s));
     else HIPCHK(st::launch_wall_ghosts(s));
     if (do_swap) HIPCHK(st::launch_fill_periodic(s, s->w, 5));
@@ -614,11 +659,13 @@ int st_compute_dt(st_solver *s, double *dt_out) {
 
 int st_rk_sync(st_solver *s) {
     NEED(s);
+    if (s->recyc && !s->recyc_set) return fail(ST_ERR_STATE, "st_set_recycling not called");
     return do_rk_sync(s);
 }
 
 int st_advance(st_solver *s, int32_t nsteps) {
     NEED(s);
+    if (s->recyc && !s->recyc_set) return fail(ST_ERR_STATE, "st_set_recycling not called");
     const int iter_dt = s->p.iter_dt_recompute > 0 ? s->p.iter_dt_recompute : 1;
     for (int n = 0; n < nsteps; n++) {
         s->icyc += 1;

@@ -377,6 +377,100 @@ __global__ void k_bc_nr(Geo g, KPar P, const double *__restrict__ w, const R *__
     for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] = R(double(fl[(size_t)m * g.nint + oi]) + df[m] * d);
 }
 
+// ------------------------------------------------ recycling-rescaling inflow
+// recyc_exchange for one block in x and z (singleideal_cpu.F90:935-999,
+// recyc_exchange_subroutine_1/2/3, kernels_cpu.F90:5003-5060): the recycled
+// plane i of the inflow is w(i_recyc+1-i) with the span shifted by nz/2.
+__device__ __forceinline__ int recyc_k(int k, int nz) {
+    const int h = (nz / 2) % nz;
+    return k > h ? k - h : k + nz - h;
+}
+
+// bcrecyc_subroutine_1 (kernels_cpu.F90:5810-5828): spanwise sums of the
+// recycled planes, one thread per (i, j), k summed in order.
+__global__ void k_recyc_avg(Geo g, const double *__restrict__ w, int i_recyc, double *__restrict__ av) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.ng * g.ny) return;
+    const int i = t % g.ng + 1, j = t / g.ng + 1;
+    const size_t nv = g.nvol;
+    for (int m = 0; m < 5; m++) {
+        double a = 0.0;
+        for (int k = 1; k <= g.nz; k++) a = a + w[m * nv + gidx(g, i_recyc + 1 - i, j, recyc_k(k, g.nz))];
+        av[(i - 1) + g.ng * ((j - 1) + g.ny * m)] = a;
+    }
+}
+
+// bcrecyc_subroutine_2/3 (kernels_cpu.F90:5832-5947, rand_type 0): one
+// thread per (j, k) of the inflow; the Favre fluctuations of the recycled
+// planes are formed where they are interpolated.
+__global__ void k_recyc_inflow(Geo g, KPar P, double *__restrict__ w, const double *__restrict__ rc,
+                               const int *__restrict__ maps, int i_recyc, double betarecyc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.ny * g.nz) return;
+    const int j = t % g.ny + 1, k = t / g.ny + 1;
+    const int ng = g.ng, ny = g.ny, nz = g.nz, ny2 = ny + 2 * ng;
+    const size_t nv = g.nvol;
+    const double *wmean = rc, *weta_a = rc + (size_t)ng * ny * 4;
+    const double *ypi = weta_a + ny, *eti = ypi + ny2, *ypr = eti + ny2, *etr = ypr + ny2;
+    const double *av = etr + ny2;
+    const double eta = eti[j + ng - 1];
+    const double weta = weta_a[j - 1];
+    const double weta1 = 1.0 - weta;
+    const double bdamp = 0.5 * (1.0 - tanh(4.0 * (eti[j + ng - 1] - 2.0)));
+    const int j_inn = maps[j - 1], j_out = maps[ny + j - 1];
+    const bool zero = j == 1 || j_inn >= ny || j_out >= ny;
+    double disty_inn = 0.0, disty_out = 0.0;
+    if (!zero) {
+        disty_inn = (ypi[j + ng - 1] - ypr[j_inn + ng - 1]) / (ypr[j_inn + 1 + ng - 1] - ypr[j_inn + ng - 1]);
+        disty_out = (eta - etr[j_out + ng - 1]) / (etr[j_out + 1 + ng - 1] - etr[j_out + ng - 1]);
+    }
+    const int ks = recyc_k(k, nz);
+    const double p0 = P.rho0 * P.rgas0 * P.t0;
+    for (int i = 1; i <= ng; i++) {
+        const double rhomean = wmean[(i - 1) + ng * ((j - 1) + ny * 0)];
+        const double uumean = wmean[(i - 1) + ng * ((j - 1) + ny * 1)] / rhomean;
+        const double vvmean = wmean[(i - 1) + ng * ((j - 1) + ny * 2)] / rhomean;
+        const double wwmean = wmean[(i - 1) + ng * ((j - 1) + ny * 3)] / rhomean;
+        double rhofluc = 0.0, ufluc = 0.0, vfluc = 0.0, wfluc = 0.0;
+        if (!zero) {
+            // fluctuation of variable m (0 rho, 1..3 velocity) of plane i at row jx
+            auto fluc = [&](int jx, int m) {
+                const int c = gidx(g, i_recyc + 1 - i, jx, ks);
+                const double *a = av + (i - 1) + ng * (jx - 1);
+                const double rho = w[c];
+                if (m == 0) return rho - a[0] / nz;
+                return w[m * nv + c] / rho - a[(size_t)ng * ny * m] / a[0];
+            };
+            double f_in[4], f_out[4];
+            for (int m = 0; m < 4; m++) {
+                f_in[m] = fluc(j_inn, m) * (1.0 - disty_inn) + fluc(j_inn + 1, m) * disty_inn;
+                f_out[m] = fluc(j_out, m) * (1.0 - disty_out) + fluc(j_out + 1, m) * disty_out;
+            }
+            rhofluc = f_in[0] * weta1 + f_out[0] * weta;
+            ufluc = f_in[1] * weta1 + f_out[1] * weta;
+            vfluc = f_in[2] * weta1 + f_out[2] * weta;
+            wfluc = f_in[3] * weta1 + f_out[3] * weta;
+            rhofluc = rhofluc * bdamp;
+            ufluc = ufluc * bdamp * betarecyc;
+            vfluc = vfluc * bdamp * betarecyc;
+            wfluc = wfluc * bdamp * betarecyc;
+        }
+        rhofluc = fmax(-0.75 * rhomean, rhofluc);
+        rhofluc = fmin(4.0 * rhomean, rhofluc);
+        const double rho = rhomean + rhofluc;
+        const double tt = p0 / rho / P.rgas0;
+        const double uu = uumean + ufluc, vv = vvmean + vfluc, ww = wwmean + wfluc;
+        const double rhou = rho * uu, rhov = rho * vv, rhow = rho * ww;
+        const double ee = e_from_temperature(P, tt);
+        const int d = gidx(g, 1 - i, j, k);
+        w[d] = rho;
+        w[nv + d] = rhou;
+        w[2 * nv + d] = rhov;
+        w[3 * nv + d] = rhow;
+        w[4 * nv + d] = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
+    }
+}
+
 // -------------------------------------------------------------------- aux
 // eval_aux_subroutine (kernels_cpu.F90:6960-7011) on the box
 // [i0,i1]x[j0,j1]x[k0,k1]; with image=true a ghost point of a periodic
@@ -1982,11 +2076,25 @@ hipError_t launch_update(st_solver *s, const void *fl_prev, const void *fl, int 
     return hipGetLastError();
 }
 
+hipError_t launch_recycling(st_solver *s) {
+    const Geo &g = s->g;
+    double *av = s->rc_real + (size_t)g.ng * g.ny * 4 + g.ny + 4 * (size_t)(g.ny + 2 * g.ng);
+    const int n1 = g.ng * g.ny, n2 = g.ny * g.nz;
+    hipLaunchKernelGGL(k_recyc_avg, dim3((n1 + 127) / 128), dim3(128), 0, s->stream, g, s->w, s->i_recyc, av);
+    hipLaunchKernelGGL(k_recyc_inflow, dim3((n2 + 127) / 128), dim3(128), 0, s->stream, g, s->kp, s->w, s->rc_real,
+                       s->rc_maps, s->i_recyc, s->betarecyc);
+    return hipGetLastError();
+}
+
 hipError_t launch_physical_ghosts(st_solver *s) {
     const Geo &g = s->g;
+    if (s->recyc) {
+        hipError_t e = launch_recycling(s);  // recyc_exchange, then face xmin
+        if (e != hipSuccess) return e;
+    }
     for (int ilat = 1; ilat <= 2 * s->kp.ndim; ilat++) {
         const int tag = s->kp.bctag[ilat - 1];
-        if (tag == 0) continue;
+        if (tag == 0 || tag == 10) continue;
         const int n = (ilat <= 2 ? g.ny : g.nx) * (ilat <= 4 ? g.nz : g.ny);
         ST_DISPATCH(hipLaunchKernelGGL(k_bc_face<R>, dim3((n + 127) / 128), dim3(128), 0, s->stream, g, s->kp, s->w,
                                        (const R *)s->w_aux, s->x, s->y, ilat, tag, s->p.is_y_staggered));

@@ -17,6 +17,12 @@ struct st_solver {
     bool mixed;         // ST_PREC_MIXED: w_aux and F in float
     bool general_bc;    // boundary faces other than periodic / staggered wall: per-face ghost kernels
     bool has_nr;        // a characteristic boundary treatment is active
+    // recycling-rescaling inflow (bctags[0] == 10)
+    bool recyc, recyc_set;
+    int i_recyc;
+    double betarecyc, glund1;
+    double *rc_real;    // wmean planes (ng*ny*4), weta (ny), y+/eta arrays (4*(ny+2ng)), averages (ng*ny*5)
+    int *rc_maps;       // map_j_inn (ny), map_j_out (ny)
     int visc_chunk;     // planes per tile of the z-marching viscous kernel (0: automatic)
     int march_chunk;    // points per thread of the y/z flux marches (0: automatic)
     double *wallprop;   // 3 x (sx*sz), wallprop(:,:,2:4)
@@ -62,6 +68,8 @@ hipError_t launch_wall_ghosts(st_solver *s);
 hipError_t launch_physical_ghosts(st_solver *s);
 // characteristic boundary treatment added to fl (bc_nr)
 hipError_t launch_bc_nr(st_solver *s, void *fl);
+// recyc_exchange + bcrecyc of face xmin
+hipError_t launch_recycling(st_solver *s);
 hipError_t launch_euler_x(st_solver *s, void *fl, bool init);
 hipError_t launch_euler_y(st_solver *s, void *fl);
 hipError_t launch_euler_z(st_solver *s, void *fl);

@@ -48,6 +48,9 @@ _SIGNATURES = {
     "st_euler_z": (C.c_int, [C.c_void_p]),
     "st_zero_flux": (C.c_int, [C.c_void_p]),
     "st_bc_nr": (C.c_int, [C.c_void_p]),
+    "st_set_recycling": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+                         + [C.POINTER(C.c_double)] * 4),
     "st_init_run": (C.c_int, [C.c_void_p]),
     "st_compute_dt": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "st_rk_sync": (C.c_int, [C.c_void_p]),

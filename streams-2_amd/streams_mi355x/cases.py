@@ -282,3 +282,66 @@ def make_bl_box(nx=40, ny=24, nz=16, seed=0, **kw) -> ChannelCase:
     g = bl_grid(nx, ny, nz)
     p = bl_params(nx, ny, nz, **kw)
     return ChannelCase(FlowCase(nx=nx, ny=ny, nz=nz), g, p, bl_initial_field(g, p, seed=seed))
+
+
+def locateval(xx: np.ndarray, x: float) -> int:
+    """Bisection locate of parameters.F90:145-164 for increasing xx(1:n):
+    the number of entries strictly below x (0..n)."""
+    return int(np.searchsorted(np.asarray(xx), x, side="left"))
+
+
+@dataclass
+class Recycling:
+    i_recyc: int
+    betarecyc: float
+    glund1: float
+    wmean: np.ndarray        # wmean(1-i, j, m), shape (4, ny, ng) in Fortran order [m][j][i]
+    weta: np.ndarray         # (ny,)
+    map_inn: np.ndarray      # (ny,) int32
+    map_out: np.ndarray
+    yplus_inflow: np.ndarray  # (ny + 2 ng,) index j + ng - 1
+    eta_inflow: np.ndarray
+    yplus_recyc: np.ndarray
+    eta_recyc: np.ndarray
+
+
+def recycling_setup(grid: Grid, x_recyc: float, delta_in: float, deltav_in: float, delta_rc: float,
+                    deltav_rc: float, wmean_planes: np.ndarray) -> Recycling:
+    """recyc_prepare (singleideal.F90:1512-1588) for one block: the station
+    i_recyc from the x grid, the rescaling factors beta = dv(rc)/dv(in) and
+    (d(rc)/d(in))**0.13, the inner (y+) and outer (eta) coordinates of the
+    inflow and of the station, the maps between them and the blending
+    weight.  The boundary-layer and viscous thicknesses at the inflow and at
+    the station are given (in the reference they come from the mean
+    profile of init_bl).  ``wmean_planes`` is the inflow mean profile
+    wmean(1-ng:0, 1:ny, 1:4) as an array [m, j, i] (i = 1..ng)."""
+    ng, nx, ny = grid.ng, grid.nx, grid.ny
+    xg = grid.x[ng:ng + nx]
+    ig = locateval(xg, x_recyc)
+    i_recyc = max(ig, ng)
+    bexp = 0.13
+    betarecyc = deltav_rc / deltav_in
+    glund1 = (delta_rc / delta_in) ** bexp
+    y = grid.y
+    yplus_in = y / deltav_in
+    eta_in = y / delta_in
+    yplus_rc = y / deltav_rc
+    eta_rc = y / delta_rc
+    map_inn = np.array([locateval(yplus_rc[ng:ng + ny], yplus_in[ng + j]) for j in range(ny)], dtype=np.int32)
+    map_out = np.array([locateval(eta_rc[ng:ng + ny], eta_in[ng + j]) for j in range(ny)], dtype=np.int32)
+    alfa, beta = 4.0, 0.2
+    cb = 1.0 - 2.0 * beta
+    e = eta_in[ng:ng + ny]
+    weta = 0.5 * (1.0 + np.tanh((alfa * (e - beta)) / (beta + e * cb)) / math.tanh(alfa))
+    return Recycling(i_recyc, betarecyc, glund1, np.ascontiguousarray(wmean_planes, dtype=np.float64), weta,
+                     map_inn, map_out, yplus_in, eta_in, yplus_rc, eta_rc)
+
+
+def bl_mean_planes(case: ChannelCase) -> np.ndarray:
+    """Spanwise/first-column mean profile of the seeded boundary-layer field
+    (rho, rho*u, rho*v, rho*w) for the ng inflow ghost planes: [m, j, i]."""
+    ng, ny = case.grid.ng, case.grid.ny
+    w = case.w0
+    prof = w[:4, ng:-ng, ng:ng + ny, ng].mean(axis=1)  # [m, j] at i = 1
+    prof[2:4] = 0.0
+    return np.repeat(prof[:, :, None], ng, axis=2)

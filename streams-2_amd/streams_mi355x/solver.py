@@ -115,6 +115,16 @@ class ChannelSolver:
     def bc_nr(self):
         capi.check(self.L.st_bc_nr(self.h))
 
+    def set_recycling(self, r):
+        """Recycling data of streams_mi355x.cases.recycling_setup."""
+        self._rc = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                    (r.wmean, r.weta, r.yplus_inflow, r.eta_inflow, r.yplus_recyc, r.eta_recyc)]
+        self._rcm = [np.ascontiguousarray(a, dtype=np.int32) for a in (r.map_inn, r.map_out)]
+        ip = C.POINTER(C.c_int32)
+        capi.check(self.L.st_set_recycling(self.h, r.i_recyc, r.betarecyc, r.glund1, _dp(self._rc[0]),
+                                           _dp(self._rc[1]), self._rcm[0].ctypes.data_as(ip),
+                                           self._rcm[1].ctypes.data_as(ip), *[_dp(a) for a in self._rc[2:]]))
+
     def init_run(self):
         capi.check(self.L.st_init_run(self.h))
 

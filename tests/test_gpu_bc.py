@@ -84,3 +84,43 @@ def test_bl_trajectory_20_steps(name):
     so, sg = o.scalars(), g.scalars()
     assert abs(sg["dt"] - so["dt"]) <= 1e-12 * so["dt"]
     g.close()
+
+
+def _recycling_pair(seed=5):
+    from streams_mi355x.cases import bl_mean_planes, recycling_setup
+
+    c = make_bl_box(bctags=(10, 2, 6, 2, 0, 0), bctags_nr=(0, 1, 0, 1, 0, 0), seed=seed)
+    r = recycling_setup(c.grid, 5.0, 0.6, 0.01, 0.7, 0.011, bl_mean_planes(c))
+    w = c.w0.copy()
+    w *= 1.0 + 0.04 * (np.random.default_rng(seed).random(w.shape[1:]) - 0.5)
+    o, g = Oracle(c.params, c.grid), ChannelSolver(c.params, c.grid)
+    o.w[:] = w
+    g.copy_from_field(w)
+    o.set_recycling(r)
+    g.set_recycling(r)
+    return c, o, g
+
+
+def test_recycling_inflow_parity():
+    c, o, g = _recycling_pair()
+    o.compute_aux(1, 1)
+    g.compute_aux(1, 1)
+    o.update_ghost(1)
+    g.update_ghost(1)
+    wg = g.copy_to_field()
+    for m in range(5):    # the inflow ghost planes
+        assert rel_l2(wg[m][NG:-NG, NG:-NG, :NG], o.w[m][NG:-NG, NG:-NG, :NG]) <= 1e-13, m
+    g.close()
+
+
+def test_recycling_trajectory_20_steps():
+    c, o, g = _recycling_pair()
+    o.init_run()
+    g.init_run()
+    o.advance(20)
+    g.advance(20)
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    print("recycling 20-step rel L2 per variable:", errs)
+    assert max(errs) <= TRAJ_TOL, errs
+    g.close()

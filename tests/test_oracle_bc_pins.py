@@ -268,3 +268,100 @@ def test_nr_reflective_wall_keeps_only_acoustic_waves():
     direction = np.array([1.0, u[0], u[1], u[2], (wb[4] + pres) / wb[0]])
     assert np.abs(np.cross(df[[0, 2, 4]], direction[[0, 2, 4]])).max() <= 1e-8 * np.abs(df).max() * np.abs(direction).max()
     assert np.allclose(df / df[0], direction, rtol=1e-8, atol=1e-10)
+
+
+# --------------------------------------------------- recycling-rescaling inflow
+def _recycling_case(same_scaling=False, uniform=False, seed=5, beta=None):
+    from streams_mi355x.cases import bl_mean_planes, recycling_setup
+
+    c = make_bl_box(bctags=(10, 2, 6, 2, 0, 0), bctags_nr=(0, 1, 0, 1, 0, 0), seed=seed)
+    if same_scaling:
+        r = recycling_setup(c.grid, 5.0, 0.6, 0.01, 0.6, 0.01, bl_mean_planes(c))
+    else:
+        r = recycling_setup(c.grid, 5.0, 0.6, 0.01, 0.7, 0.011, bl_mean_planes(c))
+    if beta is not None:
+        r.betarecyc = beta
+    o = Oracle(c.params, c.grid)
+    w = c.w0.copy()
+    if uniform:  # spanwise-uniform field
+        w[:] = w[:, NG:NG + 1, :, :]
+    else:  # density fluctuations at fixed velocity and temperature
+        w *= 1.0 + 0.04 * (np.random.default_rng(seed).random(w.shape[1:]) - 0.5)
+    o.w[:] = w
+    o.compute_aux(1, 1)
+    o.set_recycling(r)
+    return c, r, o
+
+
+def test_recycling_uniform_station_gives_the_mean_profile():
+    c, r, o = _recycling_case(uniform=True)
+    o.update_ghost(0)
+    p = c.params
+    p0 = p.rho0 * p.rgas0 * p.t0
+    e_of = lambda t: p.cv_coeff[1] * p.t0 + p.cv_coeff[0] * (t - p.t0)
+    for i in range(1, NG + 1):
+        g = o.w[:, NG:-NG, NG:-NG, NG - i]          # ghost plane 1-i: [m, k, j]
+        rho = r.wmean[0, :, i - 1]
+        u = r.wmean[1:4, :, i - 1] / rho
+        assert np.allclose(g[0], rho[None, :], rtol=1e-14)
+        assert np.allclose(g[1:4], (rho * u)[:, None, :], rtol=1e-13, atol=1e-15)
+        tt = p0 / rho / p.rgas0
+        want_e = rho * e_of(tt) + 0.5 * rho * np.sum(u * u, axis=0)
+        assert np.allclose(g[4], want_e[None, :], rtol=1e-13)
+
+
+def test_recycling_density_fluctuation_has_zero_spanwise_mean():
+    c, r, o = _recycling_case()
+    o.update_ghost(0)
+    for i in range(1, NG + 1):
+        rho_g = o.w[0, NG:-NG, NG:-NG, NG - i]       # [k, j]
+        assert np.allclose(rho_g.mean(axis=0), r.wmean[0, :, i - 1], rtol=1e-12)
+
+
+def test_recycling_shifts_the_span_by_half():
+    c, r, o = _recycling_case(same_scaling=True, uniform=True)
+    p = c.params
+    nz = p.nz
+    kb, jb = 3, 9                                   # bump at station cell (i_recyc, jb, kb)
+    o.w[2, NG + kb - 1, NG + jb - 1, NG + r.i_recyc - 1] += 0.05 * o.w[0, NG + kb - 1, NG + jb - 1, NG + r.i_recyc - 1]
+    o.update_ghost(0)
+    rv = o.w[2, NG:-NG, NG:-NG, NG - 1] / o.w[0, NG:-NG, NG:-NG, NG - 1]     # v of ghost plane 0, [k, j]
+    dev = np.abs(rv - rv.mean(axis=0))
+    k_hit = int(np.argmax(dev[:, jb - 1])) + 1
+    assert k_hit == (kb + nz // 2 - 1) % nz + 1
+
+
+def test_recycling_identical_scaling_copies_the_damped_fluctuation():
+    c, r, o = _recycling_case(same_scaling=True, beta=1.7)
+    p = c.params
+    assert np.array_equal(r.map_inn[1:], np.arange(1, p.ny))  # y maps onto itself
+    w0 = o.w.copy()
+    o.update_ghost(0)
+    nz, ny = p.nz, p.ny
+    eta = r.eta_inflow[NG:NG + ny]
+    bdamp = 0.5 * (1.0 - np.tanh(4.0 * (eta - 2.0)))
+    for i in range(1, NG + 1):
+        st = w0[:, NG:-NG, NG:-NG, NG + r.i_recyc - i]                   # station plane [m, k, j]
+        st = np.roll(st, nz // 2, axis=1)                                # half-span shift
+        ufav = st[1].sum(axis=0) / st[0].sum(axis=0)
+        uf = st[1] / st[0] - ufav[None, :]
+        g = o.w[:, NG:-NG, NG:-NG, NG - i]
+        rho_m = r.wmean[0, :, i - 1]
+        u_g = g[1] / g[0]
+        want = r.wmean[1, :, i - 1] / rho_m + uf * bdamp[None, :] * r.betarecyc
+        jj = slice(1, ny - 1)   # rows with a full interpolation stencil
+        assert np.allclose(u_g[:, jj], want[:, jj], rtol=1e-12, atol=1e-13)
+        rf = st[0] - st[0].mean(axis=0)[None, :]                         # density: damped, not rescaled by beta
+        want_rho = rho_m + np.clip(rf * bdamp[None, :], -0.75 * rho_m, 4.0 * rho_m)
+        assert np.allclose(g[0][:, jj], want_rho[:, jj], rtol=1e-13)
+
+
+def test_recycling_inflow_has_the_reference_pressure():
+    """The inflow ghosts carry T = p0/(rho R), i.e. the pressure p0."""
+    c, r, o = _recycling_case()
+    o.update_ghost(0)
+    p = c.params
+    gam = p.cp_coeff[0] / p.cv_coeff[0]
+    g = o.w[:, NG:-NG, NG:-NG, 0:NG]
+    pres = (gam - 1.0) * (g[4] - 0.5 * (g[1] ** 2 + g[2] ** 2 + g[3] ** 2) / g[0])
+    assert np.allclose(pres, p.rho0 * p.rgas0 * p.t0, rtol=1e-12)
