@@ -643,28 +643,21 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__r
         const int v = lb + eoc[ff];
         return v < 1 ? 1 : v;
     };
-    PcR<R> q0, q1, q2;
+    // The march starts three points early (b = c0-3): those iterations only
+    // load the warm-up points and feed faces below c0-1, which are never
+    // written.  Every point thus passes through the one loop body, so a
+    // face's arithmetic is the same whatever the chunk it falls in.
+    PcR<R> q0{}, q1{}, q2{};
     unsigned hist = 0;
-    {
-        int a = p0 + (c0 - 3) * stride;
-        q0 = make_pcr<R>(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[TS * nv + a], rgas0, pref, keep);
-        hist |= (wa[7 * nv + a] > thr ? 1u : 0u) << 2;
-        a += stride;
-        q1 = make_pcr<R>(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[TS * nv + a], rgas0, pref, keep);
-        hist |= (wa[7 * nv + a] > thr ? 1u : 0u) << 1;
-        a += stride;
-        q2 = make_pcr<R>(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a], wa[TS * nv + a], rgas0, pref, keep);
-        hist |= (wa[7 * nv + a] > thr ? 1u : 0u);
-    }
     Acc<R> A0 = acc_zero<R>(), A1 = acc_zero<R>(), A2 = acc_zero<R>();  // faces b-3, b-2, b-1
-    int lm0 = lm_of(c0 - 3), lm1 = lm_of(c0 - 2), lm2 = lm_of(c0 - 1);
+    int lm0 = lm_of(c0 - 6), lm1 = lm_of(c0 - 5), lm2 = lm_of(c0 - 4);
     R fprev[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
     bool wprev = false;
     // prefetch of the next point
-    int a = p0 + c0 * stride;
+    int a = p0 + (c0 - 3) * stride;
     R nr = wa[a], nu = wa[nv + a], nvv = wa[2 * nv + a], nw = wa[3 * nv + a], nh = wa[4 * nv + a],
       nt = wa[TS * nv + a], ns = wa[7 * nv + a];
-    for (int b = c0; b <= c1 + 3; b++) {
+    for (int b = c0 - 3; b <= c1 + 3; b++) {
         // issue the read of fl at the cell completed in this iteration first,
         // so that its latency overlaps the pair arithmetic below
         R flv[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};

@@ -226,7 +226,11 @@ int admit(st_server *sv, Slot &sl) {
     return ST_OK;
 }
 
-// Enqueue the next chunk of steps of a slot and record its event.
+// Steps per event of a solve run without a graph (steps_per_graph 0 or 1).
+constexpr int kPlainChunk = 10;
+
+// Enqueue the next chunk of steps of a slot and record its event.  Every
+// chunk holds at least one step, so each call advances sl.launched.
 int launch_chunk(st_server *sv, Slot &sl) {
     st_solver *s = sl.s;
     const int left = sl.r->nsteps - sl.launched;
@@ -237,7 +241,8 @@ int launch_chunk(st_server *sv, Slot &sl) {
         s->icyc += n;
         s->last_stage = 2;
     } else {
-        n = left < sv->K ? left : sv->K;
+        const int per = sv->K > 0 ? sv->K : kPlainChunk;
+        n = left < per ? left : per;
         int rc = st_advance(s, n);
         if (rc) return rc;
     }

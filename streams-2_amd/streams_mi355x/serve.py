@@ -110,10 +110,12 @@ def cell_updates(reqs) -> float:
 
 
 def run_serving(reqs, hbm_bytes: int, max_slots: int = 8, steps_per_graph: int = 10, autotune: bool = True,
-                want_fields: bool = False):
+                want_fields: bool = False, timeout: float = 600.0):
     """Submit each request at its arrival time, wait for all; returns a dict
     with throughput (Gcell-updates/s per RK stage over the makespan from the
-    first arrival), p50/p99 time-to-solution and the per-request results."""
+    first arrival), p50/p99 time-to-solution and the per-request results.
+    A request not finished ``timeout`` seconds after the wait for it began
+    raises."""
     out = []
     with Server(hbm_bytes, max_slots, steps_per_graph, autotune) as sv:
         t0 = time.perf_counter()
@@ -124,7 +126,7 @@ def run_serving(reqs, hbm_bytes: int, max_slots: int = 8, steps_per_graph: int =
                 time.sleep(delay)
             tickets.append(sv.submit(r.case.params, r.case.grid, r.case.w0, r.nsteps))
         for r, tk in zip(reqs, tickets):
-            w, sc, tm = sv.wait(tk, -1.0, want_fields)
+            w, sc, tm = sv.wait(tk, timeout, want_fields)
             out.append({"w": w, "scalars": sc, "submit": tm[0], "admit": tm[1], "done": tm[2]})
         stats = sv.stats()
     first = min(o["submit"] for o in out)

@@ -181,6 +181,38 @@ def test_trajectory_small(threshold):
     assert sg["icyc"] == so["icyc"] == 20
 
 
+def test_trajectory_bulk_temperature_control():
+    """theta_wall >= -1: force_var runs after every stage's update."""
+    c, o, g = _run_pair(20, theta_wall=1.0)
+    assert c.params.theta_wall >= -1.0
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    assert max(errs) <= TRAJ_TOL, errs
+    so, sg = o.scalars(), g.scalars()
+    for key in ("rhobulk", "ubulk", "tbulk"):
+        assert abs(sg[key] - so[key]) <= 1e-12 * abs(so[key]), key
+    g.close()
+
+
+@pytest.mark.parametrize("visc_chunk,march_chunk", [(8, 8), (1, 3), (16, 32), (3, 5)])
+def test_results_independent_of_launch_shape(visc_chunk, march_chunk):
+    """The z-chunk of the viscous march and the chunk of the y/z flux
+    marches change the launch only: results are bitwise those of the
+    automatic shape."""
+    from streams_mi355x import capi
+
+    c = make_channel(nx=32, ny=24, nz=16, mach=0.8, retau=150.0, seed=0, sensor_threshold=0.5)
+    out = []
+    for vc, mc in ((0, 0), (visc_chunk, march_chunk)):
+        with ChannelSolver(c.params, c.grid) as g:
+            capi.check(g.L.st_set_tuning(g.h, vc, mc))
+            g.copy_from_field(c.w0)
+            g.init_run()
+            g.advance(12)
+            out.append(g.copy_to_field())
+    assert np.array_equal(interior(out[0]), interior(out[1]))
+
+
 def test_residual_and_weno_count():
     c, o, g = _run_pair(5, threshold=0.0)
     o.compute_residual()

@@ -37,6 +37,17 @@ def test_same_alone_and_in_the_mix(precision):
         assert a["scalars"]["residual_rhou"] == b["scalars"]["residual_rhou"]
 
 
+def test_server_without_graphs_matches_the_plain_path():
+    """steps_per_graph 0: plain st_advance chunks, no capture."""
+    reqs = _requests()[:3]
+    seq = run_sequential(reqs, want_fields=True)
+    srv = run_serving(reqs, hbm_bytes=8 << 30, max_slots=2, steps_per_graph=0, autotune=False, want_fields=True,
+                      timeout=120.0)
+    for n, (a, b) in enumerate(zip(seq["results"], srv["results"])):
+        assert np.array_equal(interior(a["w"]), interior(b["w"])), n
+        assert b["scalars"]["icyc"] == reqs[n].nsteps
+
+
 def test_admission_respects_the_hbm_capacity():
     reqs = _requests()
     foot = [lib().st_solver_footprint(r.case.params) for r in reqs]

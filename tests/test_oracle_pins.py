@@ -532,6 +532,37 @@ def test_force_rhs_brute_force():
     assert np.array_equal(o.fln[2], fln0[2])
 
 
+def test_force_var_sets_the_bulk_temperature_to_target():
+    """After force_var the rho*u-weighted bulk temperature equals the target,
+    the shift of T is uniform, and rho, rho*u_i and the kinetic energy are
+    untouched."""
+    c = make_channel(nx=16, ny=12, nz=8, theta_wall=1.0)
+    p = c.params
+    assert p.theta_wall >= -1.0 and p.t_bulk_target > 0.0
+    o = Oracle(p, c.grid)
+    rng = np.random.default_rng(4)
+    w = c.w0.copy()
+    w[4] *= 1.0 + 0.05 * rng.random(w.shape[1:])       # T no longer at its bulk target
+    o.w[:] = w
+    o.compute_aux(1, 1)
+    o.zero_flux()
+    o.force_rhs()                                       # rho_b, u_b of this w
+    s = o.scalars()
+    T0 = o.w_aux[5, 3:-3, 3:-3, 3:-3].copy()
+    o.force_var()
+    wi = o.w[:, 3:-3, 3:-3, 3:-3]
+    w0 = w[:, 3:-3, 3:-3, 3:-3]
+    assert np.array_equal(wi[:4], w0[:4])
+    cv, cvt = p.cv_coeff[0], p.cv_coeff[1] * p.t0
+    ke = 0.5 * (wi[1] ** 2 + wi[2] ** 2 + wi[3] ** 2) / wi[0]
+    T = ((wi[4] - ke) / wi[0] - cvt) / cv + p.t0          # calorically perfect e(T) inverted
+    shift = T - T0
+    assert np.ptp(shift) < 1e-10 * abs(shift).max()
+    dy = np.diff(c.grid.yn)[None, :, None]
+    tb = (wi[1] * T * dy).sum() / p.volchan / s["rhobulk"] / s["ubulk"]
+    assert abs(tb - p.t_bulk_target) < 1e-12 * p.t_bulk_target
+
+
 def test_ghost_fills_periodic_images_and_isothermal_wall():
     c = make_channel(nx=12, ny=10, nz=8)
     o = Oracle(c.params, c.grid)
