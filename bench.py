@@ -43,6 +43,11 @@ ALGO_BYTES = {
 }
 
 
+def progress(msg):
+    """One line per finished phase on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of the same workload (scaled by the copy
@@ -112,17 +117,23 @@ def serving(args):
     from streams_mi355x.serve import cell_updates, run_sequential, run_serving, serving_workload
 
     reqs = serving_workload(args.serve_requests, args.serve_steps, rate=args.serve_rate, seed=0)
+    progress(f"serving workload of {len(reqs)} requests built")
     seq_n = min(len(reqs), args.serve_seq_sample)
     seq = run_sequential(reqs[:seq_n])
+    progress(f"sequential sample done: {seq['gcups']:.3f} Gcell-updates/s")
     srv = run_serving(reqs, hbm_bytes=int(args.serve_hbm_gib * (1 << 30)), max_slots=args.serve_slots,
                       steps_per_graph=10, autotune=True)
+    progress(f"served: {srv['gcups']:.3f} Gcell-updates/s, p50 {srv['p50_s']:.2f} s, p99 {srv['p99_s']:.2f} s")
     return {"requests": len(reqs), "steps_per_request": args.serve_steps, "arrival_rate_per_s": args.serve_rate,
             "value": srv["gcups"], "unit": "Gcell-updates/s per RK stage", "p50_s": srv["p50_s"],
             "p99_s": srv["p99_s"], "mean_queue_s": srv["mean_queue_s"], "makespan_s": srv["makespan_s"],
+            "failed_requests": srv["failed"],
+            "failed_note": "requests whose residual turned NaN (the oracle diverges on them too, within the same 10 steps); "
+                           "value and latencies cover the completed requests",
             "sequential_value": seq["gcups"], "sequential_sample": f"first {seq_n} requests, plain path, one by one",
             "speedup_vs_sequential": srv["gcups"] / seq["gcups"], "slots": args.serve_slots,
             "hbm_capacity_gib": args.serve_hbm_gib, "tuned_shape_classes": srv["stats"]["tuned_classes"],
-            "total_cell_updates": cell_updates(reqs)}
+            "total_cell_updates": cell_updates([r for r, o in zip(reqs, srv["results"]) if o["error"] is None])}
 
 
 def parse():
@@ -211,6 +222,7 @@ def main():
     el = timed_run(g, c, args.warmup, args.steps)
     cells = nx * ny * nz
     value = cells * 3 * args.steps / el / 1e9
+    progress(f"fp64 timed: {value:.3f} Gcell-updates/s per RK stage")
     res = g.compute_residual()
     finite = res["residual_rhou"] == res["residual_rhou"]
     weno_frac = [float(x) for x in g.count_weno()]
@@ -255,11 +267,13 @@ def main():
     g.close()
     if not args.no_mixed:
         out["mixed"] = mixed_form(args, c, value)
+        progress(f"mixed: {out['mixed']['value']:.3f} Gcell-updates/s per RK stage")
     if not args.no_serve:
         out["serving"] = serving(args)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args)
+            progress(f"cpu baseline: {out['cpu_baseline']['value']:.5f} Gcell-updates/s per RK stage")
         except Exception as e:  # the oracle is optional on a bare box
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out))

@@ -45,6 +45,7 @@ struct ServeReq {
     int nsteps = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+    int rc = ST_OK;  // status of a failed request (ST_ERR_NAN for a NaN residual)
     std::vector<double> w_out;
     st_scalars sc{};
     std::string err;
@@ -170,6 +171,7 @@ void finish(st_server *sv, Slot &sl, int rc, const std::string &err) {
     std::lock_guard<std::mutex> lk(sv->mu);
     r->t_done = sv->now();
     r->state = rc ? 3 : 2;
+    r->rc = rc;
     if (rc) {
         r->err = err;
         sv->failed++;
@@ -423,7 +425,7 @@ int st_server_wait(st_server *sv, int64_t ticket, double timeout_s, double *w_ou
     else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done))
         return sfail(ST_ERR_STATE, "request not finished");
     int rc = ST_OK;
-    if (r->state == 3) rc = sfail(ST_ERR_HIP, "request failed: " + r->err);
+    if (r->state == 3) rc = sfail(r->rc, "request failed: " + r->err);
     if (w_out && !r->w_out.empty()) std::memcpy(w_out, r->w_out.data(), r->w_out.size() * sizeof(double));
     if (sc) *sc = r->sc;
     if (times) {

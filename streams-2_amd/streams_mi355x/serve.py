@@ -114,8 +114,10 @@ def run_serving(reqs, hbm_bytes: int, max_slots: int = 8, steps_per_graph: int =
     """Submit each request at its arrival time, wait for all; returns a dict
     with throughput (Gcell-updates/s per RK stage over the makespan from the
     first arrival), p50/p99 time-to-solution and the per-request results.
-    A request not finished ``timeout`` seconds after the wait for it began
-    raises."""
+    A request that fails (a NaN residual, a device error) is returned with
+    its ``error`` and counted in ``failed``; the others are unaffected, and
+    throughput and latencies cover the completed requests.  A request not
+    finished ``timeout`` seconds after the wait for it began raises."""
     out = []
     with Server(hbm_bytes, max_slots, steps_per_graph, autotune) as sv:
         t0 = time.perf_counter()
@@ -126,21 +128,32 @@ def run_serving(reqs, hbm_bytes: int, max_slots: int = 8, steps_per_graph: int =
                 time.sleep(delay)
             tickets.append(sv.submit(r.case.params, r.case.grid, r.case.w0, r.nsteps))
         for r, tk in zip(reqs, tickets):
-            w, sc, tm = sv.wait(tk, timeout, want_fields)
-            out.append({"w": w, "scalars": sc, "submit": tm[0], "admit": tm[1], "done": tm[2]})
+            try:
+                w, sc, tm = sv.wait(tk, timeout, want_fields)
+                out.append({"w": w, "scalars": sc, "submit": tm[0], "admit": tm[1], "done": tm[2], "error": None})
+            except capi.StreamsError as e:
+                if e.code == capi.ST_ERR_STATE:   # time-out: the server is stuck, not the request
+                    raise
+                out.append({"w": None, "scalars": None, "error": str(e)})
         stats = sv.stats()
-    first = min(o["submit"] for o in out)
-    last = max(o["done"] for o in out)
-    lat = np.array([o["done"] - o["submit"] for o in out])
-    return {"gcups": cell_updates(reqs) / (last - first) / 1e9, "makespan_s": last - first,
+    ok = [o for o, r in zip(out, reqs) if o["error"] is None]
+    done_reqs = [r for o, r in zip(out, reqs) if o["error"] is None]
+    if not ok:
+        return {"gcups": 0.0, "failed": len(out), "results": out, "stats": stats}
+    first = min(o["submit"] for o in ok)
+    last = max(o["done"] for o in ok)
+    lat = np.array([o["done"] - o["submit"] for o in ok])
+    return {"gcups": cell_updates(done_reqs) / (last - first) / 1e9, "makespan_s": last - first,
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "mean_queue_s": float(np.mean([o["admit"] - o["submit"] for o in out])),
-            "results": out, "stats": stats}
+            "mean_queue_s": float(np.mean([o["admit"] - o["submit"] for o in ok])),
+            "failed": len(out) - len(ok), "results": out, "stats": stats}
 
 
 def run_sequential(reqs, want_fields: bool = False):
     """The same requests one after another through the plain path
-    (ChannelSolver: one stream, no graphs, no overlap)."""
+    (ChannelSolver: one stream, no graphs, no overlap).  A request whose
+    residual is NaN is returned with its ``error``; throughput counts the
+    completed requests."""
     from .solver import ChannelSolver
 
     out = []
@@ -150,8 +163,14 @@ def run_sequential(reqs, want_fields: bool = False):
             g.copy_from_field(r.case.w0)
             g.init_run()
             g.advance(r.nsteps)
-            sc = g.compute_residual()
+            try:
+                sc, err = g.compute_residual(), None
+            except capi.StreamsError as e:
+                if e.code != capi.ST_ERR_NAN:
+                    raise
+                sc, err = None, str(e)
             w = g.copy_to_field() if want_fields else None
-        out.append({"w": w, "scalars": sc})
+        out.append({"w": w, "scalars": sc, "error": err})
     el = time.perf_counter() - t0
-    return {"gcups": cell_updates(reqs) / el / 1e9, "seconds": el, "results": out}
+    done = [r for r, o in zip(reqs, out) if o["error"] is None]
+    return {"gcups": cell_updates(done) / el / 1e9, "seconds": el, "failed": len(reqs) - len(done), "results": out}
