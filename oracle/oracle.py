@@ -18,6 +18,8 @@ if _PKG not in sys.path:
 
 from streams_mi355x.params import StParams, StScalars  # noqa: E402
 
+NV_STAT = 70  # statistics of compute_stats (singleideal.F90:2003-2179)
+
 _LIB = None
 
 
@@ -67,6 +69,7 @@ def lib():
         L.or_visflx.argtypes = [vp, C.c_int]
         L.or_advance.argtypes = [vp, C.c_int]
         L.or_count_weno.argtypes = [vp, dp]
+        L.or_compute_stats.argtypes = [vp, C.c_int, dp]
         L.or_set_recycling.restype = C.c_int
         L.or_set_recycling.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, dp, ip, ip, dp, dp, dp, dp]
         _LIB = L
@@ -146,6 +149,14 @@ class Oracle:
                                      *[_dp(a) for a in keep[2:]])
         if rc:
             raise ValueError("bad recycling data")
+
+    def compute_stats(self, itav: int, w_stat: np.ndarray):
+        """compute_stats: fold this state's spanwise means into the running
+        time average ``w_stat`` (70 x nx x ny, Fortran order, updated in
+        place) as the itav-th sample."""
+        assert w_stat.flags.f_contiguous and w_stat.dtype == np.float64
+        assert w_stat.shape == (NV_STAT, self.p.nx, self.p.ny)
+        self.L.or_compute_stats(self.h, int(itav), w_stat.ctypes.data_as(C.POINTER(C.c_double)))
 
     def count_weno(self):
         out = np.zeros(3)

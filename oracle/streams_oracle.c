@@ -1903,6 +1903,88 @@ void or_advance(or_state *o, int nsteps)
     }
 }
 
+/* compute_stats (singleideal.F90:2003-2179, grid_dim 1): the 70 spanwise
+ * sums of each (i, j), accumulated over k = 1..nz in order, divided by nz
+ * and folded into the running time average
+ * w_stat = (w_stat*itav + w_stat_z)/(itav + 1).  Velocity gradients come
+ * from w (ghosts as they are), T and mu from w_aux(6), w_aux(7).
+ * w_stat is (1:70, 1:nx, 1:ny) in Fortran order. */
+void or_compute_stats(or_state *o, int itav, double *w_stat)
+{
+    const st_params *p = &o->p;
+    const int nx = o->nx, ny = o->ny, nz = o->nz, nv = OR_NV_STAT;
+    const int lmax = p->visc_order / 2;
+    const double *c1 = o->cd1[lmax - 1];
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            double z[OR_NV_STAT];
+            for (int q = 0; q < nv; q++) z[q] = 0.0;
+            for (int k = 1; k <= nz; k++) {
+                double ux = 0, vx = 0, wx = 0, uy = 0, vy = 0, wy = 0, uz = 0, vz = 0, wz = 0;
+                for (int l = 1; l <= lmax; l++) {
+                    const double ccl = c1[l - 1];
+                    ux = ux + ccl * (W(i + l, j, k, 2) / W(i + l, j, k, 1) - W(i - l, j, k, 2) / W(i - l, j, k, 1));
+                    vx = vx + ccl * (W(i + l, j, k, 3) / W(i + l, j, k, 1) - W(i - l, j, k, 3) / W(i - l, j, k, 1));
+                    wx = wx + ccl * (W(i + l, j, k, 4) / W(i + l, j, k, 1) - W(i - l, j, k, 4) / W(i - l, j, k, 1));
+                    uy = uy + ccl * (W(i, j + l, k, 2) / W(i, j + l, k, 1) - W(i, j - l, k, 2) / W(i, j - l, k, 1));
+                    vy = vy + ccl * (W(i, j + l, k, 3) / W(i, j + l, k, 1) - W(i, j - l, k, 3) / W(i, j - l, k, 1));
+                    wy = wy + ccl * (W(i, j + l, k, 4) / W(i, j + l, k, 1) - W(i, j - l, k, 4) / W(i, j - l, k, 1));
+                    uz = uz + ccl * (W(i, j, k + l, 2) / W(i, j, k + l, 1) - W(i, j, k - l, 2) / W(i, j, k - l, 1));
+                    vz = vz + ccl * (W(i, j, k + l, 3) / W(i, j, k + l, 1) - W(i, j, k - l, 3) / W(i, j, k - l, 1));
+                    wz = wz + ccl * (W(i, j, k + l, 4) / W(i, j, k + l, 1) - W(i, j, k - l, 4) / W(i, j, k - l, 1));
+                }
+                const double dx = o->dcsidx[i - 1], dy = o->detady[j - 1], dz = o->dzitdz[k - 1];
+                ux = ux * dx; vx = vx * dx; wx = wx * dx;
+                uy = uy * dy; vy = vy * dy; wy = wy * dy;
+                uz = uz * dz; vz = vz * dz; wz = wz * dz;
+                const double omx = wy - vz, omy = uz - wx, omz = vx - uy;
+                const double rho = W(i, j, k, 1), rhou = W(i, j, k, 2), rhov = W(i, j, k, 3), rhow = W(i, j, k, 4);
+                const double ri = 1.0 / rho;
+                const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
+                const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+                const double tt = WA(i, j, k, 6);
+                const double pp = rho * p->rgas0 * tt;
+                const double mu = WA(i, j, k, 7);
+                const double nu = mu / rho;
+                const double gamloc = or_get_gamloc(o, tt);
+                const double cploc = gamloc / (gamloc - 1.0) * p->rgas0;
+                const double c = sqrt(gamloc * p->rgas0 * tt);
+                const double machlocal = sqrt(2.0 * qq) / c;
+                const double t_tot = tt + qq * (gamloc - 1.0) / gamloc;
+                const double rho2 = rho * rho, uu2 = uu * uu, vv2 = vv * vv, ww2 = ww * ww, uv = uu * vv;
+                const double pp2 = pp * pp, tt2 = tt * tt;
+                const double divl = ux + vy + wz;
+                const double div3l = divl / 3.0;
+                double s11 = 2.0 * (ux - div3l), s12 = uy + vx, s13 = uz + wx;
+                double s22 = 2.0 * (vy - div3l), s23 = vz + wy, s33 = 2.0 * (wz - div3l);
+                s11 = s11 * mu; s12 = s12 * mu; s13 = s13 * mu;
+                s22 = s22 * mu; s23 = s23 * mu; s33 = s33 * mu;
+                const double s21 = s12, s31 = s13, s32 = s23;
+                (void)uv;
+                const double v[OR_NV_STAT] = {
+                    rho, uu, vv, ww, pp, tt, rho2, uu2, vv2, ww2,
+                    pp2, tt2, rhou, rhov, rhow, rhou * uu, rhov * vv, rhow * ww, rhou * vv, mu,
+                    nu, omx * omx, omy * omy, omz * omz, rho * tt, rho * tt2, t_tot, rho * t_tot, t_tot * t_tot,
+                    rhou * tt,
+                    rhov * tt, rhow * tt, machlocal, machlocal * machlocal, rhou * uu2, rhov * uu2, rhou * vv2,
+                    rhov * vv2, rhou * ww2, rhov * ww2,
+                    pp * uu, pp * vv, s11, s12, s13, s22, s23, s33, s11 * uu, s12 * uu,
+                    s21 * vv, s22 * vv, s31 * ww, s32 * ww, s11 * vv + s21 * uu, s12 * vv + s22 * uu,
+                    s11 * ux + s12 * uy + s13 * uz, s21 * vx + s22 * vy + s23 * vz, s31 * wx + s32 * wy + s33 * wz,
+                    s11 * vx + s12 * (ux + vy) + s22 * uy + s13 * vz + s23 * uz,
+                    pp * ux, pp * vy, pp * wz, pp * (uy + vx), divl * divl, rho * tt2 * tt, rho * tt2 * tt2,
+                    rhou * uu2 * uu, cploc, gamloc};
+                for (int q = 0; q < nv; q++) z[q] = z[q] + v[q];
+            }
+            double *ws = w_stat + (size_t)nv * ((size_t)(i - 1) + (size_t)nx * (size_t)(j - 1));
+            for (int q = 0; q < nv; q++) {
+                const double zq = z[q] / (double)nz;
+                ws[q] = (ws[q] * (double)itav + zq) / (double)(itav + 1);
+            }
+        }
+}
+
 /* count_weno_subroutine, kernels_cpu.F90:52-113, and the fractions of
  * count_weno (singleideal_cpu.F90:1366-1395) */
 void or_count_weno(or_state *o, double frac[3])

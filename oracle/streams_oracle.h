@@ -113,6 +113,8 @@ void or_init_run(or_state *o);
 void or_advance(or_state *o, int nsteps);
 void or_compute_residual(or_state *o);
 void or_count_weno(or_state *o, double frac[3]);
+#define OR_NV_STAT 70
+void or_compute_stats(or_state *o, int itav, double *w_stat);
 
 #ifdef __cplusplus
 }

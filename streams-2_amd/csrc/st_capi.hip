@@ -93,7 +93,8 @@ int validate(const st_params *p) {
         if ((p->bctags[2 * d] == 0) != (p->bctags[2 * d + 1] == 0))
             return fail(ST_ERR_ARG, "a periodic direction needs tag 0 on both faces");
     double vol = ((double)p->nx + 6) * ((double)p->ny + 6) * ((double)p->nz + 6);
-    if (vol * 10.0 > 2147483647.0) return fail(ST_ERR_ARG, "grid too large for 32-bit indexing of w_aux");
+    const double n_aux = p->precision == ST_PREC_MIXED ? 11.0 : 10.0;  // w_aux variables
+    if (vol * n_aux > 2147483647.0) return fail(ST_ERR_ARG, "grid too large for 32-bit indexing of w_aux");
     return ST_OK;
 }
 

@@ -786,17 +786,34 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const R *__restr
         const int v = lb + eoc[ff];
         return v < 1 ? 1 : v;
     };
+    // point of this lane in work item wk (clamped into the ghosted row)
+    auto point_of = [&](int wk_) {
+        const int seg = wk_ % nseg;
+        const int row = wk_ / nseg;
+        const int pt_ = 1 + XW * seg - 3 + lane;
+        return gidx(g, pt_ > g.nx + 3 ? g.nx + 3 : pt_, row % g.ny + 1, row / g.ny + 1);
+    };
+    // the loads of the next work item are issued before the work on the
+    // current one, so their latency overlaps the pair arithmetic
+    R nr = R(0), nu = R(0), nvv = R(0), nw = R(0), nh = R(0), nt = R(0), ns = R(0);
+    if (wave < nwork) {
+        const int a = point_of(wave);
+        nr = wa[a]; nu = wa[nv + a]; nvv = wa[2 * nv + a]; nw = wa[3 * nv + a];
+        nh = wa[4 * nv + a]; nt = wa[tp_slot<R>() * nv + a]; ns = wa[7 * nv + a];
+    }
     for (int wk = wave; wk < nwork; wk += nwaves) {
         const int seg = wk % nseg;
         const int row = wk / nseg;
         const int j = row % g.ny + 1, k = row / g.ny + 1;
         const int i0 = 1 + XW * seg;
         const int pt = i0 - 3 + lane;                     // point held by this lane
-        const int pl = pt > g.nx + 3 ? g.nx + 3 : pt;     // clamp the load into the ghosted row
-        const int a = gidx(g, pl, j, k);
-        const PcR<R> q = make_pcr<R>(wa[a], wa[nv + a], wa[2 * nv + a], wa[3 * nv + a], wa[4 * nv + a],
-                                     wa[tp_slot<R>() * nv + a], R(P.rgas0), pref, keep);
-        const bool flag = wa[7 * nv + a] > R(P.sensor_threshold);
+        const PcR<R> q = make_pcr<R>(nr, nu, nvv, nw, nh, nt, R(P.rgas0), pref, keep);
+        const bool flag = ns > R(P.sensor_threshold);
+        if (wk + nwaves < nwork) {
+            const int a = point_of(wk + nwaves);
+            nr = wa[a]; nu = wa[nv + a]; nvv = wa[2 * nv + a]; nw = wa[3 * nv + a];
+            nh = wa[4 * nv + a]; nt = wa[tp_slot<R>() * nv + a]; ns = wa[7 * nv + a];
+        }
         const PcR<R> q1 = shfl_up_pc(q, 1), q2 = shfl_up_pc(q, 2), q3 = shfl_up_pc(q, 3);
         const int lmA = lm_of(pt - 1), lmB = lm_of(pt - 2), lmC = lm_of(pt - 3);
         Acc<R> Q1 = acc_zero<R>(), Q2 = acc_zero<R>(), Q3 = acc_zero<R>();
@@ -1503,6 +1520,7 @@ __global__ __launch_bounds__(256) void k_visc_div2(Geo g, KPar P, const R *__res
 // of partials, the last workgroup to finish (atomic ticket) adds the RB
 // results in index order, so the sum does not depend on scheduling.
 constexpr int RB = 128;
+static_assert(RB <= RED_THREADS, "the last block reads one second-level value per thread");
 
 template <int NV>
 __device__ __forceinline__ bool two_level_sum(const double *__restrict__ partials, int nblk, double *__restrict__ stage2,
@@ -1526,14 +1544,20 @@ __device__ __forceinline__ bool two_level_sum(const double *__restrict__ partial
     }
     __syncthreads();
     if (!last) return false;
-    if (threadIdx.x == 0) {
-        __threadfence();
+    // the last block: one second-level value per thread (gridDim.x <=
+    // blockDim.x), read past the non-coherent caches, then the same
+    // fixed-order tree as the first level
+    __threadfence();
+    double v[NV];
 #pragma unroll
-        for (int q = 0; q < NV; q++) {
-            double a = 0.0;
-            for (int b = 0; b < (int)gridDim.x; b++) a += ((volatile double *)stage2)[b * NV + q];
-            out[q] = a;
-        }
+    for (int q = 0; q < NV; q++)
+        v[q] = threadIdx.x < gridDim.x
+                   ? __hip_atomic_load(&stage2[threadIdx.x * NV + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                   : 0.0;
+    block_sum<NV>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) out[q] = v[q];
         *ticket = 0;
     }
     return threadIdx.x == 0;
@@ -1698,20 +1722,44 @@ __global__ __launch_bounds__(256) void k_dt(Geo g, KPar P, const double *__restr
     }
 }
 
-// dt = cfl / max (singleideal_cpu.F90:1507-1514) over the block maxima
-__global__ __launch_bounds__(RED_THREADS) void k_dt_finish(KPar P, const double *__restrict__ partials, int nblk,
-                                                           DevScalars *ds) {
-    __shared__ double lds[RED_THREADS / 64];
-    double m = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) m = fmax(m, partials[b]);
+// Maximum of this block's threads, in thread 0 (order-independent).
+__device__ __forceinline__ double block_max(double m, double *lds) {
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
     __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = fmax(m, lds[q]);
+    return m;
+}
+
+// dt = cfl / max (singleideal_cpu.F90:1507-1514) over the block maxima:
+// gridDim.x blocks take a slice each, the last one to finish combines the
+// slice maxima (max is exact in any order).
+__global__ __launch_bounds__(RED_THREADS) void k_dt_finish(KPar P, const double *__restrict__ partials, int nblk,
+                                                           double *__restrict__ stage2, int *__restrict__ ticket,
+                                                           DevScalars *ds) {
+    __shared__ double lds[RED_THREADS / 64];
+    __shared__ bool last;
+    const int per = (nblk + gridDim.x - 1) / gridDim.x;
+    const int b0 = blockIdx.x * per, b1 = min(b0 + per, nblk);
+    double m = 0.0;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) m = fmax(m, partials[b]);
+    m = block_max(m, lds);
     if (threadIdx.x == 0) {
-        double dtinv_max = lds[0];
-        for (int q = 1; q < RED_THREADS / 64; q++) dtinv_max = fmax(dtinv_max, lds[q]);
-        const double dt_min = 1.0 / dtinv_max;
+        stage2[blockIdx.x] = m;
+        __threadfence();
+        last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    m = threadIdx.x < gridDim.x ? __hip_atomic_load(&stage2[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                : 0.0;
+    m = block_max(m, lds);
+    if (threadIdx.x == 0) {
+        const double dt_min = 1.0 / m;
         ds->dt = P.cfl * dt_min;
+        *ticket = 0;
     }
 }
 
@@ -2133,7 +2181,8 @@ hipError_t launch_dt(st_solver *s) {
     const Geo &g = s->g;
     ST_DISPATCH(hipLaunchKernelGGL(k_dt<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, (R *)s->w_aux,
                                    s->dcsidx, s->detady, s->dzitdz, s->dcsidxs, s->detadys, s->dzitdzs, s->partials));
-    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(g), s->ds);
+    hipLaunchKernelGGL(k_dt_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(g),
+                       s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
     return hipGetLastError();
 }
 

@@ -108,6 +108,17 @@ module streams_mi355x
       import :: c_int, c_ptr
       type(c_ptr), value :: s
     end function
+    integer(c_int) function st_set_recycling(s, i_recyc, betarecyc, glund1, wmean_inflow, weta, map_j_inn, &
+                                             map_j_out, yplus_inflow, eta_inflow, yplus_recyc, eta_recyc) &
+                                             bind(c, name="st_set_recycling")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: s
+      integer(c_int32_t), value :: i_recyc
+      real(c_double), value :: betarecyc, glund1
+      real(c_double), intent(in) :: wmean_inflow(*), weta(*), yplus_inflow(*), eta_inflow(*)
+      real(c_double), intent(in) :: yplus_recyc(*), eta_recyc(*)
+      integer(c_int32_t), intent(in) :: map_j_inn(*), map_j_out(*)
+    end function
     integer(c_int) function st_init_run(s) bind(c, name="st_init_run")
       import :: c_int, c_ptr
       type(c_ptr), value :: s
@@ -157,6 +168,7 @@ module streams_mi355x
     procedure :: euler_y => mi_euler_y
     procedure :: euler_z => mi_euler_z
     procedure :: bc_nr => mi_bc_nr
+    procedure :: set_recycling => mi_set_recycling
     procedure :: init_run => mi_init_run
     procedure :: compute_dt => mi_compute_dt
     procedure :: rk_sync => mi_rk_sync
@@ -263,6 +275,22 @@ contains
     class(equation_singleideal_mi355x_object), intent(inout) :: self
     call check(st_bc_nr(self%handle), 'bc_nr')
   end subroutine mi_bc_nr
+
+  ! recyc_prepare's arrays (singleideal.F90:1512-1588) for the xmin
+  ! recycling inflow: wmean_inflow(1-ng:0,1:ny,1:4), weta/map_j_*(1:ny),
+  ! the y+ and eta coordinates of inflow and station (1-ng:ny+ng)
+  subroutine mi_set_recycling(self, i_recyc, betarecyc, glund1, wmean_inflow, weta, map_j_inn, map_j_out, &
+                              yplus_inflow, eta_inflow, yplus_recyc, eta_recyc)
+    class(equation_singleideal_mi355x_object), intent(inout) :: self
+    integer, intent(in) :: i_recyc
+    real(c_double), intent(in) :: betarecyc, glund1
+    real(c_double), intent(in) :: wmean_inflow(:,:,:), weta(:), yplus_inflow(:), eta_inflow(:)
+    real(c_double), intent(in) :: yplus_recyc(:), eta_recyc(:)
+    integer(c_int32_t), intent(in) :: map_j_inn(:), map_j_out(:)
+    call check(st_set_recycling(self%handle, int(i_recyc, c_int32_t), betarecyc, glund1, wmean_inflow, weta, &
+                                map_j_inn, map_j_out, yplus_inflow, eta_inflow, yplus_recyc, eta_recyc), &
+               'set_recycling')
+  end subroutine mi_set_recycling
 
   subroutine mi_init_run(self)
     class(equation_singleideal_mi355x_object), intent(inout) :: self

@@ -48,6 +48,28 @@ def test_server_without_graphs_matches_the_plain_path():
         assert b["scalars"]["icyc"] == reqs[n].nsteps
 
 
+def test_a_diverging_request_fails_alone():
+    """A NaN in one request's field fails that request with ST_ERR_NAN; the
+    requests beside it finish bit-identical to their plain solves."""
+    from streams_mi355x.capi import ST_ERR_NAN
+
+    reqs = _requests()[:4]
+    bad = 1
+    c = reqs[bad].case
+    w0 = c.w0.copy()
+    w0[0, 3 + 2, 3 + 5, 3 + 7] = np.nan
+    reqs[bad] = Request(type(c)(c.case, c.grid, c.params, w0), reqs[bad].nsteps, 0.0)
+    seq = run_sequential(reqs, want_fields=True)
+    srv = run_serving(reqs, hbm_bytes=8 << 30, max_slots=4, steps_per_graph=10, autotune=True, want_fields=True,
+                      timeout=120.0)
+    assert srv["failed"] == 1 and seq["failed"] == 1
+    assert f"[st error {ST_ERR_NAN}]" in srv["results"][bad]["error"]
+    for n, (a, b) in enumerate(zip(seq["results"], srv["results"])):
+        if n != bad:
+            assert b["error"] is None
+            assert np.array_equal(interior(a["w"]), interior(b["w"])), n
+
+
 def test_admission_respects_the_hbm_capacity():
     reqs = _requests()
     foot = [lib().st_solver_footprint(r.case.params) for r in reqs]
