@@ -186,6 +186,20 @@ int st_advance(st_solver *s, int32_t nsteps);
 int st_compute_residual(st_solver *s, st_scalars *out);
 /* count_weno (singleideal_cpu.F90:1366-1395): fraction of WENO faces. */
 int st_count_weno(st_solver *s, double frac_xyz[3]);
+/* compute_stats (singleideal.F90:2003-2179, called from run() at
+ * singleideal_cpu.F90:2020-2041): the 70 spanwise means of the current
+ * state folded into the running time average held on the device,
+ * w_stat = (w_stat*itav + mean_z)/(itav + 1), as the itav-th sample
+ * (itav = 0 starts a new average).  Each spanwise sum runs over k = 1..nz in
+ * order.  Velocity gradients use the periodic ghosts of w refreshed from the
+ * current interior; the reference's run() holds those of the last stage's
+ * start at that point.  No host copy of the field is made. */
+int st_compute_stats(st_solver *s, int32_t itav);
+/* The running statistics w_stat(1:70, 1:nx, 1:ny) in Fortran order
+ * (write_stats, singleideal.F90:1779-1822, writes them to stat.bin);
+ * st_set_stats loads them back (restart_type 2, read_stats). */
+int st_get_stats(st_solver *s, double *w_stat);
+int st_set_stats(st_solver *s, const double *w_stat);
 
 /* Measurement helper, no counterpart in the reference (its only timer is the
  * 'averaged timing' of run(), singleideal_cpu.F90:1661-1666): runs nsteps

@@ -196,7 +196,8 @@ hipError_t dalloc(T **p, size_t n) {
 void free_all(st_solver *s) {
     void *ptrs[] = {s->w, s->w_aux, s->F[0], s->F[1], s->wallprop, s->eoc[0], s->eoc[1], s->eoc[2], s->x, s->y,
                     s->z, s->yn, s->dcsidx, s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz,
-                    s->dzitdzs, s->dzitdz2, s->ds, s->partials, s->dirty, s->counters, s->rc_real, s->rc_maps};
+                    s->dzitdzs, s->dzitdz2, s->ds, s->partials, s->dirty, s->counters, s->rc_real, s->rc_maps,
+                    s->w_stat};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->hs) (void)hipHostFree(s->hs);
@@ -237,8 +238,12 @@ int stage(st_solver *s, int istep) {
     void *fprev = s->F[s->cur];   // RHS of the previous stage (reference: fl before init_flux)
     void *fcur = s->F[s->cur ^ 1];
     const Geo &g = s->g;
-    // bcswap + compute_aux(ghost=1): aux over the box, periodic ghosts from images
-    TIMED(ST_K_AUX, launch_aux(s, 1 - g.ng, g.nx + g.ng, 1 - g.ng, g.ny + g.ng, 1 - g.ng, g.nz + g.ng, true));
+    // bcswap + compute_aux(ghost=1): aux over the box, periodic ghosts from
+    // images; at the first stage of a step that recomputes dt, the same pass
+    // takes the CFL reduction
+    TIMED(ST_K_AUX, launch_aux(s, 1 - g.ng, g.nx + g.ng, 1 - g.ng, g.ny + g.ng, 1 - g.ng, g.nz + g.ng, true,
+                               istep == 0 && s->dt_in_stage));
+    if (istep == 0) s->dt_in_stage = false;
     TIMED(ST_K_EULER_X, launch_euler_x(s, fcur, true));
     const bool cons = s->p.conservative_viscous == 1;
     // visflx(4), or visflx(6) + visflx(5); + bcextr_var / bcswap_var of w_aux(10)
@@ -280,7 +285,8 @@ int64_t st_solver_footprint(const st_params *p) {
     const int64_t blocks = (int64_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * p->nz;
     const int64_t rs = p->precision == ST_PREC_MIXED ? 4 : 8;  // bytes of a w_aux / right-hand-side value
     const int64_t na = p->precision == ST_PREC_MIXED ? 11 : 10;
-    return 8 * (5 * nvol + 3 * sx * sz + 7 * blocks + 1024) + rs * (na * nvol + 10 * nint) +
+    const int64_t part = 7 * blocks > (nvol + 255) / 256 ? 7 * blocks : (nvol + 255) / 256;
+    return 8 * (5 * nvol + 3 * sx * sz + part + 1024) + rs * (na * nvol + 10 * nint) +
            (nint + 4 * (p->nx + p->ny + p->nz + 3 + 4)) +
            8 * (2 * sx + 2 * sy + 2 * sz + 3 * (p->nx + p->ny + p->nz) + p->ny + 1) + 4096;
 }
@@ -314,7 +320,10 @@ int st_solver_create(const st_params *p, st_solver **out) {
     s->own_stream = true;
     const size_t nvol = g.nvol, nint = g.nint;
     const int nblk = ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz;
-    s->npart = 7 * nblk + 128 * 8;   // per-block partials + second-level sums
+    // per-block partials (7 per cell block, or one per block of the aux
+    // pass over the ghosted box, whichever is larger) + second-level sums
+    const int aux_blk = (int)((nvol + 255) / 256);
+    s->npart = (7 * nblk > aux_blk ? 7 * nblk : aux_blk) + 128 * 8;
     hipError_t es[32];
     int ne = 0;
     es[ne++] = dalloc(&s->w, 5 * nvol);
@@ -670,7 +679,12 @@ int st_advance(st_solver *s, int32_t nsteps) {
     const int iter_dt = s->p.iter_dt_recompute > 0 ? s->p.iter_dt_recompute : 1;
     for (int n = 0; n < nsteps; n++) {
         s->icyc += 1;
-        if (s->p.cfl > 0 && ((s->icyc - s->icyc0) % iter_dt) == 0) TIMED(ST_K_DT, st::launch_dt(s));
+        if (s->p.cfl > 0 && ((s->icyc - s->icyc0) % iter_dt) == 0) {
+            // compute_aux(central) + compute_dt of the step start form the
+            // same primitives of the same w as the first stage's aux, which
+            // therefore takes the reduction
+            s->dt_in_stage = true;
+        }
         int rc = do_rk_sync(s);
         if (rc) return rc;
         TIMED(ST_K_STEP_END, st::launch_step_end(s));
@@ -748,6 +762,49 @@ int st_count_weno(st_solver *s, double frac_xyz[3]) {
     frac_xyz[0] = s->hs->cnt[0] / ((double)(k.eul_hi[0] - k.eul_lo[0] + 1) * ny * nz);
     frac_xyz[1] = s->hs->cnt[1] / (nx * (double)(k.eul_hi[1] - k.eul_lo[1] + 1) * nz);
     frac_xyz[2] = s->hs->cnt[2] / (nx * ny * (double)(k.eul_hi[2] - k.eul_lo[2] + 1));
+    return ST_OK;
+}
+
+static constexpr int kNvStat = 70;
+
+static int ensure_stats(st_solver *s) {
+    if (s->w_stat) return ST_OK;
+    const size_t n = (size_t)kNvStat * s->g.nx * s->g.ny;
+    HIPCHK(hipMalloc(&s->w_stat, n * sizeof(double)));
+    HIPCHK(st::launch_zero(s, s->w_stat, n));
+    return ST_OK;
+}
+
+int st_compute_stats(st_solver *s, int32_t itav) {
+    NEED(s);
+    if (itav < 0) return fail(ST_ERR_ARG, "itav must be >= 0");
+    int rc = ensure_stats(s);
+    if (rc) return rc;
+    // the steps read periodic neighbours from their images and leave the
+    // periodic ghost cells of w alone: refresh them for the gradients
+    HIPCHK(st::launch_fill_periodic(s, s->w, 5));
+    HIPCHK(st::launch_stats(s, itav));
+    return ST_OK;
+}
+
+int st_get_stats(st_solver *s, double *w_stat_host) {
+    NEED(s);
+    if (!w_stat_host) return fail(ST_ERR_ARG, "null output");
+    if (!s->w_stat) return fail(ST_ERR_STATE, "no statistics yet (st_compute_stats or st_set_stats first)");
+    HIPCHK(hipMemcpyAsync(w_stat_host, s->w_stat, (size_t)kNvStat * s->g.nx * s->g.ny * sizeof(double),
+                          hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
+int st_set_stats(st_solver *s, const double *w_stat_host) {
+    NEED(s);
+    if (!w_stat_host) return fail(ST_ERR_ARG, "null input");
+    int rc = ensure_stats(s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->w_stat, w_stat_host, (size_t)kNvStat * s->g.nx * s->g.ny * sizeof(double),
+                          hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     return ST_OK;
 }
 

@@ -476,38 +476,76 @@ __global__ void k_recyc_inflow(Geo g, KPar P, double *__restrict__ w, const doub
 // [i0,i1]x[j0,j1]x[k0,k1]; with image=true a ghost point of a periodic
 // direction reads w at its periodic image, which is the value bcswap
 // would have copied there.
-template <typename R>
+// With DT it also forms the nine spectral radii of compute_dt_subroutine
+// (kernels_cpu.F90:6814-6880) at the interior points it covers and writes
+// one block maximum per workgroup: the first stage of a step that recomputes
+// dt evaluates the same primitives of the same w as compute_dt's own
+// compute_aux(central=1) at the start of the step.
+template <typename R, bool DT = false>
 __global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__restrict__ w, R *__restrict__ wa,
-                                             int i0, int ni, int j0, int nj, int k0, int nk, int image) {
+                                             int i0, int ni, int j0, int nj, int k0, int nk, int image,
+                                             const double *__restrict__ dx1, const double *__restrict__ dy1,
+                                             const double *__restrict__ dz1, const double *__restrict__ dxs,
+                                             const double *__restrict__ dys, const double *__restrict__ dzs,
+                                             double *__restrict__ partials) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ni * nj * nk) return;
-    const int i = i0 + t % ni;
-    const int jk = t / ni;
-    const int j = j0 + jk % nj;
-    const int k = k0 + jk / nj;
-    int is = i, js = j, ks = k;
-    if (image) {
-        if (P.periodic[0]) is = i < 1 ? i + g.nx : (i > g.nx ? i - g.nx : i);
-        if (P.periodic[1]) js = j < 1 ? j + g.ny : (j > g.ny ? j - g.ny : j);
-        if (P.periodic[2]) ks = k < 1 ? k + g.nz : (k > g.nz ? k - g.nz : k);
+    double mx = 0.0;
+    if (t < ni * nj * nk) {
+        const int i = i0 + t % ni;
+        const int jk = t / ni;
+        const int j = j0 + jk % nj;
+        const int k = k0 + jk / nj;
+        int is = i, js = j, ks = k;
+        if (image) {
+            if (P.periodic[0]) is = i < 1 ? i + g.nx : (i > g.nx ? i - g.nx : i);
+            if (P.periodic[1]) js = j < 1 ? j + g.ny : (j > g.ny ? j - g.ny : j);
+            if (P.periodic[2]) ks = k < 1 ? k + g.nz : (k > g.nz ? k - g.nz : k);
+        }
+        const int s = gidx(g, is, js, ks), d = gidx(g, i, j, k);
+        const size_t nv = g.nvol;
+        const double rho = w[s], rhou = w[nv + s], rhov = w[2 * nv + s], rhow = w[3 * nv + s], rhoe = w[4 * nv + s];
+        const double ri = 1.0 / rho;
+        const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
+        const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+        const double ee = rhoe / rho - qq;
+        const double tt = temperature_from_e(P, ee, double(wa[5 * nv + d]));
+        const double pp = rho * tt * P.rgas0;
+        const double mu = viscosity_of(P, tt);
+        wa[d] = R(rho);
+        wa[nv + d] = R(uu);
+        wa[2 * nv + d] = R(vv);
+        wa[3 * nv + d] = R(ww);
+        wa[4 * nv + d] = R((rhoe + pp) / rho);
+        wa[5 * nv + d] = R(tt);
+        wa[6 * nv + d] = R(mu);
+        if constexpr (sizeof(R) != sizeof(double)) wa[10 * nv + d] = R(pp - P.p_ref);
+        if constexpr (DT) {
+            if (i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz) {
+                const double nu = ri * mu;
+                const double kor = nu / P.prandtl;
+                const double gamloc = gamloc_of(P, tt);
+                const double cs = sqrt(gamloc * P.rgas0 * tt);
+                const double ax = dx1[i - 1], ay = dy1[j - 1], az = dz1[k - 1];
+                const double sx = dxs[i - 1], sy = dys[j - 1], sz = dzs[k - 1];
+                mx = fmax(mx, (fabs(uu) + cs) * ax);
+                mx = fmax(mx, (fabs(vv) + cs) * ay);
+                mx = fmax(mx, (fabs(ww) + cs) * az);
+                mx = fmax(mx, nu * sx);
+                mx = fmax(mx, nu * sy);
+                mx = fmax(mx, nu * sz);
+                mx = fmax(mx, kor * sx);
+                mx = fmax(mx, kor * sy);
+                mx = fmax(mx, kor * sz);
+            }
+        }
     }
-    const int s = gidx(g, is, js, ks), d = gidx(g, i, j, k);
-    const size_t nv = g.nvol;
-    const double rho = w[s], rhou = w[nv + s], rhov = w[2 * nv + s], rhow = w[3 * nv + s], rhoe = w[4 * nv + s];
-    const double ri = 1.0 / rho;
-    const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
-    const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
-    const double ee = rhoe / rho - qq;
-    const double tt = temperature_from_e(P, ee, double(wa[5 * nv + d]));
-    const double pp = rho * tt * P.rgas0;
-    wa[d] = R(rho);
-    wa[nv + d] = R(uu);
-    wa[2 * nv + d] = R(vv);
-    wa[3 * nv + d] = R(ww);
-    wa[4 * nv + d] = R((rhoe + pp) / rho);
-    wa[5 * nv + d] = R(tt);
-    wa[6 * nv + d] = R(viscosity_of(P, tt));
-    if constexpr (sizeof(R) != sizeof(double)) wa[10 * nv + d] = R(pp - P.p_ref);
+    if constexpr (DT) {
+        __shared__ double lds[4];
+        mx = wave_max(mx);
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+    }
 }
 
 // ------------------------------------------------------- dirty-cell flags
@@ -1667,6 +1705,100 @@ __global__ __launch_bounds__(256) void k_force_var2(Geo g, KPar P, double *__res
     w[4 * nv + c] = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
 }
 
+// ------------------------------------------------------------ statistics
+// compute_stats (singleideal.F90:2003-2179, grid_dim 1): one thread per
+// (i, j) walks k = 1..nz in order, so each spanwise sum is accumulated in
+// the reference's order; the 70 running sums live in LDS, one column per
+// thread.  Velocity gradients from w (with ghosts), T and mu from
+// w_aux(6), w_aux(7).  Then w_stat = (w_stat*itav + sum/nz)/(itav + 1),
+// w_stat(1:70, 1:nx, 1:ny) in Fortran order.
+constexpr int NV_STAT = 70;
+constexpr int STAT_BX = 64;
+
+template <typename R>
+__global__ __launch_bounds__(STAT_BX) void k_stats(Geo g, KPar P, const double *__restrict__ w,
+                                                   const R *__restrict__ wa, const double *__restrict__ dx1,
+                                                   const double *__restrict__ dy1, const double *__restrict__ dz1,
+                                                   double *__restrict__ w_stat, int itav) {
+    __shared__ double zs[NV_STAT][STAT_BX];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * STAT_BX + t + 1;
+    const int j = blockIdx.y + 1;
+    if (i > g.nx) return;  // no barrier below: each thread owns its column
+#pragma unroll
+    for (int q = 0; q < NV_STAT; q++) zs[q][t] = 0.0;
+    const size_t nv = g.nvol;
+    const int lmax = P.visc_lmax;
+    const double dx = dx1[i - 1], dy = dy1[j - 1];
+    const double rgas0 = P.rgas0;
+    auto vel = [&](int c, int m) { return w[m * nv + c] / w[c]; };
+    for (int k = 1; k <= g.nz; k++) {
+        const int c = gidx(g, i, j, k);
+        double ux = 0, vx = 0, wx = 0, uy = 0, vy = 0, wy = 0, uz = 0, vz = 0, wz = 0;
+        for (int l = 1; l <= lmax; l++) {
+            const double ccl = P.vd1[l - 1];
+            const int xp = c + l, xm = c - l, yp = c + l * g.sx, ym = c - l * g.sx, zp = c + l * g.sxy,
+                      zm = c - l * g.sxy;
+            ux = ux + ccl * (vel(xp, 1) - vel(xm, 1));
+            vx = vx + ccl * (vel(xp, 2) - vel(xm, 2));
+            wx = wx + ccl * (vel(xp, 3) - vel(xm, 3));
+            uy = uy + ccl * (vel(yp, 1) - vel(ym, 1));
+            vy = vy + ccl * (vel(yp, 2) - vel(ym, 2));
+            wy = wy + ccl * (vel(yp, 3) - vel(ym, 3));
+            uz = uz + ccl * (vel(zp, 1) - vel(zm, 1));
+            vz = vz + ccl * (vel(zp, 2) - vel(zm, 2));
+            wz = wz + ccl * (vel(zp, 3) - vel(zm, 3));
+        }
+        const double dz = dz1[k - 1];
+        ux = ux * dx; vx = vx * dx; wx = wx * dx;
+        uy = uy * dy; vy = vy * dy; wy = wy * dy;
+        uz = uz * dz; vz = vz * dz; wz = wz * dz;
+        const double omx = wy - vz, omy = uz - wx, omz = vx - uy;
+        const double rho = w[c], rhou = w[nv + c], rhov = w[2 * nv + c], rhow = w[3 * nv + c];
+        const double ri = 1.0 / rho;
+        const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
+        const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+        const double tt = double(wa[5 * nv + c]);
+        const double pp = rho * rgas0 * tt;
+        const double mu = double(wa[6 * nv + c]);
+        const double nu = mu / rho;
+        const double gamloc = gamloc_of(P, tt);
+        const double cploc = gamloc / (gamloc - 1.0) * rgas0;
+        const double cs = sqrt(gamloc * rgas0 * tt);
+        const double machlocal = sqrt(2.0 * qq) / cs;
+        const double t_tot = tt + qq * (gamloc - 1.0) / gamloc;
+        const double rho2 = rho * rho, uu2 = uu * uu, vv2 = vv * vv, ww2 = ww * ww;
+        const double pp2 = pp * pp, tt2 = tt * tt;
+        const double divl = ux + vy + wz;
+        const double div3l = divl / 3.0;
+        double s11 = 2.0 * (ux - div3l), s12 = uy + vx, s13 = uz + wx;
+        double s22 = 2.0 * (vy - div3l), s23 = vz + wy, s33 = 2.0 * (wz - div3l);
+        s11 = s11 * mu; s12 = s12 * mu; s13 = s13 * mu;
+        s22 = s22 * mu; s23 = s23 * mu; s33 = s33 * mu;
+        const double s21 = s12, s31 = s13, s32 = s23;
+        int q = 0;
+        auto add = [&](double v) { zs[q][t] = zs[q][t] + v; q++; };
+        add(rho); add(uu); add(vv); add(ww); add(pp); add(tt); add(rho2); add(uu2); add(vv2); add(ww2);
+        add(pp2); add(tt2); add(rhou); add(rhov); add(rhow); add(rhou * uu); add(rhov * vv); add(rhow * ww);
+        add(rhou * vv); add(mu);
+        add(nu); add(omx * omx); add(omy * omy); add(omz * omz); add(rho * tt); add(rho * tt2); add(t_tot);
+        add(rho * t_tot); add(t_tot * t_tot); add(rhou * tt);
+        add(rhov * tt); add(rhow * tt); add(machlocal); add(machlocal * machlocal); add(rhou * uu2);
+        add(rhov * uu2); add(rhou * vv2); add(rhov * vv2); add(rhou * ww2); add(rhov * ww2);
+        add(pp * uu); add(pp * vv); add(s11); add(s12); add(s13); add(s22); add(s23); add(s33); add(s11 * uu);
+        add(s12 * uu);
+        add(s21 * vv); add(s22 * vv); add(s31 * ww); add(s32 * ww); add(s11 * vv + s21 * uu);
+        add(s12 * vv + s22 * uu); add(s11 * ux + s12 * uy + s13 * uz); add(s21 * vx + s22 * vy + s23 * vz);
+        add(s31 * wx + s32 * wy + s33 * wz); add(s11 * vx + s12 * (ux + vy) + s22 * uy + s13 * vz + s23 * uz);
+        add(pp * ux); add(pp * vy); add(pp * wz); add(pp * (uy + vx)); add(divl * divl); add(rho * tt2 * tt);
+        add(rho * tt2 * tt2); add(rhou * uu2 * uu); add(cploc); add(gamloc);
+    }
+    double *ws = w_stat + (size_t)NV_STAT * ((size_t)(i - 1) + (size_t)g.nx * (size_t)(j - 1));
+    const double nzd = (double)g.nz, ta = (double)itav, tb = (double)(itav + 1);
+#pragma unroll
+    for (int q = 0; q < NV_STAT; q++) ws[q] = (ws[q] * ta + zs[q][t] / nzd) / tb;
+}
+
 // ---------------------------------------------------------------- CFL dt
 // compute_aux(central=1, ghost=0) + compute_dt_subroutine (kernels_cpu.F90:
 // 6960-7011, 6814-6880): primitives recomputed from w in registers (w_aux
@@ -1926,12 +2058,22 @@ hipError_t launch_extrapolate_aux(st_solver *s, int var) {
     return hipGetLastError();
 }
 
-hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int k1, bool image) {
+hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int k1, bool image, bool dt) {
     const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
     if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
     const size_t tot = (size_t)ni * nj * nk;
-    ST_DISPATCH(hipLaunchKernelGGL(k_aux<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, s->g, s->kp,
-                                   s->w, (R *)s->w_aux, i0, ni, j0, nj, k0, nk, image ? 1 : 0));
+    const unsigned nblk = (unsigned)((tot + 255) / 256);
+#define ST_LAUNCH_AUX(D_)                                                                                          \
+    ST_DISPATCH(hipLaunchKernelGGL((k_aux<R, D_>), dim3(nblk), dim3(256), 0, s->stream, s->g, s->kp, s->w,          \
+                                   (R *)s->w_aux, i0, ni, j0, nj, k0, nk, image ? 1 : 0, s->dcsidx, s->detady,       \
+                                   s->dzitdz, s->dcsidxs, s->detadys, s->dzitdzs, s->partials))
+    if (dt && s->p.cfl > 0) ST_LAUNCH_AUX(true);
+    else ST_LAUNCH_AUX(false);
+#undef ST_LAUNCH_AUX
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !(dt && s->p.cfl > 0)) return e;
+    hipLaunchKernelGGL(k_dt_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, (int)nblk,
+                       s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
     return hipGetLastError();
 }
 
@@ -2206,6 +2348,14 @@ hipError_t launch_residual(st_solver *s) {
 hipError_t launch_count_weno(st_solver *s) {
     ST_DISPATCH(hipLaunchKernelGGL(k_count_weno<R>, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp,
                                    (const R *)s->w_aux, s->ds));
+    return hipGetLastError();
+}
+
+hipError_t launch_stats(st_solver *s, int itav) {
+    const Geo &g = s->g;
+    const dim3 grid((g.nx + STAT_BX - 1) / STAT_BX, g.ny);
+    ST_DISPATCH(hipLaunchKernelGGL(k_stats<R>, grid, dim3(STAT_BX), 0, s->stream, g, s->kp, s->w,
+                                   (const R *)s->w_aux, s->dcsidx, s->detady, s->dzitdz, s->w_stat, itav));
     return hipGetLastError();
 }
 

@@ -23,6 +23,8 @@ struct st_solver {
     double betarecyc, glund1;
     double *rc_real;    // wmean planes (ng*ny*4), weta (ny), y+/eta arrays (4*(ny+2ng)), averages (ng*ny*5)
     int *rc_maps;       // map_j_inn (ny), map_j_out (ny)
+    double *w_stat;     // running statistics w_stat(1:70, 1:nx, 1:ny), allocated on first use
+    bool dt_in_stage;   // the CFL reduction of this step rides on the first stage's aux
     int visc_chunk;     // planes per tile of the z-marching viscous kernel (0: automatic)
     int march_chunk;    // points per thread of the y/z flux marches (0: automatic)
     double *wallprop;   // 3 x (sx*sz), wallprop(:,:,2:4)
@@ -61,7 +63,8 @@ namespace st {
 hipError_t launch_fill_periodic(st_solver *s, double *arr, int nv);
 hipError_t launch_fill_periodic_aux(st_solver *s, int var);
 hipError_t launch_extrapolate_aux(st_solver *s, int var);
-hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int k1, bool image);
+hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int k1, bool image,
+                      bool dt = false);
 hipError_t launch_wall_ghosts(st_solver *s);
 // physical ghosts of every non-periodic face in the order xmin..zmax
 // (update_ghost without the periodic swap)
@@ -85,6 +88,7 @@ hipError_t launch_dt(st_solver *s);
 hipError_t launch_step_end(st_solver *s);
 hipError_t launch_residual(st_solver *s);
 hipError_t launch_count_weno(st_solver *s);
+hipError_t launch_stats(st_solver *s, int itav);
 hipError_t launch_zero(st_solver *s, double *a, size_t n);
 hipError_t launch_zero_rhs(st_solver *s, void *a);
 

@@ -108,6 +108,21 @@ module streams_mi355x
       import :: c_int, c_ptr
       type(c_ptr), value :: s
     end function
+    integer(c_int) function st_compute_stats(s, itav) bind(c, name="st_compute_stats")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: s
+      integer(c_int32_t), value :: itav
+    end function
+    integer(c_int) function st_get_stats(s, w_stat) bind(c, name="st_get_stats")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: s
+      real(c_double), intent(out) :: w_stat(*)
+    end function
+    integer(c_int) function st_set_stats(s, w_stat) bind(c, name="st_set_stats")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: s
+      real(c_double), intent(in) :: w_stat(*)
+    end function
     integer(c_int) function st_set_recycling(s, i_recyc, betarecyc, glund1, wmean_inflow, weta, map_j_inn, &
                                              map_j_out, yplus_inflow, eta_inflow, yplus_recyc, eta_recyc) &
                                              bind(c, name="st_set_recycling")
@@ -169,6 +184,9 @@ module streams_mi355x
     procedure :: euler_z => mi_euler_z
     procedure :: bc_nr => mi_bc_nr
     procedure :: set_recycling => mi_set_recycling
+    procedure :: compute_stats => mi_compute_stats
+    procedure :: get_stats => mi_get_stats
+    procedure :: set_stats => mi_set_stats
     procedure :: init_run => mi_init_run
     procedure :: compute_dt => mi_compute_dt
     procedure :: rk_sync => mi_rk_sync
@@ -291,6 +309,27 @@ contains
                                 map_j_inn, map_j_out, yplus_inflow, eta_inflow, yplus_recyc, eta_recyc), &
                'set_recycling')
   end subroutine mi_set_recycling
+
+  ! compute_stats (singleideal.F90:2003-2179) on the device: the itav-th
+  ! sample of the running statistics; get/set move w_stat(1:70,1:nx,1:ny)
+  ! for write_stats / read_stats
+  subroutine mi_compute_stats(self, itav)
+    class(equation_singleideal_mi355x_object), intent(inout) :: self
+    integer, intent(in) :: itav
+    call check(st_compute_stats(self%handle, int(itav, c_int32_t)), 'compute_stats')
+  end subroutine mi_compute_stats
+
+  subroutine mi_get_stats(self, w_stat)
+    class(equation_singleideal_mi355x_object), intent(inout) :: self
+    real(c_double), intent(out) :: w_stat(:,:,:)
+    call check(st_get_stats(self%handle, w_stat), 'get_stats')
+  end subroutine mi_get_stats
+
+  subroutine mi_set_stats(self, w_stat)
+    class(equation_singleideal_mi355x_object), intent(inout) :: self
+    real(c_double), intent(in) :: w_stat(:,:,:)
+    call check(st_set_stats(self%handle, w_stat), 'set_stats')
+  end subroutine mi_set_stats
 
   subroutine mi_init_run(self)
     class(equation_singleideal_mi355x_object), intent(inout) :: self

@@ -57,6 +57,9 @@ _SIGNATURES = {
     "st_advance": (C.c_int, [C.c_void_p, C.c_int32]),
     "st_compute_residual": (C.c_int, [C.c_void_p, C.POINTER(StScalars)]),
     "st_count_weno": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "st_compute_stats": (C.c_int, [C.c_void_p, C.c_int32]),
+    "st_get_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "st_set_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "st_kernel_times": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     "st_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "st_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -18,6 +18,8 @@ from . import capi
 from .grid import Grid
 from .params import StParams, StScalars
 
+NV_STAT = 70  # statistics of compute_stats (singleideal.F90:2003-2179)
+
 
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
@@ -143,6 +145,22 @@ class ChannelSolver:
         s = StScalars()
         capi.check(self.L.st_compute_residual(self.h, C.byref(s)))
         return s.as_dict()
+
+    def compute_stats(self, itav: int):
+        """Fold the current state into the device's running statistics as
+        the itav-th sample (compute_stats)."""
+        capi.check(self.L.st_compute_stats(self.h, int(itav)))
+
+    def get_stats(self) -> np.ndarray:
+        """w_stat(1:70, 1:nx, 1:ny), Fortran order."""
+        out = np.zeros((NV_STAT, self.p.nx, self.p.ny), order="F")
+        capi.check(self.L.st_get_stats(self.h, _dp(out)))
+        return out
+
+    def set_stats(self, w_stat: np.ndarray):
+        a = np.asfortranarray(w_stat, dtype=np.float64)
+        assert a.shape == (NV_STAT, self.p.nx, self.p.ny)
+        capi.check(self.L.st_set_stats(self.h, _dp(a)))
 
     def count_weno(self) -> np.ndarray:
         out = np.zeros(3)

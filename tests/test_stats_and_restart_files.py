@@ -118,3 +118,41 @@ def test_stat_bin_layout(tmp_path):
         block = raw[q * nx * ny:(q + 1) * nx * ny]
         assert np.array_equal(block, ws[q].ravel(order="F"))   # i fastest, then j
     assert np.array_equal(read_stats(path, nx, ny), ws)
+
+
+def test_rst_bin_layout_and_round_trip(tmp_path):
+    """write_field: one nx*ny*nz block per conserved variable, x fastest,
+    interior cells only, no header; read_field reads it back."""
+    from streams_mi355x.io import read_restart, write_restart
+
+    nx, ny, nz, ng = 5, 4, 3, 3
+    w = np.random.default_rng(2).normal(size=(5, nz + 2 * ng, ny + 2 * ng, nx + 2 * ng))
+    path = tmp_path / "rst.bin"
+    write_restart(path, w, ng)
+    raw = np.fromfile(path, dtype="<f8")
+    assert raw.size == 5 * nx * ny * nz
+    n = nx * ny * nz
+    for m in range(5):
+        block = raw[m * n:(m + 1) * n]
+        # Fortran w(1:nx,1:ny,1:nz,m): i fastest, then j, then k
+        want = np.array([w[m, ng + k, ng + j, ng + i] for k in range(nz) for j in range(ny) for i in range(nx)])
+        assert np.array_equal(block, want)
+    assert np.array_equal(read_restart(path, nx, ny, nz), w[:, ng:-ng, ng:-ng, ng:-ng])
+
+
+def test_field_info_round_trip(tmp_path):
+    from streams_mi355x.io import FIELD_INFO_KEYS, read_field_info, write_field_info
+
+    info = {"icyc0": 1234, "time0": 0.123456789012345678, "itav": 7, "istore": 3,
+            "time_from_last_stat": 1.0 / 3.0}
+    path = tmp_path / "field_info0.dat"
+    write_field_info(path, info)
+    text = path.read_text().splitlines()
+    assert text[0] == "[field_info]"
+    assert [ln.split(" = ")[0] for ln in text[1:]] == list(FIELD_INFO_KEYS)
+    back = read_field_info(path)
+    for k, v in info.items():
+        assert back[k] == v
+    # the reference's list-directed spelling (Fortran exponent letter) reads too
+    path.write_text("[field_info]\nicyc0 = 10\ntime0 = 0.25D+01\nitav = 2\n")
+    assert read_field_info(path)["time0"] == 2.5
