@@ -948,8 +948,8 @@ __device__ __forceinline__ void hybrid_face(const KPar &P, const Pt (&q)[6], con
 // into an LDS queue with a ballot so the WENO work runs on full waves.
 template <typename R, int DIR>
 __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const R *__restrict__ wa,
-                                         const int *__restrict__ eoc, const double *__restrict__ metric,
-                                         R *__restrict__ fl, int oi) {
+                                         const double *__restrict__ w, const int *__restrict__ eoc,
+                                         const double *__restrict__ metric, R *__restrict__ fl, int oi) {
     const size_t nv = g.nvol;
     const bool keep = P.nkeep >= 0;
     const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
@@ -964,8 +964,31 @@ __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const R *_
 #pragma unroll
     for (int s = 0; s < 7; s++) {
         const int a = base + (s - 3) * stride;
-        q[s] = make_pt(double(wa[a]), double(wa[nv + a]), double(wa[2 * nv + a]), double(wa[3 * nv + a]),
-                       double(wa[4 * nv + a]), double(wa[5 * nv + a]), P.rgas0, keep);
+        if constexpr (sizeof(R) == sizeof(double)) {
+            q[s] = make_pt(double(wa[a]), double(wa[nv + a]), double(wa[2 * nv + a]), double(wa[3 * nv + a]),
+                           double(wa[4 * nv + a]), double(wa[5 * nv + a]), P.rgas0, keep);
+        } else {
+            // the mixed form stores w_aux in FP32: the WENO stencil takes the
+            // FP64 primitives of w instead, formed as eval_aux forms them
+            // (periodic neighbours from their images)
+            int pi = i, pj = j, pk = k;
+            if (DIR == 0) pi += s - 3;
+            else if (DIR == 1) pj += s - 3;
+            else pk += s - 3;
+            if (P.periodic[0]) pi = pi < 1 ? pi + g.nx : (pi > g.nx ? pi - g.nx : pi);
+            if (P.periodic[1]) pj = pj < 1 ? pj + g.ny : (pj > g.ny ? pj - g.ny : pj);
+            if (P.periodic[2]) pk = pk < 1 ? pk + g.nz : (pk > g.nz ? pk - g.nz : pk);
+            const int src = gidx(g, pi, pj, pk);
+            const double rho = w[src], rhou = w[nv + src], rhov = w[2 * nv + src], rhow = w[3 * nv + src],
+                         rhoe = w[4 * nv + src];
+            const double ri = 1.0 / rho;
+            const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
+            const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+            const double ee = rhoe / rho - qq;
+            const double tt = temperature_from_e(P, ee, double(wa[5 * nv + a]));
+            const double pp = rho * tt * P.rgas0;
+            q[s] = make_pt(rho, uu, vv, ww, (rhoe + pp) / rho, tt, P.rgas0, keep);
+        }
         sen[s] = double(wa[7 * nv + a]);
     }
     double flo[5], fhi[5];
@@ -990,8 +1013,9 @@ __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const R *_
 
 template <typename R, int DIR>
 __global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const R *__restrict__ wa,
-                                                 const int *__restrict__ eoc, const double *__restrict__ metric,
-                                                 R *__restrict__ fl, unsigned char *__restrict__ flags,
+                                                 const double *__restrict__ w, const int *__restrict__ eoc,
+                                                 const double *__restrict__ metric, R *__restrict__ fl,
+                                                 unsigned char *__restrict__ flags,
                                                  int region) {
     __shared__ int queue[64 * 5];
     const int lane = threadIdx.x;
@@ -1024,7 +1048,7 @@ __global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const R *__restr
                 if (qn >= 64 * 4) {  // keep room for the next byte position
                     __syncthreads();
                     while (qn >= 64) {
-                        fix_cell<R, DIR>(g, P, wa, eoc, metric, fl, queue[qn - 64 + lane]);
+                        fix_cell<R, DIR>(g, P, wa, w, eoc, metric, fl, queue[qn - 64 + lane]);
                         qn -= 64;
                     }
                     __syncthreads();
@@ -1033,12 +1057,12 @@ __global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const R *__restr
         }
         __syncthreads();
         while (qn >= 64) {
-            fix_cell<R, DIR>(g, P, wa, eoc, metric, fl, queue[qn - 64 + lane]);
+            fix_cell<R, DIR>(g, P, wa, w, eoc, metric, fl, queue[qn - 64 + lane]);
             qn -= 64;
         }
         __syncthreads();
     }
-    if (lane < qn) fix_cell<R, DIR>(g, P, wa, eoc, metric, fl, queue[lane]);
+    if (lane < qn) fix_cell<R, DIR>(g, P, wa, w, eoc, metric, fl, queue[lane]);
 }
 
 // ---------------------------------------------------------------- viscous
@@ -1426,6 +1450,87 @@ __global__ __launch_bounds__(256) void k_sensor(Geo g, KPar P, R *__restrict__ w
     sv = fmax(sv, R(0.0));
     wa[7 * nv + c] = sv * sv;
     if (images) write_images(g, P, wa + 7 * nv, i, j, k, sv * sv, false);
+}
+
+// Ducros sensor of the mixed form evaluated in FP64 from the conserved
+// state w, so that the central/WENO switch sees the values the FP64 form
+// computes: velocities u = rho*u * (1/rho) as eval_aux forms them (periodic
+// neighbours from their images), the first derivatives of the viscous
+// kernel in the same order, then the sensor either directly (DIRECT: as
+// visflx mode 0 does at the start of a run, kernels_cpu.F90:3598-3777) or
+// through |omega| and div/3 as they are stored (sensor_subroutine, :4726-4744,
+// at the third stage).  Only the rounding of the stored sensor is FP32.
+template <typename R, bool DIRECT>
+__global__ __launch_bounds__(256) void k_sensor_w(Geo g, KPar P, const double *__restrict__ w, R *__restrict__ wa,
+                                                  const double *__restrict__ dx1, const double *__restrict__ dy1,
+                                                  const double *__restrict__ dz1, int images) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const size_t nv = g.nvol;
+    auto at = [&](int ii, int jj, int kk) {
+        if (P.periodic[0]) ii = ii < 1 ? ii + g.nx : (ii > g.nx ? ii - g.nx : ii);
+        if (P.periodic[1]) jj = jj < 1 ? jj + g.ny : (jj > g.ny ? jj - g.ny : jj);
+        if (P.periodic[2]) kk = kk < 1 ? kk + g.nz : (kk > g.nz ? kk - g.nz : kk);
+        return gidx(g, ii, jj, kk);
+    };
+    auto vel = [&](int c, int m) {
+        const double ri = 1.0 / w[c];
+        return w[m * nv + c] * ri;
+    };
+    double ux = 0, vx = 0, wx = 0, uy = 0, vy = 0, wy = 0, uz = 0, vz = 0, wz = 0;
+    const int lmax = P.visc_lmax;
+#pragma unroll
+    for (int l = 1; l <= H; l++) {
+        if (l <= lmax) {
+            const double ccl = P.vd1[l - 1];
+            const int xp = at(i + l, j, k), xm = at(i - l, j, k);
+            ux = ux + ccl * (vel(xp, 1) - vel(xm, 1));
+            vx = vx + ccl * (vel(xp, 2) - vel(xm, 2));
+            wx = wx + ccl * (vel(xp, 3) - vel(xm, 3));
+        }
+    }
+#pragma unroll
+    for (int l = 1; l <= H; l++) {
+        if (l <= lmax) {
+            const double ccl = P.vd1[l - 1];
+            const int yp = at(i, j + l, k), ym = at(i, j - l, k);
+            uy = uy + ccl * (vel(yp, 1) - vel(ym, 1));
+            vy = vy + ccl * (vel(yp, 2) - vel(ym, 2));
+            wy = wy + ccl * (vel(yp, 3) - vel(ym, 3));
+        }
+    }
+#pragma unroll
+    for (int l = 1; l <= H; l++) {
+        if (l <= lmax) {
+            const double ccl = P.vd1[l - 1];
+            const int zp = at(i, j, k + l), zm = at(i, j, k - l);
+            uz = uz + ccl * (vel(zp, 1) - vel(zm, 1));
+            vz = vz + ccl * (vel(zp, 2) - vel(zm, 2));
+            wz = wz + ccl * (vel(zp, 3) - vel(zm, 3));
+        }
+    }
+    const double ax = dx1[i - 1], ay = dy1[j - 1], az = dz1[k - 1];
+    ux = ux * ax; vx = vx * ax; wx = wx * ax;
+    uy = uy * ay; vy = vy * ay; wy = wy * ay;
+    uz = uz * az; vz = vz * az; wz = wz * az;
+    double div = ux + vy + wz;
+    const double omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
+    double omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
+    if (!DIRECT) {
+        // the third-stage sensor reads |omega| and div/3 as stored
+        const double om = sqrt(omod2);
+        const double div3l = div / 3.0;
+        omod2 = om * om;
+        div = 3.0 * div3l;
+    }
+    const double u0l0 = P.u0 / P.l0;
+    double sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
+    sv = fmax(sv, 0.0);
+    const int c = gidx(g, i, j, k);
+    wa[7 * nv + c] = R(sv * sv);
+    if (images) write_images(g, P, wa + 7 * nv, i, j, k, R(sv * sv), false);
 }
 
 // visflx_div_subroutine (kernels_cpu.F90:3484-3532).  With REDUCE the
@@ -2098,13 +2203,13 @@ static hipError_t launch_fix(st_solver *s, R *fl, int dir) {
     const dim3 grid((unsigned)nb), block(64);
     const R *wa = (const R *)s->w_aux;
     if (dir == 0)
-        hipLaunchKernelGGL((k_weno_fix<R, 0>), grid, block, 0, s->stream, g, s->kp, wa, s->eoc[0], s->dcsidx, fl,
+        hipLaunchKernelGGL((k_weno_fix<R, 0>), grid, block, 0, s->stream, g, s->kp, wa, s->w, s->eoc[0], s->dcsidx, fl,
                            s->dirty, region);
     else if (dir == 1)
-        hipLaunchKernelGGL((k_weno_fix<R, 1>), grid, block, 0, s->stream, g, s->kp, wa, s->eoc[1], s->detady, fl,
+        hipLaunchKernelGGL((k_weno_fix<R, 1>), grid, block, 0, s->stream, g, s->kp, wa, s->w, s->eoc[1], s->detady, fl,
                            s->dirty, region);
     else
-        hipLaunchKernelGGL((k_weno_fix<R, 2>), grid, block, 0, s->stream, g, s->kp, wa, s->eoc[2], s->dzitdz, fl,
+        hipLaunchKernelGGL((k_weno_fix<R, 2>), grid, block, 0, s->stream, g, s->kp, wa, s->w, s->eoc[2], s->dzitdz, fl,
                            s->dirty, region);
     return hipGetLastError();
 }
@@ -2219,6 +2324,16 @@ hipError_t launch_visc_cons(st_solver *s, void *fl, int mode, bool ghost_images)
 hipError_t launch_sensor(st_solver *s, bool ghost_images) {
     ST_DISPATCH(hipLaunchKernelGGL(k_sensor<R>, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, (R *)s->w_aux,
                                    ghost_images ? 1 : 0));
+    return hipGetLastError();
+}
+
+hipError_t launch_sensor_w(st_solver *s, bool direct, bool ghost_images) {
+    if (direct)
+        ST_DISPATCH(hipLaunchKernelGGL((k_sensor_w<R, true>), cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp,
+                                       s->w, (R *)s->w_aux, s->dcsidx, s->detady, s->dzitdz, ghost_images ? 1 : 0));
+    else
+        ST_DISPATCH(hipLaunchKernelGGL((k_sensor_w<R, false>), cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp,
+                                       s->w, (R *)s->w_aux, s->dcsidx, s->detady, s->dzitdz, ghost_images ? 1 : 0));
     return hipGetLastError();
 }
 

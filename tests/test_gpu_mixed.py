@@ -90,3 +90,25 @@ def test_fp64_unchanged_by_the_mixed_build():
             g.advance(5)
             outs.append(g.copy_to_field())
     assert np.array_equal(interior(outs[0]), interior(outs[1]))
+
+
+def test_mixed_switch_decisions_follow_fp64():
+    """With the shock sensor active on many faces, the mixed form evaluates
+    the sensor in FP64 from w: its central/WENO decisions are the FP64
+    form's, so the WENO face counts agree and the error stays within the
+    tolerance."""
+    c = make_channel(nx=128, ny=96, nz=64, seed=2, sensor_threshold=0.3)
+    pm = params_copy(c.params)
+    pm.precision = PREC_MIXED
+    out = []
+    for p in (c.params, pm):
+        with ChannelSolver(p, c.grid) as g:
+            g.copy_from_field(c.w0)
+            g.init_run()
+            g.advance(40)
+            out.append((g.copy_to_field(), g.count_weno()))
+    (wd, cd), (wm, cm) = out
+    assert cd.max() > 1e-3                     # the switch is in play
+    assert np.allclose(cm, cd, rtol=1e-3, atol=0.0), (cm, cd)
+    errs = [rel_l2(interior(wm)[m], interior(wd)[m]) for m in range(5)]
+    assert max(errs) <= MIXED_TOL, errs
