@@ -58,7 +58,7 @@ typedef struct st_params {
     int32_t nkeep;                   /* KEEP(n): 0 KEEP, >0 KEEP(n), <0 non-KEEP split    */
     int32_t weno_scheme;             /* 1..4, WENO order 2*weno_scheme-1 (3 = WENO5)      */
     int32_t weno_version;            /* 0 Jiang-Shu, 1 WENO-Z                             */
-    int32_t flux_splitting;          /* 0 Roe characteristic (1 Rusanov: not supported)   */
+    int32_t flux_splitting;          /* WENO faces: 0 Roe characteristic, 1 Rusanov      */
     int32_t conservative_viscous;    /* 0 Laplacian form, 1 conservative form             */
     int32_t visc_model;              /* 0 inviscid, 1 power law, 2 Sutherland             */
     int32_t calorically_perfect;     /* 1 constant cp, 0 cp(T) polynomial                 */

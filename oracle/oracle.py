@@ -53,6 +53,7 @@ def lib():
         L.or_compute_roe_average.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
         L.or_eigenvectors.argtypes = [C.c_int, dp, dp, dp]
         L.or_wenorec_1d.argtypes = [dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.or_wenorec_1d_rusanov.argtypes = [dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
         L.or_eval_aux.argtypes = [vp] + [C.c_int] * 6
         L.or_compute_aux.argtypes = [vp, C.c_int, C.c_int]
         for name in ("or_bcswap", "or_zero_flux", "or_euler_x", "or_euler_y", "or_euler_z",

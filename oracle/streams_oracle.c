@@ -26,8 +26,14 @@
  *   (modes 2, 5, 6, 7)        face/cell differences are exact
  *   whole channel path        the reference's golden progress row
  *                             (tests/tests.ini, [subsonic_channel])
- * The Rusanov split and the BL/SBLI boundary conditions are not restated
- * here.
+ *   WENO faces, first order   the local Lax-Friedrichs flux of the Roe and
+ *   (Roe and Rusanov split)   the Rusanov split in closed form
+ *   or_force_var              bulk T equal to the target after the call
+ *   compute_stats             a manufactured field (tests/test_stats_and_restart_files.py)
+ * Boundary conditions (tests/test_oracle_bc_pins.py): free stream,
+ * extrapolation, symmetry, node wall and shock faces by their definitions,
+ * bc_nr against a finite-difference Jacobian and its eigen-decomposition,
+ * the recycling inflow on stations with known fluctuations.
  */
 #include "streams_oracle.h"
 
@@ -1004,6 +1010,11 @@ void or_eigenvectors(int dir, const double roe[9], double el[5][5], double er[5]
 static double sq(double a) { return a * a; }
 
 /* wenorec_1d, kernels_cpu.F90:2526-2724.  gp/gm[m][l], l = 1..2*iweno. */
+static void wenorec_scaled(const double vp[5][9], const double vm[5][9], double vhat[5], int iweno,
+                           int wenorec_ord, int weno_version, const double betascale[5]);
+
+/* wenorec_1d (kernels_cpu.F90:2526-2724): characteristic variables, scaled
+ * by rho0^2 u0^2 (waves 1, 2, 5) and rho0^2 u0^4 (waves 3, 4). */
 void or_wenorec_1d(const double vp[5][9], const double vm[5][9], double vhat[5], int iweno,
                    int wenorec_ord, int weno_version, double rho0, double u0)
 {
@@ -1016,6 +1027,30 @@ void or_wenorec_1d(const double vp[5][9], const double vm[5][9], double vhat[5],
     betascale[2] = 1.0 / rho0_2u0_4;
     betascale[3] = betascale[2];
     betascale[4] = betascale[0];
+    wenorec_scaled(vp, vm, vhat, iweno, wenorec_ord, weno_version, betascale);
+}
+
+/* wenorec_1d_rusanov (kernels_cpu.F90:2325-2523): the same reconstruction
+ * of the conserved components, scaled by rho0^2 u0^2 (mass), rho0^2 u0^4
+ * (momenta) and rho0^2 u0^6 (energy). */
+void or_wenorec_1d_rusanov(const double vp[5][9], const double vm[5][9], double vhat[5], int iweno,
+                           int wenorec_ord, int weno_version, double rho0, double u0)
+{
+    double u0_2 = u0 * u0;
+    double rho0_2u0_2 = rho0 * rho0 * u0_2;
+    double rho0_2u0_4 = rho0_2u0_2 * u0_2;
+    double betascale[5];
+    betascale[0] = 1.0 / rho0_2u0_2;
+    betascale[1] = 1.0 / rho0_2u0_4;
+    betascale[2] = betascale[1];
+    betascale[3] = betascale[1];
+    betascale[4] = betascale[1] / u0_2;
+    wenorec_scaled(vp, vm, vhat, iweno, wenorec_ord, weno_version, betascale);
+}
+
+static void wenorec_scaled(const double vp[5][9], const double vm[5][9], double vhat[5], int iweno,
+                           int wenorec_ord, int weno_version, const double betascale[5])
+{
     int i = iweno;
     if (wenorec_ord == 1) {
         for (int m = 0; m < 5; m++) vhat[m] = vp[m][i] + vm[m][i + 1];
@@ -1242,6 +1277,53 @@ void or_euler_fluxes(or_state *o, int dir, int fstart, int fend)
                     }
                     fh[1 + dir] = fh[1 + dir] + 0.5 * ft6;
                     for (int m = 0; m < 5; m++) FH(i, j, k, m + 1) = fh[m];
+                } else if (p->flux_splitting == 1) {
+                    /* Rusanov split of the WENO faces, the *_hybrid_rusanov_kernel
+                     * branches (kernels_cpu.F90:887-934, 2268-2315, 1521-1568):
+                     * one eigenvalue bound over the stencil, conserved components */
+                    double evmax = -1.0;
+                    for (int l = 1; l <= weno_size; l++) {
+                        int a, b, c;
+                        stp(dir, i, j, k, l - weno_scheme, &a, &b, &c);
+                        double un = WA(a, b, c, 2 + dir);
+                        double tt = WA(a, b, c, 6);
+                        double gamloc = or_get_gamloc(o, tt);
+                        double cs = sqrt(gamloc * rgas0 * tt);
+                        double evm = dmax(fabs(un - cs), fabs(un + cs));
+                        evmax = dmax(evm, evmax);
+                    }
+                    double gp[5][9], gm[5][9];
+                    for (int l = 1; l <= weno_size; l++) {
+                        int a, b, c;
+                        stp(dir, i, j, k, l - weno_scheme, &a, &b, &c);
+                        double rho = WA(a, b, c, 1), uu = WA(a, b, c, 2), vv = WA(a, b, c, 3);
+                        double ww = WA(a, b, c, 4), h = WA(a, b, c, 5);
+                        double un = dir == 0 ? uu : (dir == 1 ? vv : ww);
+                        double rhoun = rho * un;
+                        double pp = rho * WA(a, b, c, 6) * rgas0;
+                        double rhoevm = rho * evmax;
+                        double evm, cc;
+                        evm = rhoun;
+                        cc = 0.5 * (evm + rhoevm);
+                        gp[0][l] = cc; gm[0][l] = evm - cc;
+                        evm = uu * rhoun + (dir == 0 ? pp : 0.0);
+                        cc = 0.5 * (evm + rhoevm * uu);
+                        gp[1][l] = cc; gm[1][l] = evm - cc;
+                        evm = vv * rhoun + (dir == 1 ? pp : 0.0);
+                        cc = 0.5 * (evm + rhoevm * vv);
+                        gp[2][l] = cc; gm[2][l] = evm - cc;
+                        evm = ww * rhoun + (dir == 2 ? pp : 0.0);
+                        cc = 0.5 * (evm + rhoevm * ww);
+                        gp[3][l] = cc; gm[3][l] = evm - cc;
+                        evm = h * rhoun;
+                        cc = 0.5 * (evm + evmax * (rho * h - pp));
+                        gp[4][l] = cc; gm[4][l] = evm - cc;
+                    }
+                    int wenorec_ord = weno_scheme + eoc;
+                    if (wenorec_ord < 1) wenorec_ord = 1;
+                    double fi[5];
+                    or_wenorec_1d_rusanov(gp, gm, fi, weno_scheme, wenorec_ord, p->weno_version, p->rho0, p->u0);
+                    for (int m = 0; m < 5; m++) FH(i, j, k, m + 1) = fi[m];
                 } else {
                     double roe[9], el[5][5], er[5][5];
                     or_compute_roe_average(o, dir, i, j, k, roe);

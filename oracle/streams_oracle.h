@@ -84,6 +84,8 @@ void or_compute_roe_average(const or_state *o, int dir, int i, int j, int k, dou
 void or_eigenvectors(int dir, const double roe[9], double el[5][5], double er[5][5]);
 void or_wenorec_1d(const double gp[5][9], const double gm[5][9], double vhat[5], int iweno,
                    int wenorec_ord, int weno_version, double rho0, double u0);
+void or_wenorec_1d_rusanov(const double gp[5][9], const double gm[5][9], double vhat[5], int iweno,
+                           int wenorec_ord, int weno_version, double rho0, double u0);
 
 void or_eval_aux(or_state *o, int i0, int i1, int j0, int j1, int k0, int k1);
 void or_compute_aux(or_state *o, int central, int ghost);

@@ -61,7 +61,8 @@ int validate(const st_params *p) {
     if (p->visc_order != p->ep_order) return fail(ST_ERR_ARG, "viscous order must be equal to ep_order");
     if (p->weno_scheme < 1 || p->weno_scheme > 3) return fail(ST_ERR_ARG, "weno_scheme must be 1..3 with ng = 3");
     if (p->weno_version != 0 && p->weno_version != 1) return fail(ST_ERR_ARG, "weno_version must be 0 or 1");
-    if (p->flux_splitting != 0) return fail(ST_ERR_ARG, "flux_splitting=1 (Rusanov) is not supported");
+    if (p->flux_splitting != 0 && p->flux_splitting != 1)
+        return fail(ST_ERR_ARG, "flux_splitting must be 0 (Roe) or 1 (Rusanov)");
     if (p->conservative_viscous != 0 && p->conservative_viscous != 1)
         return fail(ST_ERR_ARG, "conservative_viscous must be 0 or 1");
     if (p->calorically_perfect != 0 && p->calorically_perfect != 1)
@@ -151,6 +152,7 @@ void fill_kpar(st_solver *s) {
     k.visc_lmax = p.visc_order / 2;
     k.weno_scheme = p.weno_scheme;
     k.weno_version = p.weno_version;
+    k.flux_splitting = p.flux_splitting;
     k.nkeep = p.nkeep;
     k.calorically_perfect = p.calorically_perfect;
     k.indx_cp_l = p.indx_cp_l;

@@ -383,13 +383,17 @@ __device__ __forceinline__ double wenorec3(const double vp[6], const double vm[6
 #undef VM
 }
 
-// Characteristic WENO flux at the face of a 6-point window (H = 3).
+// Characteristic WENO flux at the face of a 6-point window (H = 3).  The
+// eigenvalue bounds run over the weno_size = 2*weno_scheme points of the
+// reference's stencil, window points 3-ws .. 2+ws.
 template <int DIR>
 __device__ __forceinline__ void weno_flux(const KPar &P, const Pt (&q)[6], int eoc, double fh[5]) {
     const Roe s = roe_average(P, q[2], q[3]);
     double evmax[3] = {-1.0, -1.0, -1.0};  // |un-c|, |un|, |un+c|
+    const int ws = P.weno_scheme;
 #pragma unroll
     for (int l = 0; l < 6; l++) {
+        if (l < 3 - ws || l > 2 + ws) continue;
         const double un = un_of<DIR>(q[l]);
         const double tt = q[l].t;
         const double gamloc = gamloc_of(P, tt);
@@ -443,6 +447,62 @@ __device__ __forceinline__ void weno_flux(const KPar &P, const Pt (&q)[6], int e
     }
 #pragma unroll
     for (int v = 0; v < 5; v++) fh[v] = acc[v];
+}
+
+// Rusanov WENO flux (the WENO branch of the *_hybrid_rusanov_kernel
+// routines, kernels_cpu.F90:887-934, 1521-1568, 2268-2315): one bound
+// max(|un-c|, |un+c|) over the stencil splits each conserved component's
+// flux as 1/2 (f +- evmax u), and wenorec_1d_rusanov (:2325-2523)
+// reconstructs it with the component scales 1/(rho0^2 u0^2), 1/(rho0^2
+// u0^4) for the momenta and 1/(rho0^2 u0^6) for the energy.
+template <int DIR>
+__device__ __forceinline__ void weno_flux_rusanov(const KPar &P, const Pt (&q)[6], int eoc, double fh[5]) {
+    const int ws = P.weno_scheme;
+    double evmax = -1.0;
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+        if (l < 3 - ws || l > 2 + ws) continue;
+        const double un = un_of<DIR>(q[l]);
+        const double tt = q[l].t;
+        const double gamloc = gamloc_of(P, tt);
+        const double c = sqrt(gamloc * P.rgas0 * tt);
+        const double evm = fmax(fabs(un - c), fabs(un + c));
+        evmax = fmax(evm, evmax);
+    }
+    int ord = P.weno_scheme + eoc;
+    if (ord < 1) ord = 1;
+    const double u0_2 = P.u0 * P.u0;
+    const double r2u2 = P.rho0 * P.rho0 * u0_2;
+    const double r2u4 = r2u2 * u0_2;
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        double gp[6], gm[6];
+#pragma unroll
+        for (int l = 0; l < 6; l++) {
+            const Pt &a = q[l];
+            const double un = un_of<DIR>(a);
+            const double rhoun = a.r * un;
+            const double pp = a.r * a.t * P.rgas0;
+            const double rhoevm = a.r * evmax;
+            double evm, c;
+            if (m == 0) {
+                evm = rhoun;
+                c = 0.5 * (evm + rhoevm);
+            } else if (m == 4) {
+                evm = a.h * rhoun;
+                c = 0.5 * (evm + evmax * (a.r * a.h - pp));
+            } else {
+                const double vel = m == 1 ? a.u : (m == 2 ? a.v : a.w);
+                evm = vel * rhoun;
+                if (m == 1 + DIR) evm = evm + pp;
+                c = 0.5 * (evm + rhoevm * vel);
+            }
+            gp[l] = c;
+            gm[l] = evm - c;
+        }
+        const double bs = m == 0 ? 1.0 / r2u2 : (m == 4 ? 1.0 / r2u4 / u0_2 : 1.0 / r2u4);
+        fh[m] = wenorec3(gp, gm, ord, P.weno_version, bs);
+    }
 }
 
 }  // namespace st

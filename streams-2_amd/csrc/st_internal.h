@@ -38,6 +38,7 @@ struct KPar {
     int lmax_base;         // ep_order/2
     int visc_lmax;         // visc_order/2
     int weno_scheme, weno_version, nkeep;
+    int flux_splitting;    // WENO faces: 0 Roe characteristic, 1 Rusanov (conserved components)
     int calorically_perfect, indx_cp_l, indx_cp_r;
     int visc_model;
     int fzf[6];            // force_zero_flux

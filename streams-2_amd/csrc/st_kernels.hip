@@ -937,6 +937,8 @@ __device__ __forceinline__ void hybrid_face(const KPar &P, const Pt (&q)[6], con
         if (P.nkeep == 0) central_flux<DIR, H, Pc, 0>(P, c, lmax, zf, fh);
         else if (P.nkeep > 0) central_flux<DIR, H, Pc, 1>(P, c, lmax, zf, fh);
         else central_flux<DIR, H, Pc, -1>(P, c, lmax, zf, fh);
+    } else if (P.flux_splitting == 1) {
+        weno_flux_rusanov<DIR>(P, q, e, fh);
     } else {
         weno_flux<DIR>(P, q, e, fh);
     }

@@ -312,3 +312,44 @@ def test_trajectory_subsonic_channel_100_steps():
     errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
     print("subsonic channel 100-step rel L2 per variable:", errs)
     assert max(errs) <= TRAJ_TOL, errs
+
+
+def _pair_params(threshold, **fields):
+    """A channel pair whose StParams get the given fields."""
+    c = make_channel(nx=32, ny=24, nz=16, sensor_threshold=threshold, seed=0)
+    p = params_copy(c.params)
+    for k, v in fields.items():
+        setattr(p, k, v)
+    return c, Oracle(p, c.grid), ChannelSolver(p, c.grid)
+
+
+@pytest.mark.parametrize("fields", [dict(flux_splitting=1), dict(flux_splitting=1, weno_scheme=2),
+                                    dict(flux_splitting=0, weno_scheme=2), dict(flux_splitting=1, weno_version=1)],
+                         ids=["rusanov", "rusanov-weno3", "roe-weno3", "rusanov-wenoz"])
+def test_euler_all_weno_split_variants(fields):
+    """Every face WENO (threshold -1): both flux splittings, WENO3 and WENO-Z."""
+    c, o, g = _pair_params(-1.0, **fields)
+    _load_state(c, o, g)
+    for d in ("x", "y", "z"):
+        o.zero_flux()
+        g.zero_flux()
+        getattr(o, "euler_" + d)()
+        getattr(g, "euler_" + d)()
+        fo, fg = o.fl, g.get_fl()
+        for m in range(5):
+            assert rel_l2(fg[m], fo[m]) <= UNIT_TOL, (fields, d, m, rel_l2(fg[m], fo[m]))
+    g.close()
+
+
+def test_trajectory_rusanov_20_steps():
+    c, o, g = _pair_params(0.5, flux_splitting=1)
+    o.w[:] = c.w0
+    g.copy_from_field(c.w0)
+    o.init_run()
+    g.init_run()
+    o.advance(20)
+    g.advance(20)
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    assert max(errs) <= TRAJ_TOL, errs
+    g.close()
