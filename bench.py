@@ -12,8 +12,22 @@ metric: Gcell-updates/s per RK stage = nx*ny*nz * 3 * steps / seconds / 1e9.
 
 Prints one JSON line.  The roofline object covers the kernel family with the
 largest total time, from HIP events recorded on the solver's own stream
-around every launch; the CPU baseline times the oracle (a restatement of the
-reference CPU kernels) on a bounded sample of the same workload.
+around every launch (the CFL reduction of each step rides on the first
+stage's aux pass, so it is timed within "aux"); its traffic comes from the
+committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the same workload.  The
+CPU baseline times the oracle (a restatement of the reference CPU kernels)
+on a bounded sample of the same workload.
+
+Two more blocks of the same line:
+* "mixed": the mixed-precision form on the same workload, its speedup, its
+  relative L2 error per conserved variable against the FP64 form after 100
+  steps, and the FP64 form's own difference after 100 steps from an initial
+  field perturbed by 1e-7, the scale against which that error is read.
+* "serving": the serving workload of 64 seeded channel solves of uneven
+  sizes (grids uniform in [64,256] x [64,128] x [64,128], bulk Mach 0.3-3.0,
+  Re_tau 100-500, Poisson arrivals, 200 steps each) through the library's
+  continuous-batching server, with p50/p99 time-to-solution, against a
+  sample of the same requests run one after another through the plain path.
 """
 import argparse
 import json

@@ -234,8 +234,8 @@ struct TimeScope {
         HIPCHK(expr);              \
     } while (0)
 
-// The mixed form with the Laplacian viscous terms evaluates the Ducros
-// sensor in FP64 from w (its fluxes and stresses stay FP32).
+// The mixed form with the Laplacian viscous terms refines the Ducros sensor
+// near the threshold in FP64 from w (its fluxes and stresses stay FP32).
 bool fp64_sensor(const st_solver *s) { return s->mixed && s->p.conservative_viscous != 1; }
 
 // one RK stage of rk_sync (singleideal_cpu.F90:352-406), fused
@@ -257,11 +257,11 @@ int stage(st_solver *s, int istep) {
     else TIMED(ST_K_VISC, launch_visc(s, fcur, 4, true));
     TIMED(ST_K_EULER_Y, launch_euler_y(s, fcur));
     if (s->kp.ndim == 3) TIMED(ST_K_EULER_Z, launch_euler_z(s, fcur));
-    // visflx(3) + bcswap_var(w_aux 8); the mixed Laplacian form takes the
-    // sensor in FP64 from w so that its switch decisions are the FP64 form's
+    // visflx(3) + bcswap_var(w_aux 8); the mixed Laplacian form refines the
+    // sensor near the threshold in FP64 so its switch decisions are the FP64 form's
     if (istep == 2) {
-        if (fp64_sensor(s)) TIMED(ST_K_SENSOR, launch_sensor_w(s, false, true));
-        else TIMED(ST_K_SENSOR, launch_sensor(s, true));
+        TIMED(ST_K_SENSOR, launch_sensor(s, true));
+        if (fp64_sensor(s)) TIMED(ST_K_SENSOR, launch_sensor_refine(s, false, true));
     }
     TIMED(ST_K_VISC_DIV, launch_visc_div(s, fcur, fprev, istep, s->p.channel_case != 0, cons));  // visflx(1) or (7)
     if (s->p.channel_case) TIMED(ST_K_FINISH, launch_force_finish(s, istep));
@@ -602,13 +602,13 @@ int st_visflx(st_solver *s, int32_t mode) {
     switch (mode) {
     case 0:
         HIPCHK(st::launch_visc(s, fl, 0, false));
-        if (fp64_sensor(s)) HIPCHK(st::launch_sensor_w(s, true, false));
+        if (fp64_sensor(s)) HIPCHK(st::launch_sensor_refine(s, true, false));
         break;
     case 1: HIPCHK(st::launch_visc_div(s, fl, s->F[s->cur ^ 1], -1, false, false)); break;
     case 2: HIPCHK(st::launch_visc_cons(s, fl, 2, false)); break;
     case 3:
-        if (fp64_sensor(s)) HIPCHK(st::launch_sensor_w(s, false, false));
-        else HIPCHK(st::launch_sensor(s, false));
+        HIPCHK(st::launch_sensor(s, false));
+        if (fp64_sensor(s)) HIPCHK(st::launch_sensor_refine(s, false, false));
         break;
     case 4: HIPCHK(st::launch_visc(s, fl, 4, false)); break;
     case 5: HIPCHK(st::launch_visc_cons(s, fl, 5, false)); break;
@@ -665,7 +665,7 @@ int st_init_run(st_solver *s) {
     // visflx(2) for the conservative viscous form, else visflx(0)
     if (s->p.conservative_viscous == 1) HIPCHK(st::launch_visc_cons(s, s->F[s->cur], 2, false));
     else HIPCHK(st::launch_visc(s, s->F[s->cur], 0, false));
-    if (fp64_sensor(s)) HIPCHK(st::launch_sensor_w(s, true, false));
+    if (fp64_sensor(s)) HIPCHK(st::launch_sensor_refine(s, true, false));
     if ((rc = swap_var(s, 8, false))) return rc;
     if ((rc = st_zero_flux(s))) return rc;
     s->last_stage = -1;

@@ -1454,22 +1454,17 @@ __global__ __launch_bounds__(256) void k_sensor(Geo g, KPar P, R *__restrict__ w
     if (images) write_images(g, P, wa + 7 * nv, i, j, k, sv * sv, false);
 }
 
-// Ducros sensor of the mixed form evaluated in FP64 from the conserved
-// state w, so that the central/WENO switch sees the values the FP64 form
-// computes: velocities u = rho*u * (1/rho) as eval_aux forms them (periodic
-// neighbours from their images), the first derivatives of the viscous
-// kernel in the same order, then the sensor either directly (DIRECT: as
-// visflx mode 0 does at the start of a run, kernels_cpu.F90:3598-3777) or
-// through |omega| and div/3 as they are stored (sensor_subroutine, :4726-4744,
-// at the third stage).  Only the rounding of the stored sensor is FP32.
-template <typename R, bool DIRECT>
-__global__ __launch_bounds__(256) void k_sensor_w(Geo g, KPar P, const double *__restrict__ w, R *__restrict__ wa,
-                                                  const double *__restrict__ dx1, const double *__restrict__ dy1,
-                                                  const double *__restrict__ dz1, int images) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.nx || j > g.ny) return;
+// Ducros sensor in FP64 from the conserved state w at interior cell
+// (i, j, k), as the FP64 form computes it: velocities u = rho*u * (1/rho) as
+// eval_aux forms them (periodic neighbours from their images), the first
+// derivatives of the viscous kernel in the same order, then the sensor
+// either directly (DIRECT: as visflx mode 0 does at the start of a run,
+// kernels_cpu.F90:3598-3777) or through |omega| and div/3 as they are
+// stored (sensor_subroutine, :4726-4744, at the third stage).
+template <bool DIRECT>
+__device__ double sensor_fp64(const Geo &g, const KPar &P, const double *__restrict__ w,
+                              const double *__restrict__ dx1, const double *__restrict__ dy1,
+                              const double *__restrict__ dz1, int i, int j, int k) {
     const size_t nv = g.nvol;
     auto at = [&](int ii, int jj, int kk) {
         if (P.periodic[0]) ii = ii < 1 ? ii + g.nx : (ii > g.nx ? ii - g.nx : ii);
@@ -1483,35 +1478,26 @@ __global__ __launch_bounds__(256) void k_sensor_w(Geo g, KPar P, const double *_
     };
     double ux = 0, vx = 0, wx = 0, uy = 0, vy = 0, wy = 0, uz = 0, vz = 0, wz = 0;
     const int lmax = P.visc_lmax;
-#pragma unroll
-    for (int l = 1; l <= H; l++) {
-        if (l <= lmax) {
-            const double ccl = P.vd1[l - 1];
-            const int xp = at(i + l, j, k), xm = at(i - l, j, k);
-            ux = ux + ccl * (vel(xp, 1) - vel(xm, 1));
-            vx = vx + ccl * (vel(xp, 2) - vel(xm, 2));
-            wx = wx + ccl * (vel(xp, 3) - vel(xm, 3));
-        }
+    for (int l = 1; l <= lmax; l++) {
+        const double ccl = P.vd1[l - 1];
+        const int xp = at(i + l, j, k), xm = at(i - l, j, k);
+        ux = ux + ccl * (vel(xp, 1) - vel(xm, 1));
+        vx = vx + ccl * (vel(xp, 2) - vel(xm, 2));
+        wx = wx + ccl * (vel(xp, 3) - vel(xm, 3));
     }
-#pragma unroll
-    for (int l = 1; l <= H; l++) {
-        if (l <= lmax) {
-            const double ccl = P.vd1[l - 1];
-            const int yp = at(i, j + l, k), ym = at(i, j - l, k);
-            uy = uy + ccl * (vel(yp, 1) - vel(ym, 1));
-            vy = vy + ccl * (vel(yp, 2) - vel(ym, 2));
-            wy = wy + ccl * (vel(yp, 3) - vel(ym, 3));
-        }
+    for (int l = 1; l <= lmax; l++) {
+        const double ccl = P.vd1[l - 1];
+        const int yp = at(i, j + l, k), ym = at(i, j - l, k);
+        uy = uy + ccl * (vel(yp, 1) - vel(ym, 1));
+        vy = vy + ccl * (vel(yp, 2) - vel(ym, 2));
+        wy = wy + ccl * (vel(yp, 3) - vel(ym, 3));
     }
-#pragma unroll
-    for (int l = 1; l <= H; l++) {
-        if (l <= lmax) {
-            const double ccl = P.vd1[l - 1];
-            const int zp = at(i, j, k + l), zm = at(i, j, k - l);
-            uz = uz + ccl * (vel(zp, 1) - vel(zm, 1));
-            vz = vz + ccl * (vel(zp, 2) - vel(zm, 2));
-            wz = wz + ccl * (vel(zp, 3) - vel(zm, 3));
-        }
+    for (int l = 1; l <= lmax; l++) {
+        const double ccl = P.vd1[l - 1];
+        const int zp = at(i, j, k + l), zm = at(i, j, k - l);
+        uz = uz + ccl * (vel(zp, 1) - vel(zm, 1));
+        vz = vz + ccl * (vel(zp, 2) - vel(zm, 2));
+        wz = wz + ccl * (vel(zp, 3) - vel(zm, 3));
     }
     const double ax = dx1[i - 1], ay = dy1[j - 1], az = dz1[k - 1];
     ux = ux * ax; vx = vx * ax; wx = wx * ax;
@@ -1521,7 +1507,6 @@ __global__ __launch_bounds__(256) void k_sensor_w(Geo g, KPar P, const double *_
     const double omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
     double omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
     if (!DIRECT) {
-        // the third-stage sensor reads |omega| and div/3 as stored
         const double om = sqrt(omod2);
         const double div3l = div / 3.0;
         omod2 = om * om;
@@ -1530,9 +1515,43 @@ __global__ __launch_bounds__(256) void k_sensor_w(Geo g, KPar P, const double *_
     const double u0l0 = P.u0 / P.l0;
     double sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
     sv = fmax(sv, 0.0);
+    return sv * sv;
+}
+
+// Shock-sensor refinement of the mixed form.  The FP32 sensor decides the
+// central/WENO switch like the FP64 one except within its own round-off of
+// the threshold, so the cells whose FP32 value lies within kSensorBand of
+// the threshold take the FP64 value of sensor_fp64 and store it snapped
+// to the side of the threshold that value is on: the sweeps compare the
+// stored value with the FP32 threshold, the WENO fix-up with the FP64 one,
+// and both then give the FP64 form's decision.  Far from the threshold the
+// FP32 error (well below 1e-5) cannot change a decision.
+constexpr double kSensorBand = 1e-3;
+
+template <bool DIRECT>
+__global__ __launch_bounds__(256) void k_sensor_refine(Geo g, KPar P, const double *__restrict__ w,
+                                                       float *__restrict__ wa, const double *__restrict__ dx1,
+                                                       const double *__restrict__ dy1,
+                                                       const double *__restrict__ dz1, int images) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const size_t nv = g.nvol;
     const int c = gidx(g, i, j, k);
-    wa[7 * nv + c] = R(sv * sv);
-    if (images) write_images(g, P, wa + 7 * nv, i, j, k, R(sv * sv), false);
+    const double thr = P.sensor_threshold;
+    if (fabs(double(wa[7 * nv + c]) - thr) >= kSensorBand) return;
+    const double s64 = sensor_fp64<DIRECT>(g, P, w, dx1, dy1, dz1, i, j, k);
+    const float rt = float(thr);
+    float st = float(s64);
+    if (s64 > thr) {
+        st = fmaxf(st, nextafterf(rt, INFINITY));          // > thr in both precisions
+    } else {
+        const float lo = double(rt) > thr ? nextafterf(rt, -INFINITY) : rt;
+        st = fminf(st, lo);                                   // <= thr in both precisions
+    }
+    wa[7 * nv + c] = st;
+    if (images) write_images(g, P, wa + 7 * nv, i, j, k, st, false);
 }
 
 // visflx_div_subroutine (kernels_cpu.F90:3484-3532).  With REDUCE the
@@ -2329,13 +2348,14 @@ hipError_t launch_sensor(st_solver *s, bool ghost_images) {
     return hipGetLastError();
 }
 
-hipError_t launch_sensor_w(st_solver *s, bool direct, bool ghost_images) {
+hipError_t launch_sensor_refine(st_solver *s, bool direct, bool ghost_images) {
+    if (!s->mixed) return hipSuccess;
     if (direct)
-        ST_DISPATCH(hipLaunchKernelGGL((k_sensor_w<R, true>), cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp,
-                                       s->w, (R *)s->w_aux, s->dcsidx, s->detady, s->dzitdz, ghost_images ? 1 : 0));
+        hipLaunchKernelGGL((k_sensor_refine<true>), cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, s->w,
+                           (float *)s->w_aux, s->dcsidx, s->detady, s->dzitdz, ghost_images ? 1 : 0);
     else
-        ST_DISPATCH(hipLaunchKernelGGL((k_sensor_w<R, false>), cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp,
-                                       s->w, (R *)s->w_aux, s->dcsidx, s->detady, s->dzitdz, ghost_images ? 1 : 0));
+        hipLaunchKernelGGL((k_sensor_refine<false>), cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, s->w,
+                           (float *)s->w_aux, s->dcsidx, s->detady, s->dzitdz, ghost_images ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -6,9 +6,14 @@
 ! the C ABI), the loop runs on the GPU through the bind(c) backend object,
 ! and the final conserved field and run scalars are written back.
 !
-! usage: streams_mi355x_run case.bin out.bin
+! usage: streams_mi355x_run case.bin out.bin [dir]
 !        streams_mi355x_run --sizes          byte sizes of the bind(c) records
 !        streams_mi355x_run --read case.bin  what was read, no device work
+! With dir, the end of the run is also written there as the reference's
+! restart and statistics files: rst.bin (write_field, field.F90:1923-1966),
+! field_info0.dat (write_field_info, singleideal.F90:2620-2659) and stat.bin
+! (write_stats, singleideal.F90:1779-1822) of one compute_stats sample taken
+! on the device.
 program streams_mi355x_run
   use, intrinsic :: iso_c_binding
   use streams_mi355x
@@ -17,11 +22,11 @@ program streams_mi355x_run
   type(st_scalars_t) :: s
   type(equation_singleideal_mi355x_object) :: eq
   integer(c_int32_t) :: nsteps
-  integer :: sx, sy, sz, nvol, u
+  integer :: sx, sy, sz, u, l
   real(c_double), allocatable :: x(:), y(:), z(:), yn(:)
   real(c_double), allocatable :: dcsidx(:), dcsidxs(:), dcsidx2(:), detady(:), detadys(:), detady2(:)
-  real(c_double), allocatable :: dzitdz(:), dzitdzs(:), dzitdz2(:), w(:)
-  character(len=512) :: fin, fout
+  real(c_double), allocatable :: dzitdz(:), dzitdzs(:), dzitdz2(:), w(:,:,:,:), w_stat(:,:,:)
+  character(len=512) :: fin, fout, dir
   real(c_double) :: dt
 
   call get_command_argument(1, fin)
@@ -38,10 +43,11 @@ program streams_mi355x_run
   sx = p%nx + 2*p%ng
   sy = p%ny + 2*p%ng
   sz = p%nz + 2*p%ng
-  nvol = sx*sy*sz
   allocate(x(sx), y(sy), z(sz), yn(p%ny+1))
   allocate(dcsidx(p%nx), dcsidxs(p%nx), dcsidx2(p%nx), detady(p%ny), detadys(p%ny), detady2(p%ny))
-  allocate(dzitdz(p%nz), dzitdzs(p%nz), dzitdz2(p%nz), w(5*nvol))
+  allocate(dzitdz(p%nz), dzitdzs(p%nz), dzitdz2(p%nz))
+  ! the ghosted conserved field with the reference's bounds, w(1-ng:nx+ng, ..., 1:5)
+  allocate(w(1-p%ng:p%nx+p%ng, 1-p%ng:p%ny+p%ng, 1-p%ng:p%nz+p%ng, 5))
   read(u) x, y, z, yn, dcsidx, dcsidxs, dcsidx2, detady, detadys, detady2, dzitdz, dzitdzs, dzitdz2
   read(u) w
   close(u)
@@ -60,11 +66,42 @@ program streams_mi355x_run
   call eq%advance(int(nsteps))
   call eq%compute_residual(s)
   call eq%copy_to_field(w)
-  call eq%finalize()
   write(*,'(a,i0,a,es20.10)') 'steps: ', s%icyc, '  residual_rhou: ', s%residual_rhou
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) w
   write(u) s
   close(u)
+
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, dir)
+    ! rst.bin: the interior of each conserved variable, x fastest
+    open(newunit=u, file=trim(dir)//'/rst.bin', access='stream', form='unformatted', status='replace')
+    write(u) w(1:p%nx, 1:p%ny, 1:p%nz, 1:5)
+    close(u)
+    ! field_info0.dat: the [field_info] section of the run's end
+    open(newunit=u, file=trim(dir)//'/field_info0.dat', status='replace')
+    write(u, '(a)') '[field_info]'
+    write(u, '(a,i0)') 'icyc0 = ', s%icyc
+    write(u, '(a,es25.17)') 'time0 = ', s%time
+    write(u, '(a)') 'itav = 1'
+    close(u)
+    ! stat.bin: one statistics sample of the final state, one block per variable
+    call eq2_stats(w_stat)
+    open(newunit=u, file=trim(dir)//'/stat.bin', access='stream', form='unformatted', status='replace')
+    do l = 1, 70
+      write(u) w_stat(l, 1:p%nx, 1:p%ny)
+    enddo
+    close(u)
+  endif
+  call eq%finalize()
+
+contains
+
+  subroutine eq2_stats(ws)
+    real(c_double), allocatable, intent(out) :: ws(:,:,:)
+    allocate(ws(70, p%nx, p%ny))
+    call eq%compute_stats(0)
+    call eq%get_stats(ws)
+  end subroutine eq2_stats
 end program streams_mi355x_run

@@ -34,9 +34,12 @@ def read_result(path: str, shape):
     return w, s.as_dict()
 
 
-def run_driver(case_path: str, out_path: str, timeout: float = 600.0) -> str:
-    """Run the Fortran driver as a child process and return its output."""
-    r = subprocess.run([DRIVER, case_path, out_path], capture_output=True, text=True, timeout=timeout)
+def run_driver(case_path: str, out_path: str, timeout: float = 600.0, files_dir: str = None) -> str:
+    """Run the Fortran driver as a child process and return its output.
+    With files_dir the driver also writes rst.bin, field_info0.dat and
+    stat.bin there."""
+    cmd = [DRIVER, case_path, out_path] + ([files_dir] if files_dir else [])
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"Fortran driver failed ({r.returncode}): {r.stdout}\n{r.stderr}")
     return r.stdout

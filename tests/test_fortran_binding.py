@@ -42,16 +42,29 @@ def test_fortran_driver_matches_oracle(tmp_path):
     from oracle import Oracle
     from streams_mi355x import make_channel
 
+    from streams_mi355x.io import read_field_info, read_restart, read_stats
+
     c = make_channel(nx=32, ny=24, nz=16, sensor_threshold=0.0)
+    p = c.params
     case = tmp_path / "case.bin"
     outp = tmp_path / "out.bin"
-    write_case(str(case), c.params, c.grid, c.w0, 10)
-    log = run_driver(str(case), str(outp))
+    write_case(str(case), p, c.grid, c.w0, 10)
+    log = run_driver(str(case), str(outp), files_dir=str(tmp_path))
     w, s = read_result(str(outp), c.w0.shape)
-    o = Oracle(c.params, c.grid)
+    o = Oracle(p, c.grid)
     o.w[:] = c.w0
     o.init_run()
     o.advance(10)
     for m in range(5):
         assert rel_l2(interior(w)[m], interior(o.w)[m]) <= 1e-10, (m, log)
     assert s["icyc"] == 10
+    # the reference's files, written by the Fortran side
+    assert np.array_equal(read_restart(tmp_path / "rst.bin", p.nx, p.ny, p.nz), interior(w))
+    info = read_field_info(tmp_path / "field_info0.dat")
+    assert info["icyc0"] == 10 and info["time0"] == s["time"]
+    ws = np.zeros((70, p.nx, p.ny), order="F")
+    o.update_ghost(1)
+    o.compute_stats(0, ws)
+    got = read_stats(tmp_path / "stat.bin", p.nx, p.ny)
+    for q in range(70):
+        assert np.abs(got[q] - ws[q]).max() <= 1e-8 * max(np.abs(ws[q]).max(), 1e-3), q + 1

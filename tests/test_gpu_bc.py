@@ -124,3 +124,48 @@ def test_recycling_trajectory_20_steps():
     print("recycling 20-step rel L2 per variable:", errs)
     assert max(errs) <= TRAJ_TOL, errs
     g.close()
+
+
+def _c4_like(precision=0, seed=7):
+    """The boundary set of the SBLI deck (configs C3/C4): recycling inflow,
+    extrapolated outflow with the characteristic treatment, isothermal wall,
+    imposed oblique shock on top, periodic span."""
+    from streams_mi355x.cases import bl_mean_planes, recycling_setup
+
+    c = make_bl_box(nx=48, ny=32, nz=16, bctags=(10, 2, 6, 7, 0, 0), bctags_nr=(0, 1, 0, 0, 0, 0),
+                    shock_angle_deg=35.0, xshock_imp=5.0, tanhfacs=8.0 / 47, seed=seed, precision=precision)
+    r = recycling_setup(c.grid, 5.0, 0.6, 0.01, 0.7, 0.011, bl_mean_planes(c))
+    return c, r
+
+
+def test_sbli_box_trajectory_20_steps():
+    c, r = _c4_like()
+    o, g = Oracle(c.params, c.grid), ChannelSolver(c.params, c.grid)
+    o.w[:] = c.w0
+    g.copy_from_field(c.w0)
+    o.set_recycling(r)
+    g.set_recycling(r)
+    o.init_run()
+    g.init_run()
+    o.advance(20)
+    g.advance(20)
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    assert max(errs) <= TRAJ_TOL, errs
+    g.close()
+
+
+def test_sbli_box_mixed_form():
+    from streams_mi355x import PREC_MIXED
+
+    out = []
+    for prec in (0, PREC_MIXED):
+        c, r = _c4_like(prec)
+        with ChannelSolver(c.params, c.grid) as g:
+            g.copy_from_field(c.w0)
+            g.set_recycling(r)
+            g.init_run()
+            g.advance(20)
+            out.append(g.copy_to_field())
+    errs = [rel_l2(interior(out[1])[m], interior(out[0])[m]) for m in range(5)]
+    assert max(errs) <= 1e-5, errs
