@@ -171,6 +171,8 @@ or_state *or_create(const st_params *p)
     o->nint = (size_t)p->nx * p->ny * p->nz;
     o->w = (double *)calloc(5 * o->nvol, sizeof(double));
     o->w_aux = (double *)calloc(10 * o->nvol, sizeof(double));
+    /* initialize (singleideal.F90:3032): T = t0 as the first Newton start */
+    for (size_t q = 0; q < o->nvol; q++) o->w_aux[5 * o->nvol + q] = p->t0;
     o->fl = (double *)calloc(5 * o->nint, sizeof(double));
     o->fln = (double *)calloc(5 * o->nint, sizeof(double));
     o->fhat = (double *)calloc(5 * o->nvol, sizeof(double));

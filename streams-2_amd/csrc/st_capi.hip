@@ -381,7 +381,8 @@ int st_solver_create(const st_params *p, st_solver **out) {
     if (ec == hipSuccess) ec = hipMemcpy(s->eoc[1], eh[1], sizeof(int) * (g.ny + 1), hipMemcpyHostToDevice);
     if (ec == hipSuccess) ec = hipMemcpy(s->eoc[2], eh[2], sizeof(int) * (g.nz + 1), hipMemcpyHostToDevice);
     std::free(buf);
-    // zero state (w_aux = 0 as in the reference alloc, singleideal.F90:1596)
+    // zero state (w_aux = 0 as in the reference alloc, singleideal.F90:1596; then
+    // w_aux(6) = t0 as in initialize, singleideal.F90:3032)
     if (ec == hipSuccess) ec = hipMemset(s->w, 0, sizeof(double) * 5 * nvol);
     if (ec == hipSuccess) ec = hipMemset(s->w_aux, 0, rsize(s) * naux(s) * nvol);
     if (ec == hipSuccess) ec = hipMemset(s->F[0], 0, rsize(s) * 5 * nint);
@@ -391,6 +392,8 @@ int st_solver_create(const st_params *p, st_solver **out) {
     std::memset(s->hs, 0, sizeof(st::DevScalars));
     if (p->cfl < 0) s->hs->dt = -p->cfl;
     if (ec == hipSuccess) ec = hipMemcpy(s->ds, s->hs, sizeof(st::DevScalars), hipMemcpyHostToDevice);
+    if (ec == hipSuccess) ec = st::launch_init_temperature(s);
+    if (ec == hipSuccess) ec = hipStreamSynchronize(s->stream);
     if (ec != hipSuccess) {
         free_all(s);
         delete s;

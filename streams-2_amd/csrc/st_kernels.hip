@@ -2138,6 +2138,11 @@ __global__ void k_zero(T *__restrict__ a, size_t n) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) a[t] = T(0);
 }
 
+template <typename T>
+__global__ void k_fill(T *__restrict__ a, size_t n, T v) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) a[t] = v;
+}
+
 // ================================================================ launchers
 static inline dim3 cell_grid(const Geo &g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz); }
 static inline int cell_blocks(const Geo &g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz; }
@@ -2493,6 +2498,15 @@ hipError_t launch_stats(st_solver *s, int itav) {
     const dim3 grid((g.nx + STAT_BX - 1) / STAT_BX, g.ny);
     ST_DISPATCH(hipLaunchKernelGGL(k_stats<R>, grid, dim3(STAT_BX), 0, s->stream, g, s->kp, s->w,
                                    (const R *)s->w_aux, s->dcsidx, s->detady, s->dzitdz, s->w_stat, itav));
+    return hipGetLastError();
+}
+
+// initialize (singleideal.F90:3032): w_aux(:,:,:,6) = t0, the first Newton
+// start of T(e) for a thermally perfect gas
+hipError_t launch_init_temperature(st_solver *s) {
+    const size_t n = s->g.nvol;
+    ST_DISPATCH(hipLaunchKernelGGL(k_fill<R>, dim3(grid1d(n, 256)), dim3(256), 0, s->stream,
+                                   (R *)s->w_aux + 5 * n, n, R(s->p.t0)));
     return hipGetLastError();
 }
 

@@ -91,6 +91,7 @@ hipError_t launch_count_weno(st_solver *s);
 hipError_t launch_stats(st_solver *s, int itav);
 hipError_t launch_sensor_refine(st_solver *s, bool direct, bool ghost_images);
 hipError_t launch_zero(st_solver *s, double *a, size_t n);
+hipError_t launch_init_temperature(st_solver *s);
 hipError_t launch_zero_rhs(st_solver *s, void *a);
 
 }  // namespace st

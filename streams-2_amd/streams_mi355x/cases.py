@@ -32,7 +32,8 @@ def make_channel(nx=32, ny=24, nz=16, mach=1.5, retau=220.0, noise=0.10, seed=0,
                         jbgrid=case.jbgrid, dyptarget=case.dyptarget, nsmooy=case.ysmoosteps,
                         metrics_order=case.ep_order, uniform_y=uniform_y)
     params = channel_params(case, grid.yn)
-    e_ref = params.cv_coeff[1] * params.t0  # calorically perfect e(T=t0)
+    # e(T = t0): every term of get_e_from_temperature but the constant vanishes
+    e_ref = params.cv_coeff[params.indx_cp_r + 1 - params.indx_cp_l] * params.t0
     w0 = channel_initial_field(grid, params.u0, noise=noise, seed=seed, rho0=params.rho0, t0=params.t0,
                                rgas0=params.rgas0, e_ref=e_ref, noise_source=noise_source)
     return ChannelCase(case, grid, params, w0)

@@ -78,6 +78,11 @@ class FlowCase:
     gam: float = 1.4
     prandtl: float = 0.72
     visc_model: int = 1              # 1 power law, 2 Sutherland
+    calorically_perfect: int = 1     # 0: cp(T) polynomial (indx_cp_l..indx_cp_r, cp_coeff)
+    indx_cp_l: int = 0
+    indx_cp_r: int = 0
+    cp_coeff: tuple = ()             # cp/R coefficients of T_dim^l, l = indx_cp_l..indx_cp_r, then h*/R
+    flux_splitting: int = 0          # WENO faces: 0 Roe characteristic, 1 Rusanov
     conservative_viscous: int = 0    # 1: second-order conservative viscous form
     vt_exp: float = 0.76
     s_suth: float = 110.4
@@ -100,24 +105,51 @@ class FlowCase:
 
 
 def set_fluid_prop(case: FlowCase):
-    """Calorically perfect gas branch of set_fluid_prop (singleideal.F90:2685-2774).
+    """set_fluid_prop (singleideal.F90:2685-2774), both branches.
 
-    Returns (gam, gm1, rfac, indx_cp_l, indx_cp_r, cp_coeff, cv_coeff) with the
-    coefficient arrays indexed l - indx_cp_l for l = indx_cp_l..indx_cp_r+1.
+    Calorically perfect: cv = R/(gam-1), cp = gam cv.  Thermally perfect
+    (calorically_perfect = 0): cp/R is the polynomial sum_l a_l T_dim^l,
+    l = indx_cp_l..indx_cp_r, with h*/R as the last coefficient; the
+    coefficients are made non-dimensional with T_ref, gamma is that of
+    T_ref, and cv_l = cp_l except cv_0 = cp_0 - 1.  Returns (gam, gm1, rfac,
+    indx_cp_l, indx_cp_r, cp_coeff, cv_coeff) with the coefficient arrays
+    indexed l - indx_cp_l for l = indx_cp_l..indx_cp_r+1.
     """
     rgas0 = 1.0
     gam = case.gam
     rfac = case.prandtl ** (1.0 / 3.0)
-    indx_cp_l = indx_cp_r = 0
+    if case.calorically_perfect == 1:
+        indx_cp_l = indx_cp_r = 0
+    else:
+        indx_cp_l, indx_cp_r = case.indx_cp_l, case.indx_cp_r
+    n = indx_cp_r - indx_cp_l + 2
+    cv = np.zeros(n)
+    cp = np.zeros(n)
+    if case.calorically_perfect == 0:
+        if len(case.cp_coeff) != n:
+            raise ValueError(f"Error! Check number of cp coefficients ({len(case.cp_coeff)} given, {n} needed)")
+        cp[:] = case.cp_coeff
+        tref = case.t_ref
+        hstar = cp[n - 1]
+        for l in range(indx_cp_l, indx_cp_r + 1):
+            a = cp[l - indx_cp_l]
+            hstar = hstar + (a * math.log(tref) if l == -1 else a / (l + 1) * tref ** (l + 1))
+        bcoeff = hstar / tref
+        cp_tref = sum(cp[l - indx_cp_l] * tref ** l for l in range(indx_cp_l, indx_cp_r + 1)) * rgas0
+        cv_tref = cp_tref - rgas0
+        gam = cp_tref / cv_tref
+        for l in range(indx_cp_l, indx_cp_r + 1):
+            cp[l - indx_cp_l] = cp[l - indx_cp_l] * tref ** l
+            cv[l - indx_cp_l] = cp[l - indx_cp_l]
+        cv[0 - indx_cp_l] = cp[0 - indx_cp_l] - 1.0
     gm1 = gam - 1.0
     gm = 1.0 / gm1
-    cv = np.zeros(2)
-    cp = np.zeros(2)
-    cv[0] = gm
-    cp[0] = gm * gam
-    bcoeff = cv[0] + 1.0
-    cp[1] = bcoeff
-    cv[1] = bcoeff - 1.0
+    if case.calorically_perfect == 1:
+        cv[0] = gm
+        cp[0] = gm * gam
+        bcoeff = cv[0] + 1.0
+    cp[n - 1] = bcoeff
+    cv[n - 1] = bcoeff - 1.0
     cv *= rgas0
     cp *= rgas0
     return gam, gm1, rfac, indx_cp_l, indx_cp_r, cp, cv
@@ -244,10 +276,10 @@ def channel_params(case: FlowCase, yn: np.ndarray, nxmax=None, nzmax=None) -> St
     p.nkeep = case.nkeep
     p.weno_scheme = case.weno_scheme
     p.weno_version = case.weno_version
-    p.flux_splitting = 0
+    p.flux_splitting = case.flux_splitting
     p.conservative_viscous = case.conservative_viscous
     p.visc_model = case.visc_model
-    p.calorically_perfect = 1
+    p.calorically_perfect = case.calorically_perfect
     p.indx_cp_l, p.indx_cp_r = icl, icr
     p.bctags[:] = list(case.bctags)
     p.bctags_nr[:] = list(case.bctags_nr)

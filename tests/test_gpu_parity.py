@@ -353,3 +353,25 @@ def test_trajectory_rusanov_20_steps():
     errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
     assert max(errs) <= TRAJ_TOL, errs
     g.close()
+
+
+NASA_AIR = (1.009950160e+04, -1.968275610e+02, 5.009155110e+00, -5.761013730e-03, 1.066859930e-05,
+            -7.940297970e-09, 2.185231910e-12, -1.767962850e+02)   # cp/R of T, l = -2..4, then h*/R
+
+
+@pytest.mark.parametrize("kw", [
+    dict(calorically_perfect=0, indx_cp_l=-2, indx_cp_r=4, cp_coeff=NASA_AIR, t_ref=220.0),
+    dict(calorically_perfect=0, indx_cp_l=-2, indx_cp_r=4, cp_coeff=NASA_AIR, t_ref=220.0, visc_model=2),
+    dict(nkeep=2), dict(nkeep=-1), dict(weno_version=1), dict(ep_order=4), dict(cfl=-0.004),
+    dict(iter_dt_recompute=2)],
+    ids=["cp(T)", "cp(T)-sutherland", "keep-n2", "non-keep", "weno-z", "central-4th", "fixed-dt", "dt-every-2"])
+def test_trajectory_variants(kw):
+    """20 steps of the channel with the sensor at 0.5 (WENO faces present)
+    in each variant of the scheme the reference's input deck selects."""
+    c, o, g = _run_pair(20, threshold=0.5, **kw)
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    assert max(errs) <= TRAJ_TOL, (kw, errs)
+    so, sg = o.scalars(), g.scalars()
+    assert abs(sg["time"] - so["time"]) <= 1e-12 * so["time"]
+    g.close()

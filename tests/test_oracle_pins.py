@@ -68,6 +68,55 @@ def test_eos_closed_forms():
         assert abs(lib().or_get_gamloc(o.h, T) - gam) < 1e-15
 
 
+NASA_AIR = (1.009950160e+04, -1.968275610e+02, 5.009155110e+00, -5.761013730e-03, 1.066859930e-05,
+            -7.940297970e-09, 2.185231910e-12, -1.767962850e+02)
+
+
+def _thermally_perfect_oracle(tref=220.0):
+    from streams_mi355x.params import FlowCase, set_fluid_prop
+    case = FlowCase(nx=8, ny=6, nz=5, calorically_perfect=0, indx_cp_l=-2, indx_cp_r=4, cp_coeff=NASA_AIR,
+                    t_ref=tref)
+    gam, gm1, rfac, icl, icr, cp, cv = set_fluid_prop(case)
+    p, g, _, _ = box_case(nx=8, ny=6, nz=5)
+    p.calorically_perfect, p.indx_cp_l, p.indx_cp_r = 0, icl, icr
+    for i in range(len(cp)):
+        p.cp_coeff[i], p.cv_coeff[i] = cp[i], cv[i]
+    p.tol_iter_nr = 1e-14
+    return Oracle(p, g), gam, cp, cv
+
+
+def test_thermally_perfect_fluid_properties():
+    """set_fluid_prop's cp(T) branch: the scaled coefficients give back the
+    NASA polynomial cp/R at T_ref * (T/t0), gamma is that of T_ref, and
+    cv = cp - R."""
+    tref = 220.0
+    o, gam, cp, cv = _thermally_perfect_oracle(tref)
+    for T in (0.6, 1.0, 2.5):
+        td = tref * T
+        nasa = sum(a * td ** l for a, l in zip(NASA_AIR[:7], range(-2, 5)))
+        ours = sum(cp[l + 2] * T ** l for l in range(-2, 5))
+        assert abs(ours - nasa) < 1e-12 * nasa
+        assert abs(sum(cv[l + 2] * T ** l for l in range(-2, 5)) - (nasa - 1.0)) < 1e-12 * nasa
+    assert abs(gam - (lambda c: c / (c - 1.0))(sum(a * tref ** l for a, l in zip(NASA_AIR[:7], range(-2, 5))))) < 1e-14
+    assert abs(lib().or_get_gamloc(o.h, 1.0) - gam) < 1e-14
+
+
+def test_thermally_perfect_energy_is_the_integral_of_cv_and_inverts():
+    """e(T) of get_e_from_temperature is the integral of cv(T) (checked by
+    central differences), e(t0) is the constant term, and the Newton
+    iteration of get_temperature_from_e inverts it from a distant start."""
+    o, gam, cp, cv = _thermally_perfect_oracle()
+    e_of = lambda T: lib().or_get_e_from_temperature(o.h, T)
+    assert abs(e_of(1.0) - cv[-1]) < 1e-15
+    for T in (0.55, 1.0, 1.7, 3.2):
+        h = 1e-5 * T
+        dedT = (e_of(T + h) - e_of(T - h)) / (2 * h)
+        cv_T = sum(cv[l + 2] * T ** l for l in range(-2, 5))
+        assert abs(dedT - cv_T) < 1e-7 * cv_T, (T, dedT, cv_T)
+        for start in (0.5 * T, 2.0 * T):
+            assert abs(lib().or_get_temperature_from_e(o.h, e_of(T), start) - T) < 1e-12 * T
+
+
 def test_eval_aux_closed_form():
     p, g, _, co = box_case(nx=8, ny=6, nz=5)
     o = Oracle(p, g)
