@@ -47,6 +47,7 @@ HBM_PEAK_GBS = 8000.0
 ALGO_BYTES = {
     "aux": (96.0, "point"),      # read w(5) 40 B + write w_aux(1:7) 56 B
     "euler_x": (96.0, "cell"),   # read w_aux(1:6,8) 56 B + write fl(5) 40 B
+    "euler_xy": (96.0, "cell"),  # x and y in one pass: read w_aux(1:6,8) 56 B + write fl(5) 40 B
     "visc": (120.0, "cell"),     # read w_aux(2,3,4,6,7) 40 B + fl(2:5) rw 64 B + write w_aux(9,10) 16 B
     "euler_y": (136.0, "cell"),  # read w_aux(1:6,8) 56 B + fl(5) rw 80 B
     "euler_z": (136.0, "cell"),

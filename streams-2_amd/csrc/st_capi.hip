@@ -198,7 +198,7 @@ hipError_t dalloc(T **p, size_t n) {
 void free_all(st_solver *s) {
     void *ptrs[] = {s->w, s->w_aux, s->F[0], s->F[1], s->wallprop, s->eoc[0], s->eoc[1], s->eoc[2], s->x, s->y,
                     s->z, s->yn, s->dcsidx, s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz,
-                    s->dzitdzs, s->dzitdz2, s->ds, s->partials, s->dirty, s->counters, s->rc_real, s->rc_maps,
+                    s->dzitdzs, s->dzitdz2, s->ds, s->partials, s->dirty, s->counters, s->fixlist, s->rc_real, s->rc_maps,
                     s->w_stat};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -250,13 +250,29 @@ int stage(st_solver *s, int istep) {
     TIMED(ST_K_AUX, launch_aux(s, 1 - g.ng, g.nx + g.ng, 1 - g.ng, g.ny + g.ng, 1 - g.ng, g.nz + g.ng, true,
                                istep == 0 && s->dt_in_stage));
     if (istep == 0) s->dt_in_stage = false;
-    TIMED(ST_K_EULER_X, launch_euler_x(s, fcur, true));
+    // euler_x + euler_y in one pass where the y range spans all rows; each
+    // sweep's WENO faces are then fixed up (timed as their own family)
+    const bool xy = s->fuse_xy && s->kp.eul_lo[1] == 1 && s->kp.eul_hi[1] == g.ny;
+    if (xy) {
+        TIMED(ST_K_EULER_XY, launch_euler_xy(s, fcur, false));
+        TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 0, 1, 0));
+        TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 1, 2, 1));
+    } else {
+        TIMED(ST_K_EULER_X, launch_euler_x(s, fcur, true, false));
+        TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 0, 1, 0));
+    }
     const bool cons = s->p.conservative_viscous == 1;
     // visflx(4), or visflx(6) + visflx(5); + bcextr_var / bcswap_var of w_aux(10)
     if (cons) TIMED(ST_K_VISC, launch_visc_cons(s, fcur, 56, true));
     else TIMED(ST_K_VISC, launch_visc(s, fcur, 4, true));
-    TIMED(ST_K_EULER_Y, launch_euler_y(s, fcur));
-    if (s->kp.ndim == 3) TIMED(ST_K_EULER_Z, launch_euler_z(s, fcur));
+    if (!xy) {
+        TIMED(ST_K_EULER_Y, launch_euler_y(s, fcur, false));
+        TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 1, 1, 0));
+    }
+    if (s->kp.ndim == 3) {
+        TIMED(ST_K_EULER_Z, launch_euler_z(s, fcur, false));
+        TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 2, 1, 0));
+    }
     // visflx(3) + bcswap_var(w_aux 8); the mixed Laplacian form refines the
     // sensor near the threshold in FP64 so its switch decisions are the FP64 form's
     if (istep == 2) {
@@ -298,7 +314,7 @@ int64_t st_solver_footprint(const st_params *p) {
     const int64_t na = p->precision == ST_PREC_MIXED ? 11 : 10;
     const int64_t part = 7 * blocks > (nvol + 255) / 256 ? 7 * blocks : (nvol + 255) / 256;
     return 8 * (5 * nvol + 3 * sx * sz + part + 1024) + rs * (na * nvol + 10 * nint) +
-           (nint + 4 * (p->nx + p->ny + p->nz + 3 + 4)) +
+           (5 * nint + 4 * (p->nx + p->ny + p->nz + 3 + 8 + 4096)) +
            8 * (2 * sx + 2 * sy + 2 * sz + 3 * (p->nx + p->ny + p->nz) + p->ny + 1) + 4096;
 }
 
@@ -311,6 +327,7 @@ int st_solver_create(const st_params *p, st_solver **out) {
     if (!s) return fail(ST_ERR_NOMEM, "host allocation failed");
     std::memset((void *)s, 0, sizeof(st_solver));
     s->p = *p;
+    s->fuse_xy = true;
     st::Geo &g = s->g;
     g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.ng = p->ng;
     g.sx = p->nx + 2 * p->ng; g.sy = p->ny + 2 * p->ng; g.sz = p->nz + 2 * p->ng;
@@ -362,7 +379,8 @@ int st_solver_create(const st_params *p, st_solver **out) {
     es[ne++] = dalloc(&s->partials, (size_t)s->npart);
     es[ne++] = dalloc(&s->dirty, nint + 64);  // per-cell flags + the any-dirty word
     if (es[ne - 1] == hipSuccess) es[ne - 1] = hipMemset(s->dirty, 0, nint + 64);
-    es[ne++] = dalloc(&s->counters, 4);
+    es[ne++] = dalloc(&s->counters, 8 + 4096);  // 4096 = kFixMaxSeg
+    es[ne++] = dalloc(&s->fixlist, nint);
     es[ne++] = hipHostMalloc((void **)&s->hs, sizeof(st::DevScalars), hipHostMallocDefault);
     for (int q = 0; q < ne; q++) {
         if (es[q] != hipSuccess) {
@@ -388,7 +406,7 @@ int st_solver_create(const st_params *p, st_solver **out) {
     if (ec == hipSuccess) ec = hipMemset(s->F[0], 0, rsize(s) * 5 * nint);
     if (ec == hipSuccess) ec = hipMemset(s->F[1], 0, rsize(s) * 5 * nint);
     if (ec == hipSuccess) ec = hipMemset(s->wallprop, 0, sizeof(double) * 3 * g.sx * g.sz);
-    if (ec == hipSuccess) ec = hipMemset(s->counters, 0, sizeof(int) * 4);
+    if (ec == hipSuccess) ec = hipMemset(s->counters, 0, sizeof(int) * (8 + 4096));
     std::memset(s->hs, 0, sizeof(st::DevScalars));
     if (p->cfl < 0) s->hs->dt = -p->cfl;
     if (ec == hipSuccess) ec = hipMemcpy(s->ds, s->hs, sizeof(st::DevScalars), hipMemcpyHostToDevice);
@@ -719,7 +737,7 @@ int st_kernel_times(st_solver *s, int32_t nsteps, double ms[ST_NKFAM], int32_t c
     KTimer *t = new (std::nothrow) KTimer();
     if (!t) return fail(ST_ERR_NOMEM, "timer allocation failed");
     t->n = 0;
-    const int need = nsteps * 40 < KTimer::kMax ? nsteps * 40 : KTimer::kMax;
+    const int need = nsteps * 64 < KTimer::kMax ? nsteps * 64 : KTimer::kMax;  // up to ~20 scopes a stage
     int made = 0;
     hipError_t e = hipSuccess;
     for (; made < need && e == hipSuccess; made++) {

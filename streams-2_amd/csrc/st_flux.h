@@ -408,7 +408,10 @@ __device__ __forceinline__ void weno_flux(const KPar &P, const Pt (&q)[6], int e
     const double r2u2 = P.rho0 * P.rho0 * u0_2;
     const double r2u4 = r2u2 * u0_2;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
+    // one characteristic field per iteration, not unrolled: the unrolled
+    // form of the five fields (twice per fixed-up cell) outgrows the
+    // instruction cache and the register file
+#pragma unroll 1
     for (int m = 0; m < 5; m++) {
         double el[5];
         eig_left<DIR>(s, m, el);
@@ -474,7 +477,7 @@ __device__ __forceinline__ void weno_flux_rusanov(const KPar &P, const Pt (&q)[6
     const double u0_2 = P.u0 * P.u0;
     const double r2u2 = P.rho0 * P.rho0 * u0_2;
     const double r2u4 = r2u2 * u0_2;
-#pragma unroll
+#pragma unroll 1
     for (int m = 0; m < 5; m++) {
         double gp[6], gm[6];
 #pragma unroll

@@ -552,7 +552,8 @@ __global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__rest
 // Faces whose sensor window exceeds the threshold take the characteristic
 // WENO flux.  The main sweep kernels evaluate only central faces, so their
 // register budget stays that of the KEEP split; a cell with a WENO face on
-// either side gets its flag byte set and is handled by k_weno_fix, which
+// either side gets its flag byte set and is handled by the fix-up (k_fix_compact
+// lists the flagged cells, k_fix_list processes them), which
 // recomputes both faces of the cell with the full hybrid logic, applies the
 // same divergence as the reference and clears the flag.
 // flags[nint] doubles as the any-dirty word of the sweep: one plain store per
@@ -563,7 +564,12 @@ __device__ __forceinline__ void mark_dirty(unsigned char *__restrict__ flags, in
     if (mask != 0ull && __lane_id() == __ffsll((long long)mask) - 1) flags[nint] = 1;
 }
 
-__global__ void k_clear_any(unsigned char *__restrict__ flags, int nint) { flags[nint] = 0; }
+// clears both any-dirty bytes: nint (x, y and z sweeps) and nint + 1 (the y
+// term of the fused x + y pass)
+__global__ void k_clear_any(unsigned char *__restrict__ flags, int nint) {
+    flags[nint] = 0;
+    flags[nint + 1] = 0;
+}
 
 // ------------------------------------------------------------ Euler y / z
 // Streaming form of the central split along DIR.  One thread per (i, row)
@@ -574,7 +580,7 @@ __global__ void k_clear_any(unsigned char *__restrict__ flags, int nint) { flags
 // each pair is evaluated once instead of once per face).  Face b-3 is then
 // complete; its divergence goes to cell b-3 (euler_y/z_update_subroutine,
 // :991-1010, 1039-1058).  Faces whose sensor window exceeds the threshold
-// are left to k_weno_fix through the dirty-cell list.
+// are left to the fix-up through the dirty-cell flags.
 template <typename R>
 struct Acc {
     R m, mu, mv, mw, e, pp;
@@ -916,6 +922,200 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const R *__restr
     }
 }
 
+// ------------------------------------------------------------- Euler x + y
+// The x sweep and the y march in one pass.  A wave holds 64 consecutive
+// points along x (the x sweep's segment: 58 cells and 3 halo points on each
+// side) and marches them along y over a chunk of rows, with the register
+// march of k_euler_march<1> in every lane.  When the march completes row r
+// (the oldest point of each lane's window), the lanes hold row r across the
+// segment, so the x faces of row r are formed right there with the
+// shuffles of k_euler_x.  Cell (i, r) then receives x-div + y-div in one
+// store of fl (INIT: the stage's right-hand side starts here); a direction
+// whose faces need WENO leaves its term to the fix-up through bit 0 (x) or
+// bit 1 (y) of the cell's flag byte and the any-dirty bytes nint / nint+1.
+// Both terms and their order of addition are those of the separate sweeps.
+// Requires the y range of the Euler fluxes to span all interior rows.
+template <typename R, int KM>
+__global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__restrict__ wa,
+                                                  const int *__restrict__ eocx, const int *__restrict__ eocy,
+                                                  const double *__restrict__ dcsidx, const double *__restrict__ detady,
+                                                  R *__restrict__ fl, int chunk, int nchunk,
+                                                  unsigned char *__restrict__ flags) {
+    const int lane = threadIdx.x & 63;
+    const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nseg = (g.nx + XW - 1) / XW;
+    if (gw >= nseg * g.nz * nchunk) return;  // whole waves; no workgroup barrier below
+    const int seg = gw % nseg;
+    const int rr = gw / nseg;
+    const int k = rr % g.nz + 1;
+    const int cidx = rr / g.nz;
+    const int i0 = 1 + XW * seg;
+    const int pt = i0 - 3 + lane;
+    const int pl = pt > g.nx + 3 ? g.nx + 3 : pt;
+    const int lox = P.eul_lo[0], hix = P.eul_hi[0];
+    const int hiy = P.eul_hi[1];
+    const int c0 = P.eul_lo[1] + cidx * chunk;
+    if (c0 > hiy) return;
+    const int c1 = min(c0 + chunk - 1, hiy);
+    const int n = g.ny;
+    const int stride = g.sx;
+    const size_t nv = g.nvol;
+    const bool keep = KM >= 0;
+    const R thr = R(P.sensor_threshold);
+    const R rgas0 = R(P.rgas0), pref = R(P.p_ref);
+    constexpr int TS = tp_slot<R>();
+    const int ws = P.weno_scheme;
+    const unsigned smask = ((1u << (2 * ws)) - 1u) << (3 - ws);
+    const unsigned long long wmask = (1ull << (2 * ws)) - 1ull;
+    const int lb = P.lmax_base;
+    auto lmy_of = [&](int f) {
+        const int ff = f < 0 ? 0 : (f > n ? n : f);
+        const int v = lb + eocy[ff];
+        return v < 1 ? 1 : v;
+    };
+    auto lmx_of = [&](int f) {
+        const int ff = f < 0 ? 0 : (f > g.nx ? g.nx : f);
+        const int v = lb + eocx[ff];
+        return v < 1 ? 1 : v;
+    };
+    // x-face coefficients of this lane's faces (the same for every row)
+    const int lmA = lmx_of(pt - 1), lmB = lmx_of(pt - 2), lmC = lmx_of(pt - 3);
+    const bool cell = lane >= 3 && lane <= 60 && pt <= g.nx;
+    const bool inx = cell && pt >= lox && pt <= hix;
+    const double dxc = dcsidx[(pt < 1 ? 1 : (pt > g.nx ? g.nx : pt)) - 1];
+    const int p0 = gidx(g, pl, 0, k);  // offset of row index 0 of this lane's column
+    PcR<R> q0{}, q1{}, q2{};
+    unsigned hist = 0;
+    Acc<R> A0 = acc_zero<R>(), A1 = acc_zero<R>(), A2 = acc_zero<R>();  // y faces b-3, b-2, b-1
+    int lm0 = lmy_of(c0 - 6), lm1 = lmy_of(c0 - 5), lm2 = lmy_of(c0 - 4);
+    R fprev[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
+    bool wprev = false;
+    unsigned char anyx = 0, anyy = 0;
+    int a = p0 + (c0 - 3) * stride;
+    R nr = wa[a], nu = wa[nv + a], nvv = wa[2 * nv + a], nw = wa[3 * nv + a], nh = wa[4 * nv + a],
+      nt = wa[TS * nv + a], ns = wa[7 * nv + a];
+    for (int b = c0 - 3; b <= c1 + 3; b++) {
+        const PcR<R> qb = make_pcr<R>(nr, nu, nvv, nw, nh, nt, rgas0, pref, keep);
+        hist = (hist << 1) | (ns > thr ? 1u : 0u);
+        if (b < c1 + 3) {
+            a += stride;
+            nr = wa[a]; nu = wa[nv + a]; nvv = wa[2 * nv + a]; nw = wa[3 * nv + a];
+            nh = wa[4 * nv + a]; nt = wa[TS * nv + a]; ns = wa[7 * nv + a];
+        }
+        // y pairs ending at point b (k_euler_march)
+        {
+            const Acc<R> p3 = pair_terms<R, 1, KM>(P, q0, qb, pref);
+            acc_add(A0, face_coeff<R>(P, lm0, 3), p3);
+            acc_add(A1, face_coeff<R>(P, lm1, 3), p3);
+            acc_add(A2, face_coeff<R>(P, lm2, 3), p3);
+        }
+        {
+            const Acc<R> p2 = pair_terms<R, 1, KM>(P, q1, qb, pref);
+            acc_add(A1, face_coeff<R>(P, lm1, 2), p2);
+            acc_add(A2, face_coeff<R>(P, lm2, 2), p2);
+        }
+        {
+            const Acc<R> p1 = pair_terms<R, 1, KM>(P, q2, qb, pref);
+            acc_add(A2, face_coeff<R>(P, lm2, 1), p1);
+        }
+        const int f = b - 3;  // y face f complete; row f = the window's oldest point q0
+        if (f >= c0 - 1) {
+            const bool wcur = (hist & smask) != 0u;
+            R fh[5];
+            fh[0] = R(0.25) * A0.m;
+            fh[1] = R(0.25) * A0.mu;
+            fh[2] = R(0.25) * A0.mv;
+            fh[3] = R(0.25) * A0.mw;
+            fh[4] = R(0.25) * A0.e;
+            if ((f == 0 && P.fzf[2] == 1) || (f == n && P.fzf[3] == 1)) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) fh[m] = R(0.0);
+            }
+            fh[2] = fh[2] + R(0.5) * A0.pp;
+            if (f >= c0) {
+                // x faces of row f (k_euler_x), from the lanes' points of that row
+                // pairs (pt-l, pt), l = 3, 2, 1; each face sum is shuffled down
+                // as soon as it is complete, to keep few values live
+                Acc<R> Q1, Q2, T2, T3;
+                {
+                    const Acc<R> p3 = pair_terms<R, 0, KM>(P, shfl_up_pc(q0, 3), q0, pref);
+                    Q1 = acc_zero<R>();
+                    Q2 = acc_zero<R>();
+                    Acc<R> Q3 = acc_zero<R>();
+                    acc_add(Q1, face_coeff<R>(P, lmA, 3), p3);
+                    acc_add(Q2, face_coeff<R>(P, lmB, 3), p3);
+                    acc_add(Q3, face_coeff<R>(P, lmC, 3), p3);
+                    T3 = shfl_down_acc(Q3, 3);
+                }
+                {
+                    const Acc<R> p2 = pair_terms<R, 0, KM>(P, shfl_up_pc(q0, 2), q0, pref);
+                    acc_add(Q1, face_coeff<R>(P, lmA, 2), p2);
+                    acc_add(Q2, face_coeff<R>(P, lmB, 2), p2);
+                    T2 = shfl_down_acc(Q2, 2);
+                }
+                {
+                    const Acc<R> p1 = pair_terms<R, 0, KM>(P, shfl_up_pc(q0, 1), q0, pref);
+                    acc_add(Q1, face_coeff<R>(P, lmA, 1), p1);
+                }
+                Acc<R> F = shfl_down_acc(Q1, 1);
+                F.m += T2.m; F.mu += T2.mu; F.mv += T2.mv; F.mw += T2.mw; F.e += T2.e; F.pp += T2.pp;
+                F.m += T3.m; F.mu += T3.mu; F.mv += T3.mv; F.mw += T3.mw; F.e += T3.e; F.pp += T3.pp;
+                R gx[5];
+                gx[0] = R(0.25) * F.m;
+                gx[1] = R(0.25) * F.mu;
+                gx[2] = R(0.25) * F.mv;
+                gx[3] = R(0.25) * F.mw;
+                gx[4] = R(0.25) * F.e;
+                if ((pt == 0 && P.fzf[0] == 1) || (pt == g.nx && P.fzf[1] == 1)) {
+#pragma unroll
+                    for (int m = 0; m < 5; m++) gx[m] = R(0.0);
+                }
+                gx[1] = gx[1] + R(0.5) * F.pp;
+                // x WENO face: a sensor flag among points pt-ws+1 .. pt+ws of row f
+                const unsigned long long fm = __ballot(((hist >> 3) & 1u) != 0u);
+                const int sh = lane - ws + 1;
+                const bool wf = lane >= 2 && lane <= 60 && ((fm >> (sh < 0 ? 0 : sh)) & wmask) != 0ull;
+                R gl[5];
+#pragma unroll
+                for (int m = 0; m < 5; m++) gl[m] = __shfl_up(gx[m], 1, 64);
+                const bool wl = __shfl_up(wf ? 1 : 0, 1, 64) != 0;
+                const bool xdirty = inx && (wf || wl);
+                const bool ydirty = cell && (wcur || wprev);
+                const unsigned long long mx = __ballot(xdirty), my = __ballot(ydirty);
+                if (mx != 0ull) anyx = 1;
+                if (my != 0ull) anyy = 1;
+                if (cell) {
+                    const int oi = iidx(g, pt, f, k);
+                    const R d = R(detady[f - 1]);
+                    const R dx = R(dxc);
+#pragma unroll
+                    for (int m = 0; m < 5; m++) {
+                        const R xv = (inx && !xdirty) ? (gx[m] - gl[m]) * dx : R(0.0);
+                        fl[(size_t)m * g.nint + oi] = ydirty ? xv : xv + (fh[m] - fprev[m]) * d;
+                    }
+                    if (xdirty || ydirty) flags[oi] = (unsigned char)((xdirty ? 1 : 0) | (ydirty ? 2 : 0));
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 5; m++) fprev[m] = fh[m];
+            wprev = wcur;
+        }
+        A0 = A1;
+        A1 = A2;
+        A2 = acc_zero<R>();
+        lm0 = lm1;
+        lm1 = lm2;
+        lm2 = lmy_of(b);
+        q0 = q1;
+        q1 = q2;
+        q2 = qb;
+    }
+    if (lane == 0) {
+        if (anyx) flags[g.nint] = 1;
+        if (anyy) flags[g.nint + 1] = 1;
+    }
+}
+
 // ------------------------------------------------------------ WENO fix-up
 // Hybrid flux of the face between window points 2 and 3 of q (offsets
 // -2..+3): central when no sensor value of the window exceeds the threshold,
@@ -961,18 +1161,21 @@ __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const R *_
     const int k = oi / (g.nx * g.ny) + 1;
     const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
     const int base = gidx(g, i, j, k);
-    Pt q[7];
-    double sen[7];
+    // Every load of the stencil is issued before any arithmetic: the seven
+    // points of a y or z stencil lie in seven different pages, and waiting
+    // on them one after another would serialize seven memory latencies.
+    double raw[7][6], sen[7];
 #pragma unroll
     for (int s = 0; s < 7; s++) {
         const int a = base + (s - 3) * stride;
         if constexpr (sizeof(R) == sizeof(double)) {
-            q[s] = make_pt(double(wa[a]), double(wa[nv + a]), double(wa[2 * nv + a]), double(wa[3 * nv + a]),
-                           double(wa[4 * nv + a]), double(wa[5 * nv + a]), P.rgas0, keep);
+#pragma unroll
+            for (int v = 0; v < 6; v++) raw[s][v] = double(wa[v * nv + a]);
         } else {
             // the mixed form stores w_aux in FP32: the WENO stencil takes the
             // FP64 primitives of w instead, formed as eval_aux forms them
-            // (periodic neighbours from their images)
+            // (periodic neighbours from their images); raw[s][5] is the
+            // stored temperature, the start of the cp(T) inversion
             int pi = i, pj = j, pk = k;
             if (DIR == 0) pi += s - 3;
             else if (DIR == 1) pj += s - 3;
@@ -981,90 +1184,191 @@ __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const R *_
             if (P.periodic[1]) pj = pj < 1 ? pj + g.ny : (pj > g.ny ? pj - g.ny : pj);
             if (P.periodic[2]) pk = pk < 1 ? pk + g.nz : (pk > g.nz ? pk - g.nz : pk);
             const int src = gidx(g, pi, pj, pk);
-            const double rho = w[src], rhou = w[nv + src], rhov = w[2 * nv + src], rhow = w[3 * nv + src],
-                         rhoe = w[4 * nv + src];
-            const double ri = 1.0 / rho;
-            const double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
-            const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
-            const double ee = rhoe / rho - qq;
-            const double tt = temperature_from_e(P, ee, double(wa[5 * nv + a]));
-            const double pp = rho * tt * P.rgas0;
-            q[s] = make_pt(rho, uu, vv, ww, (rhoe + pp) / rho, tt, P.rgas0, keep);
+#pragma unroll
+            for (int v = 0; v < 5; v++) raw[s][v] = w[v * nv + src];
+            raw[s][5] = double(wa[5 * nv + a]);
         }
         sen[s] = double(wa[7 * nv + a]);
     }
-    double flo[5], fhi[5];
-    {
-        Pt ql[6];
-        double sl[6];
-#pragma unroll
-        for (int s = 0; s < 6; s++) { ql[s] = q[s]; sl[s] = sen[s]; }
-        hybrid_face<DIR>(P, ql, sl, c - 1, eoc[c - 1], n, flo);
-    }
-    {
-        Pt qh[6];
-        double sh[6];
-#pragma unroll
-        for (int s = 0; s < 6; s++) { qh[s] = q[s + 1]; sh[s] = sen[s + 1]; }
-        hybrid_face<DIR>(P, qh, sh, c, eoc[c], n, fhi);
-    }
+    const int elo = eoc[c - 1], ehi = eoc[c];
     const double d = metric[c - 1];
+    double flv[5];
 #pragma unroll
-    for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] = R(double(fl[(size_t)m * g.nint + oi]) + (fhi[m] - flo[m]) * d);
+    for (int m = 0; m < 5; m++) flv[m] = double(fl[(size_t)m * g.nint + oi]);
+    Pt q[7];
+#pragma unroll
+    for (int s = 0; s < 7; s++) {
+        if constexpr (sizeof(R) == sizeof(double)) {
+            q[s] = make_pt(raw[s][0], raw[s][1], raw[s][2], raw[s][3], raw[s][4], raw[s][5], P.rgas0, keep);
+        } else {
+            const double rho = raw[s][0], rhoe = raw[s][4];
+            const double ri = 1.0 / rho;
+            const double uu = raw[s][1] * ri, vv = raw[s][2] * ri, ww = raw[s][3] * ri;
+            const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+            const double ee = rhoe / rho - qq;
+            const double tt = temperature_from_e(P, ee, raw[s][5]);
+            const double pp = rho * tt * P.rgas0;
+            q[s] = make_pt(rho, uu, vv, ww, (rhoe + pp) / rho, tt, P.rgas0, keep);
+        }
+    }
+    // Face A is a WENO face of the cell where it has one (the lower one when
+    // its window is shocked, else the upper one), face B the other.  Cells
+    // of a wave mostly have one WENO face, at either end; taking that face
+    // first keeps the WENO branch of a wave on all its lanes at once instead
+    // of running it twice on about half of them.  Each face is evaluated
+    // exactly as before, so the results do not depend on the order.
+    bool shlo = false, shhi = false;
+#pragma unroll
+    for (int s = 0; s < 6; s++)
+        if (s >= H - P.weno_scheme && s < H + P.weno_scheme) {  // the windows of hybrid_face
+            shlo |= sen[s] > P.sensor_threshold;
+            shhi |= sen[s + 1] > P.sensor_threshold;
+        }
+    const bool alo = shlo || !shhi;  // face A is the lower face
+    double flo[5], fhi[5];
+    // one loop body for both faces (its code is large; two inlined copies
+    // would double the instruction footprint of the kernel)
+#pragma unroll 1
+    for (int t = 0; t < 2; t++) {
+        const bool lo = (t == 0) == alo;
+        Pt qw[6];
+        double sw[6];
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+            qw[s] = lo ? q[s] : q[s + 1];
+            sw[s] = lo ? sen[s] : sen[s + 1];
+        }
+        double fh[5];
+        hybrid_face<DIR>(P, qw, sw, lo ? c - 1 : c, lo ? elo : ehi, n, fh);
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            if (lo) flo[m] = fh[m];
+            else fhi[m] = fh[m];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] = R(flv[m] + (fhi[m] - flo[m]) * d);
+}
+
+// The WENO fix-up of one direction in two kernels.  k_fix_compact: block b
+// takes segment b of the flag array (wpb 64-bit words = 8 * wpb cells, each
+// thread a run of consecutive words), lists the cells whose flag byte holds
+// `bit` into list[b * 8 * wpb ...] in increasing order, stores their number
+// in counts[b] and clears the bit.  k_fix_list: every block forms the prefix
+// sums of counts in LDS and its waves take batches of 64 entries of the
+// concatenated lists in turn, so the work is spread over the chip however
+// the shocked cells are distributed.  No atomics: the lists and their order
+// are a function of the flags alone.
+constexpr int kFixMaxSeg = 4096;
+
+__device__ __forceinline__ unsigned dirty_bytes(unsigned long long wd, int bit, int cell0, int ncell) {
+    unsigned mine = 0;  // bit b: byte b of the word is a flagged cell
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+        if (((wd >> (8 * b)) & (unsigned long long)bit) != 0ull && cell0 + b < ncell) mine |= 1u << b;
+    return mine;
+}
+
+// exclusive prefix over the block's 256 threads of v; *total receives the sum
+__device__ __forceinline__ int block_excl_scan256(int v, int *wsum, int *total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int before = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (q < wv) before += wsum[q];
+        tot += wsum[q];
+    }
+    __syncthreads();
+    *total = tot;
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_fix_compact(unsigned char *__restrict__ flags, int ncell, int bit,
+                                                     int anyidx, int wpb, int *__restrict__ list,
+                                                     int *__restrict__ counts) {
+    __shared__ int wsum[4];
+    if (flags[ncell + anyidx] == 0) return;  // no dirty cell of this direction in this sweep
+    const int nword = (ncell + 7) >> 3;
+    const int wpt = wpb >> 8;                // words per thread
+    const int w0 = blockIdx.x * wpb + threadIdx.x * wpt;
+    const int w1 = min(w0 + wpt, nword);
+    unsigned long long *words = (unsigned long long *)flags;
+    int c = 0;
+    for (int wi = w0; wi < w1; wi++) c += __popc(dirty_bytes(words[wi], bit, 8 * wi, ncell));
+    int total;
+    int off = block_excl_scan256(c, wsum, &total);
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+    if (c == 0) return;
+    int *seg = list + (size_t)blockIdx.x * 8 * wpb;
+    for (int wi = w0; wi < w1; wi++) {
+        const unsigned long long wd = words[wi];
+        const unsigned mine = dirty_bytes(wd, bit, 8 * wi, ncell);
+        if (mine == 0u) continue;
+        // clear the bit in the listed bytes only: the last word may also
+        // hold the any-dirty bytes, which are not cells
+        unsigned long long clr = 0ull;
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+            if ((mine >> b) & 1u) {
+                seg[off++] = 8 * wi + b;
+                clr |= (unsigned long long)bit << (8 * b);
+            }
+        words[wi] = wd & ~clr;
+    }
 }
 
 template <typename R, int DIR>
-__global__ __launch_bounds__(64) void k_weno_fix(Geo g, KPar P, const R *__restrict__ wa,
-                                                 const double *__restrict__ w, const int *__restrict__ eoc,
-                                                 const double *__restrict__ metric, R *__restrict__ fl,
-                                                 unsigned char *__restrict__ flags,
-                                                 int region) {
-    __shared__ int queue[64 * 5];
-    const int lane = threadIdx.x;
-    int qn = 0;  // uniform across the (single-wave) workgroup
-    const int ncell = g.nint;
-    if (flags[ncell] == 0) return;  // no dirty cell in this sweep
-    const int r0 = blockIdx.x * region;
-    const int r1 = min(r0 + region, ncell);
-    // each lane reads 8 flags at once (one 64-bit word), 4 words in flight
-    const unsigned long long *words = (const unsigned long long *)flags;
-    for (int base = r0; base < r1; base += 4 * 512) {
-        unsigned long long wd[4];
+__global__ __launch_bounds__(256) void k_fix_list(Geo g, KPar P, const R *__restrict__ wa,
+                                                  const double *__restrict__ w, const int *__restrict__ eoc,
+                                                  const double *__restrict__ metric, R *__restrict__ fl,
+                                                  const unsigned char *__restrict__ flags, int anyidx,
+                                                  const int *__restrict__ list, const int *__restrict__ counts,
+                                                  int nseg, int seglen) {
+    __shared__ int pre[kFixMaxSeg + 1];
+    __shared__ int wsum[4];
+    if (flags[g.nint + anyidx] == 0) return;
+    // prefix sums of the segment counts: thread t sums segments 16t .. 16t+15
+    const int t = threadIdx.x;
+    int cs[16], sum = 0;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int idx0 = base + u * 512 + lane * 8;
-            wd[u] = idx0 < r1 ? words[idx0 >> 3] : 0ull;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int idx0 = base + u * 512 + lane * 8;
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const bool f = ((wd[u] >> (8 * b)) & 0xffull) != 0ull && idx0 + b < r1;
-                const unsigned long long mask = __ballot(f);
-                if (f) {
-                    queue[qn + __popcll(mask & ((1ull << lane) - 1ull))] = idx0 + b;
-                    flags[idx0 + b] = 0;
-                }
-                qn += __popcll(mask);
-                if (qn >= 64 * 4) {  // keep room for the next byte position
-                    __syncthreads();
-                    while (qn >= 64) {
-                        fix_cell<R, DIR>(g, P, wa, w, eoc, metric, fl, queue[qn - 64 + lane]);
-                        qn -= 64;
-                    }
-                    __syncthreads();
-                }
-            }
-        }
-        __syncthreads();
-        while (qn >= 64) {
-            fix_cell<R, DIR>(g, P, wa, w, eoc, metric, fl, queue[qn - 64 + lane]);
-            qn -= 64;
-        }
-        __syncthreads();
+    for (int q = 0; q < 16; q++) {
+        const int sgi = 16 * t + q;
+        cs[q] = sgi < nseg ? counts[sgi] : 0;
+        sum += cs[q];
     }
-    if (lane < qn) fix_cell<R, DIR>(g, P, wa, w, eoc, metric, fl, queue[lane]);
+    int total;
+    int run = block_excl_scan256(sum, wsum, &total);
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int sgi = 16 * t + q;
+        if (sgi < nseg) pre[sgi] = run;
+        run += cs[q];
+    }
+    if (t == 0) pre[nseg] = total;
+    __syncthreads();
+    const int lane = t & 63;
+    const int wave = (blockIdx.x * blockDim.x + t) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int base = wave * 64; base < total; base += nwaves * 64) {
+        const int e = base + lane;
+        if (e < total) {
+            int lo = 0, hi = nseg;  // pre[lo] <= e < pre[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (pre[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            fix_cell<R, DIR>(g, P, wa, w, eoc, metric, fl, list[(size_t)lo * seglen + (e - pre[lo])]);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- viscous
@@ -2218,37 +2522,45 @@ hipError_t launch_wall_ghosts(st_solver *s) {
 }
 
 template <typename R>
-static hipError_t launch_fix(st_solver *s, R *fl, int dir) {
+static hipError_t launch_fix(st_solver *s, R *fl, int dir, int bit = 1, int anyidx = 0) {
     const Geo &g = s->g;
-    // one 64-lane workgroup per contiguous region of flags; regions large
-    // enough that the LDS queue fills whole waves of WENO work
-    int nb = (g.nint + 16383) / 16384;
-    nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
-    int region = (g.nint + nb - 1) / nb;
-    region = (region + 2047) / 2048 * 2048;  // whole 64-bit flag words per lane
-    const dim3 grid((unsigned)nb), block(64);
+    // segments of 2048 words (16384 cells) or longer, at most kFixMaxSeg
+    const int nword = (g.nint + 7) / 8;
+    int nseg = (nword + 2047) / 2048;
+    if (nseg > kFixMaxSeg) nseg = kFixMaxSeg;
+    if (nseg < 1) nseg = 1;
+    int wpb = (nword + nseg - 1) / nseg;
+    wpb = (wpb + 255) / 256 * 256;
+    nseg = (nword + wpb - 1) / wpb;
+    int *counts = s->counters + 8;
+    hipLaunchKernelGGL(k_fix_compact, dim3((unsigned)nseg), dim3(256), 0, s->stream, s->dirty, g.nint, bit, anyidx,
+                       wpb, s->fixlist, counts);
+    // the fix-up holds one wave per SIMD (its register budget): one
+    // 4-wave block per CU
+    const int nb = 256;
+    const int seglen = 8 * wpb;
     const R *wa = (const R *)s->w_aux;
     if (dir == 0)
-        hipLaunchKernelGGL((k_weno_fix<R, 0>), grid, block, 0, s->stream, g, s->kp, wa, s->w, s->eoc[0], s->dcsidx, fl,
-                           s->dirty, region);
+        hipLaunchKernelGGL((k_fix_list<R, 0>), dim3(nb), dim3(256), 0, s->stream, g, s->kp, wa, s->w, s->eoc[0],
+                           s->dcsidx, fl, s->dirty, anyidx, s->fixlist, counts, nseg, seglen);
     else if (dir == 1)
-        hipLaunchKernelGGL((k_weno_fix<R, 1>), grid, block, 0, s->stream, g, s->kp, wa, s->w, s->eoc[1], s->detady, fl,
-                           s->dirty, region);
+        hipLaunchKernelGGL((k_fix_list<R, 1>), dim3(nb), dim3(256), 0, s->stream, g, s->kp, wa, s->w, s->eoc[1],
+                           s->detady, fl, s->dirty, anyidx, s->fixlist, counts, nseg, seglen);
     else
-        hipLaunchKernelGGL((k_weno_fix<R, 2>), grid, block, 0, s->stream, g, s->kp, wa, s->w, s->eoc[2], s->dzitdz, fl,
-                           s->dirty, region);
+        hipLaunchKernelGGL((k_fix_list<R, 2>), dim3(nb), dim3(256), 0, s->stream, g, s->kp, wa, s->w, s->eoc[2],
+                           s->dzitdz, fl, s->dirty, anyidx, s->fixlist, counts, nseg, seglen);
     return hipGetLastError();
 }
 
 template <typename R>
-static hipError_t euler_x_t(st_solver *s, R *fl, bool init) {
+static hipError_t euler_x_t(st_solver *s, R *fl, bool init, bool fix) {
     const Geo &g = s->g;
+    hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
     if (s->kp.eul_hi[0] < s->kp.eul_lo[0]) {
         if (init) hipLaunchKernelGGL(k_zero<R>, dim3(grid1d((size_t)5 * g.nint, 256)), dim3(256), 0, s->stream, fl,
                                      (size_t)5 * g.nint);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
     const long nwork = (long)((g.nx + XW - 1) / XW) * g.ny * g.nz;
     long nblk = (nwork + 3) / 4;
     if (nblk > 256L * 4) nblk = 256L * 4;  // resident workgroups at 4 waves per SIMD
@@ -2264,17 +2576,20 @@ static hipError_t euler_x_t(st_solver *s, R *fl, bool init) {
     }
 #undef ST_LAUNCH_X
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || !fix) return e;
     return launch_fix(s, fl, 0);
 }
 
-hipError_t launch_euler_x(st_solver *s, void *fl, bool init) { ST_DISPATCH(return euler_x_t(s, (R *)fl, init)); }
+hipError_t launch_euler_x(st_solver *s, void *fl, bool init, bool fix) {
+    ST_DISPATCH(return euler_x_t(s, (R *)fl, init, fix));
+}
 
 template <typename R>
-static hipError_t launch_march(st_solver *s, R *fl, int dir) {
+static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix) {
     const Geo &g = s->g;
     const int lo = s->kp.eul_lo[dir], hi = s->kp.eul_hi[dir];
-    if (hi < lo) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
+    if (hi < lo) return hipGetLastError();
     const int ncell = hi - lo + 1;
     const int no = dir == 1 ? g.nz : g.ny;
     // chunks of ~32 cells (3 warm-up points each) so that the grid holds
@@ -2285,7 +2600,6 @@ static hipError_t launch_march(st_solver *s, R *fl, int dir) {
     if (s->march_chunk > 0) chunk = s->march_chunk < ncell ? s->march_chunk : ncell;
     const int nchunk = (ncell + chunk - 1) / chunk;
     const size_t tot = (size_t)g.nx * no * nchunk;
-    hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
     dim3 grid((unsigned)((tot + 255) / 256));
     const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
 #define ST_LAUNCH_M(D, K, E, M)                                                                                     \
@@ -2302,12 +2616,50 @@ static hipError_t launch_march(st_solver *s, R *fl, int dir) {
     }
 #undef ST_LAUNCH_M
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || !fix) return e;
     return launch_fix(s, fl, dir);
 }
 
-hipError_t launch_euler_y(st_solver *s, void *fl) { ST_DISPATCH(return launch_march(s, (R *)fl, 1)); }
-hipError_t launch_euler_z(st_solver *s, void *fl) { ST_DISPATCH(return launch_march(s, (R *)fl, 2)); }
+// The fused x + y pass (INIT: the stage's right-hand side starts here).
+// Only for a y range of the Euler fluxes that spans all rows; returns
+// hipErrorNotSupported otherwise so the caller runs the two sweeps.
+template <typename R>
+static hipError_t launch_xy(st_solver *s, R *fl, bool fix) {
+    const Geo &g = s->g;
+    if (s->kp.eul_lo[1] != 1 || s->kp.eul_hi[1] != g.ny) return hipErrorNotSupported;
+    const int ncell = g.ny;
+    const long nseg = (g.nx + XW - 1) / XW;
+    int chunk = ncell < 32 ? ncell : 32;
+    const long minwaves = 256L * 16 * 4;
+    while (chunk > 8 && nseg * g.nz * ((ncell + chunk - 1) / chunk) < minwaves) chunk /= 2;
+    if (s->march_chunk > 0) chunk = s->march_chunk < ncell ? s->march_chunk : ncell;
+    const int nchunk = (ncell + chunk - 1) / chunk;
+    const long nw = nseg * g.nz * nchunk;
+    hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
+    const dim3 grid((unsigned)((nw + 3) / 4));
+    const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
+#define ST_LAUNCH_XY(K)                                                                                          \
+    hipLaunchKernelGGL((k_euler_xy<R, K>), grid, dim3(256), 0, s->stream, g, s->kp, (const R *)s->w_aux, s->eoc[0], \
+                       s->eoc[1], s->dcsidx, s->detady, fl, chunk, nchunk, s->dirty)
+    if (km == 0) ST_LAUNCH_XY(0); else if (km > 0) ST_LAUNCH_XY(1); else ST_LAUNCH_XY(-1);
+#undef ST_LAUNCH_XY
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !fix) return e;
+    e = launch_fix(s, fl, 0, 1, 0);
+    if (e != hipSuccess) return e;
+    return launch_fix(s, fl, 1, 2, 1);
+}
+
+hipError_t launch_euler_xy(st_solver *s, void *fl, bool fix) { ST_DISPATCH(return launch_xy(s, (R *)fl, fix)); }
+hipError_t launch_euler_y(st_solver *s, void *fl, bool fix) { ST_DISPATCH(return launch_march(s, (R *)fl, 1, fix)); }
+hipError_t launch_euler_z(st_solver *s, void *fl, bool fix) { ST_DISPATCH(return launch_march(s, (R *)fl, 2, fix)); }
+
+// The fix-up of the last sweep of direction dir, for callers that time it
+// apart from the sweep: bit 1 / any-dirty byte 0 after a single-direction
+// sweep; the fused x + y pass leaves x under (1, 0) and y under (2, 1).
+hipError_t launch_weno_fix(st_solver *s, void *fl, int dir, int bit, int anyidx) {
+    ST_DISPATCH(return launch_fix(s, (R *)fl, dir, bit, anyidx));
+}
 
 hipError_t launch_visc(st_solver *s, void *fl, int mode, bool ghost_images) {
     const Geo &g = s->g;

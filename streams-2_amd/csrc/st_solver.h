@@ -27,6 +27,7 @@ struct st_solver {
     bool dt_in_stage;   // the CFL reduction of this step rides on the first stage's aux
     int visc_chunk;     // planes per tile of the z-marching viscous kernel (0: automatic)
     int march_chunk;    // points per thread of the y/z flux marches (0: automatic)
+    bool fuse_xy;       // x and y Euler fluxes in one pass when the y range spans all rows
     double *wallprop;   // 3 x (sx*sz), wallprop(:,:,2:4)
     int *eoc[3];        // ep_ord_change per direction, faces 0..n
     // grid
@@ -38,7 +39,8 @@ struct st_solver {
     double *partials;
     int npart;           // capacity in doubles
     unsigned char *dirty;  // per-cell flag: a WENO face on either side (one sweep at a time)
-    int *counters;         // [1] reduction ticket
+    int *counters;         // [1] reduction ticket, [8 ..] segment counts of the WENO fix-up's lists
+    int *fixlist;          // cells of one direction's WENO fix-up, in segments (nint entries)
     int cur;             // F[cur] is the right-hand side being built
     double rhork[3], gamrk[3];
     int last_stage;      // RK stage (0..2) of the last completed stage, -1 none
@@ -73,9 +75,12 @@ hipError_t launch_physical_ghosts(st_solver *s);
 hipError_t launch_bc_nr(st_solver *s, void *fl);
 // recyc_exchange + bcrecyc of face xmin
 hipError_t launch_recycling(st_solver *s);
-hipError_t launch_euler_x(st_solver *s, void *fl, bool init);
-hipError_t launch_euler_y(st_solver *s, void *fl);
-hipError_t launch_euler_z(st_solver *s, void *fl);
+// fix = false leaves the WENO fix-up of the sweep to launch_weno_fix
+hipError_t launch_euler_x(st_solver *s, void *fl, bool init, bool fix = true);
+hipError_t launch_euler_y(st_solver *s, void *fl, bool fix = true);
+hipError_t launch_euler_xy(st_solver *s, void *fl, bool fix = true);
+hipError_t launch_euler_z(st_solver *s, void *fl, bool fix = true);
+hipError_t launch_weno_fix(st_solver *s, void *fl, int dir, int bit, int anyidx);
 hipError_t launch_visc(st_solver *s, void *fl, int mode, bool ghost_images);
 hipError_t launch_visc_cons(st_solver *s, void *fl, int mode, bool ghost_images);
 hipError_t launch_sensor(st_solver *s, bool ghost_images);

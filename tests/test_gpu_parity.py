@@ -213,6 +213,20 @@ def test_results_independent_of_launch_shape(visc_chunk, march_chunk):
     assert np.array_equal(interior(out[0]), interior(out[1]))
 
 
+def test_trajectory_ragged_cell_count():
+    """29 x 21 x 11 cells (6699, not a multiple of 8): the last 64-bit word of
+    the per-cell WENO flags also holds the sweeps' any-dirty bytes, which the
+    fix-up's compaction must neither list nor clear.  Threshold 0.2 keeps
+    WENO faces in every direction over the 10 steps."""
+    c, o, g = _run_pair(10, nx=29, ny=21, nz=11, threshold=0.2, seed=2)
+    assert min(o.count_weno()) > 0.005
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    assert max(errs) <= TRAJ_TOL, errs
+    assert np.allclose(g.count_weno(), o.count_weno(), rtol=0, atol=0)
+    g.close()
+
+
 def test_residual_and_weno_count():
     c, o, g = _run_pair(5, threshold=0.0)
     o.compute_residual()
