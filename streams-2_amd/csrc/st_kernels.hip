@@ -785,27 +785,61 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__r
 // pair sums regrouped by pair).  Faces 59 per wave, cells 58.
 constexpr int XW = 58;
 
+// Lane moves of the x faces by DPP wave shifts (wave_shr:1 and wave_shl:1
+// of the GFX9 family): lane i receives the value of lane i-1 (i+1); a lane
+// whose source lies outside the wave keeps its own value.  A move by d
+// chains d shifts, so lanes below d (above 63-d) hold values the x faces
+// never read.  These are VALU moves: no LDS round trip on the face chain.
+constexpr int kDppUp = 0x138;    // wave_shr:1
+constexpr int kDppDown = 0x130;  // wave_shl:1
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = dpp_move<CTRL>(__double2loint(v));
+    const int hi = dpp_move<CTRL>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
+}
+
+template <typename R>
+__device__ __forceinline__ R lane_up(R v, int d) {
+    for (int s = 0; s < d; s++) v = dpp_move<kDppUp>(v);
+    return v;
+}
+template <typename R>
+__device__ __forceinline__ R lane_down(R v, int d) {
+    for (int s = 0; s < d; s++) v = dpp_move<kDppDown>(v);
+    return v;
+}
+
 template <typename R>
 __device__ __forceinline__ PcR<R> shfl_up_pc(const PcR<R> &q, int d) {
     PcR<R> r;
-    r.r = __shfl_up(q.r, d, 64);
-    r.u = __shfl_up(q.u, d, 64);
-    r.v = __shfl_up(q.v, d, 64);
-    r.w = __shfl_up(q.w, d, 64);
-    r.p = __shfl_up(q.p, d, 64);
-    r.e = __shfl_up(q.e, d, 64);
+    r.r = lane_up(q.r, d);
+    r.u = lane_up(q.u, d);
+    r.v = lane_up(q.v, d);
+    r.w = lane_up(q.w, d);
+    r.p = lane_up(q.p, d);
+    r.e = lane_up(q.e, d);
     return r;
 }
 
 template <typename R>
 __device__ __forceinline__ Acc<R> shfl_down_acc(const Acc<R> &q, int d) {
     Acc<R> r;
-    r.m = __shfl_down(q.m, d, 64);
-    r.mu = __shfl_down(q.mu, d, 64);
-    r.mv = __shfl_down(q.mv, d, 64);
-    r.mw = __shfl_down(q.mw, d, 64);
-    r.e = __shfl_down(q.e, d, 64);
-    r.pp = __shfl_down(q.pp, d, 64);
+    r.m = lane_down(q.m, d);
+    r.mu = lane_down(q.mu, d);
+    r.mv = lane_down(q.mv, d);
+    r.mw = lane_down(q.mw, d);
+    r.e = lane_down(q.e, d);
+    r.pp = lane_down(q.pp, d);
     return r;
 }
 
@@ -903,8 +937,8 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const R *__restr
         // cell c = pt, needs face c-1 from the lane below
         R fl_[5];
 #pragma unroll
-        for (int m = 0; m < 5; m++) fl_[m] = __shfl_up(fh[m], 1, 64);
-        const bool wl = __shfl_up(wf ? 1 : 0, 1, 64) != 0;
+        for (int m = 0; m < 5; m++) fl_[m] = lane_up(fh[m], 1);
+        const bool wl = dpp_move<kDppUp>(wf ? 1 : 0) != 0;
         const int c = pt;
         const bool cell = lane >= 3 && lane <= 60 && c <= g.nx;
         const bool in = cell && c >= lo && c <= hi;
@@ -1077,8 +1111,8 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
                 const bool wf = lane >= 2 && lane <= 60 && ((fm >> (sh < 0 ? 0 : sh)) & wmask) != 0ull;
                 R gl[5];
 #pragma unroll
-                for (int m = 0; m < 5; m++) gl[m] = __shfl_up(gx[m], 1, 64);
-                const bool wl = __shfl_up(wf ? 1 : 0, 1, 64) != 0;
+                for (int m = 0; m < 5; m++) gl[m] = lane_up(gx[m], 1);
+                const bool wl = dpp_move<kDppUp>(wf ? 1 : 0) != 0;
                 const bool xdirty = inx && (wf || wl);
                 const bool ydirty = cell && (wcur || wprev);
                 const unsigned long long mx = __ballot(xdirty), my = __ballot(ydirty);
