@@ -162,7 +162,7 @@ def serving(args):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=1000, help="timed RK3 steps (the GPU-scale channel runs 1000)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=256)
@@ -277,7 +277,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded laminar Poiseuille + 10% uniform velocity noise, seed 0)",
-        "config": {"workload": f"channel {nx}x{ny}x{nz}, M_b 1.5, Re_tau {args.retau:g}, FP64 plain, "
+        "config": {"workload": f"channel {nx}x{ny}x{nz}, {args.steps} steps, M_b 1.5, Re_tau {args.retau:g}, FP64 plain, "
                                "KEEP6 + WENO5 (Ducros > 0.9), Wray RK3, CFL 0.8",
                    "grid": [nx, ny, nz], "precision": "fp64", "finite": bool(finite),
                    "weno_face_fraction_xyz": weno_frac},
