@@ -51,6 +51,8 @@ ALGO_BYTES = {
     "visc": (120.0, "cell"),     # read w_aux(2,3,4,6,7) 40 B + fl(2:5) rw 64 B + write w_aux(9,10) 16 B
     "euler_y": (136.0, "cell"),  # read w_aux(1:6,8) 56 B + fl(5) rw 80 B
     "euler_z": (136.0, "cell"),
+    # z march + viscous divergence + force_rhs partials: the march's 136 B, mu 8, div/3 8, fl_prev(1,2) 16, w(1,2) 16
+    "euler_zv": (184.0, "cell"),
     "sensor": (24.0, "cell"),    # read w_aux(9,10) + write w_aux(8)
     "visc_div": (144.0, "cell"), # w_aux(2,3,4,7,10) 40 + fl(2:5) rw 64 + fl_prev(1,2) 16 + w(1,2) 16 + T 8
     "update": (168.0, "cell"),   # fl_prev 40 + fl 40 + w rw 80 + u 8

@@ -221,7 +221,8 @@ int st_set_stats(st_solver *s, const double *w_stat);
 #define ST_K_BC         12   /* physical ghost faces and characteristic boundaries */
 #define ST_K_EULER_XY   13   /* x and y Euler fluxes in one pass (y range spanning all rows) */
 #define ST_K_WENO_FIX   14   /* WENO faces of the Euler sweeps (list compaction + fix-up) */
-#define ST_NKFAM        15
+#define ST_K_EULER_ZV   15   /* z Euler fluxes + viscous divergence terms + force_rhs partials, with the z fix-up */
+#define ST_NKFAM        16
 int st_kernel_times(st_solver *s, int32_t nsteps, double ms[ST_NKFAM], int32_t count[ST_NKFAM]);
 
 /* Stream used by every launch of this solver (NULL = the library's own). */

@@ -269,7 +269,14 @@ int stage(st_solver *s, int istep) {
         TIMED(ST_K_EULER_Y, launch_euler_y(s, fcur, false));
         TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 1, 1, 0));
     }
-    if (s->kp.ndim == 3) {
+    // euler_z, with the viscous divergence terms (visflx(1)) and the
+    // force_rhs partials of the same cells folded into the march where the
+    // z range spans all planes (Laplacian viscous form)
+    const bool zv = s->fuse_zv && s->kp.ndim == 3 && !cons && s->kp.eul_lo[2] == 1 && s->kp.eul_hi[2] == g.nz &&
+                    s->kp.visc_lmax <= 3;
+    if (zv) {
+        TIMED(ST_K_EULER_ZV, launch_euler_zv(s, fcur, fprev, istep, s->p.channel_case != 0));
+    } else if (s->kp.ndim == 3) {
         TIMED(ST_K_EULER_Z, launch_euler_z(s, fcur, false));
         TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 2, 1, 0));
     }
@@ -279,7 +286,7 @@ int stage(st_solver *s, int istep) {
         TIMED(ST_K_SENSOR, launch_sensor(s, true));
         if (fp64_sensor(s)) TIMED(ST_K_SENSOR, launch_sensor_refine(s, false, true));
     }
-    TIMED(ST_K_VISC_DIV, launch_visc_div(s, fcur, fprev, istep, s->p.channel_case != 0, cons));  // visflx(1) or (7)
+    if (!zv) TIMED(ST_K_VISC_DIV, launch_visc_div(s, fcur, fprev, istep, s->p.channel_case != 0, cons));  // visflx(1) or (7)
     if (s->p.channel_case) TIMED(ST_K_FINISH, launch_force_finish(s, istep));
     if (s->has_nr) TIMED(ST_K_BC, launch_bc_nr(s, fcur));  // bc_nr
     TIMED(ST_K_UPDATE, launch_update(s, fprev, fcur, istep));
@@ -313,7 +320,7 @@ int64_t st_solver_footprint(const st_params *p) {
     const int64_t rs = p->precision == ST_PREC_MIXED ? 4 : 8;  // bytes of a w_aux / right-hand-side value
     const int64_t na = p->precision == ST_PREC_MIXED ? 11 : 10;
     const int64_t part = 7 * blocks > (nvol + 255) / 256 ? 7 * blocks : (nvol + 255) / 256;
-    return 8 * (5 * nvol + 3 * sx * sz + part + 1024) + rs * (na * nvol + 10 * nint) +
+    return 8 * (5 * nvol + 3 * sx * sz + part + 1280 + 1024) + rs * (na * nvol + 10 * nint) +
            (5 * nint + 4 * (p->nx + p->ny + p->nz + 3 + 8 + 4096)) +
            8 * (2 * sx + 2 * sy + 2 * sz + 3 * (p->nx + p->ny + p->nz) + p->ny + 1) + 4096;
 }
@@ -328,6 +335,7 @@ int st_solver_create(const st_params *p, st_solver **out) {
     std::memset((void *)s, 0, sizeof(st_solver));
     s->p = *p;
     s->fuse_xy = true;
+    s->fuse_zv = true;
     st::Geo &g = s->g;
     g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.ng = p->ng;
     g.sx = p->nx + 2 * p->ng; g.sy = p->ny + 2 * p->ng; g.sz = p->nz + 2 * p->ng;
@@ -351,7 +359,9 @@ int st_solver_create(const st_params *p, st_solver **out) {
     // per-block partials (7 per cell block, or one per block of the aux
     // pass over the ghosted box, whichever is larger) + second-level sums
     const int aux_blk = (int)((nvol + 255) / 256);
-    s->npart = (7 * nblk > aux_blk ? 7 * nblk : aux_blk) + 128 * 8;
+    // block partials (7 per cell block at most), the z fix-up's 5 x 256
+    // force_rhs partials, and the 128 x 8 second-level values at the end
+    s->npart = (7 * nblk > aux_blk ? 7 * nblk : aux_blk) + 256 * 5 + 128 * 8;
     hipError_t es[32];
     int ne = 0;
     es[ne++] = dalloc(&s->w, 5 * nvol);

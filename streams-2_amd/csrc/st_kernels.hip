@@ -653,21 +653,37 @@ __device__ __forceinline__ R face_coeff(const KPar &P, int lm, int l) {
     return R(lm == 3 ? P.cd1tab[2][l - 1] : (lm == 2 ? P.cd1tab[1][l - 1] : P.cd1tab[0][l - 1]));
 }
 
-template <typename R, int DIR, int KM>
+// Operands of the viscous divergence terms folded into the z march (VD > 0):
+// visflx_div_subroutine (kernels_cpu.F90:3484-3532) for the cell the march
+// completes, and with VD = 2 the block partials of force_rhs as k_visc_div
+// forms them.
+struct VDArgs {
+    const void *flp;  // right-hand side of the previous stage (R)
+    const double *w, *dx1, *dy1, *yn;
+    const DevScalars *ds;
+    double rkr, rkg;
+    double *partials;
+};
+
+template <typename R, int DIR, int KM, int VD>
 __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__restrict__ wa,
                                                      const int *__restrict__ eoc, const double *__restrict__ metric,
                                                      R *__restrict__ fl, int chunk, int nchunk,
-                                                     unsigned char *__restrict__ flags) {
+                                                     unsigned char *__restrict__ flags, VDArgs V) {
+    static_assert(VD == 0 || DIR == 2, "the viscous divergence rides on the z march only");
+    __shared__ double lds[4][5];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int no = DIR == 1 ? g.nz : g.ny;  // the other transverse extent
-    if (t >= g.nx * no * nchunk) return;
     const int i = t % g.nx + 1;
     const int r = t / g.nx;
     const int o = r % no + 1;
     const int c = r / no;
     const int lo = P.eul_lo[DIR], hi = P.eul_hi[DIR];
     const int c0 = lo + c * chunk;
-    if (c0 > hi) return;
+    // threads past the grid or the last chunk do nothing, but with VD = 2
+    // they stay for the block sum
+    const bool active = t < g.nx * no * nchunk && c0 <= hi;
+    if (VD < 2 && !active) return;
     const int c1 = min(c0 + chunk - 1, hi);
     const int n = DIR == 1 ? g.ny : g.nz;
     const int stride = DIR == 1 ? g.sx : g.sxy;
@@ -687,90 +703,178 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__r
         const int v = lb + eoc[ff];
         return v < 1 ? 1 : v;
     };
-    // The march starts three points early (b = c0-3): those iterations only
-    // load the warm-up points and feed faces below c0-1, which are never
-    // written.  Every point thus passes through the one loop body, so a
-    // face's arithmetic is the same whatever the chunk it falls in.
-    PcR<R> q0{}, q1{}, q2{};
-    unsigned hist = 0;
-    Acc<R> A0 = acc_zero<R>(), A1 = acc_zero<R>(), A2 = acc_zero<R>();  // faces b-3, b-2, b-1
-    int lm0 = lm_of(c0 - 6), lm1 = lm_of(c0 - 5), lm2 = lm_of(c0 - 4);
-    R fprev[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
-    bool wprev = false;
-    // prefetch of the next point
-    int a = p0 + (c0 - 3) * stride;
-    R nr = wa[a], nu = wa[nv + a], nvv = wa[2 * nv + a], nw = wa[3 * nv + a], nh = wa[4 * nv + a],
-      nt = wa[TS * nv + a], ns = wa[7 * nv + a];
-    for (int b = c0 - 3; b <= c1 + 3; b++) {
-        // issue the read of fl at the cell completed in this iteration first,
-        // so that its latency overlaps the pair arithmetic below
-        R flv[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
-        const int fc = b - 3;
-        const int oic = DIR == 1 ? iidx(g, i, fc < 1 ? 1 : fc, o) : iidx(g, i, o, fc < 1 ? 1 : fc);
-        if (fc >= c0) {
+    double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // force_rhs partial sums (VD = 2)
+    if (active) {
+        // The march starts three points early (b = c0-3): those iterations
+        // only load the warm-up points and feed faces below c0-1, which are
+        // never written.  Every point thus passes through the one loop body,
+        // so a face's arithmetic is the same whatever the chunk it falls in.
+        PcR<R> q0{}, q1{}, q2{};
+        unsigned hist = 0;
+        Acc<R> A0 = acc_zero<R>(), A1 = acc_zero<R>(), A2 = acc_zero<R>();  // faces b-3, b-2, b-1
+        int lm0 = lm_of(c0 - 6), lm1 = lm_of(c0 - 5), lm2 = lm_of(c0 - 4);
+        R fprev[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
+        bool wprev = false;
+        R dw[7] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};  // div/3 at points b-6 .. b (VD)
+        // prefetch of the next point
+        int a = p0 + (c0 - 3) * stride;
+        R nr = wa[a], nu = wa[nv + a], nvv = wa[2 * nv + a], nw = wa[3 * nv + a], nh = wa[4 * nv + a],
+          nt = wa[TS * nv + a], ns = wa[7 * nv + a];
+        R nd = VD ? wa[9 * nv + a] : R(0.0);
+        for (int b = c0 - 3; b <= c1 + 3; b++) {
+            // issue the read of fl at the cell completed in this iteration first,
+            // so that its latency overlaps the pair arithmetic below
+            R flv[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
+            const int fc = b - 3;
+            const int oic = DIR == 1 ? iidx(g, i, fc < 1 ? 1 : fc, o) : iidx(g, i, o, fc < 1 ? 1 : fc);
+            // the viscous-divergence operands of cell fc: mu, the x and y
+            // neighbours of div/3 (the z ones are in dw), and for VD = 2 the
+            // force_rhs inputs
+            R vmu = R(0.0), dxm[3], dxp[3], dym[3], dyp[3], flp1 = R(0.0), flp2 = R(0.0), tcell = R(0.0);
+            double w1 = 0.0, w2 = 0.0;
+            if (fc >= c0) {
 #pragma unroll
-            for (int m = 0; m < 5; m++) flv[m] = fl[(size_t)m * g.nint + oic];
-        }
-        const PcR<R> qb = make_pcr<R>(nr, nu, nvv, nw, nh, nt, rgas0, pref, keep);
-        hist = (hist << 1) | (ns > thr ? 1u : 0u);
-        if (b < c1 + 3) {
-            a += stride;
-            nr = wa[a]; nu = wa[nv + a]; nvv = wa[2 * nv + a]; nw = wa[3 * nv + a];
-            nh = wa[4 * nv + a]; nt = wa[TS * nv + a]; ns = wa[7 * nv + a];
-        }
-        {
-            const Acc<R> p3 = pair_terms<R, DIR, KM>(P, q0, qb, pref);
-            acc_add(A0, face_coeff<R>(P, lm0, 3), p3);
-            acc_add(A1, face_coeff<R>(P, lm1, 3), p3);
-            acc_add(A2, face_coeff<R>(P, lm2, 3), p3);
-        }
-        {
-            const Acc<R> p2 = pair_terms<R, DIR, KM>(P, q1, qb, pref);
-            acc_add(A1, face_coeff<R>(P, lm1, 2), p2);
-            acc_add(A2, face_coeff<R>(P, lm2, 2), p2);
-        }
-        {
-            const Acc<R> p1 = pair_terms<R, DIR, KM>(P, q2, qb, pref);
-            acc_add(A2, face_coeff<R>(P, lm2, 1), p1);
-        }
-        const int f = b - 3;  // face complete
-        if (f >= c0 - 1) {
-            // (the fl values of cell f were loaded at the top of the iteration)
-            const bool wcur = (hist & smask) != 0u;
-            R fh[5];
-            fh[0] = R(0.25) * A0.m;
-            fh[1] = R(0.25) * A0.mu;
-            fh[2] = R(0.25) * A0.mv;
-            fh[3] = R(0.25) * A0.mw;
-            fh[4] = R(0.25) * A0.e;
-            if ((f == 0 && P.fzf[2 * DIR] == 1) || (f == n && P.fzf[2 * DIR + 1] == 1)) {
+                for (int m = 0; m < 5; m++) flv[m] = fl[(size_t)m * g.nint + oic];
+                if (VD) {
+                    const int cc = gidx(g, i, o, fc);
+                    const R *D = wa + 9 * nv;
+                    vmu = wa[6 * nv + cc];
 #pragma unroll
-                for (int m = 0; m < 5; m++) fh[m] = R(0.0);
-            }
-            fh[1 + DIR] = fh[1 + DIR] + R(0.5) * A0.pp;
-            bool dirty = false;
-            if (f >= c0) {
-                dirty = wcur || wprev;
-                if (!dirty) {
-                    const R d = R(metric[f - 1]);
-#pragma unroll
-                    for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oic] = flv[m] + (fh[m] - fprev[m]) * d;
+                    for (int l = 1; l <= 3; l++) {
+                        dxm[l - 1] = D[cc - l];
+                        dxp[l - 1] = D[cc + l];
+                        dym[l - 1] = D[cc - l * g.sx];
+                        dyp[l - 1] = D[cc + l * g.sx];
+                    }
+                    if (VD == 2) {
+                        const R *fp = (const R *)V.flp;
+                        flp1 = fp[oic];
+                        flp2 = fp[g.nint + oic];
+                        w1 = V.w[cc];
+                        w2 = V.w[nv + cc];
+                        tcell = wa[5 * nv + cc];
+                    }
                 }
             }
-            mark_dirty(flags, g.nint, dirty, oic);
+            const PcR<R> qb = make_pcr<R>(nr, nu, nvv, nw, nh, nt, rgas0, pref, keep);
+            hist = (hist << 1) | (ns > thr ? 1u : 0u);
+            if (VD) {
 #pragma unroll
-            for (int m = 0; m < 5; m++) fprev[m] = fh[m];
-            wprev = wcur;
+                for (int s2 = 0; s2 < 6; s2++) dw[s2] = dw[s2 + 1];
+                dw[6] = nd;
+            }
+            if (b < c1 + 3) {
+                a += stride;
+                nr = wa[a]; nu = wa[nv + a]; nvv = wa[2 * nv + a]; nw = wa[3 * nv + a];
+                nh = wa[4 * nv + a]; nt = wa[TS * nv + a]; ns = wa[7 * nv + a];
+                if (VD) nd = wa[9 * nv + a];
+            }
+            {
+                const Acc<R> p3 = pair_terms<R, DIR, KM>(P, q0, qb, pref);
+                acc_add(A0, face_coeff<R>(P, lm0, 3), p3);
+                acc_add(A1, face_coeff<R>(P, lm1, 3), p3);
+                acc_add(A2, face_coeff<R>(P, lm2, 3), p3);
+            }
+            {
+                const Acc<R> p2 = pair_terms<R, DIR, KM>(P, q1, qb, pref);
+                acc_add(A1, face_coeff<R>(P, lm1, 2), p2);
+                acc_add(A2, face_coeff<R>(P, lm2, 2), p2);
+            }
+            {
+                const Acc<R> p1 = pair_terms<R, DIR, KM>(P, q2, qb, pref);
+                acc_add(A2, face_coeff<R>(P, lm2, 1), p1);
+            }
+            const int f = b - 3;  // face complete
+            if (f >= c0 - 1) {
+                // (the fl values of cell f were loaded at the top of the iteration)
+                const bool wcur = (hist & smask) != 0u;
+                R fh[5];
+                fh[0] = R(0.25) * A0.m;
+                fh[1] = R(0.25) * A0.mu;
+                fh[2] = R(0.25) * A0.mv;
+                fh[3] = R(0.25) * A0.mw;
+                fh[4] = R(0.25) * A0.e;
+                if ((f == 0 && P.fzf[2 * DIR] == 1) || (f == n && P.fzf[2 * DIR + 1] == 1)) {
+#pragma unroll
+                    for (int m = 0; m < 5; m++) fh[m] = R(0.0);
+                }
+                fh[1 + DIR] = fh[1 + DIR] + R(0.5) * A0.pp;
+                bool dirty = false;
+                if (f >= c0) {
+                    dirty = wcur || wprev;
+                    if (VD == 0) {
+                        if (!dirty) {
+                            const R d = R(metric[f - 1]);
+#pragma unroll
+                            for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oic] = flv[m] + (fh[m] - fprev[m]) * d;
+                        }
+                    } else {
+                        // z divergence (clean cells; a WENO cell gets it from
+                        // the fix-up), then the viscous divergence terms in
+                        // the order of k_visc_div
+                        R tv[5];
+                        const R d = R(metric[f - 1]);
+#pragma unroll
+                        for (int m = 0; m < 5; m++) tv[m] = dirty ? flv[m] : flv[m] + (fh[m] - fprev[m]) * d;
+                        R dxv = R(0.0), dyv = R(0.0), dzv = R(0.0);
+#pragma unroll
+                        for (int l = 1; l <= 3; l++) {
+                            if (l <= P.visc_lmax) {
+                                const R ccl = R(P.vd1[l - 1]);
+                                dxv = dxv + ccl * (dxp[l - 1] - dxm[l - 1]);
+                                dyv = dyv + ccl * (dyp[l - 1] - dym[l - 1]);
+                                dzv = dzv + ccl * (dw[3 + l] - dw[3 - l]);
+                            }
+                        }
+                        dxv = dxv * R(V.dx1[i - 1]);
+                        dyv = dyv * R(V.dy1[o - 1]);
+                        dzv = dzv * R(metric[f - 1]);
+                        const R sigx = vmu * dxv, sigy = vmu * dyv, sigz = vmu * dzv;
+                        const R sigq = sigx * q0.u + sigy * q0.v + sigz * q0.w;
+                        tv[1] = tv[1] - sigx;
+                        tv[2] = tv[2] - sigy;
+                        tv[3] = tv[3] - sigz;
+                        tv[4] = tv[4] - sigq;
+#pragma unroll
+                        for (int m = 0; m < 5; m++)
+                            if (m > 0 || !dirty) fl[(size_t)m * g.nint + oic] = tv[m];
+                        if (VD == 2) {
+                            const double dt = V.ds->dt;
+                            const double rhodt = V.rkr * dt, gamdt = V.rkg * dt;
+                            const double dy = V.yn[o] - V.yn[o - 1];
+                            const double fln1 = -rhodt * double(flp1) - gamdt * double(tv[0]);
+                            const double fln2 = -rhodt * double(flp2) - gamdt * double(tv[1]);
+                            red[0] += fln1 * dy;
+                            red[1] += fln2 * dy;
+                            red[2] += w1 * dy;
+                            red[3] += w2 * dy;
+                            red[4] += w2 * dy * double(tcell);
+                        }
+                    }
+                }
+                mark_dirty(flags, g.nint, dirty, oic);
+#pragma unroll
+                for (int m = 0; m < 5; m++) fprev[m] = fh[m];
+                wprev = wcur;
+            }
+            A0 = A1;
+            A1 = A2;
+            A2 = acc_zero<R>();
+            lm0 = lm1;
+            lm1 = lm2;
+            lm2 = lm_of(b);
+            q0 = q1;
+            q1 = q2;
+            q2 = qb;
         }
-        A0 = A1;
-        A1 = A2;
-        A2 = acc_zero<R>();
-        lm0 = lm1;
-        lm1 = lm2;
-        lm2 = lm_of(b);
-        q0 = q1;
-        q1 = q2;
-        q2 = qb;
+    }
+    if (VD == 2) {
+        block_sum<5>(red, lds);
+        if (flat_thread() == 0) {
+            double *pp = V.partials + (size_t)flat_block() * 5;
+#pragma unroll
+            for (int q = 0; q < 5; q++) pp[q] = red[q];
+        }
     }
 }
 
@@ -1182,10 +1286,14 @@ __device__ __forceinline__ void hybrid_face(const KPar &P, const Pt (&q)[6], con
 // fl += (fhat(c) - fhat(c-1)) * metric(c), and the flag cleared.  One
 // 64-lane workgroup per grid-stride lane set; flagged cells are compacted
 // into an LDS queue with a ballot so the WENO work runs on full waves.
-template <typename R, int DIR>
+// With RED the change the fix-up makes to fl(1) and fl(2) enters the
+// force_rhs sums as k_visc_div forms them: r0, r1 += -gamdt*dfl*dy.
+template <typename R, int DIR, bool RED = false>
 __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const R *__restrict__ wa,
                                          const double *__restrict__ w, const int *__restrict__ eoc,
-                                         const double *__restrict__ metric, R *__restrict__ fl, int oi) {
+                                         const double *__restrict__ metric, R *__restrict__ fl, int oi,
+                                         double *r0 = nullptr, double *r1 = nullptr, double gamdt = 0.0,
+                                         const double *__restrict__ yn = nullptr) {
     const size_t nv = g.nvol;
     const bool keep = P.nkeep >= 0;
     const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
@@ -1280,8 +1388,17 @@ __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const R *_
             else fhi[m] = fh[m];
         }
     }
+    R fnew[5];
 #pragma unroll
-    for (int m = 0; m < 5; m++) fl[(size_t)m * g.nint + oi] = R(flv[m] + (fhi[m] - flo[m]) * d);
+    for (int m = 0; m < 5; m++) {
+        fnew[m] = R(flv[m] + (fhi[m] - flo[m]) * d);
+        fl[(size_t)m * g.nint + oi] = fnew[m];
+    }
+    if (RED) {
+        const double dy = yn[j] - yn[j - 1];
+        *r0 += -gamdt * (double(fnew[0]) - flv[0]) * dy;
+        *r1 += -gamdt * (double(fnew[1]) - flv[1]) * dy;
+    }
 }
 
 // The WENO fix-up of one direction in two kernels.  k_fix_compact: block b
@@ -1294,6 +1411,7 @@ __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const R *_
 // the shocked cells are distributed.  No atomics: the lists and their order
 // are a function of the flags alone.
 constexpr int kFixMaxSeg = 4096;
+constexpr int kFixBlocks = 256;  // workgroups of the fix-up (one per CU at its occupancy)
 
 __device__ __forceinline__ unsigned dirty_bytes(unsigned long long wd, int bit, int cell0, int ncell) {
     unsigned mine = 0;  // bit b: byte b of the word is a flagged cell
@@ -1359,48 +1477,69 @@ __global__ __launch_bounds__(256) void k_fix_compact(unsigned char *__restrict__
     }
 }
 
-template <typename R, int DIR>
+// RED: the block's partial force_rhs sums of the fix-up's changes to fl(1)
+// and fl(2) go to partials[blockIdx.x] (5 values, the last three zero),
+// written also when there is nothing to fix.
+template <typename R, int DIR, bool RED>
 __global__ __launch_bounds__(256) void k_fix_list(Geo g, KPar P, const R *__restrict__ wa,
                                                   const double *__restrict__ w, const int *__restrict__ eoc,
                                                   const double *__restrict__ metric, R *__restrict__ fl,
                                                   const unsigned char *__restrict__ flags, int anyidx,
                                                   const int *__restrict__ list, const int *__restrict__ counts,
-                                                  int nseg, int seglen) {
+                                                  int nseg, int seglen, const DevScalars *__restrict__ ds,
+                                                  double rkg, const double *__restrict__ yn,
+                                                  double *__restrict__ partials) {
     __shared__ int pre[kFixMaxSeg + 1];
     __shared__ int wsum[4];
-    if (flags[g.nint + anyidx] == 0) return;
-    // prefix sums of the segment counts: thread t sums segments 16t .. 16t+15
+    __shared__ double lds[4][5];
+    double r0 = 0.0, r1 = 0.0;
+    const bool any = flags[g.nint + anyidx] != 0;  // uniform
+    if (!RED && !any) return;
     const int t = threadIdx.x;
-    int cs[16], sum = 0;
+    if (any) {
+        // prefix sums of the segment counts: thread t sums segments 16t .. 16t+15
+        int cs[16], sum = 0;
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int sgi = 16 * t + q;
-        cs[q] = sgi < nseg ? counts[sgi] : 0;
-        sum += cs[q];
-    }
-    int total;
-    int run = block_excl_scan256(sum, wsum, &total);
+        for (int q = 0; q < 16; q++) {
+            const int sgi = 16 * t + q;
+            cs[q] = sgi < nseg ? counts[sgi] : 0;
+            sum += cs[q];
+        }
+        int total;
+        int run = block_excl_scan256(sum, wsum, &total);
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int sgi = 16 * t + q;
-        if (sgi < nseg) pre[sgi] = run;
-        run += cs[q];
-    }
-    if (t == 0) pre[nseg] = total;
-    __syncthreads();
-    const int lane = t & 63;
-    const int wave = (blockIdx.x * blockDim.x + t) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int base = wave * 64; base < total; base += nwaves * 64) {
-        const int e = base + lane;
-        if (e < total) {
-            int lo = 0, hi = nseg;  // pre[lo] <= e < pre[hi]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (pre[mid] <= e) lo = mid;
-                else hi = mid;
+        for (int q = 0; q < 16; q++) {
+            const int sgi = 16 * t + q;
+            if (sgi < nseg) pre[sgi] = run;
+            run += cs[q];
+        }
+        if (t == 0) pre[nseg] = total;
+        __syncthreads();
+        const double gamdt = RED ? rkg * ds->dt : 0.0;
+        const int lane = t & 63;
+        const int wave = (blockIdx.x * blockDim.x + t) >> 6;
+        const int nwaves = (gridDim.x * blockDim.x) >> 6;
+        for (int base = wave * 64; base < total; base += nwaves * 64) {
+            const int e = base + lane;
+            if (e < total) {
+                int lo = 0, hi = nseg;  // pre[lo] <= e < pre[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (pre[mid] <= e) lo = mid;
+                    else hi = mid;
+                }
+                fix_cell<R, DIR, RED>(g, P, wa, w, eoc, metric, fl, list[(size_t)lo * seglen + (e - pre[lo])], &r0,
+                                      &r1, gamdt, yn);
             }
-            fix_cell<R, DIR>(g, P, wa, w, eoc, metric, fl, list[(size_t)lo * seglen + (e - pre[lo])]);
+        }
+    }
+    if (RED) {
+        double red[5] = {r0, r1, 0.0, 0.0, 0.0};
+        block_sum<5>(red, lds);
+        if (t == 0) {
+            double *pp = partials + (size_t)blockIdx.x * 5;
+#pragma unroll
+            for (int q = 0; q < 5; q++) pp[q] = red[q];
         }
     }
 }
@@ -2556,7 +2695,8 @@ hipError_t launch_wall_ghosts(st_solver *s) {
 }
 
 template <typename R>
-static hipError_t launch_fix(st_solver *s, R *fl, int dir, int bit = 1, int anyidx = 0) {
+static hipError_t launch_fix(st_solver *s, R *fl, int dir, int bit = 1, int anyidx = 0, double rkg = 0.0,
+                             double *partials = nullptr) {
     const Geo &g = s->g;
     // segments of 2048 words (16384 cells) or longer, at most kFixMaxSeg
     const int nword = (g.nint + 7) / 8;
@@ -2571,18 +2711,17 @@ static hipError_t launch_fix(st_solver *s, R *fl, int dir, int bit = 1, int anyi
                        wpb, s->fixlist, counts);
     // the fix-up holds one wave per SIMD (its register budget): one
     // 4-wave block per CU
-    const int nb = 256;
+    const int nb = kFixBlocks;
     const int seglen = 8 * wpb;
     const R *wa = (const R *)s->w_aux;
-    if (dir == 0)
-        hipLaunchKernelGGL((k_fix_list<R, 0>), dim3(nb), dim3(256), 0, s->stream, g, s->kp, wa, s->w, s->eoc[0],
-                           s->dcsidx, fl, s->dirty, anyidx, s->fixlist, counts, nseg, seglen);
-    else if (dir == 1)
-        hipLaunchKernelGGL((k_fix_list<R, 1>), dim3(nb), dim3(256), 0, s->stream, g, s->kp, wa, s->w, s->eoc[1],
-                           s->detady, fl, s->dirty, anyidx, s->fixlist, counts, nseg, seglen);
-    else
-        hipLaunchKernelGGL((k_fix_list<R, 2>), dim3(nb), dim3(256), 0, s->stream, g, s->kp, wa, s->w, s->eoc[2],
-                           s->dzitdz, fl, s->dirty, anyidx, s->fixlist, counts, nseg, seglen);
+#define ST_LAUNCH_F(D_, RED_, E_, M_)                                                                              \
+    hipLaunchKernelGGL((k_fix_list<R, D_, RED_>), dim3(nb), dim3(256), 0, s->stream, g, s->kp, wa, s->w, E_, M_, fl, \
+                       s->dirty, anyidx, s->fixlist, counts, nseg, seglen, s->ds, rkg, s->yn, partials)
+    if (dir == 0) ST_LAUNCH_F(0, false, s->eoc[0], s->dcsidx);
+    else if (dir == 1) ST_LAUNCH_F(1, false, s->eoc[1], s->detady);
+    else if (partials) ST_LAUNCH_F(2, true, s->eoc[2], s->dzitdz);
+    else ST_LAUNCH_F(2, false, s->eoc[2], s->dzitdz);
+#undef ST_LAUNCH_F
     return hipGetLastError();
 }
 
@@ -2619,7 +2758,8 @@ hipError_t launch_euler_x(st_solver *s, void *fl, bool init, bool fix) {
 }
 
 template <typename R>
-static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix) {
+static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix, int vd = 0, const R *fl_prev = nullptr,
+                               int stage = 0) {
     const Geo &g = s->g;
     const int lo = s->kp.eul_lo[dir], hi = s->kp.eul_hi[dir];
     hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
@@ -2631,26 +2771,48 @@ static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix) {
     int chunk = ncell < 32 ? ncell : 32;
     const long minthreads = 256L * 16 * 64 * 4;
     while (chunk > 8 && (long)g.nx * no * ((ncell + chunk - 1) / chunk) < minthreads) chunk /= 2;
-    if (s->march_chunk > 0) chunk = s->march_chunk < ncell ? s->march_chunk : ncell;
+    // the march that carries the force_rhs partials sums them per chunk, so
+    // it keeps the automatic chunk (a function of the grid alone): a tuned
+    // chunk must not change results
+    if (s->march_chunk > 0 && vd == 0) chunk = s->march_chunk < ncell ? s->march_chunk : ncell;
     const int nchunk = (ncell + chunk - 1) / chunk;
     const size_t tot = (size_t)g.nx * no * nchunk;
-    dim3 grid((unsigned)((tot + 255) / 256));
+    const unsigned nblk = (unsigned)((tot + 255) / 256);
+    dim3 grid(nblk);
     const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
-#define ST_LAUNCH_M(D, K, E, M)                                                                                     \
-    hipLaunchKernelGGL((k_euler_march<R, D, K>), grid, dim3(256), 0, s->stream, g, s->kp, (const R *)s->w_aux, E, M, \
-                       fl, chunk, nchunk, s->dirty)
-    if (dir == 1) {
-        if (km == 0) ST_LAUNCH_M(1, 0, s->eoc[1], s->detady);
-        else if (km > 0) ST_LAUNCH_M(1, 1, s->eoc[1], s->detady);
-        else ST_LAUNCH_M(1, -1, s->eoc[1], s->detady);
-    } else {
-        if (km == 0) ST_LAUNCH_M(2, 0, s->eoc[2], s->dzitdz);
-        else if (km > 0) ST_LAUNCH_M(2, 1, s->eoc[2], s->dzitdz);
-        else ST_LAUNCH_M(2, -1, s->eoc[2], s->dzitdz);
+    VDArgs V{};
+    if (vd) {
+        V.flp = fl_prev;
+        V.w = s->w;
+        V.dx1 = s->dcsidx;
+        V.dy1 = s->detady;
+        V.yn = s->yn;
+        V.ds = s->ds;
+        V.rkr = s->rhork[stage];
+        V.rkg = s->gamrk[stage];
+        V.partials = s->partials;
     }
+#define ST_LAUNCH_M(D, K, VD_, E, M)                                                                               \
+    hipLaunchKernelGGL((k_euler_march<R, D, K, VD_>), grid, dim3(256), 0, s->stream, g, s->kp, (const R *)s->w_aux, \
+                       E, M, fl, chunk, nchunk, s->dirty, V)
+#define ST_LAUNCH_MK(D, VD_, E, M)                           \
+    do {                                                     \
+        if (km == 0) ST_LAUNCH_M(D, 0, VD_, E, M);           \
+        else if (km > 0) ST_LAUNCH_M(D, 1, VD_, E, M);       \
+        else ST_LAUNCH_M(D, -1, VD_, E, M);                  \
+    } while (0)
+    if (dir == 1) ST_LAUNCH_MK(1, 0, s->eoc[1], s->detady);
+    else if (vd == 2) ST_LAUNCH_MK(2, 2, s->eoc[2], s->dzitdz);
+    else if (vd == 1) ST_LAUNCH_MK(2, 1, s->eoc[2], s->dzitdz);
+    else ST_LAUNCH_MK(2, 0, s->eoc[2], s->dzitdz);
+#undef ST_LAUNCH_MK
 #undef ST_LAUNCH_M
     hipError_t e = hipGetLastError();
+    // the force_rhs partials of this stage: the march's blocks, then the
+    // z fix-up's (it always writes them, zero when nothing is flagged)
+    if (vd == 2) s->force_nblk = (int)nblk + kFixBlocks;
     if (e != hipSuccess || !fix) return e;
+    if (vd == 2) return launch_fix(s, fl, dir, 1, 0, s->gamrk[stage], s->partials + (size_t)nblk * 5);
     return launch_fix(s, fl, dir);
 }
 
@@ -2687,6 +2849,14 @@ static hipError_t launch_xy(st_solver *s, R *fl, bool fix) {
 hipError_t launch_euler_xy(st_solver *s, void *fl, bool fix) { ST_DISPATCH(return launch_xy(s, (R *)fl, fix)); }
 hipError_t launch_euler_y(st_solver *s, void *fl, bool fix) { ST_DISPATCH(return launch_march(s, (R *)fl, 1, fix)); }
 hipError_t launch_euler_z(st_solver *s, void *fl, bool fix) { ST_DISPATCH(return launch_march(s, (R *)fl, 2, fix)); }
+
+// z march with the viscous divergence terms (visflx mode 1) of the same
+// cells and, in the channel case, the force_rhs partials; the z fix-up is
+// part of it (its changes to fl(1:2) enter the partials).  Requires the
+// Laplacian viscous form, a z range spanning all planes and visc_lmax <= 3.
+hipError_t launch_euler_zv(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce) {
+    ST_DISPATCH(return launch_march(s, (R *)fl, 2, true, reduce ? 2 : 1, (const R *)fl_prev, stage));
+}
 
 // The fix-up of the last sweep of direction dir, for callers that time it
 // apart from the sweep: bit 1 / any-dirty byte 0 after a single-direction
@@ -2752,6 +2922,7 @@ hipError_t launch_sensor_refine(st_solver *s, bool direct, bool ghost_images) {
 
 hipError_t launch_visc_div(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce, bool ord2) {
     const Geo &g = s->g;
+    s->force_nblk = cell_blocks(g);
     const double rkr = stage >= 0 ? s->rhork[stage] : 0.0, rkg = stage >= 0 ? s->gamrk[stage] : 0.0;
 #define ST_LAUNCH_D(K_, RED_, A_, B_, C_)                                                                           \
     ST_DISPATCH(hipLaunchKernelGGL((K_<R, RED_>), cell_grid(g), kCellBlock, 0, s->stream, g, s->kp,                  \
@@ -2770,7 +2941,7 @@ hipError_t launch_visc_div(st_solver *s, void *fl, const void *fl_prev, int stag
 
 hipError_t launch_force_finish(st_solver *s, int stage) {
     hipLaunchKernelGGL(k_force_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials,
-                       cell_blocks(s->g), s->partials + s->npart - RB * 8, s->counters + 1, s->ds, stage == 0 ? 1 : 0);
+                       s->force_nblk, s->partials + s->npart - RB * 8, s->counters + 1, s->ds, stage == 0 ? 1 : 0);
     return hipGetLastError();
 }
 

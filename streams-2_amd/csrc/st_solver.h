@@ -28,6 +28,8 @@ struct st_solver {
     int visc_chunk;     // planes per tile of the z-marching viscous kernel (0: automatic)
     int march_chunk;    // points per thread of the y/z flux marches (0: automatic)
     bool fuse_xy;       // x and y Euler fluxes in one pass when the y range spans all rows
+    bool fuse_zv;       // viscous divergence terms folded into the z march (Laplacian form)
+    int force_nblk;     // force_rhs partials written in this stage (visc_div blocks, or z march + fix-up)
     double *wallprop;   // 3 x (sx*sz), wallprop(:,:,2:4)
     int *eoc[3];        // ep_ord_change per direction, faces 0..n
     // grid
@@ -81,6 +83,7 @@ hipError_t launch_euler_y(st_solver *s, void *fl, bool fix = true);
 hipError_t launch_euler_xy(st_solver *s, void *fl, bool fix = true);
 hipError_t launch_euler_z(st_solver *s, void *fl, bool fix = true);
 hipError_t launch_weno_fix(st_solver *s, void *fl, int dir, int bit, int anyidx);
+hipError_t launch_euler_zv(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce);
 hipError_t launch_visc(st_solver *s, void *fl, int mode, bool ghost_images);
 hipError_t launch_visc_cons(st_solver *s, void *fl, int mode, bool ghost_images);
 hipError_t launch_sensor(st_solver *s, bool ghost_images);

@@ -19,7 +19,7 @@ ST_ERR_STATE = 4
 ST_ERR_NAN = 5
 
 KERNEL_FAMILIES = ("aux", "euler_x", "visc", "euler_y", "euler_z", "sensor", "visc_div", "finish",
-                   "update", "force_var", "dt", "step_end", "bc", "euler_xy", "weno_fix")
+                   "update", "force_var", "dt", "step_end", "bc", "euler_xy", "weno_fix", "euler_zv")
 
 LIB_NAME = "libstreams_mi355x.so"
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
