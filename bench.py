@@ -28,6 +28,11 @@ Two more blocks of the same line:
   Re_tau 100-500, Poisson arrivals, 200 steps each) through the library's
   continuous-batching server, with p50/p99 time-to-solution, against a
   sample of the same requests run one after another through the plain path.
+
+--workload sbli runs the shock / boundary-layer box instead (1536 x 320 x
+192, recycling inflow, imposed oblique shock, conservative viscous form) in
+FP64 and mixed and prints that workload's own JSON line: rates, WENO face
+fractions, per-family times and the mixed form's error after 100 steps.
 """
 import argparse
 import json
@@ -133,6 +138,86 @@ def mixed_form(args, c, value_fp64):
             "kernels_ms": {k: round(v[0], 4) for k, v in kt.items()}}
 
 
+def sbli_workload(args):
+    """Oblique shock / boundary-layer interaction at the scale of the SBLI
+    configuration: 1536x320x192 nodes over 70 x 12 x 6.5 (delta = 1), Mach 2,
+    Re 200, adiabatic-recovery wall, recycling-rescaling inflow, supersonic
+    outflow with the characteristic treatment, imposed oblique shock of
+    38.774 deg at the top (the reference deck's angle) with the non-reflecting
+    top, periodic span, conservative viscous form, WENO5 where the Ducros
+    sensor exceeds 0.2.  Seeded BL profile plus 5 % velocity noise.  FP64 and
+    mixed rates, WENO face fractions, per-family times and the mixed form's
+    error against FP64 after 100 steps from the same field."""
+    import numpy as np
+    from streams_mi355x import PREC_MIXED
+    from streams_mi355x.cases import (ChannelCase, bl_grid, bl_initial_field, bl_mean_planes, bl_params,
+                                      recycling_setup)
+    from streams_mi355x.params import FlowCase, params_copy
+    from streams_mi355x.solver import ChannelSolver
+
+    nx, ny, nz = args.nx, args.ny, args.nz
+    lx = 70.0
+    g0 = bl_grid(nx, ny, nz, lx=lx, ly=12.0, lz=6.5, beta_y=3.0)
+    gam, pr, mach = 1.4, 0.72, 2.0
+    t_rec = 1.0 + 0.5 * (gam - 1.0) * pr ** (1.0 / 3.0) * mach ** 2
+    p = bl_params(nx, ny, nz, mach=mach, reynolds=200.0, t_wall=t_rec, bctags=(10, 2, 6, 7, 0, 0),
+                  bctags_nr=(0, 1, 0, 1, 0, 0), sensor_threshold=0.2, conservative_viscous=1,
+                  shock_angle_deg=38.774266, xshock_imp=40.0, tanhfacs=lx / (nx - 1))
+    c = ChannelCase(FlowCase(nx=nx, ny=ny, nz=nz), g0, p, bl_initial_field(g0, p, delta=1.0, noise=0.05, seed=0))
+    rc = recycling_setup(c.grid, 20.0, 1.0, 0.01, 1.1, 0.011, bl_mean_planes(c))
+    progress(f"sbli case built: {nx}x{ny}x{nz}")
+    cells = nx * ny * nz
+    out = {"metric": "Gcell-updates/s per RK stage (SBLI box, FP64 and mixed)",
+           "config": {"workload": f"SBLI box {nx}x{ny}x{nz}, {args.steps} steps, M 2, Re 200, recycling inflow, "
+                                  "shock 38.774 deg at x = 40, sensor > 0.2, conservative viscous form"}}
+    fields = []
+    for name, prec in (("fp64", 0), ("mixed", PREC_MIXED)):
+        pp = params_copy(p)
+        pp.precision = prec
+        g = ChannelSolver(pp, c.grid)
+        g.set_recycling(rc)
+        g.copy_from_field(c.w0)
+        g.init_run()
+        g.advance(args.warmup)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.advance(args.steps)
+        g.synchronize()
+        el = time.perf_counter() - t0
+        res = g.compute_residual()
+        kt = g.kernel_times(3)
+        # FP64 byte counts; without channel forcing the viscous divergence
+        # pass has no force_rhs inputs (104 B/cell)
+        nbytes = dict(ALGO_BYTES, visc_div=(104.0, "cell"))
+        nvol = (nx + 6) * (ny + 6) * (nz + 6)
+
+        def gbs(k, ms):
+            if prec != 0 or k not in nbytes:
+                return None
+            per, unit = nbytes[k]
+            return round(per * (nvol if unit == "point" else cells) / (ms * 1e-3) / 1e9, 1)
+        out[name] = {"value": cells * 3 * args.steps / el / 1e9, "ms_per_step": el / args.steps * 1e3,
+                     "finite": res["residual_rhou"] == res["residual_rhou"],
+                     "weno_face_fraction_xyz": [float(x) for x in g.count_weno()],
+                     "kernels": {k: {"ms": round(v[0], 4), "launches": v[1], "GB/s": gbs(k, v[0])}
+                                 for k, v in kt.items()}}
+        progress(f"sbli {name}: {out[name]['value']:.3f} Gcell-updates/s per RK stage, "
+                 f"WENO {out[name]['weno_face_fraction_xyz']}")
+        g.close()
+        # the same 100 steps from the initial field for the error
+        g = ChannelSolver(pp, c.grid)
+        g.set_recycling(rc)
+        g.copy_from_field(c.w0)
+        g.init_run()
+        g.advance(100)
+        fields.append(g.copy_to_field()[:, 3:-3, 3:-3, 3:-3])
+        g.close()
+    out["mixed"]["speedup_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    out["mixed"]["rel_l2_vs_fp64_after_100_steps"] = [
+        float(np.linalg.norm(fields[1][m] - fields[0][m]) / np.linalg.norm(fields[0][m])) for m in range(5)]
+    return out
+
+
 def serving(args):
     """Serving workload: a seeded stream of independent channel solves
     (grids uniform in [64,256]x[64,128]x[64,128], bulk Mach 0.3-3.0, Re_tau
@@ -180,6 +265,9 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-hbm-gib", type=float, default=64.0)
     ap.add_argument("--serve-seq-sample", type=int, default=16)
+    ap.add_argument("--workload", choices=("channel", "sbli"), default="channel",
+                    help="sbli: the shock / boundary-layer box (1536x320x192 unless --nx/--ny/--nz say otherwise), "
+                         "FP64 and mixed, printed as its own JSON line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of 2 GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -238,6 +326,11 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the hot path runs on a single MI355X; multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "sbli":
+        if (args.nx, args.ny, args.nz) == (512, 256, 256):
+            args.nx, args.ny, args.nz = 1536, 320, 192
+        print(json.dumps(sbli_workload(args)))
+        return
     from streams_mi355x import make_channel
     from streams_mi355x.solver import ChannelSolver
 
