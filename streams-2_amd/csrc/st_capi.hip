@@ -250,9 +250,9 @@ int stage(st_solver *s, int istep) {
     TIMED(ST_K_AUX, launch_aux(s, 1 - g.ng, g.nx + g.ng, 1 - g.ng, g.ny + g.ng, 1 - g.ng, g.nz + g.ng, true,
                                istep == 0 && s->dt_in_stage));
     if (istep == 0) s->dt_in_stage = false;
-    // euler_x + euler_y in one pass where the y range spans all rows; each
-    // sweep's WENO faces are then fixed up (timed as their own family)
-    const bool xy = s->fuse_xy && s->kp.eul_lo[1] == 1 && s->kp.eul_hi[1] == g.ny;
+    // euler_x + euler_y in one pass; each sweep's WENO faces are then fixed
+    // up (timed as their own family)
+    const bool xy = s->fuse_xy;
     if (xy) {
         TIMED(ST_K_EULER_XY, launch_euler_xy(s, fcur, false));
         TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 0, 1, 0));

@@ -1072,7 +1072,8 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const R *__restr
 // whose faces need WENO leaves its term to the fix-up through bit 0 (x) or
 // bit 1 (y) of the cell's flag byte and the any-dirty bytes nint / nint+1.
 // Both terms and their order of addition are those of the separate sweeps.
-// Requires the y range of the Euler fluxes to span all interior rows.
+// Rows outside the y range of the Euler fluxes (cut by a characteristic
+// boundary) receive the x term only, as from the two sweeps.
 template <typename R, int KM>
 __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__restrict__ wa,
                                                   const int *__restrict__ eocx, const int *__restrict__ eocy,
@@ -1091,10 +1092,12 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
     const int pt = i0 - 3 + lane;
     const int pl = pt > g.nx + 3 ? g.nx + 3 : pt;
     const int lox = P.eul_lo[0], hix = P.eul_hi[0];
-    const int hiy = P.eul_hi[1];
-    const int c0 = P.eul_lo[1] + cidx * chunk;
-    if (c0 > hiy) return;
-    const int c1 = min(c0 + chunk - 1, hiy);
+    const int loy = P.eul_lo[1], hiy = P.eul_hi[1];
+    // the march covers every row (the x faces); the y divergence goes to
+    // the rows of the y range only (a characteristic boundary cuts it)
+    const int c0 = 1 + cidx * chunk;
+    if (c0 > g.ny) return;
+    const int c1 = min(c0 + chunk - 1, g.ny);
     const int n = g.ny;
     const int stride = g.sx;
     const size_t nv = g.nvol;
@@ -1218,7 +1221,8 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
                 for (int m = 0; m < 5; m++) gl[m] = lane_up(gx[m], 1);
                 const bool wl = dpp_move<kDppUp>(wf ? 1 : 0) != 0;
                 const bool xdirty = inx && (wf || wl);
-                const bool ydirty = cell && (wcur || wprev);
+                const bool yin = f >= loy && f <= hiy;
+                const bool ydirty = cell && yin && (wcur || wprev);
                 const unsigned long long mx = __ballot(xdirty), my = __ballot(ydirty);
                 if (mx != 0ull) anyx = 1;
                 if (my != 0ull) anyy = 1;
@@ -1229,7 +1233,7 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
 #pragma unroll
                     for (int m = 0; m < 5; m++) {
                         const R xv = (inx && !xdirty) ? (gx[m] - gl[m]) * dx : R(0.0);
-                        fl[(size_t)m * g.nint + oi] = ydirty ? xv : xv + (fh[m] - fprev[m]) * d;
+                        fl[(size_t)m * g.nint + oi] = (ydirty || !yin) ? xv : xv + (fh[m] - fprev[m]) * d;
                     }
                     if (xdirty || ydirty) flags[oi] = (unsigned char)((xdirty ? 1 : 0) | (ydirty ? 2 : 0));
                 }
@@ -2817,12 +2821,9 @@ static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix, int vd = 
 }
 
 // The fused x + y pass (INIT: the stage's right-hand side starts here).
-// Only for a y range of the Euler fluxes that spans all rows; returns
-// hipErrorNotSupported otherwise so the caller runs the two sweeps.
 template <typename R>
 static hipError_t launch_xy(st_solver *s, R *fl, bool fix) {
     const Geo &g = s->g;
-    if (s->kp.eul_lo[1] != 1 || s->kp.eul_hi[1] != g.ny) return hipErrorNotSupported;
     const int ncell = g.ny;
     const long nseg = (g.nx + XW - 1) / XW;
     int chunk = ncell < 32 ? ncell : 32;

@@ -27,7 +27,7 @@ struct st_solver {
     bool dt_in_stage;   // the CFL reduction of this step rides on the first stage's aux
     int visc_chunk;     // planes per tile of the z-marching viscous kernel (0: automatic)
     int march_chunk;    // points per thread of the y/z flux marches (0: automatic)
-    bool fuse_xy;       // x and y Euler fluxes in one pass when the y range spans all rows
+    bool fuse_xy;       // x and y Euler fluxes in one pass
     bool fuse_zv;       // viscous divergence terms folded into the z march (Laplacian form)
     int force_nblk;     // force_rhs partials written in this stage (visc_div blocks, or z march + fix-up)
     double *wallprop;   // 3 x (sx*sz), wallprop(:,:,2:4)
