@@ -187,8 +187,9 @@ def sbli_workload(args):
         res = g.compute_residual()
         kt = g.kernel_times(3)
         # FP64 byte counts; without channel forcing the viscous divergence
-        # pass has no force_rhs inputs (104 B/cell)
-        nbytes = dict(ALGO_BYTES, visc_div=(104.0, "cell"))
+        # terms need no force_rhs inputs (104 B/cell alone, 152 B/cell folded
+        # into the z march)
+        nbytes = dict(ALGO_BYTES, visc_div=(104.0, "cell"), euler_zv=(152.0, "cell"))
         nvol = (nx + 6) * (ny + 6) * (nz + 6)
 
         def gbs(k, ms):

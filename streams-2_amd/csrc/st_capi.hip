@@ -269,13 +269,13 @@ int stage(st_solver *s, int istep) {
         TIMED(ST_K_EULER_Y, launch_euler_y(s, fcur, false));
         TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 1, 1, 0));
     }
-    // euler_z, with the viscous divergence terms (visflx(1)) and the
+    // euler_z, with the viscous divergence terms (visflx(1) or (7)) and the
     // force_rhs partials of the same cells folded into the march where the
-    // z range spans all planes (Laplacian viscous form)
-    const bool zv = s->fuse_zv && s->kp.ndim == 3 && !cons && s->kp.eul_lo[2] == 1 && s->kp.eul_hi[2] == g.nz &&
-                    s->kp.visc_lmax <= 3;
+    // z range spans all planes
+    const bool zv = s->fuse_zv && s->kp.ndim == 3 && s->kp.eul_lo[2] == 1 && s->kp.eul_hi[2] == g.nz &&
+                    (cons || s->kp.visc_lmax <= 3);
     if (zv) {
-        TIMED(ST_K_EULER_ZV, launch_euler_zv(s, fcur, fprev, istep, s->p.channel_case != 0));
+        TIMED(ST_K_EULER_ZV, launch_euler_zv(s, fcur, fprev, istep, s->p.channel_case != 0, cons));
     } else if (s->kp.ndim == 3) {
         TIMED(ST_K_EULER_Z, launch_euler_z(s, fcur, false));
         TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 2, 1, 0));

@@ -654,18 +654,20 @@ __device__ __forceinline__ R face_coeff(const KPar &P, int lm, int l) {
 }
 
 // Operands of the viscous divergence terms folded into the z march (VD > 0):
-// visflx_div_subroutine (kernels_cpu.F90:3484-3532) for the cell the march
-// completes, and with VD = 2 the block partials of force_rhs as k_visc_div
-// forms them.
+// visflx_div_subroutine (kernels_cpu.F90:3484-3532), or with O2 its
+// second-order form visflx_div_ord2_subroutine (:3446-3480), for the cell
+// the march completes, and with VD = 2 the block partials of force_rhs as
+// k_visc_div forms them.
 struct VDArgs {
     const void *flp;  // right-hand side of the previous stage (R)
     const double *w, *dx1, *dy1, *yn;
+    const double *xc, *yc, *zc;  // node coordinates (second-order form)
     const DevScalars *ds;
     double rkr, rkg;
     double *partials;
 };
 
-template <typename R, int DIR, int KM, int VD>
+template <typename R, int DIR, int KM, int VD, bool O2>
 __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__restrict__ wa,
                                                      const int *__restrict__ eoc, const double *__restrict__ metric,
                                                      R *__restrict__ fl, int chunk, int nchunk,
@@ -740,7 +742,7 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__r
                     const R *D = wa + 9 * nv;
                     vmu = wa[6 * nv + cc];
 #pragma unroll
-                    for (int l = 1; l <= 3; l++) {
+                    for (int l = 1; l <= (O2 ? 1 : 3); l++) {
                         dxm[l - 1] = D[cc - l];
                         dxp[l - 1] = D[cc + l];
                         dym[l - 1] = D[cc - l * g.sx];
@@ -816,20 +818,33 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__r
                         const R d = R(metric[f - 1]);
 #pragma unroll
                         for (int m = 0; m < 5; m++) tv[m] = dirty ? flv[m] : flv[m] + (fh[m] - fprev[m]) * d;
-                        R dxv = R(0.0), dyv = R(0.0), dzv = R(0.0);
+                        R sigx, sigy, sigz;
+                        if (O2) {
+                            const int ng = g.ng;
+                            const R dxl = vmu / R(V.xc[i + 1 + ng - 1] - V.xc[i - 1 + ng - 1]);
+                            const R dyl = vmu / R(V.yc[o + 1 + ng - 1] - V.yc[o - 1 + ng - 1]);
+                            const R dzl = vmu / R(V.zc[f + 1 + ng - 1] - V.zc[f - 1 + ng - 1]);
+                            sigx = dxl * (dxp[0] - dxm[0]);
+                            sigy = dyl * (dyp[0] - dym[0]);
+                            sigz = dzl * (dw[4] - dw[2]);
+                        } else {
+                            R dxv = R(0.0), dyv = R(0.0), dzv = R(0.0);
 #pragma unroll
-                        for (int l = 1; l <= 3; l++) {
-                            if (l <= P.visc_lmax) {
-                                const R ccl = R(P.vd1[l - 1]);
-                                dxv = dxv + ccl * (dxp[l - 1] - dxm[l - 1]);
-                                dyv = dyv + ccl * (dyp[l - 1] - dym[l - 1]);
-                                dzv = dzv + ccl * (dw[3 + l] - dw[3 - l]);
+                            for (int l = 1; l <= 3; l++) {
+                                if (l <= P.visc_lmax) {
+                                    const R ccl = R(P.vd1[l - 1]);
+                                    dxv = dxv + ccl * (dxp[l - 1] - dxm[l - 1]);
+                                    dyv = dyv + ccl * (dyp[l - 1] - dym[l - 1]);
+                                    dzv = dzv + ccl * (dw[3 + l] - dw[3 - l]);
+                                }
                             }
+                            dxv = dxv * R(V.dx1[i - 1]);
+                            dyv = dyv * R(V.dy1[o - 1]);
+                            dzv = dzv * R(metric[f - 1]);
+                            sigx = vmu * dxv;
+                            sigy = vmu * dyv;
+                            sigz = vmu * dzv;
                         }
-                        dxv = dxv * R(V.dx1[i - 1]);
-                        dyv = dyv * R(V.dy1[o - 1]);
-                        dzv = dzv * R(metric[f - 1]);
-                        const R sigx = vmu * dxv, sigy = vmu * dyv, sigz = vmu * dzv;
                         const R sigq = sigx * q0.u + sigy * q0.v + sigz * q0.w;
                         tv[1] = tv[1] - sigx;
                         tv[2] = tv[2] - sigy;
@@ -2763,7 +2778,7 @@ hipError_t launch_euler_x(st_solver *s, void *fl, bool init, bool fix) {
 
 template <typename R>
 static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix, int vd = 0, const R *fl_prev = nullptr,
-                               int stage = 0) {
+                               int stage = 0, bool ord2 = false) {
     const Geo &g = s->g;
     const int lo = s->kp.eul_lo[dir], hi = s->kp.eul_hi[dir];
     hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
@@ -2795,20 +2810,25 @@ static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix, int vd = 
         V.rkr = s->rhork[stage];
         V.rkg = s->gamrk[stage];
         V.partials = s->partials;
+        V.xc = s->x;
+        V.yc = s->y;
+        V.zc = s->z;
     }
-#define ST_LAUNCH_M(D, K, VD_, E, M)                                                                               \
-    hipLaunchKernelGGL((k_euler_march<R, D, K, VD_>), grid, dim3(256), 0, s->stream, g, s->kp, (const R *)s->w_aux, \
-                       E, M, fl, chunk, nchunk, s->dirty, V)
-#define ST_LAUNCH_MK(D, VD_, E, M)                           \
+#define ST_LAUNCH_M(D, K, VD_, O2_, E, M)                                                                          \
+    hipLaunchKernelGGL((k_euler_march<R, D, K, VD_, O2_>), grid, dim3(256), 0, s->stream, g, s->kp,                 \
+                       (const R *)s->w_aux, E, M, fl, chunk, nchunk, s->dirty, V)
+#define ST_LAUNCH_MK(D, VD_, O2_, E, M)                      \
     do {                                                     \
-        if (km == 0) ST_LAUNCH_M(D, 0, VD_, E, M);           \
-        else if (km > 0) ST_LAUNCH_M(D, 1, VD_, E, M);       \
-        else ST_LAUNCH_M(D, -1, VD_, E, M);                  \
+        if (km == 0) ST_LAUNCH_M(D, 0, VD_, O2_, E, M);      \
+        else if (km > 0) ST_LAUNCH_M(D, 1, VD_, O2_, E, M);  \
+        else ST_LAUNCH_M(D, -1, VD_, O2_, E, M);             \
     } while (0)
-    if (dir == 1) ST_LAUNCH_MK(1, 0, s->eoc[1], s->detady);
-    else if (vd == 2) ST_LAUNCH_MK(2, 2, s->eoc[2], s->dzitdz);
-    else if (vd == 1) ST_LAUNCH_MK(2, 1, s->eoc[2], s->dzitdz);
-    else ST_LAUNCH_MK(2, 0, s->eoc[2], s->dzitdz);
+    if (dir == 1) ST_LAUNCH_MK(1, 0, false, s->eoc[1], s->detady);
+    else if (vd == 2 && ord2) ST_LAUNCH_MK(2, 2, true, s->eoc[2], s->dzitdz);
+    else if (vd == 2) ST_LAUNCH_MK(2, 2, false, s->eoc[2], s->dzitdz);
+    else if (vd == 1 && ord2) ST_LAUNCH_MK(2, 1, true, s->eoc[2], s->dzitdz);
+    else if (vd == 1) ST_LAUNCH_MK(2, 1, false, s->eoc[2], s->dzitdz);
+    else ST_LAUNCH_MK(2, 0, false, s->eoc[2], s->dzitdz);
 #undef ST_LAUNCH_MK
 #undef ST_LAUNCH_M
     hipError_t e = hipGetLastError();
@@ -2851,12 +2871,12 @@ hipError_t launch_euler_xy(st_solver *s, void *fl, bool fix) { ST_DISPATCH(retur
 hipError_t launch_euler_y(st_solver *s, void *fl, bool fix) { ST_DISPATCH(return launch_march(s, (R *)fl, 1, fix)); }
 hipError_t launch_euler_z(st_solver *s, void *fl, bool fix) { ST_DISPATCH(return launch_march(s, (R *)fl, 2, fix)); }
 
-// z march with the viscous divergence terms (visflx mode 1) of the same
-// cells and, in the channel case, the force_rhs partials; the z fix-up is
-// part of it (its changes to fl(1:2) enter the partials).  Requires the
-// Laplacian viscous form, a z range spanning all planes and visc_lmax <= 3.
-hipError_t launch_euler_zv(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce) {
-    ST_DISPATCH(return launch_march(s, (R *)fl, 2, true, reduce ? 2 : 1, (const R *)fl_prev, stage));
+// z march with the viscous divergence terms (visflx mode 1, or mode 7 with
+// ord2) of the same cells and, in the channel case, the force_rhs partials;
+// the z fix-up is part of it (its changes to fl(1:2) enter the partials).
+// Requires a z range spanning all planes, and visc_lmax <= 3 for mode 1.
+hipError_t launch_euler_zv(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce, bool ord2) {
+    ST_DISPATCH(return launch_march(s, (R *)fl, 2, true, reduce ? 2 : 1, (const R *)fl_prev, stage, ord2));
 }
 
 // The fix-up of the last sweep of direction dir, for callers that time it

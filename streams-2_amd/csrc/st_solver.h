@@ -83,7 +83,7 @@ hipError_t launch_euler_y(st_solver *s, void *fl, bool fix = true);
 hipError_t launch_euler_xy(st_solver *s, void *fl, bool fix = true);
 hipError_t launch_euler_z(st_solver *s, void *fl, bool fix = true);
 hipError_t launch_weno_fix(st_solver *s, void *fl, int dir, int bit, int anyidx);
-hipError_t launch_euler_zv(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce);
+hipError_t launch_euler_zv(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce, bool ord2);
 hipError_t launch_visc(st_solver *s, void *fl, int mode, bool ghost_images);
 hipError_t launch_visc_cons(st_solver *s, void *fl, int mode, bool ghost_images);
 hipError_t launch_sensor(st_solver *s, bool ghost_images);
