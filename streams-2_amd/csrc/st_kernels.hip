@@ -748,10 +748,12 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__r
                         dym[l - 1] = D[cc - l * g.sx];
                         dyp[l - 1] = D[cc + l * g.sx];
                     }
-                    if (VD == 2) {
+                    if (VD == 2 && V.rkr != 0.0) {
                         const R *fp = (const R *)V.flp;
                         flp1 = fp[oic];
                         flp2 = fp[g.nint + oic];
+                    }
+                    if (VD == 2) {
                         w1 = V.w[cc];
                         w2 = V.w[nv + cc];
                         tcell = wa[5 * nv + cc];
@@ -2263,8 +2265,15 @@ __global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restric
         const double dt = ds->dt;
         const double rhodt = rkr * dt, gamdt = rkg * dt;
         double fln[5];
+        // the first Wray stage has rho = 0: the previous right-hand side is
+        // not read there (0 * flp adds nothing to -gamdt * fl)
+        if (rkr != 0.0) {
 #pragma unroll
-        for (int m = 0; m < 5; m++) fln[m] = -rhodt * double(flp[m * ni + oi]) - gamdt * double(fl[m * ni + oi]);
+            for (int m = 0; m < 5; m++) fln[m] = -rhodt * double(flp[m * ni + oi]) - gamdt * double(fl[m * ni + oi]);
+        } else {
+#pragma unroll
+            for (int m = 0; m < 5; m++) fln[m] = -gamdt * double(fl[m * ni + oi]);
+        }
         if (channel) {
             const double b1 = ds->b1, b2 = ds->b2;
             const double uu = double(wa[nv + c]);
