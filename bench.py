@@ -60,7 +60,9 @@ ALGO_BYTES = {
     "euler_zv": (184.0, "cell"),
     "sensor": (24.0, "cell"),    # read w_aux(9,10) + write w_aux(8)
     "visc_div": (144.0, "cell"), # w_aux(2,3,4,7,10) 40 + fl(2:5) rw 64 + fl_prev(1,2) 16 + w(1,2) 16 + T 8
-    "update": (168.0, "cell"),   # fl_prev 40 + fl 40 + w rw 80 + u 8
+    # fl_prev 40 + fl 40 + w rw 80; the first Wray stage (rho = 0) skips fl_prev:
+    # (120 + 160 + 160) / 3 per launch over the three stages of a step
+    "update": (440.0 / 3.0, "cell"),
     "dt": (40.0, "cell"),        # read w(5)
 }
 

@@ -2274,9 +2274,14 @@ __global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restric
 #pragma unroll
             for (int m = 0; m < 5; m++) fln[m] = -gamdt * double(fl[m * ni + oi]);
         }
+        double wo[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) wo[m] = w[m * nv + c];
         if (channel) {
+            // u of the stage's state, formed as k_aux forms w_aux(2) from the
+            // same w (so not read back from w_aux)
             const double b1 = ds->b1, b2 = ds->b2;
-            const double uu = double(wa[nv + c]);
+            const double uu = wo[1] * (1.0 / wo[0]);
             fln[0] = fln[0] - b1;
             fln[1] = fln[1] - b2;
             fln[4] = fln[4] - uu * b2;
@@ -2284,7 +2289,7 @@ __global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restric
         double wn[5];
 #pragma unroll
         for (int m = 0; m < 5; m++) {
-            wn[m] = w[m * nv + c] + fln[m];
+            wn[m] = wo[m] + fln[m];
             w[m * nv + c] = wn[m];
         }
         if (FORCE_VAR) {
