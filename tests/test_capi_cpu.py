@@ -55,3 +55,14 @@ def test_footprint_grows_with_grid():
     assert 0 < fa < fb
     # ~ (15 ghosted + 10 interior) doubles per cell
     assert fa > 8 * 25 * 32 * 24 * 16
+
+
+def test_kernel_families_follow_the_header():
+    """st_kernel_times fills ST_NKFAM slots indexed by the ST_K_* values; the
+    Python names must sit at those indices."""
+    text = open(HEADER).read()
+    nfam = int(re.search(r"#define\s+ST_NKFAM\s+(\d+)", text).group(1))
+    idx = {m.group(1).lower(): int(m.group(2)) for m in re.finditer(r"#define\s+ST_K_([A-Z_]+)\s+(\d+)", text)}
+    assert len(capi.KERNEL_FAMILIES) == nfam == len(idx)
+    for name, i in idx.items():
+        assert capi.KERNEL_FAMILIES[i] == name, (name, i)
