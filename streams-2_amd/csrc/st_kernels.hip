@@ -1192,32 +1192,34 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
             fh[2] = fh[2] + R(0.5) * A0.pp;
             if (f >= c0) {
                 // x faces of row f (k_euler_x), from the lanes' points of that row
-                // pairs (pt-l, pt), l = 3, 2, 1; each face sum is shuffled down
-                // as soon as it is complete, to keep few values live
-                Acc<R> Q1, Q2, T2, T3;
+                // pairs (pt-l, pt), l = 3, 2, 1, from one chain of left
+                // neighbours; the face of lane L gathers Q1(L+1) + Q2(L+2) +
+                // Q3(L+3), shifted down in Horner form ((Q3 + Q2) + Q1): three
+                // lane moves per value instead of six
+                const PcR<R> x1 = shfl_up_pc(q0, 1);
+                const PcR<R> x2 = shfl_up_pc(x1, 1);
+                const PcR<R> x3 = shfl_up_pc(x2, 1);
+                Acc<R> Q1 = acc_zero<R>(), Q2 = acc_zero<R>(), F = acc_zero<R>();
                 {
-                    const Acc<R> p3 = pair_terms<R, 0, KM>(P, shfl_up_pc(q0, 3), q0, pref);
-                    Q1 = acc_zero<R>();
-                    Q2 = acc_zero<R>();
-                    Acc<R> Q3 = acc_zero<R>();
+                    const Acc<R> p3 = pair_terms<R, 0, KM>(P, x3, q0, pref);
                     acc_add(Q1, face_coeff<R>(P, lmA, 3), p3);
                     acc_add(Q2, face_coeff<R>(P, lmB, 3), p3);
-                    acc_add(Q3, face_coeff<R>(P, lmC, 3), p3);
-                    T3 = shfl_down_acc(Q3, 3);
+                    acc_add(F, face_coeff<R>(P, lmC, 3), p3);  // Q3
                 }
                 {
-                    const Acc<R> p2 = pair_terms<R, 0, KM>(P, shfl_up_pc(q0, 2), q0, pref);
+                    const Acc<R> p2 = pair_terms<R, 0, KM>(P, x2, q0, pref);
                     acc_add(Q1, face_coeff<R>(P, lmA, 2), p2);
                     acc_add(Q2, face_coeff<R>(P, lmB, 2), p2);
-                    T2 = shfl_down_acc(Q2, 2);
                 }
+                F = shfl_down_acc(F, 1);
+                F.m += Q2.m; F.mu += Q2.mu; F.mv += Q2.mv; F.mw += Q2.mw; F.e += Q2.e; F.pp += Q2.pp;
                 {
-                    const Acc<R> p1 = pair_terms<R, 0, KM>(P, shfl_up_pc(q0, 1), q0, pref);
+                    const Acc<R> p1 = pair_terms<R, 0, KM>(P, x1, q0, pref);
                     acc_add(Q1, face_coeff<R>(P, lmA, 1), p1);
                 }
-                Acc<R> F = shfl_down_acc(Q1, 1);
-                F.m += T2.m; F.mu += T2.mu; F.mv += T2.mv; F.mw += T2.mw; F.e += T2.e; F.pp += T2.pp;
-                F.m += T3.m; F.mu += T3.mu; F.mv += T3.mv; F.mw += T3.mw; F.e += T3.e; F.pp += T3.pp;
+                F = shfl_down_acc(F, 1);
+                F.m += Q1.m; F.mu += Q1.mu; F.mv += Q1.mv; F.mw += Q1.mw; F.e += Q1.e; F.pp += Q1.pp;
+                F = shfl_down_acc(F, 1);
                 R gx[5];
                 gx[0] = R(0.25) * F.m;
                 gx[1] = R(0.25) * F.mu;
