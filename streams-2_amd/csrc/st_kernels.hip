@@ -646,11 +646,29 @@ __device__ __forceinline__ Acc<R> acc_zero() {
     return a;
 }
 
+// coeff_deriv1(l, lmax) for l, lmax = 1..3, read once per kernel: the reads
+// are uniform, so the table sits in scalar registers, and face_coeff selects
+// among values.  (Selecting among table addresses by a per-lane lm instead
+// turns every face into conditional vector loads from the argument block.)
+template <typename R>
+struct CoefTab {
+    R c[3][3];
+};
+template <typename R>
+__device__ __forceinline__ CoefTab<R> coef_tab(const KPar &P) {
+    CoefTab<R> t;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) t.c[a][b] = R(P.cd1tab[a][b]);
+    return t;
+}
+
 // coeff_deriv1(l, lmax) of a face with effective half width lm (1..3)
 template <typename R>
-__device__ __forceinline__ R face_coeff(const KPar &P, int lm, int l) {
-    if (l > lm) return R(0.0);
-    return R(lm == 3 ? P.cd1tab[2][l - 1] : (lm == 2 ? P.cd1tab[1][l - 1] : P.cd1tab[0][l - 1]));
+__device__ __forceinline__ R face_coeff(const CoefTab<R> &T, int lm, int l) {
+    const R v3 = T.c[2][l - 1], v2 = T.c[1][l - 1], v1 = T.c[0][l - 1];
+    return l > lm ? R(0.0) : (lm == 3 ? v3 : (lm == 2 ? v2 : v1));
 }
 
 // Operands of the viscous divergence terms folded into the z march (VD > 0):
@@ -700,6 +718,7 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__r
     const unsigned smask = ((1u << (2 * ws)) - 1u) << (3 - ws);
     const int p0 = DIR == 1 ? gidx(g, i, 0, o) : gidx(g, i, o, 0);  // offset of along-index 0
     const int lb = P.lmax_base;
+    const CoefTab<R> CT = coef_tab<R>(P);
     auto lm_of = [&](int f) {
         const int ff = f < 0 ? 0 : (f > n ? n : f);
         const int v = lb + eoc[ff];
@@ -775,18 +794,18 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__r
             }
             {
                 const Acc<R> p3 = pair_terms<R, DIR, KM>(P, q0, qb, pref);
-                acc_add(A0, face_coeff<R>(P, lm0, 3), p3);
-                acc_add(A1, face_coeff<R>(P, lm1, 3), p3);
-                acc_add(A2, face_coeff<R>(P, lm2, 3), p3);
+                acc_add(A0, face_coeff<R>(CT, lm0, 3), p3);
+                acc_add(A1, face_coeff<R>(CT, lm1, 3), p3);
+                acc_add(A2, face_coeff<R>(CT, lm2, 3), p3);
             }
             {
                 const Acc<R> p2 = pair_terms<R, DIR, KM>(P, q1, qb, pref);
-                acc_add(A1, face_coeff<R>(P, lm1, 2), p2);
-                acc_add(A2, face_coeff<R>(P, lm2, 2), p2);
+                acc_add(A1, face_coeff<R>(CT, lm1, 2), p2);
+                acc_add(A2, face_coeff<R>(CT, lm2, 2), p2);
             }
             {
                 const Acc<R> p1 = pair_terms<R, DIR, KM>(P, q2, qb, pref);
-                acc_add(A2, face_coeff<R>(P, lm2, 1), p1);
+                acc_add(A2, face_coeff<R>(CT, lm2, 1), p1);
             }
             const int f = b - 3;  // face complete
             if (f >= c0 - 1) {
@@ -980,6 +999,7 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const R *__restr
     const unsigned long long wmask = (1ull << (2 * ws)) - 1ull;
     const R pref = R(P.p_ref);
     const int lb = P.lmax_base;
+    const CoefTab<R> CT = coef_tab<R>(P);
     auto lm_of = [&](int f) {
         const int ff = f < 0 ? 0 : (f > g.nx ? g.nx : f);
         const int v = lb + eoc[ff];
@@ -1018,18 +1038,18 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const R *__restr
         Acc<R> Q1 = acc_zero<R>(), Q2 = acc_zero<R>(), Q3 = acc_zero<R>();
         {
             const Acc<R> p3 = pair_terms<R, 0, KM>(P, q3, q, pref);
-            acc_add(Q1, face_coeff<R>(P, lmA, 3), p3);
-            acc_add(Q2, face_coeff<R>(P, lmB, 3), p3);
-            acc_add(Q3, face_coeff<R>(P, lmC, 3), p3);
+            acc_add(Q1, face_coeff<R>(CT, lmA, 3), p3);
+            acc_add(Q2, face_coeff<R>(CT, lmB, 3), p3);
+            acc_add(Q3, face_coeff<R>(CT, lmC, 3), p3);
         }
         {
             const Acc<R> p2 = pair_terms<R, 0, KM>(P, q2, q, pref);
-            acc_add(Q1, face_coeff<R>(P, lmA, 2), p2);
-            acc_add(Q2, face_coeff<R>(P, lmB, 2), p2);
+            acc_add(Q1, face_coeff<R>(CT, lmA, 2), p2);
+            acc_add(Q2, face_coeff<R>(CT, lmB, 2), p2);
         }
         {
             const Acc<R> p1 = pair_terms<R, 0, KM>(P, q1, q, pref);
-            acc_add(Q1, face_coeff<R>(P, lmA, 1), p1);
+            acc_add(Q1, face_coeff<R>(CT, lmA, 1), p1);
         }
         // face f = pt: pairs from the lanes of points f+1, f+2, f+3
         Acc<R> F = shfl_down_acc(Q1, 1);
@@ -1126,10 +1146,13 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
     const unsigned smask = ((1u << (2 * ws)) - 1u) << (3 - ws);
     const unsigned long long wmask = (1ull << (2 * ws)) - 1ull;
     const int lb = P.lmax_base;
+    const CoefTab<R> CT = coef_tab<R>(P);
+    // every lane of a wave marches the same row index, so the y-face order
+    // is uniform across the wave: scalar table reads below
     auto lmy_of = [&](int f) {
         const int ff = f < 0 ? 0 : (f > n ? n : f);
         const int v = lb + eocy[ff];
-        return v < 1 ? 1 : v;
+        return __builtin_amdgcn_readfirstlane(v < 1 ? 1 : v);
     };
     auto lmx_of = [&](int f) {
         const int ff = f < 0 ? 0 : (f > g.nx ? g.nx : f);
@@ -1138,6 +1161,8 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
     };
     // x-face coefficients of this lane's faces (the same for every row)
     const int lmA = lmx_of(pt - 1), lmB = lmx_of(pt - 2), lmC = lmx_of(pt - 3);
+    const R cA1 = face_coeff<R>(CT, lmA, 1), cA2 = face_coeff<R>(CT, lmA, 2), cA3 = face_coeff<R>(CT, lmA, 3);
+    const R cB2 = face_coeff<R>(CT, lmB, 2), cB3 = face_coeff<R>(CT, lmB, 3), cC3 = face_coeff<R>(CT, lmC, 3);
     const bool cell = lane >= 3 && lane <= 60 && pt <= g.nx;
     const bool inx = cell && pt >= lox && pt <= hix;
     const double dxc = dcsidx[(pt < 1 ? 1 : (pt > g.nx ? g.nx : pt)) - 1];
@@ -1163,18 +1188,18 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
         // y pairs ending at point b (k_euler_march)
         {
             const Acc<R> p3 = pair_terms<R, 1, KM>(P, q0, qb, pref);
-            acc_add(A0, face_coeff<R>(P, lm0, 3), p3);
-            acc_add(A1, face_coeff<R>(P, lm1, 3), p3);
-            acc_add(A2, face_coeff<R>(P, lm2, 3), p3);
+            acc_add(A0, face_coeff<R>(CT, lm0, 3), p3);
+            acc_add(A1, face_coeff<R>(CT, lm1, 3), p3);
+            acc_add(A2, face_coeff<R>(CT, lm2, 3), p3);
         }
         {
             const Acc<R> p2 = pair_terms<R, 1, KM>(P, q1, qb, pref);
-            acc_add(A1, face_coeff<R>(P, lm1, 2), p2);
-            acc_add(A2, face_coeff<R>(P, lm2, 2), p2);
+            acc_add(A1, face_coeff<R>(CT, lm1, 2), p2);
+            acc_add(A2, face_coeff<R>(CT, lm2, 2), p2);
         }
         {
             const Acc<R> p1 = pair_terms<R, 1, KM>(P, q2, qb, pref);
-            acc_add(A2, face_coeff<R>(P, lm2, 1), p1);
+            acc_add(A2, face_coeff<R>(CT, lm2, 1), p1);
         }
         const int f = b - 3;  // y face f complete; row f = the window's oldest point q0
         if (f >= c0 - 1) {
@@ -1202,20 +1227,20 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
                 Acc<R> Q1 = acc_zero<R>(), Q2 = acc_zero<R>(), F = acc_zero<R>();
                 {
                     const Acc<R> p3 = pair_terms<R, 0, KM>(P, x3, q0, pref);
-                    acc_add(Q1, face_coeff<R>(P, lmA, 3), p3);
-                    acc_add(Q2, face_coeff<R>(P, lmB, 3), p3);
-                    acc_add(F, face_coeff<R>(P, lmC, 3), p3);  // Q3
+                    acc_add(Q1, cA3, p3);
+                    acc_add(Q2, cB3, p3);
+                    acc_add(F, cC3, p3);  // Q3
                 }
                 {
                     const Acc<R> p2 = pair_terms<R, 0, KM>(P, x2, q0, pref);
-                    acc_add(Q1, face_coeff<R>(P, lmA, 2), p2);
-                    acc_add(Q2, face_coeff<R>(P, lmB, 2), p2);
+                    acc_add(Q1, cA2, p2);
+                    acc_add(Q2, cB2, p2);
                 }
                 F = shfl_down_acc(F, 1);
                 F.m += Q2.m; F.mu += Q2.mu; F.mv += Q2.mv; F.mw += Q2.mw; F.e += Q2.e; F.pp += Q2.pp;
                 {
                     const Acc<R> p1 = pair_terms<R, 0, KM>(P, x1, q0, pref);
-                    acc_add(Q1, face_coeff<R>(P, lmA, 1), p1);
+                    acc_add(Q1, cA1, p1);
                 }
                 F = shfl_down_acc(F, 1);
                 F.m += Q1.m; F.mu += Q1.mu; F.mv += Q1.mv; F.mw += Q1.mw; F.e += Q1.e; F.pp += Q1.pp;
