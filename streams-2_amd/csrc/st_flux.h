@@ -98,7 +98,12 @@ template <int DIR, int H, class PT, int KM>
 __device__ __forceinline__ void central_flux(const KPar &P, const PT (&q)[2 * H], int lmax, bool zero_flux,
                                              double fh[5]) {
     double ft1 = 0.0, ft2 = 0.0, ft3 = 0.0, ft4 = 0.0, ft5 = 0.0, ft6 = 0.0;
-    const double *cd = P.cd1tab[lmax - 1];
+    // coefficients selected by value: a per-lane row address into the
+    // argument block would make every read a vector load
+    auto cd_of = [&](int l) {
+        const double v3 = P.cd1tab[2][l - 1], v2 = P.cd1tab[1][l - 1], v1 = P.cd1tab[0][l - 1];
+        return lmax >= 3 ? v3 : (lmax == 2 ? v2 : v1);
+    };
     const int nkeep = KM == 0 ? 0 : (KM < 0 ? -1 : P.nkeep);
 #pragma unroll
     for (int l = 1; l <= H; l++) {
@@ -154,7 +159,7 @@ __device__ __forceinline__ void central_flux(const KPar &P, const PT (&q)[2 * H]
                     uvs6 = uvs6 + 2.0 * (a.p + b.p);
                 }
             }
-            const double c = cd[l - 1];
+            const double c = cd_of(l);
             ft1 = ft1 + c * uvs1;
             ft2 = ft2 + c * uvs2;
             ft3 = ft3 + c * uvs3;
