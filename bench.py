@@ -226,7 +226,9 @@ def serving(args):
     (grids uniform in [64,256]x[64,128]x[64,128], bulk Mach 0.3-3.0, Re_tau
     100-500, Poisson arrivals), each advancing ``--serve-steps`` RK3 steps,
     FP64.  Continuous batching through the library's server against the same
-    kind of requests run one after another through the plain path."""
+    kind of requests run one after another through the plain path; then the
+    same stream in the mixed form, with each request's error against its
+    FP64 result."""
     from streams_mi355x.serve import cell_updates, run_sequential, run_serving, serving_workload
 
     reqs = serving_workload(args.serve_requests, args.serve_steps, rate=args.serve_rate, seed=0)
@@ -234,9 +236,30 @@ def serving(args):
     seq_n = min(len(reqs), args.serve_seq_sample)
     seq = run_sequential(reqs[:seq_n])
     progress(f"sequential sample done: {seq['gcups']:.3f} Gcell-updates/s")
-    srv = run_serving(reqs, hbm_bytes=int(args.serve_hbm_gib * (1 << 30)), max_slots=args.serve_slots,
-                      steps_per_graph=10, autotune=True)
+    hbm = int(args.serve_hbm_gib * (1 << 30))
+    srv = run_serving(reqs, hbm_bytes=hbm, max_slots=args.serve_slots, steps_per_graph=10, autotune=True,
+                      want_fields=True)
     progress(f"served: {srv['gcups']:.3f} Gcell-updates/s, p50 {srv['p50_s']:.2f} s, p99 {srv['p99_s']:.2f} s")
+    # the same requests in the mixed form, each compared with its FP64 served
+    # result (bitwise the FP64 plain solve of that request)
+    import numpy as np
+    from streams_mi355x import PREC_MIXED
+    reqs_m = serving_workload(args.serve_requests, args.serve_steps, rate=args.serve_rate, seed=0,
+                              precision=PREC_MIXED)
+    srv_m = run_serving(reqs_m, hbm_bytes=hbm, max_slots=args.serve_slots, steps_per_graph=10, autotune=True,
+                        want_fields=True)
+    errs = []
+    for a, b in zip(srv["results"], srv_m["results"]):
+        if a["error"] is None and b["error"] is None:
+            wa, wb = a["w"][:, 3:-3, 3:-3, 3:-3], b["w"][:, 3:-3, 3:-3, 3:-3]
+            errs.append(max(float(np.linalg.norm(wb[m] - wa[m]) / np.linalg.norm(wa[m])) for m in range(5)))
+    progress(f"served mixed: {srv_m['gcups']:.3f} Gcell-updates/s, p50 {srv_m['p50_s']:.2f} s, "
+             f"p99 {srv_m['p99_s']:.2f} s, worst rel L2 vs FP64 {max(errs):.2e}")
+    mixed = {"value": srv_m["gcups"], "p50_s": srv_m["p50_s"], "p99_s": srv_m["p99_s"],
+             "failed_requests": srv_m["failed"], "speedup_vs_fp64_served": srv_m["gcups"] / srv["gcups"],
+             "rel_l2_vs_fp64_per_request": {"requests": len(errs), "median": float(np.median(errs)),
+                                            "p90": float(np.percentile(errs, 90)), "max": float(max(errs))},
+             "note": "worst conserved variable per request after its steps, against the FP64 served result"}
     return {"requests": len(reqs), "steps_per_request": args.serve_steps, "arrival_rate_per_s": args.serve_rate,
             "value": srv["gcups"], "unit": "Gcell-updates/s per RK stage", "p50_s": srv["p50_s"],
             "p99_s": srv["p99_s"], "mean_queue_s": srv["mean_queue_s"], "makespan_s": srv["makespan_s"],
@@ -246,7 +269,8 @@ def serving(args):
             "sequential_value": seq["gcups"], "sequential_sample": f"first {seq_n} requests, plain path, one by one",
             "speedup_vs_sequential": srv["gcups"] / seq["gcups"], "slots": args.serve_slots,
             "hbm_capacity_gib": args.serve_hbm_gib, "tuned_shape_classes": srv["stats"]["tuned_classes"],
-            "total_cell_updates": cell_updates([r for r, o in zip(reqs, srv["results"]) if o["error"] is None])}
+            "total_cell_updates": cell_updates([r for r, o in zip(reqs, srv["results"]) if o["error"] is None]),
+            "mixed": mixed}
 
 
 def parse():
