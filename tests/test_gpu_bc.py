@@ -86,6 +86,24 @@ def test_bl_trajectory_20_steps(name):
     g.close()
 
 
+@pytest.mark.parametrize("name", ["inflow-outflow-wall-nr", "shock-top"])
+def test_bl_trajectory_laplacian_viscous_form(name):
+    """The Laplacian viscous form (visflx modes 4 + 1) outside the channel:
+    the viscous divergence rides on the z march without force_rhs partials."""
+    c, o, g = _pair(name, conservative_viscous=0)
+    assert c.params.conservative_viscous == 0 and c.params.channel_case == 0
+    o.w[:] = c.w0
+    g.copy_from_field(c.w0)
+    o.init_run()
+    g.init_run()
+    o.advance(20)
+    g.advance(20)
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    assert max(errs) <= TRAJ_TOL, (name, errs)
+    g.close()
+
+
 def _recycling_pair(seed=5):
     from streams_mi355x.cases import bl_mean_planes, recycling_setup
 
