@@ -927,15 +927,16 @@ constexpr int XW = 58;
 
 // Lane moves of the x faces by DPP wave shifts (wave_shr:1 and wave_shl:1
 // of the GFX9 family): lane i receives the value of lane i-1 (i+1); a lane
-// whose source lies outside the wave keeps its own value.  A move by d
-// chains d shifts, so lanes below d (above 63-d) hold values the x faces
-// never read.  These are VALU moves: no LDS round trip on the face chain.
+// whose source lies outside the wave receives 0 (bound_ctrl), which spares
+// the copy that would stage its old value.  A move by d chains d shifts, so
+// lanes below d (above 63-d) hold values the x faces never read.  These are
+// VALU moves: no LDS round trip on the face chain.
 constexpr int kDppUp = 0x138;    // wave_shr:1
 constexpr int kDppDown = 0x130;  // wave_shl:1
 
 template <int CTRL>
 __device__ __forceinline__ int dpp_move(int v) {
-    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true);
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double v) {
