@@ -2294,13 +2294,17 @@ __global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restric
         const double rhodt = rkr * dt, gamdt = rkg * dt;
         double fln[5];
         // the first Wray stage has rho = 0: the previous right-hand side is
-        // not read there (0 * flp adds nothing to -gamdt * fl)
+        // not read there (0 * flp adds nothing to -gamdt * fl).  Both
+        // right-hand sides are read with the non-temporal hint: neither is
+        // read again before a full stage of traffic has passed.
         if (rkr != 0.0) {
 #pragma unroll
-            for (int m = 0; m < 5; m++) fln[m] = -rhodt * double(flp[m * ni + oi]) - gamdt * double(fl[m * ni + oi]);
+            for (int m = 0; m < 5; m++)
+                fln[m] = -rhodt * double(__builtin_nontemporal_load(&flp[m * ni + oi])) -
+                         gamdt * double(__builtin_nontemporal_load(&fl[m * ni + oi]));
         } else {
 #pragma unroll
-            for (int m = 0; m < 5; m++) fln[m] = -gamdt * double(fl[m * ni + oi]);
+            for (int m = 0; m < 5; m++) fln[m] = -gamdt * double(__builtin_nontemporal_load(&fl[m * ni + oi]));
         }
         double wo[5];
 #pragma unroll
