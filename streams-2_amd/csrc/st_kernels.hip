@@ -1663,9 +1663,24 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, R *__restri
     // two plane buffers: plane k is read while plane k+1 is written
     __shared__ R tiles[2][5][VBY + 2 * H][VBX + 2 * H];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = blockIdx.x * VBX + tx + 1;
-    const int j = blockIdx.y * VBY + ty + 1;
-    const int k0 = blockIdx.z * chunk + 1;
+    // Workgroups are dealt round-robin to the 8 XCDs in linear order.  When
+    // the tile rows split evenly, renumber them so that each XCD takes a band
+    // of whole tile rows (every x tile of gridDim.y/8 consecutive y tiles) of
+    // each z chunk: most halo points a tile reads then belong to tiles of the
+    // same XCD and come from its L2 (1.13 -> 1.09 ms on 512x256x256).
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (gridDim.y % 8 == 0) {
+        const int band = gridDim.y / 8;
+        const int per = gridDim.x * band;  // tiles of one XCD in one z chunk
+        const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        const int r = (b >> 3) % per;
+        bz = (b >> 3) / per;
+        bx = r % gridDim.x;
+        by = (b & 7) * band + r / gridDim.x;
+    }
+    const int i = bx * VBX + tx + 1;
+    const int j = by * VBY + ty + 1;
+    const int k0 = bz * chunk + 1;
     const int k1 = min(k0 + chunk - 1, g.nz);
     const bool act = i <= g.nx && j <= g.ny;
     // loads stay inside the ghosted box; only inactive lanes see clamped values

@@ -103,6 +103,34 @@ def test_visflx_parity(mode):
         assert rel_l2(a, b) <= UNIT_TOL, (mode, v)
 
 
+@pytest.mark.parametrize("mode", [4, 0])
+def test_visflx_parity_banded_tiles(mode):
+    """64x128x16: 16 rows of viscous tiles, so the workgroups are renumbered
+    into per-XCD bands of two tile rows each; results must not change."""
+    c, o, g = _pair(nx=64, ny=128, nz=16)
+    _load_state(c, o, g, energy_noise=0.05)
+    o.zero_flux()
+    g.zero_flux()
+    o.visflx(mode)
+    g.visflx(mode)
+    fo, fg = o.fl, g.get_fl()
+    for m in range(1, 5):
+        assert rel_l2(fg[m], fo[m]) <= UNIT_TOL, (mode, m)
+    for v in ([8, 9, 10] if mode == 0 else [9, 10]):
+        a = interior(g.get_aux(v)[None])[0]
+        b = interior(o.w_aux[v - 1][None])[0]
+        assert rel_l2(a, b) <= UNIT_TOL, (mode, v)
+    g.close()
+
+
+def test_trajectory_banded_tiles():
+    c, o, g = _run_pair(5, nx=64, ny=128, nz=16, threshold=0.5)
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
+    assert max(errs) <= TRAJ_TOL, errs
+    g.close()
+
+
 def test_visflx_div_and_sensor_parity():
     c, o, g = _pair()
     _load_state(c, o, g)
