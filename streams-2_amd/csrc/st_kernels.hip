@@ -1649,8 +1649,8 @@ __device__ __forceinline__ VP<R> load_vp(const R *__restrict__ wa, size_t nv, in
 // double-buffered, so one barrier per plane suffices.  ``images`` also
 // writes the ghost images of w_aux(10) (or of the sensor with SENSOR).
 constexpr int VBX = 32, VBY = 8;
-// each lane fills at most one x-halo and one y-halo point
-static_assert(VBX >= 2 * H && VBY >= 2 * H, "tile narrower than two halos");
+// each lane fills at most one halo point of the plane tile
+static_assert(VBX * VBY >= 2 * H * (VBX + VBY), "fewer lanes than halo points");
 
 template <typename R, bool SENSOR>
 __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, R *__restrict__ wa, R *__restrict__ fl,
@@ -1691,16 +1691,27 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, R *__restri
     const int col = gidx(g, ii, jj, 0);  // + k*sz
     // halo point of this lane (x halo for the first/last H lanes of a row,
     // y halo for the first/last H rows), as a column offset + LDS slot
-    const bool hx = tx < H || tx >= VBX - H;
-    const bool hy = ty < H || ty >= VBY - H;
-    const int colx = hx ? gidx(g, tx < H ? i - H : min(i + H, g.nx + H), jj, 0) : col;
-    const int coly = hy ? gidx(g, ii, ty < H ? j - H : min(j + H, g.ny + H), 0) : col;
-    const int lx = tx < H ? tx : tx + 2 * H, ly = ty < H ? ty : ty + 2 * H;
+    // halo point of this lane, as an LDS slot (hly, hlx) and a column
+    // offset: lanes 0 .. 2H*VBX-1 take the y halo rows (H above the tile, H
+    // below), the next 2H*VBY lanes the x halo columns, the rest none
+    const int tid = ty * VBX + tx;
+    const bool hl = tid < 2 * H * (VBX + VBY);
+    int hlx = 0, hly = 0;
+    if (tid < 2 * H * VBX) {
+        const int r = tid / VBX;
+        hly = r < H ? r : r + VBY;
+        hlx = tid % VBX + H;
+    } else {
+        const int q = tid - 2 * H * VBX, c6 = q % (2 * H);
+        hly = q / (2 * H) + H;
+        hlx = c6 < H ? c6 : c6 + VBX;
+    }
+    const int colh = gidx(g, min(bx * VBX + 1 - H + hlx, g.nx + H), min(by * VBY + 1 - H + hly, g.ny + H), 0);
     VP<R> win[2 * H + 1];
 #pragma unroll
     for (int q = 0; q < 2 * H; q++) win[q + 1] = load_vp(wa, nv, col + (k0 - H + q) * sz);
     VP<R> nxt = load_vp(wa, nv, col + (k0 + H) * sz);
-    VP<R> hxv = load_vp(wa, nv, colx + k0 * sz), hyv = load_vp(wa, nv, coly + k0 * sz);
+    VP<R> hv = load_vp(wa, nv, (hl ? colh : col) + k0 * sz);
     const R ax = R(dx1[min(i, g.nx) - 1]), ay = R(dy1[min(j, g.ny) - 1]);
     const R sx_ = R(dxs[min(i, g.nx) - 1]), sy_ = R(dys[min(j, g.ny) - 1]);
     const R qx = R(dx2[min(i, g.nx) - 1]), qy = R(dy2[min(j, g.ny) - 1]);
@@ -1716,13 +1727,9 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, R *__restri
         tile[2][ty + H][tx + H] = c0.w;
         tile[3][ty + H][tx + H] = c0.t;
         tile[4][ty + H][tx + H] = c0.m;
-        if (hx) {
-            tile[0][ty + H][lx] = hxv.u; tile[1][ty + H][lx] = hxv.v; tile[2][ty + H][lx] = hxv.w;
-            tile[3][ty + H][lx] = hxv.t; tile[4][ty + H][lx] = hxv.m;
-        }
-        if (hy) {
-            tile[0][ly][tx + H] = hyv.u; tile[1][ly][tx + H] = hyv.v; tile[2][ly][tx + H] = hyv.w;
-            tile[3][ly][tx + H] = hyv.t; tile[4][ly][tx + H] = hyv.m;
+        if (hl) {
+            tile[0][hly][hlx] = hv.u; tile[1][hly][hlx] = hv.v; tile[2][hly][hlx] = hv.w;
+            tile[3][hly][hlx] = hv.t; tile[4][hly][hlx] = hv.m;
         }
         // right-hand side of this plane, read ahead of the stencil work
         const int oi = iidx(g, min(i, g.nx), min(j, g.ny), k);
@@ -1731,8 +1738,7 @@ __global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, R *__restri
         // loads of plane k+1 overlap the work on plane k
         if (k < k1) {
             nxt = load_vp(wa, nv, col + (k + 1 + H) * sz);
-            if (hx) hxv = load_vp(wa, nv, colx + (k + 1) * sz);
-            if (hy) hyv = load_vp(wa, nv, coly + (k + 1) * sz);
+            if (hl) hv = load_vp(wa, nv, colh + (k + 1) * sz);
         }
         __syncthreads();
         if (act) {
