@@ -1652,8 +1652,11 @@ constexpr int VBX = 32, VBY = 8;
 // each lane fills at most one halo point of the plane tile
 static_assert(VBX * VBY >= 2 * H * (VBX + VBY), "fewer lanes than halo points");
 
+// The FP32 form fits 128 VGPRs without spills under a 4-waves bound (137
+// unbounded, 3 waves/SIMD): 0.630 -> 0.604 ms on 512x256x256.  The FP64 form
+// stays unbounded (209 VGPRs); a 3-waves bound spills 136 B/lane there.
 template <typename R, bool SENSOR>
-__global__ __launch_bounds__(VBX *VBY) void k_visc_zm(Geo g, KPar P, R *__restrict__ wa, R *__restrict__ fl,
+__global__ __launch_bounds__(VBX *VBY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_visc_zm(Geo g, KPar P, R *__restrict__ wa, R *__restrict__ fl,
                                                       const double *__restrict__ dx1, const double *__restrict__ dxs,
                                                       const double *__restrict__ dx2, const double *__restrict__ dy1,
                                                       const double *__restrict__ dys, const double *__restrict__ dy2,
