@@ -24,9 +24,12 @@ _LIB = None
 
 
 def lib():
+    """The oracle library.  STREAMS_ORACLE_LIB selects another build of the
+    same sources (the mutation checks build copies with planted mistakes to
+    show that the pins fail on them)."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = os.environ.get("STREAMS_ORACLE_LIB", os.path.join(_HERE, "liboracle.so"))
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run make -C oracle")
         L = C.CDLL(path)
