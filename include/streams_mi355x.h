@@ -201,6 +201,24 @@ int st_compute_stats(st_solver *s, int32_t itav);
 int st_get_stats(st_solver *s, double *w_stat);
 int st_set_stats(st_solver *s, const double *w_stat);
 
+/* No counterpart in the reference: compute_dt (singleideal_cpu.F90:1476-1517)
+ * without a solver handle, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing: the block maxima live in a static
+ * device array of 257 KiB per GPU, shared by every call on that GPU, so
+ * calls that overlap on one GPU race.  All arrays are device pointers of the
+ * solve described by p: w (5 ghosted variables), w_aux (the solve's w_aux:
+ * double, or float with 11 variables when p->precision is ST_PREC_MIXED;
+ * variable 6, T, is the start of the T(e) Newton iteration and receives T
+ * when calorically_perfect = 0, as compute_aux does), the six metric arrays
+ * of st_set_grid, and dt_out (one double, written as cfl / max of the
+ * spectral radii; untouched when cfl < 0).  Work is enqueued on hip_stream
+ * (NULL = the null stream) and not waited for.  Limit: ceil(nx/64) *
+ * ceil(ny/4) <= 32768 (ST_ERR_ARG beyond). */
+int st_compute_dt_noalloc(const st_params *p, void *hip_stream, const double *w, void *w_aux,
+                          const double *dcsidx, const double *detady, const double *dzitdz,
+                          const double *dcsidxs, const double *detadys, const double *dzitdzs, double *dt_out);
+
 /* Measurement helper, no counterpart in the reference (its only timer is the
  * 'averaged timing' of run(), singleideal_cpu.F90:1661-1666): runs nsteps
  * steps of st_advance with HIP events recorded on the solver's stream around

@@ -12,13 +12,19 @@
 #include "st_solver.h"
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string &msg) {
+namespace st {
+int set_error(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+}  // namespace st
+
+namespace {
+
+int fail(int code, const std::string &msg) { return st::set_error(code, msg); }
 
 #define HIPCHK(expr)                                                                              \
     do {                                                                                          \
@@ -145,9 +151,11 @@ void build_tables(st_solver *s, int *eoc_host[3]) {
     }
 }
 
-void fill_kpar(st_solver *s) {
-    const st_params &p = s->p;
-    st::KPar &k = s->kp;
+}  // namespace
+
+// The KPar fields that follow from the parameters alone (the boundary
+// tables of build_tables come on top for a solver).
+void st::params_to_kpar(const st_params &p, st::KPar &k) {
     k.lmax_base = p.ep_order / 2;
     k.visc_lmax = p.visc_order / 2;
     k.weno_scheme = p.weno_scheme;
@@ -184,6 +192,22 @@ void fill_kpar(st_solver *s) {
     for (int l = 0; l < 4; l++) k.vd1[l] = d1[l];
     for (int l = 0; l < 5; l++) k.vd2[l] = d2[l];
 }
+
+st::Geo st::params_to_geo(const st_params &p) {
+    st::Geo g;
+    g.nx = p.nx; g.ny = p.ny; g.nz = p.nz; g.ng = p.ng;
+    g.sx = p.nx + 2 * p.ng; g.sy = p.ny + 2 * p.ng; g.sz = p.nz + 2 * p.ng;
+    g.sxy = g.sx * g.sy;
+    g.nvol = g.sxy * g.sz;
+    g.nint = p.nx * p.ny * p.nz;
+    return g;
+}
+
+int st::validate_params(const st_params *p) { return validate(p); }
+
+namespace {
+
+void fill_kpar(st_solver *s) { st::params_to_kpar(s->p, s->kp); }
 
 // bytes of one w_aux / right-hand-side value
 size_t rsize(const st_solver *s) { return s->mixed ? sizeof(float) : sizeof(double); }
@@ -336,12 +360,8 @@ int st_solver_create(const st_params *p, st_solver **out) {
     s->p = *p;
     s->fuse_xy = true;
     s->fuse_zv = true;
+    s->g = st::params_to_geo(*p);
     st::Geo &g = s->g;
-    g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.ng = p->ng;
-    g.sx = p->nx + 2 * p->ng; g.sy = p->ny + 2 * p->ng; g.sz = p->nz + 2 * p->ng;
-    g.sxy = g.sx * g.sy;
-    g.nvol = g.sxy * g.sz;
-    g.nint = p->nx * p->ny * p->nz;
     s->rhork[0] = 0.0; s->rhork[1] = -17.0 / 60.0; s->rhork[2] = -5.0 / 12.0;   // Wray RK3,
     s->gamrk[0] = 8.0 / 15.0; s->gamrk[1] = 5.0 / 12.0; s->gamrk[2] = 3.0 / 4.0; // singleideal.F90:4780-4782
     s->last_stage = -1;

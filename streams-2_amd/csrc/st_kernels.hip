@@ -2496,18 +2496,22 @@ __global__ __launch_bounds__(STAT_BX) void k_stats(Geo g, KPar P, const double *
 // 6960-7011, 6814-6880): primitives recomputed from w in registers (w_aux
 // written only for T when cp(T) needs it as the Newton start), block max of
 // the nine spectral radii, atomic max of the block maxima.
+// A workgroup covers 64 x 4 points of (i, j) and walks k from blockIdx.z+1
+// with stride gridDim.z (one plane when gridDim.z = nz).  Workgroup 0
+// clears the ticket of k_dt_finish, which runs after this kernel.
 template <typename R>
 __global__ __launch_bounds__(256) void k_dt(Geo g, KPar P, const double *__restrict__ w, R *__restrict__ wa,
                                             const double *__restrict__ dx1, const double *__restrict__ dy1,
                                             const double *__restrict__ dz1, const double *__restrict__ dxs,
                                             const double *__restrict__ dys, const double *__restrict__ dzs,
-                                            double *__restrict__ partials) {
+                                            double *__restrict__ partials, int *__restrict__ ticket) {
     __shared__ double lds[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    if (flat_block() == 0 && flat_thread() == 0) *ticket = 0;
     double mx = 0.0;
-    if (i <= g.nx && j <= g.ny) {
+    for (int k = blockIdx.z + 1; k <= g.nz; k += gridDim.z) {
+        if (i > g.nx || j > g.ny) break;
         const size_t nv = g.nvol;
         const int c = gidx(g, i, j, k);
         const double rho = w[c];
@@ -2561,7 +2565,7 @@ __device__ __forceinline__ double block_max(double m, double *lds) {
 // slice maxima (max is exact in any order).
 __global__ __launch_bounds__(RED_THREADS) void k_dt_finish(KPar P, const double *__restrict__ partials, int nblk,
                                                            double *__restrict__ stage2, int *__restrict__ ticket,
-                                                           DevScalars *ds) {
+                                                           double *__restrict__ dt_out) {
     __shared__ double lds[RED_THREADS / 64];
     __shared__ bool last;
     const int per = (nblk + gridDim.x - 1) / gridDim.x;
@@ -2582,7 +2586,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_dt_finish(KPar P, const double 
     m = block_max(m, lds);
     if (threadIdx.x == 0) {
         const double dt_min = 1.0 / m;
-        ds->dt = P.cfl * dt_min;
+        *dt_out = P.cfl * dt_min;
         *ticket = 0;
     }
 }
@@ -2770,7 +2774,7 @@ hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !(dt && s->p.cfl > 0)) return e;
     hipLaunchKernelGGL(k_dt_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, (int)nblk,
-                       s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
+                       s->partials + s->npart - RB * 8, s->counters + 1, &s->ds->dt);
     return hipGetLastError();
 }
 
@@ -3109,13 +3113,29 @@ hipError_t launch_force_var(st_solver *s) {
     return launch_wall_ghosts(s);
 }
 
-hipError_t launch_dt(st_solver *s) {
-    const Geo &g = s->g;
-    ST_DISPATCH(hipLaunchKernelGGL(k_dt<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w, (R *)s->w_aux,
-                                   s->dcsidx, s->detady, s->dzitdz, s->dcsidxs, s->detadys, s->dzitdzs, s->partials));
-    hipLaunchKernelGGL(k_dt_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(g),
-                       s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
+int dt_blocks_per_plane(const Geo &g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4); }
+
+hipError_t launch_dt_raw(hipStream_t stream, bool mixed, const Geo &g, const KPar &P, const double *w, void *wa,
+                         const double *dx1, const double *dy1, const double *dz1, const double *dxs,
+                         const double *dys, const double *dzs, double *partials, int kplanes, double *stage2,
+                         int *ticket, double *dt_out) {
+    static_assert(kDtStage2 == RB, "second-level slots of k_dt_finish");
+    const dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, kplanes);
+    if (mixed)
+        hipLaunchKernelGGL(k_dt<float>, grid, kCellBlock, 0, stream, g, P, w, (float *)wa, dx1, dy1, dz1, dxs, dys,
+                           dzs, partials, ticket);
+    else
+        hipLaunchKernelGGL(k_dt<double>, grid, kCellBlock, 0, stream, g, P, w, (double *)wa, dx1, dy1, dz1, dxs, dys,
+                           dzs, partials, ticket);
+    hipLaunchKernelGGL(k_dt_finish, dim3(RB), dim3(RED_THREADS), 0, stream, P, partials,
+                       dt_blocks_per_plane(g) * kplanes, stage2, ticket, dt_out);
     return hipGetLastError();
+}
+
+hipError_t launch_dt(st_solver *s) {
+    return launch_dt_raw(s->stream, s->mixed, s->g, s->kp, s->w, s->w_aux, s->dcsidx, s->detady, s->dzitdz,
+                         s->dcsidxs, s->detadys, s->dzitdzs, s->partials, s->g.nz, s->partials + s->npart - RB * 8,
+                         s->counters + 1, &s->ds->dt);
 }
 
 hipError_t launch_step_end(st_solver *s) {

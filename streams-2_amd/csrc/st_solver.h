@@ -1,6 +1,8 @@
 // st_solver.h -- host-side solver state shared by the C ABI and the launchers.
 #pragma once
 
+#include <string>
+
 #include "st_internal.h"
 
 struct st_solver {
@@ -62,6 +64,24 @@ struct KTimer {
 };
 
 namespace st {
+
+// parameter handling shared by the entry points (st_capi.hip)
+int set_error(int code, const std::string &msg);  // message for st_last_error; returns code
+int validate_params(const st_params *p);
+void params_to_kpar(const st_params &p, KPar &k);
+Geo params_to_geo(const st_params &p);
+
+// CFL reduction over caller-given arrays: k_dt over nx x ny x kplanes
+// workgroup columns (each walks k with stride kplanes) writing one block
+// maximum each into partials, then the two-level maximum into *dt_out.
+// ticket must point at an int the kernels may use; k_dt clears it.
+hipError_t launch_dt_raw(hipStream_t stream, bool mixed, const Geo &g, const KPar &P, const double *w, void *wa,
+                         const double *dx1, const double *dy1, const double *dz1, const double *dxs,
+                         const double *dys, const double *dzs, double *partials, int kplanes, double *stage2,
+                         int *ticket, double *dt_out);
+// workgroups of one k plane of k_dt; k_dt_finish's second-level slots
+int dt_blocks_per_plane(const Geo &g);
+constexpr int kDtStage2 = 128;
 
 // launchers (st_kernels.hip); all enqueue on s->stream and return a hipError_t
 hipError_t launch_fill_periodic(st_solver *s, double *arr, int nv);

@@ -53,6 +53,8 @@ _SIGNATURES = {
                          + [C.POINTER(C.c_double)] * 4),
     "st_init_run": (C.c_int, [C.c_void_p]),
     "st_compute_dt": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "st_compute_dt_noalloc": (C.c_int, [C.POINTER(StParams), C.c_void_p, C.c_void_p, C.c_void_p]
+                              + [C.c_void_p] * 7),
     "st_rk_sync": (C.c_int, [C.c_void_p]),
     "st_advance": (C.c_int, [C.c_void_p, C.c_int32]),
     "st_compute_residual": (C.c_int, [C.c_void_p, C.POINTER(StScalars)]),
