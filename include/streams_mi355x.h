@@ -63,10 +63,15 @@ typedef struct st_params {
     int32_t visc_model;              /* 0 inviscid, 1 power law, 2 Sutherland             */
     int32_t calorically_perfect;     /* 1 constant cp, 0 cp(T) polynomial                 */
     int32_t indx_cp_l, indx_cp_r;    /* exponent range of the cp polynomial               */
-    int32_t bctags[6];               /* xmin,xmax,ymin,ymax,zmin,zmax: 0 periodic, 1 free
-                                        stream, 2 extrapolation, 5 symmetry (ymin), 6 wall
-                                        (ymin; ymax too when staggered), 7 oblique shock (ymax),
-                                        10 recycling-rescaling inflow (xmin) */
+    int32_t bctags[6];               /* xmin,xmax,ymin,ymax,zmin,zmax (update_ghost,
+                                        singleideal_cpu.F90:1064-1138): 0 periodic, 1 free stream,
+                                        2 extrapolation, 3 subsonic free stream from total
+                                        conditions (xmin, ymax), 4 extrapolation at p0 (all but
+                                        ymin), 5 symmetry (ymin), 6 isothermal wall (ymin; ymax
+                                        too when staggered), 7 oblique shock (ymax), 8 none,
+                                        9 laminar inflow from wmean (xmin), 10 recycling-rescaling
+                                        inflow (xmin).  On the faces not listed a tag leaves the
+                                        ghosts alone, as in the reference. */
     int32_t bctags_nr[6];            /* characteristic BC: 0 none, 1 non-reflecting, 2 relaxation
                                         to the neighbour, 3 relaxation to winf, 6 reflective wall */
     int32_t is_y_staggered;          /* 1 for the channel (cell-centred y)                */
@@ -74,7 +79,11 @@ typedef struct st_params {
     int32_t correct_bound_ord;       /* 1 lowers the stencil order at non-periodic faces  */
     int32_t iter_dt_recompute;       /* CFL dt recomputed every this many steps           */
     int32_t precision;               /* ST_PREC_FP64 or ST_PREC_MIXED                     */
-    int32_t pad0, pad1;              /* explicit: the record has no implicit padding      */
+    int32_t rand_type;               /* random inflow plane of the recycling inflow: 0 off,
+                                        > 0 reproducible (C library rand() seeded 1, the
+                                        single-rank seed rank+1), < 0 seeded from the clock
+                                        (singleideal_cpu.F90:1585-1594) */
+    int32_t pad1;                    /* explicit: the record has no implicit padding      */
     double rgas0, prandtl, t0, rho0, u0, l0, mu0;
     double sutherland_s, t_ref_dim, powerlaw_vtexp;
     double sensor_threshold;
@@ -92,6 +101,10 @@ typedef struct st_params {
      * (singleideal.F90:2798-2826 set_flow_params, :2955-3017 set_oblique_shock) */
     double winf[5], winf_past_shock[5];
     double xshock_imp, shock_angle, tanhfacs;
+    /* total conditions of the subsonic free-stream boundary (bctags 3):
+     * angle of attack in radians, ptot0 and ttot0 (set_flow_params,
+     * singleideal.F90:2807-2814) */
+    double aoa, ptot0, ttot0;
 } st_params;
 
 /* Device-resident scalars of the run, copied to the host on request. */
@@ -147,18 +160,65 @@ int st_compute_aux(st_solver *s, int32_t central, int32_t ghost);
  *   conservative form (conservative_viscous=1): 2 cell terms + sensor,
  *   5 face terms, 6 cell terms, 7 second-order div/3 terms. */
 int st_visflx(st_solver *s, int32_t mode);
-/* euler_x/y/z (singleideal_cpu.F90:674-843): flux + divergence into fl. */
-int st_euler_x(st_solver *s);
-int st_euler_y(st_solver *s);
-int st_euler_z(st_solver *s);
+/* euler_x(istart, iend, istart_face, iend_face, do_update)
+ * (singleideal_cpu.F90:674-747): when iend >= istart, the hybrid fluxes of
+ * faces istart_face..iend_face (the reference's defaults are istart-1 and
+ * iend) and, with do_update != 0, their divergence into fl over cells
+ * eul_imin..eul_imax.  Face fluxes persist between calls, so a sweep split
+ * into ranges (as rk_async splits it) updates from all of them.
+ * euler_y(eul_jmin, eul_jmax) / euler_z(eul_kmin, eul_kmax) (:749-843):
+ * faces eul_min-1..eul_max and cells eul_min..eul_max; euler_z also redraws
+ * the random inflow plane of a recycling inflow.  The ranges of bc_preproc
+ * (st_get_eul_range) run the fused sweeps; any other range runs the face
+ * kernels of the reference's decomposition. */
+int st_euler_x(st_solver *s, int32_t istart, int32_t iend, int32_t istart_face, int32_t iend_face,
+               int32_t do_update);
+int st_euler_y(st_solver *s, int32_t eul_jmin, int32_t eul_jmax);
+int st_euler_z(st_solver *s, int32_t eul_kmin, int32_t eul_kmax);
+/* eul_imin, eul_imax, eul_jmin, eul_jmax, eul_kmin, eul_kmax of bc_preproc
+ * (singleideal.F90:4703-4750). */
+int st_get_eul_range(st_solver *s, int32_t range[6]);
+
+/* The remaining procedures of one rk_sync stage (singleideal_cpu.F90:
+ * 352-406), for a backend that calls them one by one instead of st_rk_sync:
+ * init_flux (fln = -rhodt*fl, fl = 0; kernels_cpu.F90:29-47), update_flux
+ * (fln -= gamdt*fl; :3400-3418), update_field (w += fln; :3422-3442),
+ * force_rhs (bulk sums of fln, w and T -> dpdx, rho_b, u_b, T_b, then the
+ * forcing of fln; singleideal_cpu.F90:845-889), force_var (bulk-temperature
+ * control when theta_wall >= -1; :890-933), the dpdx = 0 of rk_sync's step
+ * start (:347-350), bcextr_var and bcswap_var of one w_aux variable
+ * (:411-426, base_cpu.F90:1835-1912) and recyc_exchange (the recycling
+ * station planes with the half-span shift; :935-999).  rhodt and gamdt are
+ * rhork*dt and gamrk*dt of the stage.  fl is the right-hand side the sweeps
+ * and visflx accumulate into. */
+int st_init_flux(st_solver *s, double rhodt);
+int st_update_flux(st_solver *s, double gamdt);
+int st_update_field(st_solver *s);
+int st_force_rhs(st_solver *s);
+int st_force_var(st_solver *s);
+int st_reset_forcing(st_solver *s);
+int st_bcextr_var(st_solver *s, int32_t var);
+int st_bcswap_var(st_solver *s, int32_t var);
+int st_recyc_exchange(st_solver *s);
 /* Recycling-rescaling inflow of face xmin (bctags[0] == 10): the data of
  * recyc_prepare (singleideal.F90:1512-1588).  i_recyc is the station (cell
  * index >= ng), betarecyc / glund1 the rescaling factors, wmean_inflow the
  * inflow mean profile wmean(1-ng:0, 1:ny, 1:4) in Fortran order, weta the
  * blending weight (1:ny), map_j_inn / map_j_out the locateval maps (1:ny;
  * 1..ny-1 where used), the four coordinate arrays y+ and eta of the inflow
- * and of the station over 1-ng:ny+ng.  Without the random inflow plane
- * (rand_type = 0).  Required before the first step of such a solve. */
+ * and of the station over 1-ng:ny+ng.  With rand_type != 0 the random
+ * inflow plane (rows 2..jbl_inflow, jbl_inflow = locateval(y(1:ny), l0))
+ * is redrawn on the device after every z sweep in the order of
+ * get_crandom_f (j, then k, then component), bit-identical to the C
+ * library's rand() seeded as the reference seeds it, and adds
+ * 0.02*u0*(r - 0.5)*min(eta, 1) to the recycled velocity fluctuations
+ * (kernels_cpu.F90:5879-5923).  Required before the first step of such a
+ * solve. */
+/* The inflow mean planes wmean(1-ng:0, 1:ny, 1:4) (rho, rho*u, rho*v,
+ * rho*w) in Fortran order, offset 0 = plane 1-ng: the data of the laminar
+ * inflow (bctags[0] == 9, bclam_subroutine, kernels_cpu.F90:5951-5985).
+ * st_set_recycling sets them too. */
+int st_set_wmean(st_solver *s, const double *wmean);
 int st_set_recycling(st_solver *s, int32_t i_recyc, double betarecyc, double glund1, const double *wmean_inflow,
                      const double *weta, const int32_t *map_j_inn, const int32_t *map_j_out,
                      const double *yplus_inflow, const double *eta_inflow, const double *yplus_recyc,

@@ -74,6 +74,11 @@ def lib():
         L.or_advance.argtypes = [vp, C.c_int]
         L.or_count_weno.argtypes = [vp, dp]
         L.or_compute_stats.argtypes = [vp, C.c_int, dp]
+        L.or_inflow_plane.restype = dp
+        L.or_inflow_plane.argtypes = [vp]
+        L.or_set_wmean.restype = C.c_int
+        L.or_set_wmean.argtypes = [vp, dp]
+        L.or_seed_random.argtypes = [vp]
         L.or_set_recycling.restype = C.c_int
         L.or_set_recycling.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, dp, ip, ip, dp, dp, dp, dp]
         _LIB = L
@@ -153,6 +158,19 @@ class Oracle:
                                      *[_dp(a) for a in keep[2:]])
         if rc:
             raise ValueError("bad recycling data")
+
+    def set_wmean(self, wmean: np.ndarray):
+        self._wm = np.ascontiguousarray(wmean, dtype=np.float64)
+        if self.L.or_set_wmean(self.h, _dp(self._wm)):
+            raise MemoryError("wmean allocation failed")
+
+    def inflow_plane(self) -> np.ndarray:
+        """inflow_random_plane(1:ny, 1:nz, 1:3) as an array [c, k, j] (a view)."""
+        ptr = self.L.or_inflow_plane(self.h)
+        if not ptr:
+            raise RuntimeError("no recycling inflow set")
+        p = self.p
+        return np.ctypeslib.as_array(ptr, shape=(3 * p.ny * p.nz,)).reshape((3, p.nz, p.ny))
 
     def compute_stats(self, itav: int, w_stat: np.ndarray):
         """compute_stats: fold this state's spanwise means into the running

@@ -40,6 +40,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 /* ---------------------------------------------------------------- indexing */
 static inline size_t G(const or_state *o, int i, int j, int k)
@@ -212,6 +213,7 @@ void or_destroy(or_state *o)
     if (!o) return;
     free(o->w); free(o->w_aux); free(o->fl); free(o->fln); free(o->fhat); free(o->wallprop);
     free(o->rc_wmean); free(o->rc_weta); free(o->rc_inn); free(o->rc_out); free(o->rc_y); free(o->rc_wrecyc);
+    free(o->rc_plane);
     free(o->eoc); free(o->x); free(o->y); free(o->z); free(o->yn);
     free(o->dcsidx); free(o->dcsidxs); free(o->dcsidx2);
     free(o->detady); free(o->detadys); free(o->detady2);
@@ -594,6 +596,55 @@ static void bcshock(or_state *o, int ilat)
  * caller: station i_recyc, rescaling factors, the inflow mean profile
  * wmean(1-ng:0, 1:ny, 1:4), blending weights, the y maps between inflow
  * and recycling station and the y+ / eta coordinates of both. */
+/* wmean(1-ng:0, 1:ny, 1:4) of the inflow, Fortran order (offset 0 is plane
+ * 1-ng): the data of bclam and of the recycling inflow. */
+double *or_inflow_plane(or_state *o) { return o->rc_plane; }
+
+int or_set_wmean(or_state *o, const double *wmean)
+{
+    const size_t n = (size_t)o->ng * o->ny * 4;
+    if (!o->rc_wmean) o->rc_wmean = (double *)calloc(n, sizeof(double));
+    if (!o->rc_wmean) return 1;
+    memcpy(o->rc_wmean, wmean, sizeof(double) * n);
+    return 0;
+}
+
+/* init_crandom_f (crandom_f.F90, crandom.c) as run() calls it before the
+ * integration loop (singleideal_cpu.F90:1585-1594): seed rank+1 = 1 for one
+ * rank when rand_type > 0, a clock-derived seed when rand_type < 0.  The C
+ * library's generator is used through random_r with a private state, whose
+ * sequence is that of srand(seed)/rand() (TYPE_3, 128-byte state). */
+void or_seed_random(or_state *o)
+{
+    struct random_data *rd = (struct random_data *)o->rng_data;
+    _Static_assert(sizeof(struct random_data) <= sizeof(o->rng_data), "random_data does not fit");
+    memset(o->rng_data, 0, sizeof(o->rng_data));
+    unsigned seed = 1;
+    if (o->p.rand_type < 0) seed = (unsigned)time(NULL) + 1u;
+    initstate_r(seed, o->rng_state, sizeof(o->rng_state), rd);
+    srandom_r(seed, rd);
+}
+
+/* inflow_random_plane(2:jbl_inflow, 1:nz, 1:3) redrawn by get_crandom_f
+ * after the z sweep (singleideal_cpu.F90:841): j outermost, component
+ * innermost (get_crandom_f_array3d, crandom_f.F90), each rand()/RAND_MAX;
+ * jbl_inflow = locateval(y(1:ny), ny, l0) (singleideal.F90:3284). */
+static void draw_inflow_plane(or_state *o)
+{
+    if (o->p.rand_type == 0 || !o->rc_plane) return;  /* rescale 0: the term vanishes */
+    int ny = o->ny, nz = o->nz, jbl = 0;
+    for (int j = 1; j <= ny; j++) if (o->p.l0 > YC(j)) jbl = j;
+    struct random_data *rd = (struct random_data *)o->rng_data;
+    for (int j = 2; j <= jbl; j++)
+        for (int k = 1; k <= nz; k++)
+            for (int c = 1; c <= 3; c++) {
+                int32_t r;
+                random_r(rd, &r);
+                o->rc_plane[(size_t)(j - 1) + (size_t)ny * ((size_t)(k - 1) + (size_t)nz * (c - 1))] =
+                    (double)r / 2147483647.0;
+            }
+}
+
 int or_set_recycling(or_state *o, int i_recyc, double betarecyc, double glund1, const double *wmean,
                      const double *weta, const int *map_inn, const int *map_out, const double *yplus_inflow,
                      const double *eta_inflow, const double *yplus_recyc, const double *eta_recyc)
@@ -603,8 +654,20 @@ int or_set_recycling(or_state *o, int i_recyc, double betarecyc, double glund1, 
     o->i_recyc = i_recyc;
     o->betarecyc = betarecyc;
     o->glund1 = glund1;
+    if (!o->rc_plane) {
+        o->rc_plane = (double *)malloc(sizeof(double) * (size_t)ny * nz * 3);
+        for (size_t q = 0; q < (size_t)ny * nz * 3; q++) o->rc_plane[q] = 0.5;
+        or_seed_random(o);
+    }
     if (!o->rc_wmean) {
         o->rc_wmean = (double *)calloc((size_t)ng * ny * 4, sizeof(double));
+        o->rc_weta = (double *)calloc(ny, sizeof(double));
+        o->rc_inn = (int *)calloc(ny, sizeof(int));
+        o->rc_out = (int *)calloc(ny, sizeof(int));
+        o->rc_y = (double *)calloc(4 * (size_t)ny2, sizeof(double));
+        o->rc_wrecyc = (double *)calloc((size_t)ng * ny * nz * 5, sizeof(double));
+    }
+    if (!o->rc_weta) {
         o->rc_weta = (double *)calloc(ny, sizeof(double));
         o->rc_inn = (int *)calloc(ny, sizeof(int));
         o->rc_out = (int *)calloc(ny, sizeof(int));
@@ -640,10 +703,11 @@ static void recyc_exchange(or_state *o)
 }
 
 /* bcrecyc (singleideal_cpu.F90:1140-1176) with bcrecyc_subroutine_1/2/3
- * (kernels_cpu.F90:5810-5947) for rand_type = 0 (no random inflow plane):
- * spanwise Favre averages of the recycled planes, their fluctuations
- * interpolated to the inflow y (inner y+ and outer eta scaling blended by
- * weta) and added, rescaled, to the inflow mean profile. */
+ * (kernels_cpu.F90:5810-5947): spanwise Favre averages of the recycled
+ * planes, their fluctuations interpolated to the inflow y (inner y+ and
+ * outer eta scaling blended by weta) and added, rescaled, to the inflow
+ * mean profile; with rand_type != 0 plus 0.02*u0*(r - 0.5)*min(eta, 1) from
+ * the random inflow plane on each velocity fluctuation. */
 static void bcrecyc(or_state *o)
 {
     const st_params *p = &o->p;
@@ -675,9 +739,12 @@ static void bcrecyc(or_state *o)
     const double *ypi = o->rc_y, *eti = o->rc_y + ny2, *ypr = o->rc_y + 2 * ny2, *etr = o->rc_y + 3 * ny2;
 #define YI(a, j) a[(j) + ng - 1]
     const double p0 = p->rho0 * p->rgas0 * p->t0;
+    const double u0_02 = p->rand_type == 0 ? 0.0 : 0.02 * p->u0;
+#define RP(j, k, c) o->rc_plane[(size_t)((j)-1) + (size_t)ny * ((size_t)((k)-1) + (size_t)nz * ((c)-1))]
     for (int k = 1; k <= nz; k++)
         for (int j = 1; j <= ny; j++) {
             double eta = YI(eti, j);
+            double etamin = dmin(eta, 1.0);
             double weta = o->rc_weta[j - 1];
             double weta1 = 1.0 - weta;
             double bdamp = 0.5 * (1.0 - tanh(4.0 * (YI(eti, j) - 2.0)));
@@ -688,10 +755,10 @@ static void bcrecyc(or_state *o)
                 disty_out = (eta - YI(etr, j_out)) / (YI(etr, j_out + 1) - YI(etr, j_out));
             }
             for (int i = 1; i <= ng; i++) {
-                double rhomean = o->rc_wmean[(size_t)(i - 1) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 0)];
-                double uumean = o->rc_wmean[(size_t)(i - 1) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 1)] / rhomean;
-                double vvmean = o->rc_wmean[(size_t)(i - 1) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 2)] / rhomean;
-                double wwmean = o->rc_wmean[(size_t)(i - 1) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 3)] / rhomean;
+                double rhomean = o->rc_wmean[(size_t)(ng - i) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 0)];
+                double uumean = o->rc_wmean[(size_t)(ng - i) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 1)] / rhomean;
+                double vvmean = o->rc_wmean[(size_t)(ng - i) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 2)] / rhomean;
+                double wwmean = o->rc_wmean[(size_t)(ng - i) + (size_t)ng * ((size_t)(j - 1) + (size_t)ny * 3)] / rhomean;
                 double rhofluc, ufluc, vfluc, wfluc;
                 if (j == 1 || j_inn >= ny || j_out >= ny) {
                     rhofluc = 0.0; ufluc = 0.0; vfluc = 0.0; wfluc = 0.0;
@@ -709,6 +776,9 @@ static void bcrecyc(or_state *o)
                     ufluc = ufluc * bdamp * o->betarecyc;
                     vfluc = vfluc * bdamp * o->betarecyc;
                     wfluc = wfluc * bdamp * o->betarecyc;
+                    ufluc = ufluc + u0_02 * (RP(j, k, 1) - 0.5) * etamin;
+                    vfluc = vfluc + u0_02 * (RP(j, k, 2) - 0.5) * etamin;
+                    wfluc = wfluc + u0_02 * (RP(j, k, 3) - 0.5) * etamin;
                 }
                 rhofluc = dmax(-0.75 * rhomean, rhofluc);
                 rhofluc = dmin(4.0 * rhomean, rhofluc);
@@ -725,6 +795,101 @@ static void bcrecyc(or_state *o)
             }
         }
 #undef YI
+#undef RP
+}
+
+/* bcfree_sub_subroutine (kernels_cpu.F90:6060-6155), xmin and ymax only:
+ * the boundary cell's pressure with the total conditions ptot0, ttot0
+ * gives the Mach number, T and the velocity along the angle of attack;
+ * every ghost of the line takes that state. */
+static void bcfree_sub(or_state *o, int ilat)
+{
+    const st_params *p = &o->p;
+    if (ilat != 1 && ilat != 4) return;
+    const double cosangle = cos(p->aoa), sinangle = sin(p->aoa);
+    int na, nb;
+    face_extent(o, ilat, &na, &nb);
+    for (int b = 1; b <= nb; b++)
+        for (int a = 1; a <= na; a++) {
+            const size_t c = boundary_at(o, ilat, a, b);
+            double rho = o->w[c], rhou = o->w[o->nvol + c], rhov = o->w[2 * o->nvol + c];
+            double rhow = o->w[3 * o->nvol + c], rhoe = o->w[4 * o->nvol + c];
+            double ri = 1.0 / rho;
+            double uu = rhou * ri, vv = rhov * ri, ww = rhow * ri;
+            double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+            double ee = rhoe / rho - qq;
+            double tt = or_get_temperature_from_e(o, ee, o->w_aux[5 * o->nvol + c]);
+            double pp = rho * tt * p->rgas0;
+            double gamloc = or_get_gamloc(o, tt);
+            double del = 0.5 * (gamloc - 1.0);
+            double rmfac = pow(pp / p->ptot0, -(gamloc - 1.0) / gamloc);
+            double rml = sqrt((rmfac - 1.0) / del);
+            tt = p->ttot0 / rmfac;
+            double vel_mod = rml * sqrt(gamloc * p->rgas0 * tt);
+            uu = vel_mod * cosangle;
+            vv = vel_mod * sinangle;
+            ww = 0.0;
+            rho = pp / tt / p->rgas0;
+            rhou = rho * uu; rhov = rho * vv; rhow = rho * ww;
+            ee = or_get_e_from_temperature(o, tt);
+            for (int l = 1; l <= o->ng; l++) {
+                const size_t d = ghost_at(o, ilat, a, b, l);
+                o->w[d] = rho;
+                o->w[o->nvol + d] = rhou;
+                o->w[2 * o->nvol + d] = rhov;
+                o->w[3 * o->nvol + d] = rhow;
+                o->w[4 * o->nvol + d] = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
+            }
+        }
+}
+
+/* bcextr_sub_subroutine (kernels_cpu.F90:5064-5164), every face but ymin:
+ * density and momentum of the boundary cell, temperature p0/(rho R). */
+static void bcextr_sub(or_state *o, int ilat)
+{
+    const st_params *p = &o->p;
+    if (ilat == 3) return;
+    const double p0 = p->rho0 * p->rgas0 * p->t0;
+    int na, nb;
+    face_extent(o, ilat, &na, &nb);
+    for (int b = 1; b <= nb; b++)
+        for (int a = 1; a <= na; a++)
+            for (int l = 1; l <= o->ng; l++) {
+                const size_t c = boundary_at(o, ilat, a, b), d = ghost_at(o, ilat, a, b, l);
+                double rho = o->w[c], rhou = o->w[o->nvol + c], rhov = o->w[2 * o->nvol + c];
+                double rhow = o->w[3 * o->nvol + c];
+                double tt = p0 / rho / p->rgas0;
+                double ee = or_get_e_from_temperature(o, tt);
+                o->w[d] = rho;
+                o->w[o->nvol + d] = rhou;
+                o->w[2 * o->nvol + d] = rhov;
+                o->w[3 * o->nvol + d] = rhow;
+                o->w[4 * o->nvol + d] = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
+            }
+}
+
+/* bclam_subroutine (kernels_cpu.F90:5951-5985), xmin only: ghost plane 1-l
+ * takes wmean(1-l, j, 1:4) with T = p0/(rho R). */
+static void bclam(or_state *o, int ilat)
+{
+    const st_params *p = &o->p;
+    if (ilat != 1 || !o->rc_wmean) return;
+    const int ng = o->ng, ny = o->ny;
+    const double p0 = p->rho0 * p->rgas0 * p->t0;
+    for (int k = 1; k <= o->nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int l = 1; l <= ng; l++) {
+                const double *wm = o->rc_wmean + (size_t)(ng - l) + (size_t)ng * (size_t)(j - 1);
+                const size_t st = (size_t)ng * ny;
+                double rho = wm[0], rhou = wm[st], rhov = wm[2 * st], rhow = wm[3 * st];
+                double tt = p0 / rho / p->rgas0;
+                W(1 - l, j, k, 1) = rho;
+                W(1 - l, j, k, 2) = rhou;
+                W(1 - l, j, k, 3) = rhov;
+                W(1 - l, j, k, 4) = rhow;
+                double ee = or_get_e_from_temperature(o, tt);
+                W(1 - l, j, k, 5) = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
+            }
 }
 
 /* update_ghost, singleideal_cpu.F90:1064-1138: the recycled planes are
@@ -738,6 +903,9 @@ void or_update_ghost(or_state *o, int do_swap)
         switch (o->p.bctags[ilat - 1]) {
         case 1: bcfree(o, ilat); break;
         case 2: bcextr(o, ilat); break;
+        case 3: bcfree_sub(o, ilat); break;
+        case 4: bcextr_sub(o, ilat); break;
+        case 9: bclam(o, ilat); break;
         case 5: bcsym(o, ilat); break;
         case 6:
             if (o->p.is_y_staggered) bcwall_staggered(o, ilat);
@@ -1429,6 +1597,7 @@ void or_euler_z(or_state *o)
 {
     or_euler_fluxes(o, 2, o->eul_kmin - 1, o->eul_kmax);
     or_euler_update(o, 2, o->eul_kmin, o->eul_kmax);
+    if (o->p.bctags[0] == 10) draw_inflow_plane(o);
 }
 
 /* ----------------------------------------------------------------- viscous */
@@ -1946,6 +2115,7 @@ void or_init_run(or_state *o)
     or_bcswap_var(o, 8);
     or_zero_flux(o);
     or_compute_dt(o);
+    if (o->rc_plane) or_seed_random(o);
 }
 
 /* compute_residual_subroutine (kernels_cpu.F90:6682-6701),

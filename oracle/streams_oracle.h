@@ -44,7 +44,10 @@ typedef struct or_state {
     /* recycling-rescaling inflow (bctags(1) == 10) */
     int recyc_set, i_recyc;
     double betarecyc, glund1;
-    double *rc_wmean;        /* wmean(1-i, j, m): [(i-1) + ng*((j-1) + ny*(m-1))] */
+    double *rc_wmean;        /* wmean(1-i, j, m): [(ng-i) + ng*((j-1) + ny*(m-1))] */
+    double *rc_plane;        /* inflow_random_plane(1:ny, 1:nz, 1:3), starts at 0.5 */
+    char rng_state[128];     /* C library random_r state of get_crandom (TYPE_3) */
+    double rng_data[8];      /* struct random_data of the C library, kept opaque here */
     double *rc_weta;         /* weta_inflow(1:ny)                         */
     int *rc_inn, *rc_out;    /* map_j_inn / map_j_out (1:ny)              */
     double *rc_y;            /* yplus_inflow, eta_inflow, yplus_recyc, eta_recyc (1-ng:ny+ng) */
@@ -71,6 +74,7 @@ double *or_fln(or_state *o);
 double *or_fhat(or_state *o);
 double *or_wallprop(or_state *o);
 int *or_eoc(or_state *o);
+double *or_inflow_plane(or_state *o);  /* inflow_random_plane(1:ny,1:nz,1:3), NULL before or_set_recycling */
 void or_get_scalars(const or_state *o, st_scalars *out);
 void or_set_time(or_state *o, double time, double dt, int64_t icyc);
 void or_rk_coeffs(const or_state *o, double rho[3], double gam[3]);
@@ -94,6 +98,10 @@ void or_bcswap_var(or_state *o, int var);
 void or_bcextr_var(or_state *o, int var);
 void or_update_ghost(or_state *o, int do_swap);
 void or_bc_nr(or_state *o);
+int or_set_wmean(or_state *o, const double *wmean);
+/* init_crandom_f of run() (singleideal_cpu.F90:1585-1594): seed the draws of
+ * the random inflow plane (rand_type > 0: seed 1 = rank+1 of one rank) */
+void or_seed_random(or_state *o);
 int or_set_recycling(or_state *o, int i_recyc, double betarecyc, double glund1, const double *wmean,
                      const double *weta, const int *map_inn, const int *map_out, const double *yplus_inflow,
                      const double *eta_inflow, const double *yplus_recyc, const double *eta_recyc);

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <new>
 #include <string>
 #include <vector>
@@ -73,19 +74,22 @@ int validate(const st_params *p) {
         return fail(ST_ERR_ARG, "conservative_viscous must be 0 or 1");
     if (p->calorically_perfect != 0 && p->calorically_perfect != 1)
         return fail(ST_ERR_ARG, "calorically_perfect must be 0 or 1");
+    if (p->indx_cp_r < p->indx_cp_l) return fail(ST_ERR_ARG, "indx_cp_r must be >= indx_cp_l");
     if (p->indx_cp_r - p->indx_cp_l + 2 > ST_MAX_CP_COEFF) return fail(ST_ERR_ARG, "too many cp coefficients");
+    // the constant-cp branch reads the coefficient of exponent 0
+    if (p->calorically_perfect == 1 && (p->indx_cp_l > 0 || p->indx_cp_r + 1 < 0))
+        return fail(ST_ERR_ARG, "calorically_perfect = 1 needs indx_cp_l <= 0 <= indx_cp_r + 1");
     if (p->precision != ST_PREC_FP64 && p->precision != ST_PREC_MIXED)
         return fail(ST_ERR_ARG, "precision must be ST_PREC_FP64 or ST_PREC_MIXED");
     bool any_nr = false;
     for (int l = 0; l < 6; l++) {
         const int t = p->bctags[l];
-        const bool ok = t == 0 || t == 1 || t == 2 || (t == 5 && l == 2) ||
-                        (t == 6 && (l == 2 || (l == 3 && p->is_y_staggered))) || (t == 7 && l == 3) ||
-                        (t == 10 && l == 0);
-        if (!ok)
+        // each tag acts on the faces update_ghost gives it and leaves the
+        // ghosts of the other faces alone, as the reference does; tag 11
+        // (airfoil wall) belongs to the curvilinear grids
+        if (t < 0 || t > 10)
             return fail(ST_ERR_ARG, "boundary tag " + std::to_string(t) + " on face " + std::to_string(l + 1) +
-                                        " is not supported (0 periodic, 1 free stream, 2 extrapolation, 5 symmetry"
-                                        " on ymin, 6 wall on ymin (and ymax when staggered), 7 shock on ymax)");
+                                        " is not supported (0..10)");
         const int nr = p->bctags_nr[l];
         if (nr != 0 && nr != 1 && nr != 2 && nr != 3 && nr != 6)
             return fail(ST_ERR_ARG, "characteristic tag " + std::to_string(nr) + " is not supported (0, 1, 2, 3, 6)");
@@ -176,6 +180,9 @@ void st::params_to_kpar(const st_params &p, st::KPar &k) {
     k.xshock_imp = p.xshock_imp;
     k.shock_angle = p.shock_angle;
     k.tanhfacs = p.tanhfacs;
+    k.aoa = p.aoa;
+    k.ptot0 = p.ptot0;
+    k.ttot0 = p.ttot0;
     k.mu0 = p.mu0; k.sutherland_s = p.sutherland_s; k.t_ref_dim = p.t_ref_dim; k.vtexp = p.powerlaw_vtexp;
     k.sensor_threshold = p.sensor_threshold; k.t_wall = p.t_wall; k.t_bulk_target = p.t_bulk_target;
     k.volchan = p.volchan; k.cfl = p.cfl; k.tol_iter_nr = p.tol_iter_nr;
@@ -223,7 +230,7 @@ void free_all(st_solver *s) {
     void *ptrs[] = {s->w, s->w_aux, s->F[0], s->F[1], s->wallprop, s->eoc[0], s->eoc[1], s->eoc[2], s->x, s->y,
                     s->z, s->yn, s->dcsidx, s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz,
                     s->dzitdzs, s->dzitdz2, s->ds, s->partials, s->dirty, s->counters, s->fixlist, s->rc_real, s->rc_maps,
-                    s->w_stat};
+                    s->w_stat, s->rplane, s->rng_coef, s->rng_win, s->rng_ctl, s->wrecyc, s->fhat, s->fln};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->hs) (void)hipHostFree(s->hs);
@@ -315,9 +322,12 @@ int stage(st_solver *s, int istep) {
     if (s->has_nr) TIMED(ST_K_BC, launch_bc_nr(s, fcur));  // bc_nr
     TIMED(ST_K_UPDATE, launch_update(s, fprev, fcur, istep));
     if (s->p.channel_case && s->p.theta_wall >= -1.0) TIMED(ST_K_FORCE_VAR, launch_force_var(s));
+    // get_crandom_f of euler_z: the random inflow plane of this stage
+    if (s->recyc) TIMED(ST_K_BC, launch_inflow_draws(s));
     if (s->general_bc) TIMED(ST_K_BC, launch_physical_ghosts(s));  // update_ghost(do_swap=0)
     s->cur ^= 1;
     s->last_stage = istep;
+    s->fln_current = false;
     return ST_OK;
 }
 
@@ -476,6 +486,11 @@ int st_set_grid(st_solver *s, const double *x, const double *y, const double *z,
         HIPCHK(hipMemcpyAsync(dst[q], src[q], n[q] * sizeof(double), hipMemcpyHostToDevice, s->stream));
     }
     HIPCHK(hipStreamSynchronize(s->stream));
+    // jbl_inflow = locateval(y(1:ny), ny, l0) (singleideal.F90:3284): rows
+    // below l0 in an increasing y
+    s->jbl = 0;
+    for (int j = 1; j <= g.ny; j++)
+        if (s->p.l0 > y[j + g.ng - 1]) s->jbl = j;
     s->grid_set = true;
     return ST_OK;
 }
@@ -571,6 +586,86 @@ int st_set_time(st_solver *s, double time, double dt, int64_t icyc) {
     return ST_OK;
 }
 
+namespace {
+
+// The C library's rand() after srand(seed): r[0] = seed, r[i] = 16807 r[i-1]
+// mod (2^31 - 1) for i < 31 (Schrage's form), r[31..33] = r[0..2], then
+// r[i] = r[i-31] + r[i-3] (mod 2^32) with outputs r[i] >> 1 from i = 344 on.
+// Fills the 31 words r[313..343] that precede the first output.
+void crandom_seed_window(uint32_t seed, uint32_t win[31]) {
+    std::vector<int64_t> r(344);
+    r[0] = seed == 0 ? 1 : (int32_t)seed;
+    for (int i = 1; i < 31; i++) {
+        const int64_t hi = r[i - 1] / 127773, lo = r[i - 1] % 127773;
+        int64_t word = 16807 * lo - 2836 * hi;
+        if (word < 0) word += 2147483647;
+        r[i] = word;
+    }
+    for (int i = 31; i < 34; i++) r[i] = r[i - 31];
+    for (int i = 34; i < 344; i++) r[i] = (int64_t)(uint32_t)((uint32_t)r[i - 31] + (uint32_t)r[i - 3]);
+    for (int q = 0; q < 31; q++) win[q] = (uint32_t)r[313 + q];
+}
+
+// Set-up of the random inflow plane: seed, the jump coefficients of the
+// draws of one z sweep and the device buffers (plane at 0.5,
+// singleideal.F90:1543).
+int setup_inflow_random(st_solver *s) {
+    const int ny = s->g.ny, nz = s->g.nz;
+    const int n = s->jbl >= 2 ? 3 * (s->jbl - 1) * nz : 0;
+    if (s->rplane) return ST_OK;
+    uint32_t seed = 1;  // init_crandom_f(rank+1) of one rank
+    if (s->p.rand_type < 0) seed = (uint32_t)std::time(nullptr) + 1u;
+    crandom_seed_window(seed, s->rng_seeded);
+    // coef(t) = coef(t-31) + coef(t-3), coef(-31+q) = e_q
+    const size_t tot = (size_t)(n + 31) * 31;
+    std::vector<uint32_t> c(tot, 0u);
+    for (int q = 0; q < 31; q++) c[(size_t)q * 31 + q] = 1u;
+    for (int t = 0; t < n; t++) {
+        uint32_t *dst = &c[(size_t)(t + 31) * 31];
+        const uint32_t *a = &c[(size_t)t * 31], *b = &c[(size_t)(t + 28) * 31];
+        for (int q = 0; q < 31; q++) dst[q] = a[q] + b[q];
+    }
+    std::vector<uint32_t> coef((size_t)31 * (n > 0 ? n : 1));
+    for (int t = 0; t < n; t++)
+        for (int q = 0; q < 31; q++) coef[(size_t)q * n + t] = c[(size_t)(t + 31) * 31 + q];
+    std::vector<double> plane((size_t)ny * nz * 3, 0.5);
+    HIPCHK(hipMalloc(&s->rplane, plane.size() * sizeof(double)));
+    HIPCHK(hipMalloc(&s->rng_coef, coef.size() * sizeof(uint32_t)));
+    HIPCHK(hipMalloc(&s->rng_win, 62 * sizeof(uint32_t)));
+    HIPCHK(hipMalloc(&s->rng_ctl, 2 * sizeof(int)));
+    HIPCHK(hipMemcpyAsync(s->rplane, plane.data(), plane.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->rng_coef, coef.data(), coef.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          s->stream));
+    s->rng_n = n;
+    HIPCHK(st::launch_rng_reset(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
+int ensure_rc_real(st_solver *s) {
+    const int ng = s->g.ng, ny = s->g.ny, ny2 = ny + 2 * ng;
+    const size_t nreal = (size_t)ng * ny * 4 + ny + 4 * (size_t)ny2 + (size_t)ng * ny * 5;
+    if (!s->rc_real) {
+        HIPCHK(hipMalloc(&s->rc_real, nreal * sizeof(double)));
+        HIPCHK(hipMalloc(&s->rc_maps, 2 * (size_t)ny * sizeof(int)));
+    }
+    return ST_OK;
+}
+
+}  // namespace
+
+int st_set_wmean(st_solver *s, const double *wmean) {
+    NEED(s);
+    if (!wmean) return fail(ST_ERR_ARG, "null wmean");
+    int rc = ensure_rc_real(s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->rc_real, wmean, sizeof(double) * (size_t)s->g.ng * s->g.ny * 4, hipMemcpyHostToDevice,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->wmean_set = true;
+    return ST_OK;
+}
+
 int st_set_recycling(st_solver *s, int32_t i_recyc, double betarecyc, double glund1, const double *wmean_inflow,
                      const double *weta, const int32_t *map_j_inn, const int32_t *map_j_out,
                      const double *yplus_inflow, const double *eta_inflow, const double *yplus_recyc,
@@ -588,10 +683,10 @@ int st_set_recycling(st_solver *s, int32_t i_recyc, double betarecyc, double glu
             return fail(ST_ERR_ARG, "recycling y map out of range at j = " + std::to_string(j));
     }
     const size_t nreal = (size_t)ng * ny * 4 + ny + 4 * (size_t)ny2 + (size_t)ng * ny * 5;
-    if (!s->rc_real) {
-        HIPCHK(hipMalloc(&s->rc_real, nreal * sizeof(double)));
-        HIPCHK(hipMalloc(&s->rc_maps, 2 * (size_t)ny * sizeof(int)));
-    }
+    int rc = ensure_rc_real(s);
+    if (rc) return rc;
+    if (!s->wrecyc) HIPCHK(hipMalloc(&s->wrecyc, sizeof(double) * 5 * (size_t)ng * ny * s->g.nz));
+    if (s->p.rand_type != 0 && (rc = setup_inflow_random(s))) return rc;
     std::vector<double> h(nreal, 0.0);
     size_t o = 0;
     std::memcpy(&h[o], wmean_inflow, sizeof(double) * ng * ny * 4); o += (size_t)ng * ny * 4;
@@ -610,6 +705,7 @@ int st_set_recycling(st_solver *s, int32_t i_recyc, double betarecyc, double glu
     s->betarecyc = betarecyc;
     s->glund1 = glund1;
     s->recyc_set = true;
+    s->wmean_set = true;
     return ST_OK;
 }
 
@@ -670,19 +766,136 @@ int st_visflx(st_solver *s, int32_t mode) {
     return ST_OK;
 }
 
-int st_euler_x(st_solver *s) {
-    NEED(s);
-    HIPCHK(st::launch_euler_x(s, s->F[s->cur], false));
+namespace {
+
+int ensure_piece_buffers(st_solver *s, bool fhat, bool fln) {
+    if (fhat && !s->fhat) {
+        HIPCHK(hipMalloc(&s->fhat, sizeof(double) * 5 * (size_t)s->g.nvol));
+        HIPCHK(st::launch_zero(s, s->fhat, 5 * (size_t)s->g.nvol));
+    }
+    if (fln && !s->fln) {
+        HIPCHK(hipMalloc(&s->fln, sizeof(double) * 5 * (size_t)s->g.nint));
+        HIPCHK(st::launch_zero(s, s->fln, 5 * (size_t)s->g.nint));
+    }
     return ST_OK;
 }
-int st_euler_y(st_solver *s) {
-    NEED(s);
-    HIPCHK(st::launch_euler_y(s, s->F[s->cur]));
+
+// One sweep over an explicit range: face fluxes of faces f0..f1 into fhat,
+// then (update) the divergence of cells c0..c1.  The default range runs the
+// fused sweep instead (fluxes and divergence in registers, WENO fix-up).
+int sweep_range(st_solver *s, int dir, int f0, int f1, bool update, int c0, int c1, bool fused) {
+    const int n = dir == 0 ? s->g.nx : (dir == 1 ? s->g.ny : s->g.nz);
+    if (fused) {
+        if (dir == 0) HIPCHK(st::launch_euler_x(s, s->F[s->cur], false));
+        else if (dir == 1) HIPCHK(st::launch_euler_y(s, s->F[s->cur]));
+        else HIPCHK(st::launch_euler_z(s, s->F[s->cur]));
+        return ST_OK;
+    }
+    if (f0 < 0 || f1 > n || (update && (c0 < 1 || c1 > n)))
+        return fail(ST_ERR_ARG, "sweep range outside faces 0..n / cells 1..n of direction " + std::to_string(dir + 1));
+    int rc = ensure_piece_buffers(s, true, false);
+    if (rc) return rc;
+    if (f1 >= f0) HIPCHK(st::launch_face_range(s, dir, f0, f1));
+    if (update && c1 >= c0) HIPCHK(st::launch_face_update(s, dir, s->F[s->cur], c0, c1));
     return ST_OK;
 }
-int st_euler_z(st_solver *s) {
+
+}  // namespace
+
+int st_euler_x(st_solver *s, int32_t istart, int32_t iend, int32_t istart_face, int32_t iend_face,
+               int32_t do_update) {
     NEED(s);
-    HIPCHK(st::launch_euler_z(s, s->F[s->cur]));
+    if (iend - istart < 0) return ST_OK;
+    const st::KPar &k = s->kp;
+    const bool fused = istart == k.eul_lo[0] && iend == k.eul_hi[0] && istart_face == istart - 1 &&
+                       iend_face == iend && do_update != 0;
+    return sweep_range(s, 0, istart_face, iend_face, do_update != 0, k.eul_lo[0], k.eul_hi[0], fused);
+}
+int st_euler_y(st_solver *s, int32_t eul_jmin, int32_t eul_jmax) {
+    NEED(s);
+    const st::KPar &k = s->kp;
+    const bool fused = eul_jmin == k.eul_lo[1] && eul_jmax == k.eul_hi[1];
+    return sweep_range(s, 1, eul_jmin - 1, eul_jmax, true, eul_jmin, eul_jmax, fused);
+}
+int st_euler_z(st_solver *s, int32_t eul_kmin, int32_t eul_kmax) {
+    NEED(s);
+    const st::KPar &k = s->kp;
+    const bool fused = eul_kmin == k.eul_lo[2] && eul_kmax == k.eul_hi[2];
+    int rc = sweep_range(s, 2, eul_kmin - 1, eul_kmax, true, eul_kmin, eul_kmax, fused);
+    if (rc) return rc;
+    if (s->recyc) HIPCHK(st::launch_inflow_draws(s));  // get_crandom_f of euler_z
+    return ST_OK;
+}
+
+int st_get_eul_range(st_solver *s, int32_t range[6]) {
+    if (!s || !range) return fail(ST_ERR_ARG, "null argument");
+    for (int d = 0; d < 3; d++) {
+        range[2 * d] = s->kp.eul_lo[d];
+        range[2 * d + 1] = s->kp.eul_hi[d];
+    }
+    return ST_OK;
+}
+
+int st_init_flux(st_solver *s, double rhodt) {
+    NEED(s);
+    int rc = ensure_piece_buffers(s, false, true);
+    if (rc) return rc;
+    HIPCHK(st::launch_flux_step(s, s->F[s->cur], rhodt, true));
+    return ST_OK;
+}
+
+int st_update_flux(st_solver *s, double gamdt) {
+    NEED(s);
+    if (!s->fln) return fail(ST_ERR_STATE, "st_init_flux not called");
+    HIPCHK(st::launch_flux_step(s, s->F[s->cur], gamdt, false));
+    return ST_OK;
+}
+
+int st_update_field(st_solver *s) {
+    NEED(s);
+    if (!s->fln) return fail(ST_ERR_STATE, "st_init_flux not called");
+    HIPCHK(st::launch_update_field(s));
+    s->fln_current = true;
+    return ST_OK;
+}
+
+int st_force_rhs(st_solver *s) {
+    NEED(s);
+    if (!s->fln) return fail(ST_ERR_STATE, "st_init_flux not called");
+    HIPCHK(st::launch_force_rhs_piece(s));
+    return ST_OK;
+}
+
+int st_force_var(st_solver *s) {
+    NEED(s);
+    if (s->p.theta_wall >= -1.0) HIPCHK(st::launch_force_var_piece(s));
+    return ST_OK;
+}
+
+int st_reset_forcing(st_solver *s) {
+    NEED(s);
+    HIPCHK(st::launch_zero(s, &s->ds->dpdx, 1));
+    return ST_OK;
+}
+
+int st_bcextr_var(st_solver *s, int32_t var) {
+    NEED(s);
+    if (var < 1 || var > 10) return fail(ST_ERR_ARG, "var must be 1..10");
+    HIPCHK(st::launch_extrapolate_aux(s, var));
+    return ST_OK;
+}
+
+int st_bcswap_var(st_solver *s, int32_t var) {
+    NEED(s);
+    if (var < 1 || var > 10) return fail(ST_ERR_ARG, "var must be 1..10");
+    HIPCHK(st::launch_fill_periodic_aux(s, var));
+    return ST_OK;
+}
+
+int st_recyc_exchange(st_solver *s) {
+    NEED(s);
+    if (!s->recyc || !s->recyc_set) return fail(ST_ERR_STATE, "no recycling inflow set (st_set_recycling)");
+    HIPCHK(st::launch_recyc_exchange(s));
     return ST_OK;
 }
 
@@ -720,7 +933,10 @@ int st_init_run(st_solver *s) {
     if ((rc = swap_var(s, 8, false))) return rc;
     if ((rc = st_zero_flux(s))) return rc;
     s->last_stage = -1;
-    return st_compute_dt(s, nullptr);
+    if ((rc = st_compute_dt(s, nullptr))) return rc;
+    // init_crandom_f before the integration loop (singleideal_cpu.F90:1585-1594)
+    HIPCHK(st::launch_rng_reset(s));
+    return ST_OK;
 }
 
 int st_compute_dt(st_solver *s, double *dt_out) {

@@ -50,6 +50,7 @@ struct KPar {
     double rgas0, prandtl, t0, rho0, u0, l0, mu0;
     double p_ref;          // rho0*rgas0*t0: the mixed form stores p - p_ref
     double winf[5], winf_ps[5], xshock_imp, shock_angle, tanhfacs;  // free-stream / shock boundary data
+    double aoa, ptot0, ttot0;  // total conditions of the subsonic free-stream boundary (tag 3)
     double sutherland_s, t_ref_dim, vtexp;
     double sensor_threshold, t_wall, t_bulk_target, volchan, cfl, tol_iter_nr;
     double cp[ST_MAX_CP_COEFF], cv[ST_MAX_CP_COEFF];

@@ -215,7 +215,7 @@ __device__ __forceinline__ int face_boundary(const Geo &g, int ilat, int a, int 
 template <typename R>
 __global__ void k_bc_face(Geo g, KPar P, double *__restrict__ w, const R *__restrict__ wa,
                           const double *__restrict__ xc, const double *__restrict__ yc, int ilat, int tag,
-                          int staggered) {
+                          int staggered, const double *__restrict__ wmean) {
     const int na = ilat <= 2 ? g.ny : g.nx;
     const int nb = ilat <= 4 ? g.nz : g.ny;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -227,6 +227,70 @@ __global__ void k_bc_face(Geo g, KPar P, double *__restrict__ w, const R *__rest
         for (int l = 1; l <= ng; l++) {
             const int d = face_ghost(g, ilat, a, b, l);
             for (int m = 0; m < 5; m++) w[m * nv + d] = P.winf[m];
+        }
+    } else if (tag == 3 && (ilat == 1 || ilat == 4)) {
+        // bcfree_sub_subroutine (kernels_cpu.F90:6060-6155): the boundary
+        // cell's pressure with the total conditions gives M, T and |u|
+        const int c = face_boundary(g, ilat, a, b);
+        double rho = w[c];
+        const double ri = 1.0 / rho;
+        double uu = w[nv + c] * ri, vv = w[2 * nv + c] * ri, ww = w[3 * nv + c] * ri;
+        const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+        double ee = w[4 * nv + c] / rho - qq;
+        double tt = temperature_from_e(P, ee, double(wa[5 * nv + c]));
+        const double pp = rho * tt * P.rgas0;
+        const double gamloc = gamloc_of(P, tt);
+        const double del = 0.5 * (gamloc - 1.0);
+        const double rmfac = pow(pp / P.ptot0, -(gamloc - 1.0) / gamloc);
+        const double rml = sqrt((rmfac - 1.0) / del);
+        tt = P.ttot0 / rmfac;
+        const double vel_mod = rml * sqrt(gamloc * P.rgas0 * tt);
+        uu = vel_mod * cos(P.aoa);
+        vv = vel_mod * sin(P.aoa);
+        ww = 0.0;
+        rho = pp / tt / P.rgas0;
+        const double rhou = rho * uu, rhov = rho * vv, rhow = rho * ww;
+        ee = e_from_temperature(P, tt);
+        const double rhoe = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
+        for (int l = 1; l <= ng; l++) {
+            const int d = face_ghost(g, ilat, a, b, l);
+            w[d] = rho;
+            w[nv + d] = rhou;
+            w[2 * nv + d] = rhov;
+            w[3 * nv + d] = rhow;
+            w[4 * nv + d] = rhoe;
+        }
+    } else if (tag == 4 && ilat != 3) {
+        // bcextr_sub_subroutine (kernels_cpu.F90:5064-5164): rho and momentum
+        // of the boundary cell at the reference pressure p0
+        const int c = face_boundary(g, ilat, a, b);
+        const double rho = w[c], rhou = w[nv + c], rhov = w[2 * nv + c], rhow = w[3 * nv + c];
+        const double tt = P.p_ref / rho / P.rgas0;
+        const double ee = e_from_temperature(P, tt);
+        const double rhoe = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
+        for (int l = 1; l <= ng; l++) {
+            const int d = face_ghost(g, ilat, a, b, l);
+            w[d] = rho;
+            w[nv + d] = rhou;
+            w[2 * nv + d] = rhov;
+            w[3 * nv + d] = rhow;
+            w[4 * nv + d] = rhoe;
+        }
+    } else if (tag == 9 && ilat == 1) {
+        // bclam_subroutine (kernels_cpu.F90:5951-5985): the mean planes
+        const int j = a, k = b;
+        for (int l = 1; l <= ng; l++) {
+            const double *wm = wmean + (ng - l) + (size_t)ng * (j - 1);
+            const size_t st = (size_t)ng * g.ny;
+            const double rho = wm[0], rhou = wm[st], rhov = wm[2 * st], rhow = wm[3 * st];
+            const double tt = P.p_ref / rho / P.rgas0;
+            const double ee = e_from_temperature(P, tt);
+            const int d = gidx(g, 1 - l, j, k);
+            w[d] = rho;
+            w[nv + d] = rhou;
+            w[2 * nv + d] = rhov;
+            w[3 * nv + d] = rhow;
+            w[4 * nv + d] = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
         }
     } else if (tag == 2) {
         const int s0 = face_boundary(g, ilat, a, b);
@@ -386,25 +450,33 @@ __device__ __forceinline__ int recyc_k(int k, int nz) {
     return k > h ? k - h : k + nz - h;
 }
 
+// wrecyc(i, j, k, m), i = 1..ng, of recyc_exchange (k_recyc_exchange)
+__device__ __forceinline__ size_t wrc(const Geo &g, int i, int j, int k, int m) {
+    return (size_t)(i - 1) + (size_t)g.ng * ((size_t)(j - 1) + (size_t)g.ny * ((size_t)(k - 1) + (size_t)g.nz * m));
+}
+
 // bcrecyc_subroutine_1 (kernels_cpu.F90:5810-5828): spanwise sums of the
 // recycled planes, one thread per (i, j), k summed in order.
-__global__ void k_recyc_avg(Geo g, const double *__restrict__ w, int i_recyc, double *__restrict__ av) {
+__global__ void k_recyc_avg(Geo g, const double *__restrict__ wrecyc, double *__restrict__ av) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.ng * g.ny) return;
     const int i = t % g.ng + 1, j = t / g.ng + 1;
-    const size_t nv = g.nvol;
     for (int m = 0; m < 5; m++) {
         double a = 0.0;
-        for (int k = 1; k <= g.nz; k++) a = a + w[m * nv + gidx(g, i_recyc + 1 - i, j, recyc_k(k, g.nz))];
+        for (int k = 1; k <= g.nz; k++) a = a + wrecyc[wrc(g, i, j, k, m)];
         av[(i - 1) + g.ng * ((j - 1) + g.ny * m)] = a;
     }
 }
 
-// bcrecyc_subroutine_2/3 (kernels_cpu.F90:5832-5947, rand_type 0): one
-// thread per (j, k) of the inflow; the Favre fluctuations of the recycled
-// planes are formed where they are interpolated.
+// bcrecyc_subroutine_2/3 (kernels_cpu.F90:5832-5947): one thread per
+// (j, k) of the inflow; the Favre fluctuations of the recycled planes are
+// formed where they are interpolated.  wmean(1-ng:0, j, m) sits at
+// [(ng-i) + ng*((j-1) + ny*m)] for plane 1-i.  With u0_02 != 0 (rand_type
+// != 0) each velocity fluctuation gains u0_02*(r - 0.5)*min(eta, 1) from
+// the random plane r(j, k, c) (Fortran order (1:ny, 1:nz, 1:3)).
 __global__ void k_recyc_inflow(Geo g, KPar P, double *__restrict__ w, const double *__restrict__ rc,
-                               const int *__restrict__ maps, int i_recyc, double betarecyc) {
+                               const int *__restrict__ maps, const double *__restrict__ wrecyc, double betarecyc,
+                               const double *__restrict__ rplane, double u0_02) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.ny * g.nz) return;
     const int j = t % g.ny + 1, k = t / g.ny + 1;
@@ -416,6 +488,7 @@ __global__ void k_recyc_inflow(Geo g, KPar P, double *__restrict__ w, const doub
     const double eta = eti[j + ng - 1];
     const double weta = weta_a[j - 1];
     const double weta1 = 1.0 - weta;
+    const double etamin = fmin(eta, 1.0);
     const double bdamp = 0.5 * (1.0 - tanh(4.0 * (eti[j + ng - 1] - 2.0)));
     const int j_inn = maps[j - 1], j_out = maps[ny + j - 1];
     const bool zero = j == 1 || j_inn >= ny || j_out >= ny;
@@ -424,22 +497,20 @@ __global__ void k_recyc_inflow(Geo g, KPar P, double *__restrict__ w, const doub
         disty_inn = (ypi[j + ng - 1] - ypr[j_inn + ng - 1]) / (ypr[j_inn + 1 + ng - 1] - ypr[j_inn + ng - 1]);
         disty_out = (eta - etr[j_out + ng - 1]) / (etr[j_out + 1 + ng - 1] - etr[j_out + ng - 1]);
     }
-    const int ks = recyc_k(k, nz);
     const double p0 = P.rho0 * P.rgas0 * P.t0;
     for (int i = 1; i <= ng; i++) {
-        const double rhomean = wmean[(i - 1) + ng * ((j - 1) + ny * 0)];
-        const double uumean = wmean[(i - 1) + ng * ((j - 1) + ny * 1)] / rhomean;
-        const double vvmean = wmean[(i - 1) + ng * ((j - 1) + ny * 2)] / rhomean;
-        const double wwmean = wmean[(i - 1) + ng * ((j - 1) + ny * 3)] / rhomean;
+        const double rhomean = wmean[(ng - i) + ng * ((j - 1) + ny * 0)];
+        const double uumean = wmean[(ng - i) + ng * ((j - 1) + ny * 1)] / rhomean;
+        const double vvmean = wmean[(ng - i) + ng * ((j - 1) + ny * 2)] / rhomean;
+        const double wwmean = wmean[(ng - i) + ng * ((j - 1) + ny * 3)] / rhomean;
         double rhofluc = 0.0, ufluc = 0.0, vfluc = 0.0, wfluc = 0.0;
         if (!zero) {
             // fluctuation of variable m (0 rho, 1..3 velocity) of plane i at row jx
             auto fluc = [&](int jx, int m) {
-                const int c = gidx(g, i_recyc + 1 - i, jx, ks);
                 const double *a = av + (i - 1) + ng * (jx - 1);
-                const double rho = w[c];
+                const double rho = wrecyc[wrc(g, i, jx, k, 0)];
                 if (m == 0) return rho - a[0] / nz;
-                return w[m * nv + c] / rho - a[(size_t)ng * ny * m] / a[0];
+                return wrecyc[wrc(g, i, jx, k, m)] / rho - a[(size_t)ng * ny * m] / a[0];
             };
             double f_in[4], f_out[4];
             for (int m = 0; m < 4; m++) {
@@ -454,6 +525,12 @@ __global__ void k_recyc_inflow(Geo g, KPar P, double *__restrict__ w, const doub
             ufluc = ufluc * bdamp * betarecyc;
             vfluc = vfluc * bdamp * betarecyc;
             wfluc = wfluc * bdamp * betarecyc;
+            if (u0_02 != 0.0) {
+                const size_t pl = (size_t)(j - 1) + (size_t)ny * (k - 1), pn = (size_t)ny * nz;
+                ufluc = ufluc + u0_02 * (rplane[pl] - 0.5) * etamin;
+                vfluc = vfluc + u0_02 * (rplane[pl + pn] - 0.5) * etamin;
+                wfluc = wfluc + u0_02 * (rplane[pl + 2 * pn] - 0.5) * etamin;
+            }
         }
         rhofluc = fmax(-0.75 * rhomean, rhofluc);
         rhofluc = fmin(4.0 * rhomean, rhofluc);
@@ -468,6 +545,59 @@ __global__ void k_recyc_inflow(Geo g, KPar P, double *__restrict__ w, const doub
         w[2 * nv + d] = rhov;
         w[3 * nv + d] = rhow;
         w[4 * nv + d] = rho * ee + 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho;
+    }
+}
+
+// ------------------------------------------------------ random inflow plane
+// get_crandom_f(inflow_random_plane(2:jbl, 1:nz, 1:3)) after a z sweep
+// (singleideal_cpu.F90:841): draw t of the sweep goes to row j = 2 +
+// t/(3 nz), k = 1 + (t/3) % nz, component t % 3, as rand()/RAND_MAX.  The C
+// library's rand() is an additive lagged-Fibonacci generator,
+// r[n] = r[n-31] + r[n-3] (mod 2^32), output r[n] >> 1; draw t of the sweep
+// is r[n0+t] = sum_q coef[q][t] win[q] (mod 2^32) over the 31 words
+// win[q] = r[n0-31+q] before the sweep, so every draw is independent.  The
+// window of the next sweep is written to the other buffer; the last
+// workgroup to finish flips the buffer index and clears the ticket.
+__global__ __launch_bounds__(256) void k_inflow_draws(Geo g, const uint32_t *__restrict__ coef, int n,
+                                                      uint32_t *__restrict__ win2, int *__restrict__ ctl,
+                                                      double *__restrict__ plane) {
+    __shared__ uint32_t wl[31];
+    __shared__ bool last;
+    const int par = __hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t *win = win2 + 31 * par;
+    uint32_t *wout = win2 + 31 * (par ^ 1);
+    if (threadIdx.x < 31) wl[threadIdx.x] = win[threadIdx.x];
+    __syncthreads();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        uint32_t r = 0u;
+        for (int q = 0; q < 31; q++) r += coef[(size_t)q * n + t] * wl[q];
+        const int c = t % 3, k = (t / 3) % g.nz, j = 2 + t / (3 * g.nz);
+        plane[(size_t)(j - 1) + (size_t)g.ny * ((size_t)k + (size_t)g.nz * c)] = (double)(r >> 1) / 2147483647.0;
+        if (t >= n - 31) wout[t - (n - 31)] = r;
+    }
+    if (t < 31 && n - 31 + t < 0) wout[t] = wl[n + t];  // fewer than 31 draws: the window slides
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&ctl[1], 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        ctl[0] = par ^ 1;
+        ctl[1] = 0;
+    }
+}
+
+struct Win31 {
+    uint32_t v[31];
+};
+
+// init_crandom_f of run(): the window of the freshly seeded generator
+__global__ void k_rng_reset(Win31 seeded, uint32_t *__restrict__ win2, int *__restrict__ ctl) {
+    const int t = threadIdx.x;
+    if (t < 31) win2[t] = seeded.v[t];
+    if (t == 0) {
+        ctl[0] = 0;
+        ctl[1] = 0;
     }
 }
 
@@ -2605,7 +2735,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_residual(Geo g, KPar P, const R *__restrict__ wa,
                                                   const R *__restrict__ flp, const R *__restrict__ fl,
                                                   const DevScalars *__restrict__ ds, double rkr, double rkg, int channel,
-                                                  double *__restrict__ partials) {
+                                                  double *__restrict__ partials, const double *__restrict__ fln) {
     __shared__ double lds[4][7];
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -2617,8 +2747,13 @@ __global__ __launch_bounds__(256) void k_residual(Geo g, KPar P, const R *__rest
         const size_t nv = g.nvol, ni = g.nint;
         const int c = gidx(g, i, j, k), oi = iidx(g, i, j, k);
         const double dt = ds->dt;
-        double fln2 = -(rkr * dt) * double(flp[ni + oi]) - (rkg * dt) * double(fl[ni + oi]);
-        if (channel) fln2 = fln2 - ds->b2;
+        double fln2;
+        if (fln) {  // the stored increment of the procedure-by-procedure stage
+            fln2 = fln[ni + oi];
+        } else {
+            fln2 = -(rkr * dt) * double(flp[ni + oi]) - (rkg * dt) * double(fl[ni + oi]);
+            if (channel) fln2 = fln2 - ds->b2;
+        }
         const double q = fln2 / dt;
         r = q * q;
         const double rho = double(wa[c]), tt = double(wa[5 * nv + c]);
@@ -2711,6 +2846,195 @@ __global__ void k_zero(T *__restrict__ a, size_t n) {
 template <typename T>
 __global__ void k_fill(T *__restrict__ a, size_t n, T v) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) a[t] = v;
+}
+
+// ======================================== piecewise stage (procedure by procedure)
+// The reference's rk_sync calls its procedures one by one on fl/fln/fhat
+// (singleideal_cpu.F90:332-409).  These kernels carry that form for callers
+// that split the stage (ranges of euler_x as rk_async does, or their own
+// ordering); st_rk_sync runs the fused stage instead.
+
+// Hybrid face fluxes of faces f0..f1 along DIR into fhat (ghosted layout,
+// the face index in the DIR coordinate), every interior line of the other
+// two directions (euler_*_fluxes_hybrid_kernel, kernels_cpu.F90:425-689 and
+// its y/z twins): one thread per face, the window of cells f-2..f+3.
+template <typename R, int DIR>
+__global__ __launch_bounds__(256) void k_face_range(Geo g, KPar P, const R *__restrict__ wa,
+                                                    const double *__restrict__ w, const int *__restrict__ eoc,
+                                                    double *__restrict__ fhat, int f0, int f1) {
+    const int nf = f1 - f0 + 1;
+    const int n1 = DIR == 0 ? nf : g.nx, n2 = DIR == 1 ? nf : g.ny, n3 = DIR == 2 ? nf : g.nz;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n1 * n2 * n3) return;
+    const int a1 = (int)(t % n1), a2 = (int)((t / n1) % n2), a3 = (int)(t / ((long long)n1 * n2));
+    const int i = DIR == 0 ? f0 + a1 : 1 + a1, j = DIR == 1 ? f0 + a2 : 1 + a2, k = DIR == 2 ? f0 + a3 : 1 + a3;
+    const int f = DIR == 0 ? i : (DIR == 1 ? j : k);
+    const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
+    const int stride = DIR == 0 ? 1 : (DIR == 1 ? g.sx : g.sxy);
+    const size_t nv = g.nvol;
+    const bool keep = P.nkeep >= 0;
+    const int base = gidx(g, i, j, k);
+    Pt q[6];
+    double sen[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        const int a = base + (s - 2) * stride;
+        sen[s] = double(wa[7 * nv + a]);
+        if constexpr (sizeof(R) == sizeof(double)) {
+            q[s] = make_pt(double(wa[a]), double(wa[nv + a]), double(wa[2 * nv + a]), double(wa[3 * nv + a]),
+                           double(wa[4 * nv + a]), double(wa[5 * nv + a]), P.rgas0, keep);
+        } else {
+            // FP32 w_aux: the primitives from w in FP64 as eval_aux forms them
+            int pi = i, pj = j, pk = k;
+            if (DIR == 0) pi += s - 2;
+            else if (DIR == 1) pj += s - 2;
+            else pk += s - 2;
+            if (P.periodic[0]) pi = pi < 1 ? pi + g.nx : (pi > g.nx ? pi - g.nx : pi);
+            if (P.periodic[1]) pj = pj < 1 ? pj + g.ny : (pj > g.ny ? pj - g.ny : pj);
+            if (P.periodic[2]) pk = pk < 1 ? pk + g.nz : (pk > g.nz ? pk - g.nz : pk);
+            const int src = gidx(g, pi, pj, pk);
+            const double rho = w[src], rhoe = w[4 * nv + src];
+            const double ri = 1.0 / rho;
+            const double uu = w[nv + src] * ri, vv = w[2 * nv + src] * ri, ww = w[3 * nv + src] * ri;
+            const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+            const double tt = temperature_from_e(P, rhoe / rho - qq, double(wa[5 * nv + a]));
+            const double pp = rho * tt * P.rgas0;
+            q[s] = make_pt(rho, uu, vv, ww, (rhoe + pp) / rho, tt, P.rgas0, keep);
+        }
+    }
+    double fh[5];
+    hybrid_face<DIR>(P, q, sen, f, eoc[f], n, fh);
+#pragma unroll
+    for (int m = 0; m < 5; m++) fhat[m * nv + base] = fh[m];
+}
+
+// euler_*_update_subroutine (kernels_cpu.F90:944-1058): cells c0..c1 along
+// DIR take fl += (fhat(c) - fhat(c-1)) * metric(c).
+template <typename R, int DIR>
+__global__ __launch_bounds__(256) void k_face_update(Geo g, const double *__restrict__ fhat,
+                                                     const double *__restrict__ metric, R *__restrict__ fl, int c0,
+                                                     int c1) {
+    const int nc = c1 - c0 + 1;
+    const int n1 = DIR == 0 ? nc : g.nx, n2 = DIR == 1 ? nc : g.ny, n3 = DIR == 2 ? nc : g.nz;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n1 * n2 * n3) return;
+    const int a1 = (int)(t % n1), a2 = (int)((t / n1) % n2), a3 = (int)(t / ((long long)n1 * n2));
+    const int i = DIR == 0 ? c0 + a1 : 1 + a1, j = DIR == 1 ? c0 + a2 : 1 + a2, k = DIR == 2 ? c0 + a3 : 1 + a3;
+    const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
+    const int stride = DIR == 0 ? 1 : (DIR == 1 ? g.sx : g.sxy);
+    const size_t nv = g.nvol;
+    const int gc = gidx(g, i, j, k), oi = iidx(g, i, j, k);
+    const double d = metric[c - 1];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const size_t q = (size_t)m * g.nint + oi;
+        fl[q] = R(double(fl[q]) + (fhat[m * nv + gc] - fhat[m * nv + gc - stride]) * d);
+    }
+}
+
+// init_flux_subroutine (kernels_cpu.F90:29-47): fln = -rhodt fl, fl = 0;
+// update_flux_subroutine (:3400-3418): fln = fln - gamdt fl.
+template <typename R>
+__global__ void k_init_flux(size_t n, R *__restrict__ fl, double *__restrict__ fln, double rhodt, int init) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        if (init) {
+            fln[t] = -rhodt * double(fl[t]);
+            fl[t] = R(0);
+        } else {
+            fln[t] = fln[t] - rhodt * double(fl[t]);
+        }
+    }
+}
+
+// update_field_subroutine (:3422-3442): w = w + fln on the interior.
+__global__ void k_update_field(Geo g, double *__restrict__ w, const double *__restrict__ fln) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const int c = gidx(g, i, j, k), oi = iidx(g, i, j, k);
+#pragma unroll
+    for (int m = 0; m < 5; m++) w[m * g.nvol + c] = w[m * g.nvol + c] + fln[(size_t)m * g.nint + oi];
+}
+
+// force_rhs_1_subroutine (:3157-3193): block partials of fln(1) dy,
+// fln(2) dy, w(1) dy, w(2) dy and w(2) dy w_aux(6).
+template <typename R>
+__global__ __launch_bounds__(256) void k_force_rhs_sums(Geo g, const double *__restrict__ w, const R *__restrict__ wa,
+                                                        const double *__restrict__ fln, const double *__restrict__ yn,
+                                                        double *__restrict__ partials) {
+    __shared__ double lds[4][5];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i <= g.nx && j <= g.ny) {
+        const int c = gidx(g, i, j, k), oi = iidx(g, i, j, k);
+        const double dy = yn[j] - yn[j - 1];
+        acc[0] = fln[oi] * dy;
+        acc[1] = fln[(size_t)g.nint + oi] * dy;
+        acc[2] = w[c] * dy;
+        acc[3] = w[g.nvol + c] * dy;
+        acc[4] = w[g.nvol + c] * dy * double(wa[5 * (size_t)g.nvol + c]);
+    }
+    block_sum<5>(acc, lds);
+    if (flat_thread() == 0)
+        for (int q = 0; q < 5; q++) partials[(size_t)flat_block() * 5 + q] = acc[q];
+}
+
+// force_rhs_2_subroutine (:3092-3116)
+template <typename R>
+__global__ void k_force_rhs_apply(Geo g, const R *__restrict__ wa, double *__restrict__ fln,
+                                  const DevScalars *__restrict__ ds) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const int c = gidx(g, i, j, k), oi = iidx(g, i, j, k);
+    const double b1 = ds->b1, b2 = ds->b2;
+    const double uu = double(wa[g.nvol + c]);
+    fln[oi] = fln[oi] - b1;
+    fln[(size_t)g.nint + oi] = fln[(size_t)g.nint + oi] - b2;
+    fln[4 * (size_t)g.nint + oi] = fln[4 * (size_t)g.nint + oi] - uu * b2;
+}
+
+// force_var_1_subroutine (:3241-3278): T from w (Newton from w_aux(6)),
+// stored in w_aux(6), and the block partials of rho*u*T*dy.
+template <typename R>
+__global__ __launch_bounds__(256) void k_force_var_sums(Geo g, KPar P, const double *__restrict__ w,
+                                                        R *__restrict__ wa, const double *__restrict__ yn,
+                                                        double *__restrict__ partials) {
+    __shared__ double lds[4][1];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double acc[1] = {0.0};
+    if (i <= g.nx && j <= g.ny) {
+        const size_t nv = g.nvol;
+        const int c = gidx(g, i, j, k);
+        const double dy = yn[j] - yn[j - 1];
+        const double rho = w[c], rhou = w[nv + c];
+        const double ri = 1.0 / rho;
+        const double uu = rhou * ri, vv = w[2 * nv + c] * ri, ww = w[3 * nv + c] * ri;
+        const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+        const double ee = w[4 * nv + c] / rho - qq;
+        const double tt = temperature_from_e(P, ee, double(wa[5 * nv + c]));
+        wa[5 * nv + c] = R(tt);
+        acc[0] = rhou * tt * dy;
+    }
+    block_sum<1>(acc, lds);
+    if (flat_thread() == 0) partials[flat_block()] = acc[0];
+}
+
+// recyc_exchange (singleideal_cpu.F90:935-999) for one block in x and z:
+// wrecyc(i, j, k, m) = w(i_recyc+1-i, j, k shifted by nz/2, m)
+__global__ void k_recyc_exchange(Geo g, const double *__restrict__ w, int i_recyc, double *__restrict__ wrecyc) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)g.ng * g.ny * g.nz;
+    if (t >= n) return;
+    const int i = (int)(t % g.ng) + 1, j = (int)((t / g.ng) % g.ny) + 1, k = (int)(t / ((long long)g.ng * g.ny)) + 1;
+    const int src = gidx(g, i_recyc + 1 - i, j, recyc_k(k, g.nz));
+    for (int m = 0; m < 5; m++) wrecyc[(size_t)m * n + t] = w[(size_t)m * g.nvol + src];
 }
 
 // ================================================================ launchers
@@ -3057,9 +3381,29 @@ hipError_t launch_recycling(st_solver *s) {
     const Geo &g = s->g;
     double *av = s->rc_real + (size_t)g.ng * g.ny * 4 + g.ny + 4 * (size_t)(g.ny + 2 * g.ng);
     const int n1 = g.ng * g.ny, n2 = g.ny * g.nz;
-    hipLaunchKernelGGL(k_recyc_avg, dim3((n1 + 127) / 128), dim3(128), 0, s->stream, g, s->w, s->i_recyc, av);
+    hipError_t e = launch_recyc_exchange(s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_recyc_avg, dim3((n1 + 127) / 128), dim3(128), 0, s->stream, g, s->wrecyc, av);
+    const double u0_02 = s->p.rand_type != 0 && s->rplane ? 0.02 * s->p.u0 : 0.0;
     hipLaunchKernelGGL(k_recyc_inflow, dim3((n2 + 127) / 128), dim3(128), 0, s->stream, g, s->kp, s->w, s->rc_real,
-                       s->rc_maps, s->i_recyc, s->betarecyc);
+                       s->rc_maps, s->wrecyc, s->betarecyc, s->rplane, u0_02);
+    return hipGetLastError();
+}
+
+hipError_t launch_inflow_draws(st_solver *s) {
+    const int n = s->rng_n;
+    if (!s->rplane || n <= 0) return hipSuccess;
+    const int nt = n > 31 ? n : 31;
+    hipLaunchKernelGGL(k_inflow_draws, dim3((nt + 255) / 256), dim3(256), 0, s->stream, s->g, s->rng_coef, n,
+                       s->rng_win, s->rng_ctl, s->rplane);
+    return hipGetLastError();
+}
+
+hipError_t launch_rng_reset(st_solver *s) {
+    if (!s->rplane) return hipSuccess;
+    Win31 seeded;
+    for (int q = 0; q < 31; q++) seeded.v[q] = s->rng_seeded[q];
+    hipLaunchKernelGGL(k_rng_reset, dim3(1), dim3(64), 0, s->stream, seeded, s->rng_win, s->rng_ctl);
     return hipGetLastError();
 }
 
@@ -3071,10 +3415,12 @@ hipError_t launch_physical_ghosts(st_solver *s) {
     }
     for (int ilat = 1; ilat <= 2 * s->kp.ndim; ilat++) {
         const int tag = s->kp.bctag[ilat - 1];
-        if (tag == 0 || tag == 10) continue;
+        if (tag == 0 || tag == 8 || tag == 10) continue;
+        if (tag == 9 && !s->wmean_set) continue;
         const int n = (ilat <= 2 ? g.ny : g.nx) * (ilat <= 4 ? g.nz : g.ny);
         ST_DISPATCH(hipLaunchKernelGGL(k_bc_face<R>, dim3((n + 127) / 128), dim3(128), 0, s->stream, g, s->kp, s->w,
-                                       (const R *)s->w_aux, s->x, s->y, ilat, tag, s->p.is_y_staggered));
+                                       (const R *)s->w_aux, s->x, s->y, ilat, tag, s->p.is_y_staggered,
+                                       (const double *)s->rc_real));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -3150,7 +3496,8 @@ hipError_t launch_residual(st_solver *s) {
     const double rkg = s->last_stage < 0 ? 0.0 : s->gamrk[st];
     ST_DISPATCH(hipLaunchKernelGGL(k_residual<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp,
                                    (const R *)s->w_aux, (const R *)s->F[s->cur ^ 1], (const R *)s->F[s->cur], s->ds,
-                                   rkr, rkg, s->p.channel_case, s->partials));
+                                   rkr, rkg, s->p.channel_case, s->partials,
+                                   s->fln_current ? (const double *)s->fln : nullptr));
     hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(RED_THREADS), 0, s->stream, g, s->partials, cell_blocks(g), s->ds);
     return hipGetLastError();
 }
@@ -3186,6 +3533,86 @@ hipError_t launch_zero(st_solver *s, double *a, size_t n) {
 hipError_t launch_zero_rhs(st_solver *s, void *a) {
     const size_t n = (size_t)5 * s->g.nint;
     ST_DISPATCH(hipLaunchKernelGGL(k_zero<R>, dim3(grid1d(n, 256)), dim3(256), 0, s->stream, (R *)a, n));
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------- piecewise stage
+template <typename R, int DIR>
+static hipError_t face_range_t(st_solver *s, int f0, int f1) {
+    const Geo &g = s->g;
+    const long long n1 = DIR == 0 ? f1 - f0 + 1 : g.nx, n2 = DIR == 1 ? f1 - f0 + 1 : g.ny,
+                    n3 = DIR == 2 ? f1 - f0 + 1 : g.nz;
+    const long long tot = n1 * n2 * n3;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_face_range<R, DIR>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, g,
+                       s->kp, (const R *)s->w_aux, s->w, s->eoc[DIR], s->fhat, f0, f1);
+    return hipGetLastError();
+}
+
+template <typename R, int DIR>
+static hipError_t face_update_t(st_solver *s, void *fl, int c0, int c1) {
+    const Geo &g = s->g;
+    const long long n1 = DIR == 0 ? c1 - c0 + 1 : g.nx, n2 = DIR == 1 ? c1 - c0 + 1 : g.ny,
+                    n3 = DIR == 2 ? c1 - c0 + 1 : g.nz;
+    const long long tot = n1 * n2 * n3;
+    if (tot <= 0) return hipSuccess;
+    const double *metric = DIR == 0 ? s->dcsidx : (DIR == 1 ? s->detady : s->dzitdz);
+    hipLaunchKernelGGL((k_face_update<R, DIR>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, g,
+                       s->fhat, metric, (R *)fl, c0, c1);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_range(st_solver *s, int dir, int f0, int f1) {
+    if (dir == 0) ST_DISPATCH(return (face_range_t<R, 0>(s, f0, f1)));
+    if (dir == 1) ST_DISPATCH(return (face_range_t<R, 1>(s, f0, f1)));
+    ST_DISPATCH(return (face_range_t<R, 2>(s, f0, f1)));
+}
+
+hipError_t launch_face_update(st_solver *s, int dir, void *fl, int c0, int c1) {
+    if (dir == 0) ST_DISPATCH(return (face_update_t<R, 0>(s, fl, c0, c1)));
+    if (dir == 1) ST_DISPATCH(return (face_update_t<R, 1>(s, fl, c0, c1)));
+    ST_DISPATCH(return (face_update_t<R, 2>(s, fl, c0, c1)));
+}
+
+hipError_t launch_flux_step(st_solver *s, void *fl, double coef, bool init) {
+    const size_t n = 5 * (size_t)s->g.nint;
+    ST_DISPATCH(hipLaunchKernelGGL(k_init_flux<R>, dim3(grid1d(n, 256)), dim3(256), 0, s->stream, n, (R *)fl,
+                                   s->fln, coef, init ? 1 : 0));
+    return hipGetLastError();
+}
+
+hipError_t launch_update_field(st_solver *s) {
+    hipLaunchKernelGGL(k_update_field, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->w, (const double *)s->fln);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_rhs_piece(st_solver *s) {
+    const Geo &g = s->g;
+    ST_DISPATCH(hipLaunchKernelGGL(k_force_rhs_sums<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->w,
+                                   (const R *)s->w_aux, (const double *)s->fln, s->yn, s->partials));
+    hipLaunchKernelGGL(k_force_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(g),
+                       s->partials + s->npart - RB * 8, s->counters + 1, s->ds, 0);
+    ST_DISPATCH(hipLaunchKernelGGL(k_force_rhs_apply<R>, cell_grid(g), kCellBlock, 0, s->stream, g,
+                                   (const R *)s->w_aux, s->fln, s->ds));
+    return hipGetLastError();
+}
+
+hipError_t launch_force_var_piece(st_solver *s) {
+    const Geo &g = s->g;
+    ST_DISPATCH(hipLaunchKernelGGL(k_force_var_sums<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w,
+                                   (R *)s->w_aux, s->yn, s->partials));
+    hipLaunchKernelGGL(k_force_var_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials,
+                       cell_blocks(g), s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
+    ST_DISPATCH(hipLaunchKernelGGL(k_force_var2<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w,
+                                   (const R *)s->w_aux, s->ds));
+    return hipGetLastError();
+}
+
+hipError_t launch_recyc_exchange(st_solver *s) {
+    const Geo &g = s->g;
+    const long long n = (long long)g.ng * g.ny * g.nz;
+    hipLaunchKernelGGL(k_recyc_exchange, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, g, s->w,
+                       s->i_recyc, s->wrecyc);
     return hipGetLastError();
 }
 

@@ -120,8 +120,12 @@ double time_step(st_solver *s, hipEvent_t a, hipEvent_t b) {
 int autotune(st_server *sv, st_solver *s, ServeReq *r) {
     const TuneKey key{pow2_ceil(r->p.nx), pow2_ceil(r->p.ny), pow2_ceil(r->p.nz), r->p.precision,
                       r->p.conservative_viscous};
-    auto it = sv->tune.find(key);
-    if (it != sv->tune.end()) return st_set_tuning(s, it->second.first, it->second.second);
+    {
+        // st_server_stats reads the cache size from the client thread
+        std::lock_guard<std::mutex> lk(sv->mu);
+        auto it = sv->tune.find(key);
+        if (it != sv->tune.end()) return st_set_tuning(s, it->second.first, it->second.second);
+    }
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess) return ST_ERR_HIP;
     if (hipEventCreate(&b) != hipSuccess) {
@@ -149,7 +153,10 @@ int autotune(st_server *sv, st_solver *s, ServeReq *r) {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (rc) return rc;
-    sv->tune[key] = {best_v, best_m};
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->tune[key] = {best_v, best_m};
+    }
     return st_set_tuning(s, best_v, best_m);
 }
 
@@ -204,6 +211,9 @@ int admit(st_server *sv, Slot &sl) {
     sl.s->cur = 0;
     sl.s->last_stage = -1;
     if ((rc = st_set_time(sl.s, 0.0, r->p.cfl < 0 ? -r->p.cfl : 0.0, 0))) return rc;
+    // the Newton start of T(e) back at t0 (the tuning steps left the
+    // temperature of an advanced state in w_aux(6))
+    if (sv->autotune && st::launch_init_temperature(sl.s) != hipSuccess) return ST_ERR_HIP;
     if ((rc = st_copy_from_field(sl.s, r->w0.data())) || (rc = st_init_run(sl.s))) return rc;
     for (auto &e : sl.ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return ST_ERR_HIP;

@@ -25,6 +25,21 @@ struct st_solver {
     double betarecyc, glund1;
     double *rc_real;    // wmean planes (ng*ny*4), weta (ny), y+/eta arrays (4*(ny+2ng)), averages (ng*ny*5)
     int *rc_maps;       // map_j_inn (ny), map_j_out (ny)
+    bool wmean_set;     // the wmean planes at the head of rc_real are loaded (tags 9, 10)
+    double *wrecyc;     // recycled station planes wrecyc(1:ng, 1:ny, 1:nz, 1:5) (recyc_exchange)
+    // the procedure-by-procedure stage (allocated on first use)
+    double *fhat;       // face fluxes, 5 ghosted variables
+    double *fln;        // RK increment of the stage, 5 interior variables
+    bool fln_current;   // the last stage was applied by st_update_field from fln
+    // random inflow plane of the recycling inflow (rand_type != 0): the C
+    // library's rand() stream, advanced on the device by jump coefficients
+    int jbl;            // locateval(y(1:ny), l0): rows 2..jbl are redrawn
+    int rng_n;          // draws per z sweep, 3*(jbl-1)*nz
+    double *rplane;     // inflow_random_plane(1:ny, 1:nz, 1:3)
+    uint32_t *rng_coef; // coefficient j of draw t at [j*rng_n + t], j = 0..30
+    uint32_t *rng_win;  // two 31-word windows of the generator state
+    int *rng_ctl;       // [0] window in use, [1] ticket of the draw kernel
+    uint32_t rng_seeded[31];  // window right after seeding
     double *w_stat;     // running statistics w_stat(1:70, 1:nx, 1:ny), allocated on first use
     bool dt_in_stage;   // the CFL reduction of this step rides on the first stage's aux
     int visc_chunk;     // planes per tile of the z-marching viscous kernel (0: automatic)
@@ -97,6 +112,21 @@ hipError_t launch_physical_ghosts(st_solver *s);
 hipError_t launch_bc_nr(st_solver *s, void *fl);
 // recyc_exchange + bcrecyc of face xmin
 hipError_t launch_recycling(st_solver *s);
+// the random inflow plane: redraw rows 2..jbl (after a z sweep), and the
+// generator window reset to its seeded state
+hipError_t launch_inflow_draws(st_solver *s);
+hipError_t launch_rng_reset(st_solver *s);
+// procedure-by-procedure stage: face fluxes of faces f0..f1 into fhat; the
+// divergence of fhat into fl over cells c0..c1; fln = -coef fl, fl = 0
+// (init) or fln -= coef fl; w += fln; force_rhs and force_var on fln / w;
+// the station planes into wrecyc
+hipError_t launch_face_range(st_solver *s, int dir, int f0, int f1);
+hipError_t launch_face_update(st_solver *s, int dir, void *fl, int c0, int c1);
+hipError_t launch_flux_step(st_solver *s, void *fl, double coef, bool init);
+hipError_t launch_update_field(st_solver *s);
+hipError_t launch_force_rhs_piece(st_solver *s);
+hipError_t launch_force_var_piece(st_solver *s);
+hipError_t launch_recyc_exchange(st_solver *s);
 // fix = false leaves the WENO fix-up of the sweep to launch_weno_fix
 hipError_t launch_euler_x(st_solver *s, void *fl, bool init, bool fix = true);
 hipError_t launch_euler_y(st_solver *s, void *fl, bool fix = true);

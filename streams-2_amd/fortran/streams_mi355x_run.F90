@@ -6,9 +6,11 @@
 ! the C ABI), the loop runs on the GPU through the bind(c) backend object,
 ! and the final conserved field and run scalars are written back.
 !
-! usage: streams_mi355x_run case.bin out.bin [dir]
+! usage: streams_mi355x_run case.bin out.bin [dir [mode]]
 !        streams_mi355x_run --sizes          byte sizes of the bind(c) records
 !        streams_mi355x_run --read case.bin  what was read, no device work
+! mode "procedure" runs each step through rk_sync_by_procedure (the stage
+! procedure by procedure, euler_x in two ranges) instead of the fused loop.
 ! With dir, the end of the run is also written there as the reference's
 ! restart and statistics files: rst.bin (write_field, field.F90:1923-1966),
 ! field_info0.dat (write_field_info, singleideal.F90:2620-2659) and stat.bin
@@ -17,6 +19,7 @@
 program streams_mi355x_run
   use, intrinsic :: iso_c_binding
   use streams_mi355x
+  use streams_equation_singleideal_mi355x_object
   implicit none
   type(st_params_t) :: p
   type(st_scalars_t) :: s
@@ -26,8 +29,9 @@ program streams_mi355x_run
   real(c_double), allocatable :: x(:), y(:), z(:), yn(:)
   real(c_double), allocatable :: dcsidx(:), dcsidxs(:), dcsidx2(:), detady(:), detadys(:), detady2(:)
   real(c_double), allocatable :: dzitdz(:), dzitdzs(:), dzitdz2(:), w(:,:,:,:), w_stat(:,:,:)
-  character(len=512) :: fin, fout, dir
+  character(len=512) :: fin, fout, dir, mode
   real(c_double) :: dt
+  integer :: n
 
   call get_command_argument(1, fin)
   if (trim(fin) == '--sizes') then
@@ -63,7 +67,15 @@ program streams_mi355x_run
   call eq%init_run()
   call eq%compute_dt(dt)
   write(*,*) 'dt =', dt
-  call eq%advance(int(nsteps))
+  mode = 'fused'
+  if (command_argument_count() >= 4) call get_command_argument(4, mode)
+  if (trim(mode) == 'procedure') then
+    do n = 1, nsteps
+      call eq%rk_sync_by_procedure()
+    enddo
+  else
+    call eq%run(int(nsteps), 0, 6)
+  endif
   call eq%compute_residual(s)
   call eq%copy_to_field(w)
   write(*,'(a,i0,a,es20.10)') 'steps: ', s%icyc, '  residual_rhou: ', s%residual_rhou

@@ -43,14 +43,25 @@ _SIGNATURES = {
     "st_bcswap": (C.c_int, [C.c_void_p]),
     "st_compute_aux": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "st_visflx": (C.c_int, [C.c_void_p, C.c_int32]),
-    "st_euler_x": (C.c_int, [C.c_void_p]),
-    "st_euler_y": (C.c_int, [C.c_void_p]),
-    "st_euler_z": (C.c_int, [C.c_void_p]),
+    "st_euler_x": (C.c_int, [C.c_void_p] + [C.c_int32] * 5),
+    "st_euler_y": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "st_euler_z": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "st_get_eul_range": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "st_init_flux": (C.c_int, [C.c_void_p, C.c_double]),
+    "st_update_flux": (C.c_int, [C.c_void_p, C.c_double]),
+    "st_update_field": (C.c_int, [C.c_void_p]),
+    "st_force_rhs": (C.c_int, [C.c_void_p]),
+    "st_force_var": (C.c_int, [C.c_void_p]),
+    "st_reset_forcing": (C.c_int, [C.c_void_p]),
+    "st_bcextr_var": (C.c_int, [C.c_void_p, C.c_int32]),
+    "st_bcswap_var": (C.c_int, [C.c_void_p, C.c_int32]),
+    "st_recyc_exchange": (C.c_int, [C.c_void_p]),
     "st_zero_flux": (C.c_int, [C.c_void_p]),
     "st_bc_nr": (C.c_int, [C.c_void_p]),
     "st_set_recycling": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
                          + [C.POINTER(C.c_double)] * 4),
+    "st_set_wmean": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "st_init_run": (C.c_int, [C.c_void_p]),
     "st_compute_dt": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "st_compute_dt_noalloc": (C.c_int, [C.POINTER(StParams), C.c_void_p, C.c_void_p, C.c_void_p]

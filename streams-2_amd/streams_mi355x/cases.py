@@ -145,14 +145,16 @@ def make_subsonic_channel(nx=80, ny=60, nz=80) -> ChannelCase:
 
 def bl_params(nx, ny, nz, mach=2.0, reynolds=200.0, t_wall=1.6, bctags=(1, 2, 6, 2, 0, 0),
               bctags_nr=(0, 1, 0, 1, 0, 0), sensor_threshold=0.2, cfl=0.5, conservative_viscous=1,
-              shock_angle_deg=None, xshock_imp=None, tanhfacs=None, precision=0) -> StParams:
+              shock_angle_deg=None, xshock_imp=None, tanhfacs=None, precision=0, rand_type=0,
+              aoa_deg=0.0) -> StParams:
     """Parameters of a boundary-layer box (flow_init = 1 family): node-centred
     grid, free-stream reference state rho0 = t0 = rgas0 = l0 = 1, u0 = M c0,
     mu0 = u0 rho0 l0 / Re (set_flow_params, singleideal.F90:2798-2826), an
     isothermal wall at T_wall, no forcing.  winf is the free-stream
     conserved state; with shock_angle_deg the post-shock state of the
     oblique-shock relations (set_oblique_shock, singleideal.F90:2955-3017)
-    fills winf_past_shock."""
+    fills winf_past_shock.  ttot0 and ptot0 are the total conditions of the
+    free stream (singleideal.F90:2813-2814), aoa its angle of attack."""
     case = FlowCase(nx=nx, ny=ny, nz=nz, mach=mach, reynolds=reynolds, sensor_threshold=sensor_threshold, cfl=cfl,
                     conservative_viscous=conservative_viscous, precision=precision)
     gam, gm1, rfac, icl, icr, cp, cv = set_fluid_prop(case)
@@ -174,6 +176,10 @@ def bl_params(nx, ny, nz, mach=2.0, reynolds=200.0, t_wall=1.6, bctags=(1, 2, 6,
     p.bctags_nr[:] = list(bctags_nr)
     p.is_y_staggered = 0
     p.channel_case = 0
+    p.rand_type = rand_type
+    p.aoa = aoa_deg * math.pi / 180.0
+    p.ttot0 = t0 * (1.0 + 0.5 * gm1 * mach ** 2)
+    p.ptot0 = rho0 * rgas0 * t0 * (1.0 + 0.5 * gm1 * mach ** 2) ** (gam / gm1)
     p.correct_bound_ord = 1
     p.iter_dt_recompute = 1
     p.precision = precision
@@ -296,7 +302,7 @@ class Recycling:
     i_recyc: int
     betarecyc: float
     glund1: float
-    wmean: np.ndarray        # wmean(1-i, j, m), shape (4, ny, ng) in Fortran order [m][j][i]
+    wmean: np.ndarray        # wmean(1-ng:0, 1:ny, 1:4) in Fortran order: array [m, j, ng-i] holds plane 1-i
     weta: np.ndarray         # (ny,)
     map_inn: np.ndarray      # (ny,) int32
     map_out: np.ndarray
@@ -315,7 +321,8 @@ def recycling_setup(grid: Grid, x_recyc: float, delta_in: float, deltav_in: floa
     weight.  The boundary-layer and viscous thicknesses at the inflow and at
     the station are given (in the reference they come from the mean
     profile of init_bl).  ``wmean_planes`` is the inflow mean profile
-    wmean(1-ng:0, 1:ny, 1:4) as an array [m, j, i] (i = 1..ng)."""
+    wmean(1-ng:0, 1:ny, 1:4) in Fortran order, an array [m, j, q] whose
+    q = 0..ng-1 is plane 1-ng+q."""
     ng, nx, ny = grid.ng, grid.nx, grid.ny
     xg = grid.x[ng:ng + nx]
     ig = locateval(xg, x_recyc)
@@ -340,7 +347,8 @@ def recycling_setup(grid: Grid, x_recyc: float, delta_in: float, deltav_in: floa
 
 def bl_mean_planes(case: ChannelCase) -> np.ndarray:
     """Spanwise/first-column mean profile of the seeded boundary-layer field
-    (rho, rho*u, rho*v, rho*w) for the ng inflow ghost planes: [m, j, i]."""
+    (rho, rho*u, rho*v, rho*w) for the ng inflow ghost planes:
+    wmean(1-ng:0, 1:ny, 1:4) as an array [m, j, q], q = 0 the plane 1-ng."""
     ng, ny = case.grid.ng, case.grid.ny
     w = case.w0
     prof = w[:4, ng:-ng, ng:ng + ny, ng].mean(axis=1)  # [m, j] at i = 1

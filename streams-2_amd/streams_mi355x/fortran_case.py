@@ -34,11 +34,13 @@ def read_result(path: str, shape):
     return w, s.as_dict()
 
 
-def run_driver(case_path: str, out_path: str, timeout: float = 600.0, files_dir: str = None) -> str:
+def run_driver(case_path: str, out_path: str, timeout: float = 600.0, files_dir: str = None,
+               mode: str = "fused") -> str:
     """Run the Fortran driver as a child process and return its output.
     With files_dir the driver also writes rst.bin, field_info0.dat and
-    stat.bin there."""
-    cmd = [DRIVER, case_path, out_path] + ([files_dir] if files_dir else [])
+    stat.bin there.  mode "procedure" steps through rk_sync_by_procedure."""
+    cmd = [DRIVER, case_path, out_path] + ([files_dir or os.path.dirname(out_path), mode]
+                                           if (files_dir or mode != "fused") else [])
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"Fortran driver failed ({r.returncode}): {r.stdout}\n{r.stderr}")

@@ -35,7 +35,7 @@ class StParams(C.Structure):
         ("bctags", C.c_int32 * 6), ("bctags_nr", C.c_int32 * 6),
         ("is_y_staggered", C.c_int32), ("channel_case", C.c_int32),
         ("correct_bound_ord", C.c_int32), ("iter_dt_recompute", C.c_int32),
-        ("precision", C.c_int32), ("pad0", C.c_int32), ("pad1", C.c_int32),
+        ("precision", C.c_int32), ("rand_type", C.c_int32), ("pad1", C.c_int32),
         ("rgas0", C.c_double), ("prandtl", C.c_double), ("t0", C.c_double), ("rho0", C.c_double),
         ("u0", C.c_double), ("l0", C.c_double), ("mu0", C.c_double),
         ("sutherland_s", C.c_double), ("t_ref_dim", C.c_double), ("powerlaw_vtexp", C.c_double),
@@ -45,6 +45,7 @@ class StParams(C.Structure):
         ("cp_coeff", C.c_double * MAX_CP), ("cv_coeff", C.c_double * MAX_CP),
         ("winf", C.c_double * 5), ("winf_past_shock", C.c_double * 5),
         ("xshock_imp", C.c_double), ("shock_angle", C.c_double), ("tanhfacs", C.c_double),
+        ("aoa", C.c_double), ("ptot0", C.c_double), ("ttot0", C.c_double),
     ]
 
 

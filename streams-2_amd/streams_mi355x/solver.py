@@ -102,14 +102,56 @@ class ChannelSolver:
     def visflx(self, mode: int):
         capi.check(self.L.st_visflx(self.h, mode))
 
-    def euler_x(self):
-        capi.check(self.L.st_euler_x(self.h))
+    def eul_range(self):
+        """eul_imin, eul_imax, eul_jmin, eul_jmax, eul_kmin, eul_kmax."""
+        r = (C.c_int32 * 6)()
+        capi.check(self.L.st_get_eul_range(self.h, r))
+        return tuple(r)
 
-    def euler_y(self):
-        capi.check(self.L.st_euler_y(self.h))
+    def euler_x(self, istart=None, iend=None, istart_face=None, iend_face=None, do_update=True):
+        """euler_x(istart, iend, istart_face, iend_face, do_update); the
+        defaults are the reference's: eul_imin..eul_imax, faces istart-1..iend."""
+        rg = self.eul_range()
+        istart = rg[0] if istart is None else istart
+        iend = rg[1] if iend is None else iend
+        istart_face = istart - 1 if istart_face is None else istart_face
+        iend_face = iend if iend_face is None else iend_face
+        capi.check(self.L.st_euler_x(self.h, istart, iend, istart_face, iend_face, int(bool(do_update))))
 
-    def euler_z(self):
-        capi.check(self.L.st_euler_z(self.h))
+    def euler_y(self, jmin=None, jmax=None):
+        rg = self.eul_range()
+        capi.check(self.L.st_euler_y(self.h, rg[2] if jmin is None else jmin, rg[3] if jmax is None else jmax))
+
+    def euler_z(self, kmin=None, kmax=None):
+        rg = self.eul_range()
+        capi.check(self.L.st_euler_z(self.h, rg[4] if kmin is None else kmin, rg[5] if kmax is None else kmax))
+
+    def init_flux(self, rhodt: float):
+        capi.check(self.L.st_init_flux(self.h, rhodt))
+
+    def update_flux(self, gamdt: float):
+        capi.check(self.L.st_update_flux(self.h, gamdt))
+
+    def update_field(self):
+        capi.check(self.L.st_update_field(self.h))
+
+    def force_rhs(self):
+        capi.check(self.L.st_force_rhs(self.h))
+
+    def force_var(self):
+        capi.check(self.L.st_force_var(self.h))
+
+    def reset_forcing(self):
+        capi.check(self.L.st_reset_forcing(self.h))
+
+    def bcextr_var(self, var: int):
+        capi.check(self.L.st_bcextr_var(self.h, var))
+
+    def bcswap_var(self, var: int):
+        capi.check(self.L.st_bcswap_var(self.h, var))
+
+    def recyc_exchange(self):
+        capi.check(self.L.st_recyc_exchange(self.h))
 
     def zero_flux(self):
         capi.check(self.L.st_zero_flux(self.h))
@@ -126,6 +168,11 @@ class ChannelSolver:
         capi.check(self.L.st_set_recycling(self.h, r.i_recyc, r.betarecyc, r.glund1, _dp(self._rc[0]),
                                            _dp(self._rc[1]), self._rcm[0].ctypes.data_as(ip),
                                            self._rcm[1].ctypes.data_as(ip), *[_dp(a) for a in self._rc[2:]]))
+
+    def set_wmean(self, wmean):
+        """Inflow mean planes wmean(1-ng:0, 1:ny, 1:4) (laminar inflow, tag 9)."""
+        self._wm = np.ascontiguousarray(wmean, dtype=np.float64)
+        capi.check(self.L.st_set_wmean(self.h, _dp(self._wm)))
 
     def init_run(self):
         capi.check(self.L.st_init_run(self.h))

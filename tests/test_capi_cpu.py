@@ -27,10 +27,48 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(capi.exported_symbols())
 
 
-def test_struct_sizes_match_header():
-    # st_params: 22 int32 + 6+6 int32 arrays = 34 int32 (136 B), 17 doubles, 2 x 16 doubles
-    assert C.sizeof(StParams) == 34 * 4 + 17 * 8 + 32 * 8 + 13 * 8
-    assert C.sizeof(StScalars) == 13 * 8 + 8
+def _c_layout(struct, fields):
+    """sizeof and offsetof of the header's struct, from a C compiler."""
+    import os
+    import subprocess
+    import tempfile
+
+    inc = os.path.join(ROOT, "include")
+    body = "\n".join(f'printf("%zu\\n", offsetof({struct}, {f}));' for f in fields)
+    src = (f'#include <stdio.h>\n#include <stddef.h>\n#include "streams_mi355x.h"\n'
+           f'int main(void) {{ printf("%zu\\n", sizeof({struct})); {body} return 0; }}\n')
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "l.c"), "w").write(src)
+        subprocess.run(["gcc", "-I", inc, "-o", os.path.join(d, "l"), os.path.join(d, "l.c")], check=True)
+        out = subprocess.run([os.path.join(d, "l")], check=True, capture_output=True, text=True).stdout.split()
+    return int(out[0]), [int(v) for v in out[1:]]
+
+
+@pytest.mark.parametrize("cls,name", [(StParams, "st_params"), (StScalars, "st_scalars")])
+def test_struct_layouts_match_header(cls, name):
+    fields = [f for f, _ in cls._fields_]
+    size, offs = _c_layout(name, fields)
+    assert C.sizeof(cls) == size
+    assert [getattr(cls, f).offset for f in fields] == offs
+
+
+@pytest.mark.parametrize("icl,icr,cp", [(1, 2, 1), (-3, -2, 1), (2, 1, 0)])
+def test_cp_exponent_range_rejected_without_gpu(icl, icr, cp):
+    from streams_mi355x import make_channel
+    p = make_channel().params
+    p.indx_cp_l, p.indx_cp_r, p.calorically_perfect = icl, icr, cp
+    h = C.c_void_p()
+    assert capi.lib().st_solver_create(C.byref(p), C.byref(h)) == capi.ST_ERR_ARG
+    assert b"indx_cp" in capi.lib().st_last_error()
+
+
+def test_unknown_boundary_tag_rejected_without_gpu():
+    from streams_mi355x import make_channel
+    p = make_channel().params
+    p.bctags[2] = 11
+    h = C.c_void_p()
+    assert capi.lib().st_solver_create(C.byref(p), C.byref(h)) == capi.ST_ERR_ARG
+    assert b"boundary tag" in capi.lib().st_last_error()
 
 
 def test_invalid_parameters_rejected_without_gpu():

@@ -68,3 +68,29 @@ def test_fortran_driver_matches_oracle(tmp_path):
     got = read_stats(tmp_path / "stat.bin", p.nx, p.ny)
     for q in range(70):
         assert np.abs(got[q] - ws[q]).max() <= 1e-8 * max(np.abs(ws[q]).max(), 1e-3), q + 1
+
+
+@pytest.mark.gpu
+def test_fortran_backend_by_procedure_matches_oracle(tmp_path):
+    """rk_sync_by_procedure of the Fortran backend object: the reference's
+    stage procedure by procedure through the range-argument calls (euler_x
+    split in two ranges), bulk forcing with bulk-temperature control, WENO
+    faces present, against the oracle's rk_sync."""
+    from oracle import Oracle
+    from streams_mi355x import make_channel
+
+    c = make_channel(nx=32, ny=24, nz=16, sensor_threshold=0.3, theta_wall=1.0)
+    p = c.params
+    case = tmp_path / "case.bin"
+    outp = tmp_path / "out.bin"
+    write_case(str(case), p, c.grid, c.w0, 6)
+    log = run_driver(str(case), str(outp), mode="procedure")
+    w, s = read_result(str(outp), c.w0.shape)
+    o = Oracle(p, c.grid)
+    o.w[:] = c.w0
+    o.init_run()
+    o.advance(6)
+    errs = [rel_l2(interior(w)[m], interior(o.w)[m]) for m in range(5)]
+    assert max(errs) <= 1e-10, (errs, log)
+    assert s["icyc"] == 6
+    assert abs(s["time"] - o.scalars()["time"]) <= 1e-12 * o.scalars()["time"]

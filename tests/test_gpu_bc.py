@@ -20,6 +20,9 @@ BC_SETS = {
                       xshock_imp=5.0, tanhfacs=8.0 / 39),
     "symmetry-relaxation": dict(bctags=(1, 2, 5, 1, 0, 0), bctags_nr=(3, 2, 0, 3, 0, 0)),
     "reflective-wall": dict(bctags=(2, 2, 6, 2, 0, 0), bctags_nr=(1, 1, 6, 1, 0, 0)),
+    "subsonic-total-conditions": dict(bctags=(3, 4, 6, 3, 0, 0), bctags_nr=(0, 0, 0, 0, 0, 0), mach=0.5,
+                                      aoa_deg=2.0),
+    "extrapolation-p0-no-op-faces": dict(bctags=(4, 4, 6, 4, 0, 0), bctags_nr=(0, 1, 0, 0, 0, 0)),
 }
 
 
@@ -104,11 +107,22 @@ def test_bl_trajectory_laplacian_viscous_form(name):
     g.close()
 
 
-def _recycling_pair(seed=5):
-    from streams_mi355x.cases import bl_mean_planes, recycling_setup
+def _distinct_planes(c):
+    """The inflow mean profile with its ng planes scaled apart (plane
+    1-ng+q by 1 + 0.01 (q+1)), so the plane order of wmean shows."""
+    from streams_mi355x.cases import bl_mean_planes
 
-    c = make_bl_box(bctags=(10, 2, 6, 2, 0, 0), bctags_nr=(0, 1, 0, 1, 0, 0), seed=seed)
-    r = recycling_setup(c.grid, 5.0, 0.6, 0.01, 0.7, 0.011, bl_mean_planes(c))
+    wm = bl_mean_planes(c).copy()
+    for q in range(NG):
+        wm[:, :, q] *= 1.0 + 0.01 * (q + 1)
+    return wm
+
+
+def _recycling_pair(seed=5, rand_type=0):
+    from streams_mi355x.cases import recycling_setup
+
+    c = make_bl_box(bctags=(10, 2, 6, 2, 0, 0), bctags_nr=(0, 1, 0, 1, 0, 0), seed=seed, rand_type=rand_type)
+    r = recycling_setup(c.grid, 5.0, 0.6, 0.01, 0.7, 0.011, _distinct_planes(c))
     w = c.w0.copy()
     w *= 1.0 + 0.04 * (np.random.default_rng(seed).random(w.shape[1:]) - 0.5)
     o, g = Oracle(c.params, c.grid), ChannelSolver(c.params, c.grid)
@@ -131,15 +145,59 @@ def test_recycling_inflow_parity():
     g.close()
 
 
-def test_recycling_trajectory_20_steps():
-    c, o, g = _recycling_pair()
+@pytest.mark.parametrize("rand_type", [0, 2])
+def test_recycling_trajectory_20_steps(rand_type):
+    """20 steps of a recycling box against the oracle; with rand_type 2 the
+    random inflow plane is redrawn every stage on the device and must
+    follow the C library's rand() stream the oracle calls."""
+    c, o, g = _recycling_pair(rand_type=rand_type)
     o.init_run()
     g.init_run()
     o.advance(20)
     g.advance(20)
     wg = g.copy_to_field()
     errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
-    print("recycling 20-step rel L2 per variable:", errs)
+    print("recycling 20-step rel L2 per variable (rand_type %d):" % rand_type, errs)
+    assert max(errs) <= TRAJ_TOL, errs
+    # the inflow ghosts carry the random term
+    for m in range(5):
+        assert rel_l2(wg[m][NG:-NG, NG:-NG, :NG], o.w[m][NG:-NG, NG:-NG, :NG]) <= TRAJ_TOL, m
+    g.close()
+
+
+def test_random_plane_changes_the_inflow():
+    """rand_type 2 against rand_type 0 from the same state: the inflow
+    velocity differs by at most 1 % of u0 after one step."""
+    out = []
+    for rt in (0, 2):
+        c, o, g = _recycling_pair(rand_type=rt)
+        g.init_run()
+        g.advance(1)
+        out.append(g.copy_to_field())
+        g.close()
+    du = out[1][1:4, NG:-NG, NG:-NG, :NG] / out[1][0, NG:-NG, NG:-NG, :NG] - \
+        out[0][1:4, NG:-NG, NG:-NG, :NG] / out[0][0, NG:-NG, NG:-NG, :NG]
+    assert 0.0 < np.abs(du).max() <= 0.011 * c.params.u0
+
+
+def test_laminar_inflow_from_wmean():
+    """Tag 9 on xmin: the ghost planes of the laminar inflow from wmean."""
+    c = make_bl_box(bctags=(9, 2, 6, 2, 0, 0), bctags_nr=(0, 1, 0, 1, 0, 0))
+    o, g = Oracle(c.params, c.grid), ChannelSolver(c.params, c.grid)
+    wm = _distinct_planes(c)
+    o.set_wmean(wm)
+    g.set_wmean(wm)
+    o.w[:] = c.w0
+    g.copy_from_field(c.w0)
+    o.init_run()
+    g.init_run()
+    wg = g.copy_to_field()
+    for m in range(5):
+        assert rel_l2(wg[m][NG:-NG, NG:-NG, :NG], o.w[m][NG:-NG, NG:-NG, :NG]) <= 1e-14, m
+    o.advance(10)
+    g.advance(10)
+    wg = g.copy_to_field()
+    errs = [rel_l2(interior(wg)[m], interior(o.w)[m]) for m in range(5)]
     assert max(errs) <= TRAJ_TOL, errs
     g.close()
 

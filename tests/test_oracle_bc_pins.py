@@ -271,14 +271,25 @@ def test_nr_reflective_wall_keeps_only_acoustic_waves():
 
 
 # --------------------------------------------------- recycling-rescaling inflow
-def _recycling_case(same_scaling=False, uniform=False, seed=5, beta=None):
-    from streams_mi355x.cases import bl_mean_planes, recycling_setup
+def _distinct_planes(c):
+    """The mean profile with each of the ng inflow planes scaled apart
+    (plane 1-ng+q by 1 + 0.01 (q+1)), so a mirrored plane order shows."""
+    from streams_mi355x.cases import bl_mean_planes
 
-    c = make_bl_box(bctags=(10, 2, 6, 2, 0, 0), bctags_nr=(0, 1, 0, 1, 0, 0), seed=seed)
+    wm = bl_mean_planes(c).copy()
+    for q in range(NG):
+        wm[:, :, q] *= 1.0 + 0.01 * (q + 1)
+    return wm
+
+
+def _recycling_case(same_scaling=False, uniform=False, seed=5, beta=None, rand_type=0):
+    from streams_mi355x.cases import recycling_setup
+
+    c = make_bl_box(bctags=(10, 2, 6, 2, 0, 0), bctags_nr=(0, 1, 0, 1, 0, 0), seed=seed, rand_type=rand_type)
     if same_scaling:
-        r = recycling_setup(c.grid, 5.0, 0.6, 0.01, 0.6, 0.01, bl_mean_planes(c))
+        r = recycling_setup(c.grid, 5.0, 0.6, 0.01, 0.6, 0.01, _distinct_planes(c))
     else:
-        r = recycling_setup(c.grid, 5.0, 0.6, 0.01, 0.7, 0.011, bl_mean_planes(c))
+        r = recycling_setup(c.grid, 5.0, 0.6, 0.01, 0.7, 0.011, _distinct_planes(c))
     if beta is not None:
         r.betarecyc = beta
     o = Oracle(c.params, c.grid)
@@ -301,8 +312,8 @@ def test_recycling_uniform_station_gives_the_mean_profile():
     e_of = lambda t: p.cv_coeff[1] * p.t0 + p.cv_coeff[0] * (t - p.t0)
     for i in range(1, NG + 1):
         g = o.w[:, NG:-NG, NG:-NG, NG - i]          # ghost plane 1-i: [m, k, j]
-        rho = r.wmean[0, :, i - 1]
-        u = r.wmean[1:4, :, i - 1] / rho
+        rho = r.wmean[0, :, NG - i]
+        u = r.wmean[1:4, :, NG - i] / rho
         assert np.allclose(g[0], rho[None, :], rtol=1e-14)
         assert np.allclose(g[1:4], (rho * u)[:, None, :], rtol=1e-13, atol=1e-15)
         tt = p0 / rho / p.rgas0
@@ -315,7 +326,7 @@ def test_recycling_density_fluctuation_has_zero_spanwise_mean():
     o.update_ghost(0)
     for i in range(1, NG + 1):
         rho_g = o.w[0, NG:-NG, NG:-NG, NG - i]       # [k, j]
-        assert np.allclose(rho_g.mean(axis=0), r.wmean[0, :, i - 1], rtol=1e-12)
+        assert np.allclose(rho_g.mean(axis=0), r.wmean[0, :, NG - i], rtol=1e-12)
 
 
 def test_recycling_shifts_the_span_by_half():
@@ -346,9 +357,9 @@ def test_recycling_identical_scaling_copies_the_damped_fluctuation():
         ufav = st[1].sum(axis=0) / st[0].sum(axis=0)
         uf = st[1] / st[0] - ufav[None, :]
         g = o.w[:, NG:-NG, NG:-NG, NG - i]
-        rho_m = r.wmean[0, :, i - 1]
+        rho_m = r.wmean[0, :, NG - i]
         u_g = g[1] / g[0]
-        want = r.wmean[1, :, i - 1] / rho_m + uf * bdamp[None, :] * r.betarecyc
+        want = r.wmean[1, :, NG - i] / rho_m + uf * bdamp[None, :] * r.betarecyc
         jj = slice(1, ny - 1)   # rows with a full interpolation stencil
         assert np.allclose(u_g[:, jj], want[:, jj], rtol=1e-12, atol=1e-13)
         rf = st[0] - st[0].mean(axis=0)[None, :]                         # density: damped, not rescaled by beta
@@ -365,3 +376,143 @@ def test_recycling_inflow_has_the_reference_pressure():
     g = o.w[:, NG:-NG, NG:-NG, 0:NG]
     pres = (gam - 1.0) * (g[4] - 0.5 * (g[1] ** 2 + g[2] ** 2 + g[3] ** 2) / g[0])
     assert np.allclose(pres, p.rho0 * p.rgas0 * p.t0, rtol=1e-12)
+
+
+# ------------------------------------------------ random inflow plane (rand_type)
+def _jbl(c):
+    """locateval(y(1:ny), ny, l0): rows strictly below l0."""
+    y = c.grid.y[NG:NG + c.params.ny]
+    return int(np.sum(y < c.params.l0))
+
+
+def test_random_plane_follows_the_c_library_stream_in_get_crandom_order():
+    """After run()'s seeding (seed rank+1 = 1) the first z sweep redraws
+    inflow_random_plane(2:jbl, 1:nz, 1:3) from rand()/RAND_MAX with j
+    outermost and the component innermost (get_crandom_f_array3d); rows 1
+    and above jbl keep 0.5.  The stream is the C library's srand/rand,
+    called directly here."""
+    from streams_mi355x.cases import crandom_stream
+
+    c, r, o = _recycling_case(rand_type=2)
+    o.init_run()
+    plane = o.inflow_plane()
+    assert np.all(plane == 0.5)
+    jbl = _jbl(c)
+    nz = c.params.nz
+    assert 2 <= jbl < c.params.ny
+    o.euler_z()
+    want = crandom_stream(1, 3 * (jbl - 1) * nz).reshape(jbl - 1, nz, 3)   # [j, k, c]
+    got = o.inflow_plane()[:, :, 1:jbl].transpose(2, 1, 0)                  # [j, k, c]
+    assert np.array_equal(got, want)
+    assert np.all(o.inflow_plane()[:, :, 0] == 0.5)
+    assert np.all(o.inflow_plane()[:, :, jbl:] == 0.5)
+    o.euler_z()  # the next sweep continues the same stream
+    want2 = crandom_stream(1, 6 * (jbl - 1) * nz)[3 * (jbl - 1) * nz:].reshape(jbl - 1, nz, 3)
+    assert np.array_equal(o.inflow_plane()[:, :, 1:jbl].transpose(2, 1, 0), want2)
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_random_plane_adds_two_percent_of_u0_after_the_rescaling(uniform):
+    """With identical inner/outer scalings the inflow velocity is the mean,
+    plus beta*bdamp times the station fluctuation, plus
+    0.02 u0 (r - 0.5) min(eta, 1) with r the plane's draw: the random term
+    is added after the rescaling, is centred (r - 0.5) and its amplitude is
+    at most 1 % of u0."""
+    c, r, o = _recycling_case(same_scaling=True, uniform=uniform, beta=1.7, rand_type=2)
+    p = c.params
+    nz, ny = p.nz, p.ny
+    o.init_run()
+    o.euler_z()
+    w0 = o.w.copy()
+    o.update_ghost(0)
+    plane = o.inflow_plane()                       # [c, k, j]
+    eta = r.eta_inflow[NG:NG + ny]
+    bdamp = 0.5 * (1.0 - np.tanh(4.0 * (eta - 2.0)))
+    jj = slice(1, ny - 1)
+    for i in range(1, NG + 1):
+        st = np.roll(w0[:, NG:-NG, NG:-NG, NG + r.i_recyc - i], nz // 2, axis=1)
+        g = o.w[:, NG:-NG, NG:-NG, NG - i]
+        rho_m = r.wmean[0, :, NG - i]
+        for m in range(3):
+            fav = st[1 + m].sum(axis=0) / st[0].sum(axis=0)
+            uf = st[1 + m] / st[0] - fav[None, :]
+            rnd = 0.02 * p.u0 * (plane[m] - 0.5) * np.minimum(eta, 1.0)[None, :]
+            want = r.wmean[1 + m, :, NG - i] / rho_m + uf * bdamp[None, :] * r.betarecyc + rnd
+            assert np.allclose((g[1 + m] / g[0])[:, jj], want[:, jj], rtol=1e-12, atol=1e-13), (i, m)
+            assert np.max(np.abs(rnd)) <= 0.01 * p.u0
+    assert abs(np.mean(plane[:, :, 1:_jbl(c)]) - 0.5) < 0.02
+
+
+# ----------------------------------------------------- ghost tags 3, 4, 8, 9
+def test_subsonic_inflow_from_total_conditions():
+    """Tag 3 on xmin and ymax (bcfree_sub): the ghosts hold the boundary
+    cell's pressure, the total temperature and pressure of the free stream
+    through the isentropic relations, and a velocity along the angle of
+    attack; w = 0 on the span."""
+    c, o = _oracle(bctags=(3, 2, 6, 3, 0, 0), bctags_nr=(0, 0, 0, 0, 0, 0), aoa_deg=3.0, mach=0.5)
+    p = c.params
+    gam = p.cp_coeff[0] / p.cv_coeff[0]
+    w0 = o.w.copy()  # the wall face, processed after xmin, changes the row j = 1 afterwards
+    o.update_ghost(0)
+    for face in ("xmin", "ymax"):
+        if face == "xmin":
+            g = o.w[:, NG:-NG, NG:-NG, 0:NG]
+            b = w0[:, NG:-NG, NG:-NG, NG:NG + 1]
+        else:
+            g = o.w[:, NG:-NG, -NG:, NG:-NG]
+            b = w0[:, NG:-NG, -NG - 1:-NG, NG:-NG]
+        pres_b = (gam - 1.0) * (b[4] - 0.5 * (b[1] ** 2 + b[2] ** 2 + b[3] ** 2) / b[0])
+        u, v, w = g[1] / g[0], g[2] / g[0], g[3] / g[0]
+        pres = (gam - 1.0) * (g[4] - 0.5 * g[0] * (u * u + v * v + w * w))
+        T = pres / (g[0] * p.rgas0)
+        mach2 = (u * u + v * v) / (gam * p.rgas0 * T)
+        fac = 1.0 + 0.5 * (gam - 1.0) * mach2
+        assert np.allclose(pres, np.broadcast_to(pres_b, pres.shape), rtol=1e-12), face
+        assert np.allclose(T * fac, p.ttot0, rtol=1e-12), face
+        assert np.allclose(pres * fac ** (gam / (gam - 1.0)), p.ptot0, rtol=1e-12), face
+        assert np.allclose(np.arctan2(v, u), p.aoa, rtol=0, atol=1e-12), face
+        assert np.all(w == 0.0)
+
+
+def test_extrapolation_at_the_reference_pressure_and_untouched_ymin():
+    """Tag 4 (bcextr_sub): density and momentum of the boundary cell and the
+    pressure p0 = rho0 R t0 on every face but ymin, whose ghosts it leaves
+    alone."""
+    c, o = _oracle(bctags=(4, 4, 4, 4, 0, 0), bctags_nr=(0, 0, 0, 0, 0, 0))
+    p = c.params
+    gam = p.cp_coeff[0] / p.cv_coeff[0]
+    p0 = p.rho0 * p.rgas0 * p.t0
+    o.w[:, :, 0:NG, :] = 7.0
+    o.update_ghost(0)
+    w = o.w
+    assert np.all(w[:, NG:-NG, 0:NG, NG:-NG] == 7.0)                # ymin untouched
+    for sl_g, sl_b in (((slice(NG, -NG), slice(NG, -NG), slice(0, NG)), (slice(NG, -NG), slice(NG, -NG), slice(NG, NG + 1))),
+                       ((slice(NG, -NG), slice(NG, -NG), slice(-NG, None)), (slice(NG, -NG), slice(NG, -NG), slice(-NG - 1, -NG))),
+                       ((slice(NG, -NG), slice(-NG, None), slice(NG, -NG)), (slice(NG, -NG), slice(-NG - 1, -NG), slice(NG, -NG)))):
+        g = w[(slice(None),) + sl_g]
+        b = w[(slice(None),) + sl_b]
+        for m in range(4):
+            assert np.array_equal(g[m], np.broadcast_to(b[m], g[m].shape))
+        pres = (gam - 1.0) * (g[4] - 0.5 * (g[1] ** 2 + g[2] ** 2 + g[3] ** 2) / g[0])
+        assert np.allclose(pres, p0, rtol=1e-12)
+
+
+def test_laminar_inflow_from_wmean_and_no_op_faces():
+    """Tag 9 on xmin (bclam): ghost plane 1-l takes wmean(1-l, j, 1:4) at
+    pressure p0.  Tag 8, and tag 9 on another face, leave the ghosts alone."""
+    c, o = _oracle(bctags=(9, 8, 6, 9, 0, 0), bctags_nr=(0, 0, 0, 0, 0, 0))
+    p = c.params
+    gam = p.cp_coeff[0] / p.cv_coeff[0]
+    wm = _distinct_planes(c)
+    o.set_wmean(wm)
+    o.w[:, :, :, -NG:] = 5.0
+    o.w[:, :, -NG:, NG:-NG] = 6.0
+    o.update_ghost(0)
+    assert np.all(o.w[:, :, :, -NG:] == 5.0)
+    assert np.all(o.w[:, NG:-NG, -NG:, NG:-NG] == 6.0)
+    for l in range(1, NG + 1):
+        g = o.w[:, NG:-NG, NG:-NG, NG - l]                          # [m, k, j]
+        for m in range(4):
+            assert np.array_equal(g[m], np.broadcast_to(wm[m, :, NG - l][None, :], g[m].shape)), (l, m)
+        pres = (gam - 1.0) * (g[4] - 0.5 * (g[1] ** 2 + g[2] ** 2 + g[3] ** 2) / g[0])
+        assert np.allclose(pres, p.rho0 * p.rgas0 * p.t0, rtol=1e-12)
