@@ -10,13 +10,15 @@ every step); the field is resident in HBM before timing starts.
 
 metric: Gcell-updates/s per RK stage = nx*ny*nz * 3 * steps / seconds / 1e9.
 
-Prints one JSON line.  The roofline object covers the kernel family with the
-largest total time, from HIP events recorded on the solver's own stream
-around every launch (the CFL reduction of each step rides on the first
-stage's aux pass, so it is timed within "aux"); its traffic comes from the
-committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the same workload.  The
-CPU baseline times the oracle (a restatement of the reference CPU kernels)
-on a bounded sample of the same workload.
+Prints one JSON line.  The roofline object covers the kernel with the
+largest total time among those with a byte count (one launch per stage: the
+z flux march, its WENO fix-up timed apart as "weno_fix"), from HIP events
+recorded on the solver's own stream around every launch (the CFL reduction
+of each step rides on the first stage's aux pass, so it is timed within
+"aux"); its traffic comes from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+summary of the same workload.  The CPU baseline times the oracle (a
+restatement of the reference CPU kernels) on a bounded sample of the same
+workload, on every core the process may use.
 
 Two more blocks of the same line:
 * "mixed": the mixed-precision form on the same workload, its speedup, its
@@ -143,35 +145,32 @@ def mixed_form(args, c, value_fp64):
 def sbli_workload(args):
     """Oblique shock / boundary-layer interaction at the scale of the SBLI
     configuration: 1536x320x192 nodes over 70 x 12 x 6.5 (delta = 1), Mach 2,
-    Re 200, adiabatic-recovery wall, recycling-rescaling inflow, supersonic
-    outflow with the characteristic treatment, imposed oblique shock of
-    38.774 deg at the top (the reference deck's angle) with the non-reflecting
-    top, periodic span, conservative viscous form, WENO5 where the Ducros
-    sensor exceeds 0.2.  Seeded BL profile plus 5 % velocity noise.  FP64 and
-    mixed rates, WENO face fractions, per-family times and the mixed form's
-    error against FP64 after 100 steps from the same field."""
+    Re 200, adiabatic-recovery wall, recycling-rescaling inflow with the
+    random inflow plane, supersonic outflow with the characteristic
+    treatment, the oblique shock of an 8 deg wedge (37.21 deg) imposed at the
+    top and meeting the wall at x = 40, with the non-reflecting top,
+    periodic span, conservative viscous form, WENO5 where the Ducros sensor
+    exceeds 0.2.  The timed steps start from a field that the incident shock
+    already crosses (seeded BL profile plus 5 % velocity noise, shock jump
+    imposed), so the WENO region of the shock is present throughout.  FP64
+    and mixed rates, WENO face fractions, per-family times and the mixed
+    form's error against FP64 after 100 steps from the same field."""
     import numpy as np
     from streams_mi355x import PREC_MIXED
-    from streams_mi355x.cases import (ChannelCase, bl_grid, bl_initial_field, bl_mean_planes, bl_params,
-                                      recycling_setup)
-    from streams_mi355x.params import FlowCase, params_copy
+    from streams_mi355x.cases import bl_recycling_case, shock_angle_from_deflection
+    from streams_mi355x.params import params_copy
     from streams_mi355x.solver import ChannelSolver
 
     nx, ny, nz = args.nx, args.ny, args.nz
-    lx = 70.0
-    g0 = bl_grid(nx, ny, nz, lx=lx, ly=12.0, lz=6.5, beta_y=3.0)
-    gam, pr, mach = 1.4, 0.72, 2.0
-    t_rec = 1.0 + 0.5 * (gam - 1.0) * pr ** (1.0 / 3.0) * mach ** 2
-    p = bl_params(nx, ny, nz, mach=mach, reynolds=200.0, t_wall=t_rec, bctags=(10, 2, 6, 7, 0, 0),
-                  bctags_nr=(0, 1, 0, 1, 0, 0), sensor_threshold=0.2, conservative_viscous=1,
-                  shock_angle_deg=38.774266, xshock_imp=40.0, tanhfacs=lx / (nx - 1))
-    c = ChannelCase(FlowCase(nx=nx, ny=ny, nz=nz), g0, p, bl_initial_field(g0, p, delta=1.0, noise=0.05, seed=0))
-    rc = recycling_setup(c.grid, 20.0, 1.0, 0.01, 1.1, 0.011, bl_mean_planes(c))
+    beta = shock_angle_from_deflection(2.0, 8.0)
+    c, rc = bl_recycling_case(nx, ny, nz, shock_angle_deg=beta, xshock_imp=40.0, developed_shock=True)
+    p = c.params
     progress(f"sbli case built: {nx}x{ny}x{nz}")
     cells = nx * ny * nz
     out = {"metric": "Gcell-updates/s per RK stage (SBLI box, FP64 and mixed)",
-           "config": {"workload": f"SBLI box {nx}x{ny}x{nz}, {args.steps} steps, M 2, Re 200, recycling inflow, "
-                                  "shock 38.774 deg at x = 40, sensor > 0.2, conservative viscous form"}}
+           "config": {"workload": f"SBLI box {nx}x{ny}x{nz}, {args.steps} steps, M 2, Re 200, recycling inflow "
+                                  f"(rand_type 2), 8 deg wedge shock ({beta:.3f} deg) meeting the wall at x = 40, "
+                                  "developed-shock start, sensor > 0.2, conservative viscous form"}}
     fields = []
     for name, prec in (("fp64", 0), ("mixed", PREC_MIXED)):
         pp = params_copy(p)
@@ -291,7 +290,7 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-hbm-gib", type=float, default=64.0)
-    ap.add_argument("--serve-seq-sample", type=int, default=16)
+    ap.add_argument("--serve-seq-sample", type=int, default=64)
     ap.add_argument("--workload", choices=("channel", "sbli"), default="channel",
                     help="sbli: the shock / boundary-layer box (1536x320x192 unless --nx/--ny/--nz say otherwise), "
                          "FP64 and mixed, printed as its own JSON line")
@@ -315,11 +314,33 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n}))
 
 
+def host_cpu():
+    """CPU model, logical CPUs of the host and the CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": allowed,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(args):
     """Oracle (plain C restatement of the reference CPU kernels, OpenMP over
-    k) on a slab of the same channel: full nx x ny, nz reduced to 16."""
+    k) on a slab of the same channel: full nx x ny, nz reduced to 16.  Threads:
+    OMP_NUM_THREADS when the environment sets it (the share of the host's
+    cores given to this process), else every CPU in the affinity mask."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or host["affinity_cpus"]
     os.environ["OMP_NUM_THREADS"] = str(threads)
     from oracle import Oracle
     from streams_mi355x import make_channel
@@ -339,7 +360,7 @@ def cpu_baseline(args):
         if el >= args.cpu_seconds or steps >= 20:
             break
     rate = args.nx * args.ny * nzs * 3 * steps / el / 1e9
-    return {"value": rate, "unit": "Gcell-updates/s per RK stage", "cores": threads, "kind": "port",
+    return {"value": rate, "unit": "Gcell-updates/s per RK stage", "cores": threads, "kind": "port", "host": host,
             "sample": f"oracle, channel {args.nx}x{args.ny}x{nzs} slab of the same workload, {steps} RK3 steps "
                       f"in {el:.1f} s, {threads} OpenMP threads"}
 

@@ -299,7 +299,7 @@ int st_compute_dt_noalloc(const st_params *p, void *hip_stream, const double *w,
 #define ST_K_BC         12   /* physical ghost faces and characteristic boundaries */
 #define ST_K_EULER_XY   13   /* x and y Euler fluxes in one pass (y range spanning all rows) */
 #define ST_K_WENO_FIX   14   /* WENO faces of the Euler sweeps (list compaction + fix-up) */
-#define ST_K_EULER_ZV   15   /* z Euler fluxes + viscous divergence terms + force_rhs partials, with the z fix-up */
+#define ST_K_EULER_ZV   15   /* z Euler fluxes + viscous divergence terms + force_rhs partials (its z fix-up counts under ST_K_WENO_FIX) */
 #define ST_NKFAM        16
 int st_kernel_times(st_solver *s, int32_t nsteps, double ms[ST_NKFAM], int32_t count[ST_NKFAM]);
 
@@ -316,11 +316,14 @@ const char *st_version(void);
 
 /* ---- serving loop (no reference counterpart) ----------------------------
  * Continuous batching of many independent solves on one GPU: requests are
- * admitted in arrival order as slots and HBM (st_solver_footprint) free up,
+ * admitted in arrival order as slots and HBM free up (the server holds
+ * hbm_capacity_bytes as one device pool from its creation and carves each
+ * solve's st_solver_footprint out of it: no allocation while solves run),
  * each runs on its own stream from a HIP graph of steps_per_graph steps
  * (0: no graph), and their work overlaps on the device.  autotune != 0
- * picks launch shapes per grid-shape class (st_set_tuning).  A result is
- * identical to the same solve run alone through st_advance. */
+ * times launch shapes (st_set_tuning) on the first solves of each
+ * grid-shape class, during their own steps, and keeps the fastest.  A
+ * result is identical to the same solve run alone through st_advance. */
 typedef struct st_server st_server;
 int st_server_create(int64_t hbm_capacity_bytes, int32_t max_slots, int32_t steps_per_graph, int32_t autotune,
                      st_server **out);
@@ -334,6 +337,20 @@ int st_server_submit(st_server *sv, const st_params *p,
                      const double *detady, const double *detadys, const double *detady2,
                      const double *dzitdz, const double *dzitdzs, const double *dzitdz2,
                      const double *w0, int32_t nsteps, int64_t *ticket);
+/* st_server_submit with a start (time0, icyc0: a request resumed from
+ * its restart files, as st_set_time) and, with checkpoint_every > 0 (a
+ * multiple of steps_per_graph), the request's restart files rst.bin and
+ * field_info0.dat written to checkpoint_dir every checkpoint_every steps and
+ * after its last step, in the formats of write_field / write_field_info
+ * (field.F90:1853-1966, singleideal.F90:2546-2659).  Requests with an inflow
+ * plane (xmin tags 9, 10) are not taken. */
+int st_server_submit2(st_server *sv, const st_params *p,
+                      const double *x, const double *y, const double *z, const double *yn,
+                      const double *dcsidx, const double *dcsidxs, const double *dcsidx2,
+                      const double *detady, const double *detadys, const double *detady2,
+                      const double *dzitdz, const double *dzitdzs, const double *dzitdz2,
+                      const double *w0, int32_t nsteps, double time0, int64_t icyc0,
+                      int32_t checkpoint_every, const char *checkpoint_dir, int64_t *ticket);
 /* Wait (timeout_s < 0: no limit) for a request, copy its final field (may
  * be NULL), scalars after st_compute_residual (may be NULL) and its submit /
  * admit / done times in seconds since server creation (may be NULL), then

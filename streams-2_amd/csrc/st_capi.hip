@@ -226,14 +226,19 @@ hipError_t dalloc(T **p, size_t n) {
     return hipMalloc((void **)p, n * sizeof(T) > 0 ? n * sizeof(T) : 8);
 }
 
+// Frees what the solver allocated; arrays inside memory lent to it
+// (create_solver with a SlotMem) and a lent scalar mirror stay with the lender.
 void free_all(st_solver *s) {
     void *ptrs[] = {s->w, s->w_aux, s->F[0], s->F[1], s->wallprop, s->eoc[0], s->eoc[1], s->eoc[2], s->x, s->y,
                     s->z, s->yn, s->dcsidx, s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz,
                     s->dzitdzs, s->dzitdz2, s->ds, s->partials, s->dirty, s->counters, s->fixlist, s->rc_real, s->rc_maps,
                     s->w_stat, s->rplane, s->rng_coef, s->rng_win, s->rng_ctl, s->wrecyc, s->fhat, s->fln};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
-    if (s->hs) (void)hipHostFree(s->hs);
+    for (void *p : ptrs) {
+        if (!p) continue;
+        if ((char *)p >= s->arena_lo && (char *)p < s->arena_hi) continue;
+        (void)hipFree(p);
+    }
+    if (s->hs && !s->borrowed_hs) (void)hipHostFree(s->hs);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -306,7 +311,8 @@ int stage(st_solver *s, int istep) {
     const bool zv = s->fuse_zv && s->kp.ndim == 3 && s->kp.eul_lo[2] == 1 && s->kp.eul_hi[2] == g.nz &&
                     (cons || s->kp.visc_lmax <= 3);
     if (zv) {
-        TIMED(ST_K_EULER_ZV, launch_euler_zv(s, fcur, fprev, istep, s->p.channel_case != 0, cons));
+        TIMED(ST_K_EULER_ZV, launch_euler_zv(s, fcur, fprev, istep, s->p.channel_case != 0, cons, false));
+        TIMED(ST_K_WENO_FIX, launch_euler_zv_fix(s, fcur, istep, s->p.channel_case != 0));
     } else if (s->kp.ndim == 3) {
         TIMED(ST_K_EULER_Z, launch_euler_z(s, fcur, false));
         TIMED(ST_K_WENO_FIX, launch_weno_fix(s, fcur, 2, 1, 0));
@@ -339,27 +345,54 @@ int do_rk_sync(st_solver *s) {
     return ST_OK;
 }
 
-}  // namespace
+// The device arrays of one solve, in allocation order, and their sizes in
+// bytes (st_solver_footprint sums them, each rounded up to 256 bytes).
+constexpr int kNArrays = 26;
+constexpr size_t kArenaAlign = 256;
+size_t arena_align(size_t n) { return (n + kArenaAlign - 1) / kArenaAlign * kArenaAlign; }
 
-extern "C" {
-
-const char *st_last_error(void) { return g_err.c_str(); }
-const char *st_version(void) { return "streams_mi355x 0.1 (gfx950)"; }
-
-int64_t st_solver_footprint(const st_params *p) {
-    if (!p) return -1;
-    const int64_t sx = p->nx + 2 * p->ng, sy = p->ny + 2 * p->ng, sz = p->nz + 2 * p->ng;
-    const int64_t nvol = sx * sy * sz, nint = (int64_t)p->nx * p->ny * p->nz;
-    const int64_t blocks = (int64_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * p->nz;
-    const int64_t rs = p->precision == ST_PREC_MIXED ? 4 : 8;  // bytes of a w_aux / right-hand-side value
-    const int64_t na = p->precision == ST_PREC_MIXED ? 11 : 10;
-    const int64_t part = 7 * blocks > (nvol + 255) / 256 ? 7 * blocks : (nvol + 255) / 256;
-    return 8 * (5 * nvol + 3 * sx * sz + part + 1280 + 1024) + rs * (na * nvol + 10 * nint) +
-           (5 * nint + 4 * (p->nx + p->ny + p->nz + 3 + 8 + 4096)) +
-           8 * (2 * sx + 2 * sy + 2 * sz + 3 * (p->nx + p->ny + p->nz) + p->ny + 1) + 4096;
+int partial_slots(const st::Geo &g) {
+    const int nblk = ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz;
+    // per-block partials (7 per cell block, or one per block of the aux
+    // pass over the ghosted box, whichever is larger), the z fix-up's
+    // 5 x 256 force_rhs partials, and the 128 x 8 second-level values at the end
+    const int aux_blk = (int)((g.nvol + 255) / 256);
+    return (7 * nblk > aux_blk ? 7 * nblk : aux_blk) + 256 * 5 + 128 * 8;
 }
 
-int st_solver_create(const st_params *p, st_solver **out) {
+void array_sizes(const st_params &p, size_t n[kNArrays]) {
+    const st::Geo g = st::params_to_geo(p);
+    const size_t nvol = g.nvol, nint = g.nint;
+    const size_t rs = p.precision == ST_PREC_MIXED ? sizeof(float) : sizeof(double);
+    const size_t na = p.precision == ST_PREC_MIXED ? 11 : 10;
+    const size_t d = sizeof(double);
+    size_t q = 0;
+    n[q++] = d * 5 * nvol;                 // w
+    n[q++] = rs * na * nvol;               // w_aux
+    n[q++] = rs * 5 * nint;                // F[0]
+    n[q++] = rs * 5 * nint;                // F[1]
+    n[q++] = d * 3 * g.sx * g.sz;          // wallprop
+    n[q++] = sizeof(int) * (g.nx + 1);     // eoc x, y, z
+    n[q++] = sizeof(int) * (g.ny + 1);
+    n[q++] = sizeof(int) * (g.nz + 1);
+    n[q++] = d * g.sx;                     // x, y, z, yn
+    n[q++] = d * g.sy;
+    n[q++] = d * g.sz;
+    n[q++] = d * (g.ny + 1);
+    for (int r = 0; r < 3; r++) n[q++] = d * g.nx;  // dcsidx, dcsidxs, dcsidx2
+    for (int r = 0; r < 3; r++) n[q++] = d * g.ny;  // detady ...
+    for (int r = 0; r < 3; r++) n[q++] = d * g.nz;  // dzitdz ...
+    n[q++] = sizeof(st::DevScalars);       // ds
+    n[q++] = d * partial_slots(g);         // partials
+    n[q++] = nint + 64;                    // dirty flags + the any-dirty word
+    n[q++] = sizeof(int) * (8 + 4096);     // counters (4096 = kFixMaxSeg)
+    n[q++] = sizeof(int) * nint;           // fixlist
+}
+
+// st_solver_create, or with mem the same solver inside memory the caller
+// lends (see st::SlotMem): no allocation, no synchronisation, every copy
+// and fill ordered on mem->stream.
+int create_solver(const st_params *p, const st::SlotMem *mem, st_solver **out) {
     if (!out) return fail(ST_ERR_ARG, "null output pointer");
     *out = nullptr;
     int rc = validate(p);
@@ -378,80 +411,94 @@ int st_solver_create(const st_params *p, st_solver **out) {
     s->mixed = p->precision == ST_PREC_MIXED;
     fill_kpar(s);
     hipError_t e = hipGetDevice(&s->device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (mem) {
+        s->stream = mem->stream;
+        s->own_stream = false;
+    } else {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        s->own_stream = true;
+    }
     if (e != hipSuccess) {
         delete s;
         return fail(ST_ERR_HIP, std::string("stream creation: ") + hipGetErrorString(e));
     }
-    s->own_stream = true;
     const size_t nvol = g.nvol, nint = g.nint;
-    const int nblk = ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz;
-    // per-block partials (7 per cell block, or one per block of the aux
-    // pass over the ghosted box, whichever is larger) + second-level sums
-    const int aux_blk = (int)((nvol + 255) / 256);
-    // block partials (7 per cell block at most), the z fix-up's 5 x 256
-    // force_rhs partials, and the 128 x 8 second-level values at the end
-    s->npart = (7 * nblk > aux_blk ? 7 * nblk : aux_blk) + 256 * 5 + 128 * 8;
-    hipError_t es[32];
-    int ne = 0;
-    es[ne++] = dalloc(&s->w, 5 * nvol);
-    es[ne++] = hipMalloc(&s->w_aux, rsize(s) * naux(s) * nvol);
-    es[ne++] = hipMalloc(&s->F[0], rsize(s) * 5 * nint);
-    es[ne++] = hipMalloc(&s->F[1], rsize(s) * 5 * nint);
-    es[ne++] = dalloc(&s->wallprop, (size_t)3 * g.sx * g.sz);
-    es[ne++] = dalloc(&s->eoc[0], (size_t)g.nx + 1);
-    es[ne++] = dalloc(&s->eoc[1], (size_t)g.ny + 1);
-    es[ne++] = dalloc(&s->eoc[2], (size_t)g.nz + 1);
-    es[ne++] = dalloc(&s->x, (size_t)g.sx);
-    es[ne++] = dalloc(&s->y, (size_t)g.sy);
-    es[ne++] = dalloc(&s->z, (size_t)g.sz);
-    es[ne++] = dalloc(&s->yn, (size_t)g.ny + 1);
-    es[ne++] = dalloc(&s->dcsidx, (size_t)g.nx);
-    es[ne++] = dalloc(&s->dcsidxs, (size_t)g.nx);
-    es[ne++] = dalloc(&s->dcsidx2, (size_t)g.nx);
-    es[ne++] = dalloc(&s->detady, (size_t)g.ny);
-    es[ne++] = dalloc(&s->detadys, (size_t)g.ny);
-    es[ne++] = dalloc(&s->detady2, (size_t)g.ny);
-    es[ne++] = dalloc(&s->dzitdz, (size_t)g.nz);
-    es[ne++] = dalloc(&s->dzitdzs, (size_t)g.nz);
-    es[ne++] = dalloc(&s->dzitdz2, (size_t)g.nz);
-    es[ne++] = dalloc(&s->ds, 1);
-    es[ne++] = dalloc(&s->partials, (size_t)s->npart);
-    es[ne++] = dalloc(&s->dirty, nint + 64);  // per-cell flags + the any-dirty word
-    if (es[ne - 1] == hipSuccess) es[ne - 1] = hipMemset(s->dirty, 0, nint + 64);
-    es[ne++] = dalloc(&s->counters, 8 + 4096);  // 4096 = kFixMaxSeg
-    es[ne++] = dalloc(&s->fixlist, nint);
-    es[ne++] = hipHostMalloc((void **)&s->hs, sizeof(st::DevScalars), hipHostMallocDefault);
-    for (int q = 0; q < ne; q++) {
-        if (es[q] != hipSuccess) {
+    s->npart = partial_slots(g);
+    size_t n[kNArrays];
+    array_sizes(*p, n);
+    void **dst[kNArrays] = {(void **)&s->w, &s->w_aux, &s->F[0], &s->F[1], (void **)&s->wallprop,
+                            (void **)&s->eoc[0], (void **)&s->eoc[1], (void **)&s->eoc[2], (void **)&s->x,
+                            (void **)&s->y, (void **)&s->z, (void **)&s->yn, (void **)&s->dcsidx,
+                            (void **)&s->dcsidxs, (void **)&s->dcsidx2, (void **)&s->detady, (void **)&s->detadys,
+                            (void **)&s->detady2, (void **)&s->dzitdz, (void **)&s->dzitdzs, (void **)&s->dzitdz2,
+                            (void **)&s->ds, (void **)&s->partials, (void **)&s->dirty, (void **)&s->counters,
+                            (void **)&s->fixlist};
+    hipError_t ea = hipSuccess;
+    if (mem) {
+        size_t off = 0;
+        for (int q = 0; q < kNArrays; q++) {
+            *dst[q] = mem->arena + off;
+            off += arena_align(n[q]);
+        }
+        if (off > mem->bytes) {
+            delete s;
+            return fail(ST_ERR_NOMEM, "the lent memory is smaller than the solve's footprint");
+        }
+        s->arena_lo = mem->arena;
+        s->arena_hi = mem->arena + off;
+        s->hs = mem->hs;
+        s->borrowed_hs = true;
+    } else {
+        for (int q = 0; q < kNArrays && ea == hipSuccess; q++) ea = hipMalloc(dst[q], n[q] > 0 ? n[q] : 8);
+        if (ea == hipSuccess) ea = hipHostMalloc((void **)&s->hs, sizeof(st::DevScalars), hipHostMallocDefault);
+        if (ea != hipSuccess) {
             free_all(s);
             delete s;
-            return fail(ST_ERR_NOMEM, std::string("device allocation failed: ") + hipGetErrorString(es[q]));
+            return fail(ST_ERR_NOMEM, std::string("device allocation failed: ") + hipGetErrorString(ea));
         }
     }
     // tables
     int *eh[3];
-    int *buf = (int *)std::calloc((size_t)(g.nx + g.ny + g.nz + 3), sizeof(int));
+    int *buf = mem ? mem->tabs : (int *)std::calloc((size_t)(g.nx + g.ny + g.nz + 3), sizeof(int));
+    if (!buf) {
+        free_all(s);
+        delete s;
+        return fail(ST_ERR_NOMEM, "host allocation failed");
+    }
+    if (mem) std::memset(buf, 0, sizeof(int) * (size_t)(g.nx + g.ny + g.nz + 3));
     eh[0] = buf; eh[1] = buf + g.nx + 1; eh[2] = buf + g.nx + g.ny + 2;
     build_tables(s, eh);
-    hipError_t ec = hipSuccess;
-    if (ec == hipSuccess) ec = hipMemcpy(s->eoc[0], eh[0], sizeof(int) * (g.nx + 1), hipMemcpyHostToDevice);
-    if (ec == hipSuccess) ec = hipMemcpy(s->eoc[1], eh[1], sizeof(int) * (g.ny + 1), hipMemcpyHostToDevice);
-    if (ec == hipSuccess) ec = hipMemcpy(s->eoc[2], eh[2], sizeof(int) * (g.nz + 1), hipMemcpyHostToDevice);
-    std::free(buf);
     // zero state (w_aux = 0 as in the reference alloc, singleideal.F90:1596; then
     // w_aux(6) = t0 as in initialize, singleideal.F90:3032)
-    if (ec == hipSuccess) ec = hipMemset(s->w, 0, sizeof(double) * 5 * nvol);
-    if (ec == hipSuccess) ec = hipMemset(s->w_aux, 0, rsize(s) * naux(s) * nvol);
-    if (ec == hipSuccess) ec = hipMemset(s->F[0], 0, rsize(s) * 5 * nint);
-    if (ec == hipSuccess) ec = hipMemset(s->F[1], 0, rsize(s) * 5 * nint);
-    if (ec == hipSuccess) ec = hipMemset(s->wallprop, 0, sizeof(double) * 3 * g.sx * g.sz);
-    if (ec == hipSuccess) ec = hipMemset(s->counters, 0, sizeof(int) * (8 + 4096));
+    const size_t nz_bytes[7] = {sizeof(double) * 5 * nvol, rsize(s) * naux(s) * nvol, rsize(s) * 5 * nint,
+                                rsize(s) * 5 * nint, sizeof(double) * 3 * g.sx * g.sz, sizeof(int) * (8 + 4096),
+                                nint + 64};
+    void *nz_ptr[7] = {s->w, s->w_aux, s->F[0], s->F[1], s->wallprop, s->counters, s->dirty};
+    hipError_t ec = hipSuccess;
     std::memset(s->hs, 0, sizeof(st::DevScalars));
     if (p->cfl < 0) s->hs->dt = -p->cfl;
-    if (ec == hipSuccess) ec = hipMemcpy(s->ds, s->hs, sizeof(st::DevScalars), hipMemcpyHostToDevice);
-    if (ec == hipSuccess) ec = st::launch_init_temperature(s);
-    if (ec == hipSuccess) ec = hipStreamSynchronize(s->stream);
+    if (mem) {
+        s->hs->time = mem->time0;
+        s->hs->icyc = mem->icyc0;
+        s->icyc = s->icyc0 = mem->icyc0;
+        // stream-ordered from pinned host memory the lender keeps alive
+        const size_t tb[3] = {sizeof(int) * (g.nx + 1), sizeof(int) * (g.ny + 1), sizeof(int) * (g.nz + 1)};
+        for (int d = 0; d < 3 && ec == hipSuccess; d++)
+            ec = hipMemcpyAsync(s->eoc[d], eh[d], tb[d], hipMemcpyHostToDevice, s->stream);
+        for (int q = 0; q < 7 && ec == hipSuccess; q++) ec = hipMemsetAsync(nz_ptr[q], 0, nz_bytes[q], s->stream);
+        if (ec == hipSuccess)
+            ec = hipMemcpyAsync(s->ds, s->hs, sizeof(st::DevScalars), hipMemcpyHostToDevice, s->stream);
+        if (ec == hipSuccess) ec = st::launch_init_temperature(s);
+    } else {
+        if (ec == hipSuccess) ec = hipMemcpy(s->eoc[0], eh[0], sizeof(int) * (g.nx + 1), hipMemcpyHostToDevice);
+        if (ec == hipSuccess) ec = hipMemcpy(s->eoc[1], eh[1], sizeof(int) * (g.ny + 1), hipMemcpyHostToDevice);
+        if (ec == hipSuccess) ec = hipMemcpy(s->eoc[2], eh[2], sizeof(int) * (g.nz + 1), hipMemcpyHostToDevice);
+        std::free(buf);
+        for (int q = 0; q < 7 && ec == hipSuccess; q++) ec = hipMemset(nz_ptr[q], 0, nz_bytes[q]);
+        if (ec == hipSuccess) ec = hipMemcpy(s->ds, s->hs, sizeof(st::DevScalars), hipMemcpyHostToDevice);
+        if (ec == hipSuccess) ec = st::launch_init_temperature(s);
+        if (ec == hipSuccess) ec = hipStreamSynchronize(s->stream);
+    }
     if (ec != hipSuccess) {
         free_all(s);
         delete s;
@@ -460,6 +507,24 @@ int st_solver_create(const st_params *p, st_solver **out) {
     *out = s;
     return ST_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+const char *st_last_error(void) { return g_err.c_str(); }
+const char *st_version(void) { return "streams_mi355x 0.1 (gfx950)"; }
+
+int64_t st_solver_footprint(const st_params *p) {
+    if (!p) return -1;
+    size_t n[kNArrays];
+    array_sizes(*p, n);
+    int64_t tot = 0;
+    for (int q = 0; q < kNArrays; q++) tot += (int64_t)arena_align(n[q]);
+    return tot;
+}
+
+int st_solver_create(const st_params *p, st_solver **out) { return create_solver(p, nullptr, out); }
 
 int st_solver_destroy(st_solver *s) {
     if (!s) return ST_OK;
@@ -1123,3 +1188,7 @@ int st_synchronize(st_solver *s) {
 }
 
 }  // extern "C"
+
+namespace st {
+int solver_create_in(const st_params *p, const SlotMem &mem, st_solver **out) { return create_solver(p, &mem, out); }
+}  // namespace st

@@ -459,13 +459,21 @@ __device__ __forceinline__ size_t wrc(const Geo &g, int i, int j, int k, int m) 
 // recycled planes, one thread per (i, j), k summed in order.
 __global__ void k_recyc_avg(Geo g, const double *__restrict__ wrecyc, double *__restrict__ av) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.ng * g.ny) return;
-    const int i = t % g.ng + 1, j = t / g.ng + 1;
-    for (int m = 0; m < 5; m++) {
-        double a = 0.0;
-        for (int k = 1; k <= g.nz; k++) a = a + wrecyc[wrc(g, i, j, k, m)];
-        av[(i - 1) + g.ng * ((j - 1) + g.ny * m)] = a;
+    if (t >= g.ng * g.ny * 5) return;
+    const int i = t % g.ng + 1, j = (t / g.ng) % g.ny + 1, m = t / (g.ng * g.ny);
+    // one (i, j, m) per thread; the sum runs over k in order, its loads
+    // issued eight at a time ahead of the additions
+    double a = 0.0;
+    int k = 1;
+    for (; k + 7 <= g.nz; k += 8) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = wrecyc[wrc(g, i, j, k + q, m)];
+#pragma unroll
+        for (int q = 0; q < 8; q++) a = a + v[q];
     }
+    for (; k <= g.nz; k++) a = a + wrecyc[wrc(g, i, j, k, m)];
+    av[(i - 1) + g.ng * ((j - 1) + g.ny * m)] = a;
 }
 
 // bcrecyc_subroutine_2/3 (kernels_cpu.F90:5832-5947): one thread per
@@ -3232,7 +3240,10 @@ static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix, int vd = 
     hipError_t e = hipGetLastError();
     // the force_rhs partials of this stage: the march's blocks, then the
     // z fix-up's (it always writes them, zero when nothing is flagged)
-    if (vd == 2) s->force_nblk = (int)nblk + kFixBlocks;
+    if (vd == 2) {
+        s->force_nblk = (int)nblk + kFixBlocks;
+        s->zv_nblk = (int)nblk;
+    }
     if (e != hipSuccess || !fix) return e;
     if (vd == 2) return launch_fix(s, fl, dir, 1, 0, s->gamrk[stage], s->partials + (size_t)nblk * 5);
     return launch_fix(s, fl, dir);
@@ -3273,8 +3284,17 @@ hipError_t launch_euler_z(st_solver *s, void *fl, bool fix) { ST_DISPATCH(return
 // ord2) of the same cells and, in the channel case, the force_rhs partials;
 // the z fix-up is part of it (its changes to fl(1:2) enter the partials).
 // Requires a z range spanning all planes, and visc_lmax <= 3 for mode 1.
-hipError_t launch_euler_zv(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce, bool ord2) {
-    ST_DISPATCH(return launch_march(s, (R *)fl, 2, true, reduce ? 2 : 1, (const R *)fl_prev, stage, ord2));
+hipError_t launch_euler_zv(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce, bool ord2,
+                           bool fix) {
+    ST_DISPATCH(return launch_march(s, (R *)fl, 2, fix, reduce ? 2 : 1, (const R *)fl_prev, stage, ord2));
+}
+
+// The z fix-up of launch_euler_zv(fix = false): with the force_rhs partials,
+// its blocks write theirs after the march's
+hipError_t launch_euler_zv_fix(st_solver *s, void *fl, int stage, bool reduce) {
+    if (reduce)
+        ST_DISPATCH(return launch_fix(s, (R *)fl, 2, 1, 0, s->gamrk[stage], s->partials + (size_t)s->zv_nblk * 5));
+    ST_DISPATCH(return launch_fix(s, (R *)fl, 2));
 }
 
 // The fix-up of the last sweep of direction dir, for callers that time it
@@ -3383,7 +3403,7 @@ hipError_t launch_recycling(st_solver *s) {
     const int n1 = g.ng * g.ny, n2 = g.ny * g.nz;
     hipError_t e = launch_recyc_exchange(s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_recyc_avg, dim3((n1 + 127) / 128), dim3(128), 0, s->stream, g, s->wrecyc, av);
+    hipLaunchKernelGGL(k_recyc_avg, dim3((5 * n1 + 127) / 128), dim3(128), 0, s->stream, g, s->wrecyc, av);
     const double u0_02 = s->p.rand_type != 0 && s->rplane ? 0.02 * s->p.u0 : 0.0;
     hipLaunchKernelGGL(k_recyc_inflow, dim3((n2 + 127) / 128), dim3(128), 0, s->stream, g, s->kp, s->w, s->rc_real,
                        s->rc_maps, s->wrecyc, s->betarecyc, s->rplane, u0_02);

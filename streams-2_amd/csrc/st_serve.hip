@@ -1,32 +1,57 @@
 // st_serve.hip -- continuous-batching server of independent flow solves.
 //
-// Requests (parameters, grid, initial field, number of RK3 steps) are
-// queued by st_server_submit.  A host thread owned by the server admits them
-// in arrival order while a slot is free and the HBM footprint of the solve
-// (st_solver_footprint) fits in the remaining capacity, so requests that
-// arrive while others run start as soon as memory frees up.  Every active
-// solve runs on its own stream; its steps are replayed from a HIP graph that
-// captures K steps (launch overhead paid once per K steps), and two chunks
-// are kept in flight per solve so the device never waits for the host.  The
-// work of all active solves therefore overlaps on the device.  A finished
-// solve is copied out, its memory returned, and the next request admitted.
+// Requests (parameters, grid, initial field, number of RK3 steps, optional
+// start time and checkpoint period) are queued by st_server_submit2 and
+// admitted in arrival order while a slot is free and their HBM footprint
+// (st_solver_footprint) fits in the server's memory pool.
 //
-// Launch shapes (the z-chunk of the viscous march and the chunk of the y/z
-// flux marches) come from an autotuner: for each grid-shape class (each
-// extent rounded up to a power of two, plus the precision) the first
-// admitted solve times candidate shapes on one step each and the fastest is
-// cached for every later solve of that class.  Results do not depend on the
-// launch shape.
+// Memory.  The server takes its whole HBM capacity as one device
+// allocation when it is created and carves every admitted solve out of it
+// (first fit on 256-byte blocks, neighbouring free blocks merged on
+// release).  Admission by footprint is therefore an allocation in that
+// pool, and no hipMalloc, hipFree or other device-wide synchronisation
+// runs while solves are on the device.  Each slot owns, for the server's
+// life, its stream, its pinned scalar mirror and table staging and its
+// events.
+//
+// Threads.  The scheduler thread never waits on the device: it admits
+// requests (pool block + slot), launches chunks of K steps on each active
+// slot's stream (from a captured HIP graph once the class's launch shape
+// is settled), keeps two chunks in flight per slot and polls their events.
+// A worker thread does what blocks: the set-up of an admitted solve on its
+// slot (tables, grid and field copies, start of run), the hand-back of a
+// finished one (residual, field copy, pool block and slot released) and
+// the writing of checkpoint files.  The work of all active solves overlaps
+// on the device.
+//
+// Launch shapes are tuned online per grid-shape class (each extent rounded
+// up to a power of two, precision, viscous form): the first chunks of the
+// first solves of a class run with the candidate shapes in turn, timed by
+// events on their own streams, and once every candidate has a time the
+// fastest is kept for the class.  Results do not depend on the launch
+// shape, so the tuning chunks are ordinary steps of the solves: no step is
+// run twice and no solve waits for the tuner.
+//
+// Checkpoints (the restart files of a request, rst.bin + field_info0.dat
+// in the formats of field.F90:1853-1966 and singleideal.F90:2546-2659):
+// every checkpoint_every steps the slot's stream copies w and the scalars
+// into pinned staging after the chunk that reaches the step, and the
+// worker writes the files once the copy has landed.  A request resumed
+// from them (st_server_submit2 with the stored time and step counter)
+// continues as a plain solve restarted from the same files does.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <sys/stat.h>
 #include <thread>
 #include <tuple>
 #include <vector>
@@ -43,28 +68,105 @@ struct ServeReq {
     std::vector<double> grid[13];
     std::vector<double> w0;
     int nsteps = 0;
+    double time0 = 0.0;
+    int64_t icyc0 = 0;
+    int ckpt_every = 0;         // steps between checkpoints (0: none)
+    std::string ckpt_dir;
+    int64_t bytes = 0;          // footprint
     double t_submit = 0, t_admit = 0, t_done = 0;
-    int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
-    int rc = ST_OK;  // status of a failed request (ST_ERR_NAN for a NaN residual)
+    int state = 0;              // 0 queued, 1 running, 2 done, 3 failed
+    int rc = ST_OK;             // status of a failed request (ST_ERR_NAN for a NaN residual)
     std::vector<double> w_out;
     st_scalars sc{};
     std::string err;
+    int checkpoints = 0;        // checkpoint files written
+};
+
+// First-fit allocator over one device block.
+struct Pool {
+    char *base = nullptr;
+    int64_t size = 0;
+    std::map<int64_t, int64_t> free_blocks;  // offset -> bytes
+    static int64_t round(int64_t n) { return (n + 255) / 256 * 256; }
+    int64_t alloc(int64_t n) {
+        n = round(n);
+        for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+            if (it->second < n) continue;
+            const int64_t off = it->first, left = it->second - n;
+            free_blocks.erase(it);
+            if (left > 0) free_blocks[off + n] = left;
+            return off;
+        }
+        return -1;
+    }
+    void release(int64_t off, int64_t n) {
+        n = round(n);
+        auto it = free_blocks.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_blocks.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_blocks.erase(nx);
+        }
+        if (it != free_blocks.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_blocks.erase(it);
+            }
+        }
+    }
+};
+
+enum SlotState { kFree, kSetup, kRunning, kDraining };
+
+constexpr int kTabInts = 1 << 16;  // pinned table staging: nx + ny + nz + 3 ints at most
+
+using TuneKey = std::tuple<int, int, int, int, int>;
+
+struct Tune {
+    std::vector<std::pair<int, int>> cand;  // (visc_chunk, march_chunk)
+    std::vector<float> ms;                  // per step, < 0 while unmeasured
+    std::vector<char> assigned;
+    bool done = false;
+    std::pair<int, int> best{0, 0};
 };
 
 struct Slot {
+    // for the server's life
+    hipStream_t stream = nullptr;
+    st::DevScalars *hs = nullptr;
+    int *tabs = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};          // end of each chunk in flight
+    hipEvent_t tev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // timed bounds of a tuning chunk
+    double *ckpt_w = nullptr;                       // pinned checkpoint staging (5 ghosted variables)
+    st::DevScalars *ckpt_sc = nullptr;
+    size_t ckpt_cap = 0;                            // doubles
+    hipEvent_t ckpt_ev = nullptr;
+    // per request
+    SlotState state = kFree;
+    int rc = ST_OK;            // failure found by the worker during set-up
+    std::string err;
     st_solver *s = nullptr;
     ServeReq *r = nullptr;
-    int64_t bytes = 0;
-    int launched = 0;       // steps enqueued
+    int64_t off = -1;
+    int launched = 0;          // steps enqueued
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
     int ev_steps[2] = {0, 0};  // steps completed when each event fires
+    int tcand[2] = {-1, -1};   // tuning candidate of each chunk in flight
+    int tsteps[2] = {0, 0};
     int nev = 0;               // events in flight (oldest first)
     int head = 0;
+    TuneKey key{};
+    bool ckpt_pending = false;  // a checkpoint copy in flight or being written
+    int ckpt_steps = 0;
 };
 
-using TuneKey = std::tuple<int, int, int, int, int>;
+enum JobKind { kSetupJob, kFinishJob, kCheckpointJob };
+struct Job {
+    JobKind kind;
+    int slot;
+};
 
 int pow2_ceil(int n) {
     int v = 1;
@@ -75,17 +177,20 @@ int pow2_ceil(int n) {
 }  // namespace
 
 struct st_server {
-    int64_t capacity = 0, used = 0;
     int max_slots = 1, K = 10;
     bool autotune = true;
     int device = 0;
-    std::thread th;
+    Pool pool;
+    int64_t used = 0;
+    std::thread th, worker;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_job;
     std::deque<ServeReq *> pending;
+    std::deque<Job> jobs;
     std::map<int64_t, std::unique_ptr<ServeReq>> reqs;
     std::vector<Slot> slots;
-    std::map<TuneKey, std::pair<int, int>> tune;
+    std::map<TuneKey, Tune> tune;
+    int tuned_classes = 0;
     bool stop = false;
     int64_t next_id = 1;
     int64_t completed = 0, failed = 0;
@@ -102,80 +207,88 @@ int sfail(int code, const std::string &msg) {
     return code;
 }
 
-// Time one RK3 step of a freshly initialised solve (the second of two).
-double time_step(st_solver *s, hipEvent_t a, hipEvent_t b) {
-    if (st_advance(s, 1)) return 1e30;
-    (void)hipEventRecord(a, s->stream);
-    if (st_advance(s, 1)) return 1e30;
-    (void)hipEventRecord(b, s->stream);
-    if (hipEventSynchronize(b) != hipSuccess) return 1e30;
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, a, b);
-    return ms;
+bool graphable(const st_server *sv, const ServeReq *r) {
+    // K steps flip the right-hand-side buffers 3K times: K even keeps the
+    // captured buffer order valid for every replay; K a multiple of the dt
+    // period keeps the captured dt launches in phase
+    const int idt = r->p.iter_dt_recompute > 0 ? r->p.iter_dt_recompute : 1;
+    return sv->K >= 2 && sv->K % 2 == 0 && sv->K % idt == 0;
 }
 
-// Pick (visc_chunk, march_chunk) for this solve's shape class: coordinate
-// search over {8, 16, 32} for each, one timed step per candidate, the field
-// reloaded afterwards.  Cached per class.
-int autotune(st_server *sv, st_solver *s, ServeReq *r) {
-    const TuneKey key{pow2_ceil(r->p.nx), pow2_ceil(r->p.ny), pow2_ceil(r->p.nz), r->p.precision,
-                      r->p.conservative_viscous};
-    {
-        // st_server_stats reads the cache size from the client thread
-        std::lock_guard<std::mutex> lk(sv->mu);
-        auto it = sv->tune.find(key);
-        if (it != sv->tune.end()) return st_set_tuning(s, it->second.first, it->second.second);
-    }
-    hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess) return ST_ERR_HIP;
-    if (hipEventCreate(&b) != hipSuccess) {
-        (void)hipEventDestroy(a);
-        return ST_ERR_HIP;
-    }
-    const int cand[3] = {8, 16, 32};
-    int best_m = 0, best_v = 0;
-    double best = 1e30;
-    int rc = ST_OK;
-    for (int m : cand) {
-        if ((rc = st_set_tuning(s, 0, m))) break;
-        if ((rc = st_copy_from_field(s, r->w0.data())) || (rc = st_init_run(s))) break;
-        const double t = time_step(s, a, b);
-        if (t < best) { best = t; best_m = m; }
-    }
-    best = 1e30;
-    for (int v : cand) {
-        if (rc) break;
-        if ((rc = st_set_tuning(s, v, best_m))) break;
-        if ((rc = st_copy_from_field(s, r->w0.data())) || (rc = st_init_run(s))) break;
-        const double t = time_step(s, a, b);
-        if (t < best) { best = t; best_v = v; }
-    }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+int capture(st_server *sv, Slot &sl) {
+    st_solver *s = sl.s;
+    const int64_t icyc = s->icyc;
+    const int cur = s->cur, last = s->last_stage;
+    if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ST_ERR_HIP;
+    int rc = st_advance(s, sv->K);
+    hipError_t e = hipStreamEndCapture(s->stream, &sl.graph);
+    s->icyc = icyc;
+    s->cur = cur;
+    s->last_stage = last;
     if (rc) return rc;
-    {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->tune[key] = {best_v, best_m};
-    }
-    return st_set_tuning(s, best_v, best_m);
+    if (e != hipSuccess) return ST_ERR_HIP;
+    if (hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0) != hipSuccess) return ST_ERR_HIP;
+    return ST_OK;
 }
 
-void release(st_server *sv, Slot &sl) {
-    if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
-    if (sl.graph) (void)hipGraphDestroy(sl.graph);
-    for (auto &e : sl.ev)
-        if (e) (void)hipEventDestroy(e);
-    if (sl.s) st_solver_destroy(sl.s);
-    {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->used -= sl.bytes;
-    }
-    sl = Slot();
-}
-
-void finish(st_server *sv, Slot &sl, int rc, const std::string &err) {
+// Worker: set up an admitted solve on its slot (no allocation; the copies
+// block this thread only).
+int setup(st_server *sv, Slot &sl) {
     ServeReq *r = sl.r;
-    std::lock_guard<std::mutex> lk(sv->mu);
+    st::SlotMem m{sv->pool.base + sl.off, (size_t)r->bytes, sl.stream, sl.hs, sl.tabs, r->time0, r->icyc0};
+    int rc = st::solver_create_in(&r->p, m, &sl.s);
+    if (rc) return rc;
+    const std::vector<double> *gd = r->grid;
+    if ((rc = st_set_grid(sl.s, gd[0].data(), gd[1].data(), gd[2].data(), gd[3].data(), gd[4].data(), gd[5].data(),
+                          gd[6].data(), gd[7].data(), gd[8].data(), gd[9].data(), gd[10].data(), gd[11].data(),
+                          gd[12].data())))
+        return rc;
+    if ((rc = st_copy_from_field(sl.s, r->w0.data())) || (rc = st_init_run(sl.s))) return rc;
+    if (r->ckpt_every > 0) {
+        const size_t need = (size_t)5 * sl.s->g.nvol;
+        if (sl.ckpt_cap < need) {
+            if (sl.ckpt_w) (void)hipHostFree(sl.ckpt_w);
+            sl.ckpt_w = nullptr;
+            sl.ckpt_cap = 0;
+            if (hipHostMalloc((void **)&sl.ckpt_w, need * sizeof(double), hipHostMallocDefault) != hipSuccess)
+                return st::set_error(ST_ERR_NOMEM, "checkpoint staging allocation failed");
+            sl.ckpt_cap = need;
+        }
+    }
+    return ST_OK;
+}
+
+bool write_checkpoint(const ServeReq *r, const st_solver *s, const double *w, const st::DevScalars &sc) {
+    (void)mkdir(r->ckpt_dir.c_str(), 0755);
+    const st::Geo &g = s->g;
+    std::string path = r->ckpt_dir + "/rst.bin";
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    // interior cells of each variable, x fastest (write_field, field.F90:1853-1900)
+    bool ok = true;
+    for (int m = 0; m < 5 && ok; m++)
+        for (int k = 1; k <= g.nz && ok; k++)
+            for (int j = 1; j <= g.ny && ok; j++) {
+                const double *row = w + (size_t)m * g.nvol + st::gidx(g, 1, j, k);
+                ok = std::fwrite(row, sizeof(double), (size_t)g.nx, f) == (size_t)g.nx;
+            }
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok) return false;
+    // field_info0.dat, the keys of write_field_info (singleideal.F90:2620-2659)
+    path = r->ckpt_dir + "/field_info0.dat";
+    f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    std::fprintf(f, "[field_info]\nicyc0 = %lld\ntime0 = %.17g\nitav = 0\nitslice_p3d = 0\n", (long long)sc.icyc,
+                 sc.time);
+    const char *zero_keys[] = {"time_from_last_rst",  "time_from_last_write", "time_from_last_stat",
+                               "time_from_last_slice", "time_from_last_probe", "time_from_last_slice_p3d"};
+    for (const char *k : zero_keys) std::fprintf(f, "%s = 0.0\n", k);
+    std::fprintf(f, "istore = 0\n");
+    return std::fclose(f) == 0;
+}
+
+void record_done(st_server *sv, ServeReq *r, int rc, const std::string &err) {
+    // caller holds sv->mu
     r->t_done = sv->now();
     r->state = rc ? 3 : 2;
     r->rc = rc;
@@ -194,148 +307,308 @@ void finish(st_server *sv, Slot &sl, int rc, const std::string &err) {
     sv->cv_done.notify_all();
 }
 
-// Set up one admitted request on its slot: solver, grid, tuning, field,
-// start of run, and the graph of K steps.
-int admit(st_server *sv, Slot &sl) {
+// Worker: hand a finished (or failed) solve back: scalars and field, then
+// the graph, the solver, the pool block and the slot.
+void finish(st_server *sv, Slot &sl) {
     ServeReq *r = sl.r;
-    int rc;
-    if ((rc = st_solver_create(&r->p, &sl.s))) return rc;
-    const std::vector<double> *gd = r->grid;
-    if ((rc = st_set_grid(sl.s, gd[0].data(), gd[1].data(), gd[2].data(), gd[3].data(), gd[4].data(), gd[5].data(),
-                          gd[6].data(), gd[7].data(), gd[8].data(), gd[9].data(), gd[10].data(), gd[11].data(),
-                          gd[12].data())))
-        return rc;
-    if (sv->autotune && (rc = autotune(sv, sl.s, r))) return rc;
-    // a fresh start of run: time, step counter and buffer order as after
-    // st_solver_create, whatever the tuning steps left behind
-    sl.s->cur = 0;
-    sl.s->last_stage = -1;
-    if ((rc = st_set_time(sl.s, 0.0, r->p.cfl < 0 ? -r->p.cfl : 0.0, 0))) return rc;
-    // the Newton start of T(e) back at t0 (the tuning steps left the
-    // temperature of an advanced state in w_aux(6))
-    if (sv->autotune && st::launch_init_temperature(sl.s) != hipSuccess) return ST_ERR_HIP;
-    if ((rc = st_copy_from_field(sl.s, r->w0.data())) || (rc = st_init_run(sl.s))) return rc;
-    for (auto &e : sl.ev)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return ST_ERR_HIP;
-    // K steps flip the right-hand-side buffers 3K times: K even keeps the
-    // captured buffer order valid for every replay; K a multiple of the dt
-    // period keeps the captured dt launches in phase.
-    const int idt = r->p.iter_dt_recompute > 0 ? r->p.iter_dt_recompute : 1;
-    st_solver *s = sl.s;
-    if (sv->K >= 2 && sv->K % 2 == 0 && sv->K % idt == 0 && r->nsteps >= sv->K) {
-        const int64_t icyc = s->icyc;
-        const int cur = s->cur, last = s->last_stage;
-        if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ST_ERR_HIP;
-        rc = st_advance(s, sv->K);
-        hipError_t e = hipStreamEndCapture(s->stream, &sl.graph);
-        s->icyc = icyc;
-        s->cur = cur;
-        s->last_stage = last;
-        if (rc) return rc;
-        if (e != hipSuccess) return ST_ERR_HIP;
-        if (hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0) != hipSuccess) return ST_ERR_HIP;
+    int rc = sl.rc;
+    std::string err = sl.err;
+    if (sl.s && rc == ST_OK) {
+        rc = st_compute_residual(sl.s, &r->sc);
+        if (rc == ST_OK || rc == ST_ERR_NAN) {
+            err = rc ? st_last_error() : "";
+            r->w_out.resize((size_t)5 * sl.s->g.nvol);
+            const int rc2 = st_copy_to_field(sl.s, r->w_out.data());
+            if (rc == ST_OK && rc2 != ST_OK) {
+                rc = rc2;
+                err = st_last_error();
+            }
+        } else {
+            err = st_last_error();
+        }
     }
+    if (sl.s) (void)hipStreamSynchronize(sl.stream);
+    if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+    if (sl.graph) (void)hipGraphDestroy(sl.graph);
+    if (sl.s) st_solver_destroy(sl.s);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    record_done(sv, r, rc, err);
+    sv->pool.release(sl.off, r->bytes);
+    sv->used -= r->bytes;
+    sl.s = nullptr;
+    sl.r = nullptr;
+    sl.off = -1;
+    sl.exec = nullptr;
+    sl.graph = nullptr;
+    sl.launched = 0;
+    sl.nev = 0;
+    sl.head = 0;
+    sl.rc = ST_OK;
+    sl.err.clear();
+    sl.ckpt_pending = false;
+    sl.ckpt_steps = 0;
+    sl.state = kFree;
+    sv->cv_work.notify_all();
+}
+
+void worker_loop(st_server *sv) {
+    (void)hipSetDevice(sv->device);
+    for (;;) {
+        Job j;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            sv->cv_job.wait(lk, [&] { return sv->stop || !sv->jobs.empty(); });
+            if (sv->jobs.empty()) return;  // stop requested and nothing left
+            j = sv->jobs.front();
+            sv->jobs.pop_front();
+        }
+        Slot &sl = sv->slots[j.slot];
+        if (j.kind == kSetupJob) {
+            int rc = setup(sv, sl);
+            std::string err = rc ? std::string(st_last_error()) : "";
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sl.rc = rc;
+            sl.err = err;
+            sl.state = kRunning;  // the scheduler launches it, or hands a failed set-up back
+            sv->cv_work.notify_all();
+        } else if (j.kind == kCheckpointJob) {
+            const bool ok = write_checkpoint(sl.r, sl.s, sl.ckpt_w, *sl.ckpt_sc);
+            std::lock_guard<std::mutex> lk(sv->mu);
+            if (ok) sl.r->checkpoints++;
+            else {
+                sl.rc = ST_ERR_ARG;
+                sl.err = "checkpoint files could not be written under " + sl.r->ckpt_dir;
+            }
+            sl.ckpt_pending = false;
+            sv->cv_work.notify_all();
+        } else {
+            finish(sv, sl);
+        }
+    }
+}
+
+void push_job(st_server *sv, JobKind k, int slot) {
+    // caller holds sv->mu
+    sv->jobs.push_back(Job{k, slot});
+    sv->cv_job.notify_all();
+}
+
+// Scheduler: the candidate shapes of a class.
+Tune make_tune() {
+    Tune t;
+    t.cand = {{0, 0}, {8, 8}, {8, 16}, {16, 16}, {16, 32}, {32, 16}, {32, 32}, {8, 32}, {32, 8}};
+    t.ms.assign(t.cand.size(), -1.f);
+    t.assigned.assign(t.cand.size(), 0);
+    return t;
+}
+
+// Scheduler: enqueue the next chunk of steps of a slot and record its
+// event.  Every chunk holds at least one step.
+int launch_chunk(st_server *sv, Slot &sl) {
+    st_solver *s = sl.s;
+    ServeReq *r = sl.r;
+    const int left = r->nsteps - sl.launched;
+    const int per = sv->K > 0 ? sv->K : 10;
+    int n = left < per ? left : per;
+    const int slot = (sl.head + sl.nev) % 2;
+    int cand = -1;
+    Tune *t = nullptr;
+    if (sv->autotune) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        auto it = sv->tune.find(sl.key);
+        if (it == sv->tune.end()) it = sv->tune.emplace(sl.key, make_tune()).first;
+        t = &it->second;
+        if (!t->done && n == per) {
+            for (size_t q = 0; q < t->cand.size(); q++)
+                if (!t->assigned[q]) {
+                    t->assigned[q] = 1;
+                    cand = (int)q;
+                    break;
+                }
+        }
+    }
+    int rc;
+    if (cand >= 0) {
+        if ((rc = st_set_tuning(s, t->cand[cand].first, t->cand[cand].second))) return rc;
+        if (hipEventRecord(sl.tev[slot][0], s->stream) != hipSuccess) return ST_ERR_HIP;
+        if ((rc = st_advance(s, n))) return rc;
+        if (hipEventRecord(sl.tev[slot][1], s->stream) != hipSuccess) return ST_ERR_HIP;
+    } else {
+        bool use_graph = false;
+        if (n == sv->K && graphable(sv, r)) {
+            if (!sl.exec && (!t || t->done)) {
+                if (t && (rc = st_set_tuning(s, t->best.first, t->best.second))) return rc;
+                if ((rc = capture(sv, sl))) return rc;
+            }
+            use_graph = sl.exec != nullptr;
+        }
+        if (use_graph) {
+            if (hipGraphLaunch(sl.exec, s->stream) != hipSuccess) return ST_ERR_HIP;
+            s->icyc += n;
+            s->last_stage = 2;
+        } else {
+            if (t && !t->done && (rc = st_set_tuning(s, 0, 0))) return rc;
+            if ((rc = st_advance(s, n))) return rc;
+        }
+    }
+    sl.launched += n;
+    // a checkpoint after the chunk that reaches the period (or the last step)
+    if (r->ckpt_every > 0 && !sl.ckpt_pending &&
+        (sl.launched % r->ckpt_every == 0 || sl.launched == r->nsteps)) {
+        if (hipMemcpyAsync(sl.ckpt_w, s->w, sizeof(double) * 5 * s->g.nvol, hipMemcpyDeviceToHost, s->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(sl.ckpt_sc, s->ds, sizeof(st::DevScalars), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+            hipEventRecord(sl.ckpt_ev, s->stream) != hipSuccess)
+            return ST_ERR_HIP;
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sl.ckpt_pending = true;
+        sl.ckpt_steps = sl.launched;
+    }
+    if (hipEventRecord(sl.ev[slot], s->stream) != hipSuccess) return ST_ERR_HIP;
+    sl.ev_steps[slot] = sl.launched;
+    sl.tcand[slot] = cand;
+    sl.tsteps[slot] = n;
+    sl.nev++;
     return ST_OK;
 }
 
-// Steps per event of a solve run without a graph (steps_per_graph 0 or 1).
-constexpr int kPlainChunk = 10;
-
-// Enqueue the next chunk of steps of a slot and record its event.  Every
-// chunk holds at least one step, so each call advances sl.launched.
-int launch_chunk(st_server *sv, Slot &sl) {
-    st_solver *s = sl.s;
-    const int left = sl.r->nsteps - sl.launched;
-    int n;
-    if (sl.exec && left >= sv->K) {
-        if (hipGraphLaunch(sl.exec, s->stream) != hipSuccess) return ST_ERR_HIP;
-        n = sv->K;
-        s->icyc += n;
-        s->last_stage = 2;
-    } else {
-        const int per = sv->K > 0 ? sv->K : kPlainChunk;
-        n = left < per ? left : per;
-        int rc = st_advance(s, n);
-        if (rc) return rc;
+// Scheduler: retire the chunks whose events have fired; a tuning chunk's
+// time goes to its class.
+void retire(st_server *sv, Slot &sl, bool *progress) {
+    while (sl.nev > 0 && hipEventQuery(sl.ev[sl.head]) == hipSuccess) {
+        const int h = sl.head;
+        if (sl.tcand[h] >= 0) {
+            float ms = 0.f;
+            const bool ok = hipEventElapsedTime(&ms, sl.tev[h][0], sl.tev[h][1]) == hipSuccess;
+            std::lock_guard<std::mutex> lk(sv->mu);
+            Tune &t = sv->tune[sl.key];
+            if (ok) t.ms[sl.tcand[h]] = ms / (float)sl.tsteps[h];
+            else t.assigned[sl.tcand[h]] = 0;
+            bool all = true;
+            for (float v : t.ms) all &= v >= 0.f;
+            if (all && !t.done) {
+                size_t b = 0;
+                for (size_t q = 1; q < t.ms.size(); q++)
+                    if (t.ms[q] < t.ms[b]) b = q;
+                t.best = t.cand[b];
+                t.done = true;
+                sv->tuned_classes++;
+            }
+            sl.tcand[h] = -1;
+        }
+        sl.head = (sl.head + 1) % 2;
+        sl.nev--;
+        *progress = true;
     }
-    sl.launched += n;
-    const int slot = (sl.head + sl.nev) % 2;
-    if (hipEventRecord(sl.ev[slot], s->stream) != hipSuccess) return ST_ERR_HIP;
-    sl.ev_steps[slot] = sl.launched;
-    sl.nev++;
-    return ST_OK;
+}
+
+// Scheduler: a slot that stops early gives back the tuning candidates of
+// its chunks in flight, so that another solve of the class measures them.
+void drop_candidates(st_server *sv, Slot &sl) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (int q = 0; q < 2; q++)
+        if (sl.tcand[q] >= 0) {
+            Tune &t = sv->tune[sl.key];
+            if (t.ms[sl.tcand[q]] < 0.f) t.assigned[sl.tcand[q]] = 0;
+            sl.tcand[q] = -1;
+        }
 }
 
 void serve_loop(st_server *sv) {
     (void)hipSetDevice(sv->device);
     for (;;) {
-        bool idle = true;
-        // admission, in arrival order, while a slot is free and memory fits
-        for (;;) {
-            ServeReq *r = nullptr;
-            int64_t need = 0;
-            {
-                std::unique_lock<std::mutex> lk(sv->mu);
-                if (sv->stop) return;
-                if (sv->pending.empty()) break;
-                bool free_slot = false;
-                for (auto &sl : sv->slots) free_slot |= sl.r == nullptr;
-                if (!free_slot) break;
-                need = st_solver_footprint(&sv->pending.front()->p);
-                if (sv->used + need > sv->capacity && sv->used > 0) break;  // wait for memory
-                r = sv->pending.front();
-                sv->pending.pop_front();
-                r->state = 1;
-                r->t_admit = sv->now();
-                sv->used += need;
-            }
-            idle = false;
-            Slot *slp = nullptr;
-            for (auto &sl : sv->slots)
-                if (sl.r == nullptr) { slp = &sl; break; }
-            Slot &sl = *slp;
-            sl.r = r;
-            sl.bytes = need;
-            int rc = admit(sv, sl);
-            if (rc) {
-                finish(sv, sl, rc, st_last_error());
-                release(sv, sl);
-            }
-        }
-        // progress of the active solves
-        for (auto &sl : sv->slots) {
-            if (!sl.r) continue;
-            while (sl.nev > 0 && hipEventQuery(sl.ev[sl.head]) == hipSuccess) {
-                sl.head = (sl.head + 1) % 2;
-                sl.nev--;
-                idle = false;
-            }
-            int rc = ST_OK;
-            while (rc == ST_OK && sl.nev < 2 && sl.launched < sl.r->nsteps) {
-                rc = launch_chunk(sv, sl);
-                idle = false;
-            }
-            if (rc == ST_OK && sl.nev == 0 && sl.launched >= sl.r->nsteps) {
-                ServeReq *r = sl.r;
-                rc = st_compute_residual(sl.s, &r->sc);
-                if (rc == ST_OK || rc == ST_ERR_NAN) {
-                    r->w_out.resize((size_t)5 * sl.s->g.nvol);
-                    const int rc2 = st_copy_to_field(sl.s, r->w_out.data());
-                    if (rc == ST_OK) rc = rc2;
-                }
-                finish(sv, sl, rc, rc ? st_last_error() : "");
-                release(sv, sl);
-                idle = false;
-            } else if (rc) {
-                finish(sv, sl, rc, st_last_error());
-                release(sv, sl);
-            }
-        }
-        if (idle) {
+        bool progress = false;
+        {
+            // admission, in arrival order, while a slot is free and the pool
+            // holds the footprint
             std::unique_lock<std::mutex> lk(sv->mu);
             if (sv->stop) return;
-            bool active = false;
-            for (auto &sl : sv->slots) active |= sl.r != nullptr;
+            while (!sv->pending.empty()) {
+                int fs = -1;
+                for (int q = 0; q < (int)sv->slots.size(); q++)
+                    if (sv->slots[q].state == kFree) {
+                        fs = q;
+                        break;
+                    }
+                if (fs < 0) break;
+                ServeReq *r = sv->pending.front();
+                const int64_t off = sv->pool.alloc(r->bytes);
+                if (off < 0) break;  // wait for memory
+                sv->pending.pop_front();
+                Slot &sl = sv->slots[fs];
+                sl.state = kSetup;
+                sl.r = r;
+                sl.off = off;
+                sl.rc = ST_OK;
+                sl.key = TuneKey{pow2_ceil(r->p.nx), pow2_ceil(r->p.ny), pow2_ceil(r->p.nz), r->p.precision,
+                                 r->p.conservative_viscous};
+                sl.tcand[0] = sl.tcand[1] = -1;
+                sv->used += r->bytes;
+                r->state = 1;
+                r->t_admit = sv->now();
+                push_job(sv, kSetupJob, fs);
+                progress = true;
+            }
+        }
+        bool active = false;
+        for (int q = 0; q < (int)sv->slots.size(); q++) {
+            Slot &sl = sv->slots[q];
+            SlotState st;
+            bool ckpt_pending;
+            {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                st = sl.state;
+                ckpt_pending = sl.ckpt_pending;
+            }
+            if (st != kRunning) {
+                active |= st != kFree;
+                continue;
+            }
+            active = true;
+            retire(sv, sl, &progress);
+            // a landed checkpoint copy goes to the worker
+            if (ckpt_pending && sl.s && hipEventQuery(sl.ckpt_ev) == hipSuccess) {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                if (sl.ckpt_steps > 0) {
+                    sl.ckpt_steps = 0;
+                    push_job(sv, kCheckpointJob, q);
+                    progress = true;
+                }
+            }
+            int rc;
+            {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                rc = sl.rc;  // a failed set-up, or a checkpoint that could not be written
+            }
+            while (rc == ST_OK && sl.nev < 2 && sl.launched < sl.r->nsteps && !ckpt_pending) {
+                rc = launch_chunk(sv, sl);
+                if (rc) {
+                    std::lock_guard<std::mutex> lk(sv->mu);
+                    sl.err = st_last_error();
+                }
+                {
+                    std::lock_guard<std::mutex> lk(sv->mu);
+                    ckpt_pending = sl.ckpt_pending;
+                }
+                progress = true;
+            }
+            bool ckpt_busy;
+            {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                ckpt_busy = sl.ckpt_pending;
+            }
+            if (rc != ST_OK || (sl.nev == 0 && sl.launched >= sl.r->nsteps && !ckpt_busy)) {
+                if (rc != ST_OK) drop_candidates(sv, sl);
+                std::lock_guard<std::mutex> lk(sv->mu);
+                sl.rc = rc;
+                sl.state = kDraining;
+                push_job(sv, kFinishJob, q);
+                progress = true;
+            }
+        }
+        if (!progress) {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            if (sv->stop) return;
             if (active) {
                 lk.unlock();
                 std::this_thread::sleep_for(std::chrono::microseconds(20));
@@ -344,6 +617,29 @@ void serve_loop(st_server *sv) {
             }
         }
     }
+}
+
+void destroy_slots(st_server *sv) {
+    for (auto &sl : sv->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+        if (sl.graph) (void)hipGraphDestroy(sl.graph);
+        if (sl.s) st_solver_destroy(sl.s);
+        for (auto &e : sl.ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &pr : sl.tev)
+            for (auto &e : pr)
+                if (e) (void)hipEventDestroy(e);
+        if (sl.ckpt_ev) (void)hipEventDestroy(sl.ckpt_ev);
+        if (sl.ckpt_w) (void)hipHostFree(sl.ckpt_w);
+        if (sl.ckpt_sc) (void)hipHostFree(sl.ckpt_sc);
+        if (sl.hs) (void)hipHostFree(sl.hs);
+        if (sl.tabs) (void)hipHostFree(sl.tabs);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        sl = Slot();
+    }
+    if (sv->pool.base) (void)hipFree(sv->pool.base);
+    sv->pool.base = nullptr;
 }
 
 }  // namespace
@@ -362,13 +658,34 @@ int st_server_create(int64_t hbm_capacity_bytes, int32_t max_slots, int32_t step
         delete sv;
         return sfail(ST_ERR_HIP, "no device");
     }
-    sv->capacity = hbm_capacity_bytes;
     sv->max_slots = max_slots;
     sv->K = steps_per_graph;
     sv->autotune = autotune != 0;
     sv->slots.resize(max_slots);
+    // the whole capacity, once: every admitted solve is carved out of it
+    sv->pool.size = Pool::round(hbm_capacity_bytes);
+    hipError_t e = hipMalloc((void **)&sv->pool.base, (size_t)sv->pool.size);
+    if (e == hipSuccess) sv->pool.free_blocks[0] = sv->pool.size;
+    for (auto &sl : sv->slots) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&sl.hs, sizeof(st::DevScalars), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&sl.tabs, sizeof(int) * kTabInts, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&sl.ckpt_sc, sizeof(st::DevScalars), hipHostMallocDefault);
+        for (auto &ev : sl.ev)
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        for (auto &pr : sl.tev)
+            for (auto &ev : pr)
+                if (e == hipSuccess) e = hipEventCreate(&ev);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ckpt_ev, hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        destroy_slots(sv);
+        delete sv;
+        return sfail(ST_ERR_NOMEM, std::string("server memory: ") + hipGetErrorString(e));
+    }
     sv->t0 = Clock::now();
     sv->th = std::thread(serve_loop, sv);
+    sv->worker = std::thread(worker_loop, sv);
     *out = sv;
     return ST_OK;
 }
@@ -381,24 +698,41 @@ int st_server_destroy(st_server *sv) {
     }
     sv->cv_work.notify_all();
     if (sv->th.joinable()) sv->th.join();
-    for (auto &sl : sv->slots) {
-        if (sl.s) (void)hipStreamSynchronize(sl.s->stream);
-        release(sv, sl);
-    }
+    // the worker finishes the jobs already queued, then stops
+    sv->cv_job.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    destroy_slots(sv);
     delete sv;
     return ST_OK;
 }
 
-int st_server_submit(st_server *sv, const st_params *p, const double *x, const double *y, const double *z,
-                     const double *yn, const double *dcsidx, const double *dcsidxs, const double *dcsidx2,
-                     const double *detady, const double *detadys, const double *detady2, const double *dzitdz,
-                     const double *dzitdzs, const double *dzitdz2, const double *w0, int32_t nsteps, int64_t *ticket) {
-    if (!sv || !p || !w0 || !ticket || nsteps < 0) return sfail(ST_ERR_ARG, "bad submit arguments");
+int st_server_submit2(st_server *sv, const st_params *p, const double *x, const double *y, const double *z,
+                      const double *yn, const double *dcsidx, const double *dcsidxs, const double *dcsidx2,
+                      const double *detady, const double *detadys, const double *detady2, const double *dzitdz,
+                      const double *dzitdzs, const double *dzitdz2, const double *w0, int32_t nsteps, double time0,
+                      int64_t icyc0, int32_t checkpoint_every, const char *checkpoint_dir, int64_t *ticket) {
+    if (!sv || !p || !w0 || !ticket || nsteps < 0 || checkpoint_every < 0 || icyc0 < 0)
+        return sfail(ST_ERR_ARG, "bad submit arguments");
+    if (checkpoint_every > 0 && (!checkpoint_dir || !*checkpoint_dir))
+        return sfail(ST_ERR_ARG, "a checkpoint period needs a checkpoint directory");
+    if (checkpoint_every > 0 && sv->K > 0 && checkpoint_every % sv->K != 0)
+        return sfail(ST_ERR_ARG, "the checkpoint period must be a multiple of steps_per_graph");
+    int rc = st::validate_params(p);
+    if (rc) return sfail(rc, st_last_error());
+    if (p->nx + p->ny + p->nz + 3 > kTabInts) return sfail(ST_ERR_ARG, "grid too large for the server");
+    if (p->bctags[0] == 10 || p->bctags[0] == 9)
+        return sfail(ST_ERR_ARG, "the server takes no inflow planes (recycling or laminar inflow)");
     const int64_t need = st_solver_footprint(p);
-    if (need <= 0 || need > sv->capacity) return sfail(ST_ERR_ARG, "request larger than the server capacity");
+    if (need <= 0 || Pool::round(need) > sv->pool.size)
+        return sfail(ST_ERR_ARG, "request larger than the server capacity");
     auto r = std::make_unique<ServeReq>();
     r->p = *p;
     r->nsteps = nsteps;
+    r->time0 = time0;
+    r->icyc0 = icyc0;
+    r->ckpt_every = checkpoint_every;
+    if (checkpoint_every > 0) r->ckpt_dir = checkpoint_dir;
+    r->bytes = need;
     const int nx = p->nx, ny = p->ny, nz = p->nz, ng = p->ng;
     const size_t n[13] = {(size_t)nx + 2 * ng, (size_t)ny + 2 * ng, (size_t)nz + 2 * ng, (size_t)ny + 1,
                           (size_t)nx, (size_t)nx, (size_t)nx, (size_t)ny, (size_t)ny, (size_t)ny,
@@ -420,10 +754,14 @@ int st_server_submit(st_server *sv, const st_params *p, const double *x, const d
     return ST_OK;
 }
 
-// Wait (up to timeout_s, < 0: forever) for a request; copies the final field
-// (may be NULL), its scalars (may be NULL) and the times of submission,
-// admission and completion in seconds since the server started (may be NULL),
-// then forgets the request.  Returns ST_ERR_STATE on time-out.
+int st_server_submit(st_server *sv, const st_params *p, const double *x, const double *y, const double *z,
+                     const double *yn, const double *dcsidx, const double *dcsidxs, const double *dcsidx2,
+                     const double *detady, const double *detadys, const double *detady2, const double *dzitdz,
+                     const double *dzitdzs, const double *dzitdz2, const double *w0, int32_t nsteps, int64_t *ticket) {
+    return st_server_submit2(sv, p, x, y, z, yn, dcsidx, dcsidxs, dcsidx2, detady, detadys, detady2, dzitdz, dzitdzs,
+                             dzitdz2, w0, nsteps, 0.0, 0, 0, nullptr, ticket);
+}
+
 int st_server_wait(st_server *sv, int64_t ticket, double timeout_s, double *w_out, st_scalars *sc, double times[3]) {
     if (!sv) return sfail(ST_ERR_ARG, "null server");
     std::unique_lock<std::mutex> lk(sv->mu);
@@ -447,8 +785,6 @@ int st_server_wait(st_server *sv, int64_t ticket, double timeout_s, double *w_ou
     return rc;
 }
 
-// Counters: [0] queued, [1] running, [2] completed, [3] failed, [4] HBM bytes
-// reserved by running solves, [5] tuned shape classes.
 int st_server_stats(st_server *sv, int64_t out[6]) {
     if (!sv || !out) return sfail(ST_ERR_ARG, "null argument");
     std::lock_guard<std::mutex> lk(sv->mu);
@@ -459,7 +795,7 @@ int st_server_stats(st_server *sv, int64_t out[6]) {
     out[2] = sv->completed;
     out[3] = sv->failed;
     out[4] = sv->used;
-    out[5] = (int64_t)sv->tune.size();
+    out[5] = sv->tuned_classes;
     return ST_OK;
 }
 

@@ -47,6 +47,7 @@ struct st_solver {
     bool fuse_xy;       // x and y Euler fluxes in one pass
     bool fuse_zv;       // viscous divergence terms folded into the z march (Laplacian form)
     int force_nblk;     // force_rhs partials written in this stage (visc_div blocks, or z march + fix-up)
+    int zv_nblk;        // blocks of the last z march with the force_rhs partials (the fix-up's come after them)
     double *wallprop;   // 3 x (sx*sz), wallprop(:,:,2:4)
     int *eoc[3];        // ep_ord_change per direction, faces 0..n
     // grid
@@ -67,6 +68,10 @@ struct st_solver {
     int64_t icyc, icyc0; // host mirror of the step counter
     // optional per-kernel timing (st_kernel_times)
     struct KTimer *timer;
+    // memory lent by the creator (st::SlotMem): [arena_lo, arena_hi) holds
+    // the solver's arrays, hs is the lender's; neither is freed with the solver
+    char *arena_lo, *arena_hi;
+    bool borrowed_hs;
 };
 
 // HIP events around each kernel family while st_kernel_times runs
@@ -79,6 +84,25 @@ struct KTimer {
 };
 
 namespace st {
+
+// Memory a caller lends a solver so that creating it allocates nothing:
+// a device arena of at least st_solver_footprint bytes (256-byte aligned)
+// carved into the solver's arrays, the stream its work is ordered on,
+// a pinned scalar mirror, and pinned room for nx + ny + nz + 3 ints of
+// tables that must stay untouched until the stream has passed the
+// creation.  The lender keeps all of it alive until the solver is
+// destroyed and its stream has drained.
+struct SlotMem {
+    char *arena;
+    size_t bytes;
+    hipStream_t stream;
+    DevScalars *hs;
+    int *tabs;
+    double time0;   // start time and step counter of the run (st_set_time without its synchronisation)
+    int64_t icyc0;
+};
+// st_solver_create inside lent memory: no allocation and no synchronisation
+int solver_create_in(const st_params *p, const SlotMem &mem, st_solver **out);
 
 // parameter handling shared by the entry points (st_capi.hip)
 int set_error(int code, const std::string &msg);  // message for st_last_error; returns code
@@ -133,7 +157,11 @@ hipError_t launch_euler_y(st_solver *s, void *fl, bool fix = true);
 hipError_t launch_euler_xy(st_solver *s, void *fl, bool fix = true);
 hipError_t launch_euler_z(st_solver *s, void *fl, bool fix = true);
 hipError_t launch_weno_fix(st_solver *s, void *fl, int dir, int bit, int anyidx);
-hipError_t launch_euler_zv(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce, bool ord2);
+// z march + viscous divergence (+ force_rhs partials); fix = false leaves
+// its WENO fix-up to launch_euler_zv_fix, which must follow it
+hipError_t launch_euler_zv(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce, bool ord2,
+                           bool fix = true);
+hipError_t launch_euler_zv_fix(st_solver *s, void *fl, int stage, bool reduce);
 hipError_t launch_visc(st_solver *s, void *fl, int mode, bool ghost_images);
 hipError_t launch_visc_cons(st_solver *s, void *fl, int mode, bool ghost_images);
 hipError_t launch_sensor(st_solver *s, bool ghost_images);

@@ -83,6 +83,8 @@ _SIGNATURES = {
     "st_server_destroy": (C.c_int, [C.c_void_p]),
     "st_server_submit": (C.c_int, [C.c_void_p, C.POINTER(StParams)] + [C.POINTER(C.c_double)] * 14
                          + [C.c_int32, C.POINTER(C.c_int64)]),
+    "st_server_submit2": (C.c_int, [C.c_void_p, C.POINTER(StParams)] + [C.POINTER(C.c_double)] * 14
+                          + [C.c_int32, C.c_double, C.c_int64, C.c_int32, C.c_char_p, C.POINTER(C.c_int64)]),
     "st_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(C.c_double), C.POINTER(StScalars),
                                  C.POINTER(C.c_double)]),
     "st_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),

@@ -354,3 +354,118 @@ def bl_mean_planes(case: ChannelCase) -> np.ndarray:
     prof = w[:4, ng:-ng, ng:ng + ny, ng].mean(axis=1)  # [m, j] at i = 1
     prof[2:4] = 0.0
     return np.repeat(prof[:, :, None], ng, axis=2)
+
+
+def oblique_shock_states(params: StParams, shock_angle_deg: float) -> dict:
+    """Ratios across an oblique shock of angle beta in the free stream of
+    ``params`` (Mach u0/c0): density, temperature, post-shock velocity
+    components (v < 0, toward the wall) and the flow deflection, from the
+    Rankine-Hugoniot relations for the normal Mach number M sin(beta)."""
+    gam = params.cp_coeff[0] / params.cv_coeff[0]
+    gm1 = gam - 1.0
+    c0 = math.sqrt(gam * params.rgas0 * params.t0)
+    mach = params.u0 / c0
+    beta = shock_angle_deg * math.pi / 180.0
+    mn2 = (mach * math.sin(beta)) ** 2
+    rho_ratio = (gam + 1.0) * mn2 / (2.0 + gm1 * mn2)
+    p_ratio = 1.0 + 2.0 * gam / (gam + 1.0) * (mn2 - 1.0)
+    t_ratio = p_ratio / rho_ratio
+    tdef = 2.0 / math.tan(beta) * (mn2 - 1.0) / (2.0 + mach ** 2 * (gam + math.cos(2.0 * beta)))
+    delta = math.atan(tdef)
+    mn_ps = math.sqrt((1.0 + 0.5 * gm1 * mn2) / (gam * mn2 - 0.5 * gm1))
+    m_ps = mn_ps / math.sin(beta - delta)
+    q_ps = m_ps * math.sqrt(gam * params.rgas0 * params.t0 * t_ratio)
+    return {"rho": rho_ratio, "t": t_ratio, "p": p_ratio, "u": q_ps * math.cos(delta), "v": -q_ps * math.sin(delta),
+            "deflection": delta, "beta": beta}
+
+
+def shock_angle_from_deflection(mach: float, theta_deg: float, gam: float = 1.4) -> float:
+    """Weak-solution shock angle (degrees) of a wedge of ``theta_deg`` in a
+    stream of Mach ``mach``: the root of the theta-beta-M relation
+    tan(theta) = 2 cot(beta) (M^2 sin^2 beta - 1) / (M^2 (gam + cos 2 beta) + 2)
+    between the Mach angle and the angle of largest deflection (bisection)."""
+    def tdef(b):
+        return 2.0 / math.tan(b) * ((mach * math.sin(b)) ** 2 - 1.0) / (mach ** 2 * (gam + math.cos(2.0 * b)) + 2.0)
+    lo = math.asin(1.0 / mach)
+    # the deflection rises from zero at the Mach angle to its maximum, then falls
+    grid = np.linspace(lo, 0.5 * math.pi, 2001)
+    hi = float(grid[int(np.argmax([tdef(b) for b in grid]))])
+    target = math.tan(theta_deg * math.pi / 180.0)
+    if tdef(hi) < target:
+        raise ValueError("deflection beyond the attached-shock limit")
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if tdef(mid) < target:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi) * 180.0 / math.pi
+
+
+def impose_oblique_shock(grid: Grid, params: StParams, w: np.ndarray, shock_angle_deg: float, x_imp: float,
+                         width: float = 2.0) -> np.ndarray:
+    """The incident oblique shock already across the field: behind the line
+    x = x_imp - y / tan(beta) (the shock that the top boundary imposes and
+    that meets the wall at x_imp) density and temperature take the jump
+    ratios, u its post-shock ratio and v the post-shock downward velocity,
+    both scaled by the local u/u0 so that the wall stays at rest; a tanh
+    ramp of ``width`` x-spacings smooths the jump.  Interior cells only;
+    total energy from the new state (calorically perfect gas)."""
+    ng, nx, ny, nz = grid.ng, grid.nx, grid.ny, grid.nz
+    s = oblique_shock_states(params, shock_angle_deg)
+    x = grid.x[ng:ng + nx]
+    y = grid.y[ng:ng + ny]
+    dx = float(x[1] - x[0])
+    xs = x_imp - np.maximum(y, 0.0) / math.tan(s["beta"])
+    f = 0.5 * (1.0 + np.tanh((x[None, :] - xs[:, None]) / (width * dx)))[None, :, :]  # [1, j, i]
+    out = w.copy()
+    sl = (slice(ng, ng + nz), slice(ng, ng + ny), slice(ng, ng + nx))
+    rho = w[0][sl]
+    u, v, ww = w[1][sl] / rho, w[2][sl] / rho, w[3][sl] / rho
+    cv0, cv1, t0 = params.cv_coeff[0], params.cv_coeff[1], params.t0
+    tt = (w[4][sl] / rho - 0.5 * (u * u + v * v + ww * ww) - cv1 * t0) / cv0 + t0
+    uu0 = u / params.u0
+    rho2 = rho * (1.0 + (s["rho"] - 1.0) * f)
+    t2 = tt * (1.0 + (s["t"] - 1.0) * f)
+    u2 = u * (1.0 + (s["u"] / params.u0 - 1.0) * f)
+    v2 = v + f * s["v"] * uu0
+    e2 = cv1 * t0 + cv0 * (t2 - t0)
+    out[0][sl] = rho2
+    out[1][sl] = rho2 * u2
+    out[2][sl] = rho2 * v2
+    out[3][sl] = rho2 * ww
+    out[4][sl] = rho2 * (e2 + 0.5 * (u2 * u2 + v2 * v2 + ww * ww))
+    return out
+
+
+def bl_recycling_case(nx=1024, ny=256, nz=192, lx=70.0, ly=12.0, lz=6.5, beta_y=3.0, mach=2.0, reynolds=200.0,
+                      shock_angle_deg=None, xshock_imp=40.0, developed_shock=False, sensor_threshold=0.2,
+                      conservative_viscous=1, rand_type=2, precision=0, noise=0.05, seed=0, x_recyc=20.0,
+                      cfl=0.5):
+    """Supersonic boundary layer over lx x ly x lz (inflow thickness delta = 1)
+    with the boundary set of the reference's boundary-layer and SBLI decks:
+    recycling-rescaling inflow at xmin (station x_recyc, random inflow plane
+    per ``rand_type``), supersonic outflow with the non-reflecting treatment
+    at xmax, isothermal wall at the recovery temperature, periodic span; the
+    top is an extrapolated non-reflecting boundary, or, with
+    ``shock_angle_deg``, the imposed oblique shock meeting the wall at
+    ``xshock_imp``.  ``developed_shock`` starts from a field that the incident
+    shock already crosses.  Initial field: the seeded compressible profile
+    plus ``noise`` velocity fluctuations.  Returns (case, recycling)."""
+    g0 = bl_grid(nx, ny, nz, lx=lx, ly=ly, lz=lz, beta_y=beta_y)
+    gam, pr = 1.4, 0.72
+    t_rec = 1.0 + 0.5 * (gam - 1.0) * pr ** (1.0 / 3.0) * mach ** 2
+    if shock_angle_deg is None:
+        tags, tags_nr, kw = (10, 2, 6, 2, 0, 0), (0, 1, 0, 1, 0, 0), {}
+    else:
+        tags, tags_nr = (10, 2, 6, 7, 0, 0), (0, 1, 0, 1, 0, 0)
+        kw = dict(shock_angle_deg=shock_angle_deg, xshock_imp=xshock_imp, tanhfacs=lx / (nx - 1))
+    p = bl_params(nx, ny, nz, mach=mach, reynolds=reynolds, t_wall=t_rec, bctags=tags, bctags_nr=tags_nr,
+                  sensor_threshold=sensor_threshold, conservative_viscous=conservative_viscous, precision=precision,
+                  rand_type=rand_type, cfl=cfl, **kw)
+    w0 = bl_initial_field(g0, p, delta=1.0, noise=noise, seed=seed)
+    c = ChannelCase(FlowCase(nx=nx, ny=ny, nz=nz), g0, p, w0)
+    rc = recycling_setup(c.grid, x_recyc, 1.0, 0.01, 1.1, 0.011, bl_mean_planes(c))
+    if shock_angle_deg is not None and developed_shock:
+        c.w0 = impose_oblique_shock(g0, p, w0, shock_angle_deg, xshock_imp)
+    return c, rc

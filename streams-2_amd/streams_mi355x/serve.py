@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 import time
 from dataclasses import dataclass
 
@@ -34,12 +35,18 @@ class Server:
         self.h = h
         self._shapes = {}
 
-    def submit(self, params: StParams, grid, w0: np.ndarray, nsteps: int) -> int:
+    def submit(self, params: StParams, grid, w0: np.ndarray, nsteps: int, time0: float = 0.0, icyc0: int = 0,
+               checkpoint_every: int = 0, checkpoint_dir: str | None = None) -> int:
+        """Queue a solve of ``nsteps`` steps from ``w0`` (ghosted), starting at
+        ``time0`` / step ``icyc0`` (a resumed request), with rst.bin and
+        field_info0.dat written to ``checkpoint_dir`` every
+        ``checkpoint_every`` steps and after the last one."""
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in grid.arrays()]
         w0 = np.ascontiguousarray(w0, dtype=np.float64)
         t = C.c_int64()
-        capi.check_server(self.L.st_server_submit(self.h, C.byref(params), *[_dp(a) for a in arrs], _dp(w0), nsteps,
-                                                  C.byref(t)))
+        cdir = None if checkpoint_dir is None else os.fspath(checkpoint_dir).encode()
+        capi.check_server(self.L.st_server_submit2(self.h, C.byref(params), *[_dp(a) for a in arrs], _dp(w0), nsteps,
+                                                   float(time0), int(icyc0), int(checkpoint_every), cdir, C.byref(t)))
         self._shapes[t.value] = w0.shape
         return t.value
 

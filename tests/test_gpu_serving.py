@@ -92,3 +92,61 @@ def test_request_larger_than_capacity_is_rejected():
     with Server(hbm_bytes=1 << 20, max_slots=2) as sv:
         with pytest.raises(Exception):
             sv.submit(c.params, c.grid, c.w0, 2)
+
+
+def test_checkpoint_files_and_resume(tmp_path):
+    """Per-request checkpoints: a served request writes rst.bin and
+    field_info0.dat every 10 steps and after its last step.  The files hold
+    its final interior field bit for bit and its step and time; the request
+    resumed from them through the server ends bit-identical to the same
+    stop/restart through the plain path, and matches the oracle continued
+    from the same files."""
+    from helpers import fill_ghosts_wrap, rel_l2
+    from oracle import Oracle
+    from streams_mi355x.io import read_field_info, read_restart
+    from streams_mi355x.solver import ChannelSolver
+
+    reqs = _requests()[:3]
+    dirs = [tmp_path / f"req{n}" for n in range(len(reqs))]
+    with Server(hbm_bytes=8 << 30, max_slots=3, steps_per_graph=10, autotune=True) as sv:
+        tk = [sv.submit(r.case.params, r.case.grid, r.case.w0, 20, checkpoint_every=10, checkpoint_dir=str(d))
+              for r, d in zip(reqs, dirs)]
+        first = [sv.wait(t, 120.0) for t in tk]
+    resumed_w = []
+    for r, d, (w, sc, _) in zip(reqs, dirs, first):
+        p = r.case.params
+        wi = read_restart(d / "rst.bin", p.nx, p.ny, p.nz)
+        info = read_field_info(d / "field_info0.dat")
+        assert np.array_equal(wi, interior(w))
+        assert info["icyc0"] == sc["icyc"] == 20 and info["time0"] == sc["time"]
+        w0 = np.zeros_like(r.case.w0)
+        w0[:, 3:-3, 3:-3, 3:-3] = wi
+        resumed_w.append((fill_ghosts_wrap(w0), info))
+    with Server(hbm_bytes=8 << 30, max_slots=3, steps_per_graph=10, autotune=True) as sv:
+        tk = [sv.submit(r.case.params, r.case.grid, w0, 10, time0=info["time0"], icyc0=info["icyc0"])
+              for r, (w0, info) in zip(reqs, resumed_w)]
+        second = [sv.wait(t, 120.0) for t in tk]
+    for n, (r, (w0, info), (w, sc, _)) in enumerate(zip(reqs, resumed_w, second)):
+        assert sc["icyc"] == 30
+        with ChannelSolver(r.case.params, r.case.grid) as g:
+            g.copy_from_field(w0)
+            g.set_time(info["time0"], 0.0, info["icyc0"])
+            g.init_run()
+            g.advance(10)
+            assert np.array_equal(interior(g.copy_to_field()), interior(w)), n
+            assert g.scalars()["time"] == sc["time"]
+        if n == 0:
+            o = Oracle(r.case.params, r.case.grid)
+            o.w[:] = w0
+            o.set_time(info["time0"], 0.0, info["icyc0"])
+            o.init_run()
+            o.advance(10)
+            errs = [rel_l2(interior(w)[m], interior(o.w)[m]) for m in range(5)]
+            assert max(errs) <= 1e-10, errs
+
+
+def test_checkpoint_period_must_be_whole_graphs(tmp_path):
+    c = make_channel()
+    with Server(hbm_bytes=8 << 30, max_slots=1, steps_per_graph=10) as sv:
+        with pytest.raises(Exception):
+            sv.submit(c.params, c.grid, c.w0, 20, checkpoint_every=7, checkpoint_dir=str(tmp_path))
