@@ -103,9 +103,9 @@ int validate(const st_params *p) {
     for (int d = 0; d < 3; d++)
         if ((p->bctags[2 * d] == 0) != (p->bctags[2 * d + 1] == 0))
             return fail(ST_ERR_ARG, "a periodic direction needs tag 0 on both faces");
-    double vol = ((double)p->nx + 6) * ((double)p->ny + 6) * ((double)p->nz + 6);
-    const double n_aux = p->precision == ST_PREC_MIXED ? 11.0 : 10.0;  // w_aux variables
-    if (vol * n_aux > 2147483647.0) return fail(ST_ERR_ARG, "grid too large for 32-bit indexing of w_aux");
+    // a point's index within one variable is 32-bit (offsets across variables are 64-bit)
+    const double vol = ((double)p->nx + 2.0 * p->ng) * ((double)p->ny + 2.0 * p->ng) * ((double)p->nz + 2.0 * p->ng);
+    if (vol > 2147483647.0 - 4096.0) return fail(ST_ERR_ARG, "grid too large: a ghosted variable must hold fewer than 2^31 points");
     return ST_OK;
 }
 
@@ -205,8 +205,8 @@ st::Geo st::params_to_geo(const st_params &p) {
     g.nx = p.nx; g.ny = p.ny; g.nz = p.nz; g.ng = p.ng;
     g.sx = p.nx + 2 * p.ng; g.sy = p.ny + 2 * p.ng; g.sz = p.nz + 2 * p.ng;
     g.sxy = g.sx * g.sy;
-    g.nvol = g.sxy * g.sz;
-    g.nint = p.nx * p.ny * p.nz;
+    g.nvol = (long long)g.sxy * g.sz;
+    g.nint = (long long)p.nx * p.ny * p.nz;
     return g;
 }
 

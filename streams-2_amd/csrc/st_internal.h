@@ -21,8 +21,8 @@ struct Geo {
     int nx, ny, nz, ng;
     int sx, sy, sz;      // ghosted extents
     int sxy;             // sx*sy
-    int nvol;            // points of one ghosted variable
-    int nint;            // points of one interior variable
+    long long nvol;      // points of one ghosted variable (offsets across variables are 64-bit)
+    long long nint;      // points of one interior variable
 };
 
 __host__ __device__ __forceinline__ int gidx(const Geo &g, int i, int j, int k) {

@@ -696,7 +696,7 @@ __global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__rest
 // same divergence as the reference and clears the flag.
 // flags[nint] doubles as the any-dirty word of the sweep: one plain store per
 // wave that has a dirty lane (no returned value, so the wave never waits).
-__device__ __forceinline__ void mark_dirty(unsigned char *__restrict__ flags, int nint, bool dirty, int cell) {
+__device__ __forceinline__ void mark_dirty(unsigned char *__restrict__ flags, long long nint, bool dirty, int cell) {
     const unsigned long long mask = __ballot(dirty);
     if (dirty) flags[cell] = 1;
     if (mask != 0ull && __lane_id() == __ffsll((long long)mask) - 1) flags[nint] = 1;
@@ -3124,7 +3124,7 @@ static hipError_t launch_fix(st_solver *s, R *fl, int dir, int bit = 1, int anyi
                              double *partials = nullptr) {
     const Geo &g = s->g;
     // segments of 2048 words (16384 cells) or longer, at most kFixMaxSeg
-    const int nword = (g.nint + 7) / 8;
+    const int nword = (int)((g.nint + 7) / 8);
     int nseg = (nword + 2047) / 2048;
     if (nseg > kFixMaxSeg) nseg = kFixMaxSeg;
     if (nseg < 1) nseg = 1;

@@ -104,3 +104,19 @@ def test_kernel_families_follow_the_header():
     assert len(capi.KERNEL_FAMILIES) == nfam == len(idx)
     for name, i in idx.items():
         assert capi.KERNEL_FAMILIES[i] == name, (name, i)
+
+
+def test_grid_limit_is_2_to_the_31_points_per_variable():
+    """A point's index within one ghosted variable is 32-bit and offsets
+    across variables are 64-bit: 1300^3 cells (2.2e9 ghosted points) are
+    rejected before any device call; the footprint of 1024 x 512 x 448 (2.4e9
+    w_aux values, beyond a 32-bit offset) is reported, 64-bit."""
+    from streams_mi355x import make_channel
+    p = make_channel().params
+    p.nx = p.ny = p.nz = 1300
+    h = C.c_void_p()
+    assert capi.lib().st_solver_create(C.byref(p), C.byref(h)) == capi.ST_ERR_ARG
+    assert b"2^31 points" in capi.lib().st_last_error()
+    p.nx, p.ny, p.nz = 1024, 512, 448
+    foot = capi.lib().st_solver_footprint(C.byref(p))
+    assert foot > 10 * 1030 * 518 * 454 * 8 > 2 ** 31 * 8
