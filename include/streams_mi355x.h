@@ -337,13 +337,14 @@ int st_server_submit(st_server *sv, const st_params *p,
                      const double *detady, const double *detadys, const double *detady2,
                      const double *dzitdz, const double *dzitdzs, const double *dzitdz2,
                      const double *w0, int32_t nsteps, int64_t *ticket);
-/* st_server_submit with a start (time0, icyc0: a request resumed from
- * its restart files, as st_set_time) and, with checkpoint_every > 0 (a
- * multiple of steps_per_graph), the request's restart files rst.bin and
- * field_info0.dat written to checkpoint_dir every checkpoint_every steps and
- * after its last step, in the formats of write_field / write_field_info
- * (field.F90:1853-1966, singleideal.F90:2546-2659).  Requests with an inflow
- * plane (xmin tags 9, 10) are not taken. */
+/* st_server_submit with a start (time0, icyc0: a request resumed from its
+ * restart files, as st_set_time) and, with checkpoint_every > 0 (a multiple
+ * of steps_per_graph, or of 10 without graphs), the request's restart files
+ * rst.bin and field_info0.dat written to checkpoint_dir every
+ * checkpoint_every steps and after its last step, in the formats of
+ * write_field / write_field_info (field.F90:1853-1966,
+ * singleideal.F90:2546-2659).  Requests with an inflow plane (xmin tags 9,
+ * 10) are not taken. */
 int st_server_submit2(st_server *sv, const st_params *p,
                       const double *x, const double *y, const double *z, const double *yn,
                       const double *dcsidx, const double *dcsidxs, const double *dcsidx2,

@@ -1588,6 +1588,102 @@ __device__ __forceinline__ void fix_cell(const Geo &g, const KPar &P, const R *_
     }
 }
 
+// The x fix-up of a wave of listed cells.  Lanes 0..62 own the cells of
+// 63 consecutive list entries; lane 63 takes the next entry as a helper.
+// Every lane evaluates the lower face of its cell; when the next lane holds
+// the next cell along x (a run of flagged cells), that face is the upper
+// face of this lane's cell and comes with a lane shift, so inside a run
+// each face is evaluated once.  Only the last cell of a run evaluates its
+// upper face itself, which is then mostly a central face.  Each face is
+// evaluated exactly as fix_cell evaluates it (the same window and branch),
+// so the results are those of fix_cell.
+template <typename R>
+__device__ __forceinline__ void fix_cell_x(const Geo &g, const KPar &P, const R *__restrict__ wa,
+                                           const double *__restrict__ w, const int *__restrict__ eoc,
+                                           const double *__restrict__ metric, R *__restrict__ fl, int oi, bool act,
+                                           bool owner) {
+    const size_t nv = g.nvol;
+    const bool keep = P.nkeep >= 0;
+    const int n = g.nx;
+    const int i = act ? oi % g.nx + 1 : 1;
+    const int j = act ? (oi / g.nx) % g.ny + 1 : 1;
+    const int k = act ? oi / (g.nx * g.ny) + 1 : 1;
+    const int base = gidx(g, i, j, k);
+    Pt q[7];
+    double sen[7];
+    double flo[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (act) {
+        double raw[7][6];
+#pragma unroll
+        for (int s = 0; s < 7; s++) {
+            const int a = base + (s - 3);
+            if constexpr (sizeof(R) == sizeof(double)) {
+#pragma unroll
+                for (int v = 0; v < 6; v++) raw[s][v] = double(wa[v * nv + a]);
+            } else {
+                int pi = i + s - 3;
+                if (P.periodic[0]) pi = pi < 1 ? pi + g.nx : (pi > g.nx ? pi - g.nx : pi);
+                const int src = gidx(g, pi, j, k);
+#pragma unroll
+                for (int v = 0; v < 5; v++) raw[s][v] = w[v * nv + src];
+                raw[s][5] = double(wa[5 * nv + a]);
+            }
+            sen[s] = double(wa[7 * nv + a]);
+        }
+#pragma unroll
+        for (int s = 0; s < 7; s++) {
+            if constexpr (sizeof(R) == sizeof(double)) {
+                q[s] = make_pt(raw[s][0], raw[s][1], raw[s][2], raw[s][3], raw[s][4], raw[s][5], P.rgas0, keep);
+            } else {
+                const double rho = raw[s][0], rhoe = raw[s][4];
+                const double ri = 1.0 / rho;
+                const double uu = raw[s][1] * ri, vv = raw[s][2] * ri, ww = raw[s][3] * ri;
+                const double qq = 0.5 * (uu * uu + vv * vv + ww * ww);
+                const double ee = rhoe / rho - qq;
+                const double tt = temperature_from_e(P, ee, raw[s][5]);
+                const double pp = rho * tt * P.rgas0;
+                q[s] = make_pt(rho, uu, vv, ww, (rhoe + pp) / rho, tt, P.rgas0, keep);
+            }
+        }
+        Pt qw[6];
+        double sw[6];
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+            qw[s] = q[s];
+            sw[s] = sen[s];
+        }
+        hybrid_face<0>(P, qw, sw, i - 1, eoc[i - 1], n, flo);
+    }
+    // the next lane's lower face is this cell's upper face when the next
+    // lane holds the next cell of the same x row
+    const int oin = __shfl_down(oi, 1, 64);
+    double nb[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) nb[m] = __shfl_down(flo[m], 1, 64);
+    if (!act || !owner) return;
+    const bool shared = __lane_id() < 63 && oin == oi + 1 && i < n;
+    double fhi[5];
+    if (shared) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) fhi[m] = nb[m];
+    } else {
+        Pt qw[6];
+        double sw[6];
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+            qw[s] = q[s + 1];
+            sw[s] = sen[s + 1];
+        }
+        hybrid_face<0>(P, qw, sw, i, eoc[i], n, fhi);
+    }
+    const double d = metric[i - 1];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const double flv = double(fl[(size_t)m * g.nint + oi]);
+        fl[(size_t)m * g.nint + oi] = R(flv + (fhi[m] - flo[m]) * d);
+    }
+}
+
 // The WENO fix-up of one direction in two kernels.  k_fix_compact: block b
 // takes segment b of the flag array (wpb 64-bit words = 8 * wpb cells, each
 // thread a run of consecutive words), lists the cells whose flag byte holds
@@ -1706,17 +1802,26 @@ __global__ __launch_bounds__(256) void k_fix_list(Geo g, KPar P, const R *__rest
         const int lane = t & 63;
         const int wave = (blockIdx.x * blockDim.x + t) >> 6;
         const int nwaves = (gridDim.x * blockDim.x) >> 6;
-        for (int base = wave * 64; base < total; base += nwaves * 64) {
-            const int e = base + lane;
-            if (e < total) {
-                int lo = 0, hi = nseg;  // pre[lo] <= e < pre[hi]
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (pre[mid] <= e) lo = mid;
-                    else hi = mid;
-                }
-                fix_cell<R, DIR, RED>(g, P, wa, w, eoc, metric, fl, list[(size_t)lo * seglen + (e - pre[lo])], &r0,
-                                      &r1, gamdt, yn);
+        auto entry = [&](int e) {
+            int lo = 0, hi = nseg;  // pre[lo] <= e < pre[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (pre[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            return list[(size_t)lo * seglen + (e - pre[lo])];
+        };
+        if constexpr (DIR == 0 && !RED) {
+            // batches of 63 cells, lane 63 the helper of the next entry (fix_cell_x)
+            for (int base = wave * 63; base < total; base += nwaves * 63) {
+                const int e = base + lane;
+                const bool act = e < total;
+                fix_cell_x<R>(g, P, wa, w, eoc, metric, fl, act ? entry(e) : -2, act, lane < 63);
+            }
+        } else {
+            for (int base = wave * 64; base < total; base += nwaves * 64) {
+                const int e = base + lane;
+                if (e < total) fix_cell<R, DIR, RED>(g, P, wa, w, eoc, metric, fl, entry(e), &r0, &r1, gamdt, yn);
             }
         }
     }

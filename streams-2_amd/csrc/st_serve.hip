@@ -162,6 +162,8 @@ struct Slot {
     int ckpt_steps = 0;
 };
 
+constexpr int kPlainChunk = 10;  // steps per chunk of a server without graphs (steps_per_graph 0 or 1)
+
 enum JobKind { kSetupJob, kFinishJob, kCheckpointJob };
 struct Job {
     JobKind kind;
@@ -408,7 +410,7 @@ int launch_chunk(st_server *sv, Slot &sl) {
     st_solver *s = sl.s;
     ServeReq *r = sl.r;
     const int left = r->nsteps - sl.launched;
-    const int per = sv->K > 0 ? sv->K : 10;
+    const int per = sv->K > 0 ? sv->K : kPlainChunk;
     int n = left < per ? left : per;
     const int slot = (sl.head + sl.nev) % 2;
     int cand = -1;
@@ -715,8 +717,10 @@ int st_server_submit2(st_server *sv, const st_params *p, const double *x, const 
         return sfail(ST_ERR_ARG, "bad submit arguments");
     if (checkpoint_every > 0 && (!checkpoint_dir || !*checkpoint_dir))
         return sfail(ST_ERR_ARG, "a checkpoint period needs a checkpoint directory");
-    if (checkpoint_every > 0 && sv->K > 0 && checkpoint_every % sv->K != 0)
-        return sfail(ST_ERR_ARG, "the checkpoint period must be a multiple of steps_per_graph");
+    // checkpoints fall on chunk boundaries: K steps, or kPlainChunk without graphs
+    const int per = sv->K > 0 ? sv->K : kPlainChunk;
+    if (checkpoint_every > 0 && checkpoint_every % per != 0)
+        return sfail(ST_ERR_ARG, "the checkpoint period must be a multiple of the steps per chunk");
     int rc = st::validate_params(p);
     if (rc) return sfail(rc, st_last_error());
     if (p->nx + p->ny + p->nz + 3 > kTabInts) return sfail(ST_ERR_ARG, "grid too large for the server");
