@@ -329,8 +329,19 @@ def host_cpu():
         allowed = len(os.sched_getaffinity(0))
     except AttributeError:
         allowed = os.cpu_count() or 1
-    return {"model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": allowed,
-            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    out = {"model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": allowed,
+           "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    # lscpu's topology lines, when the tool is there
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keep = ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)", "NUMA node(s)",
+                "CPU max MHz", "Hypervisor vendor")
+        out["lscpu"] = {k.strip(): v.strip() for k, v in (ln.split(":", 1) for ln in txt.splitlines() if ":" in ln)
+                        if k.strip() in keep}
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return out
 
 
 def cpu_baseline(args):
