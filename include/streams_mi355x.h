@@ -361,6 +361,25 @@ int st_server_wait(st_server *sv, int64_t ticket, double timeout_s, double *w_ou
 /* [0] queued, [1] running, [2] completed, [3] failed, [4] HBM bytes reserved,
  * [5] tuned shape classes. */
 int st_server_stats(st_server *sv, int64_t out[6]);
+/* Batched launches (call before the first submission; steps_per_graph even
+ * and >= 2).  With lanes > 0 (at most 8) each running solve of at most
+ * max_cells interior cells (0: any size) is bound to one of `lanes` batch
+ * streams (the one holding the fewest cells), and the chunks of a lane's
+ * solves whose launch sequences agree (same precision, scheme and boundary
+ * set, any grid) run as one sequence of launches, each launch a single grid
+ * over a table of the solves: one launch per kernel per stage and lane
+ * instead of one per solve.  Larger solves, solves whose steps have no
+ * batched form, the chunks that tune a class's launch shape and a final
+ * chunk shorter than steps_per_graph run on their own streams as without
+ * batching.  lanes = 0 turns batching off.  Results are unchanged, bit for
+ * bit. */
+int st_server_set_batching(st_server *sv, int32_t lanes, int64_t max_cells);
+/* [0] batched chunks, [1] batched launches, [2] solves summed over the
+ * batched chunks, [3] table layouts (recorded chunks; a lane whose solves
+ * are unchanged reuses its last layout), [4] microseconds the scheduler
+ * spent recording and laying out, [5] microseconds enqueuing batched
+ * launches. */
+int st_server_batch_stats(st_server *sv, int64_t out[6]);
 const char *st_server_last_error(void);
 
 #ifdef __cplusplus

@@ -620,7 +620,7 @@ __global__ void k_rng_reset(Win31 seeded, uint32_t *__restrict__ win2, int *__re
 // dt evaluates the same primitives of the same w as compute_dt's own
 // compute_aux(central=1) at the start of the step.
 template <typename R, bool DT = false>
-__global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__restrict__ w, R *__restrict__ wa,
+__device__ __forceinline__ void k_aux_body(const dim3 blockIdx, const dim3 gridDim, Geo g, KPar P, const double *__restrict__ w, R *__restrict__ wa,
                                              int i0, int ni, int j0, int nj, int k0, int nk, int image,
                                              const double *__restrict__ dx1, const double *__restrict__ dy1,
                                              const double *__restrict__ dz1, const double *__restrict__ dxs,
@@ -685,6 +685,15 @@ __global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__rest
         if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
     }
 }
+template <typename R, bool DT = false>
+__global__ __launch_bounds__(256) void k_aux(Geo g, KPar P, const double *__restrict__ w, R *__restrict__ wa,
+                                             int i0, int ni, int j0, int nj, int k0, int nk, int image,
+                                             const double *__restrict__ dx1, const double *__restrict__ dy1,
+                                             const double *__restrict__ dz1, const double *__restrict__ dxs,
+                                             const double *__restrict__ dys, const double *__restrict__ dzs,
+                                             double *__restrict__ partials) {
+    k_aux_body<R, DT>(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), g, P, w, wa, i0, ni, j0, nj, k0, nk, image, dx1, dy1, dz1, dxs, dys, dzs, partials);
+}
 
 // ------------------------------------------------------- dirty-cell flags
 // Faces whose sensor window exceeds the threshold take the characteristic
@@ -704,9 +713,12 @@ __device__ __forceinline__ void mark_dirty(unsigned char *__restrict__ flags, lo
 
 // clears both any-dirty bytes: nint (x, y and z sweeps) and nint + 1 (the y
 // term of the fused x + y pass)
-__global__ void k_clear_any(unsigned char *__restrict__ flags, int nint) {
+__device__ __forceinline__ void k_clear_any_body(const dim3 blockIdx, const dim3 gridDim, unsigned char *__restrict__ flags, int nint) {
     flags[nint] = 0;
     flags[nint + 1] = 0;
+}
+__global__ void k_clear_any(unsigned char *__restrict__ flags, int nint) {
+    k_clear_any_body(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), flags, nint);
 }
 
 // ------------------------------------------------------------ Euler y / z
@@ -824,7 +836,7 @@ struct VDArgs {
 };
 
 template <typename R, int DIR, int KM, int VD, bool O2>
-__global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__restrict__ wa,
+__device__ __forceinline__ void k_euler_march_body(const dim3 blockIdx, const dim3 gridDim, Geo g, KPar P, const R *__restrict__ wa,
                                                      const int *__restrict__ eoc, const double *__restrict__ metric,
                                                      R *__restrict__ fl, int chunk, int nchunk,
                                                      unsigned char *__restrict__ flags, VDArgs V) {
@@ -1045,11 +1057,18 @@ __global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__r
     if (VD == 2) {
         block_sum<5>(red, lds);
         if (flat_thread() == 0) {
-            double *pp = V.partials + (size_t)flat_block() * 5;
+            double *pp = V.partials + (size_t)(int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) * 5;
 #pragma unroll
             for (int q = 0; q < 5; q++) pp[q] = red[q];
         }
     }
+}
+template <typename R, int DIR, int KM, int VD, bool O2>
+__global__ __launch_bounds__(256) void k_euler_march(Geo g, KPar P, const R *__restrict__ wa,
+                                                     const int *__restrict__ eoc, const double *__restrict__ metric,
+                                                     R *__restrict__ fl, int chunk, int nchunk,
+                                                     unsigned char *__restrict__ flags, VDArgs V) {
+    k_euler_march_body<R, DIR, KM, VD, O2>(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), g, P, wa, eoc, metric, fl, chunk, nchunk, flags, V);
 }
 
 // --------------------------------------------------------------- Euler x
@@ -1251,7 +1270,7 @@ __global__ __launch_bounds__(256) void k_euler_x(Geo g, KPar P, const R *__restr
 // Rows outside the y range of the Euler fluxes (cut by a characteristic
 // boundary) receive the x term only, as from the two sweeps.
 template <typename R, int KM>
-__global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__restrict__ wa,
+__device__ __forceinline__ void k_euler_xy_body(const dim3 blockIdx, const dim3 gridDim, Geo g, KPar P, const R *__restrict__ wa,
                                                   const int *__restrict__ eocx, const int *__restrict__ eocy,
                                                   const double *__restrict__ dcsidx, const double *__restrict__ detady,
                                                   R *__restrict__ fl, int chunk, int nchunk,
@@ -1439,6 +1458,14 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
         if (anyx) flags[g.nint] = 1;
         if (anyy) flags[g.nint + 1] = 1;
     }
+}
+template <typename R, int KM>
+__global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__restrict__ wa,
+                                                  const int *__restrict__ eocx, const int *__restrict__ eocy,
+                                                  const double *__restrict__ dcsidx, const double *__restrict__ detady,
+                                                  R *__restrict__ fl, int chunk, int nchunk,
+                                                  unsigned char *__restrict__ flags) {
+    k_euler_xy_body<R, KM>(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), g, P, wa, eocx, eocy, dcsidx, detady, fl, chunk, nchunk, flags);
 }
 
 // ------------------------------------------------------------ WENO fix-up
@@ -1726,7 +1753,7 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *wsum, int *total) 
     return before + x - v;
 }
 
-__global__ __launch_bounds__(256) void k_fix_compact(unsigned char *__restrict__ flags, int ncell, int bit,
+__device__ __forceinline__ void k_fix_compact_body(const dim3 blockIdx, const dim3 gridDim, unsigned char *__restrict__ flags, int ncell, int bit,
                                                      int anyidx, int wpb, int *__restrict__ list,
                                                      int *__restrict__ counts) {
     __shared__ int wsum[4];
@@ -1759,12 +1786,17 @@ __global__ __launch_bounds__(256) void k_fix_compact(unsigned char *__restrict__
         words[wi] = wd & ~clr;
     }
 }
+__global__ __launch_bounds__(256) void k_fix_compact(unsigned char *__restrict__ flags, int ncell, int bit,
+                                                     int anyidx, int wpb, int *__restrict__ list,
+                                                     int *__restrict__ counts) {
+    k_fix_compact_body(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), flags, ncell, bit, anyidx, wpb, list, counts);
+}
 
 // RED: the block's partial force_rhs sums of the fix-up's changes to fl(1)
 // and fl(2) go to partials[blockIdx.x] (5 values, the last three zero),
 // written also when there is nothing to fix.
 template <typename R, int DIR, bool RED>
-__global__ __launch_bounds__(256) void k_fix_list(Geo g, KPar P, const R *__restrict__ wa,
+__device__ __forceinline__ void k_fix_list_body(const dim3 blockIdx, const dim3 gridDim, Geo g, KPar P, const R *__restrict__ wa,
                                                   const double *__restrict__ w, const int *__restrict__ eoc,
                                                   const double *__restrict__ metric, R *__restrict__ fl,
                                                   const unsigned char *__restrict__ flags, int anyidx,
@@ -1835,6 +1867,17 @@ __global__ __launch_bounds__(256) void k_fix_list(Geo g, KPar P, const R *__rest
         }
     }
 }
+template <typename R, int DIR, bool RED>
+__global__ __launch_bounds__(256) void k_fix_list(Geo g, KPar P, const R *__restrict__ wa,
+                                                  const double *__restrict__ w, const int *__restrict__ eoc,
+                                                  const double *__restrict__ metric, R *__restrict__ fl,
+                                                  const unsigned char *__restrict__ flags, int anyidx,
+                                                  const int *__restrict__ list, const int *__restrict__ counts,
+                                                  int nseg, int seglen, const DevScalars *__restrict__ ds,
+                                                  double rkg, const double *__restrict__ yn,
+                                                  double *__restrict__ partials) {
+    k_fix_list_body<R, DIR, RED>(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), g, P, wa, w, eoc, metric, fl, flags, anyidx, list, counts, nseg, seglen, ds, rkg, yn, partials);
+}
 
 // ---------------------------------------------------------------- viscous
 // Ghost images of an interior value: periodic copy for periodic directions
@@ -1899,7 +1942,7 @@ static_assert(VBX * VBY >= 2 * H * (VBX + VBY), "fewer lanes than halo points");
 // unbounded, 3 waves/SIMD): 0.630 -> 0.604 ms on 512x256x256.  The FP64 form
 // stays unbounded (209 VGPRs); a 3-waves bound spills 136 B/lane there.
 template <typename R, bool SENSOR>
-__global__ __launch_bounds__(VBX *VBY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_visc_zm(Geo g, KPar P, R *__restrict__ wa, R *__restrict__ fl,
+__device__ __forceinline__ void k_visc_zm_body(const dim3 blockIdx, const dim3 gridDim, Geo g, KPar P, R *__restrict__ wa, R *__restrict__ fl,
                                                       const double *__restrict__ dx1, const double *__restrict__ dxs,
                                                       const double *__restrict__ dx2, const double *__restrict__ dy1,
                                                       const double *__restrict__ dys, const double *__restrict__ dy2,
@@ -2099,6 +2142,16 @@ __global__ __launch_bounds__(VBX *VBY) __attribute__((amdgpu_waves_per_eu(sizeof
         }
     }
 }
+template <typename R, bool SENSOR>
+__global__ __launch_bounds__(VBX *VBY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_visc_zm(Geo g, KPar P, R *__restrict__ wa, R *__restrict__ fl,
+                                                      const double *__restrict__ dx1, const double *__restrict__ dxs,
+                                                      const double *__restrict__ dx2, const double *__restrict__ dy1,
+                                                      const double *__restrict__ dys, const double *__restrict__ dy2,
+                                                      const double *__restrict__ dz1, const double *__restrict__ dzs,
+                                                      const double *__restrict__ dz2, double *__restrict__ wallprop,
+                                                      int images, int chunk) {
+    k_visc_zm_body<R, SENSOR>(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), g, P, wa, fl, dx1, dxs, dx2, dy1, dys, dy2, dz1, dzs, dz2, wallprop, images, chunk);
+}
 
 // ------------------------------------------------ conservative viscous form
 // Face flux between points a (left) and b (right) at node distance h
@@ -2230,7 +2283,7 @@ __global__ __launch_bounds__(256) void k_visc_cons(Geo g, KPar P, R *__restrict_
 
 // sensor_subroutine (kernels_cpu.F90:4726-4744) + bcswap_var of w_aux(8)
 template <typename R>
-__global__ __launch_bounds__(256) void k_sensor(Geo g, KPar P, R *__restrict__ wa, int images) {
+__device__ __forceinline__ void k_sensor_body(const dim3 blockIdx, const dim3 gridDim, Geo g, KPar P, R *__restrict__ wa, int images) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -2245,6 +2298,10 @@ __global__ __launch_bounds__(256) void k_sensor(Geo g, KPar P, R *__restrict__ w
     sv = fmax(sv, R(0.0));
     wa[7 * nv + c] = sv * sv;
     if (images) write_images(g, P, wa + 7 * nv, i, j, k, sv * sv, false);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_sensor(Geo g, KPar P, R *__restrict__ wa, int images) {
+    k_sensor_body<R>(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), g, P, wa, images);
 }
 
 // Ducros sensor in FP64 from the conserved state w at interior cell
@@ -2322,7 +2379,7 @@ __device__ double sensor_fp64(const Geo &g, const KPar &P, const double *__restr
 constexpr double kSensorBand = 1e-3;
 
 template <bool DIRECT>
-__global__ __launch_bounds__(256) void k_sensor_refine(Geo g, KPar P, const double *__restrict__ w,
+__device__ __forceinline__ void k_sensor_refine_body(const dim3 blockIdx, const dim3 gridDim, Geo g, KPar P, const double *__restrict__ w,
                                                        float *__restrict__ wa, const double *__restrict__ dx1,
                                                        const double *__restrict__ dy1,
                                                        const double *__restrict__ dz1, int images) {
@@ -2345,6 +2402,13 @@ __global__ __launch_bounds__(256) void k_sensor_refine(Geo g, KPar P, const doub
     }
     wa[7 * nv + c] = st;
     if (images) write_images(g, P, wa + 7 * nv, i, j, k, st, false);
+}
+template <bool DIRECT>
+__global__ __launch_bounds__(256) void k_sensor_refine(Geo g, KPar P, const double *__restrict__ w,
+                                                       float *__restrict__ wa, const double *__restrict__ dx1,
+                                                       const double *__restrict__ dy1,
+                                                       const double *__restrict__ dz1, int images) {
+    k_sensor_refine_body<DIRECT>(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), g, P, w, wa, dx1, dy1, dz1, images);
 }
 
 // visflx_div_subroutine (kernels_cpu.F90:3484-3532).  With REDUCE the
@@ -2480,7 +2544,8 @@ constexpr int RB = 128;
 static_assert(RB <= RED_THREADS, "the last block reads one second-level value per thread");
 
 template <int NV>
-__device__ __forceinline__ bool two_level_sum(const double *__restrict__ partials, int nblk, double *__restrict__ stage2,
+__device__ __forceinline__ bool two_level_sum(const dim3 blockIdx, const dim3 gridDim, const double *__restrict__ partials,
+                                              int nblk, double *__restrict__ stage2,
                                               int *__restrict__ ticket, double (&out)[NV], double (*lds)[NV]) {
     __shared__ bool last;
     const int per = (nblk + gridDim.x - 1) / gridDim.x;
@@ -2521,12 +2586,12 @@ __device__ __forceinline__ bool two_level_sum(const double *__restrict__ partial
 }
 
 // force_rhs (singleideal_cpu.F90:845-889): bulk sums -> dpdx, rho_b, u_b, T_b
-__global__ __launch_bounds__(RED_THREADS) void k_force_finish(KPar P, const double *__restrict__ partials, int nblk,
+__device__ __forceinline__ void k_force_finish_body(const dim3 blockIdx, const dim3 gridDim, KPar P, const double *__restrict__ partials, int nblk,
                                                               double *__restrict__ stage2, int *__restrict__ ticket,
                                                               DevScalars *ds, int first_stage) {
     __shared__ double lds[RED_THREADS / 64][5];
     double acc[5];
-    if (two_level_sum<5>(partials, nblk, stage2, ticket, acc, lds)) {
+    if (two_level_sum<5>(blockIdx, gridDim, partials, nblk, stage2, ticket, acc, lds)) {
         const double v = P.volchan;
         const double b1 = acc[0] / v, b2 = acc[1] / v, b3 = acc[2] / v, b4 = acc[3] / v, b5 = acc[4] / v;
         ds->dpdx = (first_stage ? 0.0 : ds->dpdx) + b2;
@@ -2537,6 +2602,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_force_finish(KPar P, const doub
         ds->b2 = b2;
     }
 }
+__global__ __launch_bounds__(RED_THREADS) void k_force_finish(KPar P, const double *__restrict__ partials, int nblk,
+                                                              double *__restrict__ stage2, int *__restrict__ ticket,
+                                                              DevScalars *ds, int first_stage) {
+    k_force_finish_body(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), P, partials, nblk, stage2, ticket, ds, first_stage);
+}
 
 // update_flux + force_rhs_2 + update_field (kernels_cpu.F90:3400-3442,
 // 3092-3116) fused, then the isothermal-wall ghosts of update_ghost
@@ -2544,7 +2614,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_force_finish(KPar P, const doub
 // force_var on, also recovers T from the new E (force_var_1_subroutine,
 // :3241-3278) and writes the block partial sums of rho*u*T*dy.
 template <typename R, bool FORCE_VAR>
-__global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restrict__ w, R *__restrict__ wa,
+__device__ __forceinline__ void k_update_body(const dim3 blockIdx, const dim3 gridDim, Geo g, KPar P, double *__restrict__ w, R *__restrict__ wa,
                                                 const R *__restrict__ flp, const R *__restrict__ fl,
                                                 const DevScalars *__restrict__ ds, double rkr, double rkg, int channel,
                                                 const double *__restrict__ yn, double *__restrict__ partials) {
@@ -2607,8 +2677,15 @@ __global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restric
     }
     if (FORCE_VAR) {
         block_sum<1>(acc, lds);
-        if (flat_thread() == 0) partials[flat_block()] = acc[0];
+        if (flat_thread() == 0) partials[(int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z))] = acc[0];
     }
+}
+template <typename R, bool FORCE_VAR>
+__global__ __launch_bounds__(256) void k_update(Geo g, KPar P, double *__restrict__ w, R *__restrict__ wa,
+                                                const R *__restrict__ flp, const R *__restrict__ fl,
+                                                const DevScalars *__restrict__ ds, double rkr, double rkg, int channel,
+                                                const double *__restrict__ yn, double *__restrict__ partials) {
+    k_update_body<R, FORCE_VAR>(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), g, P, w, wa, flp, fl, ds, rkr, rkg, channel, yn, partials);
 }
 
 // force_var (singleideal_cpu.F90:890-933, grid_dim 1) bulk temperature
@@ -2617,7 +2694,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_force_var_finish(KPar P, const 
                                                                   DevScalars *ds) {
     __shared__ double lds[RED_THREADS / 64][1];
     double acc[1];
-    if (two_level_sum<1>(partials, nblk, stage2, ticket, acc, lds)) {
+    if (two_level_sum<1>(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), partials,
+                         nblk, stage2, ticket, acc, lds)) {
         double bulktg = acc[0] / P.volchan;
         bulktg = bulktg / ds->rhobulk / ds->ubulk;
         ds->bulkt = P.t_bulk_target - bulktg;
@@ -2806,7 +2884,7 @@ __device__ __forceinline__ double block_max(double m, double *lds) {
 // dt = cfl / max (singleideal_cpu.F90:1507-1514) over the block maxima:
 // gridDim.x blocks take a slice each, the last one to finish combines the
 // slice maxima (max is exact in any order).
-__global__ __launch_bounds__(RED_THREADS) void k_dt_finish(KPar P, const double *__restrict__ partials, int nblk,
+__device__ __forceinline__ void k_dt_finish_body(const dim3 blockIdx, const dim3 gridDim, KPar P, const double *__restrict__ partials, int nblk,
                                                            double *__restrict__ stage2, int *__restrict__ ticket,
                                                            double *__restrict__ dt_out) {
     __shared__ double lds[RED_THREADS / 64];
@@ -2833,11 +2911,19 @@ __global__ __launch_bounds__(RED_THREADS) void k_dt_finish(KPar P, const double 
         *ticket = 0;
     }
 }
+__global__ __launch_bounds__(RED_THREADS) void k_dt_finish(KPar P, const double *__restrict__ partials, int nblk,
+                                                           double *__restrict__ stage2, int *__restrict__ ticket,
+                                                           double *__restrict__ dt_out) {
+    k_dt_finish_body(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), P, partials, nblk, stage2, ticket, dt_out);
+}
 
 // end of a step of run(): icyc += 1, time += dt (singleideal_cpu.F90:1604,1642)
-__global__ void k_step_end(DevScalars *ds) {
+__device__ __forceinline__ void k_step_end_body(const dim3 blockIdx, const dim3 gridDim, DevScalars *ds) {
     ds->icyc = ds->icyc + 1;
     ds->time = ds->time + ds->dt;
+}
+__global__ void k_step_end(DevScalars *ds) {
+    k_step_end_body(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), ds);
 }
 
 // ------------------------------------------------------------- residual
@@ -3155,6 +3241,20 @@ static inline dim3 cell_grid(const Geo &g) { return dim3((g.nx + 63) / 64, (g.ny
 static inline int cell_blocks(const Geo &g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz; }
 static const dim3 kCellBlock(64, 4, 1);
 
+// Workgroups of a two-level reduction of n partials: 64 partials or more
+// each, at most RB (its second-level slots).  A function of the grid
+// alone, so the summation order of a solve does not depend on how it is
+// launched.
+static inline int red_blocks(long long n) {
+    const long long b = (n + 63) / 64;
+    return (int)(b < 1 ? 1 : (b > RB ? RB : b));
+}
+// Workgroups of the WENO fix-up: one per 16384 cells, at most one per CU
+static inline int fix_blocks(const Geo &g) {
+    const long long b = (g.nint + 16383) / 16384;
+    return (int)(b < 1 ? 1 : (b > kFixBlocks ? kFixBlocks : b));
+}
+
 static inline int grid1d(size_t n, int bs) {
     size_t b = (n + bs - 1) / bs;
     return (int)(b > 65535 * 16 ? 65535 * 16 : (b < 1 ? 1 : b));
@@ -3176,6 +3276,7 @@ static inline int grid1d(size_t n, int bs) {
 
 template <typename T>
 static hipError_t fill_periodic_t(st_solver *s, T *arr, int nv) {
+    ST_NOREC();
     const Geo &g = s->g;
     size_t tot = (size_t)2 * g.ng * ((size_t)g.ny * g.nz + (size_t)g.nx * g.nz + (size_t)g.nx * g.ny) * nv;
     hipLaunchKernelGGL(k_fill_periodic<T>, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g, s->kp, arr, nv);
@@ -3189,6 +3290,7 @@ hipError_t launch_fill_periodic_aux(st_solver *s, int var) {
 }
 
 hipError_t launch_extrapolate_aux(st_solver *s, int var) {
+    ST_NOREC();
     const Geo &g = s->g;
     size_t tot = (size_t)2 * g.ng * ((size_t)g.ny * g.nz + (size_t)g.nx * g.nz + (size_t)g.nx * g.ny);
     ST_DISPATCH(hipLaunchKernelGGL(k_extrapolate_var<R>, dim3(grid1d(tot, 256)), dim3(256), 0, s->stream, g,
@@ -3202,20 +3304,21 @@ hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int 
     const size_t tot = (size_t)ni * nj * nk;
     const unsigned nblk = (unsigned)((tot + 255) / 256);
 #define ST_LAUNCH_AUX(D_)                                                                                          \
-    ST_DISPATCH(hipLaunchKernelGGL((k_aux<R, D_>), dim3(nblk), dim3(256), 0, s->stream, s->g, s->kp, s->w,          \
-                                   (R *)s->w_aux, i0, ni, j0, nj, k0, nk, image ? 1 : 0, s->dcsidx, s->detady,       \
-                                   s->dzitdz, s->dcsidxs, s->detadys, s->dzitdzs, s->partials))
+    ST_DISPATCH(ST_GO(k_aux, ST_T(R, D_), 256, 0, dim3(nblk), dim3(256), s->g, s->kp, s->w, (R *)s->w_aux, i0, ni,  \
+                      j0, nj, k0, nk, image ? 1 : 0, s->dcsidx, s->detady, s->dzitdz, s->dcsidxs, s->detadys,        \
+                      s->dzitdzs, s->partials))
     if (dt && s->p.cfl > 0) ST_LAUNCH_AUX(true);
     else ST_LAUNCH_AUX(false);
 #undef ST_LAUNCH_AUX
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !(dt && s->p.cfl > 0)) return e;
-    hipLaunchKernelGGL(k_dt_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, (int)nblk,
-                       s->partials + s->npart - RB * 8, s->counters + 1, &s->ds->dt);
+    ST_GO(k_dt_finish, , RED_THREADS, 0, dim3(red_blocks(nblk)), dim3(RED_THREADS), s->kp, s->partials, (int)nblk,
+          s->partials + s->npart - RB * 8, s->counters + 1, &s->ds->dt);
     return hipGetLastError();
 }
 
 hipError_t launch_wall_ghosts(st_solver *s) {
+    ST_NOREC();
     const Geo &g = s->g;
     if (!s->kp.wall_stag[0] && !s->kp.wall_stag[1]) return hipSuccess;
     size_t tot = (size_t)2 * g.ng * g.nx * g.nz;
@@ -3237,16 +3340,16 @@ static hipError_t launch_fix(st_solver *s, R *fl, int dir, int bit = 1, int anyi
     wpb = (wpb + 255) / 256 * 256;
     nseg = (nword + wpb - 1) / wpb;
     int *counts = s->counters + 8;
-    hipLaunchKernelGGL(k_fix_compact, dim3((unsigned)nseg), dim3(256), 0, s->stream, s->dirty, g.nint, bit, anyidx,
-                       wpb, s->fixlist, counts);
-    // the fix-up holds one wave per SIMD (its register budget): one
+    ST_GO(k_fix_compact, , 256, 0, dim3((unsigned)nseg), dim3(256), s->dirty, g.nint, bit, anyidx, wpb, s->fixlist,
+          counts);
+    // the fix-up holds one wave per SIMD (its register budget): at most one
     // 4-wave block per CU
-    const int nb = kFixBlocks;
+    const int nb = fix_blocks(g);
     const int seglen = 8 * wpb;
     const R *wa = (const R *)s->w_aux;
 #define ST_LAUNCH_F(D_, RED_, E_, M_)                                                                              \
-    hipLaunchKernelGGL((k_fix_list<R, D_, RED_>), dim3(nb), dim3(256), 0, s->stream, g, s->kp, wa, s->w, E_, M_, fl, \
-                       s->dirty, anyidx, s->fixlist, counts, nseg, seglen, s->ds, rkg, s->yn, partials)
+    ST_GO(k_fix_list, ST_T(R, D_, RED_), 256, 0, dim3(nb), dim3(256), g, s->kp, wa, s->w, E_, M_, fl, s->dirty,     \
+          anyidx, s->fixlist, counts, nseg, seglen, s->ds, rkg, s->yn, partials)
     if (dir == 0) ST_LAUNCH_F(0, false, s->eoc[0], s->dcsidx);
     else if (dir == 1) ST_LAUNCH_F(1, false, s->eoc[1], s->detady);
     else if (partials) ST_LAUNCH_F(2, true, s->eoc[2], s->dzitdz);
@@ -3257,8 +3360,9 @@ static hipError_t launch_fix(st_solver *s, R *fl, int dir, int bit = 1, int anyi
 
 template <typename R>
 static hipError_t euler_x_t(st_solver *s, R *fl, bool init, bool fix) {
+    ST_NOREC();
     const Geo &g = s->g;
-    hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
+    ST_GO(k_clear_any, , 1024, 0, dim3(1), dim3(1), s->dirty, g.nint);
     if (s->kp.eul_hi[0] < s->kp.eul_lo[0]) {
         if (init) hipLaunchKernelGGL(k_zero<R>, dim3(grid1d((size_t)5 * g.nint, 256)), dim3(256), 0, s->stream, fl,
                                      (size_t)5 * g.nint);
@@ -3292,7 +3396,7 @@ static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix, int vd = 
                                int stage = 0, bool ord2 = false) {
     const Geo &g = s->g;
     const int lo = s->kp.eul_lo[dir], hi = s->kp.eul_hi[dir];
-    hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
+    ST_GO(k_clear_any, , 1024, 0, dim3(1), dim3(1), s->dirty, g.nint);
     if (hi < lo) return hipGetLastError();
     const int ncell = hi - lo + 1;
     const int no = dir == 1 ? g.nz : g.ny;
@@ -3326,8 +3430,8 @@ static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix, int vd = 
         V.zc = s->z;
     }
 #define ST_LAUNCH_M(D, K, VD_, O2_, E, M)                                                                          \
-    hipLaunchKernelGGL((k_euler_march<R, D, K, VD_, O2_>), grid, dim3(256), 0, s->stream, g, s->kp,                 \
-                       (const R *)s->w_aux, E, M, fl, chunk, nchunk, s->dirty, V)
+    ST_GO(k_euler_march, ST_T(R, D, K, VD_, O2_), 256, 0, grid, dim3(256), g, s->kp, (const R *)s->w_aux, E, M, fl, \
+          chunk, nchunk, s->dirty, V)
 #define ST_LAUNCH_MK(D, VD_, O2_, E, M)                      \
     do {                                                     \
         if (km == 0) ST_LAUNCH_M(D, 0, VD_, O2_, E, M);      \
@@ -3346,7 +3450,7 @@ static hipError_t launch_march(st_solver *s, R *fl, int dir, bool fix, int vd = 
     // the force_rhs partials of this stage: the march's blocks, then the
     // z fix-up's (it always writes them, zero when nothing is flagged)
     if (vd == 2) {
-        s->force_nblk = (int)nblk + kFixBlocks;
+        s->force_nblk = (int)nblk + fix_blocks(g);
         s->zv_nblk = (int)nblk;
     }
     if (e != hipSuccess || !fix) return e;
@@ -3366,12 +3470,12 @@ static hipError_t launch_xy(st_solver *s, R *fl, bool fix) {
     if (s->march_chunk > 0) chunk = s->march_chunk < ncell ? s->march_chunk : ncell;
     const int nchunk = (ncell + chunk - 1) / chunk;
     const long nw = nseg * g.nz * nchunk;
-    hipLaunchKernelGGL(k_clear_any, dim3(1), dim3(1), 0, s->stream, s->dirty, g.nint);
+    ST_GO(k_clear_any, , 1024, 0, dim3(1), dim3(1), s->dirty, g.nint);
     const dim3 grid((unsigned)((nw + 3) / 4));
     const int km = s->kp.nkeep == 0 ? 0 : (s->kp.nkeep > 0 ? 1 : -1);
 #define ST_LAUNCH_XY(K)                                                                                          \
-    hipLaunchKernelGGL((k_euler_xy<R, K>), grid, dim3(256), 0, s->stream, g, s->kp, (const R *)s->w_aux, s->eoc[0], \
-                       s->eoc[1], s->dcsidx, s->detady, fl, chunk, nchunk, s->dirty)
+    ST_GO(k_euler_xy, ST_T(R, K), 256, 0, grid, dim3(256), g, s->kp, (const R *)s->w_aux, s->eoc[0], s->eoc[1],    \
+          s->dcsidx, s->detady, fl, chunk, nchunk, s->dirty)
     if (km == 0) ST_LAUNCH_XY(0); else if (km > 0) ST_LAUNCH_XY(1); else ST_LAUNCH_XY(-1);
 #undef ST_LAUNCH_XY
     hipError_t e = hipGetLastError();
@@ -3420,9 +3524,9 @@ hipError_t launch_visc(st_solver *s, void *fl, int mode, bool ghost_images) {
     const dim3 grid((g.nx + VBX - 1) / VBX, (g.ny + VBY - 1) / VBY, (g.nz + chunk - 1) / chunk);
     const dim3 block(VBX, VBY);
 #define ST_LAUNCH_V(S_)                                                                                            \
-    hipLaunchKernelGGL((k_visc_zm<R, S_>), grid, block, 0, s->stream, g, s->kp, (R *)s->w_aux, (R *)fl, s->dcsidx,   \
-                       s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2, \
-                       s->wallprop, ghost_images ? 1 : 0, chunk)
+    ST_GO(k_visc_zm, ST_T(R, S_), VBX * VBY, (sizeof(R) == 4 ? 4 : 1), grid, block, g, s->kp, (R *)s->w_aux,       \
+          (R *)fl, s->dcsidx, s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs,     \
+          s->dzitdz2, s->wallprop, ghost_images ? 1 : 0, chunk)
     if (mode == 0) ST_DISPATCH(ST_LAUNCH_V(true));
     else ST_DISPATCH(ST_LAUNCH_V(false));
 #undef ST_LAUNCH_V
@@ -3432,6 +3536,7 @@ hipError_t launch_visc(st_solver *s, void *fl, int mode, bool ghost_images) {
 // Conservative viscous form: mode 2 (cell terms + sensor), 6 (cell terms),
 // 5 (face terms) and 56 (6 then 5 in one pass, the rk_sync order).
 hipError_t launch_visc_cons(st_solver *s, void *fl, int mode, bool ghost_images) {
+    ST_NOREC();
     const Geo &g = s->g;
 #define ST_LAUNCH_VC(C_, F_, S_)                                                                                      \
     ST_DISPATCH(hipLaunchKernelGGL((k_visc_cons<R, C_, F_, S_>), cell_grid(g), kCellBlock, 0, s->stream, g, s->kp,   \
@@ -3448,23 +3553,24 @@ hipError_t launch_visc_cons(st_solver *s, void *fl, int mode, bool ghost_images)
 }
 
 hipError_t launch_sensor(st_solver *s, bool ghost_images) {
-    ST_DISPATCH(hipLaunchKernelGGL(k_sensor<R>, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, (R *)s->w_aux,
-                                   ghost_images ? 1 : 0));
+    ST_DISPATCH(ST_GO(k_sensor, ST_T(R), 256, 0, cell_grid(s->g), kCellBlock, s->g, s->kp, (R *)s->w_aux,
+                      ghost_images ? 1 : 0));
     return hipGetLastError();
 }
 
 hipError_t launch_sensor_refine(st_solver *s, bool direct, bool ghost_images) {
     if (!s->mixed) return hipSuccess;
     if (direct)
-        hipLaunchKernelGGL((k_sensor_refine<true>), cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, s->w,
-                           (float *)s->w_aux, s->dcsidx, s->detady, s->dzitdz, ghost_images ? 1 : 0);
+        ST_GO(k_sensor_refine, ST_T(true), 256, 0, cell_grid(s->g), kCellBlock, s->g, s->kp, s->w, (float *)s->w_aux,
+              s->dcsidx, s->detady, s->dzitdz, ghost_images ? 1 : 0);
     else
-        hipLaunchKernelGGL((k_sensor_refine<false>), cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp, s->w,
-                           (float *)s->w_aux, s->dcsidx, s->detady, s->dzitdz, ghost_images ? 1 : 0);
+        ST_GO(k_sensor_refine, ST_T(false), 256, 0, cell_grid(s->g), kCellBlock, s->g, s->kp, s->w, (float *)s->w_aux,
+              s->dcsidx, s->detady, s->dzitdz, ghost_images ? 1 : 0);
     return hipGetLastError();
 }
 
 hipError_t launch_visc_div(st_solver *s, void *fl, const void *fl_prev, int stage, bool reduce, bool ord2) {
+    ST_NOREC();
     const Geo &g = s->g;
     s->force_nblk = cell_blocks(g);
     const double rkr = stage >= 0 ? s->rhork[stage] : 0.0, rkg = stage >= 0 ? s->gamrk[stage] : 0.0;
@@ -3484,8 +3590,9 @@ hipError_t launch_visc_div(st_solver *s, void *fl, const void *fl_prev, int stag
 }
 
 hipError_t launch_force_finish(st_solver *s, int stage) {
-    hipLaunchKernelGGL(k_force_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials,
-                       s->force_nblk, s->partials + s->npart - RB * 8, s->counters + 1, s->ds, stage == 0 ? 1 : 0);
+    ST_GO(k_force_finish, , RED_THREADS, 0, dim3(red_blocks(s->force_nblk)), dim3(RED_THREADS), s->kp, s->partials,
+          s->force_nblk,
+          s->partials + s->npart - RB * 8, s->counters + 1, s->ds, stage == 0 ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -3493,9 +3600,9 @@ hipError_t launch_update(st_solver *s, const void *fl_prev, const void *fl, int 
     const Geo &g = s->g;
     const bool fv = s->p.channel_case && s->p.theta_wall >= -1.0;
 #define ST_LAUNCH_U(FV_)                                                                                             \
-    ST_DISPATCH(hipLaunchKernelGGL((k_update<R, FV_>), cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w,        \
-                                   (R *)s->w_aux, (const R *)fl_prev, (const R *)fl, s->ds, s->rhork[stage],            \
-                                   s->gamrk[stage], s->p.channel_case, s->yn, s->partials))
+    ST_DISPATCH(ST_GO(k_update, ST_T(R, FV_), 256, 0, cell_grid(g), kCellBlock, g, s->kp, s->w, (R *)s->w_aux,      \
+                      (const R *)fl_prev, (const R *)fl, s->ds, s->rhork[stage], s->gamrk[stage], s->p.channel_case, \
+                      s->yn, s->partials))
     if (fv) ST_LAUNCH_U(true);
     else ST_LAUNCH_U(false);
 #undef ST_LAUNCH_U
@@ -3503,6 +3610,7 @@ hipError_t launch_update(st_solver *s, const void *fl_prev, const void *fl, int 
 }
 
 hipError_t launch_recycling(st_solver *s) {
+    ST_NOREC();
     const Geo &g = s->g;
     double *av = s->rc_real + (size_t)g.ng * g.ny * 4 + g.ny + 4 * (size_t)(g.ny + 2 * g.ng);
     const int n1 = g.ng * g.ny, n2 = g.ny * g.nz;
@@ -3516,6 +3624,7 @@ hipError_t launch_recycling(st_solver *s) {
 }
 
 hipError_t launch_inflow_draws(st_solver *s) {
+    ST_NOREC();
     const int n = s->rng_n;
     if (!s->rplane || n <= 0) return hipSuccess;
     const int nt = n > 31 ? n : 31;
@@ -3525,6 +3634,7 @@ hipError_t launch_inflow_draws(st_solver *s) {
 }
 
 hipError_t launch_rng_reset(st_solver *s) {
+    ST_NOREC();
     if (!s->rplane) return hipSuccess;
     Win31 seeded;
     for (int q = 0; q < 31; q++) seeded.v[q] = s->rng_seeded[q];
@@ -3533,6 +3643,7 @@ hipError_t launch_rng_reset(st_solver *s) {
 }
 
 hipError_t launch_physical_ghosts(st_solver *s) {
+    ST_NOREC();
     const Geo &g = s->g;
     if (s->recyc) {
         hipError_t e = launch_recycling(s);  // recyc_exchange, then face xmin
@@ -3553,6 +3664,7 @@ hipError_t launch_physical_ghosts(st_solver *s) {
 }
 
 hipError_t launch_bc_nr(st_solver *s, void *fl) {
+    ST_NOREC();
     const Geo &g = s->g;
     for (int ilat = 1; ilat <= 2 * s->kp.ndim; ilat++) {
         const int nr = s->kp.nrtag[ilat - 1];
@@ -3576,9 +3688,10 @@ hipError_t launch_bc_nr(st_solver *s, void *fl) {
 }
 
 hipError_t launch_force_var(st_solver *s) {
+    ST_NOREC();
     const Geo &g = s->g;
-    hipLaunchKernelGGL(k_force_var_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials,
-                       cell_blocks(g), s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
+    hipLaunchKernelGGL(k_force_var_finish, dim3(red_blocks(cell_blocks(g))), dim3(RED_THREADS), 0, s->stream, s->kp,
+                       s->partials, cell_blocks(g), s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
     ST_DISPATCH(hipLaunchKernelGGL(k_force_var2<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w,
                                    (const R *)s->w_aux, s->ds));
     return launch_wall_ghosts(s);
@@ -3598,23 +3711,26 @@ hipError_t launch_dt_raw(hipStream_t stream, bool mixed, const Geo &g, const KPa
     else
         hipLaunchKernelGGL(k_dt<double>, grid, kCellBlock, 0, stream, g, P, w, (double *)wa, dx1, dy1, dz1, dxs, dys,
                            dzs, partials, ticket);
-    hipLaunchKernelGGL(k_dt_finish, dim3(RB), dim3(RED_THREADS), 0, stream, P, partials,
+    hipLaunchKernelGGL(k_dt_finish, dim3(red_blocks((long long)dt_blocks_per_plane(g) * kplanes)), dim3(RED_THREADS),
+                       0, stream, P, partials,
                        dt_blocks_per_plane(g) * kplanes, stage2, ticket, dt_out);
     return hipGetLastError();
 }
 
 hipError_t launch_dt(st_solver *s) {
+    ST_NOREC();
     return launch_dt_raw(s->stream, s->mixed, s->g, s->kp, s->w, s->w_aux, s->dcsidx, s->detady, s->dzitdz,
                          s->dcsidxs, s->detadys, s->dzitdzs, s->partials, s->g.nz, s->partials + s->npart - RB * 8,
                          s->counters + 1, &s->ds->dt);
 }
 
 hipError_t launch_step_end(st_solver *s) {
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, s->stream, s->ds);
+    ST_GO(k_step_end, , 1024, 0, dim3(1), dim3(1), s->ds);
     return hipGetLastError();
 }
 
 hipError_t launch_residual(st_solver *s) {
+    ST_NOREC();
     const Geo &g = s->g;
     const int st = s->last_stage < 0 ? 0 : s->last_stage;
     const double rkr = s->last_stage < 0 ? 0.0 : s->rhork[st];
@@ -3628,12 +3744,14 @@ hipError_t launch_residual(st_solver *s) {
 }
 
 hipError_t launch_count_weno(st_solver *s) {
+    ST_NOREC();
     ST_DISPATCH(hipLaunchKernelGGL(k_count_weno<R>, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->kp,
                                    (const R *)s->w_aux, s->ds));
     return hipGetLastError();
 }
 
 hipError_t launch_stats(st_solver *s, int itav) {
+    ST_NOREC();
     const Geo &g = s->g;
     const dim3 grid((g.nx + STAT_BX - 1) / STAT_BX, g.ny);
     ST_DISPATCH(hipLaunchKernelGGL(k_stats<R>, grid, dim3(STAT_BX), 0, s->stream, g, s->kp, s->w,
@@ -3644,6 +3762,7 @@ hipError_t launch_stats(st_solver *s, int itav) {
 // initialize (singleideal.F90:3032): w_aux(:,:,:,6) = t0, the first Newton
 // start of T(e) for a thermally perfect gas
 hipError_t launch_init_temperature(st_solver *s) {
+    ST_NOREC();
     const size_t n = s->g.nvol;
     ST_DISPATCH(hipLaunchKernelGGL(k_fill<R>, dim3(grid1d(n, 256)), dim3(256), 0, s->stream,
                                    (R *)s->w_aux + 5 * n, n, R(s->p.t0)));
@@ -3651,11 +3770,13 @@ hipError_t launch_init_temperature(st_solver *s) {
 }
 
 hipError_t launch_zero(st_solver *s, double *a, size_t n) {
+    ST_NOREC();
     hipLaunchKernelGGL(k_zero<double>, dim3(grid1d(n, 256)), dim3(256), 0, s->stream, a, n);
     return hipGetLastError();
 }
 
 hipError_t launch_zero_rhs(st_solver *s, void *a) {
+    ST_NOREC();
     const size_t n = (size_t)5 * s->g.nint;
     ST_DISPATCH(hipLaunchKernelGGL(k_zero<R>, dim3(grid1d(n, 256)), dim3(256), 0, s->stream, (R *)a, n));
     return hipGetLastError();
@@ -3664,6 +3785,7 @@ hipError_t launch_zero_rhs(st_solver *s, void *a) {
 // ----------------------------------------------------- piecewise stage
 template <typename R, int DIR>
 static hipError_t face_range_t(st_solver *s, int f0, int f1) {
+    ST_NOREC();
     const Geo &g = s->g;
     const long long n1 = DIR == 0 ? f1 - f0 + 1 : g.nx, n2 = DIR == 1 ? f1 - f0 + 1 : g.ny,
                     n3 = DIR == 2 ? f1 - f0 + 1 : g.nz;
@@ -3676,6 +3798,7 @@ static hipError_t face_range_t(st_solver *s, int f0, int f1) {
 
 template <typename R, int DIR>
 static hipError_t face_update_t(st_solver *s, void *fl, int c0, int c1) {
+    ST_NOREC();
     const Geo &g = s->g;
     const long long n1 = DIR == 0 ? c1 - c0 + 1 : g.nx, n2 = DIR == 1 ? c1 - c0 + 1 : g.ny,
                     n3 = DIR == 2 ? c1 - c0 + 1 : g.nz;
@@ -3700,6 +3823,7 @@ hipError_t launch_face_update(st_solver *s, int dir, void *fl, int c0, int c1) {
 }
 
 hipError_t launch_flux_step(st_solver *s, void *fl, double coef, bool init) {
+    ST_NOREC();
     const size_t n = 5 * (size_t)s->g.nint;
     ST_DISPATCH(hipLaunchKernelGGL(k_init_flux<R>, dim3(grid1d(n, 256)), dim3(256), 0, s->stream, n, (R *)fl,
                                    s->fln, coef, init ? 1 : 0));
@@ -3707,15 +3831,18 @@ hipError_t launch_flux_step(st_solver *s, void *fl, double coef, bool init) {
 }
 
 hipError_t launch_update_field(st_solver *s) {
+    ST_NOREC();
     hipLaunchKernelGGL(k_update_field, cell_grid(s->g), kCellBlock, 0, s->stream, s->g, s->w, (const double *)s->fln);
     return hipGetLastError();
 }
 
 hipError_t launch_force_rhs_piece(st_solver *s) {
+    ST_NOREC();
     const Geo &g = s->g;
     ST_DISPATCH(hipLaunchKernelGGL(k_force_rhs_sums<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->w,
                                    (const R *)s->w_aux, (const double *)s->fln, s->yn, s->partials));
-    hipLaunchKernelGGL(k_force_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials, cell_blocks(g),
+    hipLaunchKernelGGL(k_force_finish, dim3(red_blocks(cell_blocks(g))), dim3(RED_THREADS), 0, s->stream, s->kp,
+                       s->partials, cell_blocks(g),
                        s->partials + s->npart - RB * 8, s->counters + 1, s->ds, 0);
     ST_DISPATCH(hipLaunchKernelGGL(k_force_rhs_apply<R>, cell_grid(g), kCellBlock, 0, s->stream, g,
                                    (const R *)s->w_aux, s->fln, s->ds));
@@ -3723,17 +3850,19 @@ hipError_t launch_force_rhs_piece(st_solver *s) {
 }
 
 hipError_t launch_force_var_piece(st_solver *s) {
+    ST_NOREC();
     const Geo &g = s->g;
     ST_DISPATCH(hipLaunchKernelGGL(k_force_var_sums<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w,
                                    (R *)s->w_aux, s->yn, s->partials));
-    hipLaunchKernelGGL(k_force_var_finish, dim3(RB), dim3(RED_THREADS), 0, s->stream, s->kp, s->partials,
-                       cell_blocks(g), s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
+    hipLaunchKernelGGL(k_force_var_finish, dim3(red_blocks(cell_blocks(g))), dim3(RED_THREADS), 0, s->stream, s->kp,
+                       s->partials, cell_blocks(g), s->partials + s->npart - RB * 8, s->counters + 1, s->ds);
     ST_DISPATCH(hipLaunchKernelGGL(k_force_var2<R>, cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, s->w,
                                    (const R *)s->w_aux, s->ds));
     return hipGetLastError();
 }
 
 hipError_t launch_recyc_exchange(st_solver *s) {
+    ST_NOREC();
     const Geo &g = s->g;
     const long long n = (long long)g.ng * g.ny * g.nz;
     hipLaunchKernelGGL(k_recyc_exchange, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, g, s->w,

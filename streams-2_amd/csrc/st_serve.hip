@@ -41,6 +41,7 @@
 // continues as a plain solve restarted from the same files does.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -54,6 +55,7 @@
 #include <sys/stat.h>
 #include <thread>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "st_solver.h"
@@ -160,7 +162,60 @@ struct Slot {
     TuneKey key{};
     bool ckpt_pending = false;  // a checkpoint copy in flight or being written
     int ckpt_steps = 0;
+    // batched launches
+    bool batchable = false;                  // a chunk of its steps records in body form
+    std::vector<st::BatchLaunchFn> sig;      // the launch sequence of one chunk (grouping key)
+    st::Recorder rec;                        // the chunk being batched
+    int inbatch = 0;                         // batched chunks in flight that hold this solve
+    int lane = -1;                           // its batch lane
+    bool own_work = true;                    // work on its own stream since it last joined a batch
+    bool shape_set = false;                  // the tuned launch shape of its class is applied
 };
+
+// One launch of a batched chunk: the body instance, its workgroup shape,
+// the batched grid and the byte offsets of its tables.
+struct BatchOp {
+    st::BatchLaunchFn fn;
+    dim3 block;
+    unsigned total;
+    size_t ent, own, pack;
+};
+
+// A lane runs batched chunks on its own stream, at most two in flight,
+// each with its own table buffer (device, and pinned staging for the
+// upload).  Every batched solve is bound to one lane while it runs.
+struct Lane {
+    hipStream_t stream = nullptr;
+    char *dtab[2] = {nullptr, nullptr};
+    unsigned char *htab[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::vector<int> members[2];
+    int nev = 0, head = 0;
+    // The lane's last layout (tables and launches).  A chunk of K steps
+    // returns a solve to the same host state but for the step counter, and
+    // the same arguments, so the layout is reused while the lane's solves
+    // are those it was laid out for and each is in the state the last chunk
+    // left it in.
+    std::vector<int> cmem;
+    std::vector<int64_t> cids;           // their requests (a slot is reused by later requests)
+    std::vector<st::HostMark> cafter;    // each solve's state after the last chunk
+    std::vector<unsigned char> chost;
+    std::vector<BatchOp> cops;
+    int64_t cver = 0, bver[2] = {-1, -1};  // layout version, and the version each device buffer holds
+};
+
+struct Batcher {
+    bool on = false;
+    int64_t max_cells = 0;               // solves with more interior cells run on their own streams (0: no limit)
+    std::vector<Lane> lanes;
+    size_t cap = 0;                      // bytes of one table buffer
+    hipEvent_t join = nullptr;           // orders a joining solve's own-stream work before its lane
+    std::atomic<int64_t> chunks{0}, launches{0}, solves{0};
+    std::atomic<int64_t> layouts{0}, ns_layout{0}, ns_launch{0};  // host work of the scheduler
+};
+
+constexpr size_t kBatchTabBytes = (size_t)32 << 20;  // per table buffer
+constexpr int kMaxLanes = 8;
 
 constexpr int kPlainChunk = 10;  // steps per chunk of a server without graphs (steps_per_graph 0 or 1)
 
@@ -193,6 +248,7 @@ struct st_server {
     std::vector<Slot> slots;
     std::map<TuneKey, Tune> tune;
     int tuned_classes = 0;
+    Batcher bt;
     bool stop = false;
     int64_t next_id = 1;
     int64_t completed = 0, failed = 0;
@@ -219,14 +275,11 @@ bool graphable(const st_server *sv, const ServeReq *r) {
 
 int capture(st_server *sv, Slot &sl) {
     st_solver *s = sl.s;
-    const int64_t icyc = s->icyc;
-    const int cur = s->cur, last = s->last_stage;
+    const st::HostMark m = st::host_mark(s);
     if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ST_ERR_HIP;
     int rc = st_advance(s, sv->K);
     hipError_t e = hipStreamEndCapture(s->stream, &sl.graph);
-    s->icyc = icyc;
-    s->cur = cur;
-    s->last_stage = last;
+    st::host_restore(s, m);
     if (rc) return rc;
     if (e != hipSuccess) return ST_ERR_HIP;
     if (hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0) != hipSuccess) return ST_ERR_HIP;
@@ -246,6 +299,24 @@ int setup(st_server *sv, Slot &sl) {
                           gd[12].data())))
         return rc;
     if ((rc = st_copy_from_field(sl.s, r->w0.data())) || (rc = st_init_run(sl.s))) return rc;
+    // whether a chunk of its steps records in body form, and its launch
+    // sequence (solves whose sequences agree are batched together)
+    sl.batchable = false;
+    sl.sig.clear();
+    const int64_t cells = (int64_t)r->p.nx * r->p.ny * r->p.nz;
+    if (sv->bt.on && graphable(sv, r) && (sv->bt.max_cells <= 0 || cells <= sv->bt.max_cells)) {
+        const st::HostMark m = st::host_mark(sl.s);
+        sl.rec.recs.clear();
+        sl.rec.bad = false;
+        sl.s->rec = &sl.rec;
+        rc = st_advance(sl.s, sv->K);
+        sl.s->rec = nullptr;
+        st::host_restore(sl.s, m);
+        if (rc) return rc;
+        sl.batchable = !sl.rec.bad && !sl.rec.recs.empty();
+        for (const auto &q : sl.rec.recs) sl.sig.push_back(q.fn);
+        sl.rec.recs.clear();
+    }
     if (r->ckpt_every > 0) {
         const size_t need = (size_t)5 * sl.s->g.nvol;
         if (sl.ckpt_cap < need) {
@@ -349,6 +420,13 @@ void finish(st_server *sv, Slot &sl) {
     sl.err.clear();
     sl.ckpt_pending = false;
     sl.ckpt_steps = 0;
+    sl.batchable = false;
+    sl.sig.clear();
+    sl.rec.recs.clear();
+    sl.inbatch = 0;
+    sl.lane = -1;
+    sl.own_work = true;
+    sl.shape_set = false;
     sl.state = kFree;
     sv->cv_work.notify_all();
 }
@@ -516,6 +594,274 @@ void drop_candidates(st_server *sv, Slot &sl) {
         }
 }
 
+// Scheduler: whether the launch shape of a solve's class is settled.
+bool class_done(st_server *sv, const Slot &sl, Tune *best = nullptr) {
+    if (!sv->autotune) return true;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->tune.find(sl.key);
+    if (it == sv->tune.end() || !it->second.done) return false;
+    if (best) *best = it->second;
+    return true;
+}
+
+// Scheduler: a solve whose next chunk belongs in a batch (its class is
+// tuned and a whole chunk is left); it launches nothing on its own stream.
+bool batch_bound(st_server *sv, const Slot &sl) {
+    return sv->bt.on && sl.batchable && sl.r->nsteps - sl.launched >= sv->K && class_done(sv, sl);
+}
+
+// Scheduler: retire the batched chunks whose events have fired.
+void retire_batch(st_server *sv, Lane &ln, bool *progress) {
+    while (ln.nev > 0 && hipEventQuery(ln.ev[ln.head]) == hipSuccess) {
+        for (int q : ln.members[ln.head]) sv->slots[q].inbatch--;
+        ln.members[ln.head].clear();
+        ln.head = (ln.head + 1) % 2;
+        ln.nev--;
+        *progress = true;
+    }
+}
+
+// Lay out the tables of one batched chunk of the solves `mem` (their
+// recorders hold the same launch sequence) in bt.host: per launch, the
+// solves' entries, the owner table (one int per 8 workgroups) and the
+// packed arguments, each 256-byte aligned.  Identical tables (a launch
+// repeated with the same arguments, entries shared by launches of the same
+// grids) are stored once.  false if the layout exceeds a table buffer.
+bool lay_out(st_server *sv, const std::vector<int> &mem, std::vector<unsigned char> &host,
+             std::vector<BatchOp> &ops) {
+    host.clear();
+    ops.clear();
+    std::unordered_map<std::string, size_t> seen;
+    auto put = [&](const void *p, size_t n) -> size_t {
+        std::string key((const char *)p, n);
+        auto it = seen.find(key);
+        if (it != seen.end()) return it->second;
+        const size_t off = (host.size() + 255) / 256 * 256;
+        host.resize(off + n);
+        std::memcpy(host.data() + off, p, n);
+        seen.emplace(std::move(key), off);
+        return off;
+    };
+    const size_t nops = sv->slots[mem[0]].rec.recs.size();
+    std::vector<st::BatchEntry> ent(mem.size());
+    std::vector<int> own;
+    std::vector<unsigned char> packs;
+    for (size_t i = 0; i < nops; i++) {
+        const st::BatchRec &r0 = sv->slots[mem[0]].rec.recs[i];
+        const size_t psz = r0.pack.size();
+        unsigned total = 0;
+        own.clear();
+        packs.resize(psz * mem.size());
+        for (size_t q = 0; q < mem.size(); q++) {
+            const st::BatchRec &r = sv->slots[mem[q]].rec.recs[i];
+            const unsigned nb = r.grid.x * r.grid.y * r.grid.z;
+            const unsigned nbp = (nb + st::kBatchGroup - 1) / st::kBatchGroup * st::kBatchGroup;
+            ent[q] = st::BatchEntry{(int)total, (int)nb, (int)r.grid.x, (int)r.grid.y, (int)r.grid.z, {0, 0, 0}};
+            own.insert(own.end(), nbp / st::kBatchGroup, (int)q);
+            std::memcpy(packs.data() + q * psz, r.pack.data(), psz);
+            total += nbp;
+        }
+        BatchOp op;
+        op.fn = r0.fn;
+        op.block = r0.block;
+        op.total = total;
+        op.ent = put(ent.data(), sizeof(st::BatchEntry) * mem.size());
+        op.own = put(own.data(), sizeof(int) * own.size());
+        op.pack = put(packs.data(), packs.size());
+        ops.push_back(op);
+        if (host.size() > sv->bt.cap) return false;
+    }
+    return true;
+}
+
+void batch_error(st_server *sv, Lane &ln, const std::vector<int> &mem, const char *what) {
+    // launches already enqueued still use the solves' memory
+    (void)hipStreamSynchronize(ln.stream);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (int q : mem) {
+        sv->slots[q].rc = ST_ERR_HIP;
+        sv->slots[q].err = what;
+    }
+}
+
+// Scheduler: record one chunk of each solve of `mem` and lay the lane's
+// tables out anew.  A solve whose recording does not follow its launch
+// sequence leaves the batch path; solves that do not fit in a table buffer
+// wait for a later chunk.  The layout is kept for reuse when every solve
+// came back to its state but for the step counter.  false if no solve is
+// left.
+bool record_chunk(st_server *sv, Lane &ln, std::vector<int> &mem) {
+    std::vector<st::HostMark> marks;
+    std::vector<int> rec_ok;
+    for (int q : mem) {
+        Slot &sl = sv->slots[q];
+        if (!sl.shape_set) {
+            Tune t;
+            if (sv->autotune && class_done(sv, sl, &t)) (void)st_set_tuning(sl.s, t.best.first, t.best.second);
+            sl.shape_set = true;
+        }
+        const st::HostMark m = st::host_mark(sl.s);
+        sl.rec.recs.clear();
+        sl.rec.bad = false;
+        sl.s->rec = &sl.rec;
+        const int rc = st_advance(sl.s, sv->K);
+        sl.s->rec = nullptr;
+        bool same = rc == ST_OK && !sl.rec.bad && sl.rec.recs.size() == sl.sig.size();
+        for (size_t i = 0; same && i < sl.sig.size(); i++) same = sl.rec.recs[i].fn == sl.sig[i];
+        if (!same) {
+            // not batchable after all: it runs on its own stream
+            st::host_restore(sl.s, m);
+            sl.batchable = false;
+            sl.rec.recs.clear();
+            continue;
+        }
+        marks.push_back(m);
+        rec_ok.push_back(q);
+    }
+    mem.swap(rec_ok);
+    while (!mem.empty() && !lay_out(sv, mem, ln.chost, ln.cops)) {
+        // too large for a table buffer: the last solve waits for the next chunk
+        Slot &sl = sv->slots[mem.back()];
+        st::host_restore(sl.s, marks.back());
+        sl.rec.recs.clear();
+        if (mem.size() == 1) sl.batchable = false;
+        mem.pop_back();
+        marks.pop_back();
+    }
+    ln.cver++;
+    ln.cmem.clear();
+    ln.cids.clear();
+    ln.cafter.clear();
+    if (mem.empty()) return false;
+    bool periodic = true;
+    std::vector<st::HostMark> after;
+    for (size_t i = 0; i < mem.size(); i++) {
+        Slot &sl = sv->slots[mem[i]];
+        after.push_back(st::host_mark(sl.s));
+        periodic &= st::same_state(marks[i], after.back(), false);
+        sl.rec.recs.clear();
+    }
+    if (periodic) {
+        ln.cmem = mem;
+        ln.cafter = after;
+        for (int q : mem) ln.cids.push_back(sv->slots[q].r->id);
+    }
+    sv->bt.layouts++;
+    return true;
+}
+
+// Scheduler: bind a solve that is about to join a batch to the lane that
+// holds the fewest cells.
+void bind_lane(st_server *sv, Slot &sl) {
+    std::vector<double> cells(sv->bt.lanes.size(), 0.0);
+    for (const Slot &o : sv->slots)
+        if (o.lane >= 0 && o.s) cells[o.lane] += (double)o.s->g.nint;
+    int best = 0;
+    for (int l = 1; l < (int)cells.size(); l++)
+        if (cells[l] < cells[best]) best = l;
+    sl.lane = best;
+}
+
+// Scheduler: the next batched chunk of lane `li`.  Its solves are the
+// ready ones of the lane (set up, no chunk in flight on their own stream,
+// no checkpoint copy pending) whose launch sequence is that of the oldest
+// of them.  Each is recorded for K steps (which advances its host state as
+// enqueuing does), the tables are laid out and uploaded (skipped when the
+// buffer already holds them), and one launch per recorded launch follows
+// on the lane's stream.
+void launch_batch(st_server *sv, int li, bool *progress) {
+    Batcher &bt = sv->bt;
+    Lane &ln = bt.lanes[li];
+    if (ln.nev >= 2) return;
+    std::vector<int> ready;
+    int lead = -1;
+    for (int q = 0; q < (int)sv->slots.size(); q++) {
+        Slot &sl = sv->slots[q];
+        bool ok;
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            ok = sl.state == kRunning && sl.s && sl.rc == ST_OK && !sl.ckpt_pending;
+        }
+        if (!ok || sl.nev > 0 || !batch_bound(sv, sl)) continue;
+        if (sl.lane < 0) bind_lane(sv, sl);
+        if (sl.lane != li) continue;
+        ready.push_back(q);
+        if (lead < 0 || sl.r->id < sv->slots[lead].r->id) lead = q;
+    }
+    if (lead < 0) return;
+    std::vector<int> mem;
+    for (int q : ready)
+        if (sv->slots[q].sig == sv->slots[lead].sig && (int)mem.size() < st::kBatchMax) mem.push_back(q);
+    const auto t0 = Clock::now();
+    bool reuse = mem == ln.cmem;
+    for (size_t i = 0; reuse && i < mem.size(); i++)
+        reuse = sv->slots[mem[i]].r->id == ln.cids[i] &&
+                st::same_state(st::host_mark(sv->slots[mem[i]].s), ln.cafter[i], true);
+    if (reuse) {
+        // the same chunk again: advance each solve's host state as its
+        // recording would
+        for (size_t i = 0; i < mem.size(); i++) {
+            ln.cafter[i].icyc += sv->K;
+            st::host_restore(sv->slots[mem[i]].s, ln.cafter[i]);
+        }
+    } else if (!record_chunk(sv, ln, mem)) {
+        return;
+    }
+    const auto t1 = Clock::now();
+    if (mem.empty()) return;
+    const int b = (ln.head + ln.nev) % 2;
+    for (int q : mem) {
+        Slot &sl = sv->slots[q];
+        if (!sl.own_work) continue;
+        if (hipEventRecord(bt.join, sl.stream) != hipSuccess || hipStreamWaitEvent(ln.stream, bt.join, 0) != hipSuccess)
+            return batch_error(sv, ln, mem, "batched chunk: stream ordering failed");
+        sl.own_work = false;
+    }
+    if (ln.bver[b] != ln.cver) {
+        std::memcpy(ln.htab[b], ln.chost.data(), ln.chost.size());
+        if (hipMemcpyAsync(ln.dtab[b], ln.htab[b], ln.chost.size(), hipMemcpyHostToDevice, ln.stream) != hipSuccess)
+            return batch_error(sv, ln, mem, "batched chunk: table upload failed");
+        ln.bver[b] = ln.cver;
+    }
+    char *d = ln.dtab[b];
+    for (const BatchOp &op : ln.cops)
+        if (op.fn(op.total, op.block, ln.stream, d + op.ent, d + op.pack, (const int *)(d + op.own)) != hipSuccess)
+            return batch_error(sv, ln, mem, "batched launch failed");
+    if (hipEventRecord(ln.ev[b], ln.stream) != hipSuccess)
+        return batch_error(sv, ln, mem, "batched chunk: event failed");
+    ln.members[b] = mem;
+    ln.nev++;
+    bt.chunks++;
+    bt.launches += (int64_t)ln.cops.size();
+    bt.solves += (int64_t)mem.size();
+    bt.ns_layout += std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+    bt.ns_launch += std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - t1).count();
+    for (int q : mem) {
+        Slot &sl = sv->slots[q];
+        sl.rec.recs.clear();
+        sl.launched += sv->K;
+        sl.inbatch++;
+        ServeReq *r = sl.r;
+        // a checkpoint after the chunk that reaches the period (or the last
+        // step): copied on the solve's own stream once the batch has passed
+        if (r->ckpt_every > 0 && (sl.launched % r->ckpt_every == 0 || sl.launched == r->nsteps)) {
+            st_solver *s = sl.s;
+            if (hipStreamWaitEvent(sl.stream, ln.ev[b], 0) != hipSuccess ||
+                hipMemcpyAsync(sl.ckpt_w, s->w, sizeof(double) * 5 * s->g.nvol, hipMemcpyDeviceToHost, sl.stream) !=
+                    hipSuccess ||
+                hipMemcpyAsync(sl.ckpt_sc, s->ds, sizeof(st::DevScalars), hipMemcpyDeviceToHost, sl.stream) !=
+                    hipSuccess ||
+                hipEventRecord(sl.ckpt_ev, sl.stream) != hipSuccess)
+                return batch_error(sv, ln, {q}, "checkpoint copy failed");
+            sl.own_work = true;
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sl.ckpt_pending = true;
+            sl.ckpt_steps = sl.launched;
+        }
+    }
+    *progress = true;
+}
+
 void serve_loop(st_server *sv) {
     (void)hipSetDevice(sv->device);
     for (;;) {
@@ -582,8 +928,10 @@ void serve_loop(st_server *sv) {
                 std::lock_guard<std::mutex> lk(sv->mu);
                 rc = sl.rc;  // a failed set-up, or a checkpoint that could not be written
             }
-            while (rc == ST_OK && sl.nev < 2 && sl.launched < sl.r->nsteps && !ckpt_pending) {
+            while (rc == ST_OK && sl.nev < 2 && sl.launched < sl.r->nsteps && !ckpt_pending && sl.inbatch == 0 &&
+                   !batch_bound(sv, sl)) {
                 rc = launch_chunk(sv, sl);
+                sl.own_work = true;
                 if (rc) {
                     std::lock_guard<std::mutex> lk(sv->mu);
                     sl.err = st_last_error();
@@ -599,13 +947,22 @@ void serve_loop(st_server *sv) {
                 std::lock_guard<std::mutex> lk(sv->mu);
                 ckpt_busy = sl.ckpt_pending;
             }
-            if (rc != ST_OK || (sl.nev == 0 && sl.launched >= sl.r->nsteps && !ckpt_busy)) {
+            // a solve held by a batched chunk in flight stays until the chunk has passed
+            if (sl.inbatch == 0 && (rc != ST_OK || (sl.nev == 0 && sl.launched >= sl.r->nsteps && !ckpt_busy))) {
                 if (rc != ST_OK) drop_candidates(sv, sl);
                 std::lock_guard<std::mutex> lk(sv->mu);
                 sl.rc = rc;
                 sl.state = kDraining;
                 push_job(sv, kFinishJob, q);
                 progress = true;
+            }
+        }
+        if (sv->bt.on) {
+            for (int l = 0; l < (int)sv->bt.lanes.size(); l++) {
+                Lane &ln = sv->bt.lanes[l];
+                retire_batch(sv, ln, &progress);
+                launch_batch(sv, l, &progress);
+                active |= ln.nev > 0;
             }
         }
         if (!progress) {
@@ -642,6 +999,19 @@ void destroy_slots(st_server *sv) {
     }
     if (sv->pool.base) (void)hipFree(sv->pool.base);
     sv->pool.base = nullptr;
+    Batcher &bt = sv->bt;
+    for (Lane &ln : bt.lanes) {
+        if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+        for (int b = 0; b < 2; b++) {
+            if (ln.ev[b]) (void)hipEventDestroy(ln.ev[b]);
+            if (ln.dtab[b]) (void)hipFree(ln.dtab[b]);
+            if (ln.htab[b]) (void)hipHostFree(ln.htab[b]);
+        }
+        if (ln.stream) (void)hipStreamDestroy(ln.stream);
+    }
+    bt.lanes.clear();
+    if (bt.join) (void)hipEventDestroy(bt.join);
+    bt.join = nullptr;
 }
 
 }  // namespace
@@ -787,6 +1157,50 @@ int st_server_wait(st_server *sv, int64_t ticket, double timeout_s, double *w_ou
     }
     sv->reqs.erase(it);
     return rc;
+}
+
+int st_server_set_batching(st_server *sv, int32_t lanes, int64_t max_cells) {
+    if (!sv) return sfail(ST_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->next_id != 1) return sfail(ST_ERR_STATE, "batching is set before the first submission");
+    if (sv->K < 2 || sv->K % 2 != 0)
+        return sfail(ST_ERR_ARG, "batched chunks need an even steps_per_graph of at least 2");
+    Batcher &bt = sv->bt;
+    if (lanes < 0 || lanes > kMaxLanes || max_cells < 0) return sfail(ST_ERR_ARG, "batch lanes out of range");
+    bt.max_cells = max_cells;
+    if (!bt.lanes.empty()) {
+        if (lanes != 0 && lanes != (int)bt.lanes.size())
+            return sfail(ST_ERR_STATE, "the number of batch lanes is set once");
+        bt.on = lanes != 0;
+        return ST_OK;
+    }
+    if (lanes == 0) return ST_OK;
+    // each lane's stream, its events and two table buffers, once
+    bt.lanes.resize(lanes);
+    hipError_t e = hipEventCreateWithFlags(&bt.join, hipEventDisableTiming);
+    for (Lane &ln : bt.lanes) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking);
+        for (int b = 0; b < 2; b++) {
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ln.ev[b], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipMalloc((void **)&ln.dtab[b], kBatchTabBytes);
+            if (e == hipSuccess) e = hipHostMalloc((void **)&ln.htab[b], kBatchTabBytes, hipHostMallocDefault);
+        }
+    }
+    if (e != hipSuccess) return sfail(ST_ERR_NOMEM, std::string("batch tables: ") + hipGetErrorString(e));
+    bt.cap = kBatchTabBytes;
+    bt.on = true;
+    return ST_OK;
+}
+
+int st_server_batch_stats(st_server *sv, int64_t out[6]) {
+    if (!sv || !out) return sfail(ST_ERR_ARG, "null argument");
+    out[0] = sv->bt.chunks;
+    out[1] = sv->bt.launches;
+    out[2] = sv->bt.solves;
+    out[3] = sv->bt.layouts;
+    out[4] = sv->bt.ns_layout / 1000;
+    out[5] = sv->bt.ns_launch / 1000;
+    return ST_OK;
 }
 
 int st_server_stats(st_server *sv, int64_t out[6]) {

@@ -3,6 +3,7 @@
 
 #include <string>
 
+#include "st_batch.h"
 #include "st_internal.h"
 
 struct st_solver {
@@ -68,6 +69,9 @@ struct st_solver {
     int64_t icyc, icyc0; // host mirror of the step counter
     // optional per-kernel timing (st_kernel_times)
     struct KTimer *timer;
+    // while set, the launchers record their launches here instead of
+    // enqueuing them (batched launches over several solves)
+    st::Recorder *rec;
     // memory lent by the creator (st::SlotMem): [arena_lo, arena_hi) holds
     // the solver's arrays, hs is the lender's; neither is freed with the solver
     char *arena_lo, *arena_hi;
@@ -103,6 +107,31 @@ struct SlotMem {
 };
 // st_solver_create inside lent memory: no allocation and no synchronisation
 int solver_create_in(const st_params *p, const SlotMem &mem, st_solver **out);
+
+// The host-side state that enqueuing (or recording) steps advances; a
+// recording that is not launched is undone by restoring it.
+struct HostMark {
+    int64_t icyc;
+    int cur, last_stage, force_nblk, zv_nblk;
+    bool dt_in_stage, fln_current;
+};
+inline HostMark host_mark(const st_solver *s) {
+    return HostMark{s->icyc, s->cur, s->last_stage, s->force_nblk, s->zv_nblk, s->dt_in_stage, s->fln_current};
+}
+inline bool same_state(const HostMark &a, const HostMark &b, bool with_step) {
+    return (!with_step || a.icyc == b.icyc) && a.cur == b.cur && a.last_stage == b.last_stage &&
+           a.force_nblk == b.force_nblk && a.zv_nblk == b.zv_nblk && a.dt_in_stage == b.dt_in_stage &&
+           a.fln_current == b.fln_current;
+}
+inline void host_restore(st_solver *s, const HostMark &m) {
+    s->icyc = m.icyc;
+    s->cur = m.cur;
+    s->last_stage = m.last_stage;
+    s->force_nblk = m.force_nblk;
+    s->zv_nblk = m.zv_nblk;
+    s->dt_in_stage = m.dt_in_stage;
+    s->fln_current = m.fln_current;
+}
 
 // parameter handling shared by the entry points (st_capi.hip)
 int set_error(int code, const std::string &msg);  // message for st_last_error; returns code

@@ -88,6 +88,8 @@ _SIGNATURES = {
     "st_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(C.c_double), C.POINTER(StScalars),
                                  C.POINTER(C.c_double)]),
     "st_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "st_server_set_batching": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
+    "st_server_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "st_server_last_error": (C.c_char_p, []),
 }
 

@@ -2,8 +2,10 @@
 
 ``Server`` wraps the continuous-batching server of the C library
 (st_server_*): requests are admitted as slots and HBM free up, run
-concurrently on their own streams from captured graphs, and are returned
-with their submit / admit / done times.  ``serving_workload`` draws the
+concurrently on their own streams from captured graphs (or, with
+``batch=True``, the chunks of all running solves of one launch sequence as
+one sequence of batched launches over a table of the solves), and are
+returned with their submit / admit / done times.  ``serving_workload`` draws the
 seeded stream of uneven channel solves, ``run_serving`` replays it with
 Poisson arrivals and reports throughput and latency percentiles.
 """
@@ -27,13 +29,19 @@ def _dp(a: np.ndarray):
 
 
 class Server:
-    def __init__(self, hbm_bytes: int, max_slots: int = 8, steps_per_graph: int = 10, autotune: bool = True):
+    def __init__(self, hbm_bytes: int, max_slots: int = 8, steps_per_graph: int = 10, autotune: bool = True,
+                 batch: int = 0, batch_max_cells: int = 0):
+        """``batch``: number of batch lanes (0: every solve on its own
+        stream); solves of more than ``batch_max_cells`` interior cells (0:
+        no limit) keep their own streams."""
         self.L = capi.lib()
         h = C.c_void_p()
         capi.check_server(self.L.st_server_create(int(hbm_bytes), max_slots, steps_per_graph, 1 if autotune else 0,
                                                   C.byref(h)))
         self.h = h
         self._shapes = {}
+        if batch:
+            capi.check_server(self.L.st_server_set_batching(self.h, int(batch), int(batch_max_cells)))
 
     def submit(self, params: StParams, grid, w0: np.ndarray, nsteps: int, time0: float = 0.0, icyc0: int = 0,
                checkpoint_every: int = 0, checkpoint_dir: str | None = None) -> int:
@@ -62,7 +70,12 @@ class Server:
     def stats(self):
         out = (C.c_int64 * 6)()
         capi.check_server(self.L.st_server_stats(self.h, out))
-        return dict(zip(("queued", "running", "completed", "failed", "hbm_bytes", "tuned_classes"), list(out)))
+        st = dict(zip(("queued", "running", "completed", "failed", "hbm_bytes", "tuned_classes"), list(out)))
+        b = (C.c_int64 * 6)()
+        capi.check_server(self.L.st_server_batch_stats(self.h, b))
+        st.update(zip(("batched_chunks", "batched_launches", "batched_solves", "batch_layouts",
+                       "batch_layout_us", "batch_launch_us"), list(b)))
+        return st
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -117,7 +130,7 @@ def cell_updates(reqs) -> float:
 
 
 def run_serving(reqs, hbm_bytes: int, max_slots: int = 8, steps_per_graph: int = 10, autotune: bool = True,
-                want_fields: bool = False, timeout: float = 600.0):
+                want_fields: bool = False, timeout: float = 600.0, batch: int = 0, batch_max_cells: int = 0):
     """Submit each request at its arrival time, wait for all; returns a dict
     with throughput (Gcell-updates/s per RK stage over the makespan from the
     first arrival), p50/p99 time-to-solution and the per-request results.
@@ -126,7 +139,7 @@ def run_serving(reqs, hbm_bytes: int, max_slots: int = 8, steps_per_graph: int =
     throughput and latencies cover the completed requests.  A request not
     finished ``timeout`` seconds after the wait for it began raises."""
     out = []
-    with Server(hbm_bytes, max_slots, steps_per_graph, autotune) as sv:
+    with Server(hbm_bytes, max_slots, steps_per_graph, autotune, batch, batch_max_cells) as sv:
         t0 = time.perf_counter()
         tickets = []
         for r in reqs:

@@ -1,8 +1,9 @@
 """The serving loop: continuous batching of independent solves.
 
 Each result must be the same alone (plain ChannelSolver path) and in the
-mix (server: concurrent streams, captured graphs, autotuned launch shapes),
-and admission must respect the HBM capacity given to the server.
+mix (server: concurrent streams, captured graphs, autotuned launch shapes,
+batched launches over a table of solves), and admission must respect the
+HBM capacity given to the server.
 """
 import numpy as np
 import pytest
@@ -150,3 +151,93 @@ def test_checkpoint_period_must_be_whole_graphs(tmp_path):
     with Server(hbm_bytes=8 << 30, max_slots=1, steps_per_graph=10) as sv:
         with pytest.raises(Exception):
             sv.submit(c.params, c.grid, c.w0, 20, checkpoint_every=7, checkpoint_dir=str(tmp_path))
+
+
+# ------------------------------------------------------------ batched launches
+
+@pytest.mark.parametrize("precision,lanes,max_cells", [(0, 1, 0), (PREC_MIXED, 1, 0), (0, 2, 0), (0, 2, 33000)])
+def test_batched_same_alone_and_in_the_mix(precision, lanes, max_cells):
+    """Batched launches: the chunks of all running solves go out as one
+    launch per kernel over a table of the solves.  Each result equals its
+    plain solve bit for bit; the final chunks shorter than a graph run on
+    the solves' own streams, so both paths are exercised (with a cell
+    limit, the larger solves keep their own streams throughout)."""
+    reqs = _requests(precision)
+    seq = run_sequential(reqs, want_fields=True)
+    srv = run_serving(reqs, hbm_bytes=8 << 30, max_slots=6, steps_per_graph=10, autotune=False, want_fields=True,
+                      timeout=120.0, batch=lanes, batch_max_cells=max_cells)
+    st = srv["stats"]
+    print("batched:", st)
+    assert st["batched_chunks"] > 0
+    for n, (a, b) in enumerate(zip(seq["results"], srv["results"])):
+        assert b["error"] is None, (n, b["error"])
+        assert np.array_equal(interior(a["w"]), interior(b["w"])), n
+        assert b["scalars"]["icyc"] == reqs[n].nsteps
+        assert a["scalars"]["residual_rhou"] == b["scalars"]["residual_rhou"]
+
+
+def test_batched_with_online_tuning_matches_the_plain_path():
+    """Tuning chunks run on the solves' own streams; once a class is tuned
+    its solves join the batch with the tuned launch shape."""
+    reqs = _requests()
+    reqs = [Request(r.case, 60, 0.0) for r in reqs]
+    seq = run_sequential(reqs, want_fields=True)
+    srv = run_serving(reqs, hbm_bytes=8 << 30, max_slots=6, steps_per_graph=4, autotune=True, want_fields=True,
+                      timeout=120.0, batch=1)
+    print("batched with tuning:", srv["stats"])
+    assert srv["stats"]["tuned_classes"] >= 1 and srv["stats"]["batched_chunks"] > 0
+    for n, (a, b) in enumerate(zip(seq["results"], srv["results"])):
+        assert np.array_equal(interior(a["w"]), interior(b["w"])), n
+
+
+def test_batched_diverging_request_fails_alone():
+    from streams_mi355x.capi import ST_ERR_NAN
+
+    reqs = _requests()[:4]
+    bad = 2
+    c = reqs[bad].case
+    w0 = c.w0.copy()
+    w0[0, 3 + 2, 3 + 5, 3 + 7] = np.nan
+    reqs[bad] = Request(type(c)(c.case, c.grid, c.params, w0), reqs[bad].nsteps, 0.0)
+    seq = run_sequential(reqs, want_fields=True)
+    srv = run_serving(reqs, hbm_bytes=8 << 30, max_slots=4, steps_per_graph=10, autotune=False, want_fields=True,
+                      timeout=120.0, batch=1)
+    assert srv["failed"] == 1 and srv["stats"]["batched_chunks"] > 0
+    assert f"[st error {ST_ERR_NAN}]" in srv["results"][bad]["error"]
+    for n, (a, b) in enumerate(zip(seq["results"], srv["results"])):
+        if n != bad:
+            assert b["error"] is None
+            assert np.array_equal(interior(a["w"]), interior(b["w"])), n
+
+
+def test_batched_checkpoints(tmp_path):
+    """Checkpoints of batched solves: each request's rst.bin after its last
+    step holds its final field, which equals its plain solve."""
+    from streams_mi355x.io import read_restart
+
+    reqs = _requests()[:3]
+    dirs = [tmp_path / f"req{n}" for n in range(len(reqs))]
+    with Server(hbm_bytes=8 << 30, max_slots=3, steps_per_graph=10, autotune=False, batch=1) as sv:
+        tk = [sv.submit(r.case.params, r.case.grid, r.case.w0, 30, checkpoint_every=10, checkpoint_dir=str(d))
+              for r, d in zip(reqs, dirs)]
+        out = [sv.wait(t, 120.0) for t in tk]
+        st = sv.stats()
+    assert st["batched_chunks"] > 0
+    seq = run_sequential([Request(r.case, 30, 0.0) for r in reqs], want_fields=True)
+    for r, d, (w, sc, _), a in zip(reqs, dirs, out, seq["results"]):
+        p = r.case.params
+        assert np.array_equal(read_restart(d / "rst.bin", p.nx, p.ny, p.nz), interior(w))
+        assert np.array_equal(interior(a["w"]), interior(w))
+        assert sc["icyc"] == 30
+
+
+def test_batching_is_set_before_the_first_submission():
+    c = make_channel()
+    with Server(hbm_bytes=8 << 30, max_slots=1, steps_per_graph=10) as sv:
+        sv.submit(c.params, c.grid, c.w0, 2)
+        from streams_mi355x.capi import check_server
+
+        with pytest.raises(Exception):
+            check_server(lib().st_server_set_batching(sv.h, 1, 0))
+    with pytest.raises(Exception):
+        Server(hbm_bytes=8 << 30, max_slots=1, steps_per_graph=3, batch=1)
