@@ -29,7 +29,11 @@ Two more blocks of the same line:
   sizes (grids uniform in [64,256] x [64,128] x [64,128], bulk Mach 0.3-3.0,
   Re_tau 100-500, Poisson arrivals, 200 steps each) through the library's
   continuous-batching server, with p50/p99 time-to-solution, against a
-  sample of the same requests run one after another through the plain path.
+  sample of the same requests run one after another through the plain path;
+  the same stream with batched launches (solves up to 2^20 cells share one
+  launch per kernel per lane), and a stream of 128 small solves (grids of
+  32-64 cells a side) served unbatched and batched.  Batched results are
+  checked bit for bit against the unbatched ones.
 
 --workload sbli runs the shock / boundary-layer box instead (1536 x 320 x
 192, recycling inflow, imposed oblique shock, conservative viscous form) in
@@ -239,6 +243,7 @@ def serving(args):
     srv = run_serving(reqs, hbm_bytes=hbm, max_slots=args.serve_slots, steps_per_graph=10, autotune=True,
                       want_fields=True)
     progress(f"served: {srv['gcups']:.3f} Gcell-updates/s, p50 {srv['p50_s']:.2f} s, p99 {srv['p99_s']:.2f} s")
+    batched = serving_batched(args, reqs, srv, hbm)
     # the same requests in the mixed form, each compared with its FP64 served
     # result (bitwise the FP64 plain solve of that request)
     import numpy as np
@@ -269,7 +274,62 @@ def serving(args):
             "speedup_vs_sequential": srv["gcups"] / seq["gcups"], "slots": args.serve_slots,
             "hbm_capacity_gib": args.serve_hbm_gib, "tuned_shape_classes": srv["stats"]["tuned_classes"],
             "total_cell_updates": cell_updates([r for r, o in zip(reqs, srv["results"]) if o["error"] is None]),
-            "mixed": mixed}
+            "mixed": mixed, "batched": batched, "small_grid_stream": serving_small(args, hbm)}
+
+
+def _same_fields(a_results, b_results):
+    import numpy as np
+    return all((a["error"] is None) == (b["error"] is None) and
+               (a["error"] is not None or np.array_equal(a["w"][:, 3:-3, 3:-3, 3:-3], b["w"][:, 3:-3, 3:-3, 3:-3]))
+               for a, b in zip(a_results, b_results))
+
+
+def _batch_summary(r):
+    st = r["stats"]
+    n = max(1, st["batched_chunks"])
+    return {"value": r["gcups"], "p50_s": r["p50_s"], "p99_s": r["p99_s"], "failed_requests": r["failed"],
+            "batched_chunks": st["batched_chunks"], "solves_per_batched_chunk": st["batched_solves"] / n,
+            "host_ms_per_batched_chunk": (st["batch_layout_us"] + st["batch_launch_us"]) / n / 1e3}
+
+
+def serving_batched(args, reqs, srv, hbm):
+    """The same stream with batched launches: solves of at most
+    --serve-batch-max-cells cells are bound to --serve-batch-lanes lanes and
+    each lane issues one launch per kernel over a table of its solves.
+    Every result must equal the unbatched server's bit for bit."""
+    from streams_mi355x.serve import run_serving
+
+    r = run_serving(reqs, hbm_bytes=hbm, max_slots=args.serve_slots, steps_per_graph=10, autotune=True,
+                    want_fields=True, batch=args.serve_batch_lanes, batch_max_cells=args.serve_batch_max_cells)
+    out = _batch_summary(r)
+    out.update({"lanes": args.serve_batch_lanes, "max_cells": args.serve_batch_max_cells,
+                "same_results_as_unbatched": _same_fields(srv["results"], r["results"]),
+                "speedup_vs_unbatched": r["gcups"] / srv["gcups"]})
+    progress(f"served batched: {r['gcups']:.3f} Gcell-updates/s, p50 {r['p50_s']:.2f} s, p99 {r['p99_s']:.2f} s, "
+             f"same results {out['same_results_as_unbatched']}")
+    return out
+
+
+def serving_small(args, hbm):
+    """A stream of small solves (128 requests, grids uniform in [32,64]^3,
+    Poisson arrivals at 64/s, --serve-steps steps each, FP64), where one
+    solve's launches cannot fill the device: served unbatched (32 slots on
+    their own streams) and batched (the same slots in --serve-batch-lanes
+    lanes, no cell limit).  Results must agree bit for bit."""
+    from streams_mi355x.serve import run_serving, serving_workload
+
+    small = serving_workload(128, args.serve_steps, rate=64.0, seed=1, nx_range=(32, 64), nyz_range=(32, 64))
+    a = run_serving(small, hbm_bytes=hbm, max_slots=32, steps_per_graph=10, autotune=True, want_fields=True)
+    b = run_serving(small, hbm_bytes=hbm, max_slots=32, steps_per_graph=10, autotune=True, want_fields=True,
+                    batch=args.serve_batch_lanes)
+    out = {"requests": len(small), "grids": "uniform in [32,64]^3", "arrival_rate_per_s": 64.0, "slots": 32,
+           "unbatched": {"value": a["gcups"], "p50_s": a["p50_s"], "p99_s": a["p99_s"], "failed_requests": a["failed"]},
+           "batched": _batch_summary(b), "lanes": args.serve_batch_lanes,
+           "speedup_batched_vs_unbatched": b["gcups"] / a["gcups"],
+           "same_results": _same_fields(a["results"], b["results"]), "unit": "Gcell-updates/s per RK stage"}
+    progress(f"small-grid stream: unbatched {a['gcups']:.3f}, batched {b['gcups']:.3f} Gcell-updates/s, "
+             f"same results {out['same_results']}")
+    return out
 
 
 def parse():
@@ -291,6 +351,9 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-hbm-gib", type=float, default=64.0)
     ap.add_argument("--serve-seq-sample", type=int, default=64)
+    ap.add_argument("--serve-batch-lanes", type=int, default=2, help="batch lanes of the batched serving runs")
+    ap.add_argument("--serve-batch-max-cells", type=int, default=1 << 20,
+                    help="solves of more cells keep their own streams in the batched run of the main stream")
     ap.add_argument("--workload", choices=("channel", "sbli"), default="channel",
                     help="sbli: the shock / boundary-layer box (1536x320x192 unless --nx/--ny/--nz say otherwise), "
                          "FP64 and mixed, printed as its own JSON line")
