@@ -378,8 +378,9 @@ int st_server_set_batching(st_server *sv, int32_t lanes, int64_t max_cells);
  * batched chunks, [3] table layouts (recorded chunks; a lane whose solves
  * are unchanged reuses its last layout), [4] microseconds the scheduler
  * spent recording and laying out, [5] microseconds enqueuing batched
- * launches. */
-int st_server_batch_stats(st_server *sv, int64_t out[6]);
+ * launches, [6] batched chunks replayed from a HIP graph (a reused layout
+ * is captured once per table buffer). */
+int st_server_batch_stats(st_server *sv, int64_t out[7]);
 const char *st_server_last_error(void);
 
 #ifdef __cplusplus

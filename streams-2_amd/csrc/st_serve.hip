@@ -202,6 +202,10 @@ struct Lane {
     std::vector<unsigned char> chost;
     std::vector<BatchOp> cops;
     int64_t cver = 0, bver[2] = {-1, -1};  // layout version, and the version each device buffer holds
+    // the launches of a reused layout, captured once per table buffer
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int64_t gver[2] = {-1, -1};            // layout version each graph was captured from
 };
 
 struct Batcher {
@@ -212,6 +216,7 @@ struct Batcher {
     hipEvent_t join = nullptr;           // orders a joining solve's own-stream work before its lane
     std::atomic<int64_t> chunks{0}, launches{0}, solves{0};
     std::atomic<int64_t> layouts{0}, ns_layout{0}, ns_launch{0};  // host work of the scheduler
+    std::atomic<int64_t> graph_chunks{0};  // batched chunks replayed from a captured graph
 };
 
 constexpr size_t kBatchTabBytes = (size_t)32 << 20;  // per table buffer
@@ -824,9 +829,34 @@ void launch_batch(st_server *sv, int li, bool *progress) {
         ln.bver[b] = ln.cver;
     }
     char *d = ln.dtab[b];
-    for (const BatchOp &op : ln.cops)
-        if (op.fn(op.total, op.block, ln.stream, d + op.ent, d + op.pack, (const int *)(d + op.own)) != hipSuccess)
-            return batch_error(sv, ln, mem, "batched launch failed");
+    auto enqueue = [&]() -> bool {
+        for (const BatchOp &op : ln.cops)
+            if (op.fn(op.total, op.block, ln.stream, d + op.ent, d + op.pack, (const int *)(d + op.own)) != hipSuccess)
+                return false;
+        return true;
+    };
+    // a layout used a second time is captured (once per buffer) and
+    // replayed from then on
+    if (reuse && ln.gver[b] != ln.cver) {
+        if (ln.exec[b]) (void)hipGraphExecDestroy(ln.exec[b]);
+        if (ln.graph[b]) (void)hipGraphDestroy(ln.graph[b]);
+        ln.exec[b] = nullptr;
+        ln.graph[b] = nullptr;
+        ln.gver[b] = -1;
+        if (hipStreamBeginCapture(ln.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return batch_error(sv, ln, mem, "batched chunk: capture failed");
+        const bool ok = enqueue();
+        const bool ended = hipStreamEndCapture(ln.stream, &ln.graph[b]) == hipSuccess;
+        if (!ok || !ended || hipGraphInstantiate(&ln.exec[b], ln.graph[b], nullptr, nullptr, 0) != hipSuccess)
+            return batch_error(sv, ln, mem, "batched chunk: capture failed");
+        ln.gver[b] = ln.cver;
+    }
+    if (reuse) {
+        if (hipGraphLaunch(ln.exec[b], ln.stream) != hipSuccess) return batch_error(sv, ln, mem, "batched graph launch failed");
+        bt.graph_chunks++;
+    } else if (!enqueue()) {
+        return batch_error(sv, ln, mem, "batched launch failed");
+    }
     if (hipEventRecord(ln.ev[b], ln.stream) != hipSuccess)
         return batch_error(sv, ln, mem, "batched chunk: event failed");
     ln.members[b] = mem;
@@ -1006,6 +1036,10 @@ void destroy_slots(st_server *sv) {
             if (ln.ev[b]) (void)hipEventDestroy(ln.ev[b]);
             if (ln.dtab[b]) (void)hipFree(ln.dtab[b]);
             if (ln.htab[b]) (void)hipHostFree(ln.htab[b]);
+        }
+        for (int b = 0; b < 2; b++) {
+            if (ln.exec[b]) (void)hipGraphExecDestroy(ln.exec[b]);
+            if (ln.graph[b]) (void)hipGraphDestroy(ln.graph[b]);
         }
         if (ln.stream) (void)hipStreamDestroy(ln.stream);
     }
@@ -1192,7 +1226,7 @@ int st_server_set_batching(st_server *sv, int32_t lanes, int64_t max_cells) {
     return ST_OK;
 }
 
-int st_server_batch_stats(st_server *sv, int64_t out[6]) {
+int st_server_batch_stats(st_server *sv, int64_t out[7]) {
     if (!sv || !out) return sfail(ST_ERR_ARG, "null argument");
     out[0] = sv->bt.chunks;
     out[1] = sv->bt.launches;
@@ -1200,6 +1234,7 @@ int st_server_batch_stats(st_server *sv, int64_t out[6]) {
     out[3] = sv->bt.layouts;
     out[4] = sv->bt.ns_layout / 1000;
     out[5] = sv->bt.ns_launch / 1000;
+    out[6] = sv->bt.graph_chunks;
     return ST_OK;
 }
 

@@ -71,10 +71,10 @@ class Server:
         out = (C.c_int64 * 6)()
         capi.check_server(self.L.st_server_stats(self.h, out))
         st = dict(zip(("queued", "running", "completed", "failed", "hbm_bytes", "tuned_classes"), list(out)))
-        b = (C.c_int64 * 6)()
+        b = (C.c_int64 * 7)()
         capi.check_server(self.L.st_server_batch_stats(self.h, b))
         st.update(zip(("batched_chunks", "batched_launches", "batched_solves", "batch_layouts",
-                       "batch_layout_us", "batch_launch_us"), list(b)))
+                       "batch_layout_us", "batch_launch_us", "batched_graph_chunks"), list(b)))
         return st
 
     def close(self):

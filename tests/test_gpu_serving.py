@@ -190,6 +190,20 @@ def test_batched_with_online_tuning_matches_the_plain_path():
         assert np.array_equal(interior(a["w"]), interior(b["w"])), n
 
 
+def test_batched_chunks_replay_from_graphs():
+    """A lane whose solves stay the same reuses its layout and replays the
+    chunk from a captured HIP graph; results are unchanged."""
+    reqs = [Request(r.case, 48, 0.0) for r in _requests()[:3]]
+    seq = run_sequential(reqs, want_fields=True)
+    srv = run_serving(reqs, hbm_bytes=8 << 30, max_slots=3, steps_per_graph=4, autotune=False, want_fields=True,
+                      timeout=120.0, batch=1)
+    st = srv["stats"]
+    print("graph replays:", st)
+    assert st["batched_graph_chunks"] > 0
+    for n, (a, b) in enumerate(zip(seq["results"], srv["results"])):
+        assert np.array_equal(interior(a["w"]), interior(b["w"])), n
+
+
 def test_batched_diverging_request_fails_alone():
     from streams_mi355x.capi import ST_ERR_NAN
 
