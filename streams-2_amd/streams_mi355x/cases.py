@@ -139,6 +139,7 @@ def make_subsonic_channel(nx=80, ny=60, nz=80) -> ChannelCase:
                         jbgrid=case.jbgrid, dyptarget=case.dyptarget, nsmooy=case.ysmoosteps,
                         metrics_order=6)
     params = channel_params(case, grid.yn)
+    params.rand_type = 2   # the deck's [numerics] rand_type (tests.ini's patch leaves it)
     w0 = init_channel_field(grid, params, case.lz, splits=(2, 1, 2), rand_type=2)
     return ChannelCase(case, grid, params, w0)
 
