@@ -319,10 +319,12 @@ def serving_small(args, hbm):
     from streams_mi355x.serve import run_serving, serving_workload
 
     small = serving_workload(128, args.serve_steps, rate=64.0, seed=1, nx_range=(32, 64), nyz_range=(32, 64))
-    a = run_serving(small, hbm_bytes=hbm, max_slots=32, steps_per_graph=10, autotune=True, want_fields=True)
-    b = run_serving(small, hbm_bytes=hbm, max_slots=32, steps_per_graph=10, autotune=True, want_fields=True,
+    k = args.serve_steps_per_graph
+    a = run_serving(small, hbm_bytes=hbm, max_slots=32, steps_per_graph=k, autotune=True, want_fields=True)
+    b = run_serving(small, hbm_bytes=hbm, max_slots=32, steps_per_graph=k, autotune=True, want_fields=True,
                     batch=args.serve_batch_lanes)
     out = {"requests": len(small), "grids": "uniform in [32,64]^3", "arrival_rate_per_s": 64.0, "slots": 32,
+           "steps_per_graph": k,
            "unbatched": {"value": a["gcups"], "p50_s": a["p50_s"], "p99_s": a["p99_s"], "failed_requests": a["failed"]},
            "batched": _batch_summary(b), "lanes": args.serve_batch_lanes,
            "speedup_batched_vs_unbatched": b["gcups"] / a["gcups"],
@@ -352,11 +354,14 @@ def parse():
     ap.add_argument("--serve-hbm-gib", type=float, default=64.0)
     ap.add_argument("--serve-seq-sample", type=int, default=64)
     ap.add_argument("--serve-batch-lanes", type=int, default=2, help="batch lanes of the batched serving runs")
+    ap.add_argument("--serve-steps-per-graph", type=int, default=10,
+                    help="steps per captured graph of the small-grid stream (0: chunks of 10 steps, no graphs)")
     ap.add_argument("--serve-batch-max-cells", type=int, default=1 << 20,
                     help="solves of more cells keep their own streams in the batched run of the main stream")
-    ap.add_argument("--workload", choices=("channel", "sbli"), default="channel",
+    ap.add_argument("--workload", choices=("channel", "sbli", "serving-small"), default="channel",
                     help="sbli: the shock / boundary-layer box (1536x320x192 unless --nx/--ny/--nz say otherwise), "
-                         "FP64 and mixed, printed as its own JSON line")
+                         "FP64 and mixed, printed as its own JSON line; serving-small: only the small-grid "
+                         "serving stream, unbatched and batched, as its own JSON line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of 2 GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -452,6 +457,11 @@ def main():
         if (args.nx, args.ny, args.nz) == (512, 256, 256):
             args.nx, args.ny, args.nz = 1536, 320, 192
         print(json.dumps(sbli_workload(args)))
+        return
+    if args.workload == "serving-small":
+        out = serving_small(args, int(args.serve_hbm_gib * (1 << 30)))
+        out["workload"] = "serving-small"
+        print(json.dumps(out))
         return
     from streams_mi355x import make_channel
     from streams_mi355x.solver import ChannelSolver

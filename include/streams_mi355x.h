@@ -361,8 +361,10 @@ int st_server_wait(st_server *sv, int64_t ticket, double timeout_s, double *w_ou
 /* [0] queued, [1] running, [2] completed, [3] failed, [4] HBM bytes reserved,
  * [5] tuned shape classes. */
 int st_server_stats(st_server *sv, int64_t out[6]);
-/* Batched launches (call before the first submission; steps_per_graph even
- * and >= 2).  With lanes > 0 (at most 8) each running solve of at most
+/* Batched launches (call before the first submission; an even number of
+ * steps per chunk: steps_per_graph, or 10 with steps_per_graph 0, in which
+ * case no graph is captured and batched chunks are enqueued launch by
+ * launch).  With lanes > 0 (at most 8) each running solve of at most
  * max_cells interior cells (0: any size) is bound to one of `lanes` batch
  * streams (the one holding the fewest cells), and the chunks of a lane's
  * solves whose launch sequences agree (same precision, scheme and boundary

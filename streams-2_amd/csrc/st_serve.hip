@@ -278,6 +278,18 @@ bool graphable(const st_server *sv, const ServeReq *r) {
     return sv->K >= 2 && sv->K % 2 == 0 && sv->K % idt == 0;
 }
 
+// Steps per chunk: steps_per_graph, or kPlainChunk without graphs.
+int chunk_steps(const st_server *sv) { return sv->K > 0 ? sv->K : kPlainChunk; }
+
+// A batched chunk must return every solve to the same right-hand-side
+// buffer order and dt phase, for its layout to serve the next chunk too:
+// an even number of steps, a multiple of the dt period.
+bool batch_phase_ok(const st_server *sv, const ServeReq *r) {
+    const int idt = r->p.iter_dt_recompute > 0 ? r->p.iter_dt_recompute : 1;
+    const int n = chunk_steps(sv);
+    return n % 2 == 0 && n % idt == 0;
+}
+
 int capture(st_server *sv, Slot &sl) {
     st_solver *s = sl.s;
     const st::HostMark m = st::host_mark(s);
@@ -309,12 +321,12 @@ int setup(st_server *sv, Slot &sl) {
     sl.batchable = false;
     sl.sig.clear();
     const int64_t cells = (int64_t)r->p.nx * r->p.ny * r->p.nz;
-    if (sv->bt.on && graphable(sv, r) && (sv->bt.max_cells <= 0 || cells <= sv->bt.max_cells)) {
+    if (sv->bt.on && batch_phase_ok(sv, r) && (sv->bt.max_cells <= 0 || cells <= sv->bt.max_cells)) {
         const st::HostMark m = st::host_mark(sl.s);
         sl.rec.recs.clear();
         sl.rec.bad = false;
         sl.s->rec = &sl.rec;
-        rc = st_advance(sl.s, sv->K);
+        rc = st_advance(sl.s, chunk_steps(sv));
         sl.s->rec = nullptr;
         st::host_restore(sl.s, m);
         if (rc) return rc;
@@ -493,7 +505,7 @@ int launch_chunk(st_server *sv, Slot &sl) {
     st_solver *s = sl.s;
     ServeReq *r = sl.r;
     const int left = r->nsteps - sl.launched;
-    const int per = sv->K > 0 ? sv->K : kPlainChunk;
+    const int per = chunk_steps(sv);
     int n = left < per ? left : per;
     const int slot = (sl.head + sl.nev) % 2;
     int cand = -1;
@@ -612,7 +624,7 @@ bool class_done(st_server *sv, const Slot &sl, Tune *best = nullptr) {
 // Scheduler: a solve whose next chunk belongs in a batch (its class is
 // tuned and a whole chunk is left); it launches nothing on its own stream.
 bool batch_bound(st_server *sv, const Slot &sl) {
-    return sv->bt.on && sl.batchable && sl.r->nsteps - sl.launched >= sv->K && class_done(sv, sl);
+    return sv->bt.on && sl.batchable && sl.r->nsteps - sl.launched >= chunk_steps(sv) && class_done(sv, sl);
 }
 
 // Scheduler: retire the batched chunks whose events have fired.
@@ -709,7 +721,7 @@ bool record_chunk(st_server *sv, Lane &ln, std::vector<int> &mem) {
         sl.rec.recs.clear();
         sl.rec.bad = false;
         sl.s->rec = &sl.rec;
-        const int rc = st_advance(sl.s, sv->K);
+        const int rc = st_advance(sl.s, chunk_steps(sv));
         sl.s->rec = nullptr;
         bool same = rc == ST_OK && !sl.rec.bad && sl.rec.recs.size() == sl.sig.size();
         for (size_t i = 0; same && i < sl.sig.size(); i++) same = sl.rec.recs[i].fn == sl.sig[i];
@@ -806,7 +818,7 @@ void launch_batch(st_server *sv, int li, bool *progress) {
         // the same chunk again: advance each solve's host state as its
         // recording would
         for (size_t i = 0; i < mem.size(); i++) {
-            ln.cafter[i].icyc += sv->K;
+            ln.cafter[i].icyc += chunk_steps(sv);
             st::host_restore(sv->slots[mem[i]].s, ln.cafter[i]);
         }
     } else if (!record_chunk(sv, ln, mem)) {
@@ -835,9 +847,10 @@ void launch_batch(st_server *sv, int li, bool *progress) {
                 return false;
         return true;
     };
-    // a layout used a second time is captured (once per buffer) and
-    // replayed from then on
-    if (reuse && ln.gver[b] != ln.cver) {
+    // with graphs on, a layout used a second time is captured (once per
+    // buffer) and replayed from then on
+    const bool graphs = sv->K >= 2;
+    if (reuse && graphs && ln.gver[b] != ln.cver) {
         if (ln.exec[b]) (void)hipGraphExecDestroy(ln.exec[b]);
         if (ln.graph[b]) (void)hipGraphDestroy(ln.graph[b]);
         ln.exec[b] = nullptr;
@@ -851,7 +864,7 @@ void launch_batch(st_server *sv, int li, bool *progress) {
             return batch_error(sv, ln, mem, "batched chunk: capture failed");
         ln.gver[b] = ln.cver;
     }
-    if (reuse) {
+    if (reuse && graphs) {
         if (hipGraphLaunch(ln.exec[b], ln.stream) != hipSuccess) return batch_error(sv, ln, mem, "batched graph launch failed");
         bt.graph_chunks++;
     } else if (!enqueue()) {
@@ -869,7 +882,7 @@ void launch_batch(st_server *sv, int li, bool *progress) {
     for (int q : mem) {
         Slot &sl = sv->slots[q];
         sl.rec.recs.clear();
-        sl.launched += sv->K;
+        sl.launched += chunk_steps(sv);
         sl.inbatch++;
         ServeReq *r = sl.r;
         // a checkpoint after the chunk that reaches the period (or the last
@@ -1122,7 +1135,7 @@ int st_server_submit2(st_server *sv, const st_params *p, const double *x, const 
     if (checkpoint_every > 0 && (!checkpoint_dir || !*checkpoint_dir))
         return sfail(ST_ERR_ARG, "a checkpoint period needs a checkpoint directory");
     // checkpoints fall on chunk boundaries: K steps, or kPlainChunk without graphs
-    const int per = sv->K > 0 ? sv->K : kPlainChunk;
+    const int per = chunk_steps(sv);
     if (checkpoint_every > 0 && checkpoint_every % per != 0)
         return sfail(ST_ERR_ARG, "the checkpoint period must be a multiple of the steps per chunk");
     int rc = st::validate_params(p);
@@ -1197,8 +1210,8 @@ int st_server_set_batching(st_server *sv, int32_t lanes, int64_t max_cells) {
     if (!sv) return sfail(ST_ERR_ARG, "null server");
     std::lock_guard<std::mutex> lk(sv->mu);
     if (sv->next_id != 1) return sfail(ST_ERR_STATE, "batching is set before the first submission");
-    if (sv->K < 2 || sv->K % 2 != 0)
-        return sfail(ST_ERR_ARG, "batched chunks need an even steps_per_graph of at least 2");
+    if (chunk_steps(sv) % 2 != 0)
+        return sfail(ST_ERR_ARG, "batched chunks need an even number of steps (steps_per_graph, or 10 without graphs)");
     Batcher &bt = sv->bt;
     if (lanes < 0 || lanes > kMaxLanes || max_cells < 0) return sfail(ST_ERR_ARG, "batch lanes out of range");
     bt.max_cells = max_cells;

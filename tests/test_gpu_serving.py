@@ -204,6 +204,19 @@ def test_batched_chunks_replay_from_graphs():
         assert np.array_equal(interior(a["w"]), interior(b["w"])), n
 
 
+def test_batched_without_graphs():
+    """steps_per_graph 0: batched chunks of 10 steps enqueued launch by
+    launch, nothing captured."""
+    reqs = _requests()[:4]
+    seq = run_sequential(reqs, want_fields=True)
+    srv = run_serving(reqs, hbm_bytes=8 << 30, max_slots=4, steps_per_graph=0, autotune=False, want_fields=True,
+                      timeout=120.0, batch=1)
+    st = srv["stats"]
+    assert st["batched_chunks"] > 0 and st["batched_graph_chunks"] == 0
+    for n, (a, b) in enumerate(zip(seq["results"], srv["results"])):
+        assert np.array_equal(interior(a["w"]), interior(b["w"])), n
+
+
 def test_batched_diverging_request_fails_alone():
     from streams_mi355x.capi import ST_ERR_NAN
 
