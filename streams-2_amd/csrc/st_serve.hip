@@ -167,7 +167,8 @@ struct Slot {
     std::vector<st::BatchLaunchFn> sig;      // the launch sequence of one chunk (grouping key)
     st::Recorder rec;                        // the chunk being batched
     int inbatch = 0;                         // batched chunks in flight that hold this solve
-    int lane = -1;                           // its batch lane
+    int lane = -1;                           // its batch lane (scheduler only)
+    double lane_cells = 0.0;                 // what it added to its lane's cells
     bool own_work = true;                    // work on its own stream since it last joined a batch
     bool shape_set = false;                  // the tuned launch shape of its class is applied
 };
@@ -186,6 +187,7 @@ struct BatchOp {
 // upload).  Every batched solve is bound to one lane while it runs.
 struct Lane {
     hipStream_t stream = nullptr;
+    double cells = 0.0;                  // interior cells of the solves bound to the lane
     char *dtab[2] = {nullptr, nullptr};
     unsigned char *htab[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -441,7 +443,6 @@ void finish(st_server *sv, Slot &sl) {
     sl.sig.clear();
     sl.rec.recs.clear();
     sl.inbatch = 0;
-    sl.lane = -1;
     sl.own_work = true;
     sl.shape_set = false;
     sl.state = kFree;
@@ -768,15 +769,25 @@ bool record_chunk(st_server *sv, Lane &ln, std::vector<int> &mem) {
 }
 
 // Scheduler: bind a solve that is about to join a batch to the lane that
-// holds the fewest cells.
+// holds the fewest cells (the lanes keep their own counts: the scheduler
+// never reads another slot's solver, which the worker may be creating or
+// destroying).
 void bind_lane(st_server *sv, Slot &sl) {
-    std::vector<double> cells(sv->bt.lanes.size(), 0.0);
-    for (const Slot &o : sv->slots)
-        if (o.lane >= 0 && o.s) cells[o.lane] += (double)o.s->g.nint;
+    std::vector<Lane> &lanes = sv->bt.lanes;
     int best = 0;
-    for (int l = 1; l < (int)cells.size(); l++)
-        if (cells[l] < cells[best]) best = l;
+    for (int l = 1; l < (int)lanes.size(); l++)
+        if (lanes[l].cells < lanes[best].cells) best = l;
     sl.lane = best;
+    sl.lane_cells = (double)sl.s->g.nint;
+    lanes[best].cells += sl.lane_cells;
+}
+
+// Scheduler: a solve that leaves (hand-back) gives its cells back to its lane.
+void unbind_lane(st_server *sv, Slot &sl) {
+    if (sl.lane < 0) return;
+    sv->bt.lanes[sl.lane].cells -= sl.lane_cells;
+    sl.lane = -1;
+    sl.lane_cells = 0.0;
 }
 
 // Scheduler: the next batched chunk of lane `li`.  Its solves are the
@@ -993,6 +1004,7 @@ void serve_loop(st_server *sv) {
             // a solve held by a batched chunk in flight stays until the chunk has passed
             if (sl.inbatch == 0 && (rc != ST_OK || (sl.nev == 0 && sl.launched >= sl.r->nsteps && !ckpt_busy))) {
                 if (rc != ST_OK) drop_candidates(sv, sl);
+                unbind_lane(sv, sl);
                 std::lock_guard<std::mutex> lk(sv->mu);
                 sl.rc = rc;
                 sl.state = kDraining;
