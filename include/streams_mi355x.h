@@ -366,13 +366,14 @@ int st_server_stats(st_server *sv, int64_t out[6]);
  * case no graph is captured and batched chunks are enqueued launch by
  * launch).  With lanes > 0 (at most 8) each running solve of at most
  * max_cells interior cells (0: any size) is bound to one of `lanes` batch
- * streams (the one holding the fewest cells), and the chunks of a lane's
- * solves whose launch sequences agree (same precision, scheme and boundary
- * set, any grid) run as one sequence of launches, each launch a single grid
- * over a table of the solves: one launch per kernel per stage and lane
- * instead of one per solve.  Larger solves, solves whose steps have no
- * batched form, the chunks that tune a class's launch shape and a final
- * chunk shorter than steps_per_graph run on their own streams as without
+ * streams (the one holding the fewest cells).  While a lane holds two
+ * solves or more, the chunks of its solves whose launch sequences agree
+ * (same precision, scheme and boundary set, any grid) run as one sequence
+ * of launches, each launch a single grid over a table of the solves: one
+ * launch per kernel per stage and lane instead of one per solve.  A solve
+ * alone in its lane, larger solves, solves whose steps have no batched
+ * form, the chunks that tune a class's launch shape and a final chunk
+ * shorter than steps_per_graph run on their own streams as without
  * batching.  lanes = 0 turns batching off.  Results are unchanged, bit for
  * bit. */
 int st_server_set_batching(st_server *sv, int32_t lanes, int64_t max_cells);

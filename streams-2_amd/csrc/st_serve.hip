@@ -188,6 +188,7 @@ struct BatchOp {
 struct Lane {
     hipStream_t stream = nullptr;
     double cells = 0.0;                  // interior cells of the solves bound to the lane
+    int nsolves = 0;                     // solves bound to the lane
     char *dtab[2] = {nullptr, nullptr};
     unsigned char *htab[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -622,10 +623,13 @@ bool class_done(st_server *sv, const Slot &sl, Tune *best = nullptr) {
     return true;
 }
 
-// Scheduler: a solve whose next chunk belongs in a batch (its class is
-// tuned and a whole chunk is left); it launches nothing on its own stream.
+// Scheduler: a solve whose next chunk belongs in a batch (it shares its
+// lane with another solve, its class is tuned and a whole chunk is left);
+// it launches nothing on its own stream.  A solve alone in its lane runs on
+// its own stream, as without batching.
 bool batch_bound(st_server *sv, const Slot &sl) {
-    return sv->bt.on && sl.batchable && sl.r->nsteps - sl.launched >= chunk_steps(sv) && class_done(sv, sl);
+    return sv->bt.on && sl.batchable && sl.lane >= 0 && sv->bt.lanes[sl.lane].nsolves >= 2 &&
+           sl.r->nsteps - sl.launched >= chunk_steps(sv) && class_done(sv, sl);
 }
 
 // Scheduler: retire the batched chunks whose events have fired.
@@ -780,12 +784,14 @@ void bind_lane(st_server *sv, Slot &sl) {
     sl.lane = best;
     sl.lane_cells = (double)sl.s->g.nint;
     lanes[best].cells += sl.lane_cells;
+    lanes[best].nsolves++;
 }
 
 // Scheduler: a solve that leaves (hand-back) gives its cells back to its lane.
 void unbind_lane(st_server *sv, Slot &sl) {
     if (sl.lane < 0) return;
     sv->bt.lanes[sl.lane].cells -= sl.lane_cells;
+    sv->bt.lanes[sl.lane].nsolves--;
     sl.lane = -1;
     sl.lane_cells = 0.0;
 }
@@ -810,9 +816,7 @@ void launch_batch(st_server *sv, int li, bool *progress) {
             std::lock_guard<std::mutex> lk(sv->mu);
             ok = sl.state == kRunning && sl.s && sl.rc == ST_OK && !sl.ckpt_pending;
         }
-        if (!ok || sl.nev > 0 || !batch_bound(sv, sl)) continue;
-        if (sl.lane < 0) bind_lane(sv, sl);
-        if (sl.lane != li) continue;
+        if (!ok || sl.nev > 0 || sl.lane != li || !batch_bound(sv, sl)) continue;
         ready.push_back(q);
         if (lead < 0 || sl.r->id < sv->slots[lead].r->id) lead = q;
     }
@@ -967,6 +971,8 @@ void serve_loop(st_server *sv) {
                 continue;
             }
             active = true;
+            // a batchable solve is bound to a lane once it runs
+            if (sv->bt.on && sl.batchable && sl.lane < 0 && sl.s) bind_lane(sv, sl);
             retire(sv, sl, &progress);
             // a landed checkpoint copy goes to the worker
             if (ckpt_pending && sl.s && hipEventQuery(sl.ckpt_ev) == hipSuccess) {

@@ -3,7 +3,9 @@
 * Channel, 512 x 256 x 256, Re_tau 550, bulk Mach 1.5.
 * Supersonic boundary layer with recycling-rescaling inflow, 1024 x 256 x 192.
 * Shock / boundary-layer interaction, 1536 x 320 x 192, 8 deg wedge shock.
-* Serving stream: 64 seeded channel solves, 64^3 to 256 x 128 x 128, 200 steps each.
+* Serving stream: 64 seeded channel solves, 64^3 to 256 x 128 x 128, 200 steps each,
+  served plain and with batched launches; and 128 small solves (32-64 cells a
+  side) served with batched launches.
 
 The oracle cannot run these sizes in a test, so each workload is checked
 in two parts:
@@ -212,6 +214,15 @@ def test_serving_stream_64_requests():
             ok.append(n)
     print(f"served {len(ok)} of {len(reqs)} requests; p50 {srv['p50_s']:.2f} s, p99 {srv['p99_s']:.2f} s")
     assert len(ok) >= 56
+    # the same stream with batched launches (2 lanes, solves up to 2^20 cells batched)
+    srv_b = run_serving(reqs, hbm_bytes=64 << 30, max_slots=8, steps_per_graph=10, autotune=True, want_fields=True,
+                        batch=2, batch_max_cells=1 << 20)
+    assert srv_b["stats"]["batched_chunks"] > 0
+    for n, (a, b) in enumerate(zip(seq["results"], srv_b["results"])):
+        assert (a["error"] is None) == (b["error"] is None), n
+        if a["error"] is None:
+            assert np.array_equal(interior(a["w"]), interior(b["w"])), n
+    del srv_b
     by_size = sorted(ok, key=lambda n: reqs[n].case.params.nx * reqs[n].case.params.ny * reqs[n].case.params.nz)
     for n in by_size[:2]:
         r = reqs[n]
@@ -237,3 +248,31 @@ def test_serving_stream_64_requests():
           f"p90 {np.percentile(errs, 90):.2e}, max {errs.max():.2e}, over {len(errs)} requests")
     assert len(errs) >= len(ok) - 2
     assert np.median(errs) <= 1e-4
+
+
+def test_small_grid_serving_stream_batched():
+    """128 seeded channel solves of 32-64 cells a side (200 steps each),
+    served with batched launches (2 lanes): every request equals its plain
+    FP64 solve bit for bit, and the two smallest match the oracle."""
+    from streams_mi355x.serve import run_sequential, run_serving, serving_workload
+
+    reqs = serving_workload(128, 200, rate=64.0, seed=1, nx_range=(32, 64), nyz_range=(32, 64))
+    srv = run_serving(reqs, hbm_bytes=16 << 30, max_slots=32, steps_per_graph=10, autotune=True, want_fields=True,
+                      batch=2)
+    st = srv["stats"]
+    print(f"small-grid stream batched: {srv['gcups']:.3f} Gcell-updates/s, {st['batched_chunks']} batched chunks, "
+          f"{st['batched_solves'] / max(1, st['batched_chunks']):.1f} solves per chunk, p99 {srv['p99_s']:.2f} s")
+    assert st["batched_chunks"] > 0 and srv["failed"] == 0
+    seq = run_sequential(reqs, want_fields=True)
+    for n, (a, b) in enumerate(zip(seq["results"], srv["results"])):
+        assert a["error"] is None and np.array_equal(interior(a["w"]), interior(b["w"])), n
+    by_size = sorted(range(len(reqs)), key=lambda n: reqs[n].case.params.nx * reqs[n].case.params.ny *
+                     reqs[n].case.params.nz)
+    for n in by_size[:2]:
+        r = reqs[n]
+        o = Oracle(r.case.params, r.case.grid)
+        o.w[:] = r.case.w0
+        o.init_run()
+        o.advance(r.nsteps)
+        errs = [rel_l2(interior(srv["results"][n]["w"])[m], interior(o.w)[m]) for m in range(5)]
+        assert max(errs) <= TRAJ_TOL, (n, errs)
