@@ -52,7 +52,10 @@ extern "C" {
 /* Scalar parameters of the path: the fields of equation_singleideal_object
  * that the hot loop reads (singleideal.F90:58-125) plus the grid sizes. */
 typedef struct st_params {
-    int32_t nx, ny, nz, ng;          /* interior cells per direction, ghost width (3)     */
+    int32_t nx, ny, nz, ng;          /* interior cells per direction, ghost width: 3, or
+                                        4 for weno_scheme 4 / ep_order 8 (FP64, periodic
+                                        faces and staggered walls, stage run procedure by
+                                        procedure)                                          */
     int32_t ep_order;                /* central convective order (2,4,6,8)                */
     int32_t visc_order;              /* viscous order (== ep_order)                       */
     int32_t nkeep;                   /* KEEP(n): 0 KEEP, >0 KEEP(n), <0 non-KEEP split    */

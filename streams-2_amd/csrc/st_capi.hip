@@ -60,14 +60,19 @@ void deriv_coeffs(int order1, int order2, double d1[4], double d2[5]) {
 int validate(const st_params *p) {
     if (!p) return fail(ST_ERR_ARG, "null params");
     if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(ST_ERR_ARG, "grid sizes must be positive");
-    if (p->ng != 3) return fail(ST_ERR_ARG, "ng must be 3 (stencil half width of the gfx950 kernels)");
+    // three ghost layers run the fused kernels; four (WENO7, ep_order 8)
+    // run the stage procedure by procedure on the per-face kernels
+    if (p->ng != 3 && p->ng != 4) return fail(ST_ERR_ARG, "ng must be 3, or 4 for weno_scheme 4 / ep_order 8");
     if (p->nx < p->ng || p->ny < p->ng || p->nz < p->ng)
         return fail(ST_ERR_ARG, "each grid size must be >= ng (singleideal.F90:578)");
-    if (p->ep_order < 2 || p->ep_order > 6 || (p->ep_order & 1))
-        return fail(ST_ERR_ARG, "ep_order must be 2, 4 or 6 with ng = 3");
+    if (p->ep_order < 2 || p->ep_order > 2 * p->ng || (p->ep_order & 1))
+        return fail(ST_ERR_ARG, "ep_order must be even and at most 2 ng (6 with ng = 3, 8 with ng = 4)");
     if (p->visc_order != p->ep_order) return fail(ST_ERR_ARG, "viscous order must be equal to ep_order");
-    if (p->weno_scheme < 1 || p->weno_scheme > 3) return fail(ST_ERR_ARG, "weno_scheme must be 1..3 with ng = 3");
+    if (p->weno_scheme < 1 || p->weno_scheme > p->ng)
+        return fail(ST_ERR_ARG, "weno_scheme must be 1..ng (3 with ng = 3, 4 with ng = 4)");
     if (p->weno_version != 0 && p->weno_version != 1) return fail(ST_ERR_ARG, "weno_version must be 0 or 1");
+    if (p->weno_version == 1 && p->weno_scheme != 3)
+        return fail(ST_ERR_ARG, "weno_version 1 supported only for weno_scheme = 3");
     if (p->flux_splitting != 0 && p->flux_splitting != 1)
         return fail(ST_ERR_ARG, "flux_splitting must be 0 (Roe) or 1 (Rusanov)");
     if (p->conservative_viscous != 0 && p->conservative_viscous != 1)
@@ -81,6 +86,12 @@ int validate(const st_params *p) {
         return fail(ST_ERR_ARG, "calorically_perfect = 1 needs indx_cp_l <= 0 <= indx_cp_r + 1");
     if (p->precision != ST_PREC_FP64 && p->precision != ST_PREC_MIXED)
         return fail(ST_ERR_ARG, "precision must be ST_PREC_FP64 or ST_PREC_MIXED");
+    if (p->ng == 4) {
+        if (p->precision != ST_PREC_FP64) return fail(ST_ERR_ARG, "four ghost layers run in FP64 only");
+        for (int l = 0; l < 6; l++)
+            if ((p->bctags[l] != 0 && !(p->bctags[l] == 6 && p->is_y_staggered)) || p->bctags_nr[l] != 0)
+                return fail(ST_ERR_ARG, "four ghost layers take periodic faces and staggered isothermal walls only");
+    }
     bool any_nr = false;
     for (int l = 0; l < 6; l++) {
         const int t = p->bctags[l];
@@ -850,7 +861,7 @@ int ensure_piece_buffers(st_solver *s, bool fhat, bool fln) {
 // fused sweep instead (fluxes and divergence in registers, WENO fix-up).
 int sweep_range(st_solver *s, int dir, int f0, int f1, bool update, int c0, int c1, bool fused) {
     const int n = dir == 0 ? s->g.nx : (dir == 1 ? s->g.ny : s->g.nz);
-    if (fused) {
+    if (fused && s->g.ng == 3) {
         if (dir == 0) HIPCHK(st::launch_euler_x(s, s->F[s->cur], false));
         else if (dir == 1) HIPCHK(st::launch_euler_y(s, s->F[s->cur]));
         else HIPCHK(st::launch_euler_z(s, s->F[s->cur]));
@@ -1017,9 +1028,58 @@ int st_compute_dt(st_solver *s, double *dt_out) {
     return ST_OK;
 }
 
+// rk_sync procedure by procedure (singleideal_cpu.F90:352-406) with the
+// step's dt: the stage of four ghost layers (WENO7, ep_order 8), whose
+// stencils the fused kernels do not cover.  fl is F[cur], with fln and fhat
+// as in the reference.
+static int rk_sync_pieces(st_solver *s, double dt) {
+    const st::KPar &k = s->kp;
+    const bool cons = s->p.conservative_viscous == 1;
+    int rc;
+#define ST_TRY(x)                     \
+    do {                              \
+        if ((rc = (x)) != ST_OK) return rc; \
+    } while (0)
+    for (int istep = 0; istep < 3; istep++) {
+        ST_TRY(st_init_flux(s, s->rhork[istep] * dt));
+        ST_TRY(st_bcswap(s));
+        ST_TRY(st_compute_aux(s, 1, 1));
+        ST_TRY(sweep_range(s, 0, k.eul_lo[0] - 1, k.eul_hi[0], true, k.eul_lo[0], k.eul_hi[0], false));
+        if (cons) {
+            ST_TRY(st_visflx(s, 6));
+            ST_TRY(st_visflx(s, 5));
+        } else {
+            ST_TRY(st_visflx(s, 4));
+        }
+        ST_TRY(st_bcextr_var(s, 10));
+        ST_TRY(st_bcswap_var(s, 10));
+        ST_TRY(sweep_range(s, 1, k.eul_lo[1] - 1, k.eul_hi[1], true, k.eul_lo[1], k.eul_hi[1], false));
+        ST_TRY(sweep_range(s, 2, k.eul_lo[2] - 1, k.eul_hi[2], true, k.eul_lo[2], k.eul_hi[2], false));
+        if (istep == 2) {
+            ST_TRY(st_visflx(s, 3));
+            ST_TRY(st_bcswap_var(s, 8));
+        }
+        ST_TRY(st_visflx(s, cons ? 7 : 1));
+        ST_TRY(st_bc_nr(s));
+        ST_TRY(st_update_flux(s, s->gamrk[istep] * dt));
+        if (s->p.channel_case) ST_TRY(st_force_rhs(s));
+        ST_TRY(st_update_field(s));
+        if (s->p.channel_case) ST_TRY(st_force_var(s));
+        ST_TRY(st_update_ghost(s, 0));
+    }
+#undef ST_TRY
+    s->last_stage = 2;
+    return ST_OK;
+}
+
 int st_rk_sync(st_solver *s) {
     NEED(s);
     if (s->recyc && !s->recyc_set) return fail(ST_ERR_STATE, "st_set_recycling not called");
+    if (s->g.ng == 4) {
+        int rc = pull_scalars(s);
+        if (rc) return rc;
+        return rk_sync_pieces(s, s->hs->dt);
+    }
     return do_rk_sync(s);
 }
 
@@ -1029,6 +1089,19 @@ int st_advance(st_solver *s, int32_t nsteps) {
     const int iter_dt = s->p.iter_dt_recompute > 0 ? s->p.iter_dt_recompute : 1;
     for (int n = 0; n < nsteps; n++) {
         s->icyc += 1;
+        if (s->g.ng == 4) {
+            // procedure by procedure: dt at the step start, the stages, the
+            // step's time and counter (singleideal_cpu.F90:1599-1645)
+            double dt;
+            int rc;
+            if (s->p.cfl > 0 && ((s->icyc - s->icyc0) % iter_dt) == 0) rc = st_compute_dt(s, &dt);
+            else if ((rc = pull_scalars(s)) == ST_OK) dt = s->hs->dt;
+            if (rc) return rc;
+            if (s->p.channel_case && (rc = st_reset_forcing(s))) return rc;
+            if ((rc = rk_sync_pieces(s, dt))) return rc;
+            HIPCHK(st::launch_step_end(s));
+            continue;
+        }
         if (s->p.cfl > 0 && ((s->icyc - s->icyc0) % iter_dt) == 0) {
             // compute_aux(central) + compute_dt of the step start form the
             // same primitives of the same w as the first stage's aux, which
@@ -1045,6 +1118,7 @@ int st_advance(st_solver *s, int32_t nsteps) {
 int st_kernel_times(st_solver *s, int32_t nsteps, double ms[ST_NKFAM], int32_t count[ST_NKFAM]) {
     NEED(s);
     if (!ms || !count || nsteps < 1) return fail(ST_ERR_ARG, "bad arguments");
+    if (s->g.ng == 4) return fail(ST_ERR_STATE, "kernel times cover the fused stage of three ghost layers");
     KTimer *t = new (std::nothrow) KTimer();
     if (!t) return fail(ST_ERR_NOMEM, "timer allocation failed");
     t->n = 0;

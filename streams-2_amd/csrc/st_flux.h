@@ -102,6 +102,9 @@ __device__ __forceinline__ void central_flux(const KPar &P, const PT (&q)[2 * H]
     // argument block would make every read a vector load
     auto cd_of = [&](int l) {
         const double v3 = P.cd1tab[2][l - 1], v2 = P.cd1tab[1][l - 1], v1 = P.cd1tab[0][l - 1];
+        if constexpr (H >= 4) {
+            if (lmax >= 4) return P.cd1tab[3][l - 1];  // ep_order 8
+        }
         return lmax >= 3 ? v3 : (lmax == 2 ? v2 : v1);
     };
     const int nkeep = KM == 0 ? 0 : (KM < 0 ? -1 : P.nkeep);
@@ -388,17 +391,77 @@ __device__ __forceinline__ double wenorec3(const double vp[6], const double vm[6
 #undef VM
 }
 
-// Characteristic WENO flux at the face of a 6-point window (H = 3).  The
-// eigenvalue bounds run over the weno_size = 2*weno_scheme points of the
-// reference's stencil, window points 3-ws .. 2+ws.
-template <int DIR>
-__device__ __forceinline__ void weno_flux(const KPar &P, const Pt (&q)[6], int eoc, double fh[5]) {
-    const Roe s = roe_average(P, q[2], q[3]);
+// wenorec_1d for one characteristic field with iweno = 4 (window of 8
+// values, l = 1..8 of the reference, i = 4 <-> index 3): orders 1-3 as
+// wenorec3 on the middle six values; order 4 is WENO7 (Jiang-Shu weights
+// (1, 12, 18, 4)/35 and the Balsara-Shu smoothness indicators,
+// kernels_cpu.F90:2668-2724; the reference takes WENO-Z for order 3 only).
+__device__ __forceinline__ double wenorec4(const double vp[8], const double vm[8], int ord, int version,
+                                           double betascale) {
+    if (ord <= 3) return wenorec3(vp + 1, vm + 1, ord, version, betascale);
+#define A(o) vp[3 + (o)]
+#define B(o) vm[3 + (o)]
+    const double d1 = 1.0 / 36.0, d2 = 13.0 / 12.0, d3 = 781.0 / 720.0;
+    const double dw0 = 1.0 / 35.0, dw1 = 12.0 / 35.0, dw2 = 18.0 / 35.0, dw3 = 4.0 / 35.0;
+    double bp3 = d1 * sqr(-11.0 * A(0) + 18.0 * A(1) - 9.0 * A(2) + 2.0 * A(3)) +
+                 d2 * sqr(2.0 * A(0) - 5.0 * A(1) + 4.0 * A(2) - A(3)) + d3 * sqr(-A(0) + 3.0 * A(1) - 3.0 * A(2) + A(3));
+    double bp2 = d1 * sqr(-2.0 * A(-1) - 3.0 * A(0) + 6.0 * A(1) - A(2)) + d2 * sqr(A(-1) - 2.0 * A(0) + A(1)) +
+                 d3 * sqr(-A(-1) + 3.0 * A(0) - 3.0 * A(1) + A(2));
+    double bp1 = d1 * sqr(A(-2) - 6.0 * A(-1) + 3.0 * A(0) + 2.0 * A(1)) + d2 * sqr(A(-1) - 2.0 * A(0) + A(1)) +
+                 d3 * sqr(-A(-2) + 3.0 * A(-1) - 3.0 * A(0) + A(1));
+    double bp0 = d1 * sqr(-2.0 * A(-3) + 9.0 * A(-2) - 18.0 * A(-1) + 11.0 * A(0)) +
+                 d2 * sqr(-A(-3) + 4.0 * A(-2) - 5.0 * A(-1) + 2.0 * A(0)) +
+                 d3 * sqr(-A(-3) + 3.0 * A(-2) - 3.0 * A(-1) + A(0));
+    double bm3 = d1 * sqr(-11.0 * B(1) + 18.0 * B(0) - 9.0 * B(-1) + 2.0 * B(-2)) +
+                 d2 * sqr(2.0 * B(1) - 5.0 * B(0) + 4.0 * B(-1) - B(-2)) + d3 * sqr(-B(1) + 3.0 * B(0) - 3.0 * B(-1) + B(-2));
+    double bm2 = d1 * sqr(-2.0 * B(2) - 3.0 * B(1) + 6.0 * B(0) - B(-1)) + d2 * sqr(B(2) - 2.0 * B(1) + B(0)) +
+                 d3 * sqr(-B(2) + 3.0 * B(1) - 3.0 * B(0) + B(-1));
+    double bm1 = d1 * sqr(B(3) - 6.0 * B(2) + 3.0 * B(1) + 2.0 * B(0)) + d2 * sqr(B(2) - 2.0 * B(1) + B(0)) +
+                 d3 * sqr(-B(3) + 3.0 * B(2) - 3.0 * B(1) + B(0));
+    double bm0 = d1 * sqr(-2.0 * B(4) + 9.0 * B(3) - 18.0 * B(2) + 11.0 * B(1)) +
+                 d2 * sqr(-B(4) + 4.0 * B(3) - 5.0 * B(2) + 2.0 * B(1)) + d3 * sqr(-B(4) + 3.0 * B(3) - 3.0 * B(2) + B(1));
+    bp3 = betascale * bp3; bp2 = betascale * bp2; bp1 = betascale * bp1; bp0 = betascale * bp0;
+    bm3 = betascale * bm3; bm2 = betascale * bm2; bm1 = betascale * bm1; bm0 = betascale * bm0;
+    bp0 = dw0 / sqr(0.000001 + bp0); bm0 = dw0 / sqr(0.000001 + bm0);
+    bp1 = dw1 / sqr(0.000001 + bp1); bm1 = dw1 / sqr(0.000001 + bm1);
+    bp2 = dw2 / sqr(0.000001 + bp2); bm2 = dw2 / sqr(0.000001 + bm2);
+    bp3 = dw3 / sqr(0.000001 + bp3); bm3 = dw3 / sqr(0.000001 + bm3);
+    const double sump = bp0 + bp1 + bp2 + bp3, summ = bm0 + bm1 + bm2 + bm3;
+    bp0 = bp0 / sump; bp1 = bp1 / sump; bp2 = bp2 / sump; bp3 = bp3 / sump;
+    bm0 = bm0 / summ; bm1 = bm1 / summ; bm2 = bm2 / summ; bm3 = bm3 / summ;
+    const double vminus = bp3 * (6.0 * A(0) + 26.0 * A(1) - 10.0 * A(2) + 2.0 * A(3)) +
+                          bp2 * (-2.0 * A(-1) + 14.0 * A(0) + 14.0 * A(1) - 2.0 * A(2)) +
+                          bp1 * (2.0 * A(-2) - 10.0 * A(-1) + 26.0 * A(0) + 6.0 * A(1)) +
+                          bp0 * (-6.0 * A(-3) + 26.0 * A(-2) - 46.0 * A(-1) + 50.0 * A(0));
+    const double vplus = bm3 * (6.0 * B(1) + 26.0 * B(0) - 10.0 * B(-1) + 2.0 * B(-2)) +
+                         bm2 * (-2.0 * B(2) + 14.0 * B(1) + 14.0 * B(0) - 2.0 * B(-1)) +
+                         bm1 * (2.0 * B(3) - 10.0 * B(2) + 26.0 * B(1) + 6.0 * B(0)) +
+                         bm0 * (-6.0 * B(4) + 26.0 * B(3) - 46.0 * B(2) + 50.0 * B(1));
+    return (vminus + vplus) / 24.0;
+#undef A
+#undef B
+}
+
+// reconstruction of a window of 2W values (W = 3: WENO up to order 3; W = 4: up to order 4)
+template <int W>
+__device__ __forceinline__ double wenorec_w(const double (&vp)[2 * W], const double (&vm)[2 * W], int ord,
+                                            int version, double betascale) {
+    if constexpr (W == 3) return wenorec3(vp, vm, ord, version, betascale);
+    else return wenorec4(vp, vm, ord, version, betascale);
+}
+
+// Characteristic WENO flux at the face of a 2W-point window (the face lies
+// between window points W-1 and W).  The eigenvalue bounds run over the
+// weno_size = 2*weno_scheme points of the reference's stencil, window
+// points W-ws .. W-1+ws.
+template <int DIR, int W = 3>
+__device__ __forceinline__ void weno_flux(const KPar &P, const Pt (&q)[2 * W], int eoc, double fh[5]) {
+    const Roe s = roe_average(P, q[W - 1], q[W]);
     double evmax[3] = {-1.0, -1.0, -1.0};  // |un-c|, |un|, |un+c|
     const int ws = P.weno_scheme;
 #pragma unroll
-    for (int l = 0; l < 6; l++) {
-        if (l < 3 - ws || l > 2 + ws) continue;
+    for (int l = 0; l < 2 * W; l++) {
+        if (l < W - ws || l > W - 1 + ws) continue;
         const double un = un_of<DIR>(q[l]);
         const double tt = q[l].t;
         const double gamloc = gamloc_of(P, tt);
@@ -421,9 +484,9 @@ __device__ __forceinline__ void weno_flux(const KPar &P, const Pt (&q)[6], int e
         double el[5];
         eig_left<DIR>(s, m, el);
         const double ev = m == 0 ? evmax[0] : (m == 4 ? evmax[2] : evmax[1]);
-        double gp[6], gm[6];
+        double gp[2 * W], gm[2 * W];
 #pragma unroll
-        for (int l = 0; l < 6; l++) {
+        for (int l = 0; l < 2 * W; l++) {
             const Pt &a = q[l];
             const double rhoun = a.r * un_of<DIR>(a);
             const double pp = a.r * a.t * P.rgas0;
@@ -447,7 +510,7 @@ __device__ __forceinline__ void weno_flux(const KPar &P, const Pt (&q)[6], int e
             gm[l] = gc - c;
         }
         const double bs = (m == 2 || m == 3) ? 1.0 / r2u4 : 1.0 / r2u2;
-        const double fi = wenorec3(gp, gm, ord, P.weno_version, bs);
+        const double fi = wenorec_w<W>(gp, gm, ord, P.weno_version, bs);
         double er[5];
         eig_right<DIR>(s, m, er);
 #pragma unroll
@@ -463,13 +526,13 @@ __device__ __forceinline__ void weno_flux(const KPar &P, const Pt (&q)[6], int e
 // flux as 1/2 (f +- evmax u), and wenorec_1d_rusanov (:2325-2523)
 // reconstructs it with the component scales 1/(rho0^2 u0^2), 1/(rho0^2
 // u0^4) for the momenta and 1/(rho0^2 u0^6) for the energy.
-template <int DIR>
-__device__ __forceinline__ void weno_flux_rusanov(const KPar &P, const Pt (&q)[6], int eoc, double fh[5]) {
+template <int DIR, int W = 3>
+__device__ __forceinline__ void weno_flux_rusanov(const KPar &P, const Pt (&q)[2 * W], int eoc, double fh[5]) {
     const int ws = P.weno_scheme;
     double evmax = -1.0;
 #pragma unroll
-    for (int l = 0; l < 6; l++) {
-        if (l < 3 - ws || l > 2 + ws) continue;
+    for (int l = 0; l < 2 * W; l++) {
+        if (l < W - ws || l > W - 1 + ws) continue;
         const double un = un_of<DIR>(q[l]);
         const double tt = q[l].t;
         const double gamloc = gamloc_of(P, tt);
@@ -484,9 +547,9 @@ __device__ __forceinline__ void weno_flux_rusanov(const KPar &P, const Pt (&q)[6
     const double r2u4 = r2u2 * u0_2;
 #pragma unroll 1
     for (int m = 0; m < 5; m++) {
-        double gp[6], gm[6];
+        double gp[2 * W], gm[2 * W];
 #pragma unroll
-        for (int l = 0; l < 6; l++) {
+        for (int l = 0; l < 2 * W; l++) {
             const Pt &a = q[l];
             const double un = un_of<DIR>(a);
             const double rhoun = a.r * un;
@@ -509,7 +572,7 @@ __device__ __forceinline__ void weno_flux_rusanov(const KPar &P, const Pt (&q)[6
             gm[l] = evm - c;
         }
         const double bs = m == 0 ? 1.0 / r2u2 : (m == 4 ? 1.0 / r2u4 / u0_2 : 1.0 / r2u4);
-        fh[m] = wenorec3(gp, gm, ord, P.weno_version, bs);
+        fh[m] = wenorec_w<W>(gp, gm, ord, P.weno_version, bs);
     }
 }
 

@@ -1469,30 +1469,31 @@ __global__ __launch_bounds__(256) void k_euler_xy(Geo g, KPar P, const R *__rest
 }
 
 // ------------------------------------------------------------ WENO fix-up
-// Hybrid flux of the face between window points 2 and 3 of q (offsets
-// -2..+3): central when no sensor value of the window exceeds the threshold,
-// characteristic WENO otherwise (the branch of the reference kernels).
-template <int DIR>
-__device__ __forceinline__ void hybrid_face(const KPar &P, const Pt (&q)[6], const double (&sen)[6], int f, int e,
-                                            int n, double fh[5]) {
+// Hybrid flux of the face between window points W-1 and W of q (offsets
+// -W+1..+W): central when no sensor value of the reference's switch window
+// exceeds the threshold, characteristic WENO otherwise (the branch of the
+// reference kernels).
+template <int DIR, int W = H>
+__device__ __forceinline__ void hybrid_face(const KPar &P, const Pt (&q)[2 * W], const double (&sen)[2 * W], int f,
+                                            int e, int n, double fh[5]) {
     bool shk = false;
 #pragma unroll
-    for (int s = 0; s < 6; s++)
-        if (s >= H - P.weno_scheme && s < H + P.weno_scheme) shk |= sen[s] > P.sensor_threshold;
+    for (int s = 0; s < 2 * W; s++)
+        if (s >= W - P.weno_scheme && s < W + P.weno_scheme) shk |= sen[s] > P.sensor_threshold;
     if (!shk) {
-        Pc c[6];
+        Pc c[2 * W];
 #pragma unroll
-        for (int s = 0; s < 6; s++) c[s] = pc_of(q[s]);
+        for (int s = 0; s < 2 * W; s++) c[s] = pc_of(q[s]);
         int lmax = P.lmax_base + e;
         lmax = lmax < 1 ? 1 : lmax;
         const bool zf = (f == 0 && P.fzf[2 * DIR] == 1) || (f == n && P.fzf[2 * DIR + 1] == 1);
-        if (P.nkeep == 0) central_flux<DIR, H, Pc, 0>(P, c, lmax, zf, fh);
-        else if (P.nkeep > 0) central_flux<DIR, H, Pc, 1>(P, c, lmax, zf, fh);
-        else central_flux<DIR, H, Pc, -1>(P, c, lmax, zf, fh);
+        if (P.nkeep == 0) central_flux<DIR, W, Pc, 0>(P, c, lmax, zf, fh);
+        else if (P.nkeep > 0) central_flux<DIR, W, Pc, 1>(P, c, lmax, zf, fh);
+        else central_flux<DIR, W, Pc, -1>(P, c, lmax, zf, fh);
     } else if (P.flux_splitting == 1) {
-        weno_flux_rusanov<DIR>(P, q, e, fh);
+        weno_flux_rusanov<DIR, W>(P, q, e, fh);
     } else {
-        weno_flux<DIR>(P, q, e, fh);
+        weno_flux<DIR, W>(P, q, e, fh);
     }
 }
 
@@ -2151,6 +2152,111 @@ __global__ __launch_bounds__(VBX *VBY) __attribute__((amdgpu_waves_per_eu(sizeof
                                                       const double *__restrict__ dz2, double *__restrict__ wallprop,
                                                       int images, int chunk) {
     k_visc_zm_body<R, SENSOR>(dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z), g, P, wa, fl, dx1, dxs, dx2, dy1, dys, dy2, dz1, dzs, dz2, wallprop, images, chunk);
+}
+
+// visflx (mode 0, with SENSOR) and visflx_nosensor (mode 4) for any viscous
+// order up to 8 (lmax <= 4): one thread per interior cell, the stencil read
+// from w_aux directly.  It serves four ghost layers (visc_order 8); the
+// z-marching tile kernel above serves lmax <= 3.  Same arithmetic, in the
+// same order, as k_visc_zm.
+template <typename R, bool SENSOR>
+__global__ __launch_bounds__(256) void k_visc_gen(Geo g, KPar P, R *__restrict__ wa, R *__restrict__ fl,
+                                                  const double *__restrict__ dx1, const double *__restrict__ dxs,
+                                                  const double *__restrict__ dx2, const double *__restrict__ dy1,
+                                                  const double *__restrict__ dys, const double *__restrict__ dy2,
+                                                  const double *__restrict__ dz1, const double *__restrict__ dzs,
+                                                  const double *__restrict__ dz2, double *__restrict__ wallprop,
+                                                  int images) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const size_t nv = g.nvol;
+    const int c = gidx(g, i, j, k);
+    const int lmax = P.visc_lmax;
+    const int stride[3] = {1, g.sx, g.sxy};
+    const R uu = wa[nv + c], vv = wa[2 * nv + c], ww = wa[3 * nv + c], tt = wa[5 * nv + c], mu = wa[6 * nv + c];
+    const R c20 = R(P.vd2[0]);
+    // per direction: first derivatives of u, v, w, T, mu and the
+    // second-derivative sums of u, v, w, T
+    R d[3][5], lap[3][4];
+    for (int dir = 0; dir < 3; dir++) {
+        for (int q = 0; q < 5; q++) d[dir][q] = R(0);
+        lap[dir][0] = c20 * uu; lap[dir][1] = c20 * vv; lap[dir][2] = c20 * ww; lap[dir][3] = c20 * tt;
+        const int st = stride[dir];
+        for (int l = 1; l <= 4; l++) {
+            if (l <= lmax) {
+                const R ccl = R(P.vd1[l - 1]), clapl = R(P.vd2[l]);
+                const int cp = c + l * st, cm = c - l * st;
+                const R up = wa[nv + cp], um = wa[nv + cm];
+                const R vp = wa[2 * nv + cp], vm = wa[2 * nv + cm];
+                const R wp = wa[3 * nv + cp], wm = wa[3 * nv + cm];
+                const R tp = wa[5 * nv + cp], tm = wa[5 * nv + cm];
+                const R mp = wa[6 * nv + cp], mm = wa[6 * nv + cm];
+                d[dir][0] = d[dir][0] + ccl * (up - um);
+                lap[dir][0] = lap[dir][0] + clapl * (up + um);
+                d[dir][1] = d[dir][1] + ccl * (vp - vm);
+                lap[dir][1] = lap[dir][1] + clapl * (vp + vm);
+                d[dir][2] = d[dir][2] + ccl * (wp - wm);
+                lap[dir][2] = lap[dir][2] + clapl * (wp + wm);
+                d[dir][3] = d[dir][3] + ccl * (tp - tm);
+                lap[dir][3] = lap[dir][3] + clapl * (tp + tm);
+                d[dir][4] = d[dir][4] + ccl * (mp - mm);
+            }
+        }
+    }
+    const R ax = R(dx1[i - 1]), ay = R(dy1[j - 1]), az = R(dz1[k - 1]);
+    const R sa[3] = {R(dxs[i - 1]), R(dys[j - 1]), R(dzs[k - 1])};
+    const R qa[3] = {R(dx2[i - 1]), R(dy2[j - 1]), R(dz2[k - 1])};
+    const R a1[3] = {ax, ay, az};
+    for (int dir = 0; dir < 3; dir++) {
+        for (int q = 0; q < 5; q++) d[dir][q] = d[dir][q] * a1[dir];
+        for (int q = 0; q < 4; q++) lap[dir][q] = lap[dir][q] * sa[dir] + d[dir][q] * qa[dir];
+    }
+    const R ux = d[0][0], vx = d[0][1], wx = d[0][2], tx_ = d[0][3], mux = d[0][4];
+    const R uy = d[1][0], vy = d[1][1], wy = d[1][2], ty_ = d[1][3], muy = d[1][4];
+    const R uz = d[2][0], vz = d[2][1], wz = d[2][2], tz = d[2][3], muz = d[2][4];
+    const R ulap = lap[0][0] + lap[1][0] + lap[2][0];
+    const R vlap = lap[0][1] + lap[1][1] + lap[2][1];
+    const R wlap = lap[0][2] + lap[1][2] + lap[2][2];
+    const R tlap = lap[0][3] + lap[1][3] + lap[2][3];
+    const R div = ux + vy + wz;
+    const R div3l = div / R(3.0);
+    wa[9 * nv + c] = div3l;
+    const R omegax = wy - vz, omegay = uz - wx, omegaz = vx - uy;
+    const R omod2 = omegax * omegax + omegay * omegay + omegaz * omegaz;
+    wa[8 * nv + c] = sqrt(omod2);
+    if (SENSOR) {
+        const R u0l0 = R(P.u0 / P.l0);
+        R sv = -div / sqrt(omod2 + div * div + u0l0 * u0l0);
+        sv = fmax(sv, R(0.0));
+        wa[7 * nv + c] = sv * sv;
+        if (images) write_images(g, P, wa + 7 * nv, i, j, k, sv * sv, false);
+    }
+    if (images && !SENSOR) write_images(g, P, wa + 9 * nv, i, j, k, div3l, true);
+    const R cploc = R(cploc_of(P, double(tt)));
+    const R sig11 = R(2.0) * (ux - div3l), sig12 = uy + vx, sig13 = uz + wx;
+    const R sig22 = R(2.0) * (vy - div3l), sig23 = vz + wy, sig33 = R(2.0) * (wz - div3l);
+    const R sigx = mux * sig11 + muy * sig12 + muz * sig13 + mu * ulap;
+    const R sigy = mux * sig12 + muy * sig22 + muz * sig23 + mu * vlap;
+    const R sigz = mux * sig13 + muy * sig23 + muz * sig33 + mu * wlap;
+    const R sigqt = (mux * tx_ + muy * ty_ + muz * tz + mu * tlap) * cploc / R(P.prandtl);
+    const R sigah = (sig11 * ux + sig12 * uy + sig13 * uz + sig12 * vx + sig22 * vy + sig23 * vz + sig13 * wx +
+                     sig23 * wy + sig33 * wz) * mu;
+    const R sigq = sigx * uu + sigy * vv + sigz * ww + sigah + sigqt;
+    const int oi = iidx(g, i, j, k);
+    const size_t ni = g.nint;
+    fl[ni + oi] = fl[ni + oi] - sigx;
+    fl[2 * ni + oi] = fl[2 * ni + oi] - sigy;
+    fl[3 * ni + oi] = fl[3 * ni + oi] - sigz;
+    fl[4 * ni + oi] = fl[4 * ni + oi] - sigq;
+    if (j == 1) {
+        const size_t wsz = (size_t)g.sx * g.sz;
+        const size_t wp = (size_t)(i + g.ng - 1) + (size_t)g.sx * (k + g.ng - 1);
+        wallprop[wp] = double(mu * uy);
+        wallprop[wsz + wp] = double(mu * wy);
+        wallprop[2 * wsz + wp] = double(mu * ty_ * cploc / R(P.prandtl));
+    }
 }
 
 // ------------------------------------------------ conservative viscous form
@@ -3056,8 +3162,9 @@ __global__ void k_fill(T *__restrict__ a, size_t n, T v) {
 // Hybrid face fluxes of faces f0..f1 along DIR into fhat (ghosted layout,
 // the face index in the DIR coordinate), every interior line of the other
 // two directions (euler_*_fluxes_hybrid_kernel, kernels_cpu.F90:425-689 and
-// its y/z twins): one thread per face, the window of cells f-2..f+3.
-template <typename R, int DIR>
+// its y/z twins): one thread per face, the window of cells f-W+1..f+W
+// (W = 3; W = 4 with four ghost layers, for WENO7 and ep_order 8).
+template <typename R, int DIR, int W>
 __global__ __launch_bounds__(256) void k_face_range(Geo g, KPar P, const R *__restrict__ wa,
                                                     const double *__restrict__ w, const int *__restrict__ eoc,
                                                     double *__restrict__ fhat, int f0, int f1) {
@@ -3073,11 +3180,11 @@ __global__ __launch_bounds__(256) void k_face_range(Geo g, KPar P, const R *__re
     const size_t nv = g.nvol;
     const bool keep = P.nkeep >= 0;
     const int base = gidx(g, i, j, k);
-    Pt q[6];
-    double sen[6];
+    Pt q[2 * W];
+    double sen[2 * W];
 #pragma unroll
-    for (int s = 0; s < 6; s++) {
-        const int a = base + (s - 2) * stride;
+    for (int s = 0; s < 2 * W; s++) {
+        const int a = base + (s - (W - 1)) * stride;
         sen[s] = double(wa[7 * nv + a]);
         if constexpr (sizeof(R) == sizeof(double)) {
             q[s] = make_pt(double(wa[a]), double(wa[nv + a]), double(wa[2 * nv + a]), double(wa[3 * nv + a]),
@@ -3085,9 +3192,9 @@ __global__ __launch_bounds__(256) void k_face_range(Geo g, KPar P, const R *__re
         } else {
             // FP32 w_aux: the primitives from w in FP64 as eval_aux forms them
             int pi = i, pj = j, pk = k;
-            if (DIR == 0) pi += s - 2;
-            else if (DIR == 1) pj += s - 2;
-            else pk += s - 2;
+            if (DIR == 0) pi += s - (W - 1);
+            else if (DIR == 1) pj += s - (W - 1);
+            else pk += s - (W - 1);
             if (P.periodic[0]) pi = pi < 1 ? pi + g.nx : (pi > g.nx ? pi - g.nx : pi);
             if (P.periodic[1]) pj = pj < 1 ? pj + g.ny : (pj > g.ny ? pj - g.ny : pj);
             if (P.periodic[2]) pk = pk < 1 ? pk + g.nz : (pk > g.nz ? pk - g.nz : pk);
@@ -3102,7 +3209,7 @@ __global__ __launch_bounds__(256) void k_face_range(Geo g, KPar P, const R *__re
         }
     }
     double fh[5];
-    hybrid_face<DIR>(P, q, sen, f, eoc[f], n, fh);
+    hybrid_face<DIR, W>(P, q, sen, f, eoc[f], n, fh);
 #pragma unroll
     for (int m = 0; m < 5; m++) fhat[m * nv + base] = fh[m];
 }
@@ -3515,6 +3622,18 @@ hipError_t launch_weno_fix(st_solver *s, void *fl, int dir, int bit, int anyidx)
 
 hipError_t launch_visc(st_solver *s, void *fl, int mode, bool ghost_images) {
     const Geo &g = s->g;
+    if (s->kp.visc_lmax > H) {
+        // viscous order 8 (four ghost layers): the generic per-cell kernel
+        ST_NOREC();
+#define ST_LAUNCH_VG(S_)                                                                                           \
+    hipLaunchKernelGGL((k_visc_gen<R, S_>), cell_grid(g), kCellBlock, 0, s->stream, g, s->kp, (R *)s->w_aux,       \
+                       (R *)fl, s->dcsidx, s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz,   \
+                       s->dzitdzs, s->dzitdz2, s->wallprop, ghost_images ? 1 : 0)
+        if (mode == 0) ST_DISPATCH(ST_LAUNCH_VG(true));
+        else ST_DISPATCH(ST_LAUNCH_VG(false));
+#undef ST_LAUNCH_VG
+        return hipGetLastError();
+    }
     const int bxy = ((g.nx + VBX - 1) / VBX) * ((g.ny + VBY - 1) / VBY);
     // chunks of up to 32 planes (6 warm-up planes each), enough of them to
     // give every CU several tiles
@@ -3791,8 +3910,13 @@ static hipError_t face_range_t(st_solver *s, int f0, int f1) {
                     n3 = DIR == 2 ? f1 - f0 + 1 : g.nz;
     const long long tot = n1 * n2 * n3;
     if (tot <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_face_range<R, DIR>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, g,
-                       s->kp, (const R *)s->w_aux, s->w, s->eoc[DIR], s->fhat, f0, f1);
+    const dim3 grid((unsigned)((tot + 255) / 256));
+    if (g.ng == 4)
+        hipLaunchKernelGGL((k_face_range<R, DIR, 4>), grid, dim3(256), 0, s->stream, g, s->kp, (const R *)s->w_aux,
+                           s->w, s->eoc[DIR], s->fhat, f0, f1);
+    else
+        hipLaunchKernelGGL((k_face_range<R, DIR, 3>), grid, dim3(256), 0, s->stream, g, s->kp, (const R *)s->w_aux,
+                           s->w, s->eoc[DIR], s->fhat, f0, f1);
     return hipGetLastError();
 }
 

@@ -1158,6 +1158,7 @@ int st_server_submit2(st_server *sv, const st_params *p, const double *x, const 
         return sfail(ST_ERR_ARG, "the checkpoint period must be a multiple of the steps per chunk");
     int rc = st::validate_params(p);
     if (rc) return sfail(rc, st_last_error());
+    if (p->ng != 3) return sfail(ST_ERR_ARG, "the server takes solves with three ghost layers");
     if (p->nx + p->ny + p->nz + 3 > kTabInts) return sfail(ST_ERR_ARG, "grid too large for the server");
     if (p->bctags[0] == 10 || p->bctags[0] == 9)
         return sfail(ST_ERR_ARG, "the server takes no inflow planes (recycling or laminar inflow)");

@@ -24,20 +24,19 @@ def fill_ghosts_wrap(w, ng=3):
 
 
 def box_case(nx=16, ny=12, nz=10, L=(2 * np.pi, 2 * np.pi, 2 * np.pi), mu0=1e-2, threshold=2.0,
-             nkeep=0, weno_version=0, visc_model=1):
+             nkeep=0, weno_version=0, visc_model=1, ng=3, ep_order=6, weno_scheme=3):
     """Triply periodic box on a uniform grid (no walls, no forcing):
     StParams + Grid with exact uniform metrics (dcsidx2 = 0)."""
     from streams_mi355x.grid import Grid
     from streams_mi355x.params import FlowCase, StParams, set_fluid_prop
 
-    ng = 3
     case = FlowCase(nx=nx, ny=ny, nz=nz)
     gam, gm1, rfac, icl, icr, cp, cv = set_fluid_prop(case)
     p = StParams()
     p.nx, p.ny, p.nz, p.ng = nx, ny, nz, ng
-    p.ep_order = p.visc_order = 6
+    p.ep_order = p.visc_order = ep_order
     p.nkeep = nkeep
-    p.weno_scheme = 3
+    p.weno_scheme = weno_scheme
     p.weno_version = weno_version
     p.visc_model = visc_model
     p.calorically_perfect = 1

@@ -76,7 +76,7 @@ def test_invalid_parameters_rejected_without_gpu():
     from streams_mi355x import make_channel
     c = make_channel()
     p = c.params
-    p.ng = 4
+    p.ng = 2
     h = C.c_void_p()
     rc = capi.lib().st_solver_create(C.byref(p), C.byref(h))
     assert rc == capi.ST_ERR_ARG
@@ -120,3 +120,26 @@ def test_grid_limit_is_2_to_the_31_points_per_variable():
     p.nx, p.ny, p.nz = 1024, 512, 448
     foot = capi.lib().st_solver_footprint(C.byref(p))
     assert foot > 10 * 1030 * 518 * 454 * 8 > 2 ** 31 * 8
+
+
+@pytest.mark.parametrize("change,msg", [
+    (dict(precision=1), b"FP64 only"),
+    (dict(bctags=(1, 2, 6, 2, 0, 0)), b"periodic faces and staggered"),
+    (dict(ng=3), b"ep_order must be even and at most 2 ng"),
+    (dict(weno_version=1), b"weno_version 1 supported only for weno_scheme = 3"),
+    (dict(ng=5), b"ng must be 3, or 4"),
+])
+def test_four_ghost_layers_are_validated(change, msg):
+    """ng = 4 (WENO7 / ep_order 8, singleideal.F90:497-509) runs the channel
+    procedure by procedure in FP64; every other combination is refused
+    before any device call."""
+    from streams_mi355x import make_channel
+    p = make_channel(nx=16, ny=12, nz=8, ng=4, ep_order=8, weno_scheme=4).params
+    for k, v in change.items():
+        if k == "bctags":
+            p.bctags[:] = list(v)
+        else:
+            setattr(p, k, v)
+    h = C.c_void_p()
+    assert capi.lib().st_solver_create(C.byref(p), C.byref(h)) == capi.ST_ERR_ARG
+    assert msg in capi.lib().st_last_error(), capi.lib().st_last_error()
