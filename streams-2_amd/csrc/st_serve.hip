@@ -798,8 +798,8 @@ void unbind_lane(st_server *sv, Slot &sl) {
 
 // Scheduler: the next batched chunk of lane `li`.  Its solves are the
 // ready ones of the lane (set up, no chunk in flight on their own stream,
-// no checkpoint copy pending) whose launch sequence is that of the oldest
-// of them.  Each is recorded for K steps (which advances its host state as
+// no checkpoint copy pending) whose launch sequence is that of the least
+// advanced of them.  Each is recorded for K steps (which advances its host state as
 // enqueuing does), the tables are laid out and uploaded (skipped when the
 // buffer already holds them), and one launch per recorded launch follows
 // on the lane's stream.
@@ -818,7 +818,11 @@ void launch_batch(st_server *sv, int li, bool *progress) {
         }
         if (!ok || sl.nev > 0 || sl.lane != li || !batch_bound(sv, sl)) continue;
         ready.push_back(q);
-        if (lead < 0 || sl.r->id < sv->slots[lead].r->id) lead = q;
+        // the group of the least advanced ready solve (the oldest among
+        // equals): groups of different launch sequences in one lane take
+        // turns
+        const Slot &ld = sv->slots[lead < 0 ? q : lead];
+        if (lead < 0 || sl.launched < ld.launched || (sl.launched == ld.launched && sl.r->id < ld.r->id)) lead = q;
     }
     if (lead < 0) return;
     std::vector<int> mem;

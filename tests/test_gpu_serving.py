@@ -217,6 +217,20 @@ def test_batched_without_graphs():
         assert np.array_equal(interior(a["w"]), interior(b["w"])), n
 
 
+def test_batched_lane_with_two_launch_sequences():
+    """FP64 and mixed-form solves in one lane: two launch sequences, batched
+    in turns; every result equals its plain solve."""
+    reqs = _requests(0)[:3] + _requests(PREC_MIXED)[3:6]
+    seq = run_sequential(reqs, want_fields=True)
+    srv = run_serving(reqs, hbm_bytes=8 << 30, max_slots=6, steps_per_graph=10, autotune=False, want_fields=True,
+                      timeout=120.0, batch=1)
+    st = srv["stats"]
+    assert st["batched_chunks"] > 0 and st["batch_layouts"] > 1
+    for n, (a, b) in enumerate(zip(seq["results"], srv["results"])):
+        assert b["error"] is None, (n, b["error"])
+        assert np.array_equal(interior(a["w"]), interior(b["w"])), n
+
+
 def test_batched_diverging_request_fails_alone():
     from streams_mi355x.capi import ST_ERR_NAN
 
