@@ -56,3 +56,14 @@ def test_four_ghost_layers_match_oracle(kw):
     assert np.array_equal(weno, weno_o)
     assert max(weno) > 0.0          # the WENO branch is exercised
     assert sc["icyc"] == 20
+
+
+def test_weno7_eighth_order_channel_128x96x64_100_steps():
+    """WENO7 and the eighth-order schemes on the 128 x 96 x 64 channel
+    (M_b 1.5, Re_tau 220) over 100 steps, sensor threshold 0.5."""
+    c = make_channel(nx=128, ny=96, nz=64, ng=NG, ep_order=8, weno_scheme=4, sensor_threshold=0.5)
+    errs, weno, weno_o, sc = _pair(c, steps=100)
+    print("WENO7 / ep_order 8, 128x96x64, 100 steps: rel L2", errs, "WENO fractions", list(weno))
+    assert max(errs) <= TOL, errs
+    assert np.array_equal(weno, weno_o)
+    assert sc["icyc"] == 100
