@@ -212,7 +212,9 @@ struct Lane {
 };
 
 struct Batcher {
-    bool on = false;
+    // set by the client thread before the first submission, read by the
+    // scheduler: the lanes are complete before `on` is seen true
+    std::atomic<bool> on{false};
     int64_t max_cells = 0;               // solves with more interior cells run on their own streams (0: no limit)
     std::vector<Lane> lanes;
     size_t cap = 0;                      // bytes of one table buffer
@@ -1043,7 +1045,29 @@ void serve_loop(st_server *sv) {
     }
 }
 
+// The lanes' streams, events, graphs and table buffers.
+void free_lanes(Batcher &bt) {
+    for (Lane &ln : bt.lanes) {
+        if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+        for (int b = 0; b < 2; b++) {
+            if (ln.exec[b]) (void)hipGraphExecDestroy(ln.exec[b]);
+            if (ln.graph[b]) (void)hipGraphDestroy(ln.graph[b]);
+            if (ln.ev[b]) (void)hipEventDestroy(ln.ev[b]);
+            if (ln.dtab[b]) (void)hipFree(ln.dtab[b]);
+            if (ln.htab[b]) (void)hipHostFree(ln.htab[b]);
+        }
+        if (ln.stream) (void)hipStreamDestroy(ln.stream);
+    }
+    bt.lanes.clear();
+    if (bt.join) (void)hipEventDestroy(bt.join);
+    bt.join = nullptr;
+}
+
 void destroy_slots(st_server *sv) {
+    // batched chunks still in flight use the solves' pool memory: the lanes
+    // drain first
+    sv->bt.on = false;
+    free_lanes(sv->bt);
     for (auto &sl : sv->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
@@ -1064,23 +1088,6 @@ void destroy_slots(st_server *sv) {
     }
     if (sv->pool.base) (void)hipFree(sv->pool.base);
     sv->pool.base = nullptr;
-    Batcher &bt = sv->bt;
-    for (Lane &ln : bt.lanes) {
-        if (ln.stream) (void)hipStreamSynchronize(ln.stream);
-        for (int b = 0; b < 2; b++) {
-            if (ln.ev[b]) (void)hipEventDestroy(ln.ev[b]);
-            if (ln.dtab[b]) (void)hipFree(ln.dtab[b]);
-            if (ln.htab[b]) (void)hipHostFree(ln.htab[b]);
-        }
-        for (int b = 0; b < 2; b++) {
-            if (ln.exec[b]) (void)hipGraphExecDestroy(ln.exec[b]);
-            if (ln.graph[b]) (void)hipGraphDestroy(ln.graph[b]);
-        }
-        if (ln.stream) (void)hipStreamDestroy(ln.stream);
-    }
-    bt.lanes.clear();
-    if (bt.join) (void)hipEventDestroy(bt.join);
-    bt.join = nullptr;
 }
 
 }  // namespace
@@ -1256,7 +1263,10 @@ int st_server_set_batching(st_server *sv, int32_t lanes, int64_t max_cells) {
             if (e == hipSuccess) e = hipHostMalloc((void **)&ln.htab[b], kBatchTabBytes, hipHostMallocDefault);
         }
     }
-    if (e != hipSuccess) return sfail(ST_ERR_NOMEM, std::string("batch tables: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        free_lanes(bt);  // a later call starts afresh
+        return sfail(ST_ERR_NOMEM, std::string("batch tables: ") + hipGetErrorString(e));
+    }
     bt.cap = kBatchTabBytes;
     bt.on = true;
     return ST_OK;

@@ -282,3 +282,24 @@ def test_batching_is_set_before_the_first_submission():
             check_server(lib().st_server_set_batching(sv.h, 1, 0))
     with pytest.raises(Exception):
         Server(hbm_bytes=8 << 30, max_slots=1, steps_per_graph=3, batch=1)
+
+
+def test_closing_with_batched_chunks_in_flight():
+    """A server closed while batched chunks run drains its lanes before it
+    releases the solves and the pool; a new server afterwards gives the plain
+    results."""
+    import time
+
+    reqs = _requests()[:4]
+    with Server(hbm_bytes=8 << 30, max_slots=4, steps_per_graph=10, autotune=False, batch=1) as sv:
+        for r in reqs:
+            sv.submit(r.case.params, r.case.grid, r.case.w0, 2000)
+        t_end = time.time() + 60.0
+        while sv.stats()["batched_chunks"] < 4 and time.time() < t_end:
+            time.sleep(0.01)
+        assert sv.stats()["batched_chunks"] >= 4
+    seq = run_sequential(reqs[:2], want_fields=True)
+    srv = run_serving(reqs[:2], hbm_bytes=8 << 30, max_slots=2, steps_per_graph=10, autotune=False,
+                      want_fields=True, timeout=120.0, batch=1)
+    for n, (a, b) in enumerate(zip(seq["results"], srv["results"])):
+        assert np.array_equal(interior(a["w"]), interior(b["w"])), n
