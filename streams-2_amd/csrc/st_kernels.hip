@@ -2001,7 +2001,18 @@ __device__ __forceinline__ void k_visc_zm_body(const dim3 blockIdx, const dim3 g
 #pragma unroll
     for (int q = 0; q < 2 * H; q++) win[q + 1] = load_vp(wa, nv, col + (k0 - H + q) * sz);
     VP<R> nxt = load_vp(wa, nv, col + (k0 + H) * sz);
-    VP<R> hv = load_vp(wa, nv, (hl ? colh : col) + k0 * sz);
+    // The halo points of plane k are the centre points of the neighbouring
+    // tiles, which those load H planes ahead, into their windows.  Loaded
+    // one plane ahead, the halo lines were fetched by the neighbours three
+    // planes earlier and have left L2 (the halos were re-fetched from HBM,
+    // 1.4x the kernel's algorithmic reads).  The FP64 form therefore loads
+    // them KH = H planes ahead, in step with the neighbours, and keeps KH
+    // planes of its halo point in registers; the FP32 form, at its 128-VGPR
+    // bound, loads them one plane ahead (KH = 1).
+    constexpr int KH = sizeof(R) == 8 ? H : 1;
+    VP<R> hv[KH];  // halo point of planes k .. k+KH-1 at the top of iteration k
+#pragma unroll
+    for (int q = 0; q < KH; q++) hv[q] = load_vp(wa, nv, (hl ? colh : col) + min(k0 + q, g.nz + H) * sz);
     const R ax = R(dx1[min(i, g.nx) - 1]), ay = R(dy1[min(j, g.ny) - 1]);
     const R sx_ = R(dxs[min(i, g.nx) - 1]), sy_ = R(dys[min(j, g.ny) - 1]);
     const R qx = R(dx2[min(i, g.nx) - 1]), qy = R(dy2[min(j, g.ny) - 1]);
@@ -2018,18 +2029,18 @@ __device__ __forceinline__ void k_visc_zm_body(const dim3 blockIdx, const dim3 g
         tile[3][ty + H][tx + H] = c0.t;
         tile[4][ty + H][tx + H] = c0.m;
         if (hl) {
-            tile[0][hly][hlx] = hv.u; tile[1][hly][hlx] = hv.v; tile[2][hly][hlx] = hv.w;
-            tile[3][hly][hlx] = hv.t; tile[4][hly][hlx] = hv.m;
+            tile[0][hly][hlx] = hv[0].u; tile[1][hly][hlx] = hv[0].v; tile[2][hly][hlx] = hv[0].w;
+            tile[3][hly][hlx] = hv[0].t; tile[4][hly][hlx] = hv[0].m;
         }
         // right-hand side of this plane, read ahead of the stencil work
         const int oi = iidx(g, min(i, g.nx), min(j, g.ny), k);
         const size_t ni = g.nint;
         const R fo2 = fl[ni + oi], fo3 = fl[2 * ni + oi], fo4 = fl[3 * ni + oi], fo5 = fl[4 * ni + oi];
-        // loads of plane k+1 overlap the work on plane k
-        if (k < k1) {
-            nxt = load_vp(wa, nv, col + (k + 1 + H) * sz);
-            if (hl) hv = load_vp(wa, nv, colh + (k + 1) * sz);
-        }
+        // loads of plane k+1 (centre) and k+KH (halo) overlap the work on plane k
+#pragma unroll
+        for (int q = 0; q + 1 < KH; q++) hv[q] = hv[q + 1];
+        if (k < k1) nxt = load_vp(wa, nv, col + (k + 1 + H) * sz);
+        if (hl && k + KH <= k1) hv[KH - 1] = load_vp(wa, nv, colh + (k + KH) * sz);
         __syncthreads();
         if (act) {
             const R uu = c0.u, vv = c0.v, ww = c0.w, tt = c0.t, mu = c0.m;
