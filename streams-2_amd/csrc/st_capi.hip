@@ -115,7 +115,8 @@ int validate(const st_params *p) {
         if ((p->bctags[2 * d] == 0) != (p->bctags[2 * d + 1] == 0))
             return fail(ST_ERR_ARG, "a periodic direction needs tag 0 on both faces");
     // a point's index within one variable is 32-bit (offsets across variables are 64-bit)
-    const double vol = ((double)p->nx + 2.0 * p->ng) * ((double)p->ny + 2.0 * p->ng) * ((double)p->nz + 2.0 * p->ng);
+    const double rows = std::ceil(((double)p->nx + 2.0 * p->ng) / st::kRowAlign) * st::kRowAlign;  // pitched rows
+    const double vol = rows * ((double)p->ny + 2.0 * p->ng) * ((double)p->nz + 2.0 * p->ng);
     if (vol > 2147483647.0 - 4096.0) return fail(ST_ERR_ARG, "grid too large: a ghosted variable must hold fewer than 2^31 points");
     return ST_OK;
 }
@@ -211,12 +212,15 @@ void st::params_to_kpar(const st_params &p, st::KPar &k) {
     for (int l = 0; l < 5; l++) k.vd2[l] = d2[l];
 }
 
-st::Geo st::params_to_geo(const st_params &p) {
+st::Geo st::params_to_geo(const st_params &p, bool solver) {
     st::Geo g;
     g.nx = p.nx; g.ny = p.ny; g.nz = p.nz; g.ng = p.ng;
-    g.sx = p.nx + 2 * p.ng; g.sy = p.ny + 2 * p.ng; g.sz = p.nz + 2 * p.ng;
+    g.rx = p.nx + 2 * p.ng; g.sy = p.ny + 2 * p.ng; g.sz = p.nz + 2 * p.ng;
+    const int a = st::row_align(p.precision);
+    g.sx = solver ? (g.rx + a - 1) / a * a : g.rx;
     g.sxy = g.sx * g.sy;
     g.nvol = (long long)g.sxy * g.sz;
+    g.nref = (long long)g.rx * g.sy * g.sz;
     g.nint = (long long)p.nx * p.ny * p.nz;
     return g;
 }
@@ -240,7 +244,7 @@ hipError_t dalloc(T **p, size_t n) {
 // Frees what the solver allocated; arrays inside memory lent to it
 // (create_solver with a SlotMem) and a lent scalar mirror stay with the lender.
 void free_all(st_solver *s) {
-    void *ptrs[] = {s->w, s->w_aux, s->F[0], s->F[1], s->wallprop, s->eoc[0], s->eoc[1], s->eoc[2], s->x, s->y,
+    void *ptrs[] = {s->w_alloc ? s->w_alloc : (void *)s->w, s->w_aux_alloc ? s->w_aux_alloc : s->w_aux, s->F[0], s->F[1], s->wallprop, s->eoc[0], s->eoc[1], s->eoc[2], s->x, s->y,
                     s->z, s->yn, s->dcsidx, s->dcsidxs, s->dcsidx2, s->detady, s->detadys, s->detady2, s->dzitdz,
                     s->dzitdzs, s->dzitdz2, s->ds, s->partials, s->dirty, s->counters, s->fixlist, s->rc_real, s->rc_maps,
                     s->w_stat, s->rplane, s->rng_coef, s->rng_win, s->rng_ctl, s->wrecyc, s->fhat, s->fln};
@@ -367,7 +371,7 @@ int partial_slots(const st::Geo &g) {
     // per-block partials (7 per cell block, or one per block of the aux
     // pass over the ghosted box, whichever is larger), the z fix-up's
     // 5 x 256 force_rhs partials, and the 128 x 8 second-level values at the end
-    const int aux_blk = (int)((g.nvol + 255) / 256);
+    const int aux_blk = (int)((st::aux_lanes(g) + 255) / 256);
     return (7 * nblk > aux_blk ? 7 * nblk : aux_blk) + 256 * 5 + 128 * 8;
 }
 
@@ -378,8 +382,8 @@ void array_sizes(const st_params &p, size_t n[kNArrays]) {
     const size_t na = p.precision == ST_PREC_MIXED ? 11 : 10;
     const size_t d = sizeof(double);
     size_t q = 0;
-    n[q++] = d * 5 * nvol;                 // w
-    n[q++] = rs * na * nvol;               // w_aux
+    n[q++] = d * 5 * nvol + 128;           // w (+ the lead of its first point)
+    n[q++] = rs * na * nvol + 128;         // w_aux
     n[q++] = rs * 5 * nint;                // F[0]
     n[q++] = rs * 5 * nint;                // F[1]
     n[q++] = d * 3 * g.sx * g.sz;          // wallprop
@@ -468,6 +472,11 @@ int create_solver(const st_params *p, const st::SlotMem *mem, st_solver **out) {
             return fail(ST_ERR_NOMEM, std::string("device allocation failed: ") + hipGetErrorString(ea));
         }
     }
+    // the first points of w and w_aux sit lead_bytes into their allocations
+    s->w_alloc = s->w;
+    s->w_aux_alloc = s->w_aux;
+    s->w = (double *)((char *)s->w_alloc + st::lead_bytes(g.ng, sizeof(double)));
+    s->w_aux = (char *)s->w_aux_alloc + st::lead_bytes(g.ng, rsize(s));
     // tables
     int *eh[3];
     int *buf = mem ? mem->tabs : (int *)std::calloc((size_t)(g.nx + g.ny + g.nz + 3), sizeof(int));
@@ -519,6 +528,34 @@ int create_solver(const st_params *p, const st::SlotMem *mem, st_solver **out) {
     return ST_OK;
 }
 
+// nvars ghosted variables of es-byte values between the solver's pitched
+// layout (device) and the reference layout (host): the rows are re-spaced in
+// a host buffer and the whole block moves in one copy on the solver's stream.
+// Returns once the copy has completed.
+hipError_t copy_rows(st_solver *s, void *dst, const void *src, size_t es, int nvars, hipMemcpyKind kind) {
+    const st::Geo &g = s->g;
+    const size_t dpitch = (size_t)g.sx * es, hpitch = (size_t)g.rx * es;
+    const size_t rows = (size_t)nvars * g.sy * g.sz;
+    std::vector<char> tmp;
+    try {
+        tmp.assign(rows * dpitch, 0);
+    } catch (const std::bad_alloc &) {
+        return hipErrorOutOfMemory;
+    }
+    hipError_t e;
+    if (kind == hipMemcpyHostToDevice) {
+        for (size_t r = 0; r < rows; r++) std::memcpy(tmp.data() + r * dpitch, (const char *)src + r * hpitch, hpitch);
+        e = hipMemcpyAsync(dst, tmp.data(), rows * dpitch, kind, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    } else {
+        e = hipMemcpyAsync(tmp.data(), src, rows * dpitch, kind, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e == hipSuccess)
+            for (size_t r = 0; r < rows; r++) std::memcpy((char *)dst + r * hpitch, tmp.data() + r * dpitch, hpitch);
+    }
+    return e;
+}
+
 }  // namespace
 
 extern "C" {
@@ -554,7 +591,7 @@ int st_set_grid(st_solver *s, const double *x, const double *y, const double *z,
     double *dst[13] = {s->x, s->y, s->z, s->yn, s->dcsidx, s->dcsidxs, s->dcsidx2,
                        s->detady, s->detadys, s->detady2, s->dzitdz, s->dzitdzs, s->dzitdz2};
     const st::Geo &g = s->g;
-    const size_t n[13] = {(size_t)g.sx, (size_t)g.sy, (size_t)g.sz, (size_t)g.ny + 1, (size_t)g.nx, (size_t)g.nx,
+    const size_t n[13] = {(size_t)g.rx, (size_t)g.sy, (size_t)g.sz, (size_t)g.ny + 1, (size_t)g.nx, (size_t)g.nx,
                           (size_t)g.nx, (size_t)g.ny, (size_t)g.ny, (size_t)g.ny, (size_t)g.nz, (size_t)g.nz,
                           (size_t)g.nz};
     for (int q = 0; q < 13; q++) {
@@ -574,7 +611,7 @@ int st_set_grid(st_solver *s, const double *x, const double *y, const double *z,
 int st_copy_from_field(st_solver *s, const double *w_host) {
     NEED(s);
     if (!w_host) return fail(ST_ERR_ARG, "null host array");
-    HIPCHK(hipMemcpyAsync(s->w, w_host, sizeof(double) * 5 * (size_t)s->g.nvol, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(copy_rows(s, s->w, w_host, sizeof(double), 5, hipMemcpyHostToDevice));
     HIPCHK(hipStreamSynchronize(s->stream));
     return ST_OK;
 }
@@ -582,7 +619,7 @@ int st_copy_from_field(st_solver *s, const double *w_host) {
 int st_copy_to_field(st_solver *s, double *w_host) {
     NEED(s);
     if (!w_host) return fail(ST_ERR_ARG, "null host array");
-    HIPCHK(hipMemcpyAsync(w_host, s->w, sizeof(double) * 5 * (size_t)s->g.nvol, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(copy_rows(s, w_host, s->w, sizeof(double), 5, hipMemcpyDeviceToHost));
     HIPCHK(hipStreamSynchronize(s->stream));
     return ST_OK;
 }
@@ -614,18 +651,47 @@ static int from_host(st_solver *s, void *dev, const double *host, size_t n) {
     return ST_OK;
 }
 
+// One ghosted variable (double, or float in the mixed form) between the
+// solver's pitched layout and a host array in the reference layout.
+static int ghost_to_host(st_solver *s, double *host, const void *dev) {
+    const size_t n = (size_t)s->g.nref;
+    if (!s->mixed) {
+        HIPCHK(copy_rows(s, host, dev, sizeof(double), 1, hipMemcpyDeviceToHost));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return ST_OK;
+    }
+    std::vector<float> tmp(n);
+    HIPCHK(copy_rows(s, tmp.data(), dev, sizeof(float), 1, hipMemcpyDeviceToHost));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t t = 0; t < n; t++) host[t] = tmp[t];
+    return ST_OK;
+}
+
+static int ghost_from_host(st_solver *s, void *dev, const double *host) {
+    const size_t n = (size_t)s->g.nref;
+    if (!s->mixed) {
+        HIPCHK(copy_rows(s, dev, host, sizeof(double), 1, hipMemcpyHostToDevice));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return ST_OK;
+    }
+    std::vector<float> tmp(host, host + n);
+    HIPCHK(copy_rows(s, dev, tmp.data(), sizeof(float), 1, hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ST_OK;
+}
+
 int st_get_aux(st_solver *s, int32_t var, double *host) {
     NEED(s);
     if (var < 1 || var > 10 || !host) return fail(ST_ERR_ARG, "var must be 1..10");
     const size_t nv = s->g.nvol;
-    return to_host(s, host, (const char *)s->w_aux + rsize(s) * (size_t)(var - 1) * nv, nv);
+    return ghost_to_host(s, host, (const char *)s->w_aux + rsize(s) * (size_t)(var - 1) * nv);
 }
 
 int st_set_aux(st_solver *s, int32_t var, const double *host) {
     NEED(s);
     if (var < 1 || var > 10 || !host) return fail(ST_ERR_ARG, "var must be 1..10");
     const size_t nv = s->g.nvol;
-    return from_host(s, (char *)s->w_aux + rsize(s) * (size_t)(var - 1) * nv, host, nv);
+    return ghost_from_host(s, (char *)s->w_aux + rsize(s) * (size_t)(var - 1) * nv, host);
 }
 
 int st_get_fl(st_solver *s, double *host) {

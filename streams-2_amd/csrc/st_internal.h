@@ -1,12 +1,16 @@
 // st_internal.h -- internal types and device helpers of libstreams_mi355x.
 //
-// Device layout: every ghosted array keeps the reference layout, one
-// contiguous Fortran-order block per variable (x fastest, lower bound 1-ng),
-// so host<->device copies of w are single memcpys and a wavefront's 64
-// consecutive lanes read 64 consecutive doubles along x.  The right-hand
-// side fl is double-buffered: F[cur] holds the RHS of the stage being built,
-// F[cur^1] the RHS of the previous stage, which replaces the reference's fln
-// register (init_flux/update_flux, kernels_cpu.F90:29-47,3400-3418).
+// Device layout: every ghosted array keeps the reference's Fortran order,
+// one block per variable (x fastest, lower bound 1-ng), with its rows padded
+// to a whole number of 128-byte lines of w_aux values and its first point
+// placed so that i = 1 of every row starts a line: a wavefront's 64
+// consecutive lanes read 64 consecutive values along x in whole lines.
+// (Rows of the reference's nx + 2 ng values start anywhere in a line; a
+// streaming pass over them ran 8 % slower.)  Host arrays keep the reference
+// layout; the copies are pitched.  The right-hand side fl is
+// double-buffered: F[cur] holds the RHS of the stage being built, F[cur^1]
+// the RHS of the previous stage, which replaces the reference's fln register
+// (init_flux/update_flux, kernels_cpu.F90:29-47,3400-3418).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,11 +23,45 @@ namespace st {
 // ---------------------------------------------------------------- geometry
 struct Geo {
     int nx, ny, nz, ng;
-    int sx, sy, sz;      // ghosted extents
+    int sx, sy, sz;      // row pitch (>= nx + 2 ng), ghosted y and z extents
     int sxy;             // sx*sy
-    long long nvol;      // points of one ghosted variable (offsets across variables are 64-bit)
+    long long nvol;      // points of one ghosted variable, padding included (offsets across variables are 64-bit)
     long long nint;      // points of one interior variable
+    int rx;              // nx + 2 ng: the ghosted x extent, the row length of the reference layout
+    long long nref;      // rx*sy*sz: points of one ghosted variable in the reference layout (host arrays)
 };
+
+// Row pitch of the solver layout: a multiple of one 128-byte line of w_aux
+// values, 16 in FP64 and 32 in the mixed form (whose w rows then span whole
+// pairs of lines).
+constexpr int kRowAlign = 32;  // the larger of the two
+__host__ __device__ constexpr int row_align(int precision) { return precision == ST_PREC_MIXED ? 32 : 16; }
+// Bytes from an allocation's (256-byte aligned) start to the first point of
+// a ghosted array of values of `bytes` each: i = 1 then falls on offset 128.
+__host__ __device__ constexpr size_t lead_bytes(int ng, size_t bytes) { return 128 - (size_t)ng * bytes; }
+
+// A pass over a box of ghosted points whose lanes walk x (k_aux) starts its
+// rows at the last x position at or below i0 that opens a line (i = 1 mod
+// 32) and gives each row a multiple of 64 lanes: every wavefront then reads
+// and writes whole lines of one row, and the lanes outside [i0, i0 + ni)
+// idle.  Boxes narrower than a wavefront (the x ghost slabs) pack their
+// rows instead.
+__host__ __device__ __forceinline__ int box_row_start(int i0, int ni) {
+    return ni >= 64 ? i0 - (((i0 - 1) % 32) + 32) % 32 : i0;
+}
+__host__ __device__ __forceinline__ int box_row_lanes(int i0, int ni) {
+    return ni >= 64 ? (i0 + ni - box_row_start(i0, ni) + 63) / 64 * 64 : ni;
+}
+
+// The pass over the whole ghosted box instead walks the pitched array
+// linearly, lane L + aux_lead on point L, so that its wavefronts read w in
+// whole lines from the line holding the first point; the lanes on the row
+// padding write zeros to w_aux, so that its lines are written whole too.
+__host__ __device__ __forceinline__ bool aux_whole_box(const Geo &g, int i0, int ni, int j0, int nj, int k0, int nk) {
+    return i0 == 1 - g.ng && ni == g.rx && j0 == 1 - g.ng && nj == g.sy && k0 == 1 - g.ng && nk == g.sz;
+}
+__host__ __device__ __forceinline__ int aux_lead(const Geo &g) { return (int)(lead_bytes(g.ng, 8) / 8); }
+__host__ __device__ __forceinline__ long long aux_lanes(const Geo &g) { return g.nvol + aux_lead(g); }
 
 __host__ __device__ __forceinline__ int gidx(const Geo &g, int i, int j, int k) {
     return (i + g.ng - 1) + g.sx * ((j + g.ng - 1) + g.sy * (k + g.ng - 1));

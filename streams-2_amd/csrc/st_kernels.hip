@@ -628,11 +628,29 @@ __device__ __forceinline__ void k_aux_body(const dim3 blockIdx, const dim3 gridD
                                              double *__restrict__ partials) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double mx = 0.0;
-    if (t < ni * nj * nk) {
-        const int i = i0 + t % ni;
-        const int jk = t / ni;
-        const int j = j0 + jk % nj;
-        const int k = k0 + jk / nj;
+    // lane -> point: the whole ghosted box walks the pitched array linearly
+    // from the line that holds its first point (aux_lanes), padding included;
+    // other boxes walk their rows (box_row_lanes)
+    int i, j, k;
+    bool act, pad = false;
+    if (aux_whole_box(g, i0, ni, j0, nj, k0, nk)) {
+        const int L = t - aux_lead(g);
+        act = L >= 0 && L < g.nvol;
+        const int e = act ? L % g.sx : 0, row = act ? L / g.sx : 0;
+        i = e + 1 - g.ng;
+        j = row % g.sy + 1 - g.ng;
+        k = row / g.sy + 1 - g.ng;
+        pad = act && i > g.nx + g.ng;
+        act = act && !pad;
+    } else {
+        const int ial = box_row_start(i0, ni), rl = box_row_lanes(i0, ni);
+        i = ial + t % rl;
+        const int jk = t / rl;
+        j = j0 + jk % nj;
+        k = k0 + jk / nj;
+        act = t < rl * nj * nk && i >= i0 && i < i0 + ni;
+    }
+    if (act) {
         int is = i, js = j, ks = k;
         if (image) {
             if (P.periodic[0]) is = i < 1 ? i + g.nx : (i > g.nx ? i - g.nx : i);
@@ -676,6 +694,14 @@ __device__ __forceinline__ void k_aux_body(const dim3 blockIdx, const dim3 gridD
                 mx = fmax(mx, kor * sz);
             }
         }
+    } else if (pad) {
+        // the row padding after x = nx + ng: zeros, so that the lines this
+        // pass writes are written whole (never read)
+        const int d = gidx(g, i, j, k);
+        const size_t nv = g.nvol;
+#pragma unroll
+        for (int m = 0; m < 7; m++) wa[m * nv + d] = R(0.0);
+        if constexpr (sizeof(R) != sizeof(double)) wa[10 * nv + d] = R(0.0);
     }
     if constexpr (DT) {
         __shared__ double lds[4];
@@ -3419,7 +3445,8 @@ hipError_t launch_extrapolate_aux(st_solver *s, int var) {
 hipError_t launch_aux(st_solver *s, int i0, int i1, int j0, int j1, int k0, int k1, bool image, bool dt) {
     const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
     if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
-    const size_t tot = (size_t)ni * nj * nk;
+    const size_t tot = aux_whole_box(s->g, i0, ni, j0, nj, k0, nk) ? aux_lanes(s->g)
+                                                                    : (size_t)box_row_lanes(i0, ni) * nj * nk;
     const unsigned nblk = (unsigned)((tot + 255) / 256);
 #define ST_LAUNCH_AUX(D_)                                                                                          \
     ST_DISPATCH(ST_GO(k_aux, ST_T(R, D_), 256, 0, dim3(nblk), dim3(256), s->g, s->kp, s->w, (R *)s->w_aux, i0, ni,  \

@@ -58,7 +58,7 @@ extern "C" int st_compute_dt_noalloc(const st_params *p, void *hip_stream, const
     if (rc) return rc;
     if (!w || !w_aux || !dcsidx || !detady || !dzitdz || !dcsidxs || !detadys || !dzitdzs || !dt_out)
         return st::set_error(ST_ERR_ARG, "null device array");
-    const st::Geo g = st::params_to_geo(*p);
+    const st::Geo g = st::params_to_geo(*p, false);  // the caller's arrays: the reference layout
     const int per_plane = st::dt_blocks_per_plane(g);
     if (per_plane > kPartials)
         return st::set_error(ST_ERR_ARG, "grid too large for st_compute_dt_noalloc: ceil(nx/64)*ceil(ny/4) = " +

@@ -412,7 +412,7 @@ void finish(st_server *sv, Slot &sl) {
         rc = st_compute_residual(sl.s, &r->sc);
         if (rc == ST_OK || rc == ST_ERR_NAN) {
             err = rc ? st_last_error() : "";
-            r->w_out.resize((size_t)5 * sl.s->g.nvol);
+            r->w_out.resize((size_t)5 * sl.s->g.nref);  // the reference layout
             const int rc2 = st_copy_to_field(sl.s, r->w_out.data());
             if (rc == ST_OK && rc2 != ST_OK) {
                 rc = rc2;

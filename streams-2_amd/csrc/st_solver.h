@@ -16,6 +16,7 @@ struct st_solver {
     // state
     double *w;          // 5 ghosted variables
     void *w_aux;        // 10 ghosted variables (double, or float in the mixed form)
+    void *w_alloc, *w_aux_alloc;  // the allocations of w and w_aux (their first points sit lead_bytes in)
     void *F[2];         // right-hand sides of the current and the previous stage (same type)
     bool mixed;         // ST_PREC_MIXED: w_aux and F in float
     bool general_bc;    // boundary faces other than periodic / staggered wall: per-face ghost kernels
@@ -137,7 +138,9 @@ inline void host_restore(st_solver *s, const HostMark &m) {
 int set_error(int code, const std::string &msg);  // message for st_last_error; returns code
 int validate_params(const st_params *p);
 void params_to_kpar(const st_params &p, KPar &k);
-Geo params_to_geo(const st_params &p);
+// solver: the solver's pitched layout; false: the reference layout (rows of
+// nx + 2 ng values), as caller-owned device arrays hold it
+Geo params_to_geo(const st_params &p, bool solver = true);
 
 // CFL reduction over caller-given arrays: k_dt over nx x ny x kplanes
 // workgroup columns (each walks k with stride kplanes) writing one block
