@@ -18,6 +18,9 @@
  *   fl     (1:nx, 1:ny, 1:nz, 1:5)                     right-hand side
  *   x,y,z  (1-ng:n+ng) node coordinates, yn (1:ny+1) cell faces in y
  *   metric arrays dcsidx, dcsidxs, dcsidx2 (1:nx), detady.. (1:ny), dzitdz.. (1:nz)
+ * Host arrays passed to and from the library have these layouts.  The
+ * library's own device copies of w and w_aux pad each row to whole 128-byte
+ * lines (an internal layout; the copies re-space the rows).
  *
  * Every function returns 0 on success and a non-zero ST_ERR_* code on
  * failure; st_last_error() returns a message for the calling thread.  The
