@@ -624,8 +624,8 @@ int st_copy_to_field(st_solver *s, double *w_host) {
     return ST_OK;
 }
 
-// Device array of n values (double, or float in the mixed form) to/from a
-// host double array.
+// Device array of n values (double, or float in the mixed form) to a host
+// double array.
 static int to_host(st_solver *s, double *host, const void *dev, size_t n) {
     if (!s->mixed) {
         HIPCHK(hipMemcpyAsync(host, dev, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
@@ -636,18 +636,6 @@ static int to_host(st_solver *s, double *host, const void *dev, size_t n) {
     HIPCHK(hipMemcpyAsync(tmp.data(), dev, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     for (size_t t = 0; t < n; t++) host[t] = tmp[t];
-    return ST_OK;
-}
-
-static int from_host(st_solver *s, void *dev, const double *host, size_t n) {
-    if (!s->mixed) {
-        HIPCHK(hipMemcpyAsync(dev, host, sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
-        HIPCHK(hipStreamSynchronize(s->stream));
-        return ST_OK;
-    }
-    std::vector<float> tmp(host, host + n);
-    HIPCHK(hipMemcpyAsync(dev, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
     return ST_OK;
 }
 
