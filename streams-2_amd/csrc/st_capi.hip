@@ -529,29 +529,34 @@ int create_solver(const st_params *p, const st::SlotMem *mem, st_solver **out) {
 }
 
 // nvars ghosted variables of es-byte values between the solver's pitched
-// layout (device) and the reference layout (host): the rows are re-spaced in
-// a host buffer and the whole block moves in one copy on the solver's stream.
-// Returns once the copy has completed.
+// layout (device) and the reference layout (host): each variable's rows are
+// re-spaced in a host buffer of one variable and move in one copy on the
+// solver's stream.  Returns once the copies have completed.
 hipError_t copy_rows(st_solver *s, void *dst, const void *src, size_t es, int nvars, hipMemcpyKind kind) {
     const st::Geo &g = s->g;
     const size_t dpitch = (size_t)g.sx * es, hpitch = (size_t)g.rx * es;
-    const size_t rows = (size_t)nvars * g.sy * g.sz;
+    const size_t rows = (size_t)g.sy * g.sz;  // of one variable
+    const size_t dvar = (size_t)g.nvol * es, hvar = (size_t)g.nref * es;
     std::vector<char> tmp;
     try {
         tmp.assign(rows * dpitch, 0);
     } catch (const std::bad_alloc &) {
         return hipErrorOutOfMemory;
     }
-    hipError_t e;
-    if (kind == hipMemcpyHostToDevice) {
-        for (size_t r = 0; r < rows; r++) std::memcpy(tmp.data() + r * dpitch, (const char *)src + r * hpitch, hpitch);
-        e = hipMemcpyAsync(dst, tmp.data(), rows * dpitch, kind, s->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    } else {
-        e = hipMemcpyAsync(tmp.data(), src, rows * dpitch, kind, s->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-        if (e == hipSuccess)
-            for (size_t r = 0; r < rows; r++) std::memcpy((char *)dst + r * hpitch, tmp.data() + r * dpitch, hpitch);
+    hipError_t e = hipSuccess;
+    for (int m = 0; m < nvars && e == hipSuccess; m++) {
+        if (kind == hipMemcpyHostToDevice) {
+            const char *h = (const char *)src + m * hvar;
+            for (size_t r = 0; r < rows; r++) std::memcpy(tmp.data() + r * dpitch, h + r * hpitch, hpitch);
+            e = hipMemcpyAsync((char *)dst + m * dvar, tmp.data(), dvar, kind, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        } else {
+            e = hipMemcpyAsync(tmp.data(), (const char *)src + m * dvar, dvar, kind, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            char *h = (char *)dst + m * hvar;
+            if (e == hipSuccess)
+                for (size_t r = 0; r < rows; r++) std::memcpy(h + r * hpitch, tmp.data() + r * dpitch, hpitch);
+        }
     }
     return e;
 }
