@@ -143,3 +143,21 @@ def test_four_ghost_layers_are_validated(change, msg):
     h = C.c_void_p()
     assert capi.lib().st_solver_create(C.byref(p), C.byref(h)) == capi.ST_ERR_ARG
     assert msg in capi.lib().st_last_error(), capi.lib().st_last_error()
+
+
+def test_footprint_counts_pitched_rows():
+    """w and w_aux rows are padded to whole 128-byte lines of w_aux values:
+    16 FP64 values, so nx + 6 = 16 needs no padding and nx + 6 = 17 pads to
+    32; the 15 extra values of each of the 15 ghosted variables show in the
+    footprint.  The mixed form pads to 32 values."""
+    from streams_mi355x import PREC_MIXED, make_channel
+    ny, nz = 12, 8
+    planes = (ny + 6) * (nz + 6)
+    f10 = capi.lib().st_solver_footprint(C.byref(make_channel(nx=10, ny=ny, nz=nz).params))
+    f11 = capi.lib().st_solver_footprint(C.byref(make_channel(nx=11, ny=ny, nz=nz).params))
+    # w (5) and w_aux (10) grow by 16 values a row; everything else by a few cells
+    assert f11 - f10 >= 15 * 16 * planes * 8
+    m26 = capi.lib().st_solver_footprint(C.byref(make_channel(nx=26, ny=ny, nz=nz, precision=PREC_MIXED).params))
+    m27 = capi.lib().st_solver_footprint(C.byref(make_channel(nx=27, ny=ny, nz=nz, precision=PREC_MIXED).params))
+    # w (5 doubles) and w_aux (11 floats) grow by 32 values a row
+    assert m27 - m26 >= (5 * 8 + 11 * 4) * 32 * planes
