@@ -273,7 +273,8 @@ int st_set_stats(st_solver *s, const double *w_stat);
  * hipMalloc).  It allocates nothing: the block maxima live in a static
  * device array of 257 KiB per GPU, shared by every call on that GPU, so
  * calls that overlap on one GPU race.  All arrays are device pointers of the
- * solve described by p: w (5 ghosted variables), w_aux (the solve's w_aux:
+ * solve described by p, in the reference layouts above (ghosted rows of
+ * nx + 2 ng values, no padding): w (5 ghosted variables), w_aux (the solve's w_aux:
  * double, or float with 11 variables when p->precision is ST_PREC_MIXED;
  * variable 6, T, is the start of the T(e) Newton iteration and receives T
  * when calorically_perfect = 0, as compute_aux does), the six metric arrays
